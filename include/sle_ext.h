@@ -1,0 +1,90 @@
+/*
+ * sle_ext.h -- extension entry points of libsle beyond the reference ABI.
+ *
+ * None of these has a counterpart in the reference's sle_api.h.  They exist
+ * for callers that need FP64 values regardless of the SLE_Real width, for
+ * index-based (string-free) telemetry and topology updates on large models,
+ * and for parity checks of the symbolic and numeric phases against a host
+ * oracle.  All values are FP64 and all indices are the insertion-order
+ * indices of the elements (the order of the sle_Add* calls).
+ */
+#ifndef SLE_EXT_H
+#define SLE_EXT_H
+
+#include "sle_api.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* FP64 solve result (SLE_Result carries SLE_Real). */
+typedef struct SLE_ExtResult {
+    int32_t status;      /* SLE_ConvergenceStatus */
+    int32_t iterations;
+    int32_t refactors;   /* numeric factorizations performed */
+    int32_t fp64_refactors; /* mixed form: pivot-triggered FP64 refactors */
+    double max_mismatch;
+    double objective;
+    double largest_residual; /* max |z-h|/sigma over active rows */
+    int32_t largest_residual_idx;
+    int32_t reserved;
+    double gpu_ms;       /* device time of the solve */
+    double host_ms;      /* wall time of the call */
+} SLE_ExtResult;
+
+/* Solve with an explicit solver configuration; FP64 outputs. */
+SLE_API SLE_StatusCode SLE_CALL sle_ext_Solve(SLE_Handle handle, const SLE_SolverConfig* config,
+                                              SLE_ExtResult* result);
+
+/* Telemetry: values[q] for measurement idx[q] (count may be any size). */
+SLE_API SLE_StatusCode SLE_CALL sle_ext_UpdateTelemetryIndexed(SLE_Handle handle, const int32_t* idx,
+                                                               const double* values, int32_t count);
+/* Full telemetry vector in FP64, insertion order, count == measurement count. */
+SLE_API SLE_StatusCode SLE_CALL sle_ext_UpdateTelemetryF64(SLE_Handle handle, const double* values,
+                                                           int32_t count);
+
+/* State in FP64 (hot start in / estimate out). count == bus count. */
+SLE_API SLE_StatusCode SLE_CALL sle_ext_SetState(SLE_Handle handle, const double* v, const double* theta,
+                                                 int32_t count);
+SLE_API SLE_StatusCode SLE_CALL sle_ext_GetState(SLE_Handle handle, double* v, double* theta, int32_t count);
+
+/* Results at the final state in FP64; any pointer may be NULL. */
+SLE_API SLE_StatusCode SLE_CALL sle_ext_GetResults(SLE_Handle handle, double* p_inj, double* q_inj,
+                                                   double* p_from, double* q_from, double* p_to, double* q_to,
+                                                   double* h, double* residual);
+
+/* Breaker status of branch k (insertion index); applied before the next solve. */
+SLE_API SLE_StatusCode SLE_CALL sle_ext_SetBranchStatus(SLE_Handle handle, int32_t branch_index,
+                                                        SLE_SwitchStatus status);
+
+/* --- symbolic introspection (after sle_UploadModel) --------------------- */
+/* sizes[0..11]: n_states, nnz(Ybus), nnz(H), nnz(G lower), nnz(L),
+ * n_supernodes, n_levels, max front size, product terms of G, front storage
+ * (elements), factor flops (sum of column counts squared), symbolic seconds*1e6 */
+SLE_API SLE_StatusCode SLE_CALL sle_ext_GetSizes(SLE_Handle handle, int64_t* sizes);
+SLE_API SLE_StatusCode SLE_CALL sle_ext_GetYbus(SLE_Handle handle, int32_t* row_ptr, int32_t* col, double* g,
+                                                double* b);
+SLE_API SLE_StatusCode SLE_CALL sle_ext_GetHPattern(SLE_Handle handle, int32_t* row_ptr, int32_t* col);
+/* natural state order, lower triangle, CSC */
+SLE_API SLE_StatusCode SLE_CALL sle_ext_GetGPattern(SLE_Handle handle, int32_t* col_ptr, int32_t* row);
+/* bus_perm[n], perm[n_states], parent[n_states], colcount[n_states], snode_first[n_supernodes+1] */
+SLE_API SLE_StatusCode SLE_CALL sle_ext_GetOrdering(SLE_Handle handle, int32_t* bus_perm, int32_t* perm,
+                                                    int32_t* parent, int32_t* colcount, int32_t* snode_first);
+
+/* --- numeric introspection at the current state ------------------------- */
+SLE_API SLE_StatusCode SLE_CALL sle_ext_Evaluate(SLE_Handle handle, double* h, double* p_inj, double* q_inj,
+                                                 double* p_from, double* q_from, double* p_to, double* q_to);
+/* H values in the H pattern order and G = H^T H (unit weights) in the natural G pattern order */
+SLE_API SLE_StatusCode SLE_CALL sle_ext_JacobianGain(SLE_Handle handle, double* h_values, double* g_values);
+/* Factor P G P^T (current state, unit Huber weights) and return L as CSC in
+ * permuted order: col_ptr[n_states+1] (int64), row[nnz(L)], val[nnz(L)].
+ * Returns SLE_ERROR_SINGULAR_MATRIX when a pivot falls below threshold. */
+SLE_API SLE_StatusCode SLE_CALL sle_ext_Factor(SLE_Handle handle, int64_t* col_ptr, int32_t* row, double* val);
+/* Solve G dx = b with the factor of the last sle_ext_Factor (natural order). */
+SLE_API SLE_StatusCode SLE_CALL sle_ext_SolveFactor(SLE_Handle handle, const double* b, double* dx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SLE_EXT_H */

@@ -1,0 +1,1460 @@
+// device.hip -- gfx950 kernels and the device-resident Gauss-Newton loop.
+//
+// Precision (the mixed form): h(x), residuals, Huber weights, the right-hand
+// side H^T W r, the state and the convergence test are FP64; the scaled
+// Jacobian sqrt(W) H, the gain entries and the factor are FP32 (T = float).
+// The plain form instantiates the same kernels with T = double.  A pivot that
+// falls below threshold in the FP32 factor switches the instance to the FP64
+// factor (pivot-triggered refactor).
+//
+// Kernel map (reference counterparts, replaced rather than translated):
+//   k_ybus        fillYbusKernel / initYbusDiagonalKernel, sparse_matrix.cu:389-492
+//   k_power       computePowerInjectionsKernel, kernels.cu:129-230
+//   k_flows       computeBranchFlowsKernel, kernels.cu:241-333
+//   k_meas        computeMeasurementFunctionKernel + computeResidualsKernel +
+//                 computeObjectiveFunctionKernel + computeHuberWeightsKernel +
+//                 computeJacobianValuesKernel + applyWeightScalingKernel
+//                 (kernels.cu:381-549, 634-671; sparse_matrix.cu:905-926,
+//                 1627-2147), fused into one pass over the measurement rows
+//   k_spgemm      cusparseSpGEMM G = H^T W H, sparse_matrix.cu:1209-1356
+//   k_factor_*    cusolverSpXcsrlsvchol factor, sparse_matrix.cu:856-882
+//   k_fwd / k_bwd cusolverSpXcsrlsvchol solve (same call)
+//   k_rhs         computeRHSVectorKernel, kernels.cu:823-862
+//   k_update      applyStateUpdateKernel + computeMaxMismatchKernel +
+//                 checkValidityKernel, kernels.cu:721-782, 926-941
+//   k_flat        flatStartKernel, kernels.cu:799-817
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "instance.h"
+
+namespace sle {
+
+namespace {
+
+struct HipError : GpuError {
+    explicit HipError(const std::string& s) : GpuError(s) {}
+};
+
+#define HIPCHECK(x)                                                                                     \
+    do {                                                                                                \
+        hipError_t e_ = (x);                                                                            \
+        if (e_ != hipSuccess)                                                                           \
+            throw HipError(std::string("HIP error ") + hipGetErrorString(e_) + " at " + #x + " (" + \
+                           __FILE__ + ":" + std::to_string(__LINE__) + ")");                            \
+    } while (0)
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr int kFrontA = 32;    // fronts with c <= 32: LDS front of 32x32
+constexpr int kFrontB = 64;    // 32 < c <= 64: LDS front of 64x64
+constexpr int kVecMax = 2048;  // triangular-solve LDS vector bound for large fronts
+
+// measurement code byte: type (4 bits) | end << 4 | active << 5
+__host__ __device__ inline uint8_t meas_code(int type, int end, int active) {
+    return (uint8_t)((type & 15) | ((end & 1) << 4) | ((active & 1) << 5));
+}
+
+struct DevStatus {
+    double maxdx;
+    double J;
+    double rmax;
+    int ridx;
+    int nan;
+    int piv;
+    int pad;
+};
+
+// ---------------------------------------------------------------------------
+// wave64 / block reductions
+// ---------------------------------------------------------------------------
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+__device__ inline double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+// (value, index) max with ties to the lower index
+__device__ inline void wave_argmax(double& v, int& i) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        double ov = __shfl_xor(v, o, kWave);
+        int oi = __shfl_xor(i, o, kWave);
+        if (ov > v || (ov == v && oi >= 0 && (i < 0 || oi < i))) {
+            v = ov;
+            i = oi;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Ybus values from the per-entry branch contribution lists (FP64, no atomics)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_ybus(int nY, const int32_t* __restrict__ ycptr,
+                                                 const int32_t* __restrict__ ycon, const double* __restrict__ gs,
+                                                 const double* __restrict__ bs, const double* __restrict__ bc,
+                                                 const double* __restrict__ tap, const double* __restrict__ shift,
+                                                 const uint8_t* __restrict__ status, double* __restrict__ yG,
+                                                 double* __restrict__ yB) {
+    int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= nY) return;
+    double gr = 0.0, gi = 0.0;
+    for (int p = ycptr[e]; p < ycptr[e + 1]; ++p) {
+        int con = ycon[p];
+        int k = con >> 2, kind = con & 3;
+        if (!status[k]) continue;
+        double g = gs[k], b = bs[k], a = tap[k], hb = 0.5 * bc[k];
+        double a2 = a * a;
+        if (kind == 0) {
+            gr += g / a2;
+            gi += (b + hb) / a2;
+        } else if (kind == 1) {
+            gr += g;
+            gi += b + hb;
+        } else {
+            double cp = cos(shift[k]), sp = sin(shift[k]);
+            if (kind == 2) {
+                gr += -(g * cp - b * sp) / a;
+                gi += -(g * sp + b * cp) / a;
+            } else {
+                gr += -(g * cp + b * sp) / a;
+                gi += -(b * cp - g * sp) / a;
+            }
+        }
+    }
+    yG[e] = gr;
+    yB[e] = gi;
+}
+
+// ---------------------------------------------------------------------------
+// Bus injections; per Ybus entry P_ij, Q_ij terms kept for the Jacobian
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_power(int n, const int32_t* __restrict__ yrow,
+                                                  const int32_t* __restrict__ ycol, const double* __restrict__ yG,
+                                                  const double* __restrict__ yB, const double* __restrict__ V,
+                                                  const double* __restrict__ th, double* __restrict__ P,
+                                                  double* __restrict__ Q, double* __restrict__ Pij,
+                                                  double* __restrict__ Qij, DevStatus* st) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i == 0) {
+        st->nan = 0;
+        st->piv = 0;
+    }
+    if (i >= n) return;
+    double vi = V[i], ti = th[i];
+    double p = 0.0, q = 0.0;
+    for (int e = yrow[i]; e < yrow[i + 1]; ++e) {
+        int j = ycol[e];
+        double s, c;
+        sincos(ti - th[j], &s, &c);
+        double vv = vi * V[j];
+        double G = yG[e], B = yB[e];
+        double pe = vv * (G * c + B * s);
+        double qe = vv * (G * s - B * c);
+        Pij[e] = pe;
+        Qij[e] = qe;
+        p += pe;
+        q += qe;
+    }
+    P[i] = p;
+    Q[i] = q;
+}
+
+// Flows at both ends of every branch (results; the solver loop evaluates the
+// measured ones inside k_meas).
+__global__ __launch_bounds__(kBlock) void k_flows(int nb, const int32_t* __restrict__ from,
+                                                  const int32_t* __restrict__ to, const double* __restrict__ gs,
+                                                  const double* __restrict__ bs, const double* __restrict__ bc,
+                                                  const double* __restrict__ tap, const double* __restrict__ shift,
+                                                  const uint8_t* __restrict__ status, const double* __restrict__ V,
+                                                  const double* __restrict__ th, double* __restrict__ Pf,
+                                                  double* __restrict__ Qf, double* __restrict__ Pt,
+                                                  double* __restrict__ Qt) {
+    int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= nb) return;
+    if (!status[k]) {
+        Pf[k] = Qf[k] = Pt[k] = Qt[k] = 0.0;
+        return;
+    }
+    int i = from[k], j = to[k];
+    double g = gs[k], b = bs[k], a = tap[k], bsh = b + 0.5 * bc[k];
+    double s, c;
+    sincos(th[i] - th[j] - shift[k], &s, &c);
+    double Vi = V[i], Vj = V[j], A = Vi * Vj / a;
+    Pf[k] = Vi * Vi * g / (a * a) - A * (g * c + b * s);
+    Qf[k] = -Vi * Vi * bsh / (a * a) - A * (g * s - b * c);
+    Pt[k] = Vj * Vj * g - A * (g * c - b * s);
+    Qt[k] = -Vj * Vj * bsh + A * (g * s + b * c);
+}
+
+// ---------------------------------------------------------------------------
+// Fused measurement pass: h(x), r = z - h, Huber weight, sqrt(w) r, the
+// Jacobian row scaled by sqrt(w), and block partials of J and max |r|/sigma.
+// ---------------------------------------------------------------------------
+struct MeasArgs {
+    int m, n, slack;
+    int robust, unit_weights, want_jac;
+    double gamma;
+    const uint8_t* mcode;
+    const int32_t* mloc;
+    const double* z;
+    const double* sigma;
+    const int32_t* hrow;
+    const int32_t* yrow;
+    const int32_t* ycol;
+    const double* yG;
+    const double* yB;
+    const double* Pij;
+    const double* Qij;
+    const double* P;
+    const double* Q;
+    const double* V;
+    const double* th;
+    const int32_t* from;
+    const int32_t* to;
+    const double* gs;
+    const double* bs;
+    const double* bc;
+    const double* tap;
+    const double* shift;
+    double* rs;
+    double* hout;
+    double* rout;
+    double* jpart;
+    double* rpart;
+    int* ipart;
+};
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_meas(MeasArgs A, T* __restrict__ hs) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    double jw = 0.0, rn = -1.0;
+    int ri = -1;
+    if (i < A.m) {
+        const uint8_t code = A.mcode[i];
+        const int t = code & 15, end = (code >> 4) & 1;
+        const bool act = (code >> 5) & 1;
+        const int l = A.mloc[i];
+        const int q0 = A.hrow[i], q1 = A.hrow[i + 1];
+        double hv = 0.0;
+        // flow partials wrt (th_from, th_to, V_from, V_to)
+        double d[4] = {0, 0, 0, 0};
+        int fb = 0, tb = 0;
+        if (t == 0) {
+            hv = A.V[l];
+        } else if (t == 1) {
+            hv = A.th[l];
+        } else if (t == 2 || t == 3 || t == 7 || t == 8) {
+            hv = (t == 2 || t == 7) ? A.P[l] : A.Q[l];
+        } else {
+            fb = A.from[l];
+            tb = A.to[l];
+            double g = A.gs[l], b = A.bs[l], a = A.tap[l], bsh = b + 0.5 * A.bc[l];
+            double s, c;
+            sincos(A.th[fb] - A.th[tb] - A.shift[l], &s, &c);
+            double Vi = A.V[fb], Vj = A.V[tb], AA = Vi * Vj / a;
+            double p, q, dP[4], dQ[4];
+            if (end == 0) {
+                p = Vi * Vi * g / (a * a) - AA * (g * c + b * s);
+                q = -Vi * Vi * bsh / (a * a) - AA * (g * s - b * c);
+                dP[0] = AA * (g * s - b * c);
+                dP[1] = -dP[0];
+                dP[2] = 2.0 * Vi * g / (a * a) - (Vj / a) * (g * c + b * s);
+                dP[3] = -(Vi / a) * (g * c + b * s);
+                dQ[0] = -AA * (g * c + b * s);
+                dQ[1] = -dQ[0];
+                dQ[2] = -2.0 * Vi * bsh / (a * a) - (Vj / a) * (g * s - b * c);
+                dQ[3] = -(Vi / a) * (g * s - b * c);
+            } else {
+                p = Vj * Vj * g - AA * (g * c - b * s);
+                q = -Vj * Vj * bsh + AA * (g * s + b * c);
+                dP[0] = AA * (g * s + b * c);
+                dP[1] = -dP[0];
+                dP[2] = -(Vj / a) * (g * c - b * s);
+                dP[3] = 2.0 * Vj * g - (Vi / a) * (g * c - b * s);
+                dQ[0] = AA * (g * c - b * s);
+                dQ[1] = -dQ[0];
+                dQ[2] = (Vj / a) * (g * s + b * c);
+                dQ[3] = -2.0 * Vj * bsh + (Vi / a) * (g * s + b * c);
+            }
+            if (t == 4) {
+                hv = p;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) d[k] = dP[k];
+            } else if (t == 5) {
+                hv = q;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) d[k] = dQ[k];
+            } else {  // I_MAG = |S| / V_end
+                double ve = end ? Vj : Vi;
+                double S = sqrt(p * p + q * q);
+                hv = S / ve;
+                double inv = S > 1e-12 ? 1.0 / (S * ve) : 0.0;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) d[k] = (p * dP[k] + q * dQ[k]) * inv;
+                d[end ? 3 : 2] -= S / (ve * ve);
+            }
+        }
+        const double r = act ? A.z[i] - hv : 0.0;
+        const double sg = A.sigma[i];
+        const double wb = 1.0 / (sg * sg);
+        double w = wb;
+        if (A.robust) {
+            double ar = fabs(r);
+            if (ar / sg > A.gamma) w = A.gamma / (sg * ar);
+        }
+        if (!act) w = 0.0;
+        const double sw = A.unit_weights ? 1.0 : sqrt(w);
+        A.rs[i] = sw * r;
+        if (A.hout) {
+            A.hout[i] = act ? hv : 0.0;
+            A.rout[i] = r;
+        }
+        if (act) {
+            jw = wb * r * r;
+            rn = fabs(r) / sg;
+            ri = i;
+        }
+        if (A.want_jac && q1 > q0) {
+            if (t == 0 || t == 1) {
+                hs[q0] = (T)sw;
+            } else if (t == 2 || t == 3 || t == 7 || t == 8) {
+                const bool isP = (t == 2 || t == 7);
+                const int e0 = A.yrow[l], e1 = A.yrow[l + 1];
+                const int nth = (q1 - q0) - (e1 - e0);
+                int kth = 0;
+                const double Vi = A.V[l];
+                for (int e = e0; e < e1; ++e) {
+                    int j = A.ycol[e];
+                    double dth, dv;
+                    if (j == l) {
+                        double Gii = A.yG[e], Bii = A.yB[e];
+                        if (isP) {
+                            dth = -A.Q[l] - Vi * Vi * Bii;
+                            dv = A.P[l] / Vi + Vi * Gii;
+                        } else {
+                            dth = A.P[l] - Vi * Vi * Gii;
+                            dv = A.Q[l] / Vi - Vi * Bii;
+                        }
+                    } else {
+                        double pe = A.Pij[e], qe = A.Qij[e], Vj = A.V[j];
+                        if (isP) {
+                            dth = qe;
+                            dv = pe / Vj;
+                        } else {
+                            dth = -pe;
+                            dv = qe / Vj;
+                        }
+                    }
+                    if (j != A.slack) hs[q0 + kth++] = (T)(sw * dth);
+                    hs[q0 + nth + (e - e0)] = (T)(sw * dv);
+                }
+            } else {
+                // sorted columns: theta_min, theta_max, V_min, V_max (slack theta absent)
+                int q = q0;
+                if (fb == tb) {
+                    if (fb != A.slack) hs[q++] = (T)(sw * (d[0] + d[1]));
+                    hs[q] = (T)(sw * (d[2] + d[3]));
+                } else {
+                    bool fl = fb < tb;
+                    int lo = fl ? fb : tb, hi = fl ? tb : fb;
+                    double dlo_t = fl ? d[0] : d[1], dhi_t = fl ? d[1] : d[0];
+                    double dlo_v = fl ? d[2] : d[3], dhi_v = fl ? d[3] : d[2];
+                    if (lo != A.slack) hs[q++] = (T)(sw * dlo_t);
+                    if (hi != A.slack) hs[q++] = (T)(sw * dhi_t);
+                    hs[q++] = (T)(sw * dlo_v);
+                    hs[q] = (T)(sw * dhi_v);
+                }
+            }
+        }
+    }
+    // block partials (deterministic: fixed tree order)
+    __shared__ double sj[kBlock / kWave], sr[kBlock / kWave];
+    __shared__ int si[kBlock / kWave];
+    jw = wave_sum(jw);
+    wave_argmax(rn, ri);
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        sj[w] = jw;
+        sr[w] = rn;
+        si[w] = ri;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double J = 0.0, R = -1.0;
+        int I = -1;
+        for (int k = 0; k < kBlock / kWave; ++k) {
+            J += sj[k];
+            if (sr[k] > R || (sr[k] == R && si[k] >= 0 && (I < 0 || si[k] < I))) {
+                R = sr[k];
+                I = si[k];
+            }
+        }
+        A.jpart[blockIdx.x] = J;
+        A.rpart[blockIdx.x] = R;
+        A.ipart[blockIdx.x] = I;
+    }
+}
+
+// Single-block finish of the J / largest-residual partials.
+__global__ __launch_bounds__(kBlock) void k_jfinish(int nparts, const double* __restrict__ jpart,
+                                                    const double* __restrict__ rpart, const int* __restrict__ ipart,
+                                                    DevStatus* st) {
+    double J = 0.0, R = -1.0;
+    int I = -1;
+    for (int p = threadIdx.x; p < nparts; p += kBlock) {
+        J += jpart[p];
+        if (rpart[p] > R || (rpart[p] == R && ipart[p] >= 0 && (I < 0 || ipart[p] < I))) {
+            R = rpart[p];
+            I = ipart[p];
+        }
+    }
+    __shared__ double sj[kBlock / kWave], sr[kBlock / kWave];
+    __shared__ int si[kBlock / kWave];
+    J = wave_sum(J);
+    wave_argmax(R, I);
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        sj[w] = J;
+        sr[w] = R;
+        si[w] = I;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double JJ = 0.0, RR = -1.0;
+        int II = -1;
+        for (int k = 0; k < kBlock / kWave; ++k) {
+            JJ += sj[k];
+            if (sr[k] > RR || (sr[k] == RR && si[k] >= 0 && (II < 0 || si[k] < II))) {
+                RR = sr[k];
+                II = si[k];
+            }
+        }
+        st->J = JJ;
+        st->rmax = RR;
+        st->ridx = II;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Gain entries G_e = sum over the cached product list of (sqrt(w)H)_a (sqrt(w)H)_b
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_spgemm(int nG, const int32_t* __restrict__ gptr,
+                                                   const int2* __restrict__ gpair, const T* __restrict__ hs,
+                                                   T* __restrict__ Gval) {
+    int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= nG) return;
+    T acc = T(0);
+    for (int p = gptr[e]; p < gptr[e + 1]; ++p) {
+        int2 pr = gpair[p];
+        acc += hs[pr.x] * hs[pr.y];
+    }
+    Gval[e] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Multifrontal supernodal Cholesky.  One workgroup per front; a front is the
+// dense c x c (column-major) matrix of supernode s: its G columns plus the
+// extend-add of its children's update blocks.  The first k columns are
+// factored; the trailing (c-k)^2 block is the update passed to the parent.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct FactArgs {
+    const int32_t* sn_first;
+    const int32_t* sn_k;
+    const int32_t* sn_c;
+    const int64_t* foff;
+    const int32_t* child_ptr;
+    const int32_t* child;
+    const int64_t* rmoff;
+    const int32_t* rmap;
+    const int32_t* g_sn_ptr;
+    const int32_t* gloc;
+    const int32_t* gdiag;
+    const T* Gval;
+    T* F;
+    T piv_tol;
+    DevStatus* st;
+};
+
+template <typename T, int NT>
+__device__ inline void factor_front(const FactArgs<T>& A, int s, T* F) {
+    const int tid = threadIdx.x;
+    const int c = A.sn_c[s], k = A.sn_k[s], first = A.sn_first[s];
+    for (int idx = tid; idx < c * c; idx += NT) F[idx] = T(0);
+    __syncthreads();
+    for (int e = A.g_sn_ptr[s] + tid; e < A.g_sn_ptr[s + 1]; e += NT) F[A.gloc[e]] = A.Gval[e];
+    __syncthreads();
+    for (int p = A.child_ptr[s]; p < A.child_ptr[s + 1]; ++p) {
+        const int ch = A.child[p];
+        const int cc = A.sn_c[ch], kc = A.sn_k[ch], u = cc - kc;
+        const T* Fc = A.F + A.foff[ch];
+        const int32_t* rm = A.rmap + A.rmoff[ch];
+        for (int idx = tid; idx < u * u; idx += NT) {
+            int a = idx % u, b = idx / u;
+            if (a >= b) F[rm[b] * c + rm[a]] += Fc[(int64_t)(kc + b) * cc + kc + a];
+        }
+        __syncthreads();
+    }
+    for (int j = 0; j < k; ++j) {
+        T dd = F[j * c + j];
+        const int ge = A.gdiag[first + j];
+        const T gd = ge >= 0 ? A.Gval[ge] : T(0);
+        __syncthreads();
+        if (!(dd > A.piv_tol * gd) || !(dd > T(0))) {
+            if (tid == 0) A.st->piv = 1;
+            dd = gd > T(0) ? gd : T(1);
+        }
+        const T dj = sqrt(dd);
+        if (tid == 0) F[j * c + j] = dj;
+        for (int i = j + 1 + tid; i < c; i += NT) F[j * c + i] = F[j * c + i] / dj;
+        __syncthreads();
+        const int w = c - j - 1;
+        for (int idx = tid; idx < w * w; idx += NT) {
+            int a = idx % w, b = idx / w;
+            if (a >= b) {
+                int aa = j + 1 + a, bb = j + 1 + b;
+                F[bb * c + aa] -= F[j * c + aa] * F[j * c + bb];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T, int CM>
+__global__ __launch_bounds__(kWave) void k_factor_lds(FactArgs<T> A, const int32_t* __restrict__ list) {
+    __shared__ T F[CM * CM];
+    const int s = list[blockIdx.x];
+    factor_front<T, kWave>(A, s, F);
+    const int c = A.sn_c[s];
+    T* Fo = A.F + A.foff[s];
+    for (int idx = threadIdx.x; idx < c * c; idx += kWave) Fo[idx] = F[idx];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_factor_global(FactArgs<T> A, const int32_t* __restrict__ list) {
+    const int s = list[blockIdx.x];
+    factor_front<T, kBlock>(A, s, A.F + A.foff[s]);
+}
+
+// ---------------------------------------------------------------------------
+// Triangular solves over the supernodal tree (one wave per front).
+// Forward: f = [b_s ; 0] + extend-add of children's update vectors,
+//          L11 y = f_top, u_s = f_bot - L21 y.
+// Backward: x_s = L11^-T (y - L21^T x_rows).
+// ---------------------------------------------------------------------------
+template <typename T>
+struct SolveArgs {
+    const int32_t* sn_first;
+    const int32_t* sn_k;
+    const int32_t* sn_c;
+    const int64_t* foff;
+    const int64_t* rows_off;
+    const int32_t* rows;
+    const int32_t* child_ptr;
+    const int32_t* child;
+    const int64_t* rmoff;
+    const int32_t* rmap;
+    const int64_t* uoff;
+    const T* F;
+    T* x;
+    T* ubuf;
+};
+
+template <typename T, int VM>
+__global__ __launch_bounds__(kWave) void k_fwd(SolveArgs<T> A, const int32_t* __restrict__ list) {
+    __shared__ T f[VM];
+    const int s = list[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int c = A.sn_c[s], k = A.sn_k[s], first = A.sn_first[s];
+    const T* L = A.F + A.foff[s];
+    for (int i = tid; i < c; i += kWave) f[i] = i < k ? A.x[first + i] : T(0);
+    __syncthreads();
+    for (int p = A.child_ptr[s]; p < A.child_ptr[s + 1]; ++p) {
+        const int ch = A.child[p];
+        const int u = A.sn_c[ch] - A.sn_k[ch];
+        const T* uc = A.ubuf + A.uoff[ch];
+        const int32_t* rm = A.rmap + A.rmoff[ch];
+        for (int a = tid; a < u; a += kWave) f[rm[a]] += uc[a];
+        __syncthreads();
+    }
+    for (int j = 0; j < k; ++j) {
+        const T yj = f[j] / L[j * c + j];
+        __syncthreads();
+        if (tid == 0) f[j] = yj;
+        for (int i = j + 1 + tid; i < c; i += kWave) f[i] -= L[j * c + i] * yj;
+        __syncthreads();
+    }
+    for (int i = tid; i < k; i += kWave) A.x[first + i] = f[i];
+    T* us = A.ubuf + A.uoff[s];
+    for (int a = tid; a < c - k; a += kWave) us[a] = f[k + a];
+}
+
+template <typename T, int VM>
+__global__ __launch_bounds__(kWave) void k_bwd(SolveArgs<T> A, const int32_t* __restrict__ list) {
+    __shared__ T xs[VM];
+    const int s = list[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int c = A.sn_c[s], k = A.sn_k[s], first = A.sn_first[s];
+    const T* L = A.F + A.foff[s];
+    const int32_t* rows = A.rows + A.rows_off[s];
+    for (int i = tid; i < c; i += kWave) xs[i] = i < k ? A.x[first + i] : A.x[rows[i]];
+    __syncthreads();
+    for (int j = k - 1; j >= 0; --j) {
+        double part = 0.0;
+        for (int i = j + 1 + tid; i < c; i += kWave) part += (double)(L[j * c + i] * xs[i]);
+        part = wave_sum(part);
+        if (tid == 0) xs[j] = (T)(((double)xs[j] - part) / (double)L[j * c + j]);
+        __syncthreads();
+    }
+    for (int i = tid; i < k; i += kWave) A.x[first + i] = xs[i];
+}
+
+// ---------------------------------------------------------------------------
+// Right-hand side b = H^T W r in permuted state order (FP64 accumulation)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_rhs(int ns, const int32_t* __restrict__ tptr,
+                                                const int32_t* __restrict__ tpos, const int32_t* __restrict__ trow,
+                                                const T* __restrict__ hs, const double* __restrict__ rs,
+                                                T* __restrict__ x) {
+    int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= ns) return;
+    double acc = 0.0;
+    for (int p = tptr[k]; p < tptr[k + 1]; ++p) acc += (double)hs[tpos[p]] * rs[trow[p]];
+    x[k] = (T)acc;
+}
+
+// ---------------------------------------------------------------------------
+// State update (FP64 accumulation), V clamp, max |dx| partials, NaN flag.
+// Skipped entirely when the factor raised its pivot flag.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_update(int n, const int32_t* __restrict__ pth,
+                                                   const int32_t* __restrict__ pv, const T* __restrict__ x,
+                                                   double* __restrict__ V, double* __restrict__ th, double vmin,
+                                                   double vmax, double* __restrict__ mpart, DevStatus* st) {
+    const int b = blockIdx.x * kBlock + threadIdx.x;
+    const bool skip = st->piv != 0;
+    double mx = 0.0;
+    if (b < n && !skip) {
+        double dv = (double)x[pv[b]];
+        double dt = pth[b] >= 0 ? (double)x[pth[b]] : 0.0;
+        if (!isfinite(dv) || !isfinite(dt)) {
+            st->nan = 1;
+        } else {
+            double v = V[b] + dv;
+            V[b] = fmin(fmax(v, vmin), vmax);
+            th[b] += dt;
+            mx = fmax(fabs(dv), fabs(dt));
+        }
+    }
+    __shared__ double sm[kBlock / kWave];
+    mx = wave_max(mx);
+    if ((threadIdx.x & (kWave - 1)) == 0) sm[threadIdx.x / kWave] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0.0;
+        for (int k = 0; k < kBlock / kWave; ++k) r = fmax(r, sm[k]);
+        mpart[blockIdx.x] = r;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_maxfinish(int nparts, const double* __restrict__ mpart, DevStatus* st) {
+    double r = 0.0;
+    for (int p = threadIdx.x; p < nparts; p += kBlock) r = fmax(r, mpart[p]);
+    __shared__ double sm[kBlock / kWave];
+    r = wave_max(r);
+    if ((threadIdx.x & (kWave - 1)) == 0) sm[threadIdx.x / kWave] = r;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double q = 0.0;
+        for (int k = 0; k < kBlock / kWave; ++k) q = fmax(q, sm[k]);
+        st->maxdx = q;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_flat(int n, const int32_t* __restrict__ bus_type,
+                                                 const double* __restrict__ vset, double* __restrict__ V,
+                                                 double* __restrict__ th) {
+    int b = blockIdx.x * kBlock + threadIdx.x;
+    if (b >= n) return;
+    V[b] = bus_type[b] != 0 ? vset[b] : 1.0;
+    th[b] = 0.0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scatter_z(int count, const int32_t* __restrict__ idx,
+                                                      const double* __restrict__ val, double* __restrict__ z) {
+    int q = blockIdx.x * kBlock + threadIdx.x;
+    if (q < count) z[idx[q]] = val[q];
+}
+
+// natural-order vector <-> permuted solve vector (debug solve only)
+template <typename T>
+__global__ void k_permute_in(int ns, const int32_t* __restrict__ perm, const double* __restrict__ b,
+                             T* __restrict__ x) {
+    int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k < ns) x[k] = (T)b[perm[k]];
+}
+template <typename T>
+__global__ void k_permute_out(int ns, const int32_t* __restrict__ perm, const T* __restrict__ x,
+                              double* __restrict__ dx) {
+    int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k < ns) dx[perm[k]] = (double)x[k];
+}
+
+inline int nblk(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
+
+// ---------------------------------------------------------------------------
+// device buffers
+// ---------------------------------------------------------------------------
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t cnt, size_t& bytes) {
+        release(bytes);
+        n = cnt;
+        if (cnt) {
+            HIPCHECK(hipMalloc(&p, cnt * sizeof(T)));
+            bytes += cnt * sizeof(T);
+        }
+    }
+    void release(size_t& bytes) {
+        if (p) {
+            (void)hipFree(p);
+            bytes -= n * sizeof(T);
+        }
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const T* h, size_t cnt, hipStream_t s) {
+        if (cnt) HIPCHECK(hipMemcpyAsync(p, h, cnt * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void upload(const std::vector<T>& v, size_t& bytes, hipStream_t s) {
+        if (n != v.size()) alloc(v.size(), bytes);
+        upload(v.data(), v.size(), s);
+    }
+};
+
+template <typename T>
+struct PrecBufs {
+    DBuf<T> hs, Gval, F, x, ubuf;
+    bool ready = false;
+    void alloc(const Symbolic& S, size_t& bytes) {
+        hs.alloc(std::max<size_t>(1, S.hcol.size()), bytes);
+        Gval.alloc(std::max<int64_t>(1, S.nnz_g), bytes);
+        F.alloc(std::max<int64_t>(1, S.ftotal), bytes);
+        x.alloc(std::max(1, S.ns), bytes);
+        ubuf.alloc(std::max<int64_t>(1, S.utotal), bytes);
+        ready = true;
+    }
+    void release(size_t& bytes) {
+        hs.release(bytes);
+        Gval.release(bytes);
+        F.release(bytes);
+        x.release(bytes);
+        ubuf.release(bytes);
+        ready = false;
+    }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Instance
+// ---------------------------------------------------------------------------
+struct Instance::Impl {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int n = 0, nb = 0, m = 0, ns = 0, slack = 0;
+    // topology / parameters
+    DBuf<int32_t> bus_type, from, to;
+    DBuf<double> vset, gs, bs, bc, tap, shift;
+    DBuf<uint8_t> status;
+    // Ybus
+    DBuf<int32_t> yrow, ycol, ycptr, ycon;
+    DBuf<double> yG, yB, Pij, Qij;
+    // state & powers
+    DBuf<double> V, th, P, Q, Pf, Qf, Pt, Qt;
+    // measurements
+    DBuf<uint8_t> mcode;
+    DBuf<int32_t> mloc, hrow;
+    DBuf<double> z, sigma, rs, hx, r;
+    DBuf<double> jpart, rpart, mpart;
+    DBuf<int> ipart;
+    DBuf<int32_t> zidx;
+    DBuf<double> zval;
+    // symbolic structures
+    DBuf<int32_t> tptr, tpos, trow, gptr, gloc, gdiag, g_sn_ptr;
+    DBuf<int2> gpair;
+    DBuf<int32_t> sn_first, sn_k, sn_c, child_ptr, child, rmap, rows, perm, pth, pv, lists;
+    DBuf<int64_t> foff, rmoff, uoff, rows_off;
+    DBuf<DevStatus> st;
+    DevStatus* hst = nullptr;  // pinned
+    PrecBufs<float> f32;
+    PrecBufs<double> f64;
+    // per-level launch lists: [level][class] -> (offset into lists, count)
+    std::vector<int> lv_off[3], lv_cnt[3];
+    int n_levels = 0;
+    bool results_valid = false;
+};
+
+Instance::Instance(int device, int precision) : device_(device), precision_(precision) {
+    HIPCHECK(hipSetDevice(device_));
+    d_ = new Impl();
+    HIPCHECK(hipStreamCreateWithFlags(&d_->stream, hipStreamNonBlocking));
+    HIPCHECK(hipEventCreate(&d_->ev0));
+    HIPCHECK(hipEventCreate(&d_->ev1));
+    HIPCHECK(hipHostMalloc((void**)&d_->hst, sizeof(DevStatus), hipHostMallocDefault));
+    d_->st.alloc(1, bytes_);
+}
+
+Instance::~Instance() {
+    if (!d_) return;
+    (void)hipSetDevice(device_);
+    if (d_->stream) (void)hipStreamSynchronize(d_->stream);
+    size_t b = bytes_;
+    // DBuf destructors do not free; release explicitly
+    auto rel = [&](auto& buf) { buf.release(b); };
+    rel(d_->bus_type); rel(d_->from); rel(d_->to); rel(d_->vset); rel(d_->gs); rel(d_->bs); rel(d_->bc);
+    rel(d_->tap); rel(d_->shift); rel(d_->status); rel(d_->yrow); rel(d_->ycol); rel(d_->ycptr); rel(d_->ycon);
+    rel(d_->yG); rel(d_->yB); rel(d_->Pij); rel(d_->Qij); rel(d_->V); rel(d_->th); rel(d_->P); rel(d_->Q);
+    rel(d_->Pf); rel(d_->Qf); rel(d_->Pt); rel(d_->Qt); rel(d_->mcode); rel(d_->mloc); rel(d_->hrow);
+    rel(d_->z); rel(d_->sigma); rel(d_->rs); rel(d_->hx); rel(d_->r); rel(d_->jpart); rel(d_->rpart);
+    rel(d_->mpart); rel(d_->ipart); rel(d_->zidx); rel(d_->zval); rel(d_->tptr); rel(d_->tpos); rel(d_->trow);
+    rel(d_->gptr); rel(d_->gloc); rel(d_->gdiag); rel(d_->g_sn_ptr); rel(d_->gpair); rel(d_->sn_first);
+    rel(d_->sn_k); rel(d_->sn_c); rel(d_->child_ptr); rel(d_->child); rel(d_->rmap); rel(d_->rows);
+    rel(d_->perm); rel(d_->pth); rel(d_->pv); rel(d_->lists); rel(d_->foff); rel(d_->rmoff); rel(d_->uoff);
+    rel(d_->rows_off); rel(d_->st);
+    d_->f32.release(b);
+    d_->f64.release(b);
+    if (d_->hst) (void)hipHostFree(d_->hst);
+    if (d_->ev0) (void)hipEventDestroy(d_->ev0);
+    if (d_->ev1) (void)hipEventDestroy(d_->ev1);
+    if (d_->stream) (void)hipStreamDestroy(d_->stream);
+    delete d_;
+    d_ = nullptr;
+}
+
+void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
+    HIPCHECK(hipSetDevice(device_));
+    S_ = std::move(Sp);
+    const Symbolic& S = *S_;
+    Impl& D = *d_;
+    hipStream_t s = D.stream;
+    D.n = h.n;
+    D.nb = h.nb;
+    D.m = h.m;
+    D.ns = S.ns;
+    D.slack = h.slack;
+    if (S.max_c > kVecMax) throw std::runtime_error("front larger than the supported maximum (2048)");
+    D.bus_type.upload(h.bus_type, bytes_, s);
+    D.vset.upload(h.vset, bytes_, s);
+    D.from.upload(h.from, bytes_, s);
+    D.to.upload(h.to, bytes_, s);
+    D.gs.upload(h.gs, bytes_, s);
+    D.bs.upload(h.bs, bytes_, s);
+    D.bc.upload(h.bc, bytes_, s);
+    D.tap.upload(h.tap, bytes_, s);
+    D.shift.upload(h.shift, bytes_, s);
+    D.status.upload(h.status, bytes_, s);
+    D.yrow.upload(S.yrow, bytes_, s);
+    D.ycol.upload(S.ycol, bytes_, s);
+    D.ycptr.upload(S.ycptr, bytes_, s);
+    D.ycon.upload(S.ycon, bytes_, s);
+    size_t nY = S.ycol.size();
+    if (D.yG.n != nY) {
+        D.yG.alloc(nY, bytes_);
+        D.yB.alloc(nY, bytes_);
+        D.Pij.alloc(nY, bytes_);
+        D.Qij.alloc(nY, bytes_);
+    }
+    if (D.V.n != (size_t)h.n) {
+        std::vector<double> V0(h.n, 1.0), t0(h.n, 0.0);
+        for (int b = 0; b < h.n; ++b)
+            if (h.bus_type[b] != 0) V0[b] = h.vset[b];
+        D.V.upload(V0, bytes_, s);
+        D.th.upload(t0, bytes_, s);
+        D.P.alloc(h.n, bytes_);
+        D.Q.alloc(h.n, bytes_);
+    }
+    if (D.Pf.n != (size_t)std::max(1, h.nb)) {
+        D.Pf.alloc(std::max(1, h.nb), bytes_);
+        D.Qf.alloc(std::max(1, h.nb), bytes_);
+        D.Pt.alloc(std::max(1, h.nb), bytes_);
+        D.Qt.alloc(std::max(1, h.nb), bytes_);
+    }
+    std::vector<uint8_t> code(h.m);
+    for (int i = 0; i < h.m; ++i) code[i] = meas_code(h.mtype[i], h.mend[i], h.active[i]);
+    D.mcode.upload(code, bytes_, s);
+    D.mloc.upload(h.mloc, bytes_, s);
+    D.hrow.upload(S.hrow, bytes_, s);
+    D.z.upload(h.z, bytes_, s);
+    D.sigma.upload(h.sigma, bytes_, s);
+    if (D.rs.n != (size_t)std::max(1, h.m)) {
+        D.rs.alloc(std::max(1, h.m), bytes_);
+        D.hx.alloc(std::max(1, h.m), bytes_);
+        D.r.alloc(std::max(1, h.m), bytes_);
+        int nparts = nblk(std::max(1, h.m));
+        D.jpart.alloc(nparts, bytes_);
+        D.rpart.alloc(nparts, bytes_);
+        D.ipart.alloc(nparts, bytes_);
+    }
+    D.mpart.alloc(nblk(std::max(1, h.n)), bytes_);
+    D.tptr.upload(S.tptr, bytes_, s);
+    D.tpos.upload(S.tpos, bytes_, s);
+    D.trow.upload(S.trow, bytes_, s);
+    D.gptr.upload(S.gpair_ptr, bytes_, s);
+    {
+        std::vector<int2> gp(S.gpair.size() / 2);
+        for (size_t q = 0; q < gp.size(); ++q) gp[q] = make_int2(S.gpair[2 * q], S.gpair[2 * q + 1]);
+        if (gp.empty()) gp.push_back(make_int2(0, 0));
+        D.gpair.upload(gp, bytes_, s);
+        HIPCHECK(hipStreamSynchronize(s));
+    }
+    D.gloc.upload(S.gloc, bytes_, s);
+    D.gdiag.upload(S.gdiag, bytes_, s);
+    D.g_sn_ptr.upload(S.g_sn_ptr, bytes_, s);
+    D.sn_first.upload(S.sn_first, bytes_, s);
+    D.sn_k.upload(S.sn_k, bytes_, s);
+    D.sn_c.upload(S.sn_c, bytes_, s);
+    D.child_ptr.upload(S.sn_child_ptr, bytes_, s);
+    {
+        std::vector<int32_t> ch = S.sn_child;
+        if (ch.empty()) ch.push_back(0);
+        D.child.upload(ch, bytes_, s);
+        std::vector<int32_t> rm = S.rmap;
+        if (rm.empty()) rm.push_back(0);
+        D.rmap.upload(rm, bytes_, s);
+        HIPCHECK(hipStreamSynchronize(s));
+    }
+    D.rows.upload(S.sn_rows, bytes_, s);
+    D.perm.upload(S.perm, bytes_, s);
+    D.pth.upload(S.bus_pth, bytes_, s);
+    D.pv.upload(S.bus_pv, bytes_, s);
+    D.foff.upload(S.sn_foff, bytes_, s);
+    D.rmoff.upload(S.sn_rmoff, bytes_, s);
+    D.uoff.upload(S.sn_uoff, bytes_, s);
+    D.rows_off.upload(S.sn_rows_off, bytes_, s);
+    // level lists split by front class
+    std::vector<int32_t> lists;
+    D.n_levels = S.n_levels;
+    for (int c = 0; c < 3; ++c) {
+        D.lv_off[c].assign(S.n_levels, 0);
+        D.lv_cnt[c].assign(S.n_levels, 0);
+    }
+    for (int l = 0; l < S.n_levels; ++l)
+        for (int cls = 0; cls < 3; ++cls) {
+            D.lv_off[cls][l] = (int)lists.size();
+            for (int p = S.level_ptr[l]; p < S.level_ptr[l + 1]; ++p) {
+                int sn = S.level_sn[p];
+                int c = S.sn_c[sn];
+                int k = c <= kFrontA ? 0 : (c <= kFrontB ? 1 : 2);
+                if (k == cls) lists.push_back(sn);
+            }
+            D.lv_cnt[cls][l] = (int)lists.size() - D.lv_off[cls][l];
+        }
+    if (lists.empty()) lists.push_back(0);
+    D.lists.upload(lists, bytes_, s);
+    HIPCHECK(hipStreamSynchronize(s));
+    // precision buffers
+    if (precision_ == 1) {
+        d_->f32.release(bytes_);
+        d_->f64.alloc(S, bytes_);
+    } else {
+        d_->f64.release(bytes_);
+        d_->f32.alloc(S, bytes_);
+    }
+    fp64_factor_active_ = false;
+    factor_valid_ = false;
+    d_->results_valid = false;
+    if (nY) k_ybus<<<nblk(nY), kBlock, 0, s>>>((int)nY, D.ycptr.p, D.ycon.p, D.gs.p, D.bs.p, D.bc.p, D.tap.p,
+                                                D.shift.p, D.status.p, D.yG.p, D.yB.p);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(s));
+}
+
+void Instance::update_branches(const HostArrays& h) {
+    HIPCHECK(hipSetDevice(device_));
+    Impl& D = *d_;
+    hipStream_t s = D.stream;
+    D.gs.upload(h.gs.data(), h.nb, s);
+    D.bs.upload(h.bs.data(), h.nb, s);
+    D.bc.upload(h.bc.data(), h.nb, s);
+    D.tap.upload(h.tap.data(), h.nb, s);
+    D.shift.upload(h.shift.data(), h.nb, s);
+    D.status.upload(h.status.data(), h.nb, s);
+    size_t nY = S_->ycol.size();
+    if (nY) k_ybus<<<nblk(nY), kBlock, 0, s>>>((int)nY, D.ycptr.p, D.ycon.p, D.gs.p, D.bs.p, D.bc.p, D.tap.p,
+                                                D.shift.p, D.status.p, D.yG.p, D.yB.p);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(s));
+    factor_valid_ = false;
+}
+
+void Instance::set_z(const double* z) {
+    HIPCHECK(hipSetDevice(device_));
+    d_->z.upload(z, d_->m, d_->stream);
+    HIPCHECK(hipStreamSynchronize(d_->stream));
+}
+
+void Instance::set_z_indexed(const int32_t* idx, const double* val, int count) {
+    if (count <= 0) return;
+    HIPCHECK(hipSetDevice(device_));
+    Impl& D = *d_;
+    if (D.zidx.n < (size_t)count) {
+        D.zidx.alloc(count, bytes_);
+        D.zval.alloc(count, bytes_);
+    }
+    D.zidx.upload(idx, count, D.stream);
+    D.zval.upload(val, count, D.stream);
+    k_scatter_z<<<nblk(count), kBlock, 0, D.stream>>>(count, D.zidx.p, D.zval.p, D.z.p);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(D.stream));
+}
+
+void Instance::set_state(const double* V, const double* th) {
+    HIPCHECK(hipSetDevice(device_));
+    d_->V.upload(V, d_->n, d_->stream);
+    d_->th.upload(th, d_->n, d_->stream);
+    HIPCHECK(hipStreamSynchronize(d_->stream));
+}
+
+void Instance::get_state(double* V, double* th) {
+    HIPCHECK(hipSetDevice(device_));
+    if (V) HIPCHECK(hipMemcpyAsync(V, d_->V.p, d_->n * sizeof(double), hipMemcpyDeviceToHost, d_->stream));
+    if (th) HIPCHECK(hipMemcpyAsync(th, d_->th.p, d_->n * sizeof(double), hipMemcpyDeviceToHost, d_->stream));
+    HIPCHECK(hipStreamSynchronize(d_->stream));
+}
+
+void Instance::flat_start() {
+    HIPCHECK(hipSetDevice(device_));
+    k_flat<<<nblk(d_->n), kBlock, 0, d_->stream>>>(d_->n, d_->bus_type.p, d_->vset.p, d_->V.p, d_->th.p);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(d_->stream));
+}
+
+namespace {
+
+template <typename T>
+PrecBufs<T>& bufs(Instance::Impl& D);
+template <>
+PrecBufs<float>& bufs<float>(Instance::Impl& D) {
+    return D.f32;
+}
+template <>
+PrecBufs<double>& bufs<double>(Instance::Impl& D) {
+    return D.f64;
+}
+
+}  // namespace
+
+// Kernel launch helpers (member-free so they can be templated on T)
+namespace {
+
+void launch_power(Instance::Impl& D) {
+    k_power<<<nblk(D.n), kBlock, 0, D.stream>>>(D.n, D.yrow.p, D.ycol.p, D.yG.p, D.yB.p, D.V.p, D.th.p, D.P.p,
+                                                D.Q.p, D.Pij.p, D.Qij.p, D.st.p);
+}
+
+template <typename T>
+void launch_meas(Instance::Impl& D, int robust, double gamma, int unit_weights, int want_jac, bool want_out) {
+    MeasArgs A;
+    A.m = D.m;
+    A.n = D.n;
+    A.slack = D.slack;
+    A.robust = robust;
+    A.unit_weights = unit_weights;
+    A.want_jac = want_jac;
+    A.gamma = gamma;
+    A.mcode = D.mcode.p;
+    A.mloc = D.mloc.p;
+    A.z = D.z.p;
+    A.sigma = D.sigma.p;
+    A.hrow = D.hrow.p;
+    A.yrow = D.yrow.p;
+    A.ycol = D.ycol.p;
+    A.yG = D.yG.p;
+    A.yB = D.yB.p;
+    A.Pij = D.Pij.p;
+    A.Qij = D.Qij.p;
+    A.P = D.P.p;
+    A.Q = D.Q.p;
+    A.V = D.V.p;
+    A.th = D.th.p;
+    A.from = D.from.p;
+    A.to = D.to.p;
+    A.gs = D.gs.p;
+    A.bs = D.bs.p;
+    A.bc = D.bc.p;
+    A.tap = D.tap.p;
+    A.shift = D.shift.p;
+    A.rs = D.rs.p;
+    A.hout = want_out ? D.hx.p : nullptr;
+    A.rout = want_out ? D.r.p : nullptr;
+    A.jpart = D.jpart.p;
+    A.rpart = D.rpart.p;
+    A.ipart = D.ipart.p;
+    if (D.m) k_meas<T><<<nblk(D.m), kBlock, 0, D.stream>>>(A, bufs<T>(D).hs.p);
+}
+
+template <typename T>
+void launch_spgemm(Instance::Impl& D, const Symbolic& S) {
+    if (S.nnz_g)
+        k_spgemm<T><<<nblk(S.nnz_g), kBlock, 0, D.stream>>>((int)S.nnz_g, D.gptr.p, D.gpair.p, bufs<T>(D).hs.p,
+                                                            bufs<T>(D).Gval.p);
+}
+
+template <typename T>
+void launch_factor(Instance::Impl& D, T piv_tol) {
+    PrecBufs<T>& B = bufs<T>(D);
+    FactArgs<T> A;
+    A.sn_first = D.sn_first.p;
+    A.sn_k = D.sn_k.p;
+    A.sn_c = D.sn_c.p;
+    A.foff = D.foff.p;
+    A.child_ptr = D.child_ptr.p;
+    A.child = D.child.p;
+    A.rmoff = D.rmoff.p;
+    A.rmap = D.rmap.p;
+    A.g_sn_ptr = D.g_sn_ptr.p;
+    A.gloc = D.gloc.p;
+    A.gdiag = D.gdiag.p;
+    A.Gval = B.Gval.p;
+    A.F = B.F.p;
+    A.piv_tol = piv_tol;
+    A.st = D.st.p;
+    for (int l = 0; l < D.n_levels; ++l) {
+        if (D.lv_cnt[0][l]) k_factor_lds<T, kFrontA><<<D.lv_cnt[0][l], kWave, 0, D.stream>>>(A, D.lists.p + D.lv_off[0][l]);
+        if (D.lv_cnt[1][l]) k_factor_lds<T, kFrontB><<<D.lv_cnt[1][l], kWave, 0, D.stream>>>(A, D.lists.p + D.lv_off[1][l]);
+        if (D.lv_cnt[2][l]) k_factor_global<T><<<D.lv_cnt[2][l], kBlock, 0, D.stream>>>(A, D.lists.p + D.lv_off[2][l]);
+    }
+}
+
+template <typename T>
+SolveArgs<T> solve_args(Instance::Impl& D) {
+    PrecBufs<T>& B = bufs<T>(D);
+    SolveArgs<T> A;
+    A.sn_first = D.sn_first.p;
+    A.sn_k = D.sn_k.p;
+    A.sn_c = D.sn_c.p;
+    A.foff = D.foff.p;
+    A.rows_off = D.rows_off.p;
+    A.rows = D.rows.p;
+    A.child_ptr = D.child_ptr.p;
+    A.child = D.child.p;
+    A.rmoff = D.rmoff.p;
+    A.rmap = D.rmap.p;
+    A.uoff = D.uoff.p;
+    A.F = B.F.p;
+    A.x = B.x.p;
+    A.ubuf = B.ubuf.p;
+    return A;
+}
+
+template <typename T>
+void launch_trisolve(Instance::Impl& D) {
+    SolveArgs<T> A = solve_args<T>(D);
+    for (int l = 0; l < D.n_levels; ++l) {
+        if (D.lv_cnt[0][l]) k_fwd<T, kFrontA><<<D.lv_cnt[0][l], kWave, 0, D.stream>>>(A, D.lists.p + D.lv_off[0][l]);
+        if (D.lv_cnt[1][l]) k_fwd<T, kFrontB><<<D.lv_cnt[1][l], kWave, 0, D.stream>>>(A, D.lists.p + D.lv_off[1][l]);
+        if (D.lv_cnt[2][l]) k_fwd<T, kVecMax><<<D.lv_cnt[2][l], kWave, 0, D.stream>>>(A, D.lists.p + D.lv_off[2][l]);
+    }
+    for (int l = D.n_levels - 1; l >= 0; --l) {
+        if (D.lv_cnt[0][l]) k_bwd<T, kFrontA><<<D.lv_cnt[0][l], kWave, 0, D.stream>>>(A, D.lists.p + D.lv_off[0][l]);
+        if (D.lv_cnt[1][l]) k_bwd<T, kFrontB><<<D.lv_cnt[1][l], kWave, 0, D.stream>>>(A, D.lists.p + D.lv_off[1][l]);
+        if (D.lv_cnt[2][l]) k_bwd<T, kVecMax><<<D.lv_cnt[2][l], kWave, 0, D.stream>>>(A, D.lists.p + D.lv_off[2][l]);
+    }
+}
+
+template <typename T>
+void launch_rhs(Instance::Impl& D) {
+    if (D.ns)
+        k_rhs<T><<<nblk(D.ns), kBlock, 0, D.stream>>>(D.ns, D.tptr.p, D.tpos.p, D.trow.p, bufs<T>(D).hs.p, D.rs.p,
+                                                      bufs<T>(D).x.p);
+}
+
+template <typename T>
+void launch_update(Instance::Impl& D, double vmin, double vmax) {
+    int nbk = nblk(D.n);
+    k_update<T><<<nbk, kBlock, 0, D.stream>>>(D.n, D.pth.p, D.pv.p, bufs<T>(D).x.p, D.V.p, D.th.p, vmin, vmax,
+                                              D.mpart.p, D.st.p);
+    k_maxfinish<<<1, kBlock, 0, D.stream>>>(nbk, D.mpart.p, D.st.p);
+}
+
+void read_status(Instance::Impl& D) {
+    HIPCHECK(hipMemcpyAsync(D.hst, D.st.p, sizeof(DevStatus), hipMemcpyDeviceToHost, D.stream));
+    HIPCHECK(hipStreamSynchronize(D.stream));
+}
+
+template <typename T>
+void iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p, bool refactor, T piv_tol) {
+    launch_power(D);
+    launch_meas<T>(D, p.robust, p.gamma, 0, 1, false);
+    if (refactor) {
+        launch_spgemm<T>(D, S);
+        launch_factor<T>(D, piv_tol);
+    }
+    launch_rhs<T>(D);
+    launch_trisolve<T>(D);
+    launch_update<T>(D, p.vmin, p.vmax);
+}
+
+}  // namespace
+
+SolveOutcome Instance::solve(const SolveParams& p) {
+    HIPCHECK(hipSetDevice(device_));
+    Impl& D = *d_;
+    const Symbolic& S = *S_;
+    SolveOutcome out;
+    auto t0 = std::chrono::steady_clock::now();
+    HIPCHECK(hipEventRecord(D.ev0, D.stream));
+    if (p.flat_start) k_flat<<<nblk(D.n), kBlock, 0, D.stream>>>(D.n, D.bus_type.p, D.vset.p, D.V.p, D.th.p);
+    int retries = 0;
+    bool retry = false;
+    do {
+        if (retry) {
+            k_flat<<<nblk(D.n), kBlock, 0, D.stream>>>(D.n, D.bus_type.p, D.vset.p, D.V.p, D.th.p);
+            factor_valid_ = false;
+            retry = false;
+        }
+        bool converged = false, diverged = false, singular = false, timed_out = false;
+        out.iterations = 0;
+        for (int it = 0; it < p.max_iter; ++it) {
+            const bool fp64 = precision_ == 1 || fp64_factor_active_;
+            const bool refactor = p.robust || !factor_valid_ || p.mode == 1;
+            if (fp64) {
+                if (!D.f64.ready) D.f64.alloc(S, bytes_);
+                iteration<double>(D, S, p, refactor, 1e-13);
+            } else {
+                iteration<float>(D, S, p, refactor, 1e-5f);
+            }
+            HIPCHECK(hipGetLastError());
+            read_status(D);
+            if (D.hst->piv) {
+                factor_valid_ = false;
+                if (!fp64) {
+                    // FP32 pivot below threshold: refactor this iteration in FP64
+                    fp64_factor_active_ = true;
+                    out.fp64_refactors++;
+                    --it;
+                    continue;
+                }
+                singular = true;
+                break;
+            }
+            if (refactor) {
+                factor_valid_ = true;
+                out.refactors++;
+            }
+            out.iterations = it + 1;
+            out.max_mismatch = D.hst->maxdx;
+            if (D.hst->nan) {
+                diverged = true;
+                break;
+            }
+            if (D.hst->maxdx < p.tol) {
+                converged = true;
+                break;
+            }
+            if (p.mode == 0 && p.time_limit_ms > 0) {
+                double el = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                if (el > p.time_limit_ms) {
+                    timed_out = true;
+                    break;
+                }
+            }
+        }
+        if (singular)
+            out.status = 3;
+        else if (diverged)
+            out.status = 2;
+        else if (converged)
+            out.status = 0;
+        else
+            out.status = 1;
+        (void)timed_out;
+        if (diverged && retries < p.retries && p.mode != 0) {
+            retry = true;
+            retries++;
+        }
+    } while (retry);
+    // quantities at the final state
+    launch_power(D);
+    if (D.nb)
+        k_flows<<<nblk(D.nb), kBlock, 0, D.stream>>>(D.nb, D.from.p, D.to.p, D.gs.p, D.bs.p, D.bc.p, D.tap.p,
+                                                     D.shift.p, D.status.p, D.V.p, D.th.p, D.Pf.p, D.Qf.p, D.Pt.p,
+                                                     D.Qt.p);
+    if (precision_ == 1 || fp64_factor_active_)
+        launch_meas<double>(D, 0, p.gamma, 0, 0, true);
+    else
+        launch_meas<float>(D, 0, p.gamma, 0, 0, true);
+    if (D.m) k_jfinish<<<1, kBlock, 0, D.stream>>>(nblk(D.m), D.jpart.p, D.rpart.p, D.ipart.p, D.st.p);
+    HIPCHECK(hipEventRecord(D.ev1, D.stream));
+    HIPCHECK(hipGetLastError());
+    read_status(D);
+    out.objective = D.m ? D.hst->J : 0.0;
+    out.largest_residual = D.m ? D.hst->rmax : 0.0;
+    out.largest_residual_idx = D.m ? D.hst->ridx : -1;
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, D.ev0, D.ev1));
+    out.ms = ms;
+    D.results_valid = true;
+    return out;
+}
+
+void Instance::get_results(double* P, double* Q, double* Pf, double* Qf, double* Pt, double* Qt, double* hx,
+                           double* r) {
+    HIPCHECK(hipSetDevice(device_));
+    Impl& D = *d_;
+    auto cp = [&](double* dst, const DBuf<double>& src, size_t cnt) {
+        if (dst && cnt) HIPCHECK(hipMemcpyAsync(dst, src.p, cnt * sizeof(double), hipMemcpyDeviceToHost, D.stream));
+    };
+    cp(P, D.P, D.n);
+    cp(Q, D.Q, D.n);
+    cp(Pf, D.Pf, D.nb);
+    cp(Qf, D.Qf, D.nb);
+    cp(Pt, D.Pt, D.nb);
+    cp(Qt, D.Qt, D.nb);
+    cp(hx, D.hx, D.m);
+    cp(r, D.r, D.m);
+    HIPCHECK(hipStreamSynchronize(D.stream));
+}
+
+void Instance::debug_eval(double* hx, double* P, double* Q, double* Pf, double* Qf, double* Pt, double* Qt) {
+    HIPCHECK(hipSetDevice(device_));
+    Impl& D = *d_;
+    launch_power(D);
+    if (D.nb)
+        k_flows<<<nblk(D.nb), kBlock, 0, D.stream>>>(D.nb, D.from.p, D.to.p, D.gs.p, D.bs.p, D.bc.p, D.tap.p,
+                                                     D.shift.p, D.status.p, D.V.p, D.th.p, D.Pf.p, D.Qf.p, D.Pt.p,
+                                                     D.Qt.p);
+    if (precision_ == 1)
+        launch_meas<double>(D, 0, 1.5, 0, 0, true);
+    else
+        launch_meas<float>(D, 0, 1.5, 0, 0, true);
+    HIPCHECK(hipGetLastError());
+    get_results(P, Q, Pf, Qf, Pt, Qt, hx, nullptr);
+}
+
+void Instance::debug_ybus(double* G, double* B) {
+    HIPCHECK(hipSetDevice(device_));
+    size_t nY = S_->ycol.size();
+    HIPCHECK(hipMemcpyAsync(G, d_->yG.p, nY * sizeof(double), hipMemcpyDeviceToHost, d_->stream));
+    HIPCHECK(hipMemcpyAsync(B, d_->yB.p, nY * sizeof(double), hipMemcpyDeviceToHost, d_->stream));
+    HIPCHECK(hipStreamSynchronize(d_->stream));
+}
+
+namespace {
+template <typename T>
+void fetch_as_double(const DBuf<T>& src, size_t cnt, double* dst, hipStream_t s) {
+    std::vector<T> tmp(cnt);
+    if (cnt) HIPCHECK(hipMemcpyAsync(tmp.data(), src.p, cnt * sizeof(T), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    for (size_t q = 0; q < cnt; ++q) dst[q] = (double)tmp[q];
+}
+}  // namespace
+
+void Instance::debug_jacobian_gain(double* hv, double* gv) {
+    HIPCHECK(hipSetDevice(device_));
+    Impl& D = *d_;
+    const Symbolic& S = *S_;
+    launch_power(D);
+    if (precision_ == 1) {
+        launch_meas<double>(D, 0, 1.5, 1, 1, false);
+        launch_spgemm<double>(D, S);
+        HIPCHECK(hipGetLastError());
+        if (hv) fetch_as_double(D.f64.hs, S.hcol.size(), hv, D.stream);
+        if (gv) fetch_as_double(D.f64.Gval, S.nnz_g, gv, D.stream);
+    } else {
+        launch_meas<float>(D, 0, 1.5, 1, 1, false);
+        launch_spgemm<float>(D, S);
+        HIPCHECK(hipGetLastError());
+        if (hv) fetch_as_double(D.f32.hs, S.hcol.size(), hv, D.stream);
+        if (gv) fetch_as_double(D.f32.Gval, S.nnz_g, gv, D.stream);
+    }
+}
+
+int Instance::debug_factor(double* fronts) {
+    HIPCHECK(hipSetDevice(device_));
+    Impl& D = *d_;
+    const Symbolic& S = *S_;
+    launch_power(D);
+    if (precision_ == 1) {
+        launch_meas<double>(D, 0, 1.5, 0, 1, false);
+        launch_spgemm<double>(D, S);
+        launch_factor<double>(D, 1e-13);
+        HIPCHECK(hipGetLastError());
+        if (fronts) fetch_as_double(D.f64.F, S.ftotal, fronts, D.stream);
+    } else {
+        launch_meas<float>(D, 0, 1.5, 0, 1, false);
+        launch_spgemm<float>(D, S);
+        launch_factor<float>(D, 1e-5f);
+        HIPCHECK(hipGetLastError());
+        if (fronts) fetch_as_double(D.f32.F, S.ftotal, fronts, D.stream);
+    }
+    read_status(D);
+    factor_valid_ = D.hst->piv == 0;
+    return D.hst->piv;
+}
+
+void Instance::debug_solve(const double* b, double* dx) {
+    HIPCHECK(hipSetDevice(device_));
+    Impl& D = *d_;
+    DBuf<double> bb, xx;
+    size_t tmp = 0;
+    bb.alloc(D.ns, tmp);
+    xx.alloc(D.ns, tmp);
+    bb.upload(b, D.ns, D.stream);
+    if (precision_ == 1) {
+        k_permute_in<double><<<nblk(D.ns), kBlock, 0, D.stream>>>(D.ns, D.perm.p, bb.p, D.f64.x.p);
+        launch_trisolve<double>(D);
+        k_permute_out<double><<<nblk(D.ns), kBlock, 0, D.stream>>>(D.ns, D.perm.p, D.f64.x.p, xx.p);
+    } else {
+        k_permute_in<float><<<nblk(D.ns), kBlock, 0, D.stream>>>(D.ns, D.perm.p, bb.p, D.f32.x.p);
+        launch_trisolve<float>(D);
+        k_permute_out<float><<<nblk(D.ns), kBlock, 0, D.stream>>>(D.ns, D.perm.p, D.f32.x.p, xx.p);
+    }
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(dx, xx.p, D.ns * sizeof(double), hipMemcpyDeviceToHost, D.stream));
+    HIPCHECK(hipStreamSynchronize(D.stream));
+    bb.release(tmp);
+    xx.release(tmp);
+}
+
+double Instance::last_kernel_ms(const char*) const { return 0.0; }
+
+int device_count() {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+    return c;
+}
+
+int runtime_version() {
+    int v = 0;
+    (void)hipRuntimeGetVersion(&v);
+    return v;
+}
+
+const char* device_name(int dev) {
+    static thread_local char buf[256];
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return "unavailable";
+    snprintf(buf, sizeof(buf), "%s (%s), %d CUs, %.1f GiB HBM", prop.name, prop.gcnArchName,
+             prop.multiProcessorCount, prop.totalGlobalMem / (1024.0 * 1024.0 * 1024.0));
+    return buf;
+}
+
+}  // namespace sle

@@ -1,0 +1,224 @@
+// engine.h -- host engine behind the sle_* C-ABI (FP64 internally).
+//
+// Replaces sle::SLEEngine (sle_engine.h:288-1081, sle_engine.cu:55-1773) and
+// sle::NetworkModel (network_model.h:79-668, network_model.cu:47-734).  The
+// engine keeps the host network model and id maps, runs the symbolic analysis
+// when the topology changes, and drives one GPU Instance.  Status codes and
+// enum values are those of sle_api.h (as plain ints here, so this file does not
+// depend on the width of SLE_Real).
+#pragma once
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "instance.h"
+#include "symbolic.h"
+
+namespace sle {
+
+enum Status : int {
+    ST_OK = 0,
+    ST_INVALID_HANDLE = -1,
+    ST_NOT_INITIALIZED = -2,
+    ST_GPU_FAILURE = -3,
+    ST_INVALID_ARGUMENT = -4,
+    ST_MODEL_INCOMPLETE = -5,
+    ST_NOT_OBSERVABLE = -6,
+    ST_SINGULAR = -7,
+    ST_DIVERGED = -8,
+    ST_MAX_ITERATIONS = -9,
+    ST_OUT_OF_MEMORY = -10,
+    ST_NOT_FOUND = -11,
+    ST_DUPLICATE = -12,
+    ST_CONFIG_FILE = -13,
+    ST_UNKNOWN = -99
+};
+
+// sle_engine.config keys (sle_config.h:81-113) with the reference defaults
+// (sle_engine.config:33-73).
+struct EngineConfig {
+    int device_id = 0;
+    size_t max_buses = 200000, max_branches = 400000, max_measurements = 2000000, max_switching_devices = 100000;
+    bool enable_robust = false;
+    double huber_gamma = 1.5;
+    double convergence_tolerance = 1e-4;
+    int max_realtime_iterations = 5;
+    int max_precision_iterations = 100;
+    double realtime_time_limit_ms = 20.0;
+    bool use_flat_start_default = false;
+    bool enable_profiling = false;
+    bool use_cuda_graphs = true;
+    int block_size_standard = 256;
+    int block_size_reduction = 512;
+    bool use_double_precision = false;
+    double voltage_min_pu = 0.5, voltage_max_pu = 1.5;
+    double bad_data_threshold = 3.0;
+    bool auto_bad_data_rejection = false;
+};
+
+bool load_config_file(const std::string& path, EngineConfig& cfg, std::string& err);
+bool save_config_file(const std::string& path, const EngineConfig& cfg, std::string& err);
+
+struct SolverConfigIn {
+    int mode = 0;
+    int max_iterations = 5;
+    double tolerance = 1e-4;
+    double huber_gamma = 1.5;
+    bool use_robust = false;
+    bool use_flat_start = false;
+    double time_limit_ms = 0;
+};
+
+struct ResultOut {
+    int status = 5;
+    int iterations = 0;
+    double max_mismatch = 0, objective = 0, ms = 0, largest_residual = 0;
+    int largest_residual_idx = -1, bad_data_count = 0;
+    int refactors = 0, fp64_refactors = 0;
+};
+
+class Engine {
+   public:
+    Engine();
+    ~Engine();
+
+    // lifecycle / config
+    int initialize();
+    bool initialized() const { return initialized_; }
+    EngineConfig& config() { return cfg_; }
+    void set_config(const EngineConfig& c);
+    std::string config_path;
+
+    // model building (returns Status)
+    int add_bus(const char* id, double base_kv, int type, double vset, double p, double q);
+    int add_branch(const char* id, const char* from, const char* to, double r, double x, double b, double g,
+                   double tap, double shift, int is_xfmr, const char* sd_id);
+    int add_measurement(const char* id, int type, const char* loc, int end, double sigma, double pt, double ct,
+                        int pseudo);
+    int add_switch(const char* id, const char* branch_id, int status);
+    int remove_bus(const char* id);
+    int remove_branch(const char* id);
+    int update_branch(const char* id, double r, double x, double b, double g, double tap, double shift, int is_xfmr);
+    int set_tap(const char* id, double tap);
+    int set_phase_shift(const char* id, double shift);
+    int set_impedance(const char* id, double r, double x, double b, double g);
+    bool get_tap(const char* id, double& tap) const;
+    bool get_phase_shift(const char* id, double& s) const;
+
+    // meters (network_model.cu:312-455 semantics: channels map to measurements)
+    int add_meter(const char* id, int type, const char* bus_id, const char* branch_id, int end, double pt, double ct,
+                  double sv, double sp, double si);
+    int meter_reading(const char* meter, const char* channel, double* value, int which) const;  // 0 z, 1 est, 2 res
+    int update_meter_reading(const char* meter, const char* channel, double value);
+
+    // data
+    int upload_model();
+    int update_telemetry(const double* values, int count);
+    int update_telemetry_indexed(const int32_t* idx, const double* values, int count);
+    int update_measurement(const char* id, double value);
+    int sync_measurements();
+
+    // switches
+    int set_switch_status(const char* sd_id, int status);
+    int get_switch_status(const char* sd_id) const;
+    int apply_switch_changes();
+    int set_branch_status_index(int k, int status);
+
+    // solve
+    int solve(const SolverConfigIn& sc, ResultOut& out);
+    SolverConfigIn mode_config(int mode) const;
+    int check_observability(int& observable, int& islands);
+    int apply_flat_start();
+
+    // results
+    const std::vector<double>& V() const { return V_; }
+    const std::vector<double>& th() const { return th_; }
+    int set_state(const double* V, const double* th, int count);
+    void ensure_results();
+    std::vector<double> P_, Q_, Pf_, Qf_, Pt_, Qt_, h_, r_;
+    double objective() const { return last_.objective; }
+    double last_solve_ms() const { return last_.ms; }
+    const ResultOut& last_result() const { return last_; }
+    std::vector<int32_t> identify_bad_data(double threshold);
+
+    // lookups
+    int bus_index(const char* id) const;
+    int branch_index(const char* id) const;
+    int meas_index(const char* id) const;
+    int switch_index(const char* id) const;
+    int n_bus() const { return (int)bus_id_.size(); }
+    int n_branch() const { return (int)br_id_.size(); }
+    int n_meas() const { return (int)m_id_.size(); }
+    int n_meter() const { return (int)meter_id_.size(); }
+    size_t gpu_bytes() const;
+
+    // measurement metadata (for sle_GetMeasurementDetails)
+    const std::string& meas_id(int i) const { return m_id_[i]; }
+    int meas_type(int i) const { return m_type_[i]; }
+    int meas_loc(int i) const { return m_loc_[i]; }
+    int meas_end(int i) const { return m_end_[i]; }
+    double meas_sigma(int i) const { return m_sigma_[i]; }
+    double meas_pt(int i) const { return m_pt_[i]; }
+    double meas_ct(int i) const { return m_ct_[i]; }
+    int meas_pseudo(int i) const { return m_pseudo_[i]; }
+    double meas_z(int i) const { return m_z_[i]; }
+    bool meas_active(int i) const;
+    const std::string& bus_id(int i) const { return bus_id_[i]; }
+    const std::string& branch_id(int i) const { return br_id_[i]; }
+
+    // introspection for parity tests
+    Instance* instance() { return inst_.get(); }
+    std::shared_ptr<const Symbolic> symbolic() const { return sym_; }
+    int ensure_ready();  // upload/rebuild as needed without solving
+    void load_ieee14();
+
+   private:
+    void build_host_arrays(HostArrays& h) const;
+    void admittance(int k, double& g, double& b) const;
+    int rebuild();
+
+    EngineConfig cfg_;
+    bool initialized_ = false;
+    // buses
+    std::vector<std::string> bus_id_;
+    std::vector<double> bus_kv_, bus_vset_, bus_p_, bus_q_;
+    std::vector<int32_t> bus_type_;
+    std::vector<uint8_t> bus_removed_;
+    int slack_ = -1;
+    // branches
+    std::vector<std::string> br_id_;
+    std::vector<int32_t> br_from_, br_to_, br_xfmr_, br_sd_;
+    std::vector<double> br_r_, br_x_, br_b_, br_g_, br_tap_, br_shift_;
+    std::vector<uint8_t> br_status_, br_removed_;
+    // measurements
+    std::vector<std::string> m_id_;
+    std::vector<int32_t> m_type_, m_loc_, m_end_, m_pseudo_;
+    std::vector<double> m_sigma_, m_pt_, m_ct_, m_z_;
+    std::vector<uint8_t> m_removed_;
+    // switches
+    std::vector<std::string> sd_id_;
+    std::vector<int32_t> sd_branch_, sd_status_, sd_pending_;
+    // meters
+    struct Meter {
+        int type, bus, branch, end;
+        int mv = -1, mp = -1, mq = -1, mi = -1;
+    };
+    std::vector<std::string> meter_id_;
+    std::vector<Meter> meters_;
+    std::unordered_map<std::string, int> bus_map_, br_map_, m_map_, sd_map_, meter_map_;
+
+    // state
+    bool model_dirty_ = true;     // structure changed: symbolic + full upload
+    bool params_dirty_ = false;   // branch parameters changed: Ybus numeric
+    bool z_dirty_ = true;         // telemetry changed on host
+    bool results_dirty_ = true;   // host copies of P/Q/flows/h/r stale
+    std::vector<double> V_, th_;
+    std::shared_ptr<Symbolic> sym_;
+    std::unique_ptr<Instance> inst_;
+    int inst_precision_ = -1;
+    ResultOut last_;
+};
+
+}  // namespace sle

@@ -1,0 +1,131 @@
+// instance.h -- one estimator instance resident on the GPU (no HIP types here).
+//
+// An Instance owns every device buffer of one network: branch parameters,
+// Ybus values, state, measurements, the Jacobian values, the gain entries, the
+// dense supernodal fronts of the factor and the triangular-solve vectors.  It
+// runs the Gauss-Newton / Huber loop (wls_solver.cu:221-431 semantics) with all
+// arithmetic on the device; the host only reads one small status record per
+// iteration to test convergence.
+#pragma once
+#include <cstddef>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "symbolic.h"
+
+namespace sle {
+
+// Host-side branch / bus / measurement arrays in internal indices (FP64).
+struct HostArrays {
+    int n = 0, nb = 0, m = 0, slack = 0;
+    std::vector<int32_t> bus_type;
+    std::vector<double> vset;
+    std::vector<int32_t> from, to;
+    std::vector<double> gs, bs, bc, tap, shift;  // series g, b; total charging; tap; phase shift
+    std::vector<uint8_t> status;
+    std::vector<int32_t> mtype, mloc, mend;
+    std::vector<uint8_t> active;
+    std::vector<double> sigma, z;
+};
+
+struct SolveParams {
+    int mode = 1;          // 0 REALTIME (reuse a valid factor), 1 PRECISION (refactor every iteration)
+    int max_iter = 100;
+    double tol = 1e-4;
+    int robust = 0;
+    double gamma = 1.5;
+    int flat_start = 0;
+    double vmin = 0.5, vmax = 1.5;
+    double time_limit_ms = 0;  // REALTIME only, 0 = none
+    int retries = 2;           // flat-start retries after divergence (not in REALTIME)
+};
+
+struct SolveOutcome {
+    int status = 5;  // SLE_ConvergenceStatus
+    int iterations = 0;
+    int refactors = 0;
+    int fp64_refactors = 0;  // mixed form: pivots that fell below threshold
+    double max_mismatch = 0;
+    double objective = 0;
+    double largest_residual = 0;  // max |r|/sigma over active rows
+    int largest_residual_idx = -1;
+    double ms = 0;
+};
+
+struct KernelTimes {  // accumulated milliseconds per kernel class (profiling mode)
+    double power = 0, meas = 0, spgemm = 0, factor = 0, rhs = 0, fwd = 0, bwd = 0, update = 0;
+    int iters = 0;
+};
+
+// Raised for any HIP runtime failure (mapped to SLE_ERROR_CUDA_FAILURE).
+struct GpuError : std::runtime_error {
+    explicit GpuError(const std::string& s) : std::runtime_error(s) {}
+};
+
+class Instance {
+   public:
+    struct Impl;
+    // precision: 1 = plain FP64 form, 0 = mixed (FP32 H/G/factor, FP64 residual/RHS/state)
+    Instance(int device, int precision);
+    ~Instance();
+    Instance(const Instance&) = delete;
+    Instance& operator=(const Instance&) = delete;
+
+    // Allocate and upload everything for this symbolic analysis.
+    void setup(const HostArrays& h, std::shared_ptr<const Symbolic> S);
+    // Re-upload branch parameters + status and rebuild Ybus values (pattern unchanged).
+    void update_branches(const HostArrays& h);
+    void set_z(const double* z);
+    void set_z_indexed(const int32_t* idx, const double* val, int count);
+    void set_state(const double* V, const double* th);
+    void get_state(double* V, double* th);
+    void flat_start();
+    void invalidate_factor() { factor_valid_ = false; }
+    bool factor_valid() const { return factor_valid_; }
+
+    SolveOutcome solve(const SolveParams& p);
+
+    // Results of the last solve at the final state (downloaded on demand).
+    void get_results(double* P, double* Q, double* Pf, double* Qf, double* Pt, double* Qt, double* hx, double* r);
+
+    // Introspection for parity tests: evaluate at the current state.
+    void debug_eval(double* hx, double* P, double* Q, double* Pf, double* Qf, double* Pt, double* Qt);
+    void debug_ybus(double* G, double* B);
+    // H values (unscaled, natural pattern order) and G entries (symbolic order),
+    // computed with unit Huber weights, in the instance precision.
+    void debug_jacobian_gain(double* hv, double* gv);
+    // Factor the current G and return the dense fronts (symbolic order).
+    int debug_factor(double* fronts);
+    // Solve G dx = b with the current factor (b, dx in natural state order).
+    void debug_solve(const double* b, double* dx);
+
+    size_t device_bytes() const { return bytes_; }
+    int precision() const { return precision_; }
+    const Symbolic& symbolic() const { return *S_; }
+    KernelTimes& kernel_times() { return ktimes_; }
+    void set_profiling(bool on) { profiling_ = on; }
+    void set_use_graphs(bool on) { use_graphs_ = on; }
+    double last_kernel_ms(const char* which) const;
+
+   private:
+    Impl* d_ = nullptr;
+    std::shared_ptr<const Symbolic> S_;
+    int device_ = 0;
+    int precision_ = 0;
+    bool factor_valid_ = false;
+    bool fp64_factor_active_ = false;  // mixed form fell back to FP64 after a pivot trigger
+    bool profiling_ = false;
+    bool use_graphs_ = false;
+    size_t bytes_ = 0;
+    KernelTimes ktimes_;
+};
+
+// Device helpers exposed to the engine.
+int device_count();
+int runtime_version();
+const char* device_name(int dev);
+
+}  // namespace sle
