@@ -180,6 +180,7 @@ int Engine::add_bus(const char* id, double kv, int type, double vset, double p, 
     V_.push_back(type != 0 ? vset : 1.0);
     th_.push_back(0.0);
     model_dirty_ = true;
+    sym_dirty_ = true;
     return ST_OK;
 }
 
@@ -209,6 +210,7 @@ int Engine::add_branch(const char* id, const char* from, const char* to, double 
     br_removed_.push_back(0);
     (void)sd_id;  // a switching device is attached with sle_AddSwitch
     model_dirty_ = true;
+    sym_dirty_ = true;
     return ST_OK;
 }
 
@@ -234,6 +236,7 @@ int Engine::add_measurement(const char* id, int type, const char* loc, int end, 
     m_z_.push_back(0.0);
     m_removed_.push_back(0);
     model_dirty_ = true;
+    sym_dirty_ = true;
     return ST_OK;
 }
 
@@ -253,6 +256,7 @@ int Engine::add_switch(const char* id, const char* branch_id, int status) {
     br_sd_[k] = idx;
     br_status_[k] = status ? 1 : 0;
     model_dirty_ = true;
+    sym_dirty_ = true;
     return ST_OK;
 }
 
@@ -266,6 +270,7 @@ int Engine::remove_branch(const char* id) {
     br_status_[k] = 0;
     br_map_.erase(id);
     model_dirty_ = true;
+    sym_dirty_ = true;
     return ST_OK;
 }
 
@@ -284,6 +289,7 @@ int Engine::remove_bus(const char* id) {
         if (is_bus_meas(m_type_[i]) && m_loc_[i] == b) m_removed_[i] = 1;
     bus_map_.erase(id);
     model_dirty_ = true;
+    sym_dirty_ = true;
     return ST_OK;
 }
 
@@ -468,9 +474,9 @@ void Engine::build_host_arrays(HostArrays& h) const {
     h.z = m_z_;
 }
 
-int Engine::rebuild() {
-    if (!initialized_) return ST_NOT_INITIALIZED;
+int Engine::ensure_symbolic() {
     if (bus_id_.empty() || slack_ < 0) return ST_MODEL_INCOMPLETE;
+    if (sym_ && !model_dirty_ && !sym_dirty_) return ST_OK;
     HostArrays h;
     build_host_arrays(h);
     ModelView mv;
@@ -488,6 +494,17 @@ int Engine::rebuild() {
     mv.active = h.active.data();
     auto S = std::make_shared<Symbolic>();
     analyze(mv, *S);
+    sym_ = S;
+    sym_dirty_ = false;
+    return ST_OK;
+}
+
+int Engine::rebuild() {
+    if (!initialized_) return ST_NOT_INITIALIZED;
+    int rc = ensure_symbolic();
+    if (rc != ST_OK) return rc;
+    HostArrays h;
+    build_host_arrays(h);
     int prec = cfg_.use_double_precision ? 1 : 0;
     if (!inst_ || inst_precision_ != prec) {
         inst_.reset();
@@ -496,9 +513,8 @@ int Engine::rebuild() {
     }
     inst_->set_profiling(cfg_.enable_profiling);
     inst_->set_use_graphs(cfg_.use_cuda_graphs);
-    inst_->setup(h, S);
+    inst_->setup(h, sym_);
     inst_->set_state(V_.data(), th_.data());
-    sym_ = S;
     model_dirty_ = false;
     params_dirty_ = false;
     z_dirty_ = false;
@@ -601,6 +617,7 @@ int Engine::apply_switch_changes() {
         if (!br_removed_[k]) {
             br_status_[k] = (uint8_t)st;
             model_dirty_ = true;
+            sym_dirty_ = true;
         }
     }
     return ST_OK;
@@ -614,6 +631,7 @@ int Engine::set_branch_status_index(int k, int status) {
         br_status_[k] = st;
         if (br_sd_[k] >= 0) sd_status_[br_sd_[k]] = st;
         model_dirty_ = true;
+        sym_dirty_ = true;
     }
     return ST_OK;
 }

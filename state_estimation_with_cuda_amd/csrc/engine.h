@@ -171,7 +171,8 @@ class Engine {
     // introspection for parity tests
     Instance* instance() { return inst_.get(); }
     std::shared_ptr<const Symbolic> symbolic() const { return sym_; }
-    int ensure_ready();  // upload/rebuild as needed without solving
+    int ensure_ready();     // upload/rebuild as needed without solving
+    int ensure_symbolic();  // host symbolic analysis only (no device needed)
     void load_ieee14();
 
    private:
@@ -211,6 +212,7 @@ class Engine {
 
     // state
     bool model_dirty_ = true;     // structure changed: symbolic + full upload
+    bool sym_dirty_ = true;       // host symbolic analysis stale
     bool params_dirty_ = false;   // branch parameters changed: Ybus numeric
     bool z_dirty_ = true;         // telemetry changed on host
     bool results_dirty_ = true;   // host copies of P/Q/flows/h/r stale

@@ -918,8 +918,9 @@ SLE_API SLE_StatusCode SLE_CALL sle_ext_SetBranchStatus(SLE_Handle handle, int32
     });
 }
 
-static SLE_StatusCode need_symbolic(SLE_Handle handle, std::shared_ptr<const sle::Symbolic>& S) {
-    int rc = E(handle).ensure_ready();
+static SLE_StatusCode need_symbolic(SLE_Handle handle, std::shared_ptr<const sle::Symbolic>& S,
+                                    bool device = true) {
+    int rc = (device && E(handle).initialized()) ? E(handle).ensure_ready() : E(handle).ensure_symbolic();
     if (rc != sle::ST_OK) return from_int(rc, "prepare model");
     S = E(handle).symbolic();
     if (!S) return fail(SLE_ERROR_NOT_INITIALIZED, "model not uploaded");
@@ -930,7 +931,7 @@ SLE_API SLE_StatusCode SLE_CALL sle_ext_GetSizes(SLE_Handle handle, int64_t* siz
     if (!handle || !sizes) return fail(SLE_ERROR_INVALID_ARGUMENT, "NULL argument");
     return guarded([&]() -> SLE_StatusCode {
         std::shared_ptr<const sle::Symbolic> S;
-        SLE_StatusCode rc = need_symbolic(handle, S);
+        SLE_StatusCode rc = need_symbolic(handle, S, false);
         if (rc != SLE_OK) return rc;
         sizes[0] = S->ns;
         sizes[1] = (int64_t)S->ycol.size();
@@ -953,11 +954,14 @@ SLE_API SLE_StatusCode SLE_CALL sle_ext_GetYbus(SLE_Handle handle, int32_t* row_
     if (!handle) return fail(SLE_ERROR_INVALID_HANDLE, "NULL handle");
     return guarded([&]() -> SLE_StatusCode {
         std::shared_ptr<const sle::Symbolic> S;
-        SLE_StatusCode rc = need_symbolic(handle, S);
+        SLE_StatusCode rc = need_symbolic(handle, S, false);
         if (rc != SLE_OK) return rc;
         if (row_ptr) std::copy(S->yrow.begin(), S->yrow.end(), row_ptr);
         if (col) std::copy(S->ycol.begin(), S->ycol.end(), col);
-        if (g && b) E(handle).instance()->debug_ybus(g, b);
+        if (g && b) {
+            if (!E(handle).instance()) return fail(SLE_ERROR_NOT_INITIALIZED, "Ybus values need an initialized engine");
+            E(handle).instance()->debug_ybus(g, b);
+        }
         return SLE_OK;
     });
 }
@@ -966,7 +970,7 @@ SLE_API SLE_StatusCode SLE_CALL sle_ext_GetHPattern(SLE_Handle handle, int32_t* 
     if (!handle) return fail(SLE_ERROR_INVALID_HANDLE, "NULL handle");
     return guarded([&]() -> SLE_StatusCode {
         std::shared_ptr<const sle::Symbolic> S;
-        SLE_StatusCode rc = need_symbolic(handle, S);
+        SLE_StatusCode rc = need_symbolic(handle, S, false);
         if (rc != SLE_OK) return rc;
         if (row_ptr) std::copy(S->hrow.begin(), S->hrow.end(), row_ptr);
         if (col) std::copy(S->hcol.begin(), S->hcol.end(), col);
@@ -978,7 +982,7 @@ SLE_API SLE_StatusCode SLE_CALL sle_ext_GetGPattern(SLE_Handle handle, int32_t* 
     if (!handle) return fail(SLE_ERROR_INVALID_HANDLE, "NULL handle");
     return guarded([&]() -> SLE_StatusCode {
         std::shared_ptr<const sle::Symbolic> S;
-        SLE_StatusCode rc = need_symbolic(handle, S);
+        SLE_StatusCode rc = need_symbolic(handle, S, false);
         if (rc != SLE_OK) return rc;
         std::vector<int32_t> cp, rw;
         sle::natural_gain_pattern(*S, cp, rw);
@@ -993,7 +997,7 @@ SLE_API SLE_StatusCode SLE_CALL sle_ext_GetOrdering(SLE_Handle handle, int32_t* 
     if (!handle) return fail(SLE_ERROR_INVALID_HANDLE, "NULL handle");
     return guarded([&]() -> SLE_StatusCode {
         std::shared_ptr<const sle::Symbolic> S;
-        SLE_StatusCode rc = need_symbolic(handle, S);
+        SLE_StatusCode rc = need_symbolic(handle, S, false);
         if (rc != SLE_OK) return rc;
         if (bus_perm) std::copy(S->bus_perm.begin(), S->bus_perm.end(), bus_perm);
         if (perm) std::copy(S->perm.begin(), S->perm.end(), perm);
@@ -1012,6 +1016,7 @@ SLE_API SLE_StatusCode SLE_CALL sle_ext_Evaluate(SLE_Handle handle, double* h, d
     if (!handle) return fail(SLE_ERROR_INVALID_HANDLE, "NULL handle");
     return guarded([&]() -> SLE_StatusCode {
         std::shared_ptr<const sle::Symbolic> S;
+        if (!E(handle).initialized()) return fail(SLE_ERROR_NOT_INITIALIZED, "engine not initialized");
         SLE_StatusCode rc = need_symbolic(handle, S);
         if (rc != SLE_OK) return rc;
         E(handle).instance()->debug_eval(h, p_inj, q_inj, p_from, q_from, p_to, q_to);
@@ -1023,6 +1028,7 @@ SLE_API SLE_StatusCode SLE_CALL sle_ext_JacobianGain(SLE_Handle handle, double* 
     if (!handle) return fail(SLE_ERROR_INVALID_HANDLE, "NULL handle");
     return guarded([&]() -> SLE_StatusCode {
         std::shared_ptr<const sle::Symbolic> S;
+        if (!E(handle).initialized()) return fail(SLE_ERROR_NOT_INITIALIZED, "engine not initialized");
         SLE_StatusCode rc = need_symbolic(handle, S);
         if (rc != SLE_OK) return rc;
         std::vector<double> gsym(std::max<int64_t>(1, S->nnz_g));
@@ -1041,6 +1047,7 @@ SLE_API SLE_StatusCode SLE_CALL sle_ext_Factor(SLE_Handle handle, int64_t* col_p
     if (!handle) return fail(SLE_ERROR_INVALID_HANDLE, "NULL handle");
     return guarded([&]() -> SLE_StatusCode {
         std::shared_ptr<const sle::Symbolic> S;
+        if (!E(handle).initialized()) return fail(SLE_ERROR_NOT_INITIALIZED, "engine not initialized");
         SLE_StatusCode rc = need_symbolic(handle, S);
         if (rc != SLE_OK) return rc;
         std::vector<double> fr(std::max<int64_t>(1, S->ftotal));
@@ -1069,6 +1076,7 @@ SLE_API SLE_StatusCode SLE_CALL sle_ext_SolveFactor(SLE_Handle handle, const dou
     if (!handle || !b || !dx) return fail(SLE_ERROR_INVALID_ARGUMENT, "NULL argument");
     return guarded([&]() -> SLE_StatusCode {
         std::shared_ptr<const sle::Symbolic> S;
+        if (!E(handle).initialized()) return fail(SLE_ERROR_NOT_INITIALIZED, "engine not initialized");
         SLE_StatusCode rc = need_symbolic(handle, S);
         if (rc != SLE_OK) return rc;
         E(handle).instance()->debug_solve(b, dx);

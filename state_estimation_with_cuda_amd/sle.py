@@ -251,7 +251,7 @@ class Library:
             "sle_ext_GetResults": (C.c_int, [H] + [C.c_void_p] * 8),
             "sle_ext_SetBranchStatus": (C.c_int, [H, i32, C.c_int]),
             "sle_ext_GetSizes": (C.c_int, [H, lp]),
-            "sle_ext_GetYbus": (C.c_int, [H, ip, ip, dp, dp]),
+            "sle_ext_GetYbus": (C.c_int, [H, ip, ip, C.c_void_p, C.c_void_p]),
             "sle_ext_GetHPattern": (C.c_int, [H, ip, ip]),
             "sle_ext_GetGPattern": (C.c_int, [H, ip, ip]),
             "sle_ext_GetOrdering": (C.c_int, [H, ip, ip, ip, ip, ip]),
@@ -515,9 +515,12 @@ class Engine:
         sz = self.sizes()
         rp = np.zeros(self.n + 1, np.int32)
         col = np.zeros(sz["nnz_y"], np.int32)
+        if not values:
+            self._check(self.L.sle_ext_GetYbus(self.h, rp, col, None, None), "sle_ext_GetYbus")
+            return rp, col
         G = np.zeros(sz["nnz_y"])
         B = np.zeros(sz["nnz_y"])
-        self._check(self.L.sle_ext_GetYbus(self.h, rp, col, G, B), "sle_ext_GetYbus")
+        self._check(self.L.sle_ext_GetYbus(self.h, rp, col, G.ctypes.data, B.ctypes.data), "sle_ext_GetYbus")
         return rp, col, G, B
 
     def h_pattern(self):

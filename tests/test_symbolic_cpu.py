@@ -1,0 +1,93 @@
+"""Host symbolic analysis of the library (no GPU needed) against the oracle.
+
+The library's symbolic phase (Ybus / H / G patterns, minimum-degree ordering,
+elimination tree, column counts, fundamental supernodes) is an independent
+implementation from the oracle's; both must agree bit-exactly.  The oracle's
+own symbolic results are pinned by brute force in test_oracle_pins.py.
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle
+from state_estimation_with_cuda_amd import gridgen as gg
+from state_estimation_with_cuda_amd import sle
+
+
+def host_engine(g):
+    e = sle.Engine(double=True, initialize=False)
+    e.load_grid(g)
+    return e
+
+
+CASES = {
+    "grid14": lambda: gg.grid14(0),
+    "grid14_seed5": lambda: gg.grid14(5),
+    "geo500_partial_flows": lambda: gg.geometric(500, seed=4, flow_meas=700),
+    "geo3000": lambda: gg.geometric(3000, seed=9, flow_meas="all", dup_meas=500),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_patterns_and_ordering_match_oracle(name):
+    g = CASES[name]()
+    e = host_engine(g)
+    o = Oracle(g)
+    rp, col = e.h_pattern()
+    orp, ocol = o.h_pattern()
+    assert np.array_equal(rp, orp) and np.array_equal(col, ocol)
+    gc, gr = e.g_pattern()
+    ogc, ogr = o.g_pattern()
+    assert np.array_equal(gc, ogc) and np.array_equal(gr, ogr)
+    es, os_ = e.ordering(), o.symbolic()
+    for k in ("bus_perm", "perm", "parent", "colcount", "snode_ptr"):
+        assert np.array_equal(es[k], os_[k]), k
+    sz = e.sizes()
+    assert sz["nnz_l"] == o.nnz_l
+    assert sz["n_snodes"] == o.n_snodes
+    # Ybus pattern (values need the device)
+    yrp, ycol = e.ybus(values=False)
+    orp_y, ocol_y, _, _ = o.ybus()
+    assert np.array_equal(yrp, orp_y) and np.array_equal(ycol, ocol_y)
+    e.close()
+
+
+def test_open_branch_changes_patterns_like_oracle():
+    g = gg.geometric(400, seed=8, flow_meas="all")
+    # open a branch on a cycle (keep the grid connected)
+    import scipy.sparse as sp
+    import scipy.sparse.csgraph as cg
+
+    k = None
+    for cand in range(g.nb):
+        keep = np.ones(g.nb, bool)
+        keep[cand] = False
+        A = sp.coo_matrix((np.ones(keep.sum()), (g.br_from[keep], g.br_to[keep])), shape=(g.n, g.n))
+        if cg.connected_components(A, directed=False)[0] == 1:
+            k = cand
+            break
+    assert k is not None
+    e = host_engine(g)
+    e.set_branch_status(k, 0)
+    o = Oracle(g)
+    o.set_branch_status(k, 0)
+    rp, col = e.h_pattern()
+    orp, ocol = o.h_pattern()
+    assert np.array_equal(rp, orp) and np.array_equal(col, ocol)
+    # flow rows of the open branch are empty
+    flows_k = np.nonzero((g.m_loc == k) & np.isin(g.m_type, (4, 5)))[0]
+    assert flows_k.size and np.all(rp[flows_k + 1] == rp[flows_k])
+    es, os_ = e.ordering(), o.symbolic()
+    for key in ("bus_perm", "perm", "parent", "colcount", "snode_ptr"):
+        assert np.array_equal(es[key], os_[key]), key
+    e.close()
+
+
+def test_symbolic_sizes_200k_scale_structure():
+    """Structure of a 20k-bus mesh: fronts stay small under minimum degree."""
+    g = gg.geometric(20_000, seed=2, flow_meas="all")
+    e = host_engine(g)
+    sz = e.sizes()
+    assert sz["ns"] == 2 * g.n - 1
+    assert sz["max_front"] <= 64
+    assert sz["nnz_l"] < 3 * sz["nnz_g"]
+    e.close()
