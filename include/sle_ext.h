@@ -27,7 +27,7 @@ typedef struct SLE_ExtResult {
     double objective;
     double largest_residual; /* max |z-h|/sigma over active rows */
     int32_t largest_residual_idx;
-    int32_t reserved;
+    int32_t flags;       /* bit 0: dependency-driven factor/solve sweeps; bit 1: FP64 factor in use */
     double gpu_ms;       /* device time of the solve */
     double host_ms;      /* wall time of the call */
 } SLE_ExtResult;
@@ -52,6 +52,23 @@ SLE_API SLE_StatusCode SLE_CALL sle_ext_GetState(SLE_Handle handle, double* v, d
 SLE_API SLE_StatusCode SLE_CALL sle_ext_GetResults(SLE_Handle handle, double* p_inj, double* q_inj,
                                                    double* p_from, double* q_from, double* p_to, double* q_to,
                                                    double* h, double* residual);
+
+/* Device-resident telemetry batches.  Stage copies (idx, values) into HBM
+ * slot `slot` ahead of time; Apply scatters the slot into the measurement
+ * vector on the device (asynchronously, ordered before the next solve).  A
+ * slot can be applied any number of times. */
+SLE_API SLE_StatusCode SLE_CALL sle_ext_StageTelemetry(SLE_Handle handle, int32_t slot, const int32_t* idx,
+                                                       const double* values, int32_t count);
+SLE_API SLE_StatusCode SLE_CALL sle_ext_ApplyStagedTelemetry(SLE_Handle handle, int32_t slot);
+
+/* Per-kernel-class device times (HIP events on the engine's stream), recorded
+ * while SLE_Config.enable_profiling is set.  Classes, in order: power
+ * injections, fused measurement/Jacobian pass, gain SpGEMM, supernodal factor,
+ * right-hand side, forward solve, backward solve, state update.  ms[c] is the
+ * accumulated time, launches[c] the kernel launches, calls[c] the phase runs. */
+SLE_API SLE_StatusCode SLE_CALL sle_ext_GetKernelTimes(SLE_Handle handle, double* ms, int64_t* launches,
+                                                       int64_t* calls, int32_t count);
+SLE_API SLE_StatusCode SLE_CALL sle_ext_ResetKernelTimes(SLE_Handle handle);
 
 /* Breaker status of branch k (insertion index); applied before the next solve. */
 SLE_API SLE_StatusCode SLE_CALL sle_ext_SetBranchStatus(SLE_Handle handle, int32_t branch_index,

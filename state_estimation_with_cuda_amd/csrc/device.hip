@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -55,6 +56,7 @@ constexpr int kBlock = 256;
 constexpr int kFrontA = 32;    // fronts with c <= 32: LDS front of 32x32
 constexpr int kFrontB = 64;    // 32 < c <= 64: LDS front of 64x64
 constexpr int kVecMax = 2048;  // triangular-solve LDS vector bound for large fronts
+constexpr int kSpinLimit = 1 << 22;  // polls before a dependency wait gives up (~0.1 s)
 
 // measurement code byte: type (4 bits) | end << 4 | active << 5
 __host__ __device__ inline uint8_t meas_code(int type, int end, int active) {
@@ -68,7 +70,7 @@ struct DevStatus {
     int ridx;
     int nan;
     int piv;
-    int pad;
+    int flow_err;
 };
 
 // ---------------------------------------------------------------------------
@@ -149,6 +151,7 @@ __global__ __launch_bounds__(kBlock) void k_power(int n, const int32_t* __restri
     if (i == 0) {
         st->nan = 0;
         st->piv = 0;
+        st->flow_err = 0;
     }
     if (i >= n) return;
     double vi = V[i], ti = th[i];
@@ -572,10 +575,8 @@ struct SolveArgs {
     T* ubuf;
 };
 
-template <typename T, int VM>
-__global__ __launch_bounds__(kWave) void k_fwd(SolveArgs<T> A, const int32_t* __restrict__ list) {
-    __shared__ T f[VM];
-    const int s = list[blockIdx.x];
+template <typename T>
+__device__ inline void fwd_front(const SolveArgs<T>& A, int s, T* f) {
     const int tid = threadIdx.x;
     const int c = A.sn_c[s], k = A.sn_k[s], first = A.sn_first[s];
     const T* L = A.F + A.foff[s];
@@ -601,10 +602,8 @@ __global__ __launch_bounds__(kWave) void k_fwd(SolveArgs<T> A, const int32_t* __
     for (int a = tid; a < c - k; a += kWave) us[a] = f[k + a];
 }
 
-template <typename T, int VM>
-__global__ __launch_bounds__(kWave) void k_bwd(SolveArgs<T> A, const int32_t* __restrict__ list) {
-    __shared__ T xs[VM];
-    const int s = list[blockIdx.x];
+template <typename T>
+__device__ inline void bwd_front(const SolveArgs<T>& A, int s, T* xs) {
     const int tid = threadIdx.x;
     const int c = A.sn_c[s], k = A.sn_k[s], first = A.sn_first[s];
     const T* L = A.F + A.foff[s];
@@ -619,6 +618,91 @@ __global__ __launch_bounds__(kWave) void k_bwd(SolveArgs<T> A, const int32_t* __
         __syncthreads();
     }
     for (int i = tid; i < k; i += kWave) A.x[first + i] = xs[i];
+}
+
+template <typename T, int VM>
+__global__ __launch_bounds__(kWave) void k_fwd(SolveArgs<T> A, const int32_t* __restrict__ list) {
+    __shared__ T f[VM];
+    fwd_front<T>(A, list[blockIdx.x], f);
+}
+
+template <typename T, int VM>
+__global__ __launch_bounds__(kWave) void k_bwd(SolveArgs<T> A, const int32_t* __restrict__ list) {
+    __shared__ T xs[VM];
+    bwd_front<T>(A, list[blockIdx.x], xs);
+}
+
+// ---------------------------------------------------------------------------
+// Dependency-driven sweeps: one launch covers the whole supernodal tree.
+// Block b handles front order[b]; `order` lists every child before its parent
+// (forward, factor) or every parent before its children (backward).  A front
+// waits until the fronts it depends on have published `epoch` in done[] with
+// release semantics (agent scope), works, then publishes its own flag.  The
+// blocks a front waits for have lower block indices, so they were dispatched
+// earlier and are resident or finished.  The wait is bounded: past spin_limit
+// polls the block raises DevStatus::flow_err and proceeds, the host discards
+// the sweep and reruns it level by level, so no launch can hang.
+// ---------------------------------------------------------------------------
+// Polls with relaxed agent-scope loads (served by L2, no cache invalidation per
+// poll) and takes one acquire fence once the flag has flipped.
+__device__ inline void wait_flag(const uint32_t* flag, uint32_t epoch, int spin_limit, DevStatus* st) {
+    int it = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+        if (++it > spin_limit) {
+            st->flow_err = 1;
+            break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+__device__ inline void publish_flag(uint32_t* flag, uint32_t epoch) {
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kWave) void k_factor_flow(FactArgs<T> A, const int32_t* __restrict__ order,
+                                                       uint32_t* done, uint32_t epoch, int spin_limit) {
+    extern __shared__ unsigned char smem_raw[];
+    T* F = reinterpret_cast<T*>(smem_raw);
+    const int s = order[blockIdx.x];
+    if (threadIdx.x == 0)
+        for (int p = A.child_ptr[s]; p < A.child_ptr[s + 1]; ++p) wait_flag(done + A.child[p], epoch, spin_limit, A.st);
+    __syncthreads();
+    factor_front<T, kWave>(A, s, F);
+    const int c = A.sn_c[s];
+    T* Fo = A.F + A.foff[s];
+    for (int idx = threadIdx.x; idx < c * c; idx += kWave) Fo[idx] = F[idx];
+    publish_flag(done + s, epoch);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kWave) void k_fwd_flow(SolveArgs<T> A, const int32_t* __restrict__ order,
+                                                    uint32_t* done, uint32_t epoch, int spin_limit, DevStatus* st) {
+    extern __shared__ unsigned char smem_raw[];
+    T* f = reinterpret_cast<T*>(smem_raw);
+    const int s = order[blockIdx.x];
+    if (threadIdx.x == 0)
+        for (int p = A.child_ptr[s]; p < A.child_ptr[s + 1]; ++p) wait_flag(done + A.child[p], epoch, spin_limit, st);
+    __syncthreads();
+    fwd_front<T>(A, s, f);
+    publish_flag(done + s, epoch);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kWave) void k_bwd_flow(SolveArgs<T> A, const int32_t* __restrict__ order,
+                                                    const int32_t* __restrict__ sn_parent, uint32_t* done,
+                                                    uint32_t epoch, int spin_limit, DevStatus* st) {
+    extern __shared__ unsigned char smem_raw[];
+    T* xs = reinterpret_cast<T*>(smem_raw);
+    const int s = order[blockIdx.x];
+    if (threadIdx.x == 0 && sn_parent[s] >= 0) wait_flag(done + sn_parent[s], epoch, spin_limit, st);
+    __syncthreads();
+    bwd_front<T>(A, s, xs);
+    publish_flag(done + s, epoch);
 }
 
 // ---------------------------------------------------------------------------
@@ -646,7 +730,7 @@ __global__ __launch_bounds__(kBlock) void k_update(int n, const int32_t* __restr
                                                    double* __restrict__ V, double* __restrict__ th, double vmin,
                                                    double vmax, double* __restrict__ mpart, DevStatus* st) {
     const int b = blockIdx.x * kBlock + threadIdx.x;
-    const bool skip = st->piv != 0;
+    const bool skip = st->piv != 0 || st->flow_err != 0;
     double mx = 0.0;
     if (b < n && !skip) {
         double dv = (double)x[pv[b]];
@@ -808,7 +892,25 @@ struct Instance::Impl {
     // per-level launch lists: [level][class] -> (offset into lists, count)
     std::vector<int> lv_off[3], lv_cnt[3];
     int n_levels = 0;
+    int level_launches = 0;  // non-empty (level, class) launches of one factor / solve sweep
+    // dependency-driven sweeps (all fronts fit in LDS)
+    bool use_flow = false;
+    int max_c = 0;
+    uint32_t epoch = 0;
+    DBuf<uint32_t> done;
+    DBuf<int32_t> order_fwd, order_bwd, sn_parent;
     bool results_valid = false;
+    // profiling events per phase
+    hipEvent_t pev[PH_COUNT][2] = {};
+    bool pev_used[PH_COUNT] = {};
+    int pev_launches[PH_COUNT] = {};
+    // staged telemetry batches resident in HBM
+    struct Staged {
+        DBuf<int32_t> idx;
+        DBuf<double> val;
+        int count = 0;
+    };
+    std::vector<Staged> staged;
 };
 
 Instance::Instance(int device, int precision) : device_(device), precision_(precision) {
@@ -819,6 +921,8 @@ Instance::Instance(int device, int precision) : device_(device), precision_(prec
     HIPCHECK(hipEventCreate(&d_->ev1));
     HIPCHECK(hipHostMalloc((void**)&d_->hst, sizeof(DevStatus), hipHostMallocDefault));
     d_->st.alloc(1, bytes_);
+    for (int p = 0; p < PH_COUNT; ++p)
+        for (int q = 0; q < 2; ++q) HIPCHECK(hipEventCreate(&d_->pev[p][q]));
 }
 
 Instance::~Instance() {
@@ -837,9 +941,16 @@ Instance::~Instance() {
     rel(d_->gptr); rel(d_->gloc); rel(d_->gdiag); rel(d_->g_sn_ptr); rel(d_->gpair); rel(d_->sn_first);
     rel(d_->sn_k); rel(d_->sn_c); rel(d_->child_ptr); rel(d_->child); rel(d_->rmap); rel(d_->rows);
     rel(d_->perm); rel(d_->pth); rel(d_->pv); rel(d_->lists); rel(d_->foff); rel(d_->rmoff); rel(d_->uoff);
-    rel(d_->rows_off); rel(d_->st);
+    rel(d_->rows_off); rel(d_->st); rel(d_->done); rel(d_->order_fwd); rel(d_->order_bwd); rel(d_->sn_parent);
     d_->f32.release(b);
     d_->f64.release(b);
+    for (auto& sb : d_->staged) {
+        sb.idx.release(b);
+        sb.val.release(b);
+    }
+    for (int p = 0; p < PH_COUNT; ++p)
+        for (int q = 0; q < 2; ++q)
+            if (d_->pev[p][q]) (void)hipEventDestroy(d_->pev[p][q]);
     if (d_->hst) (void)hipHostFree(d_->hst);
     if (d_->ev0) (void)hipEventDestroy(d_->ev0);
     if (d_->ev1) (void)hipEventDestroy(d_->ev1);
@@ -966,6 +1077,22 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
             }
             D.lv_cnt[cls][l] = (int)lists.size() - D.lv_off[cls][l];
         }
+    D.level_launches = 0;
+    for (int l = 0; l < S.n_levels; ++l)
+        for (int cls = 0; cls < 3; ++cls) D.level_launches += D.lv_cnt[cls][l] ? 1 : 0;
+    // dependency-driven sweeps: level order lists children before parents
+    D.max_c = S.max_c;
+    D.use_flow = S.nsn > 0 && S.max_c <= kFrontB && std::getenv("SLE_LEVEL_SCHEDULE") == nullptr;
+    if (S.nsn > 0) {
+        D.order_fwd.upload(S.level_sn, bytes_, s);
+        std::vector<int32_t> rev(S.level_sn.rbegin(), S.level_sn.rend());
+        D.order_bwd.upload(rev, bytes_, s);
+        D.sn_parent.upload(S.sn_parent, bytes_, s);
+        D.done.alloc(S.nsn, bytes_);
+        HIPCHECK(hipMemsetAsync(D.done.p, 0, S.nsn * sizeof(uint32_t), s));
+        D.epoch = 0;
+        HIPCHECK(hipStreamSynchronize(s));
+    }
     if (lists.empty()) lists.push_back(0);
     D.lists.upload(lists, bytes_, s);
     HIPCHECK(hipStreamSynchronize(s));
@@ -1136,6 +1263,12 @@ void launch_factor(Instance::Impl& D, T piv_tol) {
     A.F = B.F.p;
     A.piv_tol = piv_tol;
     A.st = D.st.p;
+    if (D.use_flow) {
+        const uint32_t ep = ++D.epoch;
+        k_factor_flow<T><<<D.sn_first.n, kWave, D.max_c * D.max_c * sizeof(T), D.stream>>>(A, D.order_fwd.p, D.done.p,
+                                                                                         ep, kSpinLimit);
+        return;
+    }
     for (int l = 0; l < D.n_levels; ++l) {
         if (D.lv_cnt[0][l]) k_factor_lds<T, kFrontA><<<D.lv_cnt[0][l], kWave, 0, D.stream>>>(A, D.lists.p + D.lv_off[0][l]);
         if (D.lv_cnt[1][l]) k_factor_lds<T, kFrontB><<<D.lv_cnt[1][l], kWave, 0, D.stream>>>(A, D.lists.p + D.lv_off[1][l]);
@@ -1164,19 +1297,40 @@ SolveArgs<T> solve_args(Instance::Impl& D) {
     return A;
 }
 
+struct Prof;
+void prof_begin(Prof* pf, int ph);
+void prof_end(Prof* pf, int ph, int launches);
+
 template <typename T>
-void launch_trisolve(Instance::Impl& D) {
+void launch_trisolve(Instance::Impl& D, Prof* pf = nullptr) {
     SolveArgs<T> A = solve_args<T>(D);
+    if (D.use_flow) {
+        const int nsn = (int)D.sn_first.n;
+        const size_t sm = D.max_c * sizeof(T);
+        prof_begin(pf, PH_FWD);
+        const uint32_t e1 = ++D.epoch;
+        k_fwd_flow<T><<<nsn, kWave, sm, D.stream>>>(A, D.order_fwd.p, D.done.p, e1, kSpinLimit, D.st.p);
+        prof_end(pf, PH_FWD, 1);
+        prof_begin(pf, PH_BWD);
+        const uint32_t e2 = ++D.epoch;
+        k_bwd_flow<T><<<nsn, kWave, sm, D.stream>>>(A, D.order_bwd.p, D.sn_parent.p, D.done.p, e2, kSpinLimit, D.st.p);
+        prof_end(pf, PH_BWD, 1);
+        return;
+    }
+    prof_begin(pf, PH_FWD);
     for (int l = 0; l < D.n_levels; ++l) {
         if (D.lv_cnt[0][l]) k_fwd<T, kFrontA><<<D.lv_cnt[0][l], kWave, 0, D.stream>>>(A, D.lists.p + D.lv_off[0][l]);
         if (D.lv_cnt[1][l]) k_fwd<T, kFrontB><<<D.lv_cnt[1][l], kWave, 0, D.stream>>>(A, D.lists.p + D.lv_off[1][l]);
         if (D.lv_cnt[2][l]) k_fwd<T, kVecMax><<<D.lv_cnt[2][l], kWave, 0, D.stream>>>(A, D.lists.p + D.lv_off[2][l]);
     }
+    prof_end(pf, PH_FWD, D.level_launches);
+    prof_begin(pf, PH_BWD);
     for (int l = D.n_levels - 1; l >= 0; --l) {
         if (D.lv_cnt[0][l]) k_bwd<T, kFrontA><<<D.lv_cnt[0][l], kWave, 0, D.stream>>>(A, D.lists.p + D.lv_off[0][l]);
         if (D.lv_cnt[1][l]) k_bwd<T, kFrontB><<<D.lv_cnt[1][l], kWave, 0, D.stream>>>(A, D.lists.p + D.lv_off[1][l]);
         if (D.lv_cnt[2][l]) k_bwd<T, kVecMax><<<D.lv_cnt[2][l], kWave, 0, D.stream>>>(A, D.lists.p + D.lv_off[2][l]);
     }
+    prof_end(pf, PH_BWD, D.level_launches);
 }
 
 template <typename T>
@@ -1199,17 +1353,63 @@ void read_status(Instance::Impl& D) {
     HIPCHECK(hipStreamSynchronize(D.stream));
 }
 
-template <typename T>
-void iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p, bool refactor, T piv_tol) {
-    launch_power(D);
-    launch_meas<T>(D, p.robust, p.gamma, 0, 1, false);
-    if (refactor) {
-        launch_spgemm<T>(D, S);
-        launch_factor<T>(D, piv_tol);
+struct Prof {
+    Instance::Impl& D;
+    bool on;
+    void begin(int ph) {
+        if (on) HIPCHECK(hipEventRecord(D.pev[ph][0], D.stream));
     }
+    void end(int ph, int launches) {
+        if (!on) return;
+        HIPCHECK(hipEventRecord(D.pev[ph][1], D.stream));
+        D.pev_used[ph] = true;
+        D.pev_launches[ph] = launches;
+    }
+    // call after the stream has been synchronized
+    void collect(KernelTimes& kt) {
+        if (!on) return;
+        for (int p = 0; p < PH_COUNT; ++p) {
+            if (!D.pev_used[p]) continue;
+            float ms = 0;
+            HIPCHECK(hipEventElapsedTime(&ms, D.pev[p][0], D.pev[p][1]));
+            kt.ms[p] += ms;
+            kt.launches[p] += D.pev_launches[p];
+            kt.calls[p] += 1;
+            D.pev_used[p] = false;
+        }
+    }
+};
+
+void prof_begin(Prof* pf, int ph) {
+    if (pf) pf->begin(ph);
+}
+void prof_end(Prof* pf, int ph, int launches) {
+    if (pf) pf->end(ph, launches);
+}
+
+template <typename T>
+void iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p, bool refactor, T piv_tol, Prof& pf) {
+    pf.begin(PH_POWER);
+    launch_power(D);
+    pf.end(PH_POWER, 1);
+    pf.begin(PH_MEAS);
+    launch_meas<T>(D, p.robust, p.gamma, 0, 1, false);
+    pf.end(PH_MEAS, 1);
+    if (refactor) {
+        pf.begin(PH_SPGEMM);
+        launch_spgemm<T>(D, S);
+        pf.end(PH_SPGEMM, 1);
+        pf.begin(PH_FACTOR);
+        launch_factor<T>(D, piv_tol);
+        pf.end(PH_FACTOR, D.level_launches);
+    }
+    pf.begin(PH_RHS);
     launch_rhs<T>(D);
-    launch_trisolve<T>(D);
+    pf.end(PH_RHS, 1);
+    launch_trisolve<T>(D, &pf);
+    pf.begin(PH_UPDATE);
     launch_update<T>(D, p.vmin, p.vmax);
+    pf.end(PH_UPDATE, 2);
 }
 
 }  // namespace
@@ -1219,6 +1419,7 @@ SolveOutcome Instance::solve(const SolveParams& p) {
     Impl& D = *d_;
     const Symbolic& S = *S_;
     SolveOutcome out;
+    Prof pf{D, profiling_};
     auto t0 = std::chrono::steady_clock::now();
     HIPCHECK(hipEventRecord(D.ev0, D.stream));
     if (p.flat_start) k_flat<<<nblk(D.n), kBlock, 0, D.stream>>>(D.n, D.bus_type.p, D.vset.p, D.V.p, D.th.p);
@@ -1237,12 +1438,20 @@ SolveOutcome Instance::solve(const SolveParams& p) {
             const bool refactor = p.robust || !factor_valid_ || p.mode == 1;
             if (fp64) {
                 if (!D.f64.ready) D.f64.alloc(S, bytes_);
-                iteration<double>(D, S, p, refactor, 1e-13);
+                iteration<double>(D, S, p, refactor, 1e-13, pf);
             } else {
-                iteration<float>(D, S, p, refactor, 1e-5f);
+                iteration<float>(D, S, p, refactor, 1e-5f, pf);
             }
             HIPCHECK(hipGetLastError());
             read_status(D);
+            pf.collect(ktimes_);
+            if (D.hst->flow_err) {
+                // a dependency wait timed out: redo this iteration level by level
+                D.use_flow = false;
+                factor_valid_ = false;
+                --it;
+                continue;
+            }
             if (D.hst->piv) {
                 factor_valid_ = false;
                 if (!fp64) {
@@ -1311,6 +1520,7 @@ SolveOutcome Instance::solve(const SolveParams& p) {
     float ms = 0;
     HIPCHECK(hipEventElapsedTime(&ms, D.ev0, D.ev1));
     out.ms = ms;
+    out.flags = (D.use_flow ? 1 : 0) | (fp64_factor_active_ ? 2 : 0);
     D.results_valid = true;
     return out;
 }
@@ -1391,21 +1601,28 @@ int Instance::debug_factor(double* fronts) {
     HIPCHECK(hipSetDevice(device_));
     Impl& D = *d_;
     const Symbolic& S = *S_;
-    launch_power(D);
-    if (precision_ == 1) {
-        launch_meas<double>(D, 0, 1.5, 0, 1, false);
-        launch_spgemm<double>(D, S);
-        launch_factor<double>(D, 1e-13);
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        launch_power(D);
+        if (precision_ == 1) {
+            launch_meas<double>(D, 0, 1.5, 0, 1, false);
+            launch_spgemm<double>(D, S);
+            launch_factor<double>(D, 1e-13);
+        } else {
+            launch_meas<float>(D, 0, 1.5, 0, 1, false);
+            launch_spgemm<float>(D, S);
+            launch_factor<float>(D, 1e-5f);
+        }
         HIPCHECK(hipGetLastError());
-        if (fronts) fetch_as_double(D.f64.F, S.ftotal, fronts, D.stream);
-    } else {
-        launch_meas<float>(D, 0, 1.5, 0, 1, false);
-        launch_spgemm<float>(D, S);
-        launch_factor<float>(D, 1e-5f);
-        HIPCHECK(hipGetLastError());
-        if (fronts) fetch_as_double(D.f32.F, S.ftotal, fronts, D.stream);
+        read_status(D);
+        if (!D.hst->flow_err) break;
+        D.use_flow = false;
     }
-    read_status(D);
+    if (fronts) {
+        if (precision_ == 1)
+            fetch_as_double(D.f64.F, S.ftotal, fronts, D.stream);
+        else
+            fetch_as_double(D.f32.F, S.ftotal, fronts, D.stream);
+    }
     factor_valid_ = D.hst->piv == 0;
     return D.hst->piv;
 }
@@ -1418,23 +1635,61 @@ void Instance::debug_solve(const double* b, double* dx) {
     bb.alloc(D.ns, tmp);
     xx.alloc(D.ns, tmp);
     bb.upload(b, D.ns, D.stream);
-    if (precision_ == 1) {
-        k_permute_in<double><<<nblk(D.ns), kBlock, 0, D.stream>>>(D.ns, D.perm.p, bb.p, D.f64.x.p);
-        launch_trisolve<double>(D);
-        k_permute_out<double><<<nblk(D.ns), kBlock, 0, D.stream>>>(D.ns, D.perm.p, D.f64.x.p, xx.p);
-    } else {
-        k_permute_in<float><<<nblk(D.ns), kBlock, 0, D.stream>>>(D.ns, D.perm.p, bb.p, D.f32.x.p);
-        launch_trisolve<float>(D);
-        k_permute_out<float><<<nblk(D.ns), kBlock, 0, D.stream>>>(D.ns, D.perm.p, D.f32.x.p, xx.p);
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        D.hst->flow_err = 0;
+        HIPCHECK(hipMemcpyAsync(D.st.p, D.hst, sizeof(DevStatus), hipMemcpyHostToDevice, D.stream));
+        if (precision_ == 1) {
+            k_permute_in<double><<<nblk(D.ns), kBlock, 0, D.stream>>>(D.ns, D.perm.p, bb.p, D.f64.x.p);
+            launch_trisolve<double>(D);
+            k_permute_out<double><<<nblk(D.ns), kBlock, 0, D.stream>>>(D.ns, D.perm.p, D.f64.x.p, xx.p);
+        } else {
+            k_permute_in<float><<<nblk(D.ns), kBlock, 0, D.stream>>>(D.ns, D.perm.p, bb.p, D.f32.x.p);
+            launch_trisolve<float>(D);
+            k_permute_out<float><<<nblk(D.ns), kBlock, 0, D.stream>>>(D.ns, D.perm.p, D.f32.x.p, xx.p);
+        }
+        HIPCHECK(hipGetLastError());
+        read_status(D);
+        if (!D.hst->flow_err) break;
+        D.use_flow = false;
     }
-    HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpyAsync(dx, xx.p, D.ns * sizeof(double), hipMemcpyDeviceToHost, D.stream));
     HIPCHECK(hipStreamSynchronize(D.stream));
     bb.release(tmp);
     xx.release(tmp);
 }
 
-double Instance::last_kernel_ms(const char*) const { return 0.0; }
+void Instance::stage_telemetry(int slot, const int32_t* idx, const double* val, int count) {
+    if (slot < 0 || count < 0) throw std::invalid_argument("bad staging slot or count");
+    for (int q = 0; q < count; ++q)
+        if (idx[q] < 0 || idx[q] >= d_->m) throw std::invalid_argument("staged measurement index out of range");
+    HIPCHECK(hipSetDevice(device_));
+    Impl& D = *d_;
+    if ((int)D.staged.size() <= slot) D.staged.resize(slot + 1);
+    Impl::Staged& sb = D.staged[slot];
+    if (sb.idx.n < (size_t)count) {
+        sb.idx.alloc(count, bytes_);
+        sb.val.alloc(count, bytes_);
+    }
+    sb.idx.upload(idx, count, D.stream);
+    sb.val.upload(val, count, D.stream);
+    sb.count = count;
+    HIPCHECK(hipStreamSynchronize(D.stream));
+}
+
+void Instance::apply_staged(int slot) {
+    Impl& D = *d_;
+    if (slot < 0 || slot >= (int)D.staged.size()) throw std::invalid_argument("unknown staging slot");
+    HIPCHECK(hipSetDevice(device_));
+    const Impl::Staged& sb = D.staged[slot];
+    if (sb.count)
+        k_scatter_z<<<nblk(sb.count), kBlock, 0, D.stream>>>(sb.count, sb.idx.p, sb.val.p, D.z.p);
+    HIPCHECK(hipGetLastError());
+}
+
+int Instance::staged_count(int slot) const {
+    if (slot < 0 || slot >= (int)d_->staged.size()) return -1;
+    return d_->staged[slot].count;
+}
 
 int device_count() {
     int c = 0;

@@ -136,7 +136,10 @@ int Engine::initialize() {
 void Engine::set_config(const EngineConfig& c) {
     bool prec_change = c.use_double_precision != cfg_.use_double_precision;
     cfg_ = c;
-    if (inst_) inst_->invalidate_factor();
+    if (inst_) {
+        inst_->set_profiling(cfg_.enable_profiling);
+        inst_->set_use_graphs(cfg_.use_cuda_graphs);
+    }
     if (prec_change) model_dirty_ = true;
 }
 
@@ -408,6 +411,7 @@ int Engine::meter_reading(const char* meter, const char* channel, double* value,
     int i = channel_meas(mt.mv, mt.mp, mt.mq, mt.mi, channel);
     if (i < 0) return ST_NOT_FOUND;
     if (which == 0) {
+        const_cast<Engine*>(this)->flush_staged_host();
         *value = m_z_[i];
     } else {
         const_cast<Engine*>(this)->ensure_results();
@@ -426,6 +430,7 @@ int Engine::update_meter_reading(const char* meter, const char* channel, double 
     const Meter& mt = meters_[it->second];
     int i = channel_meas(mt.mv, mt.mp, mt.mq, mt.mi, channel);
     if (i < 0) return ST_NOT_FOUND;
+    flush_staged_host();
     m_z_[i] = value;
     z_dirty_ = true;
     return ST_OK;
@@ -501,6 +506,8 @@ int Engine::ensure_symbolic() {
 
 int Engine::rebuild() {
     if (!initialized_) return ST_NOT_INITIALIZED;
+    ensure_state();
+    flush_staged_host();
     int rc = ensure_symbolic();
     if (rc != ST_OK) return rc;
     HostArrays h;
@@ -551,6 +558,7 @@ int Engine::upload_model() {
 // sle_engine.cu:508-524: values in insertion order, count == measurement count
 int Engine::update_telemetry(const double* values, int count) {
     if (!values || count != n_meas()) return ST_INVALID_ARGUMENT;
+    staged_pending_.clear();
     std::copy(values, values + count, m_z_.begin());
     if (inst_ && !model_dirty_) {
         inst_->set_z(m_z_.data());
@@ -565,6 +573,7 @@ int Engine::update_telemetry_indexed(const int32_t* idx, const double* values, i
     if (!idx || !values || count < 0) return ST_INVALID_ARGUMENT;
     for (int q = 0; q < count; ++q)
         if (idx[q] < 0 || idx[q] >= n_meas()) return ST_INVALID_ARGUMENT;
+    flush_staged_host();
     for (int q = 0; q < count; ++q) m_z_[idx[q]] = values[q];
     if (inst_ && !model_dirty_ && !z_dirty_) {
         inst_->set_z_indexed(idx, values, count);
@@ -577,9 +586,51 @@ int Engine::update_telemetry_indexed(const int32_t* idx, const double* values, i
 int Engine::update_measurement(const char* id, double value) {
     int i = meas_index(id);
     if (i < 0) return ST_NOT_FOUND;
+    flush_staged_host();
     m_z_[i] = value;
     z_dirty_ = true;
     return ST_OK;
+}
+
+int Engine::stage_telemetry(int slot, const int32_t* idx, const double* values, int count) {
+    if (slot < 0 || slot > 1 << 20 || count < 0 || (count && (!idx || !values))) return ST_INVALID_ARGUMENT;
+    for (int q = 0; q < count; ++q)
+        if (idx[q] < 0 || idx[q] >= n_meas()) return ST_INVALID_ARGUMENT;
+    int rc = ensure_ready();
+    if (rc != ST_OK) return rc;
+    if ((int)staged_host_.size() <= slot) staged_host_.resize(slot + 1);
+    staged_host_[slot].first.assign(idx, idx + count);
+    staged_host_[slot].second.assign(values, values + count);
+    inst_->stage_telemetry(slot, idx, values, count);
+    return ST_OK;
+}
+
+int Engine::apply_staged(int slot) {
+    if (slot < 0 || slot >= (int)staged_host_.size()) return ST_INVALID_ARGUMENT;
+    if (!inst_ || model_dirty_ || z_dirty_) {
+        // no current device copy of z: mirror on the host, upload at the next solve
+        flush_staged_host();
+        for (size_t q = 0; q < staged_host_[slot].first.size(); ++q)
+            m_z_[staged_host_[slot].first[q]] = staged_host_[slot].second[q];
+        z_dirty_ = true;
+        return ST_OK;
+    }
+    inst_->apply_staged(slot);
+    staged_pending_.push_back(slot);
+    return ST_OK;
+}
+
+void Engine::flush_staged_host() {
+    for (int slot : staged_pending_)
+        for (size_t q = 0; q < staged_host_[slot].first.size(); ++q)
+            m_z_[staged_host_[slot].first[q]] = staged_host_[slot].second[q];
+    staged_pending_.clear();
+}
+
+void Engine::ensure_state() {
+    if (!state_dirty_) return;
+    if (inst_) inst_->get_state(V_.data(), th_.data());
+    state_dirty_ = false;
 }
 
 int Engine::sync_measurements() {
@@ -675,7 +726,7 @@ int Engine::solve(const SolverConfigIn& sc, ResultOut& out) {
     p.vmax = cfg_.voltage_max_pu;
     p.time_limit_ms = sc.mode == 0 ? sc.time_limit_ms : 0.0;
     SolveOutcome o = inst_->solve(p);
-    inst_->get_state(V_.data(), th_.data());
+    state_dirty_ = true;
     results_dirty_ = true;
     out.status = o.status;
     out.iterations = o.iterations;
@@ -686,6 +737,7 @@ int Engine::solve(const SolverConfigIn& sc, ResultOut& out) {
     out.largest_residual_idx = o.largest_residual_idx;
     out.refactors = o.refactors;
     out.fp64_refactors = o.fp64_refactors;
+    out.flags = o.flags;
     out.bad_data_count = 0;
     last_ = out;
     return ST_OK;
@@ -723,6 +775,7 @@ int Engine::check_observability(int& observable, int& islands) {
 }
 
 int Engine::apply_flat_start() {
+    state_dirty_ = false;
     for (int b = 0; b < n_bus(); ++b) {
         V_[b] = bus_type_[b] != 0 ? bus_vset_[b] : 1.0;
         th_[b] = 0.0;
@@ -733,6 +786,7 @@ int Engine::apply_flat_start() {
 
 int Engine::set_state(const double* V, const double* th, int count) {
     if (!V || !th || count != n_bus()) return ST_INVALID_ARGUMENT;
+    state_dirty_ = false;
     std::copy(V, V + count, V_.begin());
     std::copy(th, th + count, th_.begin());
     if (inst_ && !model_dirty_) inst_->set_state(V_.data(), th_.data());
@@ -760,6 +814,7 @@ void Engine::ensure_results() {
 std::vector<int32_t> Engine::identify_bad_data(double threshold) {
     ensure_results();
     double th = threshold > 0 ? threshold : cfg_.bad_data_threshold;
+    flush_staged_host();
     std::vector<int32_t> out;
     for (int i = 0; i < n_meas(); ++i)
         if (meas_active(i) && std::fabs(r_[i]) / m_sigma_[i] > th) out.push_back(i);

@@ -76,7 +76,7 @@ struct ResultOut {
     int iterations = 0;
     double max_mismatch = 0, objective = 0, ms = 0, largest_residual = 0;
     int largest_residual_idx = -1, bad_data_count = 0;
-    int refactors = 0, fp64_refactors = 0;
+    int refactors = 0, fp64_refactors = 0, flags = 0;
 };
 
 class Engine {
@@ -119,6 +119,10 @@ class Engine {
     int update_telemetry_indexed(const int32_t* idx, const double* values, int count);
     int update_measurement(const char* id, double value);
     int sync_measurements();
+    // device-resident telemetry batches (see Instance::stage_telemetry)
+    int stage_telemetry(int slot, const int32_t* idx, const double* values, int count);
+    int apply_staged(int slot);
+    void flush_staged_host();
 
     // switches
     int set_switch_status(const char* sd_id, int status);
@@ -133,8 +137,15 @@ class Engine {
     int apply_flat_start();
 
     // results
-    const std::vector<double>& V() const { return V_; }
-    const std::vector<double>& th() const { return th_; }
+    const std::vector<double>& V() {
+        ensure_state();
+        return V_;
+    }
+    const std::vector<double>& th() {
+        ensure_state();
+        return th_;
+    }
+    void ensure_state();  // download the estimate after a solve (on first use)
     int set_state(const double* V, const double* th, int count);
     void ensure_results();
     std::vector<double> P_, Q_, Pf_, Qf_, Pt_, Qt_, h_, r_;
@@ -163,7 +174,10 @@ class Engine {
     double meas_pt(int i) const { return m_pt_[i]; }
     double meas_ct(int i) const { return m_ct_[i]; }
     int meas_pseudo(int i) const { return m_pseudo_[i]; }
-    double meas_z(int i) const { return m_z_[i]; }
+    double meas_z(int i) {
+        flush_staged_host();
+        return m_z_[i];
+    }
     bool meas_active(int i) const;
     const std::string& bus_id(int i) const { return bus_id_[i]; }
     const std::string& branch_id(int i) const { return br_id_[i]; }
@@ -216,6 +230,9 @@ class Engine {
     bool params_dirty_ = false;   // branch parameters changed: Ybus numeric
     bool z_dirty_ = true;         // telemetry changed on host
     bool results_dirty_ = true;   // host copies of P/Q/flows/h/r stale
+    bool state_dirty_ = false;    // host copy of V/theta older than the device state
+    std::vector<std::pair<std::vector<int32_t>, std::vector<double>>> staged_host_;
+    std::vector<int> staged_pending_;  // slots applied on the device, not yet mirrored into m_z_
     std::vector<double> V_, th_;
     std::shared_ptr<Symbolic> sym_;
     std::unique_ptr<Instance> inst_;

@@ -53,11 +53,16 @@ struct SolveOutcome {
     double largest_residual = 0;  // max |r|/sigma over active rows
     int largest_residual_idx = -1;
     double ms = 0;
+    int flags = 0;  // bit 0: dependency-driven sweeps; bit 1: FP64 factor active (mixed form fell back)
 };
 
-struct KernelTimes {  // accumulated milliseconds per kernel class (profiling mode)
-    double power = 0, meas = 0, spgemm = 0, factor = 0, rhs = 0, fwd = 0, bwd = 0, update = 0;
-    int iters = 0;
+// Kernel classes timed with HIP events on the instance stream (profiling mode).
+enum Phase : int { PH_POWER = 0, PH_MEAS, PH_SPGEMM, PH_FACTOR, PH_RHS, PH_FWD, PH_BWD, PH_UPDATE, PH_COUNT };
+
+struct KernelTimes {
+    double ms[PH_COUNT] = {0};       // accumulated device time between the phase's first launch and last completion
+    int64_t launches[PH_COUNT] = {0};  // kernel launches issued in the phase
+    int64_t calls[PH_COUNT] = {0};     // times the phase ran
 };
 
 // Raised for any HIP runtime failure (mapped to SLE_ERROR_CUDA_FAILURE).
@@ -106,9 +111,16 @@ class Instance {
     int precision() const { return precision_; }
     const Symbolic& symbolic() const { return *S_; }
     KernelTimes& kernel_times() { return ktimes_; }
+    void reset_kernel_times() { ktimes_ = KernelTimes(); }
     void set_profiling(bool on) { profiling_ = on; }
+    bool profiling() const { return profiling_; }
     void set_use_graphs(bool on) { use_graphs_ = on; }
-    double last_kernel_ms(const char* which) const;
+
+    // Device-resident telemetry batches: stage (host -> HBM) ahead of time,
+    // apply (scatter into z on the device) right before a solve.
+    void stage_telemetry(int slot, const int32_t* idx, const double* val, int count);
+    void apply_staged(int slot);
+    int staged_count(int slot) const;
 
    private:
     Impl* d_ = nullptr;

@@ -851,6 +851,7 @@ SLE_API SLE_StatusCode SLE_CALL sle_ext_Solve(SLE_Handle handle, const SLE_Solve
             result->objective = r.objective;
             result->largest_residual = r.largest_residual;
             result->largest_residual_idx = r.largest_residual_idx;
+            result->flags = r.flags;
             result->gpu_ms = r.ms;
             result->host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         }
@@ -916,6 +917,37 @@ SLE_API SLE_StatusCode SLE_CALL sle_ext_SetBranchStatus(SLE_Handle handle, int32
     return guarded([&]() -> SLE_StatusCode {
         return from_int(E(handle).set_branch_status_index(branch_index, (int)status), "set branch status");
     });
+}
+
+SLE_API SLE_StatusCode SLE_CALL sle_ext_StageTelemetry(SLE_Handle handle, int32_t slot, const int32_t* idx,
+                                                       const double* values, int32_t count) {
+    if (!handle) return fail(SLE_ERROR_INVALID_HANDLE, "NULL handle");
+    return guarded([&]() -> SLE_StatusCode {
+        return from_int(E(handle).stage_telemetry(slot, idx, values, count), "stage telemetry");
+    });
+}
+
+SLE_API SLE_StatusCode SLE_CALL sle_ext_ApplyStagedTelemetry(SLE_Handle handle, int32_t slot) {
+    if (!handle) return fail(SLE_ERROR_INVALID_HANDLE, "NULL handle");
+    return guarded([&]() -> SLE_StatusCode { return from_int(E(handle).apply_staged(slot), "apply staged"); });
+}
+
+SLE_API SLE_StatusCode SLE_CALL sle_ext_GetKernelTimes(SLE_Handle handle, double* ms, int64_t* launches,
+                                                       int64_t* calls, int32_t count) {
+    if (!handle) return fail(SLE_ERROR_INVALID_HANDLE, "NULL handle");
+    sle::Instance* inst = E(handle).instance();
+    for (int32_t c = 0; c < count && c < sle::PH_COUNT; ++c) {
+        if (ms) ms[c] = inst ? inst->kernel_times().ms[c] : 0.0;
+        if (launches) launches[c] = inst ? inst->kernel_times().launches[c] : 0;
+        if (calls) calls[c] = inst ? inst->kernel_times().calls[c] : 0;
+    }
+    return SLE_OK;
+}
+
+SLE_API SLE_StatusCode SLE_CALL sle_ext_ResetKernelTimes(SLE_Handle handle) {
+    if (!handle) return fail(SLE_ERROR_INVALID_HANDLE, "NULL handle");
+    if (E(handle).instance()) E(handle).instance()->reset_kernel_times();
+    return SLE_OK;
 }
 
 static SLE_StatusCode need_symbolic(SLE_Handle handle, std::shared_ptr<const sle::Symbolic>& S,

@@ -112,7 +112,7 @@ def _structs(real):
 class ExtResult(C.Structure):
     _fields_ = [("status", C.c_int32), ("iterations", C.c_int32), ("refactors", C.c_int32),
                 ("fp64_refactors", C.c_int32), ("max_mismatch", C.c_double), ("objective", C.c_double),
-                ("largest_residual", C.c_double), ("largest_residual_idx", C.c_int32), ("reserved", C.c_int32),
+                ("largest_residual", C.c_double), ("largest_residual_idx", C.c_int32), ("flags", C.c_int32),
                 ("gpu_ms", C.c_double), ("host_ms", C.c_double)]
 
 
@@ -140,8 +140,10 @@ EXT_FUNCTIONS = [
     "sle_ext_Solve", "sle_ext_UpdateTelemetryIndexed", "sle_ext_UpdateTelemetryF64", "sle_ext_SetState",
     "sle_ext_GetState", "sle_ext_GetResults", "sle_ext_SetBranchStatus", "sle_ext_GetSizes", "sle_ext_GetYbus",
     "sle_ext_GetHPattern", "sle_ext_GetGPattern", "sle_ext_GetOrdering", "sle_ext_Evaluate",
-    "sle_ext_JacobianGain", "sle_ext_Factor", "sle_ext_SolveFactor",
+    "sle_ext_JacobianGain", "sle_ext_Factor", "sle_ext_SolveFactor", "sle_ext_StageTelemetry",
+    "sle_ext_ApplyStagedTelemetry", "sle_ext_GetKernelTimes", "sle_ext_ResetKernelTimes",
 ]
+KERNEL_CLASSES = ["power", "meas", "spgemm", "factor", "rhs", "fwd", "bwd", "update"]
 
 
 class Library:
@@ -259,6 +261,10 @@ class Library:
             "sle_ext_JacobianGain": (C.c_int, [H, dp, dp]),
             "sle_ext_Factor": (C.c_int, [H, lp, ip, dp]),
             "sle_ext_SolveFactor": (C.c_int, [H, dp, dp]),
+            "sle_ext_StageTelemetry": (C.c_int, [H, i32, ip, dp, i32]),
+            "sle_ext_ApplyStagedTelemetry": (C.c_int, [H, i32]),
+            "sle_ext_GetKernelTimes": (C.c_int, [H, dp, lp, lp, i32]),
+            "sle_ext_ResetKernelTimes": (C.c_int, [H]),
         }
         missing = [n for n in API_FUNCTIONS + EXT_FUNCTIONS if n not in sig]
         assert not missing, missing
@@ -474,7 +480,7 @@ class Engine:
             self._check(rc, "sle_ext_Solve")
         return dict(rc=rc, status=r.status, iterations=r.iterations, refactors=r.refactors,
                     fp64_refactors=r.fp64_refactors, max_mismatch=r.max_mismatch, objective=r.objective,
-                    largest_residual=r.largest_residual, largest_residual_idx=r.largest_residual_idx,
+                    largest_residual=r.largest_residual, largest_residual_idx=r.largest_residual_idx, flags=r.flags,
                     gpu_ms=r.gpu_ms, host_ms=r.host_ms)
 
     def solve_mode(self, mode):
@@ -499,6 +505,26 @@ class Engine:
         ptrs = [out[k].ctypes.data for k in ("P", "Q", "Pf", "Qf", "Pt", "Qt", "h", "r")]
         self._check(self.L.sle_ext_GetResults(self.h, *ptrs), "sle_ext_GetResults")
         return out
+
+    def stage_telemetry(self, slot, idx, val):
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        val = np.ascontiguousarray(val, dtype=np.float64)
+        self._check(self.L.sle_ext_StageTelemetry(self.h, int(slot), idx, val, int(idx.shape[0])),
+                    "sle_ext_StageTelemetry")
+
+    def apply_staged(self, slot):
+        self._check(self.L.sle_ext_ApplyStagedTelemetry(self.h, int(slot)), "sle_ext_ApplyStagedTelemetry")
+
+    def kernel_times(self):
+        n = len(KERNEL_CLASSES)
+        ms = np.zeros(n)
+        la = np.zeros(n, np.int64)
+        ca = np.zeros(n, np.int64)
+        self._check(self.L.sle_ext_GetKernelTimes(self.h, ms, la, ca, n), "sle_ext_GetKernelTimes")
+        return {k: dict(ms=float(ms[i]), launches=int(la[i]), calls=int(ca[i])) for i, k in enumerate(KERNEL_CLASSES)}
+
+    def reset_kernel_times(self):
+        self._check(self.L.sle_ext_ResetKernelTimes(self.h), "sle_ext_ResetKernelTimes")
 
     def set_branch_status(self, k, status):
         self._check(self.L.sle_ext_SetBranchStatus(self.h, int(k), int(status)), "sle_ext_SetBranchStatus")
