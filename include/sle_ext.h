@@ -30,7 +30,17 @@ typedef struct SLE_ExtResult {
     int32_t flags;       /* bit 0: dependency-driven factor/solve sweeps; bit 1: FP64 factor in use */
     double gpu_ms;       /* device time of the solve */
     double host_ms;      /* wall time of the call */
+    int32_t pivots_substituted; /* mixed form: iterations whose FP32 factor had pivots substituted */
+    int32_t pcg_steps;   /* conjugate-gradient steps per Newton step (0: direct solve) */
 } SLE_ExtResult;
+
+/* Engine options outside SLE_Config:
+ *   "pcg_steps"       mixed form: conjugate-gradient steps refining each Newton
+ *                     step with the FP32 factor as preconditioner (0..16, default 3)
+ *   "level_schedule"  1 (default): factor and solve level by level (kernel
+ *                     boundaries order the fronts); 0: one dependency-driven
+ *                     launch per sweep with release/acquire flags per front */
+SLE_API SLE_StatusCode SLE_CALL sle_ext_SetOption(SLE_Handle handle, const char* key, double value);
 
 /* Solve with an explicit solver configuration; FP64 outputs. */
 SLE_API SLE_StatusCode SLE_CALL sle_ext_Solve(SLE_Handle handle, const SLE_SolverConfig* config,

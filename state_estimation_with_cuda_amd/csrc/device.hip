@@ -28,7 +28,6 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
-#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -238,8 +237,7 @@ struct MeasArgs {
     int* ipart;
 };
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_meas(MeasArgs A, T* __restrict__ hs) {
+__global__ __launch_bounds__(kBlock) void k_meas(MeasArgs A, double* __restrict__ hs) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     double jw = 0.0, rn = -1.0;
     int ri = -1;
@@ -330,7 +328,7 @@ __global__ __launch_bounds__(kBlock) void k_meas(MeasArgs A, T* __restrict__ hs)
         }
         if (A.want_jac && q1 > q0) {
             if (t == 0 || t == 1) {
-                hs[q0] = (T)sw;
+                hs[q0] = sw;
             } else if (t == 2 || t == 3 || t == 7 || t == 8) {
                 const bool isP = (t == 2 || t == 7);
                 const int e0 = A.yrow[l], e1 = A.yrow[l + 1];
@@ -359,24 +357,24 @@ __global__ __launch_bounds__(kBlock) void k_meas(MeasArgs A, T* __restrict__ hs)
                             dv = qe / Vj;
                         }
                     }
-                    if (j != A.slack) hs[q0 + kth++] = (T)(sw * dth);
-                    hs[q0 + nth + (e - e0)] = (T)(sw * dv);
+                    if (j != A.slack) hs[q0 + kth++] = sw * dth;
+                    hs[q0 + nth + (e - e0)] = sw * dv;
                 }
             } else {
                 // sorted columns: theta_min, theta_max, V_min, V_max (slack theta absent)
                 int q = q0;
                 if (fb == tb) {
-                    if (fb != A.slack) hs[q++] = (T)(sw * (d[0] + d[1]));
-                    hs[q] = (T)(sw * (d[2] + d[3]));
+                    if (fb != A.slack) hs[q++] = sw * (d[0] + d[1]);
+                    hs[q] = sw * (d[2] + d[3]);
                 } else {
                     bool fl = fb < tb;
                     int lo = fl ? fb : tb, hi = fl ? tb : fb;
                     double dlo_t = fl ? d[0] : d[1], dhi_t = fl ? d[1] : d[0];
                     double dlo_v = fl ? d[2] : d[3], dhi_v = fl ? d[3] : d[2];
-                    if (lo != A.slack) hs[q++] = (T)(sw * dlo_t);
-                    if (hi != A.slack) hs[q++] = (T)(sw * dhi_t);
-                    hs[q++] = (T)(sw * dlo_v);
-                    hs[q] = (T)(sw * dhi_v);
+                    if (lo != A.slack) hs[q++] = sw * dlo_t;
+                    if (hi != A.slack) hs[q++] = sw * dhi_t;
+                    hs[q++] = sw * dlo_v;
+                    hs[q] = sw * dhi_v;
                 }
             }
         }
@@ -454,16 +452,16 @@ __global__ __launch_bounds__(kBlock) void k_jfinish(int nparts, const double* __
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_spgemm(int nG, const int32_t* __restrict__ gptr,
-                                                   const int2* __restrict__ gpair, const T* __restrict__ hs,
+                                                   const int2* __restrict__ gpair, const double* __restrict__ hs,
                                                    T* __restrict__ Gval) {
     int e = blockIdx.x * kBlock + threadIdx.x;
     if (e >= nG) return;
-    T acc = T(0);
+    double acc = 0.0;
     for (int p = gptr[e]; p < gptr[e + 1]; ++p) {
         int2 pr = gpair[p];
         acc += hs[pr.x] * hs[pr.y];
     }
-    Gval[e] = acc;
+    Gval[e] = (T)acc;  // FP64 products, rounded once to the factor precision
 }
 
 // ---------------------------------------------------------------------------
@@ -516,8 +514,10 @@ __device__ inline void factor_front(const FactArgs<T>& A, int s, T* F) {
         const T gd = ge >= 0 ? A.Gval[ge] : T(0);
         __syncthreads();
         if (!(dd > A.piv_tol * gd) || !(dd > T(0))) {
+            // below threshold: flag it and substitute threshold * G_jj, so the
+            // factor stays SPD (usable as a preconditioner)
             if (tid == 0) A.st->piv = 1;
-            dd = gd > T(0) ? gd : T(1);
+            dd = gd > T(0) ? A.piv_tol * gd : T(1);
         }
         const T dj = sqrt(dd);
         if (tid == 0) F[j * c + j] = dj;
@@ -708,16 +708,108 @@ __global__ __launch_bounds__(kWave) void k_bwd_flow(SolveArgs<T> A, const int32_
 // ---------------------------------------------------------------------------
 // Right-hand side b = H^T W r in permuted state order (FP64 accumulation)
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename TO>
 __global__ __launch_bounds__(kBlock) void k_rhs(int ns, const int32_t* __restrict__ tptr,
                                                 const int32_t* __restrict__ tpos, const int32_t* __restrict__ trow,
-                                                const T* __restrict__ hs, const double* __restrict__ rs,
-                                                T* __restrict__ x) {
+                                                const double* __restrict__ hs, const double* __restrict__ rs,
+                                                TO* __restrict__ x) {
     int k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= ns) return;
     double acc = 0.0;
-    for (int p = tptr[k]; p < tptr[k + 1]; ++p) acc += (double)hs[tpos[p]] * rs[trow[p]];
-    x[k] = (T)acc;
+    for (int p = tptr[k]; p < tptr[k + 1]; ++p) acc += hs[tpos[p]] * rs[trow[p]];
+    x[k] = (TO)acc;
+}
+
+// ---------------------------------------------------------------------------
+// Preconditioned conjugate gradients on the FP64 normal equations
+// (sqrt(W) H)^T (sqrt(W) H) dx = b, the factor (FP32 in the mixed form) as
+// preconditioner.  G p is applied matrix-free as H^T (H p) with the FP64
+// Jacobian values, so the Newton step is accurate to FP64 whatever the
+// accuracy of the FP32 factor.  All scalars stay on the device: step k reads
+// rz[k], pq[k] written by the reductions of the same step.
+// ---------------------------------------------------------------------------
+constexpr int kMaxCg = 16;
+struct CgScalars {
+    double rz[kMaxCg + 1];
+    double pq[kMaxCg];
+};
+
+// t = (sqrt(W) H) p, one measurement row per thread (H columns in permuted order)
+__global__ __launch_bounds__(kBlock) void k_hmul(int m, const int32_t* __restrict__ hrow,
+                                                 const int32_t* __restrict__ hcolp, const double* __restrict__ hs,
+                                                 const double* __restrict__ p, double* __restrict__ t) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= m) return;
+    double acc = 0.0;
+    for (int q = hrow[i]; q < hrow[i + 1]; ++q) acc += hs[q] * p[hcolp[q]];
+    t[i] = acc;
+}
+
+template <typename TA, typename TB>
+__global__ __launch_bounds__(kBlock) void k_convert(int ns, const TA* __restrict__ a, TB* __restrict__ b) {
+    int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k < ns) b[k] = (TB)a[k];
+}
+
+__global__ __launch_bounds__(kBlock) void k_dot_part(int ns, const double* __restrict__ a,
+                                                     const double* __restrict__ b, double* __restrict__ parts) {
+    int k = blockIdx.x * kBlock + threadIdx.x;
+    double v = k < ns ? a[k] * b[k] : 0.0;
+    __shared__ double sm[kBlock / kWave];
+    v = wave_sum(v);
+    if ((threadIdx.x & (kWave - 1)) == 0) sm[threadIdx.x / kWave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) r += sm[w];
+        parts[blockIdx.x] = r;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_dot_finish(int nparts, const double* __restrict__ parts,
+                                                       double* __restrict__ out) {
+    double v = 0.0;
+    for (int p = threadIdx.x; p < nparts; p += kBlock) v += parts[p];
+    __shared__ double sm[kBlock / kWave];
+    v = wave_sum(v);
+    if ((threadIdx.x & (kWave - 1)) == 0) sm[threadIdx.x / kWave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) r += sm[w];
+        *out = r;
+    }
+}
+
+// x = 0, r = b
+__global__ __launch_bounds__(kBlock) void k_cg_init(int ns, const double* __restrict__ b, double* __restrict__ x,
+                                                    double* __restrict__ r) {
+    int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= ns) return;
+    x[k] = 0.0;
+    r[k] = b[k];
+}
+
+// x += alpha p, r -= alpha q with alpha = rz[k] / pq[k]
+__global__ __launch_bounds__(kBlock) void k_cg_xr(int ns, double* __restrict__ x, double* __restrict__ r,
+                                                  const double* __restrict__ p, const double* __restrict__ q,
+                                                  const CgScalars* __restrict__ cg, int step) {
+    int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= ns) return;
+    const double pq = cg->pq[step], rz = cg->rz[step];
+    const double alpha = (pq > 0.0 && rz > 0.0 && isfinite(rz / pq)) ? rz / pq : 0.0;
+    x[k] += alpha * p[k];
+    r[k] -= alpha * q[k];
+}
+
+// p = z + beta p with beta = rz[k+1] / rz[k]
+__global__ __launch_bounds__(kBlock) void k_cg_p(int ns, double* __restrict__ p, const double* __restrict__ z,
+                                                 const CgScalars* __restrict__ cg, int step) {
+    int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= ns) return;
+    const double rz0 = cg->rz[step], rz1 = cg->rz[step + 1];
+    const double beta = (rz0 > 0.0 && isfinite(rz1 / rz0)) ? rz1 / rz0 : 0.0;
+    p[k] = z[k] + beta * p[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -728,9 +820,10 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_update(int n, const int32_t* __restrict__ pth,
                                                    const int32_t* __restrict__ pv, const T* __restrict__ x,
                                                    double* __restrict__ V, double* __restrict__ th, double vmin,
-                                                   double vmax, double* __restrict__ mpart, DevStatus* st) {
+                                                   double vmax, double* __restrict__ mpart, DevStatus* st,
+                                                   int skip_on_pivot) {
     const int b = blockIdx.x * kBlock + threadIdx.x;
-    const bool skip = st->piv != 0 || st->flow_err != 0;
+    const bool skip = (skip_on_pivot && st->piv != 0) || st->flow_err != 0;
     double mx = 0.0;
     if (b < n && !skip) {
         double dv = (double)x[pv[b]];
@@ -834,10 +927,9 @@ struct DBuf {
 
 template <typename T>
 struct PrecBufs {
-    DBuf<T> hs, Gval, F, x, ubuf;
+    DBuf<T> Gval, F, x, ubuf;
     bool ready = false;
     void alloc(const Symbolic& S, size_t& bytes) {
-        hs.alloc(std::max<size_t>(1, S.hcol.size()), bytes);
         Gval.alloc(std::max<int64_t>(1, S.nnz_g), bytes);
         F.alloc(std::max<int64_t>(1, S.ftotal), bytes);
         x.alloc(std::max(1, S.ns), bytes);
@@ -845,7 +937,6 @@ struct PrecBufs {
         ready = true;
     }
     void release(size_t& bytes) {
-        hs.release(bytes);
         Gval.release(bytes);
         F.release(bytes);
         x.release(bytes);
@@ -876,6 +967,11 @@ struct Instance::Impl {
     DBuf<uint8_t> mcode;
     DBuf<int32_t> mloc, hrow;
     DBuf<double> z, sigma, rs, hx, r;
+    DBuf<double> hs;  // sqrt(w) * Jacobian values in the H pattern (FP64 in both forms)
+    // conjugate-gradient refinement (mixed form)
+    DBuf<int32_t> hcolp;  // H column indices in permuted state order
+    DBuf<double> cg_b, cg_x, cg_r, cg_p, cg_q, cg_z, cg_t, cg_parts;
+    DBuf<CgScalars> cg_s;
     DBuf<double> jpart, rpart, mpart;
     DBuf<int> ipart;
     DBuf<int32_t> zidx;
@@ -895,6 +991,8 @@ struct Instance::Impl {
     int level_launches = 0;  // non-empty (level, class) launches of one factor / solve sweep
     // dependency-driven sweeps (all fronts fit in LDS)
     bool use_flow = false;
+    bool flow_ok = false;       // every front fits the LDS front of the dataflow kernels
+    bool force_levels = false;  // caller asked for level-by-level sweeps
     int max_c = 0;
     uint32_t epoch = 0;
     DBuf<uint32_t> done;
@@ -936,7 +1034,8 @@ Instance::~Instance() {
     rel(d_->tap); rel(d_->shift); rel(d_->status); rel(d_->yrow); rel(d_->ycol); rel(d_->ycptr); rel(d_->ycon);
     rel(d_->yG); rel(d_->yB); rel(d_->Pij); rel(d_->Qij); rel(d_->V); rel(d_->th); rel(d_->P); rel(d_->Q);
     rel(d_->Pf); rel(d_->Qf); rel(d_->Pt); rel(d_->Qt); rel(d_->mcode); rel(d_->mloc); rel(d_->hrow);
-    rel(d_->z); rel(d_->sigma); rel(d_->rs); rel(d_->hx); rel(d_->r); rel(d_->jpart); rel(d_->rpart);
+    rel(d_->hs); rel(d_->hcolp); rel(d_->cg_b); rel(d_->cg_x); rel(d_->cg_r); rel(d_->cg_p); rel(d_->cg_q);
+    rel(d_->cg_z); rel(d_->cg_t); rel(d_->cg_parts); rel(d_->cg_s); rel(d_->z); rel(d_->sigma); rel(d_->rs); rel(d_->hx); rel(d_->r); rel(d_->jpart); rel(d_->rpart);
     rel(d_->mpart); rel(d_->ipart); rel(d_->zidx); rel(d_->zval); rel(d_->tptr); rel(d_->tpos); rel(d_->trow);
     rel(d_->gptr); rel(d_->gloc); rel(d_->gdiag); rel(d_->g_sn_ptr); rel(d_->gpair); rel(d_->sn_first);
     rel(d_->sn_k); rel(d_->sn_c); rel(d_->child_ptr); rel(d_->child); rel(d_->rmap); rel(d_->rows);
@@ -1024,6 +1123,19 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
         D.ipart.alloc(nparts, bytes_);
     }
     D.mpart.alloc(nblk(std::max(1, h.n)), bytes_);
+    D.hs.alloc(std::max<size_t>(1, S.hcol.size()), bytes_);
+    {
+        std::vector<int32_t> hp(S.hcol.size());
+        for (size_t q = 0; q < hp.size(); ++q) hp[q] = S.iperm[S.hcol[q]];
+        if (hp.empty()) hp.push_back(0);
+        D.hcolp.upload(hp, bytes_, s);
+        HIPCHECK(hipStreamSynchronize(s));
+        const size_t nsv = std::max(1, S.ns);
+        for (DBuf<double>* v : {&D.cg_b, &D.cg_x, &D.cg_r, &D.cg_p, &D.cg_q, &D.cg_z}) v->alloc(nsv, bytes_);
+        D.cg_t.alloc(std::max(1, h.m), bytes_);
+        D.cg_parts.alloc(nblk(nsv), bytes_);
+        D.cg_s.alloc(1, bytes_);
+    }
     D.tptr.upload(S.tptr, bytes_, s);
     D.tpos.upload(S.tpos, bytes_, s);
     D.trow.upload(S.trow, bytes_, s);
@@ -1082,7 +1194,8 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
         for (int cls = 0; cls < 3; ++cls) D.level_launches += D.lv_cnt[cls][l] ? 1 : 0;
     // dependency-driven sweeps: level order lists children before parents
     D.max_c = S.max_c;
-    D.use_flow = S.nsn > 0 && S.max_c <= kFrontB && std::getenv("SLE_LEVEL_SCHEDULE") == nullptr;
+    D.flow_ok = S.nsn > 0 && S.max_c <= kFrontB;
+    D.use_flow = D.flow_ok && !D.force_levels;
     if (S.nsn > 0) {
         D.order_fwd.upload(S.level_sn, bytes_, s);
         std::vector<int32_t> rev(S.level_sn.rbegin(), S.level_sn.rend());
@@ -1196,7 +1309,6 @@ void launch_power(Instance::Impl& D) {
                                                 D.Q.p, D.Pij.p, D.Qij.p, D.st.p);
 }
 
-template <typename T>
 void launch_meas(Instance::Impl& D, int robust, double gamma, int unit_weights, int want_jac, bool want_out) {
     MeasArgs A;
     A.m = D.m;
@@ -1234,13 +1346,13 @@ void launch_meas(Instance::Impl& D, int robust, double gamma, int unit_weights, 
     A.jpart = D.jpart.p;
     A.rpart = D.rpart.p;
     A.ipart = D.ipart.p;
-    if (D.m) k_meas<T><<<nblk(D.m), kBlock, 0, D.stream>>>(A, bufs<T>(D).hs.p);
+    if (D.m) k_meas<<<nblk(D.m), kBlock, 0, D.stream>>>(A, D.hs.p);
 }
 
 template <typename T>
 void launch_spgemm(Instance::Impl& D, const Symbolic& S) {
     if (S.nnz_g)
-        k_spgemm<T><<<nblk(S.nnz_g), kBlock, 0, D.stream>>>((int)S.nnz_g, D.gptr.p, D.gpair.p, bufs<T>(D).hs.p,
+        k_spgemm<T><<<nblk(S.nnz_g), kBlock, 0, D.stream>>>((int)S.nnz_g, D.gptr.p, D.gpair.p, D.hs.p,
                                                             bufs<T>(D).Gval.p);
 }
 
@@ -1336,16 +1448,54 @@ void launch_trisolve(Instance::Impl& D, Prof* pf = nullptr) {
 template <typename T>
 void launch_rhs(Instance::Impl& D) {
     if (D.ns)
-        k_rhs<T><<<nblk(D.ns), kBlock, 0, D.stream>>>(D.ns, D.tptr.p, D.tpos.p, D.trow.p, bufs<T>(D).hs.p, D.rs.p,
+        k_rhs<T><<<nblk(D.ns), kBlock, 0, D.stream>>>(D.ns, D.tptr.p, D.tpos.p, D.trow.p, D.hs.p, D.rs.p,
                                                       bufs<T>(D).x.p);
 }
 
+// dx source: the factor-precision solve vector, or the FP64 CG solution
 template <typename T>
-void launch_update(Instance::Impl& D, double vmin, double vmax) {
+void launch_update(Instance::Impl& D, double vmin, double vmax, const T* dx, int skip_on_pivot) {
     int nbk = nblk(D.n);
-    k_update<T><<<nbk, kBlock, 0, D.stream>>>(D.n, D.pth.p, D.pv.p, bufs<T>(D).x.p, D.V.p, D.th.p, vmin, vmax,
-                                              D.mpart.p, D.st.p);
+    k_update<T><<<nbk, kBlock, 0, D.stream>>>(D.n, D.pth.p, D.pv.p, dx, D.V.p, D.th.p, vmin, vmax, D.mpart.p, D.st.p,
+                                              skip_on_pivot);
     k_maxfinish<<<1, kBlock, 0, D.stream>>>(nbk, D.mpart.p, D.st.p);
+}
+
+void launch_dot(Instance::Impl& D, const double* a, const double* b, double* out) {
+    int nb = nblk(D.ns);
+    k_dot_part<<<nb, kBlock, 0, D.stream>>>(D.ns, a, b, D.cg_parts.p);
+    k_dot_finish<<<1, kBlock, 0, D.stream>>>(nb, D.cg_parts.p, out);
+}
+
+// z = M^{-1} r with the factor in precision T
+template <typename T>
+void launch_precond(Instance::Impl& D, const double* r, double* z) {
+    int nb = nblk(D.ns);
+    k_convert<double, T><<<nb, kBlock, 0, D.stream>>>(D.ns, r, bufs<T>(D).x.p);
+    launch_trisolve<T>(D, nullptr);
+    k_convert<T, double><<<nb, kBlock, 0, D.stream>>>(D.ns, bufs<T>(D).x.p, z);
+}
+
+// PCG: cg_x ~ G^{-1} b with `steps` conjugate-gradient steps (b = H^T W r in FP64)
+template <typename T>
+void launch_pcg(Instance::Impl& D, int steps) {
+    const int ns = D.ns, nb = nblk(ns);
+    CgScalars* cs = D.cg_s.p;
+    k_rhs<double><<<nb, kBlock, 0, D.stream>>>(ns, D.tptr.p, D.tpos.p, D.trow.p, D.hs.p, D.rs.p, D.cg_b.p);
+    k_cg_init<<<nb, kBlock, 0, D.stream>>>(ns, D.cg_b.p, D.cg_x.p, D.cg_r.p);
+    launch_precond<T>(D, D.cg_r.p, D.cg_z.p);
+    launch_dot(D, D.cg_r.p, D.cg_z.p, &cs->rz[0]);
+    k_convert<double, double><<<nb, kBlock, 0, D.stream>>>(ns, D.cg_z.p, D.cg_p.p);
+    for (int k = 0; k < steps; ++k) {
+        if (D.m) k_hmul<<<nblk(D.m), kBlock, 0, D.stream>>>(D.m, D.hrow.p, D.hcolp.p, D.hs.p, D.cg_p.p, D.cg_t.p);
+        k_rhs<double><<<nb, kBlock, 0, D.stream>>>(ns, D.tptr.p, D.tpos.p, D.trow.p, D.hs.p, D.cg_t.p, D.cg_q.p);
+        launch_dot(D, D.cg_p.p, D.cg_q.p, &cs->pq[k]);
+        k_cg_xr<<<nb, kBlock, 0, D.stream>>>(ns, D.cg_x.p, D.cg_r.p, D.cg_p.p, D.cg_q.p, cs, k);
+        if (k + 1 == steps) break;
+        launch_precond<T>(D, D.cg_r.p, D.cg_z.p);
+        launch_dot(D, D.cg_r.p, D.cg_z.p, &cs->rz[k + 1]);
+        k_cg_p<<<nb, kBlock, 0, D.stream>>>(ns, D.cg_p.p, D.cg_z.p, cs, k);
+    }
 }
 
 void read_status(Instance::Impl& D) {
@@ -1393,7 +1543,7 @@ void iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p, bool 
     launch_power(D);
     pf.end(PH_POWER, 1);
     pf.begin(PH_MEAS);
-    launch_meas<T>(D, p.robust, p.gamma, 0, 1, false);
+    launch_meas(D, p.robust, p.gamma, 0, 1, false);
     pf.end(PH_MEAS, 1);
     if (refactor) {
         pf.begin(PH_SPGEMM);
@@ -1403,12 +1553,21 @@ void iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p, bool 
         launch_factor<T>(D, piv_tol);
         pf.end(PH_FACTOR, D.level_launches);
     }
+    if (p.pcg_steps > 0) {
+        pf.begin(PH_PCG);
+        launch_pcg<T>(D, std::min(p.pcg_steps, kMaxCg));
+        pf.end(PH_PCG, 0);
+        pf.begin(PH_UPDATE);
+        launch_update<double>(D, p.vmin, p.vmax, D.cg_x.p, 0);
+        pf.end(PH_UPDATE, 2);
+        return;
+    }
     pf.begin(PH_RHS);
     launch_rhs<T>(D);
     pf.end(PH_RHS, 1);
     launch_trisolve<T>(D, &pf);
     pf.begin(PH_UPDATE);
-    launch_update<T>(D, p.vmin, p.vmax);
+    launch_update<T>(D, p.vmin, p.vmax, bufs<T>(D).x.p, 1);
     pf.end(PH_UPDATE, 2);
 }
 
@@ -1436,9 +1595,12 @@ SolveOutcome Instance::solve(const SolveParams& p) {
         for (int it = 0; it < p.max_iter; ++it) {
             const bool fp64 = precision_ == 1 || fp64_factor_active_;
             const bool refactor = p.robust || !factor_valid_ || p.mode == 1;
+            const bool pcg = !fp64 && p.pcg_steps > 0;
             if (fp64) {
                 if (!D.f64.ready) D.f64.alloc(S, bytes_);
-                iteration<double>(D, S, p, refactor, 1e-13, pf);
+                SolveParams pd = p;
+                pd.pcg_steps = 0;  // the FP64 factor solves the Newton system directly
+                iteration<double>(D, S, pd, refactor, 1e-13, pf);
             } else {
                 iteration<float>(D, S, p, refactor, 1e-5f, pf);
             }
@@ -1452,7 +1614,11 @@ SolveOutcome Instance::solve(const SolveParams& p) {
                 --it;
                 continue;
             }
-            if (D.hst->piv) {
+            if (D.hst->piv && pcg) {
+                // FP32 pivots below threshold were substituted: the factor is a
+                // (weaker) preconditioner, CG keeps the Newton step exact
+                out.pivots_substituted++;
+            } else if (D.hst->piv) {
                 factor_valid_ = false;
                 if (!fp64) {
                     // FP32 pivot below threshold: refactor this iteration in FP64
@@ -1506,10 +1672,7 @@ SolveOutcome Instance::solve(const SolveParams& p) {
         k_flows<<<nblk(D.nb), kBlock, 0, D.stream>>>(D.nb, D.from.p, D.to.p, D.gs.p, D.bs.p, D.bc.p, D.tap.p,
                                                      D.shift.p, D.status.p, D.V.p, D.th.p, D.Pf.p, D.Qf.p, D.Pt.p,
                                                      D.Qt.p);
-    if (precision_ == 1 || fp64_factor_active_)
-        launch_meas<double>(D, 0, p.gamma, 0, 0, true);
-    else
-        launch_meas<float>(D, 0, p.gamma, 0, 0, true);
+    launch_meas(D, 0, p.gamma, 0, 0, true);
     if (D.m) k_jfinish<<<1, kBlock, 0, D.stream>>>(nblk(D.m), D.jpart.p, D.rpart.p, D.ipart.p, D.st.p);
     HIPCHECK(hipEventRecord(D.ev1, D.stream));
     HIPCHECK(hipGetLastError());
@@ -1551,10 +1714,7 @@ void Instance::debug_eval(double* hx, double* P, double* Q, double* Pf, double* 
         k_flows<<<nblk(D.nb), kBlock, 0, D.stream>>>(D.nb, D.from.p, D.to.p, D.gs.p, D.bs.p, D.bc.p, D.tap.p,
                                                      D.shift.p, D.status.p, D.V.p, D.th.p, D.Pf.p, D.Qf.p, D.Pt.p,
                                                      D.Qt.p);
-    if (precision_ == 1)
-        launch_meas<double>(D, 0, 1.5, 0, 0, true);
-    else
-        launch_meas<float>(D, 0, 1.5, 0, 0, true);
+    launch_meas(D, 0, 1.5, 0, 0, true);
     HIPCHECK(hipGetLastError());
     get_results(P, Q, Pf, Qf, Pt, Qt, hx, nullptr);
 }
@@ -1583,16 +1743,16 @@ void Instance::debug_jacobian_gain(double* hv, double* gv) {
     const Symbolic& S = *S_;
     launch_power(D);
     if (precision_ == 1) {
-        launch_meas<double>(D, 0, 1.5, 1, 1, false);
+        launch_meas(D, 0, 1.5, 1, 1, false);
         launch_spgemm<double>(D, S);
         HIPCHECK(hipGetLastError());
-        if (hv) fetch_as_double(D.f64.hs, S.hcol.size(), hv, D.stream);
+        if (hv) fetch_as_double(D.hs, S.hcol.size(), hv, D.stream);
         if (gv) fetch_as_double(D.f64.Gval, S.nnz_g, gv, D.stream);
     } else {
-        launch_meas<float>(D, 0, 1.5, 1, 1, false);
+        launch_meas(D, 0, 1.5, 1, 1, false);
         launch_spgemm<float>(D, S);
         HIPCHECK(hipGetLastError());
-        if (hv) fetch_as_double(D.f32.hs, S.hcol.size(), hv, D.stream);
+        if (hv) fetch_as_double(D.hs, S.hcol.size(), hv, D.stream);
         if (gv) fetch_as_double(D.f32.Gval, S.nnz_g, gv, D.stream);
     }
 }
@@ -1604,11 +1764,11 @@ int Instance::debug_factor(double* fronts) {
     for (int attempt = 0; attempt < 2; ++attempt) {
         launch_power(D);
         if (precision_ == 1) {
-            launch_meas<double>(D, 0, 1.5, 0, 1, false);
+            launch_meas(D, 0, 1.5, 0, 1, false);
             launch_spgemm<double>(D, S);
             launch_factor<double>(D, 1e-13);
         } else {
-            launch_meas<float>(D, 0, 1.5, 0, 1, false);
+            launch_meas(D, 0, 1.5, 0, 1, false);
             launch_spgemm<float>(D, S);
             launch_factor<float>(D, 1e-5f);
         }
@@ -1685,6 +1845,13 @@ void Instance::apply_staged(int slot) {
         k_scatter_z<<<nblk(sb.count), kBlock, 0, D.stream>>>(sb.count, sb.idx.p, sb.val.p, D.z.p);
     HIPCHECK(hipGetLastError());
 }
+
+void Instance::set_level_schedule(bool on) {
+    d_->force_levels = on;
+    d_->use_flow = !on && d_->flow_ok;
+}
+
+bool Instance::dataflow_sweeps() const { return d_->use_flow; }
 
 int Instance::staged_count(int slot) const {
     if (slot < 0 || slot >= (int)d_->staged.size()) return -1;

@@ -143,6 +143,21 @@ void Engine::set_config(const EngineConfig& c) {
     if (prec_change) model_dirty_ = true;
 }
 
+int Engine::set_option(const char* key, double value) {
+    if (!key) return ST_INVALID_ARGUMENT;
+    std::string k = key;
+    if (k == "pcg_steps") {
+        if (value < 0 || value > 16) return ST_INVALID_ARGUMENT;
+        opt_.pcg_steps = (int)value;
+    } else if (k == "level_schedule") {
+        opt_.level_schedule = value != 0;
+        if (inst_) inst_->set_level_schedule(opt_.level_schedule);
+    } else {
+        return ST_INVALID_ARGUMENT;
+    }
+    return ST_OK;
+}
+
 int Engine::bus_index(const char* id) const {
     if (!id) return -1;
     auto it = bus_map_.find(id);
@@ -520,6 +535,7 @@ int Engine::rebuild() {
     }
     inst_->set_profiling(cfg_.enable_profiling);
     inst_->set_use_graphs(cfg_.use_cuda_graphs);
+    inst_->set_level_schedule(opt_.level_schedule);
     inst_->setup(h, sym_);
     inst_->set_state(V_.data(), th_.data());
     model_dirty_ = false;
@@ -725,6 +741,7 @@ int Engine::solve(const SolverConfigIn& sc, ResultOut& out) {
     p.vmin = cfg_.voltage_min_pu;
     p.vmax = cfg_.voltage_max_pu;
     p.time_limit_ms = sc.mode == 0 ? sc.time_limit_ms : 0.0;
+    p.pcg_steps = cfg_.use_double_precision ? 0 : opt_.pcg_steps;
     SolveOutcome o = inst_->solve(p);
     state_dirty_ = true;
     results_dirty_ = true;
@@ -738,6 +755,8 @@ int Engine::solve(const SolverConfigIn& sc, ResultOut& out) {
     out.refactors = o.refactors;
     out.fp64_refactors = o.fp64_refactors;
     out.flags = o.flags;
+    out.pivots_substituted = o.pivots_substituted;
+    out.pcg_steps = p.pcg_steps;
     out.bad_data_count = 0;
     last_ = out;
     return ST_OK;

@@ -76,7 +76,13 @@ struct ResultOut {
     int iterations = 0;
     double max_mismatch = 0, objective = 0, ms = 0, largest_residual = 0;
     int largest_residual_idx = -1, bad_data_count = 0;
-    int refactors = 0, fp64_refactors = 0, flags = 0;
+    int refactors = 0, fp64_refactors = 0, flags = 0, pivots_substituted = 0, pcg_steps = 0;
+};
+
+// Engine options beyond SLE_Config (sle_ext_SetOption).
+struct EngineOptions {
+    int pcg_steps = 3;         // mixed form: CG steps refining each Newton step (0 = plain FP32 solve)
+    bool level_schedule = true;   // level-by-level sweeps (0: single dependency-driven launch per sweep)
 };
 
 class Engine {
@@ -88,6 +94,8 @@ class Engine {
     int initialize();
     bool initialized() const { return initialized_; }
     EngineConfig& config() { return cfg_; }
+    int set_option(const char* key, double value);
+    const EngineOptions& options() const { return opt_; }
     void set_config(const EngineConfig& c);
     std::string config_path;
 
@@ -195,6 +203,7 @@ class Engine {
     int rebuild();
 
     EngineConfig cfg_;
+    EngineOptions opt_;
     bool initialized_ = false;
     // buses
     std::vector<std::string> bus_id_;

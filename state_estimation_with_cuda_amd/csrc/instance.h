@@ -41,13 +41,15 @@ struct SolveParams {
     double vmin = 0.5, vmax = 1.5;
     double time_limit_ms = 0;  // REALTIME only, 0 = none
     int retries = 2;           // flat-start retries after divergence (not in REALTIME)
+    int pcg_steps = 0;         // >0: conjugate-gradient refinement of each Newton step (factor as preconditioner)
 };
 
 struct SolveOutcome {
     int status = 5;  // SLE_ConvergenceStatus
     int iterations = 0;
     int refactors = 0;
-    int fp64_refactors = 0;  // mixed form: pivots that fell below threshold
+    int fp64_refactors = 0;  // mixed form without CG: pivots that fell below threshold
+    int pivots_substituted = 0;  // mixed form with CG: iterations whose FP32 factor had substituted pivots
     double max_mismatch = 0;
     double objective = 0;
     double largest_residual = 0;  // max |r|/sigma over active rows
@@ -57,7 +59,7 @@ struct SolveOutcome {
 };
 
 // Kernel classes timed with HIP events on the instance stream (profiling mode).
-enum Phase : int { PH_POWER = 0, PH_MEAS, PH_SPGEMM, PH_FACTOR, PH_RHS, PH_FWD, PH_BWD, PH_UPDATE, PH_COUNT };
+enum Phase : int { PH_POWER = 0, PH_MEAS, PH_SPGEMM, PH_FACTOR, PH_RHS, PH_FWD, PH_BWD, PH_UPDATE, PH_PCG, PH_COUNT };
 
 struct KernelTimes {
     double ms[PH_COUNT] = {0};       // accumulated device time between the phase's first launch and last completion
@@ -115,6 +117,9 @@ class Instance {
     void set_profiling(bool on) { profiling_ = on; }
     bool profiling() const { return profiling_; }
     void set_use_graphs(bool on) { use_graphs_ = on; }
+    // dependency-driven sweeps (default when every front fits in LDS) or level-by-level launches
+    void set_level_schedule(bool on);
+    bool dataflow_sweeps() const;
 
     // Device-resident telemetry batches: stage (host -> HBM) ahead of time,
     // apply (scatter into z on the device) right before a solve.

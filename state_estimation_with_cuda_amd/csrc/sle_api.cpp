@@ -852,12 +852,19 @@ SLE_API SLE_StatusCode SLE_CALL sle_ext_Solve(SLE_Handle handle, const SLE_Solve
             result->largest_residual = r.largest_residual;
             result->largest_residual_idx = r.largest_residual_idx;
             result->flags = r.flags;
+            result->pivots_substituted = r.pivots_substituted;
+            result->pcg_steps = r.pcg_steps;
             result->gpu_ms = r.ms;
             result->host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         }
         if (rc != sle::ST_OK) return from_int(rc, "solve");
         return map_status(r.status);
     });
+}
+
+SLE_API SLE_StatusCode SLE_CALL sle_ext_SetOption(SLE_Handle handle, const char* key, double value) {
+    if (!handle) return fail(SLE_ERROR_INVALID_HANDLE, "NULL handle");
+    return guarded([&]() -> SLE_StatusCode { return from_int(E(handle).set_option(key, value), "set option"); });
 }
 
 SLE_API SLE_StatusCode SLE_CALL sle_ext_UpdateTelemetryIndexed(SLE_Handle handle, const int32_t* idx,

@@ -113,7 +113,8 @@ class ExtResult(C.Structure):
     _fields_ = [("status", C.c_int32), ("iterations", C.c_int32), ("refactors", C.c_int32),
                 ("fp64_refactors", C.c_int32), ("max_mismatch", C.c_double), ("objective", C.c_double),
                 ("largest_residual", C.c_double), ("largest_residual_idx", C.c_int32), ("flags", C.c_int32),
-                ("gpu_ms", C.c_double), ("host_ms", C.c_double)]
+                ("gpu_ms", C.c_double), ("host_ms", C.c_double), ("pivots_substituted", C.c_int32),
+                ("pcg_steps", C.c_int32)]
 
 
 # every function of the reference ABI (sle_api.h:447-1391) and the extensions
@@ -141,9 +142,9 @@ EXT_FUNCTIONS = [
     "sle_ext_GetState", "sle_ext_GetResults", "sle_ext_SetBranchStatus", "sle_ext_GetSizes", "sle_ext_GetYbus",
     "sle_ext_GetHPattern", "sle_ext_GetGPattern", "sle_ext_GetOrdering", "sle_ext_Evaluate",
     "sle_ext_JacobianGain", "sle_ext_Factor", "sle_ext_SolveFactor", "sle_ext_StageTelemetry",
-    "sle_ext_ApplyStagedTelemetry", "sle_ext_GetKernelTimes", "sle_ext_ResetKernelTimes",
+    "sle_ext_ApplyStagedTelemetry", "sle_ext_GetKernelTimes", "sle_ext_ResetKernelTimes", "sle_ext_SetOption",
 ]
-KERNEL_CLASSES = ["power", "meas", "spgemm", "factor", "rhs", "fwd", "bwd", "update"]
+KERNEL_CLASSES = ["power", "meas", "spgemm", "factor", "rhs", "fwd", "bwd", "update", "pcg"]
 
 
 class Library:
@@ -265,6 +266,7 @@ class Library:
             "sle_ext_ApplyStagedTelemetry": (C.c_int, [H, i32]),
             "sle_ext_GetKernelTimes": (C.c_int, [H, dp, lp, lp, i32]),
             "sle_ext_ResetKernelTimes": (C.c_int, [H]),
+            "sle_ext_SetOption": (C.c_int, [H, C.c_char_p, C.c_double]),
         }
         missing = [n for n in API_FUNCTIONS + EXT_FUNCTIONS if n not in sig]
         assert not missing, missing
@@ -481,6 +483,7 @@ class Engine:
         return dict(rc=rc, status=r.status, iterations=r.iterations, refactors=r.refactors,
                     fp64_refactors=r.fp64_refactors, max_mismatch=r.max_mismatch, objective=r.objective,
                     largest_residual=r.largest_residual, largest_residual_idx=r.largest_residual_idx, flags=r.flags,
+                    pivots_substituted=r.pivots_substituted, pcg_steps=r.pcg_steps,
                     gpu_ms=r.gpu_ms, host_ms=r.host_ms)
 
     def solve_mode(self, mode):
@@ -505,6 +508,9 @@ class Engine:
         ptrs = [out[k].ctypes.data for k in ("P", "Q", "Pf", "Qf", "Pt", "Qt", "h", "r")]
         self._check(self.L.sle_ext_GetResults(self.h, *ptrs), "sle_ext_GetResults")
         return out
+
+    def set_option(self, key, value):
+        self._check(self.L.sle_ext_SetOption(self.h, key.encode(), float(value)), "sle_ext_SetOption")
 
     def stage_telemetry(self, slot, idx, val):
         idx = np.ascontiguousarray(idx, dtype=np.int32)
