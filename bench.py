@@ -95,6 +95,7 @@ def run_form(g, fp64, reqs, batches, warmup, steps, profile_steps):
     setup_s = time.perf_counter() - t0
     sz = e.sizes()
     log(f"{name}: model loaded + symbolic in {setup_s:.1f}s; sizes {sz}")
+    log(f"{name}: plan: {e.describe()}")
     t0 = time.perf_counter()
     r0 = e.solve(mode=1, max_iterations=30, tolerance=TOL, flat_start=True)
     init_ms = (time.perf_counter() - t0) * 1e3
@@ -122,6 +123,7 @@ def run_form(g, fp64, reqs, batches, warmup, steps, profile_steps):
         ok += r["status"] == 0
     total = time.perf_counter() - t_start
     V, th = e.state()
+    e_plan = e.describe()
     last_z = request_z(g, reqs[warmup + steps - 1])
     # profiled pass (separate, so the events do not perturb the timed loop)
     kt = None
@@ -152,6 +154,7 @@ def run_form(g, fp64, reqs, batches, warmup, steps, profile_steps):
         kernel_times=kt,
         sizes=sz,
         flags=sorted(flags),
+        plan=e_plan,
     )
 
 
@@ -228,6 +231,7 @@ def main():
         "kernel_ms_per_step": {k: round(v, 4) for k, v in per_step.items()},
         "kernel_launches_per_step": {k: round(v["launches"] / max(args.profile_steps, 1), 2) for k, v in kt.items()},
         "solve_flags": mixed["flags"],
+        "plan": mixed["plan"],
         "symbolic": {k: sz[k] for k in ("ns", "nnz_h", "nnz_g", "nnz_l", "n_snodes", "n_levels", "max_front")},
     }
     if fp64:

@@ -36,7 +36,11 @@ typedef struct SLE_ExtResult {
 
 /* Engine options outside SLE_Config:
  *   "pcg_steps"       mixed form: conjugate-gradient steps refining each Newton
- *                     step with the FP32 factor as preconditioner (0..16, default 3)
+ *                     step with the FP32 factor as preconditioner (0..16, default 0:
+ *                     the Gauss-Newton iteration itself refines the FP32 solve)
+ *   "task_sweeps"     1 (default): walk whole bottom subtrees and the narrow
+ *                     top of the supernodal tree inside single workgroups
+ *                     (fewer launches); 0: one launch per level
  *   "level_schedule"  1 (default): factor and solve level by level (kernel
  *                     boundaries order the fronts); 0: one dependency-driven
  *                     launch per sweep with release/acquire flags per front */
@@ -79,6 +83,10 @@ SLE_API SLE_StatusCode SLE_CALL sle_ext_ApplyStagedTelemetry(SLE_Handle handle, 
 SLE_API SLE_StatusCode SLE_CALL sle_ext_GetKernelTimes(SLE_Handle handle, double* ms, int64_t* launches,
                                                        int64_t* calls, int32_t count);
 SLE_API SLE_StatusCode SLE_CALL sle_ext_ResetKernelTimes(SLE_Handle handle);
+
+/* One-line description of the engine's current execution plan (precision
+ * form, factor/solve sweep schedule); valid until the next call on the thread. */
+SLE_API const char* SLE_CALL sle_ext_Describe(SLE_Handle handle);
 
 /* Breaker status of branch k (insertion index); applied before the next solve. */
 SLE_API SLE_StatusCode SLE_CALL sle_ext_SetBranchStatus(SLE_Handle handle, int32_t branch_index,

@@ -489,14 +489,36 @@ struct FactArgs {
     DevStatus* st;
 };
 
-template <typename T, int NT>
+// Who works on one front: the whole workgroup (Team<false>, NT threads) or a
+// single wave inside a larger workgroup (Team<true>, 64 lanes) when several
+// waves each process a different front.
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+template <bool WAVE>
+struct Team;
+template <>
+struct Team<false> {
+    static __device__ int id() { return threadIdx.x; }
+    static __device__ void sync() { __syncthreads(); }
+};
+template <>
+struct Team<true> {
+    static __device__ int id() { return threadIdx.x & (kWave - 1); }
+    static __device__ void sync() { wave_sync(); }
+};
+
+template <typename T, int NT, bool WAVE = false>
 __device__ inline void factor_front(const FactArgs<T>& A, int s, T* F) {
-    const int tid = threadIdx.x;
+    using Tm = Team<WAVE>;
+    const int tid = Tm::id();
     const int c = A.sn_c[s], k = A.sn_k[s], first = A.sn_first[s];
     for (int idx = tid; idx < c * c; idx += NT) F[idx] = T(0);
-    __syncthreads();
+    Tm::sync();
     for (int e = A.g_sn_ptr[s] + tid; e < A.g_sn_ptr[s + 1]; e += NT) F[A.gloc[e]] = A.Gval[e];
-    __syncthreads();
+    Tm::sync();
     for (int p = A.child_ptr[s]; p < A.child_ptr[s + 1]; ++p) {
         const int ch = A.child[p];
         const int cc = A.sn_c[ch], kc = A.sn_k[ch], u = cc - kc;
@@ -506,13 +528,13 @@ __device__ inline void factor_front(const FactArgs<T>& A, int s, T* F) {
             int a = idx % u, b = idx / u;
             if (a >= b) F[rm[b] * c + rm[a]] += Fc[(int64_t)(kc + b) * cc + kc + a];
         }
-        __syncthreads();
+        Tm::sync();
     }
     for (int j = 0; j < k; ++j) {
         T dd = F[j * c + j];
         const int ge = A.gdiag[first + j];
         const T gd = ge >= 0 ? A.Gval[ge] : T(0);
-        __syncthreads();
+        Tm::sync();
         if (!(dd > A.piv_tol * gd) || !(dd > T(0))) {
             // below threshold: flag it and substitute threshold * G_jj, so the
             // factor stays SPD (usable as a preconditioner)
@@ -522,7 +544,7 @@ __device__ inline void factor_front(const FactArgs<T>& A, int s, T* F) {
         const T dj = sqrt(dd);
         if (tid == 0) F[j * c + j] = dj;
         for (int i = j + 1 + tid; i < c; i += NT) F[j * c + i] = F[j * c + i] / dj;
-        __syncthreads();
+        Tm::sync();
         const int w = c - j - 1;
         for (int idx = tid; idx < w * w; idx += NT) {
             int a = idx % w, b = idx / w;
@@ -531,7 +553,7 @@ __device__ inline void factor_front(const FactArgs<T>& A, int s, T* F) {
                 F[bb * c + aa] -= F[j * c + aa] * F[j * c + bb];
             }
         }
-        __syncthreads();
+        Tm::sync();
     }
 }
 
@@ -575,47 +597,49 @@ struct SolveArgs {
     T* ubuf;
 };
 
-template <typename T>
+template <typename T, bool WAVE = false>
 __device__ inline void fwd_front(const SolveArgs<T>& A, int s, T* f) {
-    const int tid = threadIdx.x;
+    using Tm = Team<WAVE>;
+    const int tid = Tm::id();
     const int c = A.sn_c[s], k = A.sn_k[s], first = A.sn_first[s];
     const T* L = A.F + A.foff[s];
     for (int i = tid; i < c; i += kWave) f[i] = i < k ? A.x[first + i] : T(0);
-    __syncthreads();
+    Tm::sync();
     for (int p = A.child_ptr[s]; p < A.child_ptr[s + 1]; ++p) {
         const int ch = A.child[p];
         const int u = A.sn_c[ch] - A.sn_k[ch];
         const T* uc = A.ubuf + A.uoff[ch];
         const int32_t* rm = A.rmap + A.rmoff[ch];
         for (int a = tid; a < u; a += kWave) f[rm[a]] += uc[a];
-        __syncthreads();
+        Tm::sync();
     }
     for (int j = 0; j < k; ++j) {
         const T yj = f[j] / L[j * c + j];
-        __syncthreads();
+        Tm::sync();
         if (tid == 0) f[j] = yj;
         for (int i = j + 1 + tid; i < c; i += kWave) f[i] -= L[j * c + i] * yj;
-        __syncthreads();
+        Tm::sync();
     }
     for (int i = tid; i < k; i += kWave) A.x[first + i] = f[i];
     T* us = A.ubuf + A.uoff[s];
     for (int a = tid; a < c - k; a += kWave) us[a] = f[k + a];
 }
 
-template <typename T>
+template <typename T, bool WAVE = false>
 __device__ inline void bwd_front(const SolveArgs<T>& A, int s, T* xs) {
-    const int tid = threadIdx.x;
+    using Tm = Team<WAVE>;
+    const int tid = Tm::id();
     const int c = A.sn_c[s], k = A.sn_k[s], first = A.sn_first[s];
     const T* L = A.F + A.foff[s];
     const int32_t* rows = A.rows + A.rows_off[s];
     for (int i = tid; i < c; i += kWave) xs[i] = i < k ? A.x[first + i] : A.x[rows[i]];
-    __syncthreads();
+    Tm::sync();
     for (int j = k - 1; j >= 0; --j) {
         double part = 0.0;
         for (int i = j + 1 + tid; i < c; i += kWave) part += (double)(L[j * c + i] * xs[i]);
         part = wave_sum(part);
         if (tid == 0) xs[j] = (T)(((double)xs[j] - part) / (double)L[j * c + j]);
-        __syncthreads();
+        Tm::sync();
     }
     for (int i = tid; i < k; i += kWave) A.x[first + i] = xs[i];
 }
@@ -630,6 +654,48 @@ template <typename T, int VM>
 __global__ __launch_bounds__(kWave) void k_bwd(SolveArgs<T> A, const int32_t* __restrict__ list) {
     __shared__ T xs[VM];
     bwd_front<T>(A, list[blockIdx.x], xs);
+}
+
+// ---------------------------------------------------------------------------
+// Task sweeps: one workgroup walks a whole group of fronts level by level
+// (fronts of one level spread over its waves, one front per wave at a time,
+// __syncthreads between levels).  A task is either a complete bottom subtree
+// of the supernodal tree or the narrow top of the tree; all data exchanged
+// between its fronts stays inside one CU, so no cross-XCD ordering is needed
+// below the kernel boundary.  MODE 0 factor, 1 forward solve, 2 backward solve
+// (levels walked top-down).
+// ---------------------------------------------------------------------------
+template <typename T, int MODE>
+__global__ __launch_bounds__(1024) void k_sweep_tasks(FactArgs<T> FA, SolveArgs<T> SA,
+                                                      const int32_t* __restrict__ fronts,
+                                                      const int32_t* __restrict__ lvl_ptr,
+                                                      const int32_t* __restrict__ lvl_off, int task_base,
+                                                      int slice) {
+    extern __shared__ unsigned char smem_raw[];
+    const int task = task_base + blockIdx.x;
+    const int nw = blockDim.x / kWave, w = threadIdx.x / kWave;
+    T* my = reinterpret_cast<T*>(smem_raw) + (size_t)w * slice;
+    const int l0 = lvl_off[task];
+    const int nlev = lvl_off[task + 1] - l0 - 1;
+    for (int li = 0; li < nlev; ++li) {
+        const int L = MODE == 2 ? nlev - 1 - li : li;
+        const int a = lvl_ptr[l0 + L], b = lvl_ptr[l0 + L + 1];
+        for (int q = a + w; q < b; q += nw) {
+            const int s = fronts[q];
+            if (MODE == 0) {
+                factor_front<T, kWave, true>(FA, s, my);
+                const int c = FA.sn_c[s];
+                T* Fo = FA.F + FA.foff[s];
+                for (int idx = threadIdx.x & (kWave - 1); idx < c * c; idx += kWave) Fo[idx] = my[idx];
+            } else if (MODE == 1) {
+                fwd_front<T, true>(SA, s, my);
+            } else {
+                bwd_front<T, true>(SA, s, my);
+            }
+            wave_sync();
+        }
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -948,6 +1014,100 @@ struct PrecBufs {
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// Sweep plan for the task sweeps (host).  Fronts of subtrees with at most
+// `cut` fronts form bottom tasks; the remaining top of the tree is walked level
+// by level (one launch per level) while a level holds more than `wcut` fronts,
+// and the rest (the narrow top) is one single-workgroup task.  A small cost
+// model (per-front latency, per-launch overhead) picks cut and wcut.
+// ---------------------------------------------------------------------------
+struct SweepPlan {
+    std::vector<int32_t> fronts, lvl_ptr, lvl_off, mid;
+    std::vector<int> mid_off, mid_cnt;
+    int n_bottom = 0;
+    bool has_tail = false;
+    int cut = 0, wcut = 0;
+    double cost_us = 0;
+};
+
+constexpr double kFrontUs = 2.0, kLaunchUs = 8.0;
+
+static void append_task(SweepPlan& P, const std::vector<std::vector<int32_t>>& levels) {
+    P.lvl_off.push_back((int32_t)P.lvl_ptr.size());
+    for (const auto& L : levels) {
+        P.lvl_ptr.push_back((int32_t)P.fronts.size());
+        P.fronts.insert(P.fronts.end(), L.begin(), L.end());
+    }
+    P.lvl_ptr.push_back((int32_t)P.fronts.size());
+}
+
+static SweepPlan make_plan(const Symbolic& S, int cut, int wcut, int bottom_waves, int tail_waves, bool build) {
+    const int nsn = S.nsn;
+    SweepPlan P;
+    P.cut = cut;
+    P.wcut = wcut;
+    std::vector<int32_t> size(nsn, 1);
+    for (int s = 0; s < nsn; ++s)
+        if (S.sn_parent[s] >= 0) size[S.sn_parent[s]] += size[s];
+    auto inside = [&](int s) { return size[s] <= cut; };
+    // bottom tasks: maximal subtrees with <= cut fronts, fronts by level
+    std::vector<int32_t> root_of(nsn, -1);
+    std::vector<int32_t> roots;
+    for (int s = nsn - 1; s >= 0; --s) {  // parents have larger indices
+        if (!inside(s)) continue;
+        int p = S.sn_parent[s];
+        if (p < 0 || !inside(p)) {
+            root_of[s] = (int32_t)roots.size();
+            roots.push_back(s);
+        } else {
+            root_of[s] = root_of[p];
+        }
+    }
+    std::vector<std::vector<std::vector<int32_t>>> tl(roots.size());
+    for (size_t t = 0; t < roots.size(); ++t) tl[t].resize(S.sn_level[roots[t]] + 1);
+    for (int s = 0; s < nsn; ++s)
+        if (root_of[s] >= 0) tl[root_of[s]][S.sn_level[s]].push_back(s);
+    double costA = 0;
+    for (size_t t = 0; t < roots.size(); ++t) {
+        double c = 0;
+        for (const auto& L : tl[t]) c += std::ceil((double)L.size() / bottom_waves) * kFrontUs;
+        costA = std::max(costA, c);
+    }
+    // top tree levels
+    std::vector<int32_t> htop(nsn, 0);
+    int H = 0;
+    for (int s = 0; s < nsn; ++s) {
+        if (inside(s)) continue;
+        H = std::max(H, htop[s] + 1);
+        int p = S.sn_parent[s];
+        if (p >= 0) htop[p] = std::max(htop[p], htop[s] + 1);
+    }
+    std::vector<std::vector<int32_t>> top(H);
+    for (int s = 0; s < nsn; ++s)
+        if (!inside(s)) top[htop[s]].push_back(s);
+    int Lstar = H;
+    while (Lstar > 0 && (int)top[Lstar - 1].size() <= wcut) --Lstar;
+    double costB = Lstar * (kLaunchUs + kFrontUs);
+    double costC = 0;
+    for (int l = Lstar; l < H; ++l) costC += std::ceil((double)top[l].size() / tail_waves) * kFrontUs;
+    P.cost_us = (roots.empty() ? 0 : costA + kLaunchUs) + costB + (Lstar < H ? costC + kLaunchUs : 0);
+    if (!build) return P;
+    for (size_t t = 0; t < roots.size(); ++t) append_task(P, tl[t]);
+    P.n_bottom = (int)roots.size();
+    for (int l = 0; l < Lstar; ++l) {
+        P.mid_off.push_back((int)P.mid.size());
+        P.mid_cnt.push_back((int)top[l].size());
+        P.mid.insert(P.mid.end(), top[l].begin(), top[l].end());
+    }
+    if (Lstar < H) {
+        std::vector<std::vector<int32_t>> tail(top.begin() + Lstar, top.end());
+        append_task(P, tail);
+        P.has_tail = true;
+    }
+    P.lvl_off.push_back((int32_t)P.lvl_ptr.size());
+    return P;
+}
+
+// ---------------------------------------------------------------------------
 // Instance
 // ---------------------------------------------------------------------------
 struct Instance::Impl {
@@ -989,6 +1149,16 @@ struct Instance::Impl {
     std::vector<int> lv_off[3], lv_cnt[3];
     int n_levels = 0;
     int level_launches = 0;  // non-empty (level, class) launches of one factor / solve sweep
+    // task sweeps: bottom subtrees per workgroup, wide top levels per launch,
+    // narrow top tail in one workgroup
+    bool tasks_ok = false;
+    bool use_tasks = true;  // option: task sweeps when the plan exists
+    int n_bottom = 0, tail_waves = 0, bottom_waves = 4;
+    bool has_tail = false;
+    DBuf<int32_t> t_fronts, t_lvl_ptr, t_lvl_off, t_mid;
+    std::vector<int> mid_off, mid_cnt;
+    int plan_cut = 0, plan_wcut = 0;
+    double plan_cost_us = 0, level_cost_us = 0;
     // dependency-driven sweeps (all fronts fit in LDS)
     bool use_flow = false;
     bool flow_ok = false;       // every front fits the LDS front of the dataflow kernels
@@ -1041,6 +1211,7 @@ Instance::~Instance() {
     rel(d_->sn_k); rel(d_->sn_c); rel(d_->child_ptr); rel(d_->child); rel(d_->rmap); rel(d_->rows);
     rel(d_->perm); rel(d_->pth); rel(d_->pv); rel(d_->lists); rel(d_->foff); rel(d_->rmoff); rel(d_->uoff);
     rel(d_->rows_off); rel(d_->st); rel(d_->done); rel(d_->order_fwd); rel(d_->order_bwd); rel(d_->sn_parent);
+    rel(d_->t_fronts); rel(d_->t_lvl_ptr); rel(d_->t_lvl_off); rel(d_->t_mid);
     d_->f32.release(b);
     d_->f64.release(b);
     for (auto& sb : d_->staged) {
@@ -1192,8 +1363,45 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
     D.level_launches = 0;
     for (int l = 0; l < S.n_levels; ++l)
         for (int cls = 0; cls < 3; ++cls) D.level_launches += D.lv_cnt[cls][l] ? 1 : 0;
-    // dependency-driven sweeps: level order lists children before parents
+    // task sweeps: pick the cheapest plan, compare with plain level launches
     D.max_c = S.max_c;
+    D.tasks_ok = false;
+    D.level_cost_us = S.n_levels * (kLaunchUs + kFrontUs);
+    if (S.nsn > 0 && S.max_c <= kFrontA) {
+        const size_t slice_bytes = (size_t)S.max_c * S.max_c * sizeof(double);
+        D.bottom_waves = 4;
+        D.tail_waves = (int)std::max<size_t>(1, std::min<size_t>(16, 65536 / std::max<size_t>(1, slice_bytes)));
+        int best_cut = 0, best_w = 0;
+        double best = D.level_cost_us;
+        for (int cut : {8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096})
+            for (int wc : {2, 4, 8, 16, 32, 64, 128}) {
+                SweepPlan P = make_plan(S, cut, wc, D.bottom_waves, D.tail_waves, false);
+                if (P.cost_us < best) {
+                    best = P.cost_us;
+                    best_cut = cut;
+                    best_w = wc;
+                }
+            }
+        if (best_cut) {
+            SweepPlan P = make_plan(S, best_cut, best_w, D.bottom_waves, D.tail_waves, true);
+            D.t_fronts.upload(P.fronts, bytes_, s);
+            D.t_lvl_ptr.upload(P.lvl_ptr, bytes_, s);
+            D.t_lvl_off.upload(P.lvl_off, bytes_, s);
+            std::vector<int32_t> mid = P.mid;
+            if (mid.empty()) mid.push_back(0);
+            D.t_mid.upload(mid, bytes_, s);
+            HIPCHECK(hipStreamSynchronize(s));
+            D.mid_off = P.mid_off;
+            D.mid_cnt = P.mid_cnt;
+            D.n_bottom = P.n_bottom;
+            D.has_tail = P.has_tail;
+            D.plan_cut = best_cut;
+            D.plan_wcut = best_w;
+            D.plan_cost_us = best;
+            D.tasks_ok = true;
+        }
+    }
+    // dependency-driven sweeps: level order lists children before parents
     D.flow_ok = S.nsn > 0 && S.max_c <= kFrontB;
     D.use_flow = D.flow_ok && !D.force_levels;
     if (S.nsn > 0) {
@@ -1375,6 +1583,19 @@ void launch_factor(Instance::Impl& D, T piv_tol) {
     A.F = B.F.p;
     A.piv_tol = piv_tol;
     A.st = D.st.p;
+    if (!D.use_flow && D.use_tasks && D.tasks_ok) {
+        SolveArgs<T> none{};
+        const int slice = D.max_c * D.max_c;
+        if (D.n_bottom)
+            k_sweep_tasks<T, 0><<<D.n_bottom, kWave * D.bottom_waves, D.bottom_waves * slice * sizeof(T), D.stream>>>(
+                A, none, D.t_fronts.p, D.t_lvl_ptr.p, D.t_lvl_off.p, 0, slice);
+        for (size_t l = 0; l < D.mid_off.size(); ++l)
+            k_factor_lds<T, kFrontA><<<D.mid_cnt[l], kWave, 0, D.stream>>>(A, D.t_mid.p + D.mid_off[l]);
+        if (D.has_tail)
+            k_sweep_tasks<T, 0><<<1, kWave * D.tail_waves, D.tail_waves * slice * sizeof(T), D.stream>>>(
+                A, none, D.t_fronts.p, D.t_lvl_ptr.p, D.t_lvl_off.p, D.n_bottom, slice);
+        return;
+    }
     if (D.use_flow) {
         const uint32_t ep = ++D.epoch;
         k_factor_flow<T><<<D.sn_first.n, kWave, D.max_c * D.max_c * sizeof(T), D.stream>>>(A, D.order_fwd.p, D.done.p,
@@ -1416,6 +1637,32 @@ void prof_end(Prof* pf, int ph, int launches);
 template <typename T>
 void launch_trisolve(Instance::Impl& D, Prof* pf = nullptr) {
     SolveArgs<T> A = solve_args<T>(D);
+    if (!D.use_flow && D.use_tasks && D.tasks_ok) {
+        FactArgs<T> none{};
+        const int slice = D.max_c;
+        const int nmid = (int)D.mid_off.size();
+        prof_begin(pf, PH_FWD);
+        if (D.n_bottom)
+            k_sweep_tasks<T, 1><<<D.n_bottom, kWave * D.bottom_waves, D.bottom_waves * slice * sizeof(T), D.stream>>>(
+                none, A, D.t_fronts.p, D.t_lvl_ptr.p, D.t_lvl_off.p, 0, slice);
+        for (int l = 0; l < nmid; ++l)
+            k_fwd<T, kFrontA><<<D.mid_cnt[l], kWave, 0, D.stream>>>(A, D.t_mid.p + D.mid_off[l]);
+        if (D.has_tail)
+            k_sweep_tasks<T, 1><<<1, kWave * D.tail_waves, D.tail_waves * slice * sizeof(T), D.stream>>>(
+                none, A, D.t_fronts.p, D.t_lvl_ptr.p, D.t_lvl_off.p, D.n_bottom, slice);
+        prof_end(pf, PH_FWD, (D.n_bottom ? 1 : 0) + nmid + (D.has_tail ? 1 : 0));
+        prof_begin(pf, PH_BWD);
+        if (D.has_tail)
+            k_sweep_tasks<T, 2><<<1, kWave * D.tail_waves, D.tail_waves * slice * sizeof(T), D.stream>>>(
+                none, A, D.t_fronts.p, D.t_lvl_ptr.p, D.t_lvl_off.p, D.n_bottom, slice);
+        for (int l = nmid - 1; l >= 0; --l)
+            k_bwd<T, kFrontA><<<D.mid_cnt[l], kWave, 0, D.stream>>>(A, D.t_mid.p + D.mid_off[l]);
+        if (D.n_bottom)
+            k_sweep_tasks<T, 2><<<D.n_bottom, kWave * D.bottom_waves, D.bottom_waves * slice * sizeof(T), D.stream>>>(
+                none, A, D.t_fronts.p, D.t_lvl_ptr.p, D.t_lvl_off.p, 0, slice);
+        prof_end(pf, PH_BWD, (D.n_bottom ? 1 : 0) + nmid + (D.has_tail ? 1 : 0));
+        return;
+    }
     if (D.use_flow) {
         const int nsn = (int)D.sn_first.n;
         const size_t sm = D.max_c * sizeof(T);
@@ -1498,6 +1745,12 @@ void launch_pcg(Instance::Impl& D, int steps) {
     }
 }
 
+int factor_launches(const Instance::Impl& D) {
+    if (D.use_flow) return 1;
+    if (D.use_tasks && D.tasks_ok) return (D.n_bottom ? 1 : 0) + (int)D.mid_off.size() + (D.has_tail ? 1 : 0);
+    return D.level_launches;
+}
+
 void read_status(Instance::Impl& D) {
     HIPCHECK(hipMemcpyAsync(D.hst, D.st.p, sizeof(DevStatus), hipMemcpyDeviceToHost, D.stream));
     HIPCHECK(hipStreamSynchronize(D.stream));
@@ -1551,7 +1804,7 @@ void iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p, bool 
         pf.end(PH_SPGEMM, 1);
         pf.begin(PH_FACTOR);
         launch_factor<T>(D, piv_tol);
-        pf.end(PH_FACTOR, D.level_launches);
+        pf.end(PH_FACTOR, factor_launches(D));
     }
     if (p.pcg_steps > 0) {
         pf.begin(PH_PCG);
@@ -1567,7 +1820,7 @@ void iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p, bool 
     pf.end(PH_RHS, 1);
     launch_trisolve<T>(D, &pf);
     pf.begin(PH_UPDATE);
-    launch_update<T>(D, p.vmin, p.vmax, bufs<T>(D).x.p, 1);
+    launch_update<T>(D, p.vmin, p.vmax, bufs<T>(D).x.p, sizeof(T) == sizeof(double) ? 1 : 0);
     pf.end(PH_UPDATE, 2);
 }
 
@@ -1591,11 +1844,12 @@ SolveOutcome Instance::solve(const SolveParams& p) {
             retry = false;
         }
         bool converged = false, diverged = false, singular = false, timed_out = false;
+        int stalls = 0;
         out.iterations = 0;
+        out.max_mismatch = 0;
         for (int it = 0; it < p.max_iter; ++it) {
             const bool fp64 = precision_ == 1 || fp64_factor_active_;
             const bool refactor = p.robust || !factor_valid_ || p.mode == 1;
-            const bool pcg = !fp64 && p.pcg_steps > 0;
             if (fp64) {
                 if (!D.f64.ready) D.f64.alloc(S, bytes_);
                 SolveParams pd = p;
@@ -1614,19 +1868,13 @@ SolveOutcome Instance::solve(const SolveParams& p) {
                 --it;
                 continue;
             }
-            if (D.hst->piv && pcg) {
-                // FP32 pivots below threshold were substituted: the factor is a
-                // (weaker) preconditioner, CG keeps the Newton step exact
+            if (D.hst->piv && !fp64) {
+                // FP32 pivots below threshold were substituted (threshold * G_jj):
+                // the factor stays SPD; the FP64 residual / RHS of the next
+                // iterations (or CG) correct the approximate Newton step
                 out.pivots_substituted++;
             } else if (D.hst->piv) {
                 factor_valid_ = false;
-                if (!fp64) {
-                    // FP32 pivot below threshold: refactor this iteration in FP64
-                    fp64_factor_active_ = true;
-                    out.fp64_refactors++;
-                    --it;
-                    continue;
-                }
                 singular = true;
                 break;
             }
@@ -1635,7 +1883,23 @@ SolveOutcome Instance::solve(const SolveParams& p) {
                 out.refactors++;
             }
             out.iterations = it + 1;
+            const double prev = out.max_mismatch;
             out.max_mismatch = D.hst->maxdx;
+            if (!fp64 && it > 0 && (D.hst->nan || D.hst->maxdx > 0.9 * prev)) {
+                // the FP32 factor stopped contracting: continue with an FP64 factor
+                if (++stalls >= 2 || D.hst->nan) {
+                    fp64_factor_active_ = true;
+                    factor_valid_ = false;
+                    out.fp64_refactors++;
+                    stalls = 0;
+                    if (D.hst->nan) {
+                        diverged = true;
+                        break;
+                    }
+                }
+            } else {
+                stalls = 0;
+            }
             if (D.hst->nan) {
                 diverged = true;
                 break;
@@ -1852,6 +2116,19 @@ void Instance::set_level_schedule(bool on) {
 }
 
 bool Instance::dataflow_sweeps() const { return d_->use_flow; }
+
+void Instance::set_task_sweeps(bool on) { d_->use_tasks = on; }
+
+std::string Instance::sweep_plan() const {
+    const Impl& D = *d_;
+    char buf[256];
+    if (D.use_flow) return "dependency-driven";
+    if (!(D.use_tasks && D.tasks_ok))
+        return "levels(" + std::to_string(D.n_levels) + ")";
+    snprintf(buf, sizeof(buf), "tasks(cut=%d bottom=%d mid=%d tail=%d waves=%d est=%.0fus vs levels=%.0fus)", D.plan_cut,
+             D.n_bottom, (int)D.mid_off.size(), D.has_tail ? 1 : 0, D.tail_waves, D.plan_cost_us, D.level_cost_us);
+    return buf;
+}
 
 int Instance::staged_count(int slot) const {
     if (slot < 0 || slot >= (int)d_->staged.size()) return -1;

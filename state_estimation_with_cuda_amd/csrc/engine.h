@@ -81,8 +81,9 @@ struct ResultOut {
 
 // Engine options beyond SLE_Config (sle_ext_SetOption).
 struct EngineOptions {
-    int pcg_steps = 3;         // mixed form: CG steps refining each Newton step (0 = plain FP32 solve)
+    int pcg_steps = 0;         // mixed form: CG steps refining each Newton step (0 = FP32 factor solve, GN refines)
     bool level_schedule = true;   // level-by-level sweeps (0: single dependency-driven launch per sweep)
+    bool task_sweeps = true;      // group bottom subtrees / the narrow top into single-workgroup tasks
 };
 
 class Engine {

@@ -120,6 +120,9 @@ class Instance {
     // dependency-driven sweeps (default when every front fits in LDS) or level-by-level launches
     void set_level_schedule(bool on);
     bool dataflow_sweeps() const;
+    // task sweeps (bottom subtrees per workgroup, narrow top in one workgroup)
+    void set_task_sweeps(bool on);
+    std::string sweep_plan() const;
 
     // Device-resident telemetry batches: stage (host -> HBM) ahead of time,
     // apply (scatter into z on the device) right before a solve.

@@ -862,6 +862,20 @@ SLE_API SLE_StatusCode SLE_CALL sle_ext_Solve(SLE_Handle handle, const SLE_Solve
     });
 }
 
+SLE_API const char* SLE_CALL sle_ext_Describe(SLE_Handle handle) {
+    if (!handle) return nullptr;
+    try {
+        sle::Engine& e = E(handle);
+        std::string d = e.config().use_double_precision ? "plain FP64" : "mixed FP32 factor + FP64 refinement";
+        d += "; pcg_steps=" + std::to_string(e.config().use_double_precision ? 0 : e.options().pcg_steps);
+        if (e.instance()) d += "; sweeps=" + e.instance()->sweep_plan();
+        g_str_cache = d;
+    } catch (...) {
+        g_str_cache = "unavailable";
+    }
+    return g_str_cache.c_str();
+}
+
 SLE_API SLE_StatusCode SLE_CALL sle_ext_SetOption(SLE_Handle handle, const char* key, double value) {
     if (!handle) return fail(SLE_ERROR_INVALID_HANDLE, "NULL handle");
     return guarded([&]() -> SLE_StatusCode { return from_int(E(handle).set_option(key, value), "set option"); });

@@ -143,6 +143,7 @@ EXT_FUNCTIONS = [
     "sle_ext_GetHPattern", "sle_ext_GetGPattern", "sle_ext_GetOrdering", "sle_ext_Evaluate",
     "sle_ext_JacobianGain", "sle_ext_Factor", "sle_ext_SolveFactor", "sle_ext_StageTelemetry",
     "sle_ext_ApplyStagedTelemetry", "sle_ext_GetKernelTimes", "sle_ext_ResetKernelTimes", "sle_ext_SetOption",
+    "sle_ext_Describe",
 ]
 KERNEL_CLASSES = ["power", "meas", "spgemm", "factor", "rhs", "fwd", "bwd", "update", "pcg"]
 
@@ -267,6 +268,7 @@ class Library:
             "sle_ext_GetKernelTimes": (C.c_int, [H, dp, lp, lp, i32]),
             "sle_ext_ResetKernelTimes": (C.c_int, [H]),
             "sle_ext_SetOption": (C.c_int, [H, C.c_char_p, C.c_double]),
+            "sle_ext_Describe": (C.c_char_p, [H]),
         }
         missing = [n for n in API_FUNCTIONS + EXT_FUNCTIONS if n not in sig]
         assert not missing, missing
@@ -508,6 +510,10 @@ class Engine:
         ptrs = [out[k].ctypes.data for k in ("P", "Q", "Pf", "Qf", "Pt", "Qt", "h", "r")]
         self._check(self.L.sle_ext_GetResults(self.h, *ptrs), "sle_ext_GetResults")
         return out
+
+    def describe(self) -> str:
+        d = self.L.sle_ext_Describe(self.h)
+        return d.decode() if d else ""
 
     def set_option(self, key, value):
         self._check(self.L.sle_ext_SetOption(self.h, key.encode(), float(value)), "sle_ext_SetOption")
