@@ -82,7 +82,7 @@ def algorithmic_bytes(sz, g, prec_bytes):
     }
 
 
-def run_form(g, fp64, reqs, batches, warmup, steps, profile_steps):
+def run_form(g, fp64, reqs, batches, warmup, steps, profile_steps, pcg_steps=0):
     from state_estimation_with_cuda_amd import sle
 
     name = "fp64" if fp64 else "mixed"
@@ -90,6 +90,7 @@ def run_form(g, fp64, reqs, batches, warmup, steps, profile_steps):
     e = sle.Engine(double=True)
     e.load_grid(g)
     e.set_config(use_double_precision=int(fp64), enable_profiling=0)
+    e.set_option("pcg_steps", 0 if fp64 else pcg_steps)
     e.upload()
     e.set_telemetry_f64(g.z)
     setup_s = time.perf_counter() - t0
@@ -168,6 +169,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=2, help="incremental updates timed on the CPU oracle")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true", help="time the mixed form only")
+    ap.add_argument("--pcg-steps", type=int, default=2, help="mixed form: CG steps per Newton step")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs (--gpus > 1) are not supported", file=sys.stderr)
@@ -183,7 +185,7 @@ def main():
     log(f"grid: n={g.n} nb={g.nb} m={g.m} generated in {time.perf_counter() - t0:.1f}s")
     reqs, batches = make_updates(g, args.warmup + max(args.steps, args.profile_steps), 0.10, seed=4)
 
-    mixed = run_form(g, False, reqs, batches, args.warmup, args.steps, args.profile_steps)
+    mixed = run_form(g, False, reqs, batches, args.warmup, args.steps, args.profile_steps, args.pcg_steps)
     fp64 = None if args.skip_fp64 else run_form(g, True, reqs, batches, args.warmup, args.steps, 0)
 
     # roofline of the dominant kernel class (HIP-event timed over the profiled pass)

@@ -41,6 +41,8 @@ typedef struct SLE_ExtResult {
  *   "task_sweeps"     1 (default): walk whole bottom subtrees and the narrow
  *                     top of the supernodal tree inside single workgroups
  *                     (fewer launches); 0: one launch per level
+ *   "lds_solve"       1 (default): triangular solves walk subtrees / top
+ *                     chunks with the L panels resident in LDS
  *   "level_schedule"  1 (default): factor and solve level by level (kernel
  *                     boundaries order the fronts); 0: one dependency-driven
  *                     launch per sweep with release/acquire flags per front */

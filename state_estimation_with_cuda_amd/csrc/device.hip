@@ -699,6 +699,187 @@ __global__ __launch_bounds__(1024) void k_sweep_tasks(FactArgs<T> FA, SolveArgs<
 }
 
 // ---------------------------------------------------------------------------
+// LDS-resident task triangular solves.  A task (a bottom subtree, or a chunk of
+// consecutive levels of the narrow top of the tree) is solved by one
+// workgroup: it first loads the L panels of all its fronts, their metadata,
+// relative maps and child lists into LDS with coalesced loads, then walks its
+// levels with LDS-only traffic (one wave per front, __syncthreads per level),
+// and finally writes back the solution of its columns and, for fronts whose
+// parent lies outside the task, the update vector.  Per level this costs LDS
+// latency instead of a chain of dependent HBM/L2 loads.
+// ---------------------------------------------------------------------------
+constexpr int kTH = 12;  // task header ints
+enum { TH_NF = 0, TH_NLEV, TH_META, TH_LVL, TH_CH, TH_NCH, TH_RM, TH_NRM, TH_SRC, TH_NSRC, TH_SUML, TH_SUMF };
+constexpr int kFM = 11;  // per-front ints
+enum { FM_S = 0, FM_C, FM_K, FM_FIRST, FM_L, FM_F, FM_NCH, FM_CH, FM_RM, FM_EXTPAR, FM_SRC };
+
+struct LdsTaskArgs {
+    const int32_t* th;
+    const int32_t* meta;
+    const int32_t* lvl;
+    const int32_t* ch;
+    const int32_t* rm;
+    const int32_t* src;
+};
+
+template <typename T>
+struct LdsView {
+    T* Lp;
+    T* fv;
+    int* mt;
+    int* lv;
+    int* chl;
+    int* rml;
+    int* srl;
+    int nf, nlev;
+};
+
+template <typename T>
+__device__ inline LdsView<T> lds_task_load(const LdsTaskArgs& A, const int32_t* H, unsigned char* smem, bool with_src) {
+    LdsView<T> v;
+    v.nf = H[TH_NF];
+    v.nlev = H[TH_NLEV];
+    v.Lp = reinterpret_cast<T*>(smem);
+    v.fv = v.Lp + H[TH_SUML];
+    v.mt = reinterpret_cast<int*>(v.fv + H[TH_SUMF]);
+    v.lv = v.mt + v.nf * kFM;
+    v.chl = v.lv + v.nlev + 1;
+    v.rml = v.chl + H[TH_NCH];
+    v.srl = v.rml + H[TH_NRM];
+    const int nt = blockDim.x;
+    for (int i = threadIdx.x; i < v.nf * kFM; i += nt) v.mt[i] = A.meta[H[TH_META] + i];
+    for (int i = threadIdx.x; i <= v.nlev; i += nt) v.lv[i] = A.lvl[H[TH_LVL] + i];
+    for (int i = threadIdx.x; i < H[TH_NCH]; i += nt) v.chl[i] = A.ch[H[TH_CH] + i];
+    for (int i = threadIdx.x; i < H[TH_NRM]; i += nt) v.rml[i] = A.rm[H[TH_RM] + i];
+    if (with_src)
+        for (int i = threadIdx.x; i < H[TH_NSRC]; i += nt) v.srl[i] = A.src[H[TH_SRC] + i];
+    return v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_fwd_lds(LdsTaskArgs A, SolveArgs<T> SA, int task_base) {
+    extern __shared__ unsigned char smem_raw[];
+    const int32_t* H = A.th + (size_t)(task_base + blockIdx.x) * kTH;
+    LdsView<T> v = lds_task_load<T>(A, H, smem_raw, false);
+    const int nw = blockDim.x / kWave, w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    __syncthreads();
+    // L panels (first k columns of each front, contiguous) and right-hand sides
+    for (int lf = w; lf < v.nf; lf += nw) {
+        const int* m = v.mt + lf * kFM;
+        const int s = m[FM_S], c = m[FM_C], k = m[FM_K], first = m[FM_FIRST];
+        const T* Lg = SA.F + SA.foff[s];
+        for (int e = lane; e < c * k; e += kWave) v.Lp[m[FM_L] + e] = Lg[e];
+        for (int i = lane; i < c; i += kWave) v.fv[m[FM_F] + i] = i < k ? SA.x[first + i] : T(0);
+    }
+    __syncthreads();
+    // update vectors of children outside the task
+    for (int lf = w; lf < v.nf; lf += nw) {
+        const int* m = v.mt + lf * kFM;
+        for (int q = m[FM_CH]; q < m[FM_CH] + m[FM_NCH]; ++q) {
+            const int cv = v.chl[q];
+            if (cv >= 0) continue;
+            const int ch = -cv - 1;
+            const int u = SA.sn_c[ch] - SA.sn_k[ch];
+            const T* uc = SA.ubuf + SA.uoff[ch];
+            const int32_t* rmg = SA.rmap + SA.rmoff[ch];
+            for (int a = lane; a < u; a += kWave) v.fv[m[FM_F] + rmg[a]] += uc[a];
+            wave_sync();
+        }
+    }
+    __syncthreads();
+    for (int L = 0; L < v.nlev; ++L) {
+        for (int lf = v.lv[L] + w; lf < v.lv[L + 1]; lf += nw) {
+            const int* m = v.mt + lf * kFM;
+            const int c = m[FM_C], k = m[FM_K];
+            T* f = v.fv + m[FM_F];
+            const T* Lp = v.Lp + m[FM_L];
+            for (int q = m[FM_CH]; q < m[FM_CH] + m[FM_NCH]; ++q) {
+                const int lc = v.chl[q];
+                if (lc < 0) continue;
+                const int* mc = v.mt + lc * kFM;
+                const int u = mc[FM_C] - mc[FM_K];
+                const T* uc = v.fv + mc[FM_F] + mc[FM_K];
+                const int* rmc = v.rml + mc[FM_RM];
+                for (int a = lane; a < u; a += kWave) f[rmc[a]] += uc[a];
+                wave_sync();
+            }
+            for (int j = 0; j < k; ++j) {
+                const T yj = f[j] / Lp[j * c + j];
+                wave_sync();
+                if (lane == 0) f[j] = yj;
+                for (int i = j + 1 + lane; i < c; i += kWave) f[i] -= Lp[j * c + i] * yj;
+                wave_sync();
+            }
+        }
+        __syncthreads();
+    }
+    for (int lf = w; lf < v.nf; lf += nw) {
+        const int* m = v.mt + lf * kFM;
+        const int s = m[FM_S], c = m[FM_C], k = m[FM_K], first = m[FM_FIRST];
+        const T* f = v.fv + m[FM_F];
+        for (int j = lane; j < k; j += kWave) SA.x[first + j] = f[j];
+        if (m[FM_EXTPAR]) {
+            T* us = SA.ubuf + SA.uoff[s];
+            for (int a = lane; a < c - k; a += kWave) us[a] = f[k + a];
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_bwd_lds(LdsTaskArgs A, SolveArgs<T> SA, int task_base) {
+    extern __shared__ unsigned char smem_raw[];
+    const int32_t* H = A.th + (size_t)(task_base + blockIdx.x) * kTH;
+    LdsView<T> v = lds_task_load<T>(A, H, smem_raw, true);
+    const int nw = blockDim.x / kWave, w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    __syncthreads();
+    for (int lf = w; lf < v.nf; lf += nw) {
+        const int* m = v.mt + lf * kFM;
+        const int s = m[FM_S], c = m[FM_C], k = m[FM_K], first = m[FM_FIRST];
+        const T* Lg = SA.F + SA.foff[s];
+        for (int e = lane; e < c * k; e += kWave) v.Lp[m[FM_L] + e] = Lg[e];
+        T* xv = v.fv + m[FM_F];
+        const int* sr = v.srl + m[FM_SRC];
+        for (int i = lane; i < c; i += kWave) {
+            if (i < k) {
+                xv[i] = SA.x[first + i];
+            } else {
+                const int code = sr[i - k];
+                if (code < 0) xv[i] = SA.x[-code - 1];  // ancestor outside the task: final
+            }
+        }
+    }
+    __syncthreads();
+    for (int L = v.nlev - 1; L >= 0; --L) {
+        for (int lf = v.lv[L] + w; lf < v.lv[L + 1]; lf += nw) {
+            const int* m = v.mt + lf * kFM;
+            const int c = m[FM_C], k = m[FM_K];
+            T* xv = v.fv + m[FM_F];
+            const T* Lp = v.Lp + m[FM_L];
+            const int* sr = v.srl + m[FM_SRC];
+            for (int i = k + lane; i < c; i += kWave) {
+                const int code = sr[i - k];
+                if (code >= 0) xv[i] = v.fv[code];  // ancestor inside the task, solved at a higher level
+            }
+            wave_sync();
+            for (int j = k - 1; j >= 0; --j) {
+                double part = 0.0;
+                for (int i = j + 1 + lane; i < c; i += kWave) part += (double)(Lp[j * c + i] * xv[i]);
+                part = wave_sum(part);
+                if (lane == 0) xv[j] = (T)(((double)xv[j] - part) / (double)Lp[j * c + j]);
+                wave_sync();
+            }
+        }
+        __syncthreads();
+    }
+    for (int lf = w; lf < v.nf; lf += nw) {
+        const int* m = v.mt + lf * kFM;
+        const int k = m[FM_K], first = m[FM_FIRST];
+        const T* xv = v.fv + m[FM_F];
+        for (int j = lane; j < k; j += kWave) SA.x[first + j] = xv[j];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Dependency-driven sweeps: one launch covers the whole supernodal tree.
 // Block b handles front order[b]; `order` lists every child before its parent
 // (forward, factor) or every parent before its children (backward).  A front
@@ -1027,6 +1208,9 @@ struct SweepPlan {
     bool has_tail = false;
     int cut = 0, wcut = 0;
     double cost_us = 0;
+    // level lists kept for the LDS-resident solve plan
+    std::vector<std::vector<std::vector<int32_t>>> bottom_levels;
+    std::vector<std::vector<int32_t>> tail_levels;
 };
 
 constexpr double kFrontUs = 2.0, kLaunchUs = 8.0;
@@ -1093,6 +1277,7 @@ static SweepPlan make_plan(const Symbolic& S, int cut, int wcut, int bottom_wave
     if (!build) return P;
     for (size_t t = 0; t < roots.size(); ++t) append_task(P, tl[t]);
     P.n_bottom = (int)roots.size();
+    P.bottom_levels = tl;
     for (int l = 0; l < Lstar; ++l) {
         P.mid_off.push_back((int)P.mid.size());
         P.mid_cnt.push_back((int)top[l].size());
@@ -1102,9 +1287,154 @@ static SweepPlan make_plan(const Symbolic& S, int cut, int wcut, int bottom_wave
         std::vector<std::vector<int32_t>> tail(top.begin() + Lstar, top.end());
         append_task(P, tail);
         P.has_tail = true;
+        P.tail_levels = tail;
     }
     P.lvl_off.push_back((int32_t)P.lvl_ptr.size());
     return P;
+}
+
+
+// LDS-resident solve plan built on a SweepPlan: every bottom subtree is one
+// task; the narrow top is cut into chunks of consecutive levels that fit the
+// LDS budget.  Returns false when a bottom task does not fit.
+struct LdsPlan {
+    std::vector<int32_t> th, meta, lvl, ch, rm, src;
+    int n_bottom = 0;
+    std::vector<int> tail_tasks;
+    std::vector<size_t> tail_bytes;
+    size_t bottom_bytes = 0;
+    std::vector<int> mid_off, mid_cnt;
+    std::vector<int32_t> mid;
+    double cost_us = 0;
+    int cut = 0, wcut = 0;
+};
+
+constexpr double kLdsLevelUs = 0.6, kLdsFixedUs = 6.0, kMidLevelUs = 10.0;
+
+static size_t lds_task_bytes(const Symbolic& S, const std::vector<std::vector<int32_t>>& levels, size_t elem,
+                             int64_t* nfronts = nullptr) {
+    int64_t nf = 0, sumL = 0, sumF = 0, nch = 0, nrm = 0, nsrc = 0;
+    for (const auto& L : levels)
+        for (int s : L) {
+            ++nf;
+            sumL += (int64_t)S.sn_c[s] * S.sn_k[s];
+            sumF += S.sn_c[s];
+            nch += S.sn_child_ptr[s + 1] - S.sn_child_ptr[s];
+            nrm += S.sn_c[s] - S.sn_k[s];
+            nsrc += S.sn_c[s] - S.sn_k[s];
+        }
+    if (nfronts) *nfronts = nf;
+    return (size_t)(sumL + sumF) * elem + 4 * (size_t)(nf * kFM + levels.size() + 1 + nch + nrm + nsrc);
+}
+
+static void add_lds_task(LdsPlan& P, const Symbolic& S, const std::vector<int32_t>& snode_of_col,
+                         std::vector<int32_t>& local, const std::vector<std::vector<int32_t>>& levels) {
+    const int32_t meta_off = (int32_t)P.meta.size(), lvl_off = (int32_t)P.lvl.size(), ch_off = (int32_t)P.ch.size(),
+                  rm_off = (int32_t)P.rm.size(), src_off = (int32_t)P.src.size();
+    std::vector<int32_t> order;
+    for (const auto& L : levels) {
+        P.lvl.push_back((int32_t)order.size());
+        for (int s : L) {
+            local[s] = (int32_t)order.size();
+            order.push_back(s);
+        }
+    }
+    P.lvl.push_back((int32_t)order.size());
+    std::vector<int32_t> Foff(order.size());
+    int32_t sumL = 0, sumF = 0;
+    for (size_t lf = 0; lf < order.size(); ++lf) {
+        Foff[lf] = sumF;
+        sumF += S.sn_c[order[lf]];
+    }
+    for (size_t lf = 0; lf < order.size(); ++lf) {
+        const int s = order[lf], c = S.sn_c[s], k = S.sn_k[s], first = S.sn_first[s];
+        int32_t m[kFM];
+        m[FM_S] = s;
+        m[FM_C] = c;
+        m[FM_K] = k;
+        m[FM_FIRST] = first;
+        m[FM_L] = sumL;
+        sumL += c * k;
+        m[FM_F] = Foff[lf];
+        m[FM_CH] = (int32_t)P.ch.size() - ch_off;
+        m[FM_NCH] = S.sn_child_ptr[s + 1] - S.sn_child_ptr[s];
+        for (int q = S.sn_child_ptr[s]; q < S.sn_child_ptr[s + 1]; ++q) {
+            const int chd = S.sn_child[q];
+            P.ch.push_back(local[chd] >= 0 ? local[chd] : -chd - 1);
+        }
+        const int par = S.sn_parent[s];
+        if (par >= 0 && local[par] >= 0) {
+            m[FM_RM] = (int32_t)P.rm.size() - rm_off;
+            m[FM_EXTPAR] = 0;
+            P.rm.insert(P.rm.end(), S.rmap.begin() + S.sn_rmoff[s], S.rmap.begin() + S.sn_rmoff[s] + (c - k));
+        } else {
+            m[FM_RM] = -1;
+            m[FM_EXTPAR] = c - k > 0 ? 1 : 0;
+        }
+        m[FM_SRC] = (int32_t)P.src.size() - src_off;
+        for (int i = k; i < c; ++i) {
+            const int r = S.sn_rows[S.sn_rows_off[s] + i];
+            const int owner = snode_of_col[r];
+            P.src.push_back(local[owner] >= 0 ? Foff[local[owner]] + (r - S.sn_first[owner]) : -r - 1);
+        }
+        P.meta.insert(P.meta.end(), m, m + kFM);
+    }
+    const int32_t hdr[kTH] = {(int32_t)order.size(), (int32_t)levels.size(), meta_off, lvl_off, ch_off,
+                              (int32_t)P.ch.size() - ch_off, rm_off, (int32_t)P.rm.size() - rm_off, src_off,
+                              (int32_t)P.src.size() - src_off, sumL, sumF};
+    P.th.insert(P.th.end(), hdr, hdr + kTH);
+    for (int s : order) local[s] = -1;
+}
+
+static bool make_lds_plan(const Symbolic& S, const SweepPlan& SP, size_t elem, size_t budget, int nwb, int nwt,
+                          bool build, LdsPlan& P) {
+    P = LdsPlan();
+    P.cut = SP.cut;
+    P.wcut = SP.wcut;
+    double costA = 0;
+    for (const auto& T : SP.bottom_levels) {
+        size_t b = lds_task_bytes(S, T, elem);
+        if (b > budget) return false;
+        P.bottom_bytes = std::max(P.bottom_bytes, b);
+        double c = kLdsFixedUs;
+        for (const auto& L : T) c += std::ceil((double)L.size() / nwb) * kLdsLevelUs;
+        costA = std::max(costA, c);
+    }
+    // tail chunks of consecutive levels within the budget
+    std::vector<std::vector<std::vector<int32_t>>> chunks;
+    std::vector<std::vector<int32_t>> cur;
+    double costC = 0;
+    for (const auto& L : SP.tail_levels) {
+        cur.push_back(L);
+        if (lds_task_bytes(S, cur, elem) > budget) {
+            cur.pop_back();
+            if (cur.empty()) return false;
+            chunks.push_back(cur);
+            cur.assign(1, L);
+            if (lds_task_bytes(S, cur, elem) > budget) return false;
+        }
+    }
+    if (!cur.empty()) chunks.push_back(cur);
+    for (const auto& C : chunks) {
+        costC += kLaunchUs + kLdsFixedUs;
+        for (const auto& L : C) costC += std::ceil((double)L.size() / nwt) * kLdsLevelUs;
+    }
+    P.cost_us = (SP.bottom_levels.empty() ? 0 : kLaunchUs + costA) + SP.mid_off.size() * kMidLevelUs + costC;
+    if (!build) return true;
+    std::vector<int32_t> snode_of_col(S.ns), local(S.nsn, -1);
+    for (int sn = 0; sn < S.nsn; ++sn)
+        for (int j = 0; j < S.sn_k[sn]; ++j) snode_of_col[S.sn_first[sn] + j] = sn;
+    for (const auto& T : SP.bottom_levels) add_lds_task(P, S, snode_of_col, local, T);
+    P.n_bottom = (int)SP.bottom_levels.size();
+    for (const auto& C : chunks) {
+        P.tail_tasks.push_back((int)P.th.size() / kTH);
+        P.tail_bytes.push_back(lds_task_bytes(S, C, elem));
+        add_lds_task(P, S, snode_of_col, local, C);
+    }
+    P.mid = SP.mid;
+    P.mid_off = SP.mid_off;
+    P.mid_cnt = SP.mid_cnt;
+    return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -1159,6 +1489,19 @@ struct Instance::Impl {
     std::vector<int> mid_off, mid_cnt;
     int plan_cut = 0, plan_wcut = 0;
     double plan_cost_us = 0, level_cost_us = 0;
+    // LDS-resident solve plan (per factor precision: element size differs)
+    struct LdsDev {
+        bool ok = false;
+        DBuf<int32_t> th, meta, lvl, ch, rm, src, mid;
+        int n_bottom = 0;
+        std::vector<int> tail_tasks, mid_off, mid_cnt;
+        std::vector<size_t> tail_bytes;
+        size_t bottom_bytes = 0;
+        double cost_us = 0;
+        int cut = 0, wcut = 0;
+    } lds[2];  // [0] float, [1] double
+    bool use_lds = true;
+    int lds_waves_bottom = 4, lds_waves_tail = 16;
     // dependency-driven sweeps (all fronts fit in LDS)
     bool use_flow = false;
     bool flow_ok = false;       // every front fits the LDS front of the dataflow kernels
@@ -1212,6 +1555,9 @@ Instance::~Instance() {
     rel(d_->perm); rel(d_->pth); rel(d_->pv); rel(d_->lists); rel(d_->foff); rel(d_->rmoff); rel(d_->uoff);
     rel(d_->rows_off); rel(d_->st); rel(d_->done); rel(d_->order_fwd); rel(d_->order_bwd); rel(d_->sn_parent);
     rel(d_->t_fronts); rel(d_->t_lvl_ptr); rel(d_->t_lvl_off); rel(d_->t_mid);
+    for (auto& L : d_->lds) {
+        rel(L.th); rel(L.meta); rel(L.lvl); rel(L.ch); rel(L.rm); rel(L.src); rel(L.mid);
+    }
     d_->f32.release(b);
     d_->f64.release(b);
     for (auto& sb : d_->staged) {
@@ -1400,6 +1746,53 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
             D.plan_cost_us = best;
             D.tasks_ok = true;
         }
+    }
+    // LDS-resident solve plans (float and double elements)
+    for (int e = 0; e < 2; ++e) {
+        Impl::LdsDev& L = D.lds[e];
+        L.ok = false;
+        if (S.nsn == 0 || S.max_c > kFrontB) continue;
+        const size_t elem = e ? sizeof(double) : sizeof(float);
+        const size_t budget = 64 * 1024;
+        double best = D.level_cost_us;
+        int bc = 0, bw = 0;
+        for (int cut : {32, 64, 128, 256, 512, 1024})
+            for (int wc : {8, 16, 32, 64}) {
+                SweepPlan SP = make_plan(S, cut, wc, D.lds_waves_bottom, D.lds_waves_tail, true);
+                LdsPlan LP;
+                if (make_lds_plan(S, SP, elem, budget, D.lds_waves_bottom, D.lds_waves_tail, false, LP) &&
+                    LP.cost_us < best) {
+                    best = LP.cost_us;
+                    bc = cut;
+                    bw = wc;
+                }
+            }
+        if (!bc) continue;
+        SweepPlan SP = make_plan(S, bc, bw, D.lds_waves_bottom, D.lds_waves_tail, true);
+        LdsPlan LP;
+        make_lds_plan(S, SP, elem, budget, D.lds_waves_bottom, D.lds_waves_tail, true, LP);
+        auto up = [&](DBuf<int32_t>& b, std::vector<int32_t> v) {
+            if (v.empty()) v.push_back(0);
+            b.upload(v, bytes_, s);
+        };
+        up(L.th, LP.th);
+        up(L.meta, LP.meta);
+        up(L.lvl, LP.lvl);
+        up(L.ch, LP.ch);
+        up(L.rm, LP.rm);
+        up(L.src, LP.src);
+        up(L.mid, LP.mid);
+        HIPCHECK(hipStreamSynchronize(s));
+        L.n_bottom = LP.n_bottom;
+        L.tail_tasks = LP.tail_tasks;
+        L.tail_bytes = LP.tail_bytes;
+        L.bottom_bytes = LP.bottom_bytes;
+        L.mid_off = LP.mid_off;
+        L.mid_cnt = LP.mid_cnt;
+        L.cost_us = best;
+        L.cut = bc;
+        L.wcut = bw;
+        L.ok = true;
     }
     // dependency-driven sweeps: level order lists children before parents
     D.flow_ok = S.nsn > 0 && S.max_c <= kFrontB;
@@ -1637,6 +2030,29 @@ void prof_end(Prof* pf, int ph, int launches);
 template <typename T>
 void launch_trisolve(Instance::Impl& D, Prof* pf = nullptr) {
     SolveArgs<T> A = solve_args<T>(D);
+    Instance::Impl::LdsDev& LD = D.lds[sizeof(T) == sizeof(double) ? 1 : 0];
+    if (!D.use_flow && D.use_lds && LD.ok) {
+        LdsTaskArgs LA{LD.th.p, LD.meta.p, LD.lvl.p, LD.ch.p, LD.rm.p, LD.src.p};
+        const int nmid = (int)LD.mid_off.size();
+        const int nlaunch = (LD.n_bottom ? 1 : 0) + nmid + (int)LD.tail_tasks.size();
+        prof_begin(pf, PH_FWD);
+        if (LD.n_bottom)
+            k_fwd_lds<T><<<LD.n_bottom, kWave * D.lds_waves_bottom, LD.bottom_bytes, D.stream>>>(LA, A, 0);
+        for (int l = 0; l < nmid; ++l)
+            k_fwd<T, kFrontB><<<LD.mid_cnt[l], kWave, 0, D.stream>>>(A, LD.mid.p + LD.mid_off[l]);
+        for (size_t c = 0; c < LD.tail_tasks.size(); ++c)
+            k_fwd_lds<T><<<1, kWave * D.lds_waves_tail, LD.tail_bytes[c], D.stream>>>(LA, A, LD.tail_tasks[c]);
+        prof_end(pf, PH_FWD, nlaunch);
+        prof_begin(pf, PH_BWD);
+        for (int c = (int)LD.tail_tasks.size() - 1; c >= 0; --c)
+            k_bwd_lds<T><<<1, kWave * D.lds_waves_tail, LD.tail_bytes[c], D.stream>>>(LA, A, LD.tail_tasks[c]);
+        for (int l = nmid - 1; l >= 0; --l)
+            k_bwd<T, kFrontB><<<LD.mid_cnt[l], kWave, 0, D.stream>>>(A, LD.mid.p + LD.mid_off[l]);
+        if (LD.n_bottom)
+            k_bwd_lds<T><<<LD.n_bottom, kWave * D.lds_waves_bottom, LD.bottom_bytes, D.stream>>>(LA, A, 0);
+        prof_end(pf, PH_BWD, nlaunch);
+        return;
+    }
     if (!D.use_flow && D.use_tasks && D.tasks_ok) {
         FactArgs<T> none{};
         const int slice = D.max_c;
@@ -2119,15 +2535,29 @@ bool Instance::dataflow_sweeps() const { return d_->use_flow; }
 
 void Instance::set_task_sweeps(bool on) { d_->use_tasks = on; }
 
+void Instance::set_lds_solve(bool on) { d_->use_lds = on; }
+
 std::string Instance::sweep_plan() const {
     const Impl& D = *d_;
     char buf[256];
     if (D.use_flow) return "dependency-driven";
-    if (!(D.use_tasks && D.tasks_ok))
-        return "levels(" + std::to_string(D.n_levels) + ")";
-    snprintf(buf, sizeof(buf), "tasks(cut=%d bottom=%d mid=%d tail=%d waves=%d est=%.0fus vs levels=%.0fus)", D.plan_cut,
-             D.n_bottom, (int)D.mid_off.size(), D.has_tail ? 1 : 0, D.tail_waves, D.plan_cost_us, D.level_cost_us);
-    return buf;
+    std::string out;
+    if (!(D.use_tasks && D.tasks_ok)) {
+        out = "factor levels(" + std::to_string(D.n_levels) + ")";
+    } else {
+        snprintf(buf, sizeof(buf), "factor tasks(cut=%d bottom=%d mid=%d tail=%d est=%.0fus vs levels=%.0fus)",
+                 D.plan_cut, D.n_bottom, (int)D.mid_off.size(), D.has_tail ? 1 : 0, D.plan_cost_us, D.level_cost_us);
+        out = buf;
+    }
+    const Impl::LdsDev& L = D.lds[precision_ == 1 || fp64_factor_active_ ? 1 : 0];
+    if (D.use_lds && L.ok) {
+        snprintf(buf, sizeof(buf), "; solve lds-tasks(cut=%d bottom=%d/%zuB mid=%d tail-chunks=%zu est=%.0fus)", L.cut,
+                 L.n_bottom, L.bottom_bytes, (int)L.mid_off.size(), L.tail_tasks.size(), L.cost_us);
+        out += buf;
+    } else {
+        out += "; solve as factor";
+    }
+    return out;
 }
 
 int Instance::staged_count(int slot) const {

@@ -149,6 +149,9 @@ int Engine::set_option(const char* key, double value) {
     if (k == "pcg_steps") {
         if (value < 0 || value > 16) return ST_INVALID_ARGUMENT;
         opt_.pcg_steps = (int)value;
+    } else if (k == "lds_solve") {
+        opt_.lds_solve = value != 0;
+        if (inst_) inst_->set_lds_solve(opt_.lds_solve);
     } else if (k == "task_sweeps") {
         opt_.task_sweeps = value != 0;
         if (inst_) inst_->set_task_sweeps(opt_.task_sweeps);
@@ -540,6 +543,7 @@ int Engine::rebuild() {
     inst_->set_use_graphs(cfg_.use_cuda_graphs);
     inst_->set_level_schedule(opt_.level_schedule);
     inst_->set_task_sweeps(opt_.task_sweeps);
+    inst_->set_lds_solve(opt_.lds_solve);
     inst_->setup(h, sym_);
     inst_->set_state(V_.data(), th_.data());
     model_dirty_ = false;

@@ -84,6 +84,7 @@ struct EngineOptions {
     int pcg_steps = 0;         // mixed form: CG steps refining each Newton step (0 = FP32 factor solve, GN refines)
     bool level_schedule = true;   // level-by-level sweeps (0: single dependency-driven launch per sweep)
     bool task_sweeps = true;      // group bottom subtrees / the narrow top into single-workgroup tasks
+    bool lds_solve = true;        // LDS-resident task triangular solves
 };
 
 class Engine {

@@ -122,6 +122,8 @@ class Instance {
     bool dataflow_sweeps() const;
     // task sweeps (bottom subtrees per workgroup, narrow top in one workgroup)
     void set_task_sweeps(bool on);
+    // LDS-resident task triangular solves
+    void set_lds_solve(bool on);
     std::string sweep_plan() const;
 
     // Device-resident telemetry batches: stage (host -> HBM) ahead of time,
