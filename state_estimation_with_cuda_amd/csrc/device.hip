@@ -1501,7 +1501,7 @@ struct Instance::Impl {
         int cut = 0, wcut = 0;
     } lds[2];  // [0] float, [1] double
     bool use_lds = true;
-    int lds_waves_bottom = 4, lds_waves_tail = 16;
+    int lds_waves_bottom = 8, lds_waves_tail = 16;
     // dependency-driven sweeps (all fronts fit in LDS)
     bool use_flow = false;
     bool flow_ok = false;       // every front fits the LDS front of the dataflow kernels
@@ -2536,6 +2536,11 @@ bool Instance::dataflow_sweeps() const { return d_->use_flow; }
 void Instance::set_task_sweeps(bool on) { d_->use_tasks = on; }
 
 void Instance::set_lds_solve(bool on) { d_->use_lds = on; }
+
+void Instance::set_lds_waves(int bottom, int tail) {
+    if (bottom >= 1 && bottom <= 16) d_->lds_waves_bottom = bottom;
+    if (tail >= 1 && tail <= 16) d_->lds_waves_tail = tail;
+}
 
 std::string Instance::sweep_plan() const {
     const Impl& D = *d_;

@@ -152,6 +152,10 @@ int Engine::set_option(const char* key, double value) {
     } else if (k == "lds_solve") {
         opt_.lds_solve = value != 0;
         if (inst_) inst_->set_lds_solve(opt_.lds_solve);
+    } else if (k == "lds_waves_bottom" || k == "lds_waves_tail") {
+        if (value < 1 || value > 16) return ST_INVALID_ARGUMENT;
+        (k == "lds_waves_bottom" ? opt_.lds_waves_bottom : opt_.lds_waves_tail) = (int)value;
+        model_dirty_ = true;  // the plan is rebuilt at the next upload
     } else if (k == "task_sweeps") {
         opt_.task_sweeps = value != 0;
         if (inst_) inst_->set_task_sweeps(opt_.task_sweeps);
@@ -544,6 +548,7 @@ int Engine::rebuild() {
     inst_->set_level_schedule(opt_.level_schedule);
     inst_->set_task_sweeps(opt_.task_sweeps);
     inst_->set_lds_solve(opt_.lds_solve);
+    inst_->set_lds_waves(opt_.lds_waves_bottom, opt_.lds_waves_tail);
     inst_->setup(h, sym_);
     inst_->set_state(V_.data(), th_.data());
     model_dirty_ = false;

@@ -124,6 +124,8 @@ class Instance {
     void set_task_sweeps(bool on);
     // LDS-resident task triangular solves
     void set_lds_solve(bool on);
+    // waves per workgroup of the bottom-subtree tasks and of the top chunks (takes effect at setup)
+    void set_lds_waves(int bottom, int tail);
     std::string sweep_plan() const;
 
     // Device-resident telemetry batches: stage (host -> HBM) ahead of time,
