@@ -44,7 +44,7 @@ typedef struct SLE_ExtResult {
  *   "lds_solve"       1 (default): triangular solves walk subtrees / top
  *                     chunks with the L panels resident in LDS
  *   "lds_waves_bottom", "lds_waves_tail"  waves per workgroup of the LDS
- *                     subtree tasks (default 8) and top chunks (default 16)
+ *                     subtree tasks and top chunks (default 16 each)
  *   "level_schedule"  1 (default): factor and solve level by level (kernel
  *                     boundaries order the fronts); 0: one dependency-driven
  *                     launch per sweep with release/acquire flags per front */

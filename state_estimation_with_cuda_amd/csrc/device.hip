@@ -57,9 +57,11 @@ constexpr int kFrontB = 64;    // 32 < c <= 64: LDS front of 64x64
 constexpr int kVecMax = 2048;  // triangular-solve LDS vector bound for large fronts
 constexpr int kSpinLimit = 1 << 22;  // polls before a dependency wait gives up (~0.1 s)
 
-// measurement code byte: type (4 bits) | end << 4 | active << 5
-__host__ __device__ inline uint8_t meas_code(int type, int end, int active) {
-    return (uint8_t)((type & 15) | ((end & 1) << 4) | ((active & 1) << 5));
+// measurement code byte: type (4 bits) | end << 4 | active << 5 | from-first << 6
+// (from-first: the branch's from bus precedes its to bus in the natural state
+// order, which fixes the column order of a flow row)
+__host__ __device__ inline uint8_t meas_code(int type, int end, int active, int from_first) {
+    return (uint8_t)((type & 15) | ((end & 1) << 4) | ((active & 1) << 5) | ((from_first & 1) << 6));
 }
 
 struct DevStatus {
@@ -367,7 +369,7 @@ __global__ __launch_bounds__(kBlock) void k_meas(MeasArgs A, double* __restrict_
                     if (fb != A.slack) hs[q++] = sw * (d[0] + d[1]);
                     hs[q] = sw * (d[2] + d[3]);
                 } else {
-                    bool fl = fb < tb;
+                    const bool fl = (code >> 6) & 1;
                     int lo = fl ? fb : tb, hi = fl ? tb : fb;
                     double dlo_t = fl ? d[0] : d[1], dhi_t = fl ? d[1] : d[0];
                     double dlo_v = fl ? d[2] : d[3], dhi_v = fl ? d[3] : d[2];
@@ -955,16 +957,24 @@ __global__ __launch_bounds__(kWave) void k_bwd_flow(SolveArgs<T> A, const int32_
 // ---------------------------------------------------------------------------
 // Right-hand side b = H^T W r in permuted state order (FP64 accumulation)
 // ---------------------------------------------------------------------------
+// States are walked in bus-layout order (thread t handles permuted state
+// tout[t]) so that the measurement rows it reads are neighbours in memory.
 template <typename TO>
 __global__ __launch_bounds__(kBlock) void k_rhs(int ns, const int32_t* __restrict__ tptr,
                                                 const int32_t* __restrict__ tpos, const int32_t* __restrict__ trow,
-                                                const double* __restrict__ hs, const double* __restrict__ rs,
-                                                TO* __restrict__ x) {
-    int k = blockIdx.x * kBlock + threadIdx.x;
-    if (k >= ns) return;
+                                                const int32_t* __restrict__ tout, const double* __restrict__ hs,
+                                                const double* __restrict__ rs, TO* __restrict__ x) {
+    int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= ns) return;
     double acc = 0.0;
-    for (int p = tptr[k]; p < tptr[k + 1]; ++p) acc += hs[tpos[p]] * rs[trow[p]];
-    x[k] = (TO)acc;
+    for (int p = tptr[t]; p < tptr[t + 1]; ++p) acc += hs[tpos[p]] * rs[trow[p]];
+    x[tout[t]] = (TO)acc;
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather(int n, const int32_t* __restrict__ idx,
+                                                   const double* __restrict__ src, double* __restrict__ dst) {
+    int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p < n) dst[p] = src[idx[p]];
 }
 
 // ---------------------------------------------------------------------------
@@ -1467,7 +1477,14 @@ struct Instance::Impl {
     DBuf<int32_t> zidx;
     DBuf<double> zval;
     // symbolic structures
-    DBuf<int32_t> tptr, tpos, trow, gptr, gloc, gdiag, g_sn_ptr;
+    DBuf<int32_t> tptr, tpos, trow, tout, gptr, gloc, gdiag, g_sn_ptr;
+    // device data layout: buses in Cuthill-McKee order from the slack, branches
+    // and measurement rows sorted by the layout position of their (from) bus
+    std::vector<int32_t> lb, bl, lk, kl, lm, ml;
+    std::vector<int32_t> qmap;    // natural H position -> device H position
+    std::vector<int32_t> yrow_d;  // device Ybus row pointers
+    DBuf<int32_t> ml_d;
+    DBuf<double> ztmp;
     DBuf<int2> gpair;
     DBuf<int32_t> sn_first, sn_k, sn_c, child_ptr, child, rmap, rows, perm, pth, pv, lists;
     DBuf<int64_t> foff, rmoff, uoff, rows_off;
@@ -1501,7 +1518,7 @@ struct Instance::Impl {
         int cut = 0, wcut = 0;
     } lds[2];  // [0] float, [1] double
     bool use_lds = true;
-    int lds_waves_bottom = 8, lds_waves_tail = 16;
+    int lds_waves_bottom = 16, lds_waves_tail = 16;
     // dependency-driven sweeps (all fronts fit in LDS)
     bool use_flow = false;
     bool flow_ok = false;       // every front fits the LDS front of the dataflow kernels
@@ -1550,6 +1567,7 @@ Instance::~Instance() {
     rel(d_->hs); rel(d_->hcolp); rel(d_->cg_b); rel(d_->cg_x); rel(d_->cg_r); rel(d_->cg_p); rel(d_->cg_q);
     rel(d_->cg_z); rel(d_->cg_t); rel(d_->cg_parts); rel(d_->cg_s); rel(d_->z); rel(d_->sigma); rel(d_->rs); rel(d_->hx); rel(d_->r); rel(d_->jpart); rel(d_->rpart);
     rel(d_->mpart); rel(d_->ipart); rel(d_->zidx); rel(d_->zval); rel(d_->tptr); rel(d_->tpos); rel(d_->trow);
+    rel(d_->tout); rel(d_->ml_d); rel(d_->ztmp);
     rel(d_->gptr); rel(d_->gloc); rel(d_->gdiag); rel(d_->g_sn_ptr); rel(d_->gpair); rel(d_->sn_first);
     rel(d_->sn_k); rel(d_->sn_c); rel(d_->child_ptr); rel(d_->child); rel(d_->rmap); rel(d_->rows);
     rel(d_->perm); rel(d_->pth); rel(d_->pv); rel(d_->lists); rel(d_->foff); rel(d_->rmoff); rel(d_->uoff);
@@ -1587,20 +1605,100 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
     D.ns = S.ns;
     D.slack = h.slack;
     if (S.max_c > kVecMax) throw std::runtime_error("front larger than the supported maximum (2048)");
-    D.bus_type.upload(h.bus_type, bytes_, s);
-    D.vset.upload(h.vset, bytes_, s);
-    D.from.upload(h.from, bytes_, s);
-    D.to.upload(h.to, bytes_, s);
-    D.gs.upload(h.gs, bytes_, s);
-    D.bs.upload(h.bs, bytes_, s);
-    D.bc.upload(h.bc, bytes_, s);
-    D.tap.upload(h.tap, bytes_, s);
-    D.shift.upload(h.shift, bytes_, s);
-    D.status.upload(h.status, bytes_, s);
-    D.yrow.upload(S.yrow, bytes_, s);
-    D.ycol.upload(S.ycol, bytes_, s);
-    D.ycptr.upload(S.ycptr, bytes_, s);
-    D.ycon.upload(S.ycon, bytes_, s);
+    // ---- device data layout ----------------------------------------------
+    const int n = h.n, nb = h.nb, m = h.m;
+    D.lb.assign(n, -1);
+    D.bl.clear();
+    {
+        // Cuthill-McKee over the Ybus graph from the slack (then other components)
+        std::vector<int32_t> q;
+        q.reserve(n);
+        for (int root = -1; (int)D.bl.size() < n;) {
+            int start = root < 0 ? h.slack : -1;
+            if (start < 0)
+                for (int b = 0; b < n; ++b)
+                    if (D.lb[b] < 0) {
+                        start = b;
+                        break;
+                    }
+            root = start;
+            D.lb[start] = (int32_t)D.bl.size();
+            D.bl.push_back(start);
+            for (size_t head = D.bl.size() - 1; head < D.bl.size(); ++head) {
+                int b = D.bl[head];
+                for (int e = S.yrow[b]; e < S.yrow[b + 1]; ++e) {
+                    int j = S.ycol[e];
+                    if (D.lb[j] < 0) {
+                        D.lb[j] = (int32_t)D.bl.size();
+                        D.bl.push_back(j);
+                    }
+                }
+            }
+        }
+    }
+    D.kl.resize(nb);
+    for (int k = 0; k < nb; ++k) D.kl[k] = k;
+    std::stable_sort(D.kl.begin(), D.kl.end(), [&](int a, int b) { return D.lb[h.from[a]] < D.lb[h.from[b]]; });
+    D.lk.assign(nb, 0);
+    for (int p = 0; p < nb; ++p) D.lk[D.kl[p]] = p;
+    auto anchor = [&](int i) { return is_bus_measurement(h.mtype[i]) ? h.mloc[i] : h.from[h.mloc[i]]; };
+    D.ml.resize(m);
+    for (int i = 0; i < m; ++i) D.ml[i] = i;
+    std::stable_sort(D.ml.begin(), D.ml.end(), [&](int a, int b) { return D.lb[anchor(a)] < D.lb[anchor(b)]; });
+    D.lm.assign(m, 0);
+    for (int p = 0; p < m; ++p) D.lm[D.ml[p]] = p;
+    auto bus_perm = [&](const auto& v) {
+        std::vector<std::decay_t<decltype(v[0])>> o(n);
+        for (int p = 0; p < n; ++p) o[p] = v[D.bl[p]];
+        return o;
+    };
+    auto br_perm = [&](const auto& v) {
+        std::vector<std::decay_t<decltype(v[0])>> o(nb);
+        for (int p = 0; p < nb; ++p) o[p] = v[D.kl[p]];
+        return o;
+    };
+    D.slack = D.lb[h.slack];
+    D.bus_type.upload(bus_perm(h.bus_type), bytes_, s);
+    D.vset.upload(bus_perm(h.vset), bytes_, s);
+    {
+        std::vector<int32_t> f(nb), t(nb);
+        for (int p = 0; p < nb; ++p) {
+            f[p] = D.lb[h.from[D.kl[p]]];
+            t[p] = D.lb[h.to[D.kl[p]]];
+        }
+        D.from.upload(f, bytes_, s);
+        D.to.upload(t, bytes_, s);
+        HIPCHECK(hipStreamSynchronize(s));
+    }
+    D.gs.upload(br_perm(h.gs), bytes_, s);
+    D.bs.upload(br_perm(h.bs), bytes_, s);
+    D.bc.upload(br_perm(h.bc), bytes_, s);
+    D.tap.upload(br_perm(h.tap), bytes_, s);
+    D.shift.upload(br_perm(h.shift), bytes_, s);
+    D.status.upload(br_perm(h.status), bytes_, s);
+    HIPCHECK(hipStreamSynchronize(s));
+    {
+        // Ybus rows in bus layout order; entries keep their natural order (it
+        // fixes the column order of injection rows of H)
+        std::vector<int32_t> yr(n + 1, 0), yc, cp(1, 0), cn;
+        yc.reserve(S.ycol.size());
+        for (int p = 0; p < n; ++p) {
+            const int b = D.bl[p];
+            for (int e = S.yrow[b]; e < S.yrow[b + 1]; ++e) {
+                yc.push_back(D.lb[S.ycol[e]]);
+                for (int c = S.ycptr[e]; c < S.ycptr[e + 1]; ++c) cn.push_back(D.lk[S.ycon[c] >> 2] * 4 + (S.ycon[c] & 3));
+                cp.push_back((int32_t)cn.size());
+            }
+            yr[p + 1] = (int32_t)yc.size();
+        }
+        if (cn.empty()) cn.push_back(0);
+        D.yrow_d = yr;
+        D.yrow.upload(yr, bytes_, s);
+        D.ycol.upload(yc, bytes_, s);
+        D.ycptr.upload(cp, bytes_, s);
+        D.ycon.upload(cn, bytes_, s);
+        HIPCHECK(hipStreamSynchronize(s));
+    }
     size_t nY = S.ycol.size();
     if (D.yG.n != nY) {
         D.yG.alloc(nY, bytes_);
@@ -1610,12 +1708,13 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
     }
     if (D.V.n != (size_t)h.n) {
         std::vector<double> V0(h.n, 1.0), t0(h.n, 0.0);
-        for (int b = 0; b < h.n; ++b)
-            if (h.bus_type[b] != 0) V0[b] = h.vset[b];
+        for (int p = 0; p < h.n; ++p)
+            if (h.bus_type[D.bl[p]] != 0) V0[p] = h.vset[D.bl[p]];
         D.V.upload(V0, bytes_, s);
         D.th.upload(t0, bytes_, s);
         D.P.alloc(h.n, bytes_);
         D.Q.alloc(h.n, bytes_);
+        HIPCHECK(hipStreamSynchronize(s));
     }
     if (D.Pf.n != (size_t)std::max(1, h.nb)) {
         D.Pf.alloc(std::max(1, h.nb), bytes_);
@@ -1623,13 +1722,31 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
         D.Pt.alloc(std::max(1, h.nb), bytes_);
         D.Qt.alloc(std::max(1, h.nb), bytes_);
     }
-    std::vector<uint8_t> code(h.m);
-    for (int i = 0; i < h.m; ++i) code[i] = meas_code(h.mtype[i], h.mend[i], h.active[i]);
-    D.mcode.upload(code, bytes_, s);
-    D.mloc.upload(h.mloc, bytes_, s);
-    D.hrow.upload(S.hrow, bytes_, s);
-    D.z.upload(h.z, bytes_, s);
-    D.sigma.upload(h.sigma, bytes_, s);
+    {
+        std::vector<uint8_t> code(m);
+        std::vector<int32_t> loc(m), hr(m + 1, 0);
+        std::vector<double> sg(m), zz(m);
+        for (int p = 0; p < m; ++p) {
+            const int i = D.ml[p];
+            const bool bm = is_bus_measurement(h.mtype[i]);
+            const int l = h.mloc[i];
+            code[p] = meas_code(h.mtype[i], h.mend[i], h.active[i], bm ? 0 : (h.from[l] < h.to[l] ? 1 : 0));
+            loc[p] = bm ? D.lb[l] : D.lk[l];
+            sg[p] = h.sigma[i];
+            zz[p] = h.z[i];
+            hr[p + 1] = hr[p] + (S.hrow[i + 1] - S.hrow[i]);
+        }
+        D.mcode.upload(code, bytes_, s);
+        D.mloc.upload(loc, bytes_, s);
+        D.sigma.upload(sg, bytes_, s);
+        D.z.upload(zz, bytes_, s);
+        D.hrow.upload(hr, bytes_, s);
+        D.ml_d.upload(D.ml, bytes_, s);
+        HIPCHECK(hipStreamSynchronize(s));
+        D.qmap.assign(S.hcol.size(), 0);
+        for (int i = 0; i < m; ++i)
+            for (int q = S.hrow[i]; q < S.hrow[i + 1]; ++q) D.qmap[q] = hr[D.lm[i]] + (q - S.hrow[i]);
+    }
     if (D.rs.n != (size_t)std::max(1, h.m)) {
         D.rs.alloc(std::max(1, h.m), bytes_);
         D.hx.alloc(std::max(1, h.m), bytes_);
@@ -1642,9 +1759,8 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
     D.mpart.alloc(nblk(std::max(1, h.n)), bytes_);
     D.hs.alloc(std::max<size_t>(1, S.hcol.size()), bytes_);
     {
-        std::vector<int32_t> hp(S.hcol.size());
-        for (size_t q = 0; q < hp.size(); ++q) hp[q] = S.iperm[S.hcol[q]];
-        if (hp.empty()) hp.push_back(0);
+        std::vector<int32_t> hp(std::max<size_t>(1, S.hcol.size()), 0);
+        for (size_t q = 0; q < S.hcol.size(); ++q) hp[D.qmap[q]] = S.iperm[S.hcol[q]];
         D.hcolp.upload(hp, bytes_, s);
         HIPCHECK(hipStreamSynchronize(s));
         const size_t nsv = std::max(1, S.ns);
@@ -1653,13 +1769,40 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
         D.cg_parts.alloc(nblk(nsv), bytes_);
         D.cg_s.alloc(1, bytes_);
     }
-    D.tptr.upload(S.tptr, bytes_, s);
-    D.tpos.upload(S.tpos, bytes_, s);
-    D.trow.upload(S.trow, bytes_, s);
+    {
+        // H^T lists walked in bus-layout order of the states
+        std::vector<int32_t> sorder;
+        sorder.reserve(S.ns);
+        for (int p = 0; p < n; ++p) {
+            const int b = D.bl[p];
+            if (S.bus_pth[b] >= 0) sorder.push_back(S.bus_pth[b]);
+            sorder.push_back(S.bus_pv[b]);
+        }
+        std::vector<int32_t> tp(S.ns + 1, 0), tq, tr;
+        tq.reserve(S.tpos.size());
+        tr.reserve(S.tpos.size());
+        for (int t = 0; t < S.ns; ++t) {
+            const int k = sorder[t];
+            for (int p = S.tptr[k]; p < S.tptr[k + 1]; ++p) {
+                tq.push_back(D.qmap[S.tpos[p]]);
+                tr.push_back(D.lm[S.trow[p]]);
+            }
+            tp[t + 1] = (int32_t)tq.size();
+        }
+        if (tq.empty()) {
+            tq.push_back(0);
+            tr.push_back(0);
+        }
+        D.tptr.upload(tp, bytes_, s);
+        D.tpos.upload(tq, bytes_, s);
+        D.trow.upload(tr, bytes_, s);
+        D.tout.upload(sorder, bytes_, s);
+        HIPCHECK(hipStreamSynchronize(s));
+    }
     D.gptr.upload(S.gpair_ptr, bytes_, s);
     {
         std::vector<int2> gp(S.gpair.size() / 2);
-        for (size_t q = 0; q < gp.size(); ++q) gp[q] = make_int2(S.gpair[2 * q], S.gpair[2 * q + 1]);
+        for (size_t q = 0; q < gp.size(); ++q) gp[q] = make_int2(D.qmap[S.gpair[2 * q]], D.qmap[S.gpair[2 * q + 1]]);
         if (gp.empty()) gp.push_back(make_int2(0, 0));
         D.gpair.upload(gp, bytes_, s);
         HIPCHECK(hipStreamSynchronize(s));
@@ -1682,8 +1825,8 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
     }
     D.rows.upload(S.sn_rows, bytes_, s);
     D.perm.upload(S.perm, bytes_, s);
-    D.pth.upload(S.bus_pth, bytes_, s);
-    D.pv.upload(S.bus_pv, bytes_, s);
+    D.pth.upload(bus_perm(S.bus_pth), bytes_, s);
+    D.pv.upload(bus_perm(S.bus_pv), bytes_, s);
     D.foff.upload(S.sn_foff, bytes_, s);
     D.rmoff.upload(S.sn_rmoff, bytes_, s);
     D.uoff.upload(S.sn_uoff, bytes_, s);
@@ -1827,16 +1970,26 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
     HIPCHECK(hipStreamSynchronize(s));
 }
 
+namespace {
+template <typename V>
+std::vector<typename V::value_type> take(const V& v, const std::vector<int32_t>& idx) {
+    std::vector<typename V::value_type> o(idx.size());
+    for (size_t p = 0; p < idx.size(); ++p) o[p] = v[idx[p]];
+    return o;
+}
+}  // namespace
+
 void Instance::update_branches(const HostArrays& h) {
     HIPCHECK(hipSetDevice(device_));
     Impl& D = *d_;
     hipStream_t s = D.stream;
-    D.gs.upload(h.gs.data(), h.nb, s);
-    D.bs.upload(h.bs.data(), h.nb, s);
-    D.bc.upload(h.bc.data(), h.nb, s);
-    D.tap.upload(h.tap.data(), h.nb, s);
-    D.shift.upload(h.shift.data(), h.nb, s);
-    D.status.upload(h.status.data(), h.nb, s);
+    D.gs.upload(take(h.gs, D.kl), bytes_, s);
+    D.bs.upload(take(h.bs, D.kl), bytes_, s);
+    D.bc.upload(take(h.bc, D.kl), bytes_, s);
+    D.tap.upload(take(h.tap, D.kl), bytes_, s);
+    D.shift.upload(take(h.shift, D.kl), bytes_, s);
+    D.status.upload(take(h.status, D.kl), bytes_, s);
+    HIPCHECK(hipStreamSynchronize(s));
     size_t nY = S_->ycol.size();
     if (nY) k_ybus<<<nblk(nY), kBlock, 0, s>>>((int)nY, D.ycptr.p, D.ycon.p, D.gs.p, D.bs.p, D.bc.p, D.tap.p,
                                                 D.shift.p, D.status.p, D.yG.p, D.yB.p);
@@ -1845,10 +1998,16 @@ void Instance::update_branches(const HostArrays& h) {
     factor_valid_ = false;
 }
 
+// z arrives in insertion order; the device holds it in row layout order
 void Instance::set_z(const double* z) {
     HIPCHECK(hipSetDevice(device_));
-    d_->z.upload(z, d_->m, d_->stream);
-    HIPCHECK(hipStreamSynchronize(d_->stream));
+    Impl& D = *d_;
+    if (!D.m) return;
+    if (D.ztmp.n != (size_t)D.m) D.ztmp.alloc(D.m, bytes_);
+    D.ztmp.upload(z, D.m, D.stream);
+    k_gather<<<nblk(D.m), kBlock, 0, D.stream>>>(D.m, D.ml_d.p, D.ztmp.p, D.z.p);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(D.stream));
 }
 
 void Instance::set_z_indexed(const int32_t* idx, const double* val, int count) {
@@ -1859,7 +2018,9 @@ void Instance::set_z_indexed(const int32_t* idx, const double* val, int count) {
         D.zidx.alloc(count, bytes_);
         D.zval.alloc(count, bytes_);
     }
-    D.zidx.upload(idx, count, D.stream);
+    std::vector<int32_t> di(count);
+    for (int q = 0; q < count; ++q) di[q] = D.lm[idx[q]];
+    D.zidx.upload(di.data(), count, D.stream);
     D.zval.upload(val, count, D.stream);
     k_scatter_z<<<nblk(count), kBlock, 0, D.stream>>>(count, D.zidx.p, D.zval.p, D.z.p);
     HIPCHECK(hipGetLastError());
@@ -1868,16 +2029,32 @@ void Instance::set_z_indexed(const int32_t* idx, const double* val, int count) {
 
 void Instance::set_state(const double* V, const double* th) {
     HIPCHECK(hipSetDevice(device_));
-    d_->V.upload(V, d_->n, d_->stream);
-    d_->th.upload(th, d_->n, d_->stream);
-    HIPCHECK(hipStreamSynchronize(d_->stream));
+    Impl& D = *d_;
+    std::vector<double> v(D.n), t(D.n);
+    for (int p = 0; p < D.n; ++p) {
+        v[p] = V[D.bl[p]];
+        t[p] = th[D.bl[p]];
+    }
+    D.V.upload(v.data(), D.n, D.stream);
+    D.th.upload(t.data(), D.n, D.stream);
+    HIPCHECK(hipStreamSynchronize(D.stream));
 }
+
+namespace {
+// device (layout order) -> caller (insertion order)
+void fetch_unpermute(const DBuf<double>& src, const std::vector<int32_t>& pos_of, double* dst, hipStream_t s) {
+    if (!dst || pos_of.empty()) return;
+    std::vector<double> tmp(pos_of.size());
+    HIPCHECK(hipMemcpyAsync(tmp.data(), src.p, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    for (size_t i = 0; i < pos_of.size(); ++i) dst[i] = tmp[pos_of[i]];
+}
+}  // namespace
 
 void Instance::get_state(double* V, double* th) {
     HIPCHECK(hipSetDevice(device_));
-    if (V) HIPCHECK(hipMemcpyAsync(V, d_->V.p, d_->n * sizeof(double), hipMemcpyDeviceToHost, d_->stream));
-    if (th) HIPCHECK(hipMemcpyAsync(th, d_->th.p, d_->n * sizeof(double), hipMemcpyDeviceToHost, d_->stream));
-    HIPCHECK(hipStreamSynchronize(d_->stream));
+    fetch_unpermute(d_->V, d_->lb, V, d_->stream);
+    fetch_unpermute(d_->th, d_->lb, th, d_->stream);
 }
 
 void Instance::flat_start() {
@@ -2111,7 +2288,7 @@ void launch_trisolve(Instance::Impl& D, Prof* pf = nullptr) {
 template <typename T>
 void launch_rhs(Instance::Impl& D) {
     if (D.ns)
-        k_rhs<T><<<nblk(D.ns), kBlock, 0, D.stream>>>(D.ns, D.tptr.p, D.tpos.p, D.trow.p, D.hs.p, D.rs.p,
+        k_rhs<T><<<nblk(D.ns), kBlock, 0, D.stream>>>(D.ns, D.tptr.p, D.tpos.p, D.trow.p, D.tout.p, D.hs.p, D.rs.p,
                                                       bufs<T>(D).x.p);
 }
 
@@ -2144,14 +2321,15 @@ template <typename T>
 void launch_pcg(Instance::Impl& D, int steps) {
     const int ns = D.ns, nb = nblk(ns);
     CgScalars* cs = D.cg_s.p;
-    k_rhs<double><<<nb, kBlock, 0, D.stream>>>(ns, D.tptr.p, D.tpos.p, D.trow.p, D.hs.p, D.rs.p, D.cg_b.p);
+    k_rhs<double><<<nb, kBlock, 0, D.stream>>>(ns, D.tptr.p, D.tpos.p, D.trow.p, D.tout.p, D.hs.p, D.rs.p, D.cg_b.p);
     k_cg_init<<<nb, kBlock, 0, D.stream>>>(ns, D.cg_b.p, D.cg_x.p, D.cg_r.p);
     launch_precond<T>(D, D.cg_r.p, D.cg_z.p);
     launch_dot(D, D.cg_r.p, D.cg_z.p, &cs->rz[0]);
     k_convert<double, double><<<nb, kBlock, 0, D.stream>>>(ns, D.cg_z.p, D.cg_p.p);
     for (int k = 0; k < steps; ++k) {
         if (D.m) k_hmul<<<nblk(D.m), kBlock, 0, D.stream>>>(D.m, D.hrow.p, D.hcolp.p, D.hs.p, D.cg_p.p, D.cg_t.p);
-        k_rhs<double><<<nb, kBlock, 0, D.stream>>>(ns, D.tptr.p, D.tpos.p, D.trow.p, D.hs.p, D.cg_t.p, D.cg_q.p);
+        k_rhs<double><<<nb, kBlock, 0, D.stream>>>(ns, D.tptr.p, D.tpos.p, D.trow.p, D.tout.p, D.hs.p, D.cg_t.p,
+                                                   D.cg_q.p);
         launch_dot(D, D.cg_p.p, D.cg_q.p, &cs->pq[k]);
         k_cg_xr<<<nb, kBlock, 0, D.stream>>>(ns, D.cg_x.p, D.cg_r.p, D.cg_p.p, D.cg_q.p, cs, k);
         if (k + 1 == steps) break;
@@ -2359,7 +2537,7 @@ SolveOutcome Instance::solve(const SolveParams& p) {
     read_status(D);
     out.objective = D.m ? D.hst->J : 0.0;
     out.largest_residual = D.m ? D.hst->rmax : 0.0;
-    out.largest_residual_idx = D.m ? D.hst->ridx : -1;
+    out.largest_residual_idx = (D.m && D.hst->ridx >= 0) ? D.ml[D.hst->ridx] : -1;
     float ms = 0;
     HIPCHECK(hipEventElapsedTime(&ms, D.ev0, D.ev1));
     out.ms = ms;
@@ -2372,18 +2550,14 @@ void Instance::get_results(double* P, double* Q, double* Pf, double* Qf, double*
                            double* r) {
     HIPCHECK(hipSetDevice(device_));
     Impl& D = *d_;
-    auto cp = [&](double* dst, const DBuf<double>& src, size_t cnt) {
-        if (dst && cnt) HIPCHECK(hipMemcpyAsync(dst, src.p, cnt * sizeof(double), hipMemcpyDeviceToHost, D.stream));
-    };
-    cp(P, D.P, D.n);
-    cp(Q, D.Q, D.n);
-    cp(Pf, D.Pf, D.nb);
-    cp(Qf, D.Qf, D.nb);
-    cp(Pt, D.Pt, D.nb);
-    cp(Qt, D.Qt, D.nb);
-    cp(hx, D.hx, D.m);
-    cp(r, D.r, D.m);
-    HIPCHECK(hipStreamSynchronize(D.stream));
+    fetch_unpermute(D.P, D.lb, P, D.stream);
+    fetch_unpermute(D.Q, D.lb, Q, D.stream);
+    fetch_unpermute(D.Pf, D.lk, Pf, D.stream);
+    fetch_unpermute(D.Qf, D.lk, Qf, D.stream);
+    fetch_unpermute(D.Pt, D.lk, Pt, D.stream);
+    fetch_unpermute(D.Qt, D.lk, Qt, D.stream);
+    fetch_unpermute(D.hx, D.lm, hx, D.stream);
+    fetch_unpermute(D.r, D.lm, r, D.stream);
 }
 
 void Instance::debug_eval(double* hx, double* P, double* Q, double* Pf, double* Qf, double* Pt, double* Qt) {
@@ -2401,10 +2575,19 @@ void Instance::debug_eval(double* hx, double* P, double* Q, double* Pf, double* 
 
 void Instance::debug_ybus(double* G, double* B) {
     HIPCHECK(hipSetDevice(device_));
-    size_t nY = S_->ycol.size();
-    HIPCHECK(hipMemcpyAsync(G, d_->yG.p, nY * sizeof(double), hipMemcpyDeviceToHost, d_->stream));
-    HIPCHECK(hipMemcpyAsync(B, d_->yB.p, nY * sizeof(double), hipMemcpyDeviceToHost, d_->stream));
-    HIPCHECK(hipStreamSynchronize(d_->stream));
+    Impl& D = *d_;
+    const Symbolic& S = *S_;
+    const size_t nY = S.ycol.size();
+    std::vector<double> g(nY), b(nY);
+    HIPCHECK(hipMemcpyAsync(g.data(), D.yG.p, nY * sizeof(double), hipMemcpyDeviceToHost, D.stream));
+    HIPCHECK(hipMemcpyAsync(b.data(), D.yB.p, nY * sizeof(double), hipMemcpyDeviceToHost, D.stream));
+    HIPCHECK(hipStreamSynchronize(D.stream));
+    for (int i = 0; i < S.n; ++i)
+        for (int e = S.yrow[i]; e < S.yrow[i + 1]; ++e) {
+            const int de = D.yrow_d[D.lb[i]] + (e - S.yrow[i]);
+            G[e] = g[de];
+            B[e] = b[de];
+        }
 }
 
 namespace {
@@ -2426,13 +2609,21 @@ void Instance::debug_jacobian_gain(double* hv, double* gv) {
         launch_meas(D, 0, 1.5, 1, 1, false);
         launch_spgemm<double>(D, S);
         HIPCHECK(hipGetLastError());
-        if (hv) fetch_as_double(D.hs, S.hcol.size(), hv, D.stream);
+        if (hv) {
+            std::vector<double> tmp(S.hcol.size());
+            fetch_as_double(D.hs, S.hcol.size(), tmp.data(), D.stream);
+            for (size_t q = 0; q < S.hcol.size(); ++q) hv[q] = tmp[D.qmap[q]];
+        }
         if (gv) fetch_as_double(D.f64.Gval, S.nnz_g, gv, D.stream);
     } else {
         launch_meas(D, 0, 1.5, 1, 1, false);
         launch_spgemm<float>(D, S);
         HIPCHECK(hipGetLastError());
-        if (hv) fetch_as_double(D.hs, S.hcol.size(), hv, D.stream);
+        if (hv) {
+            std::vector<double> tmp(S.hcol.size());
+            fetch_as_double(D.hs, S.hcol.size(), tmp.data(), D.stream);
+            for (size_t q = 0; q < S.hcol.size(); ++q) hv[q] = tmp[D.qmap[q]];
+        }
         if (gv) fetch_as_double(D.f32.Gval, S.nnz_g, gv, D.stream);
     }
 }
@@ -2510,7 +2701,9 @@ void Instance::stage_telemetry(int slot, const int32_t* idx, const double* val, 
         sb.idx.alloc(count, bytes_);
         sb.val.alloc(count, bytes_);
     }
-    sb.idx.upload(idx, count, D.stream);
+    std::vector<int32_t> di(count);
+    for (int q = 0; q < count; ++q) di[q] = D.lm[idx[q]];
+    sb.idx.upload(di.data(), count, D.stream);
     sb.val.upload(val, count, D.stream);
     sb.count = count;
     HIPCHECK(hipStreamSynchronize(D.stream));

@@ -85,7 +85,7 @@ struct EngineOptions {
     bool level_schedule = true;   // level-by-level sweeps (0: single dependency-driven launch per sweep)
     bool task_sweeps = true;      // group bottom subtrees / the narrow top into single-workgroup tasks
     bool lds_solve = true;        // LDS-resident task triangular solves
-    int lds_waves_bottom = 8, lds_waves_tail = 16;
+    int lds_waves_bottom = 16, lds_waves_tail = 16;
 };
 
 class Engine {
