@@ -9,9 +9,16 @@ REALTIME Gauss-Newton solve runs to ||dx||_inf < 1e-6 reusing the cached gain
 factor.  The mixed form (FP32 Jacobian / gain / factor, FP64 residual, RHS and
 state) and the plain FP64 form are timed on the same sequence of updates.
 
-Prints ONE JSON line: value = mixed-form solves/s, with p50/p99 step latency,
-the FP64 form beside it, errors against the FP64 CPU oracle, a roofline entry
+Prints ONE JSON line.  `value` is the solves/s of the faster of the two forms
+(named in config.form; on this workload the plain FP64 form, because the FP32
+factor of the 200k-bus gain matrix needs conjugate-gradient refinement to
+contract), with p50/p99 step latency of that form; both forms are reported in
+full beside it, with their errors against the FP64 CPU oracle, a roofline entry
 for the dominant kernel (HIP-event timed) and the oracle's CPU baseline.
+
+--copy-calibration runs a plain device copy of known size first; under
+`rocprofv3 --pmc FETCH_SIZE` / `WRITE_SIZE` it calibrates the counters that
+give the HBM traffic per launch recorded in profiles/hbm_traffic_200k.md.
 """
 from __future__ import annotations
 
@@ -29,6 +36,8 @@ sys.path.insert(0, ROOT)
 METRIC = "p99 update latency + solves/s, 200k-bus WLS, mixed-prec vs FP64 at 1e-6"
 HBM_PEAK_GBS = 8000.0
 TOL = 1e-6
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_200k.md")
+COPY_BYTES = 512 << 20
 
 
 def log(*a):
@@ -64,22 +73,63 @@ def request_z(g, req):
 
 
 def algorithmic_bytes(sz, g, prec_bytes):
-    """Compulsory HBM bytes per launch of each kernel class (each element once)."""
+    """Compulsory HBM bytes per launch of each kernel class (each element once).
+    Jacobian values are FP64 in both forms; prec_bytes is the factor/solve
+    vector width (4 mixed, 8 plain FP64)."""
     n, nb, m, ns = g.n, g.nb, g.m, sz["ns"]
     nnz_y, nnz_h = sz["nnz_y"], sz["nnz_h"]
     s = prec_bytes
+    ybus = (n + 1) * 4 + nnz_y * (4 + 8 * 4)  # yrow, ycol, yG, yB, Pij, Qij
     return {
-        # V, th read; P, Q write; yrow, ycol, yG, yB read; Pij, Qij write
-        "power": n * 8 * 4 + (n + 1) * 4 + nnz_y * (4 + 8 * 4),
-        # per row: code, loc, z, sigma read, rs write, hrow; H values write;
-        # bus state/P/Q, Ybus row data and branch data read once
-        "meas": m * (1 + 4 + 8 + 8 + 8) + (m + 1) * 4 + nnz_h * s + n * 8 * 4 + (n + 1) * 4
-        + nnz_y * (4 + 8 * 4) + nb * (4 + 4 + 8 * 5),
-        # tptr, tpos, trow, hs read, rs read, b write
-        "rhs": (ns + 1) * 4 + nnz_h * (4 + 4 + s) + m * 8 + ns * s,
-        # update: x read, V/theta read+write, pv/pth
+        # V, th read; P, Q write
+        "power": n * 8 * 4 + ybus,
+        # per row: code (1 B), loc, z, sigma read, residual write (FP64), hrow;
+        # FP64 H values write; bus V/th/P/Q, Ybus rows and branch data read once
+        "meas": m * (1 + 4 + 8 + 8 + 8) + (m + 1) * 4 + nnz_h * 8 + n * 8 * 4 + ybus + nb * (4 + 4 + 8 * 5),
+        # tptr, tout; per H entry tpos, trow, value (FP64); residual read; b write
+        "rhs": (ns + 1) * 4 + ns * 4 + nnz_h * (4 + 4 + 8) + m * 8 + ns * s,
+        # update: dx read, V/theta read+write, pth/pv
         "update": ns * s + n * (8 * 4 + 8),
     }
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of a kernel class from the committed counter summary
+    (lines `traffic <class> <bytes>`), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for ln in f:
+                parts = ln.split()
+                if len(parts) == 3 and parts[0] == "traffic" and parts[1] == kernel:
+                    return int(parts[2])
+    except OSError:
+        pass
+    return None
+
+
+def copy_calibration():
+    """A plain device-to-device copy of COPY_BYTES (read COPY_BYTES, write COPY_BYTES)."""
+    import torch
+
+    a = torch.ones(COPY_BYTES // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    del a, b
+    log(f"copy calibration: 3 x {COPY_BYTES} B read + write")
+
+
+def roofline_for(kt, ab, profile_steps):
+    cand = {k: v for k, v in (kt or {}).items() if k in ab and v["launches"]}
+    if not cand:
+        return None
+    dom = max(cand, key=lambda k: cand[k]["ms"])
+    avg_ms = cand[dom]["ms"] / cand[dom]["launches"]
+    achieved = ab[dom] / (avg_ms * 1e-3) / 1e9
+    return dict(kernel=dom, bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(dom), bytes_per_launch=int(ab[dom]),
+                avg_launch_ms=round(avg_ms, 5))
 
 
 def run_form(g, fp64, reqs, batches, warmup, steps, profile_steps, pcg_steps=0):
@@ -120,6 +170,7 @@ def run_form(g, fp64, reqs, batches, warmup, steps, profile_steps, pcg_steps=0):
         r = step(k)
         lat.append((time.perf_counter() - ts) * 1e3)
         iters.append(r["iterations"])
+        objective = r["objective"]
         flags.add(r["flags"])
         ok += r["status"] == 0
     total = time.perf_counter() - t_start
@@ -152,6 +203,7 @@ def run_form(g, fp64, reqs, batches, warmup, steps, profile_steps, pcg_steps=0):
         V=V,
         th=th,
         last_z=last_z,
+        objective=objective,
         kernel_times=kt,
         sizes=sz,
         flags=sorted(flags),
@@ -170,6 +222,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true", help="time the mixed form only")
     ap.add_argument("--pcg-steps", type=int, default=2, help="mixed form: CG steps per Newton step")
+    ap.add_argument("--copy-calibration", action="store_true", help="run a plain device copy first")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs (--gpus > 1) are not supported", file=sys.stderr)
@@ -177,6 +230,8 @@ def main():
 
     from state_estimation_with_cuda_amd import gridgen
 
+    if args.copy_calibration:
+        copy_calibration()
     t0 = time.perf_counter()
     if args.buses == 200_000:
         g = gridgen.workload_200k()
@@ -186,71 +241,67 @@ def main():
     reqs, batches = make_updates(g, args.warmup + max(args.steps, args.profile_steps), 0.10, seed=4)
 
     mixed = run_form(g, False, reqs, batches, args.warmup, args.steps, args.profile_steps, args.pcg_steps)
-    fp64 = None if args.skip_fp64 else run_form(g, True, reqs, batches, args.warmup, args.steps, 0)
+    fp64 = None if args.skip_fp64 else run_form(g, True, reqs, batches, args.warmup, args.steps, args.profile_steps)
+    forms = [f for f in (mixed, fp64) if f]
+    best = max(forms, key=lambda f: f["solves_per_s"])
+    sz = best["sizes"]
 
-    # roofline of the dominant kernel class (HIP-event timed over the profiled pass)
-    kt = mixed["kernel_times"] or {}
-    sz = mixed["sizes"]
-    ab = algorithmic_bytes(sz, g, 4)
-    per_step = {k: v["ms"] / max(args.profile_steps, 1) for k, v in kt.items()}
-    roofline = None
-    if kt:
-        cand = {k: v for k, v in kt.items() if k in ab and v["launches"]}
-        dom = max(cand, key=lambda k: cand[k]["ms"])
-        avg_ms = cand[dom]["ms"] / cand[dom]["launches"]
-        achieved = ab[dom] / (avg_ms * 1e-3) / 1e9
-        roofline = dict(kernel=dom, bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                        frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, bytes_per_launch=int(ab[dom]),
-                        avg_launch_ms=round(avg_ms, 5))
+    def summary(f):
+        kt = f["kernel_times"] or {}
+        ab = algorithmic_bytes(f["sizes"], g, 8 if f["name"] == "fp64" else 4)
+        return {
+            "solves_per_s": round(f["solves_per_s"], 2),
+            "ms_per_step": round(f["ms_per_step"], 4),
+            "p50_ms": round(f["p50_ms"], 4),
+            "p99_ms": round(f["p99_ms"], 4),
+            "max_ms": round(f["max_ms"], 4),
+            "mean_iterations": f["mean_iterations"],
+            "converged_steps": f["converged"],
+            "initial_solve_ms": round(f["initial_solve_ms"], 2),
+            "initial_iterations": f["initial_iterations"],
+            "solve_flags": f["flags"],
+            "plan": f["plan"],
+            "kernel_ms_per_step": {k: round(v["ms"] / max(args.profile_steps, 1), 4) for k, v in kt.items()},
+            "kernel_launches_per_step": {k: round(v["launches"] / max(args.profile_steps, 1), 2)
+                                         for k, v in kt.items()},
+            "roofline": roofline_for(kt, ab, args.profile_steps),
+        }
 
     out = {
         "metric": METRIC,
-        "value": round(mixed["solves_per_s"], 2),
+        "value": round(best["solves_per_s"], 2),
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(mixed["ms_per_step"], 4),
+        "ms_per_step": round(best["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64",
+        "dtype": "f64" if best["name"] == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {
             "workload": "incremental WLS update of a 200k-bus grid: 10% of z replaced (staged in HBM), REALTIME "
                         "Gauss-Newton to ||dx||inf<1e-6 with the cached gain factor",
+            "form": best["name"],
             "buses": g.n,
             "branches": g.nb,
             "measurements": g.m,
             "tolerance": TOL,
         },
-        "p50_ms": round(mixed["p50_ms"], 4),
-        "p99_ms": round(mixed["p99_ms"], 4),
-        "max_ms": round(mixed["max_ms"], 4),
-        "mean_iterations": mixed["mean_iterations"],
-        "converged_steps": mixed["converged"],
-        "initial_solve_ms": round(mixed["initial_solve_ms"], 2),
-        "kernel_ms_per_step": {k: round(v, 4) for k, v in per_step.items()},
-        "kernel_launches_per_step": {k: round(v["launches"] / max(args.profile_steps, 1), 2) for k, v in kt.items()},
-        "solve_flags": mixed["flags"],
-        "plan": mixed["plan"],
+        "p50_ms": round(best["p50_ms"], 4),
+        "p99_ms": round(best["p99_ms"], 4),
         "symbolic": {k: sz[k] for k in ("ns", "nnz_h", "nnz_g", "nnz_l", "n_snodes", "n_levels", "max_front")},
     }
+    for f in forms:
+        out[f["name"]] = summary(f)
     if fp64:
-        out["fp64"] = {
-            "solves_per_s": round(fp64["solves_per_s"], 2),
-            "p50_ms": round(fp64["p50_ms"], 4),
-            "p99_ms": round(fp64["p99_ms"], 4),
-            "mean_iterations": fp64["mean_iterations"],
-            "initial_solve_ms": round(fp64["initial_solve_ms"], 2),
-            "solve_flags": fp64["flags"],
-        }
         out["mixed_vs_fp64"] = round(mixed["solves_per_s"] / fp64["solves_per_s"], 3)
         out["mixed_vs_fp64_state_diff"] = {
             "V": float(np.abs(mixed["V"] - fp64["V"]).max()),
             "theta": float(np.abs(mixed["th"] - fp64["th"]).max()),
         }
-    out["roofline"] = roofline
+    out["roofline"] = out[best["name"]]["roofline"]
 
     if not args.no_cpu_baseline:
         from oracle.oracle import Oracle, num_threads
@@ -270,9 +321,10 @@ def main():
         o.solve(mode=1, max_iter=30, tol=1e-10)
         oV, oth = o.state()
         oJ = o.solve(mode=1, max_iter=1, tol=1.0)["objective"]
-        err = {"mixed": {"V": float(np.abs(mixed["V"] - oV).max()), "theta": float(np.abs(mixed["th"] - oth).max())}}
-        if fp64:
-            err["fp64"] = {"V": float(np.abs(fp64["V"] - oV).max()), "theta": float(np.abs(fp64["th"] - oth).max())}
+        err = {}
+        for f in forms:
+            err[f["name"]] = {"V": float(np.abs(f["V"] - oV).max()), "theta": float(np.abs(f["th"] - oth).max()),
+                              "J_rel": abs(f["objective"] - oJ) / abs(oJ)}
         out["error_vs_oracle"] = err
         out["oracle_objective"] = oJ
         out["cpu_baseline"] = {
