@@ -93,14 +93,14 @@ def algorithmic_bytes(sz, g, prec_bytes):
     }
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of a kernel class from the committed counter summary
-    (lines `traffic <class> <bytes>`), or None."""
+def measured_traffic(form, kernel):
+    """HBM bytes per launch of a kernel class of one form from the committed
+    counter summary (lines `traffic <form>.<class> <bytes>`), or None."""
     try:
         with open(TRAFFIC_FILE) as f:
             for ln in f:
                 parts = ln.split()
-                if len(parts) == 3 and parts[0] == "traffic" and parts[1] == kernel:
+                if len(parts) == 3 and parts[0] == "traffic" and parts[1] == f"{form}.{kernel}":
                     return int(parts[2])
     except OSError:
         pass
@@ -120,7 +120,7 @@ def copy_calibration():
     log(f"copy calibration: 3 x {COPY_BYTES} B read + write")
 
 
-def roofline_for(kt, ab, profile_steps):
+def roofline_for(form, kt, ab):
     cand = {k: v for k, v in (kt or {}).items() if k in ab and v["launches"]}
     if not cand:
         return None
@@ -128,7 +128,7 @@ def roofline_for(kt, ab, profile_steps):
     avg_ms = cand[dom]["ms"] / cand[dom]["launches"]
     achieved = ab[dom] / (avg_ms * 1e-3) / 1e9
     return dict(kernel=dom, bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(dom), bytes_per_launch=int(ab[dom]),
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(form, dom), bytes_per_launch=int(ab[dom]),
                 avg_launch_ms=round(avg_ms, 5))
 
 
@@ -218,7 +218,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--buses", type=int, default=200_000, help="grid size (200000 is the metric's workload)")
     ap.add_argument("--profile-steps", type=int, default=20)
-    ap.add_argument("--cpu-sample", type=int, default=2, help="incremental updates timed on the CPU oracle")
+    ap.add_argument("--cpu-sample", type=int, default=20, help="incremental updates timed on the CPU oracle")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true", help="time the mixed form only")
     ap.add_argument("--pcg-steps", type=int, default=2, help="mixed form: CG steps per Newton step")
@@ -264,7 +264,7 @@ def main():
             "kernel_ms_per_step": {k: round(v["ms"] / max(args.profile_steps, 1), 4) for k, v in kt.items()},
             "kernel_launches_per_step": {k: round(v["launches"] / max(args.profile_steps, 1), 2)
                                          for k, v in kt.items()},
-            "roofline": roofline_for(kt, ab, args.profile_steps),
+            "roofline": roofline_for(f["name"], kt, ab),
         }
 
     out = {

@@ -45,6 +45,11 @@ typedef struct SLE_ExtResult {
  *                     chunks with the L panels resident in LDS
  *   "lds_waves_bottom", "lds_waves_tail"  waves per workgroup of the LDS
  *                     subtree tasks and top chunks (default 16 each)
+ *   "pinv"            1 (default): the top of the supernodal tree is solved with
+ *                     explicit blocks of L^-1 (partitioned inverse, recomputed
+ *                     with every factorization) instead of level sweeps
+ *   "pinv_cut"        fronts per bottom subtree under it (0: default, 32)
+ *   "pinv_waves"      waves per bottom-subtree workgroup under it (default 4)
  *   "level_schedule"  1 (default): factor and solve level by level (kernel
  *                     boundaries order the fronts); 0: one dependency-driven
  *                     launch per sweep with release/acquire flags per front */

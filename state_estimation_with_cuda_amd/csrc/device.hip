@@ -232,6 +232,8 @@ struct MeasArgs {
     const double* tap;
     const double* shift;
     double* rs;
+    double* hr;            // optional: (sqrt(w) H)_q (sqrt(w) r)_i at the H^T position of q
+    const int32_t* hpos;   // H^T position of each H entry
     double* hout;
     double* rout;
     double* jpart;
@@ -241,6 +243,12 @@ struct MeasArgs {
 
 __global__ __launch_bounds__(kBlock) void k_meas(MeasArgs A, double* __restrict__ hs) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
+    double rsi = 0.0;
+    // one Jacobian entry: row-major value and/or its product with sqrt(w) r
+    auto put = [&](int q, double v) {
+        if (A.want_jac) hs[q] = v;
+        if (A.hr) A.hr[A.hpos[q]] = v * rsi;
+    };
     double jw = 0.0, rn = -1.0;
     int ri = -1;
     if (i < A.m) {
@@ -318,7 +326,8 @@ __global__ __launch_bounds__(kBlock) void k_meas(MeasArgs A, double* __restrict_
         }
         if (!act) w = 0.0;
         const double sw = A.unit_weights ? 1.0 : sqrt(w);
-        A.rs[i] = sw * r;
+        rsi = sw * r;
+        A.rs[i] = rsi;
         if (A.hout) {
             A.hout[i] = act ? hv : 0.0;
             A.rout[i] = r;
@@ -328,9 +337,9 @@ __global__ __launch_bounds__(kBlock) void k_meas(MeasArgs A, double* __restrict_
             rn = fabs(r) / sg;
             ri = i;
         }
-        if (A.want_jac && q1 > q0) {
+        if ((A.want_jac || A.hr) && q1 > q0) {
             if (t == 0 || t == 1) {
-                hs[q0] = sw;
+                put(q0, sw);
             } else if (t == 2 || t == 3 || t == 7 || t == 8) {
                 const bool isP = (t == 2 || t == 7);
                 const int e0 = A.yrow[l], e1 = A.yrow[l + 1];
@@ -359,24 +368,24 @@ __global__ __launch_bounds__(kBlock) void k_meas(MeasArgs A, double* __restrict_
                             dv = qe / Vj;
                         }
                     }
-                    if (j != A.slack) hs[q0 + kth++] = sw * dth;
-                    hs[q0 + nth + (e - e0)] = sw * dv;
+                    if (j != A.slack) put(q0 + kth++, sw * dth);
+                    put(q0 + nth + (e - e0), sw * dv);
                 }
             } else {
                 // sorted columns: theta_min, theta_max, V_min, V_max (slack theta absent)
                 int q = q0;
                 if (fb == tb) {
-                    if (fb != A.slack) hs[q++] = sw * (d[0] + d[1]);
-                    hs[q] = sw * (d[2] + d[3]);
+                    if (fb != A.slack) put(q++, sw * (d[0] + d[1]));
+                    put(q, sw * (d[2] + d[3]));
                 } else {
                     const bool fl = (code >> 6) & 1;
                     int lo = fl ? fb : tb, hi = fl ? tb : fb;
                     double dlo_t = fl ? d[0] : d[1], dhi_t = fl ? d[1] : d[0];
                     double dlo_v = fl ? d[2] : d[3], dhi_v = fl ? d[3] : d[2];
-                    if (lo != A.slack) hs[q++] = sw * dlo_t;
-                    if (hi != A.slack) hs[q++] = sw * dhi_t;
-                    hs[q++] = sw * dlo_v;
-                    hs[q] = sw * dhi_v;
+                    if (lo != A.slack) put(q++, sw * dlo_t);
+                    if (hi != A.slack) put(q++, sw * dhi_t);
+                    put(q++, sw * dlo_v);
+                    put(q, sw * dhi_v);
                 }
             }
         }
@@ -882,6 +891,160 @@ __global__ __launch_bounds__(1024) void k_bwd_lds(LdsTaskArgs A, SolveArgs<T> SA
 }
 
 // ---------------------------------------------------------------------------
+// Partitioned-inverse top solve (the "partitioned sparse inverse" of Alvarado,
+// Yu and Betancourt, IEEE Trans. Power Systems 5(2), 1990).  The supernodal
+// tree is split into bottom subtrees (solved by the LDS tasks above) and the
+// top T, which is closed under taking parents.  Because no bottom column is an
+// ancestor of a top column, L = [L_BB 0; L_TB L_TT] and
+//   forward:  c_T = b_T + (update vectors of the bottom roots),  y_T = L_TT^-1 c_T
+//   backward: x_T = L_TT^-T y_T,  then the bottom tasks finish x_B.
+// L_TT^-1 is stored explicitly: column j is non-zero only on the ancestors of j
+// (its path to the root), so each top supernode s owns a dense block of
+// P_s x k_s values (P_s = columns on its path), recomputed after every
+// factorization by walking the path once (k_pi_inv).  The two triangular
+// sweeps of the top -- dozens of dependent levels, each a launch or a barrier
+// -- become two independent gathers.
+// ---------------------------------------------------------------------------
+struct PiArgs {
+    const int32_t* sn;      // top supernodes, ascending (children before parents)
+    const int32_t* plen;    // P_s, path rows of each top supernode
+    const int64_t* boff;    // offset of each supernode's block
+    const int64_t* poff;    // offset of each supernode's path rows in ploc
+    const int32_t* ploc;    // top-local index of each path row
+    const int32_t* parent;  // supernode parent (-1 at a root)
+    int ntop;               // top supernodes
+    int slice;              // LDS elements of one per-wave buffer (cmax * kmax)
+};
+
+// One wave per top supernode s: the k_s unit right-hand sides of its columns are
+// eliminated front by front along the path (extend-add through rmap, as in the
+// forward solve), and each front's solved rows are the block rows of L_TT^-1.
+template <typename T>
+__global__ __launch_bounds__(256) void k_pi_inv(PiArgs A, SolveArgs<T> SA, T* __restrict__ B) {
+    extern __shared__ unsigned char smem_raw[];
+    const int nw = blockDim.x / kWave, w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const int u = blockIdx.x * nw + w;
+    if (u >= A.ntop) return;  // the whole wave leaves; no workgroup barrier below
+    T* F0 = reinterpret_cast<T*>(smem_raw) + (size_t)w * 3 * A.slice;
+    T* F1 = F0 + A.slice;
+    T* Lp = F1 + A.slice;
+    const int s0 = A.sn[u];
+    const int ks = SA.sn_k[s0];
+    const int Ps = A.plen[u];
+    T* Bs = B + A.boff[u];
+    int p = s0;
+    int c = SA.sn_c[p];
+    // F is c x ks, row-major: F[i * ks + jj]
+    for (int e = lane; e < c * ks; e += kWave) {
+        const int i = e / ks, jj = e - i * ks;
+        F0[e] = i == jj ? T(1) : T(0);
+    }
+    int ro = 0;
+    for (;;) {
+        const int k = SA.sn_k[p];
+        const T* Lg = SA.F + SA.foff[p];
+        for (int e = lane; e < c * k; e += kWave) Lp[e] = Lg[e];  // first k columns, col-major
+        wave_sync();
+        for (int j = 0; j < k; ++j) {
+            const T d = Lp[j * c + j];
+            for (int jj = lane; jj < ks; jj += kWave) F0[j * ks + jj] /= d;
+            wave_sync();
+            const int rest = (c - j - 1) * ks;
+            for (int e = lane; e < rest; e += kWave) {
+                const int i = j + 1 + e / ks, jj = e % ks;
+                F0[i * ks + jj] -= Lp[j * c + i] * F0[j * ks + jj];
+            }
+            wave_sync();
+        }
+        for (int e = lane; e < k * ks; e += kWave) {
+            const int j = e / ks, jj = e - j * ks;
+            Bs[(ro + j) + (int64_t)Ps * jj] = F0[e];
+        }
+        ro += k;
+        const int q = A.parent[p];
+        if (q < 0) break;
+        const int cq = SA.sn_c[q];
+        for (int e = lane; e < cq * ks; e += kWave) F1[e] = T(0);
+        wave_sync();
+        const int32_t* rm = SA.rmap + SA.rmoff[p];
+        for (int e = lane; e < (c - k) * ks; e += kWave) {
+            const int a = e / ks, jj = e - a * ks;
+            F1[rm[a] * ks + jj] = F0[(k + a) * ks + jj];
+        }
+        wave_sync();
+        T* t = F0;
+        F0 = F1;
+        F1 = t;
+        p = q;
+        c = cq;
+    }
+}
+
+// Row-grouped copy of the blocks for the forward product.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pi_gather(int64_t nq, const int64_t* __restrict__ pos,
+                                                     const T* __restrict__ B, T* __restrict__ val) {
+    const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (q < nq) val[q] = B[pos[q]];
+}
+
+// c_T = b_T + update vectors of the bottom subtree roots (fixed order per row).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pi_assemble(int ntc, const int32_t* __restrict__ tcol,
+                                                       const int32_t* __restrict__ cptr,
+                                                       const int64_t* __restrict__ cpos, const T* __restrict__ x,
+                                                       const T* __restrict__ ubuf, T* __restrict__ ctop) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= ntc) return;
+    T acc = x[tcol[t]];
+    for (int q = cptr[t]; q < cptr[t + 1]; ++q) acc += ubuf[cpos[q]];
+    ctop[t] = acc;
+}
+
+// Forward product y_T = L_TT^-1 c_T: one wave per segment of a row's entries.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pi_fwd(int nseg, const int64_t* __restrict__ segq,
+                                                  const int32_t* __restrict__ src, const T* __restrict__ val,
+                                                  const T* __restrict__ ctop, double* __restrict__ part) {
+    const int g = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    if (g >= nseg) return;
+    double acc = 0.0;
+    for (int64_t q = segq[g] + lane; q < segq[g + 1]; q += kWave) acc += (double)val[q] * (double)ctop[src[q]];
+    acc = wave_sum(acc);
+    if (lane == 0) part[g] = acc;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pi_rowsum(int ntc, const int32_t* __restrict__ segptr,
+                                                     const double* __restrict__ part, T* __restrict__ ytop) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= ntc) return;
+    double acc = 0.0;
+    for (int g = segptr[t]; g < segptr[t + 1]; ++g) acc += part[g];
+    ytop[t] = (T)acc;
+}
+
+// Backward product x_T = L_TT^-T y_T: one wave per top supernode, one dot
+// product over its path rows per column.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pi_bwd(PiArgs A, SolveArgs<T> SA, const T* __restrict__ B,
+                                                  const T* __restrict__ ytop) {
+    const int u = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    if (u >= A.ntop) return;
+    const int s = A.sn[u], ks = SA.sn_k[s], first = SA.sn_first[s], Ps = A.plen[u];
+    const T* Bs = B + A.boff[u];
+    const int32_t* pl = A.ploc + A.poff[u];
+    for (int jj = 0; jj < ks; ++jj) {
+        double acc = 0.0;
+        for (int r = jj + lane; r < Ps; r += kWave) acc += (double)Bs[r + (int64_t)Ps * jj] * (double)ytop[pl[r]];
+        acc = wave_sum(acc);
+        if (lane == 0) SA.x[first + jj] = (T)acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Dependency-driven sweeps: one launch covers the whole supernodal tree.
 // Block b handles front order[b]; `order` lists every child before its parent
 // (forward, factor) or every parent before its children (backward).  A front
@@ -968,6 +1131,19 @@ __global__ __launch_bounds__(kBlock) void k_rhs(int ns, const int32_t* __restric
     if (t >= ns) return;
     double acc = 0.0;
     for (int p = tptr[t]; p < tptr[t + 1]; ++p) acc += hs[tpos[p]] * rs[trow[p]];
+    x[tout[t]] = (TO)acc;
+}
+
+// b = H^T W r from the products k_meas wrote in H^T order: a contiguous
+// segmented sum per state (states in bus-layout order, written to x[tout[t]]).
+template <typename TO>
+__global__ __launch_bounds__(kBlock) void k_rhs_sum(int ns, const int32_t* __restrict__ tptr,
+                                                    const int32_t* __restrict__ tout, const double* __restrict__ hr,
+                                                    TO* __restrict__ x) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= ns) return;
+    double acc = 0.0;
+    for (int p = tptr[t]; p < tptr[t + 1]; ++p) acc += hr[p];
     x[tout[t]] = (TO)acc;
 }
 
@@ -1182,22 +1358,30 @@ struct DBuf {
     }
 };
 
+// Sizes of the partitioned-inverse buffers (zero when that solve is off).
+struct PiSizes {
+    int64_t nblock = 0, nq = 0;
+    int ntc = 0;
+};
+
 template <typename T>
 struct PrecBufs {
     DBuf<T> Gval, F, x, ubuf;
+    DBuf<T> pinv, pinv_rows, ctop, ytop;  // L_TT^-1 blocks, row-grouped copy, top vectors
     bool ready = false;
-    void alloc(const Symbolic& S, size_t& bytes) {
+    void alloc(const Symbolic& S, size_t& bytes, const PiSizes& pz) {
         Gval.alloc(std::max<int64_t>(1, S.nnz_g), bytes);
         F.alloc(std::max<int64_t>(1, S.ftotal), bytes);
         x.alloc(std::max(1, S.ns), bytes);
         ubuf.alloc(std::max<int64_t>(1, S.utotal), bytes);
+        pinv.alloc(std::max<int64_t>(1, pz.nblock), bytes);
+        pinv_rows.alloc(std::max<int64_t>(1, pz.nq), bytes);
+        ctop.alloc(std::max(1, pz.ntc), bytes);
+        ytop.alloc(std::max(1, pz.ntc), bytes);
         ready = true;
     }
     void release(size_t& bytes) {
-        Gval.release(bytes);
-        F.release(bytes);
-        x.release(bytes);
-        ubuf.release(bytes);
+        for (DBuf<T>* b : {&Gval, &F, &x, &ubuf, &pinv, &pinv_rows, &ctop, &ytop}) b->release(bytes);
         ready = false;
     }
 };
@@ -1447,6 +1631,106 @@ static bool make_lds_plan(const Symbolic& S, const SweepPlan& SP, size_t elem, s
     return true;
 }
 
+// Partitioned-inverse plan (host): bottom subtrees of at most `cut` fronts, the
+// rest is the top T.  Block, path, forward-product and assembly index lists.
+struct PiPlan {
+    int cut = 0;
+    int ntop = 0, ntc = 0, nseg = 0, slice = 0;
+    int64_t nblock = 0, nq = 0;
+    std::vector<int32_t> sn, plen, ploc, tcol, src, segptr, cptr, parent;
+    std::vector<int64_t> boff, poff, pos, segq, cpos;
+};
+
+constexpr int kPiSeg = 256;  // entries per forward-product segment
+constexpr int kPiCutDefault = 32;  // fronts per bottom subtree
+
+static PiPlan make_pi_plan(const Symbolic& S, int cut) {
+    PiPlan P;
+    P.cut = cut;
+    const int nsn = S.nsn;
+    std::vector<int32_t> size(nsn, 1);
+    for (int s = 0; s < nsn; ++s)
+        if (S.sn_parent[s] >= 0) size[S.sn_parent[s]] += size[s];
+    std::vector<int32_t> tloc(S.ns, -1), tsl(nsn, -1);
+    int cmax = 1, kmax = 1;
+    for (int s = 0; s < nsn; ++s) {
+        if (size[s] <= cut) continue;
+        tsl[s] = (int32_t)P.sn.size();
+        P.sn.push_back(s);
+        for (int j = 0; j < S.sn_k[s]; ++j) {
+            tloc[S.sn_first[s] + j] = (int32_t)P.tcol.size();
+            P.tcol.push_back(S.sn_first[s] + j);
+        }
+        cmax = std::max(cmax, S.sn_c[s]);
+        kmax = std::max(kmax, S.sn_k[s]);
+    }
+    P.ntop = (int)P.sn.size();
+    P.ntc = (int)P.tcol.size();
+    P.slice = cmax * kmax;
+    P.parent = S.sn_parent;
+    // blocks and path rows
+    P.plen.resize(P.ntop);
+    P.boff.resize(P.ntop);
+    P.poff.resize(P.ntop);
+    for (int u = 0; u < P.ntop; ++u) {
+        const int s0 = P.sn[u];
+        P.boff[u] = P.nblock;
+        P.poff[u] = (int64_t)P.ploc.size();
+        int len = 0;
+        for (int p = s0; p >= 0; p = S.sn_parent[p])
+            for (int j = 0; j < S.sn_k[p]; ++j, ++len) P.ploc.push_back(tloc[S.sn_first[p] + j]);
+        P.plen[u] = len;
+        P.nblock += (int64_t)len * S.sn_k[s0];
+    }
+    // forward product grouped by row: entry (u, r, jj) lands in row ploc[r]
+    std::vector<int64_t> rcnt(P.ntc + 1, 0);
+    for (int u = 0; u < P.ntop; ++u) {
+        const int ks = S.sn_k[P.sn[u]];
+        for (int r = 0; r < P.plen[u]; ++r) rcnt[P.ploc[P.poff[u] + r] + 1] += std::min(r + 1, ks);
+    }
+    for (int t = 0; t < P.ntc; ++t) rcnt[t + 1] += rcnt[t];
+    P.nq = rcnt[P.ntc];
+    P.pos.resize(P.nq);
+    P.src.resize(P.nq);
+    std::vector<int64_t> fill(rcnt.begin(), rcnt.end() - 1);
+    for (int u = 0; u < P.ntop; ++u) {
+        const int s0 = P.sn[u], ks = S.sn_k[s0], Ps = P.plen[u];
+        for (int r = 0; r < Ps; ++r) {
+            const int t = P.ploc[P.poff[u] + r];
+            for (int jj = 0; jj < std::min(r + 1, ks); ++jj) {
+                const int64_t q = fill[t]++;
+                P.pos[q] = P.boff[u] + r + (int64_t)Ps * jj;
+                P.src[q] = tloc[S.sn_first[s0] + jj];
+            }
+        }
+    }
+    P.segptr.assign(P.ntc + 1, 0);
+    for (int t = 0; t < P.ntc; ++t) {
+        for (int64_t q = rcnt[t]; q < rcnt[t + 1]; q += kPiSeg) P.segq.push_back(q);
+        P.segptr[t + 1] = (int32_t)P.segq.size();
+    }
+    P.nseg = (int)P.segq.size();
+    P.segq.push_back(P.nq);
+    // assembly: update rows of the bottom subtree roots, per top column
+    std::vector<std::vector<int64_t>> per(P.ntc);
+    for (int s = 0; s < nsn; ++s) {
+        const int par = S.sn_parent[s];
+        if (tsl[s] >= 0 || par < 0 || tsl[par] < 0) continue;
+        const int c = S.sn_c[s], k = S.sn_k[s];
+        for (int a = 0; a < c - k; ++a) {
+            const int t = tloc[S.sn_rows[S.sn_rows_off[s] + k + a]];
+            if (t < 0) throw std::logic_error("partitioned inverse: update row outside the top");
+            per[t].push_back(S.sn_uoff[s] + a);
+        }
+    }
+    P.cptr.assign(P.ntc + 1, 0);
+    for (int t = 0; t < P.ntc; ++t) {
+        P.cpos.insert(P.cpos.end(), per[t].begin(), per[t].end());
+        P.cptr[t + 1] = (int32_t)P.cpos.size();
+    }
+    return P;
+}
+
 // ---------------------------------------------------------------------------
 // Instance
 // ---------------------------------------------------------------------------
@@ -1478,6 +1762,8 @@ struct Instance::Impl {
     DBuf<double> zval;
     // symbolic structures
     DBuf<int32_t> tptr, tpos, trow, tout, gptr, gloc, gdiag, g_sn_ptr;
+    DBuf<int32_t> hpos;  // H^T position of each device H entry
+    DBuf<double> hr;     // (sqrt(w) H)(sqrt(w) r) products in H^T order
     // device data layout: buses in Cuthill-McKee order from the slack, branches
     // and measurement rows sorted by the layout position of their (from) bus
     std::vector<int32_t> lb, bl, lk, kl, lm, ml;
@@ -1519,6 +1805,28 @@ struct Instance::Impl {
     } lds[2];  // [0] float, [1] double
     bool use_lds = true;
     int lds_waves_bottom = 16, lds_waves_tail = 16;
+    // partitioned-inverse top solve: bottom subtrees in LDS tasks, L_TT^-1 products on top
+    struct PiDev {
+        bool ok = false;
+        int cut = 0, ntop = 0, ntc = 0, nseg = 0, slice = 0;
+        int64_t nblock = 0, nq = 0;
+        DBuf<int32_t> sn, plen, ploc, tcol, src, segptr, cptr, parent;
+        DBuf<int64_t> boff, poff, pos, segq, cpos;
+        DBuf<double> part;
+        LdsDev bottom[2];  // bottom-subtree LDS tasks, [0] float, [1] double
+        PiSizes sizes() const {
+            PiSizes z;
+            if (ok) {
+                z.nblock = nblock;
+                z.nq = nq;
+                z.ntc = ntc;
+            }
+            return z;
+        }
+    } pi;
+    bool use_pi = true;
+    int pi_cut = 0;    // fronts per bottom subtree; 0 = default
+    int pi_waves = 4;  // waves per bottom-subtree workgroup under the partitioned inverse
     // dependency-driven sweeps (all fronts fit in LDS)
     bool use_flow = false;
     bool flow_ok = false;       // every front fits the LDS front of the dataflow kernels
@@ -1567,7 +1875,7 @@ Instance::~Instance() {
     rel(d_->hs); rel(d_->hcolp); rel(d_->cg_b); rel(d_->cg_x); rel(d_->cg_r); rel(d_->cg_p); rel(d_->cg_q);
     rel(d_->cg_z); rel(d_->cg_t); rel(d_->cg_parts); rel(d_->cg_s); rel(d_->z); rel(d_->sigma); rel(d_->rs); rel(d_->hx); rel(d_->r); rel(d_->jpart); rel(d_->rpart);
     rel(d_->mpart); rel(d_->ipart); rel(d_->zidx); rel(d_->zval); rel(d_->tptr); rel(d_->tpos); rel(d_->trow);
-    rel(d_->tout); rel(d_->ml_d); rel(d_->ztmp);
+    rel(d_->tout); rel(d_->ml_d); rel(d_->ztmp); rel(d_->hpos); rel(d_->hr);
     rel(d_->gptr); rel(d_->gloc); rel(d_->gdiag); rel(d_->g_sn_ptr); rel(d_->gpair); rel(d_->sn_first);
     rel(d_->sn_k); rel(d_->sn_c); rel(d_->child_ptr); rel(d_->child); rel(d_->rmap); rel(d_->rows);
     rel(d_->perm); rel(d_->pth); rel(d_->pv); rel(d_->lists); rel(d_->foff); rel(d_->rmoff); rel(d_->uoff);
@@ -1575,6 +1883,14 @@ Instance::~Instance() {
     rel(d_->t_fronts); rel(d_->t_lvl_ptr); rel(d_->t_lvl_off); rel(d_->t_mid);
     for (auto& L : d_->lds) {
         rel(L.th); rel(L.meta); rel(L.lvl); rel(L.ch); rel(L.rm); rel(L.src); rel(L.mid);
+    }
+    {
+        auto& P = d_->pi;
+        rel(P.sn); rel(P.plen); rel(P.ploc); rel(P.tcol); rel(P.src); rel(P.segptr); rel(P.cptr); rel(P.parent);
+        rel(P.boff); rel(P.poff); rel(P.pos); rel(P.segq); rel(P.cpos); rel(P.part);
+        for (auto& L : P.bottom) {
+            rel(L.th); rel(L.meta); rel(L.lvl); rel(L.ch); rel(L.rm); rel(L.src); rel(L.mid);
+        }
     }
     d_->f32.release(b);
     d_->f64.release(b);
@@ -1781,14 +2097,22 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
         std::vector<int32_t> tp(S.ns + 1, 0), tq, tr;
         tq.reserve(S.tpos.size());
         tr.reserve(S.tpos.size());
+        std::vector<std::pair<int32_t, int32_t>> ent;
         for (int t = 0; t < S.ns; ++t) {
             const int k = sorder[t];
-            for (int p = S.tptr[k]; p < S.tptr[k + 1]; ++p) {
-                tq.push_back(D.qmap[S.tpos[p]]);
-                tr.push_back(D.lm[S.trow[p]]);
+            ent.clear();
+            for (int p = S.tptr[k]; p < S.tptr[k + 1]; ++p) ent.emplace_back(D.lm[S.trow[p]], D.qmap[S.tpos[p]]);
+            std::sort(ent.begin(), ent.end());  // by device row: neighbouring rows write neighbouring slots
+            for (const auto& e : ent) {
+                tq.push_back(e.second);
+                tr.push_back(e.first);
             }
             tp[t + 1] = (int32_t)tq.size();
         }
+        std::vector<int32_t> hp(std::max<size_t>(1, S.hcol.size()), 0);
+        for (size_t p = 0; p < S.hcol.size(); ++p) hp[tq[p]] = (int32_t)p;
+        D.hpos.upload(hp, bytes_, s);
+        D.hr.alloc(std::max<size_t>(1, S.hcol.size()), bytes_);
         if (tq.empty()) {
             tq.push_back(0);
             tr.push_back(0);
@@ -1937,6 +2261,67 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
         L.wcut = bw;
         L.ok = true;
     }
+    // partitioned-inverse top solve: bottom LDS tasks of at most pi_cut fronts
+    D.pi.ok = false;
+    if (S.nsn > 0 && S.max_c <= kFrontB) {
+        const int cut = D.pi_cut > 0 ? D.pi_cut : kPiCutDefault;
+        const size_t budget = 64 * 1024;
+        SweepPlan SP = make_plan(S, cut, -1, D.pi_waves, D.lds_waves_tail, true);
+        LdsPlan LP[2];
+        bool fit = true;
+        for (int e = 0; e < 2; ++e)
+            fit = fit && make_lds_plan(S, SP, e ? sizeof(double) : sizeof(float), budget, D.pi_waves,
+                                       D.lds_waves_tail, true, LP[e]);
+        PiPlan PP;
+        if (fit) PP = make_pi_plan(S, cut);
+        if (fit && 3 * (size_t)PP.slice * sizeof(double) <= 65536 && PP.nblock < ((int64_t)1 << 40)) {
+            auto up = [&](DBuf<int32_t>& b, std::vector<int32_t> v) {
+                if (v.empty()) v.push_back(0);
+                b.upload(v, bytes_, s);
+            };
+            auto up64 = [&](DBuf<int64_t>& b, std::vector<int64_t> v) {
+                if (v.empty()) v.push_back(0);
+                b.upload(v, bytes_, s);
+            };
+            for (int e = 0; e < 2; ++e) {
+                Impl::LdsDev& L = D.pi.bottom[e];
+                up(L.th, LP[e].th);
+                up(L.meta, LP[e].meta);
+                up(L.lvl, LP[e].lvl);
+                up(L.ch, LP[e].ch);
+                up(L.rm, LP[e].rm);
+                up(L.src, LP[e].src);
+                L.n_bottom = LP[e].n_bottom;
+                L.bottom_bytes = LP[e].bottom_bytes;
+                L.cut = cut;
+                L.ok = true;
+            }
+            Impl::PiDev& Q = D.pi;
+            up(Q.sn, PP.sn);
+            up(Q.plen, PP.plen);
+            up(Q.ploc, PP.ploc);
+            up(Q.tcol, PP.tcol);
+            up(Q.src, PP.src);
+            up(Q.segptr, PP.segptr);
+            up(Q.cptr, PP.cptr);
+            up(Q.parent, PP.parent);
+            up64(Q.boff, PP.boff);
+            up64(Q.poff, PP.poff);
+            up64(Q.pos, PP.pos);
+            up64(Q.segq, PP.segq);
+            up64(Q.cpos, PP.cpos);
+            Q.part.alloc(std::max(1, PP.nseg), bytes_);
+            HIPCHECK(hipStreamSynchronize(s));
+            Q.cut = cut;
+            Q.ntop = PP.ntop;
+            Q.ntc = PP.ntc;
+            Q.nseg = PP.nseg;
+            Q.slice = PP.slice;
+            Q.nblock = PP.nblock;
+            Q.nq = PP.nq;
+            Q.ok = true;
+        }
+    }
     // dependency-driven sweeps: level order lists children before parents
     D.flow_ok = S.nsn > 0 && S.max_c <= kFrontB;
     D.use_flow = D.flow_ok && !D.force_levels;
@@ -1956,10 +2341,10 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
     // precision buffers
     if (precision_ == 1) {
         d_->f32.release(bytes_);
-        d_->f64.alloc(S, bytes_);
+        d_->f64.alloc(S, bytes_, d_->pi.sizes());
     } else {
         d_->f64.release(bytes_);
-        d_->f32.alloc(S, bytes_);
+        d_->f32.alloc(S, bytes_, d_->pi.sizes());
     }
     fp64_factor_active_ = false;
     factor_valid_ = false;
@@ -2087,7 +2472,8 @@ void launch_power(Instance::Impl& D) {
                                                 D.Q.p, D.Pij.p, D.Qij.p, D.st.p);
 }
 
-void launch_meas(Instance::Impl& D, int robust, double gamma, int unit_weights, int want_jac, bool want_out) {
+void launch_meas(Instance::Impl& D, int robust, double gamma, int unit_weights, int want_jac, bool want_out,
+                 bool want_rhs = false) {
     MeasArgs A;
     A.m = D.m;
     A.n = D.n;
@@ -2119,6 +2505,8 @@ void launch_meas(Instance::Impl& D, int robust, double gamma, int unit_weights, 
     A.tap = D.tap.p;
     A.shift = D.shift.p;
     A.rs = D.rs.p;
+    A.hr = want_rhs ? D.hr.p : nullptr;
+    A.hpos = D.hpos.p;
     A.hout = want_out ? D.hx.p : nullptr;
     A.rout = want_out ? D.r.p : nullptr;
     A.jpart = D.jpart.p;
@@ -2135,7 +2523,7 @@ void launch_spgemm(Instance::Impl& D, const Symbolic& S) {
 }
 
 template <typename T>
-void launch_factor(Instance::Impl& D, T piv_tol) {
+void launch_factor_sweeps(Instance::Impl& D, T piv_tol) {
     PrecBufs<T>& B = bufs<T>(D);
     FactArgs<T> A;
     A.sn_first = D.sn_first.p;
@@ -2200,6 +2588,33 @@ SolveArgs<T> solve_args(Instance::Impl& D) {
     return A;
 }
 
+PiArgs pi_args(const Instance::Impl& D) {
+    const Instance::Impl::PiDev& P = D.pi;
+    return PiArgs{P.sn.p, P.plen.p, P.boff.p, P.poff.p, P.ploc.p, P.parent.p, P.ntop, P.slice};
+}
+
+bool pi_active(const Instance::Impl& D) { return D.use_pi && D.pi.ok; }
+
+// L_TT^-1 blocks of the current factor and their row-grouped copy
+template <typename T>
+void launch_pi_inverse(Instance::Impl& D) {
+    Instance::Impl::PiDev& PI = D.pi;
+    if (!pi_active(D) || PI.ntop == 0) return;
+    PrecBufs<T>& B = bufs<T>(D);
+    const size_t per_wave = 3 * (size_t)PI.slice * sizeof(T);
+    const int wpb = (int)std::max<size_t>(1, std::min<size_t>(4, 65536 / per_wave));
+    k_pi_inv<T><<<(PI.ntop + wpb - 1) / wpb, kWave * wpb, wpb * per_wave, D.stream>>>(pi_args(D), solve_args<T>(D),
+                                                                                     B.pinv.p);
+    k_pi_gather<T><<<(unsigned)((PI.nq + kBlock - 1) / kBlock), kBlock, 0, D.stream>>>(PI.nq, PI.pos.p, B.pinv.p,
+                                                                                       B.pinv_rows.p);
+}
+
+template <typename T>
+void launch_factor(Instance::Impl& D, T piv_tol) {
+    launch_factor_sweeps<T>(D, piv_tol);
+    launch_pi_inverse<T>(D);
+}
+
 struct Prof;
 void prof_begin(Prof* pf, int ph);
 void prof_end(Prof* pf, int ph, int launches);
@@ -2207,6 +2622,32 @@ void prof_end(Prof* pf, int ph, int launches);
 template <typename T>
 void launch_trisolve(Instance::Impl& D, Prof* pf = nullptr) {
     SolveArgs<T> A = solve_args<T>(D);
+    if (!D.use_flow && pi_active(D)) {
+        // bottom subtrees in LDS tasks, the top by the partitioned inverse
+        const Instance::Impl::PiDev& PI = D.pi;
+        const Instance::Impl::LdsDev& PB = PI.bottom[sizeof(T) == sizeof(double) ? 1 : 0];
+        LdsTaskArgs LA{PB.th.p, PB.meta.p, PB.lvl.p, PB.ch.p, PB.rm.p, PB.src.p};
+        PrecBufs<T>& B = bufs<T>(D);
+        const int nw = kBlock / kWave;
+        prof_begin(pf, PH_FWD);
+        if (PB.n_bottom)
+            k_fwd_lds<T><<<PB.n_bottom, kWave * D.pi_waves, PB.bottom_bytes, D.stream>>>(LA, A, 0);
+        if (PI.ntc) {
+            k_pi_assemble<T><<<nblk(PI.ntc), kBlock, 0, D.stream>>>(PI.ntc, PI.tcol.p, PI.cptr.p, PI.cpos.p, A.x,
+                                                                   A.ubuf, B.ctop.p);
+            k_pi_fwd<T><<<(PI.nseg + nw - 1) / nw, kBlock, 0, D.stream>>>(PI.nseg, PI.segq.p, PI.src.p,
+                                                                           B.pinv_rows.p, B.ctop.p, PI.part.p);
+            k_pi_rowsum<T><<<nblk(PI.ntc), kBlock, 0, D.stream>>>(PI.ntc, PI.segptr.p, PI.part.p, B.ytop.p);
+        }
+        prof_end(pf, PH_FWD, (PB.n_bottom ? 1 : 0) + (PI.ntc ? 3 : 0));
+        prof_begin(pf, PH_BWD);
+        if (PI.ntc)
+            k_pi_bwd<T><<<(PI.ntop + nw - 1) / nw, kBlock, 0, D.stream>>>(pi_args(D), A, B.pinv.p, B.ytop.p);
+        if (PB.n_bottom)
+            k_bwd_lds<T><<<PB.n_bottom, kWave * D.pi_waves, PB.bottom_bytes, D.stream>>>(LA, A, 0);
+        prof_end(pf, PH_BWD, (PB.n_bottom ? 1 : 0) + (PI.ntc ? 1 : 0));
+        return;
+    }
     Instance::Impl::LdsDev& LD = D.lds[sizeof(T) == sizeof(double) ? 1 : 0];
     if (!D.use_flow && D.use_lds && LD.ok) {
         LdsTaskArgs LA{LD.th.p, LD.meta.p, LD.lvl.p, LD.ch.p, LD.rm.p, LD.src.p};
@@ -2287,9 +2728,7 @@ void launch_trisolve(Instance::Impl& D, Prof* pf = nullptr) {
 
 template <typename T>
 void launch_rhs(Instance::Impl& D) {
-    if (D.ns)
-        k_rhs<T><<<nblk(D.ns), kBlock, 0, D.stream>>>(D.ns, D.tptr.p, D.tpos.p, D.trow.p, D.tout.p, D.hs.p, D.rs.p,
-                                                      bufs<T>(D).x.p);
+    if (D.ns) k_rhs_sum<T><<<nblk(D.ns), kBlock, 0, D.stream>>>(D.ns, D.tptr.p, D.tout.p, D.hr.p, bufs<T>(D).x.p);
 }
 
 // dx source: the factor-precision solve vector, or the FP64 CG solution
@@ -2340,9 +2779,10 @@ void launch_pcg(Instance::Impl& D, int steps) {
 }
 
 int factor_launches(const Instance::Impl& D) {
-    if (D.use_flow) return 1;
-    if (D.use_tasks && D.tasks_ok) return (D.n_bottom ? 1 : 0) + (int)D.mid_off.size() + (D.has_tail ? 1 : 0);
-    return D.level_launches;
+    const int inv = pi_active(D) && D.pi.ntop ? 2 : 0;
+    if (D.use_flow) return 1 + inv;
+    if (D.use_tasks && D.tasks_ok) return (D.n_bottom ? 1 : 0) + (int)D.mid_off.size() + (D.has_tail ? 1 : 0) + inv;
+    return D.level_launches + inv;
 }
 
 void read_status(Instance::Impl& D) {
@@ -2390,7 +2830,10 @@ void iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p, bool 
     launch_power(D);
     pf.end(PH_POWER, 1);
     pf.begin(PH_MEAS);
-    launch_meas(D, p.robust, p.gamma, 0, 1, false);
+    // row-major Jacobian only where G or the CG product needs it; the RHS
+    // products always
+    const bool direct = p.pcg_steps <= 0;
+    launch_meas(D, p.robust, p.gamma, 0, (refactor || !direct) ? 1 : 0, false, direct);
     pf.end(PH_MEAS, 1);
     if (refactor) {
         pf.begin(PH_SPGEMM);
@@ -2445,7 +2888,7 @@ SolveOutcome Instance::solve(const SolveParams& p) {
             const bool fp64 = precision_ == 1 || fp64_factor_active_;
             const bool refactor = p.robust || !factor_valid_ || p.mode == 1;
             if (fp64) {
-                if (!D.f64.ready) D.f64.alloc(S, bytes_);
+                if (!D.f64.ready) D.f64.alloc(S, bytes_, D.pi.sizes());
                 SolveParams pd = p;
                 pd.pcg_steps = 0;  // the FP64 factor solves the Newton system directly
                 iteration<double>(D, S, pd, refactor, 1e-13, pf);
@@ -2730,6 +3173,13 @@ void Instance::set_task_sweeps(bool on) { d_->use_tasks = on; }
 
 void Instance::set_lds_solve(bool on) { d_->use_lds = on; }
 
+void Instance::set_partitioned_inverse(bool on, int cut, int waves) {
+    if (d_->use_pi != on) factor_valid_ = false;  // the inverse blocks are made with the factor
+    d_->use_pi = on;
+    if (cut >= 0) d_->pi_cut = cut;
+    if (waves >= 1 && waves <= 16) d_->pi_waves = waves;
+}
+
 void Instance::set_lds_waves(int bottom, int tail) {
     if (bottom >= 1 && bottom <= 16) d_->lds_waves_bottom = bottom;
     if (tail >= 1 && tail <= 16) d_->lds_waves_tail = tail;
@@ -2748,7 +3198,15 @@ std::string Instance::sweep_plan() const {
         out = buf;
     }
     const Impl::LdsDev& L = D.lds[precision_ == 1 || fp64_factor_active_ ? 1 : 0];
-    if (D.use_lds && L.ok) {
+    if (D.use_pi && D.pi.ok && !D.use_flow) {
+        const Impl::LdsDev& PB = D.pi.bottom[precision_ == 1 || fp64_factor_active_ ? 1 : 0];
+        snprintf(buf, sizeof(buf),
+                 "; solve partitioned-inverse(cut=%d bottom=%d/%zuB top supernodes=%d columns=%d blocks=%lld "
+                 "row-entries=%lld segments=%d)",
+                 D.pi.cut, PB.n_bottom, PB.bottom_bytes, D.pi.ntop, D.pi.ntc, (long long)D.pi.nblock,
+                 (long long)D.pi.nq, D.pi.nseg);
+        out += buf;
+    } else if (D.use_lds && L.ok) {
         snprintf(buf, sizeof(buf), "; solve lds-tasks(cut=%d bottom=%d/%zuB mid=%d tail-chunks=%zu est=%.0fus)", L.cut,
                  L.n_bottom, L.bottom_bytes, (int)L.mid_off.size(), L.tail_tasks.size(), L.cost_us);
         out += buf;

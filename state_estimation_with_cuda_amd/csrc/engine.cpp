@@ -156,6 +156,17 @@ int Engine::set_option(const char* key, double value) {
         if (value < 1 || value > 16) return ST_INVALID_ARGUMENT;
         (k == "lds_waves_bottom" ? opt_.lds_waves_bottom : opt_.lds_waves_tail) = (int)value;
         model_dirty_ = true;  // the plan is rebuilt at the next upload
+    } else if (k == "pinv") {
+        opt_.pinv = value != 0;
+        if (inst_) inst_->set_partitioned_inverse(opt_.pinv, -1, -1);
+    } else if (k == "pinv_cut") {
+        if (value < 0 || value > 1e6) return ST_INVALID_ARGUMENT;
+        opt_.pinv_cut = (int)value;
+        model_dirty_ = true;  // the plan is rebuilt at the next upload
+    } else if (k == "pinv_waves") {
+        if (value < 1 || value > 16) return ST_INVALID_ARGUMENT;
+        opt_.pinv_waves = (int)value;
+        model_dirty_ = true;
     } else if (k == "task_sweeps") {
         opt_.task_sweeps = value != 0;
         if (inst_) inst_->set_task_sweeps(opt_.task_sweeps);
@@ -549,6 +560,7 @@ int Engine::rebuild() {
     inst_->set_task_sweeps(opt_.task_sweeps);
     inst_->set_lds_solve(opt_.lds_solve);
     inst_->set_lds_waves(opt_.lds_waves_bottom, opt_.lds_waves_tail);
+    inst_->set_partitioned_inverse(opt_.pinv, opt_.pinv_cut, opt_.pinv_waves);
     inst_->setup(h, sym_);
     inst_->set_state(V_.data(), th_.data());
     model_dirty_ = false;

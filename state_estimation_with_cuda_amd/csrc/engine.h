@@ -86,6 +86,9 @@ struct EngineOptions {
     bool task_sweeps = true;      // group bottom subtrees / the narrow top into single-workgroup tasks
     bool lds_solve = true;        // LDS-resident task triangular solves
     int lds_waves_bottom = 16, lds_waves_tail = 16;
+    bool pinv = true;             // partitioned-inverse solve of the top of the supernodal tree
+    int pinv_cut = 0;             // fronts per bottom subtree under it (0: default)
+    int pinv_waves = 4;           // waves per bottom-subtree workgroup under it
 };
 
 class Engine {

@@ -124,6 +124,10 @@ class Instance {
     void set_task_sweeps(bool on);
     // LDS-resident task triangular solves
     void set_lds_solve(bool on);
+    // partitioned-inverse solve of the top of the tree; cut = fronts per bottom
+    // subtree (0: default), waves = waves per bottom-subtree workgroup (cut and
+    // waves take effect at setup; -1 keeps the current value)
+    void set_partitioned_inverse(bool on, int cut, int waves);
     // waves per workgroup of the bottom-subtree tasks and of the top chunks (takes effect at setup)
     void set_lds_waves(int bottom, int tail);
     std::string sweep_plan() const;
