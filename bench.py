@@ -72,22 +72,25 @@ def request_z(g, req):
     return z
 
 
-def algorithmic_bytes(sz, g, prec_bytes):
-    """Compulsory HBM bytes per launch of each kernel class (each element once).
-    Jacobian values are FP64 in both forms; prec_bytes is the factor/solve
-    vector width (4 mixed, 8 plain FP64)."""
+def algorithmic_bytes(sz, g, prec_bytes, direct=True):
+    """Compulsory HBM bytes per launch of each kernel class (each element once),
+    for an iteration that reuses the factor.  prec_bytes is the factor/solve
+    vector width (4 mixed, 8 plain FP64).  direct: the measurement pass writes
+    the RHS products (FP64, in H^T order, through the position map) instead of
+    the row-major Jacobian (FP64) that the CG refinement reads."""
     n, nb, m, ns = g.n, g.nb, g.m, sz["ns"]
     nnz_y, nnz_h = sz["nnz_y"], sz["nnz_h"]
     s = prec_bytes
     ybus = (n + 1) * 4 + nnz_y * (4 + 8 * 4)  # yrow, ycol, yG, yB, Pij, Qij
+    jac = nnz_h * (4 + 8) if direct else nnz_h * 8
     return {
         # V, th read; P, Q write
         "power": n * 8 * 4 + ybus,
-        # per row: code (1 B), loc, z, sigma read, residual write (FP64), hrow;
-        # FP64 H values write; bus V/th/P/Q, Ybus rows and branch data read once
-        "meas": m * (1 + 4 + 8 + 8 + 8) + (m + 1) * 4 + nnz_h * 8 + n * 8 * 4 + ybus + nb * (4 + 4 + 8 * 5),
-        # tptr, tout; per H entry tpos, trow, value (FP64); residual read; b write
-        "rhs": (ns + 1) * 4 + ns * 4 + nnz_h * (4 + 4 + 8) + m * 8 + ns * s,
+        # per row: code (1 B), loc, z, sigma read, scaled residual write, hrow;
+        # Jacobian products / values; bus V/th/P/Q, Ybus rows and branch data once
+        "meas": m * (1 + 4 + 8 + 8 + 8) + (m + 1) * 4 + jac + n * 8 * 4 + ybus + nb * (4 + 4 + 8 * 5),
+        # tptr, tout, the products, b write
+        "rhs": (ns + 1) * 4 + ns * 4 + nnz_h * 8 + ns * s,
         # update: dx read, V/theta read+write, pth/pv
         "update": ns * s + n * (8 * 4 + 8),
     }
@@ -211,6 +214,105 @@ def run_form(g, fp64, reqs, batches, warmup, steps, profile_steps, pcg_steps=0):
     )
 
 
+def run_areas(args):
+    """Batched uneven areas: `--areas` independent area estimators (bus counts
+    log-uniform in [500, 20000], redundancy ~3) solved as one model -- every
+    kernel launch covers all areas -- against one engine per area solved one
+    after the other.  A step re-solves every area from its previous solution
+    perturbed by N(0, 0.01) per state element (REALTIME, cached factor, tol 1e-6)."""
+    from state_estimation_with_cuda_amd import gridgen, sle
+
+    t0 = time.perf_counter()
+    sizes = gridgen.area_sizes(args.areas, seed=3)
+    areas = [gridgen.workload_area(int(n), seed=1000 + k) for k, n in enumerate(sizes)]
+    g = gridgen.concat(areas, name=f"areas{args.areas}")
+    log(f"areas: {len(areas)} areas, n={g.n} nb={g.nb} m={g.m}, generated in {time.perf_counter() - t0:.1f}s")
+    fp64 = args.form == "fp64"
+    offs = np.cumsum([0] + [a.n for a in areas])
+
+    def engine(grid):
+        e = sle.Engine(double=True)
+        e.load_grid(grid)
+        e.set_config(use_double_precision=int(fp64), enable_profiling=0)
+        e.set_option("pcg_steps", 0 if fp64 else args.pcg_steps)
+        e.upload()
+        e.set_telemetry_f64(grid.z)
+        return e
+
+    # batched: one model
+    t0 = time.perf_counter()
+    eb = engine(g)
+    r = eb.solve(mode=1, max_iterations=30, tolerance=TOL, flat_start=True)
+    V0, th0 = eb.state()
+    log(f"batched: setup + first solve {time.perf_counter() - t0:.1f}s, status {r['status']} iters {r['iterations']}")
+    tune = eb.autotune(3) if args.autotune else ""
+    starts = [gridgen.hot_start(g, V0, th0, 0.01, seed=50 + k) for k in range(args.warmup + args.steps)]
+    for k in range(args.warmup):
+        eb.set_state(*starts[k])
+        eb.solve(mode=0, max_iterations=10, tolerance=TOL)
+    tb, iters, ok = 0.0, [], 0
+    for k in range(args.warmup, args.warmup + args.steps):
+        eb.set_state(*starts[k])  # host upload of the perturbed start, outside the timed solve
+        ts = time.perf_counter()
+        r = eb.solve(mode=0, max_iterations=10, tolerance=TOL)
+        tb += time.perf_counter() - ts
+        iters.append(r["iterations"])
+        ok += r["status"] == 0
+    Vb, thb = eb.state()
+    plan = eb.describe()
+    eb.close()
+    batched_rate = len(areas) * args.steps / tb
+    # per-instance: one engine per area, solved one after the other
+    t0 = time.perf_counter()
+    engines = [engine(a) for a in areas]
+    for e in engines:
+        e.solve(mode=1, max_iterations=30, tolerance=TOL, flat_start=True)
+    log(f"per-area engines: setup + first solves {time.perf_counter() - t0:.1f}s")
+    tp = 0.0
+    worst = {"V": 0.0, "theta": 0.0}
+    ksteps = min(args.steps, 3)
+    for k in range(args.warmup + args.steps - ksteps, args.warmup + args.steps):  # ends on the batch's last start
+        V, th = starts[k]
+        for a, e, b0 in zip(areas, engines, offs):
+            e.set_state(V[b0:b0 + a.n], th[b0:b0 + a.n])
+            ts = time.perf_counter()
+            e.solve(mode=0, max_iterations=10, tolerance=TOL)
+            tp += time.perf_counter() - ts
+    # the same last start solved per area vs inside the batch (tolerance 1e-6)
+    for a, e, b0 in zip(areas, engines, offs):
+        Va, tha = e.state()
+        worst["V"] = max(worst["V"], float(np.abs(Va - Vb[b0:b0 + a.n]).max()))
+        worst["theta"] = max(worst["theta"], float(np.abs(tha - thb[b0:b0 + a.n]).max()))
+    for e in engines:
+        e.close()
+    per_rate = len(areas) * ksteps / tp
+    out = {
+        "metric": "area solves/s, batched uneven areas (one launch sequence for all areas)",
+        "value": round(batched_rate, 1),
+        "unit": "area-solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(tb / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if fp64 else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"{len(areas)} independent areas, n log-uniform [500, 20000] (sum {g.n}), m {g.m}; "
+                               "hot start = previous solution + N(0, 0.01); REALTIME to 1e-6",
+                   "form": args.form},
+        "per_instance_solves_per_s": round(per_rate, 1),
+        "batched_vs_per_instance": round(batched_rate / per_rate, 2),
+        "mean_iterations": float(np.mean(iters)),
+        "converged_steps": ok,
+        "plan": plan,
+        "autotune": tune,
+        "batched_vs_alone_state_diff": worst,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -221,8 +323,14 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=20, help="incremental updates timed on the CPU oracle")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true", help="time the mixed form only")
+    ap.add_argument("--skip-mixed", action="store_true", help="time the plain FP64 form only")
     ap.add_argument("--pcg-steps", type=int, default=2, help="mixed form: CG steps per Newton step")
     ap.add_argument("--copy-calibration", action="store_true", help="run a plain device copy first")
+    ap.add_argument("--workload", choices=["update200k", "areas"], default="update200k",
+                    help="update200k: the headline metric; areas: batched uneven areas")
+    ap.add_argument("--areas", type=int, default=256)
+    ap.add_argument("--form", choices=["fp64", "mixed"], default="fp64", help="areas workload: precision form")
+    ap.add_argument("--autotune", action="store_true", help="areas workload: autotune the solve plan first")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs (--gpus > 1) are not supported", file=sys.stderr)
@@ -232,6 +340,9 @@ def main():
 
     if args.copy_calibration:
         copy_calibration()
+    if args.workload == "areas":
+        run_areas(args)
+        return
     t0 = time.perf_counter()
     if args.buses == 200_000:
         g = gridgen.workload_200k()
@@ -240,7 +351,8 @@ def main():
     log(f"grid: n={g.n} nb={g.nb} m={g.m} generated in {time.perf_counter() - t0:.1f}s")
     reqs, batches = make_updates(g, args.warmup + max(args.steps, args.profile_steps), 0.10, seed=4)
 
-    mixed = run_form(g, False, reqs, batches, args.warmup, args.steps, args.profile_steps, args.pcg_steps)
+    mixed = None if args.skip_mixed else run_form(g, False, reqs, batches, args.warmup, args.steps,
+                                                  args.profile_steps, args.pcg_steps)
     fp64 = None if args.skip_fp64 else run_form(g, True, reqs, batches, args.warmup, args.steps, args.profile_steps)
     forms = [f for f in (mixed, fp64) if f]
     best = max(forms, key=lambda f: f["solves_per_s"])
@@ -248,7 +360,8 @@ def main():
 
     def summary(f):
         kt = f["kernel_times"] or {}
-        ab = algorithmic_bytes(f["sizes"], g, 8 if f["name"] == "fp64" else 4)
+        ab = algorithmic_bytes(f["sizes"], g, 8 if f["name"] == "fp64" else 4,
+                               direct=f["name"] == "fp64" or args.pcg_steps == 0)
         return {
             "solves_per_s": round(f["solves_per_s"], 2),
             "ms_per_step": round(f["ms_per_step"], 4),
@@ -295,7 +408,7 @@ def main():
     }
     for f in forms:
         out[f["name"]] = summary(f)
-    if fp64:
+    if fp64 and mixed:
         out["mixed_vs_fp64"] = round(mixed["solves_per_s"] / fp64["solves_per_s"], 3)
         out["mixed_vs_fp64_state_diff"] = {
             "V": float(np.abs(mixed["V"] - fp64["V"]).max()),
@@ -317,7 +430,7 @@ def main():
             times.append(r["seconds"])
         cpu_rate = len(times) / sum(times)
         # reference optimum for the last request, then the GPU errors against it
-        o.set_z(mixed["last_z"])
+        o.set_z(best["last_z"])
         o.solve(mode=1, max_iter=30, tol=1e-10)
         oV, oth = o.state()
         oJ = o.solve(mode=1, max_iter=1, tol=1.0)["objective"]

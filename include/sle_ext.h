@@ -55,6 +55,13 @@ typedef struct SLE_ExtResult {
  *                     launch per sweep with release/acquire flags per front */
 SLE_API SLE_StatusCode SLE_CALL sle_ext_SetOption(SLE_Handle handle, const char* key, double value);
 
+/* Autotuner of the triangular-solve plan: on the current factor (solve once
+ * first), time `repetitions` forward + backward solves for each candidate
+ * bottom-subtree size and workgroup width of the partitioned-inverse solve and
+ * keep the fastest (the measurements appear in sle_ext_Describe); the choice
+ * becomes the "pinv_cut" / "pinv_waves" options. */
+SLE_API SLE_StatusCode SLE_CALL sle_ext_AutotuneSolve(SLE_Handle handle, int32_t repetitions);
+
 /* Solve with an explicit solver configuration; FP64 outputs. */
 SLE_API SLE_StatusCode SLE_CALL sle_ext_Solve(SLE_Handle handle, const SLE_SolverConfig* config,
                                               SLE_ExtResult* result);

@@ -55,6 +55,8 @@ enum { V_MAG = 0, V_ANGLE = 1, P_INJ = 2, Q_INJ = 3, P_FLOW = 4, Q_FLOW = 5, I_M
 
 struct Case {
     int n = 0, nb = 0, m = 0, slack = 0, ns = 0;
+    int nth = 0;           // angle states: every bus except the angle reference of its island
+    std::vector<int> thi;  // angle state of each bus (-1 for a reference)
     std::vector<int> bus_type;
     std::vector<double> vset;
     std::vector<int> from, to;
@@ -94,8 +96,37 @@ struct Case {
 };
 
 inline bool is_bus_meas(int t) { return t == V_MAG || t == V_ANGLE || t == P_INJ || t == Q_INJ || t == P_PSEUDO || t == Q_PSEUDO; }
-inline int th_idx(const Case& c, int bus) { return bus == c.slack ? -1 : (bus < c.slack ? bus : bus - 1); }
-inline int v_idx(const Case& c, int bus) { return c.n - 1 + bus; }
+inline int th_idx(const Case& c, int bus) { return c.thi[bus]; }
+inline int v_idx(const Case& c, int bus) { return c.nth + bus; }
+inline bool has_th(const Case& c, int bus) { return c.thi[bus] >= 0; }
+
+// Angle references: in every island of closed branches the SLACK-typed bus of
+// highest index (the reference keeps the last SLACK bus added,
+// network_model.cu:85-87; with one island this is that bus).  An island without
+// one keeps all its angles as states (its gain matrix is then singular, as in
+// the reference).  State layout: angles of the other buses in bus order, then
+// all magnitudes (sparse_matrix.cu:1677-1685 for one island).
+void compute_references(Case& c) {
+    std::vector<int> up(c.n);
+    std::iota(up.begin(), up.end(), 0);
+    auto find = [&](int a) {
+        while (up[a] != a) a = up[a] = up[up[a]];
+        return a;
+    };
+    for (int k = 0; k < c.nb; ++k)
+        if (c.status[k]) {
+            int a = find(c.from[k]), b = find(c.to[k]);
+            if (a != b) up[std::max(a, b)] = std::min(a, b);
+        }
+    std::vector<int> ref(c.n, -1);
+    for (int b = 0; b < c.n; ++b)
+        if (c.bus_type[b] == 2) ref[find(b)] = b;  // ascending: the last one wins
+    c.thi.assign(c.n, -1);
+    c.nth = 0;
+    for (int b = 0; b < c.n; ++b)
+        if (ref[find(b)] != b) c.thi[b] = c.nth++;
+    c.ns = c.nth + c.n;
+}
 
 void compute_admittances(Case& c) {
     c.gs.resize(c.nb);
@@ -231,17 +262,17 @@ void h_row_pattern(const Case& c, int i, std::vector<int>& cols) {
     if (t == V_MAG) {
         cols.push_back(v_idx(c, l));
     } else if (t == V_ANGLE) {
-        if (l != c.slack) cols.push_back(th_idx(c, l));
+        if (has_th(c, l)) cols.push_back(th_idx(c, l));
     } else if (is_bus_meas(t)) {
         for (int e = c.yrow[l]; e < c.yrow[l + 1]; ++e) {
             int j = c.ycol[e];
-            if (j != c.slack) cols.push_back(th_idx(c, j));
+            if (has_th(c, j)) cols.push_back(th_idx(c, j));
         }
         for (int e = c.yrow[l]; e < c.yrow[l + 1]; ++e) cols.push_back(v_idx(c, c.ycol[e]));
     } else {
         int a = std::min(c.from[l], c.to[l]), b = std::max(c.from[l], c.to[l]);
-        if (a != c.slack) cols.push_back(th_idx(c, a));
-        if (b != c.slack && b != a) cols.push_back(th_idx(c, b));
+        if (has_th(c, a)) cols.push_back(th_idx(c, a));
+        if (has_th(c, b) && b != a) cols.push_back(th_idx(c, b));
         cols.push_back(v_idx(c, a));
         if (b != a) cols.push_back(v_idx(c, b));
     }
@@ -277,7 +308,7 @@ void eval_jacobian(const Case& c, const double* V, const double* th, const doubl
         if (t == V_MAG) {
             put(v_idx(c, l), 1.0);
         } else if (t == V_ANGLE) {
-            put(th_idx(c, l), 1.0);
+            if (has_th(c, l)) put(th_idx(c, l), 1.0);
         } else if (is_bus_meas(t)) {
             bool isP = (t == P_INJ || t == P_PSEUDO);
             int ii = l;
@@ -299,7 +330,7 @@ void eval_jacobian(const Case& c, const double* V, const double* th, const doubl
                     dth = -V[ii] * V[j] * (G * ct + B * st);
                     dv = V[ii] * (G * st - B * ct);
                 }
-                if (j != c.slack) put(th_idx(c, j), dth);
+                if (has_th(c, j)) put(th_idx(c, j), dth);
                 put(v_idx(c, j), dv);
             }
             double Vi = V[ii];
@@ -311,7 +342,7 @@ void eval_jacobian(const Case& c, const double* V, const double* th, const doubl
                 dthi = P[ii] - Vi * Vi * Gii;
                 dvi = Q[ii] / Vi - Vi * Bii;
             }
-            if (ii != c.slack) put(th_idx(c, ii), dthi);
+            if (has_th(c, ii)) put(th_idx(c, ii), dthi);
             put(v_idx(c, ii), dvi);
         } else {
             int fi = c.from[l], tj = c.to[l];
@@ -352,8 +383,8 @@ void eval_jacobian(const Case& c, const double* V, const double* th, const doubl
                 for (int k = 0; k < 4; ++k) d[k] = (p * dP[k] + q * dQ[k]) * inv;
                 d[c.mend[i] ? 3 : 2] -= S / (ve * ve);
             }
-            if (fi != c.slack) put(th_idx(c, fi), d[0]);
-            if (tj != c.slack) put(th_idx(c, tj), d[1]);
+            if (has_th(c, fi)) put(th_idx(c, fi), d[0]);
+            if (has_th(c, tj)) put(th_idx(c, tj), d[1]);
             put(v_idx(c, fi), d[2]);
             put(v_idx(c, tj), d[3]);
         }
@@ -430,7 +461,7 @@ void order_buses(Case& c) {
     std::vector<std::set<int>> adj(c.n);
     std::vector<int> sbus(c.ns);
     for (int b = 0; b < c.n; ++b) {
-        if (b != c.slack) sbus[th_idx(c, b)] = b;
+        if (has_th(c, b)) sbus[th_idx(c, b)] = b;
         sbus[v_idx(c, b)] = b;
     }
     for (int j = 0; j < c.ns; ++j)
@@ -464,7 +495,7 @@ void order_buses(Case& c) {
 void symbolic(Case& c) {
     c.perm.clear();
     for (int b : c.bus_perm) {
-        if (b != c.slack) c.perm.push_back(th_idx(c, b));
+        if (has_th(c, b)) c.perm.push_back(th_idx(c, b));
         c.perm.push_back(v_idx(c, b));
     }
     c.iperm.assign(c.ns, 0);
@@ -593,7 +624,7 @@ void solve_factor(const Case& c, const std::vector<double>& b, std::vector<doubl
 }
 
 void prepare(Case& c) {
-    c.ns = 2 * c.n - 1;
+    compute_references(c);
     compute_admittances(c);
     recompute_active(c);
     build_ybus(c);
@@ -859,7 +890,7 @@ void or_solve(void* hh, const OracleSolveParams* prm, OracleSolveResult* out) {
                 break;
             }
             for (int bI = 0; bI < c.n; ++bI) {
-                if (bI != c.slack) c.th[bI] += dx[th_idx(c, bI)];
+                if (has_th(c, bI)) c.th[bI] += dx[th_idx(c, bI)];
                 double v = c.V[bI] + dx[v_idx(c, bI)];
                 c.V[bI] = std::min(std::max(v, prm->vmin), prm->vmax);
             }

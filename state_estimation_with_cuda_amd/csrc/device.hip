@@ -206,8 +206,9 @@ __global__ __launch_bounds__(kBlock) void k_flows(int nb, const int32_t* __restr
 // Jacobian row scaled by sqrt(w), and block partials of J and max |r|/sigma.
 // ---------------------------------------------------------------------------
 struct MeasArgs {
-    int m, n, slack;
+    int m, n;
     int robust, unit_weights, want_jac;
+    const int32_t* pth;  // angle state of each bus (-1 for an angle reference)
     double gamma;
     const uint8_t* mcode;
     const int32_t* mloc;
@@ -368,22 +369,22 @@ __global__ __launch_bounds__(kBlock) void k_meas(MeasArgs A, double* __restrict_
                             dv = qe / Vj;
                         }
                     }
-                    if (j != A.slack) put(q0 + kth++, sw * dth);
+                    if (A.pth[j] >= 0) put(q0 + kth++, sw * dth);
                     put(q0 + nth + (e - e0), sw * dv);
                 }
             } else {
-                // sorted columns: theta_min, theta_max, V_min, V_max (slack theta absent)
+                // sorted columns: theta_min, theta_max, V_min, V_max (reference angles absent)
                 int q = q0;
                 if (fb == tb) {
-                    if (fb != A.slack) put(q++, sw * (d[0] + d[1]));
+                    if (A.pth[fb] >= 0) put(q++, sw * (d[0] + d[1]));
                     put(q, sw * (d[2] + d[3]));
                 } else {
                     const bool fl = (code >> 6) & 1;
                     int lo = fl ? fb : tb, hi = fl ? tb : fb;
                     double dlo_t = fl ? d[0] : d[1], dhi_t = fl ? d[1] : d[0];
                     double dlo_v = fl ? d[2] : d[3], dhi_v = fl ? d[3] : d[2];
-                    if (lo != A.slack) put(q++, sw * dlo_t);
-                    if (hi != A.slack) put(q++, sw * dhi_t);
+                    if (A.pth[lo] >= 0) put(q++, sw * dlo_t);
+                    if (A.pth[hi] >= 0) put(q++, sw * dhi_t);
                     put(q++, sw * dlo_v);
                     put(q, sw * dhi_v);
                 }
@@ -606,6 +607,8 @@ struct SolveArgs {
     const T* F;
     T* x;
     T* ubuf;
+    const T* M;            // per-front solve matrices [L11^-1; -L21 L11^-1] (c x k, column-major)
+    const int64_t* moff;
 };
 
 template <typename T, bool WAVE = false>
@@ -653,6 +656,34 @@ __device__ inline void bwd_front(const SolveArgs<T>& A, int s, T* xs) {
         Tm::sync();
     }
     for (int i = tid; i < k; i += kWave) A.x[first + i] = xs[i];
+}
+
+// Per-front solve matrix M = [W; -L21 W], W = L11^-1 (lower triangular), so
+// that a front's forward step is f <- M f_top + [0; f_bot] and its backward
+// step x_s = M^T [y_s; x_rows]: one parallel product instead of k dependent
+// pivot steps.  One wave per front; lane jj builds column jj.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_front_m(int nsn, SolveArgs<T> A, T* __restrict__ M) {
+    const int s = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    if (s >= nsn) return;  // uniform per wave
+    const int c = A.sn_c[s], k = A.sn_k[s];
+    const T* L = A.F + A.foff[s];
+    T* Ms = M + A.moff[s];
+    for (int jj = lane; jj < k; jj += kWave) {
+        T* w = Ms + (int64_t)jj * c;
+        for (int i = 0; i < jj; ++i) w[i] = T(0);
+        for (int i = jj; i < k; ++i) {
+            double acc = i == jj ? 1.0 : 0.0;
+            for (int j = jj; j < i; ++j) acc -= (double)L[j * c + i] * (double)w[j];
+            w[i] = (T)(acc / (double)L[i * c + i]);
+        }
+        for (int i = k; i < c; ++i) {
+            double acc = 0.0;
+            for (int j = jj; j < k; ++j) acc -= (double)L[j * c + i] * (double)w[j];
+            w[i] = (T)acc;
+        }
+    }
 }
 
 template <typename T, int VM>
@@ -774,12 +805,12 @@ __global__ __launch_bounds__(1024) void k_fwd_lds(LdsTaskArgs A, SolveArgs<T> SA
     LdsView<T> v = lds_task_load<T>(A, H, smem_raw, false);
     const int nw = blockDim.x / kWave, w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
     __syncthreads();
-    // L panels (first k columns of each front, contiguous) and right-hand sides
+    // solve matrices (c x k per front, contiguous) and right-hand sides
     for (int lf = w; lf < v.nf; lf += nw) {
         const int* m = v.mt + lf * kFM;
         const int s = m[FM_S], c = m[FM_C], k = m[FM_K], first = m[FM_FIRST];
-        const T* Lg = SA.F + SA.foff[s];
-        for (int e = lane; e < c * k; e += kWave) v.Lp[m[FM_L] + e] = Lg[e];
+        const T* Mg = SA.M + SA.moff[s];
+        for (int e = lane; e < c * k; e += kWave) v.Lp[m[FM_L] + e] = Mg[e];
         for (int i = lane; i < c; i += kWave) v.fv[m[FM_F] + i] = i < k ? SA.x[first + i] : T(0);
     }
     __syncthreads();
@@ -814,13 +845,15 @@ __global__ __launch_bounds__(1024) void k_fwd_lds(LdsTaskArgs A, SolveArgs<T> SA
                 for (int a = lane; a < u; a += kWave) f[rmc[a]] += uc[a];
                 wave_sync();
             }
-            for (int j = 0; j < k; ++j) {
-                const T yj = f[j] / Lp[j * c + j];
-                wave_sync();
-                if (lane == 0) f[j] = yj;
-                for (int i = j + 1 + lane; i < c; i += kWave) f[i] -= Lp[j * c + i] * yj;
-                wave_sync();
+            // f <- M f_top + [0; f_bot]  (c <= 64: one row per lane)
+            T out = T(0);
+            if (lane < c) {
+                out = lane < k ? T(0) : f[lane];
+                for (int j = 0; j < k; ++j) out += Lp[j * c + lane] * f[j];
             }
+            wave_sync();
+            if (lane < c) f[lane] = out;
+            wave_sync();
         }
         __syncthreads();
     }
@@ -846,8 +879,8 @@ __global__ __launch_bounds__(1024) void k_bwd_lds(LdsTaskArgs A, SolveArgs<T> SA
     for (int lf = w; lf < v.nf; lf += nw) {
         const int* m = v.mt + lf * kFM;
         const int s = m[FM_S], c = m[FM_C], k = m[FM_K], first = m[FM_FIRST];
-        const T* Lg = SA.F + SA.foff[s];
-        for (int e = lane; e < c * k; e += kWave) v.Lp[m[FM_L] + e] = Lg[e];
+        const T* Mg = SA.M + SA.moff[s];
+        for (int e = lane; e < c * k; e += kWave) v.Lp[m[FM_L] + e] = Mg[e];
         T* xv = v.fv + m[FM_F];
         const int* sr = v.srl + m[FM_SRC];
         for (int i = lane; i < c; i += kWave) {
@@ -872,13 +905,13 @@ __global__ __launch_bounds__(1024) void k_bwd_lds(LdsTaskArgs A, SolveArgs<T> SA
                 if (code >= 0) xv[i] = v.fv[code];  // ancestor inside the task, solved at a higher level
             }
             wave_sync();
-            for (int j = k - 1; j >= 0; --j) {
-                double part = 0.0;
-                for (int i = j + 1 + lane; i < c; i += kWave) part += (double)(Lp[j * c + i] * xv[i]);
-                part = wave_sum(part);
-                if (lane == 0) xv[j] = (T)(((double)xv[j] - part) / (double)Lp[j * c + j]);
-                wave_sync();
-            }
+            // x_s = M^T [y_s; x_rows]  (lane j < k: one column of M)
+            double out = 0.0;
+            if (lane < k)
+                for (int i = lane; i < c; ++i) out += (double)Lp[lane * c + i] * (double)xv[i];
+            wave_sync();
+            if (lane < k) xv[lane] = (T)out;
+            wave_sync();
         }
         __syncthreads();
     }
@@ -912,7 +945,10 @@ struct PiArgs {
     const int64_t* poff;    // offset of each supernode's path rows in ploc
     const int32_t* ploc;    // top-local index of each path row
     const int32_t* parent;  // supernode parent (-1 at a root)
+    const int32_t* tcol;    // column of each top-local index
+    const int32_t* colsn;   // top supernode (local) of each top-local column
     int ntop;               // top supernodes
+    int ntc;                // top columns
     int slice;              // LDS elements of one per-wave buffer (cmax * kmax)
 };
 
@@ -1015,33 +1051,35 @@ __global__ __launch_bounds__(kBlock) void k_pi_fwd(int nseg, const int64_t* __re
     if (lane == 0) part[g] = acc;
 }
 
+// One wave per top column (rows near the root have hundreds of segments).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pi_rowsum(int ntc, const int32_t* __restrict__ segptr,
                                                      const double* __restrict__ part, T* __restrict__ ytop) {
-    const int t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= ntc) return;
+    const int t = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    if (t >= ntc) return;  // uniform per wave
     double acc = 0.0;
-    for (int g = segptr[t]; g < segptr[t + 1]; ++g) acc += part[g];
-    ytop[t] = (T)acc;
+    for (int g = segptr[t] + lane; g < segptr[t + 1]; g += kWave) acc += part[g];
+    acc = wave_sum(acc);
+    if (lane == 0) ytop[t] = (T)acc;
 }
 
-// Backward product x_T = L_TT^-T y_T: one wave per top supernode, one dot
-// product over its path rows per column.
+// Backward product x_T = L_TT^-T y_T: one wave per top column, a dot product
+// of its (contiguous) block column with y_T gathered along the path.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pi_bwd(PiArgs A, SolveArgs<T> SA, const T* __restrict__ B,
                                                   const T* __restrict__ ytop) {
-    const int u = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    const int t = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
     const int lane = threadIdx.x & (kWave - 1);
-    if (u >= A.ntop) return;
-    const int s = A.sn[u], ks = SA.sn_k[s], first = SA.sn_first[s], Ps = A.plen[u];
-    const T* Bs = B + A.boff[u];
+    if (t >= A.ntc) return;  // uniform per wave
+    const int u = A.colsn[t], col = A.tcol[t];
+    const int jj = col - SA.sn_first[A.sn[u]], Ps = A.plen[u];
+    const T* Bc = B + A.boff[u] + (int64_t)Ps * jj;
     const int32_t* pl = A.ploc + A.poff[u];
-    for (int jj = 0; jj < ks; ++jj) {
-        double acc = 0.0;
-        for (int r = jj + lane; r < Ps; r += kWave) acc += (double)Bs[r + (int64_t)Ps * jj] * (double)ytop[pl[r]];
-        acc = wave_sum(acc);
-        if (lane == 0) SA.x[first + jj] = (T)acc;
-    }
+    double acc = 0.0;
+    for (int r = jj + lane; r < Ps; r += kWave) acc += (double)Bc[r] * (double)ytop[pl[r]];
+    acc = wave_sum(acc);
+    if (lane == 0) SA.x[col] = (T)acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -1136,15 +1174,21 @@ __global__ __launch_bounds__(kBlock) void k_rhs(int ns, const int32_t* __restric
 
 // b = H^T W r from the products k_meas wrote in H^T order: a contiguous
 // segmented sum per state (states in bus-layout order, written to x[tout[t]]).
+constexpr int kRhsLanes = 8;  // lanes per state in k_rhs_sum
+
 template <typename TO>
 __global__ __launch_bounds__(kBlock) void k_rhs_sum(int ns, const int32_t* __restrict__ tptr,
                                                     const int32_t* __restrict__ tout, const double* __restrict__ hr,
                                                     TO* __restrict__ x) {
-    const int t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= ns) return;
+    // kRhsLanes consecutive lanes read one state's segment together
+    const int t = (blockIdx.x * kBlock + threadIdx.x) / kRhsLanes;
+    const int l = threadIdx.x & (kRhsLanes - 1);
     double acc = 0.0;
-    for (int p = tptr[t]; p < tptr[t + 1]; ++p) acc += hr[p];
-    x[tout[t]] = (TO)acc;
+    if (t < ns)
+        for (int p = tptr[t] + l; p < tptr[t + 1]; p += kRhsLanes) acc += hr[p];
+#pragma unroll
+    for (int o = kRhsLanes / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kRhsLanes);
+    if (t < ns && l == 0) x[tout[t]] = (TO)acc;
 }
 
 __global__ __launch_bounds__(kBlock) void k_gather(int n, const int32_t* __restrict__ idx,
@@ -1366,7 +1410,7 @@ struct PiSizes {
 
 template <typename T>
 struct PrecBufs {
-    DBuf<T> Gval, F, x, ubuf;
+    DBuf<T> Gval, F, x, ubuf, M;
     DBuf<T> pinv, pinv_rows, ctop, ytop;  // L_TT^-1 blocks, row-grouped copy, top vectors
     bool ready = false;
     void alloc(const Symbolic& S, size_t& bytes, const PiSizes& pz) {
@@ -1374,14 +1418,21 @@ struct PrecBufs {
         F.alloc(std::max<int64_t>(1, S.ftotal), bytes);
         x.alloc(std::max(1, S.ns), bytes);
         ubuf.alloc(std::max<int64_t>(1, S.utotal), bytes);
+        M.alloc(std::max<int64_t>(1, S.mtotal), bytes);
         pinv.alloc(std::max<int64_t>(1, pz.nblock), bytes);
         pinv_rows.alloc(std::max<int64_t>(1, pz.nq), bytes);
         ctop.alloc(std::max(1, pz.ntc), bytes);
         ytop.alloc(std::max(1, pz.ntc), bytes);
         ready = true;
     }
+    void alloc_pi(const PiSizes& pz, size_t& bytes) {
+        pinv.alloc(std::max<int64_t>(1, pz.nblock), bytes);
+        pinv_rows.alloc(std::max<int64_t>(1, pz.nq), bytes);
+        ctop.alloc(std::max(1, pz.ntc), bytes);
+        ytop.alloc(std::max(1, pz.ntc), bytes);
+    }
     void release(size_t& bytes) {
-        for (DBuf<T>* b : {&Gval, &F, &x, &ubuf, &pinv, &pinv_rows, &ctop, &ytop}) b->release(bytes);
+        for (DBuf<T>* b : {&Gval, &F, &x, &ubuf, &M, &pinv, &pinv_rows, &ctop, &ytop}) b->release(bytes);
         ready = false;
     }
 };
@@ -1637,7 +1688,7 @@ struct PiPlan {
     int cut = 0;
     int ntop = 0, ntc = 0, nseg = 0, slice = 0;
     int64_t nblock = 0, nq = 0;
-    std::vector<int32_t> sn, plen, ploc, tcol, src, segptr, cptr, parent;
+    std::vector<int32_t> sn, plen, ploc, tcol, colsn, src, segptr, cptr, parent;
     std::vector<int64_t> boff, poff, pos, segq, cpos;
 };
 
@@ -1660,6 +1711,7 @@ static PiPlan make_pi_plan(const Symbolic& S, int cut) {
         for (int j = 0; j < S.sn_k[s]; ++j) {
             tloc[S.sn_first[s] + j] = (int32_t)P.tcol.size();
             P.tcol.push_back(S.sn_first[s] + j);
+            P.colsn.push_back(tsl[s]);
         }
         cmax = std::max(cmax, S.sn_c[s]);
         kmax = std::max(kmax, S.sn_k[s]);
@@ -1773,7 +1825,7 @@ struct Instance::Impl {
     DBuf<double> ztmp;
     DBuf<int2> gpair;
     DBuf<int32_t> sn_first, sn_k, sn_c, child_ptr, child, rmap, rows, perm, pth, pv, lists;
-    DBuf<int64_t> foff, rmoff, uoff, rows_off;
+    DBuf<int64_t> foff, rmoff, uoff, rows_off, moff;
     DBuf<DevStatus> st;
     DevStatus* hst = nullptr;  // pinned
     PrecBufs<float> f32;
@@ -1810,7 +1862,7 @@ struct Instance::Impl {
         bool ok = false;
         int cut = 0, ntop = 0, ntc = 0, nseg = 0, slice = 0;
         int64_t nblock = 0, nq = 0;
-        DBuf<int32_t> sn, plen, ploc, tcol, src, segptr, cptr, parent;
+        DBuf<int32_t> sn, plen, ploc, tcol, colsn, src, segptr, cptr, parent;
         DBuf<int64_t> boff, poff, pos, segq, cpos;
         DBuf<double> part;
         LdsDev bottom[2];  // bottom-subtree LDS tasks, [0] float, [1] double
@@ -1849,6 +1901,74 @@ struct Instance::Impl {
     std::vector<Staged> staged;
 };
 
+// Partitioned-inverse plan for the current pi_cut / pi_waves: bottom LDS tasks
+// of both element sizes and the top index lists (D.pi.ok = false when the
+// fronts or blocks do not fit).
+static void build_pi(Instance::Impl& D, const Symbolic& S, size_t& bytes, hipStream_t s) {
+    // partitioned-inverse top solve: bottom LDS tasks of at most pi_cut fronts
+    D.pi.ok = false;
+    if (S.nsn > 0 && S.max_c <= kFrontB) {
+        const int cut = D.pi_cut > 0 ? D.pi_cut : kPiCutDefault;
+        const size_t budget = 64 * 1024;
+        SweepPlan SP = make_plan(S, cut, -1, D.pi_waves, D.lds_waves_tail, true);
+        LdsPlan LP[2];
+        bool fit = true;
+        for (int e = 0; e < 2; ++e)
+            fit = fit && make_lds_plan(S, SP, e ? sizeof(double) : sizeof(float), budget, D.pi_waves,
+                                       D.lds_waves_tail, true, LP[e]);
+        PiPlan PP;
+        if (fit) PP = make_pi_plan(S, cut);
+        if (fit && 3 * (size_t)PP.slice * sizeof(double) <= 65536 && PP.nblock < ((int64_t)1 << 40)) {
+            auto up = [&](DBuf<int32_t>& b, std::vector<int32_t> v) {
+                if (v.empty()) v.push_back(0);
+                b.upload(v, bytes, s);
+            };
+            auto up64 = [&](DBuf<int64_t>& b, std::vector<int64_t> v) {
+                if (v.empty()) v.push_back(0);
+                b.upload(v, bytes, s);
+            };
+            for (int e = 0; e < 2; ++e) {
+                Instance::Impl::LdsDev& L = D.pi.bottom[e];
+                up(L.th, LP[e].th);
+                up(L.meta, LP[e].meta);
+                up(L.lvl, LP[e].lvl);
+                up(L.ch, LP[e].ch);
+                up(L.rm, LP[e].rm);
+                up(L.src, LP[e].src);
+                L.n_bottom = LP[e].n_bottom;
+                L.bottom_bytes = LP[e].bottom_bytes;
+                L.cut = cut;
+                L.ok = true;
+            }
+            Instance::Impl::PiDev& Q = D.pi;
+            up(Q.sn, PP.sn);
+            up(Q.plen, PP.plen);
+            up(Q.ploc, PP.ploc);
+            up(Q.tcol, PP.tcol);
+            up(Q.colsn, PP.colsn);
+            up(Q.src, PP.src);
+            up(Q.segptr, PP.segptr);
+            up(Q.cptr, PP.cptr);
+            up(Q.parent, PP.parent);
+            up64(Q.boff, PP.boff);
+            up64(Q.poff, PP.poff);
+            up64(Q.pos, PP.pos);
+            up64(Q.segq, PP.segq);
+            up64(Q.cpos, PP.cpos);
+            Q.part.alloc(std::max(1, PP.nseg), bytes);
+            HIPCHECK(hipStreamSynchronize(s));
+            Q.cut = cut;
+            Q.ntop = PP.ntop;
+            Q.ntc = PP.ntc;
+            Q.nseg = PP.nseg;
+            Q.slice = PP.slice;
+            Q.nblock = PP.nblock;
+            Q.nq = PP.nq;
+            Q.ok = true;
+        }
+    }
+}
+
 Instance::Instance(int device, int precision) : device_(device), precision_(precision) {
     HIPCHECK(hipSetDevice(device_));
     d_ = new Impl();
@@ -1879,14 +1999,14 @@ Instance::~Instance() {
     rel(d_->gptr); rel(d_->gloc); rel(d_->gdiag); rel(d_->g_sn_ptr); rel(d_->gpair); rel(d_->sn_first);
     rel(d_->sn_k); rel(d_->sn_c); rel(d_->child_ptr); rel(d_->child); rel(d_->rmap); rel(d_->rows);
     rel(d_->perm); rel(d_->pth); rel(d_->pv); rel(d_->lists); rel(d_->foff); rel(d_->rmoff); rel(d_->uoff);
-    rel(d_->rows_off); rel(d_->st); rel(d_->done); rel(d_->order_fwd); rel(d_->order_bwd); rel(d_->sn_parent);
+    rel(d_->rows_off); rel(d_->moff); rel(d_->st); rel(d_->done); rel(d_->order_fwd); rel(d_->order_bwd); rel(d_->sn_parent);
     rel(d_->t_fronts); rel(d_->t_lvl_ptr); rel(d_->t_lvl_off); rel(d_->t_mid);
     for (auto& L : d_->lds) {
         rel(L.th); rel(L.meta); rel(L.lvl); rel(L.ch); rel(L.rm); rel(L.src); rel(L.mid);
     }
     {
         auto& P = d_->pi;
-        rel(P.sn); rel(P.plen); rel(P.ploc); rel(P.tcol); rel(P.src); rel(P.segptr); rel(P.cptr); rel(P.parent);
+        rel(P.sn); rel(P.plen); rel(P.ploc); rel(P.tcol); rel(P.colsn); rel(P.src); rel(P.segptr); rel(P.cptr); rel(P.parent);
         rel(P.boff); rel(P.poff); rel(P.pos); rel(P.segq); rel(P.cpos); rel(P.part);
         for (auto& L : P.bottom) {
             rel(L.th); rel(L.meta); rel(L.lvl); rel(L.ch); rel(L.rm); rel(L.src); rel(L.mid);
@@ -2152,6 +2272,7 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
     D.pth.upload(bus_perm(S.bus_pth), bytes_, s);
     D.pv.upload(bus_perm(S.bus_pv), bytes_, s);
     D.foff.upload(S.sn_foff, bytes_, s);
+    D.moff.upload(S.sn_moff, bytes_, s);
     D.rmoff.upload(S.sn_rmoff, bytes_, s);
     D.uoff.upload(S.sn_uoff, bytes_, s);
     D.rows_off.upload(S.sn_rows_off, bytes_, s);
@@ -2261,67 +2382,7 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
         L.wcut = bw;
         L.ok = true;
     }
-    // partitioned-inverse top solve: bottom LDS tasks of at most pi_cut fronts
-    D.pi.ok = false;
-    if (S.nsn > 0 && S.max_c <= kFrontB) {
-        const int cut = D.pi_cut > 0 ? D.pi_cut : kPiCutDefault;
-        const size_t budget = 64 * 1024;
-        SweepPlan SP = make_plan(S, cut, -1, D.pi_waves, D.lds_waves_tail, true);
-        LdsPlan LP[2];
-        bool fit = true;
-        for (int e = 0; e < 2; ++e)
-            fit = fit && make_lds_plan(S, SP, e ? sizeof(double) : sizeof(float), budget, D.pi_waves,
-                                       D.lds_waves_tail, true, LP[e]);
-        PiPlan PP;
-        if (fit) PP = make_pi_plan(S, cut);
-        if (fit && 3 * (size_t)PP.slice * sizeof(double) <= 65536 && PP.nblock < ((int64_t)1 << 40)) {
-            auto up = [&](DBuf<int32_t>& b, std::vector<int32_t> v) {
-                if (v.empty()) v.push_back(0);
-                b.upload(v, bytes_, s);
-            };
-            auto up64 = [&](DBuf<int64_t>& b, std::vector<int64_t> v) {
-                if (v.empty()) v.push_back(0);
-                b.upload(v, bytes_, s);
-            };
-            for (int e = 0; e < 2; ++e) {
-                Impl::LdsDev& L = D.pi.bottom[e];
-                up(L.th, LP[e].th);
-                up(L.meta, LP[e].meta);
-                up(L.lvl, LP[e].lvl);
-                up(L.ch, LP[e].ch);
-                up(L.rm, LP[e].rm);
-                up(L.src, LP[e].src);
-                L.n_bottom = LP[e].n_bottom;
-                L.bottom_bytes = LP[e].bottom_bytes;
-                L.cut = cut;
-                L.ok = true;
-            }
-            Impl::PiDev& Q = D.pi;
-            up(Q.sn, PP.sn);
-            up(Q.plen, PP.plen);
-            up(Q.ploc, PP.ploc);
-            up(Q.tcol, PP.tcol);
-            up(Q.src, PP.src);
-            up(Q.segptr, PP.segptr);
-            up(Q.cptr, PP.cptr);
-            up(Q.parent, PP.parent);
-            up64(Q.boff, PP.boff);
-            up64(Q.poff, PP.poff);
-            up64(Q.pos, PP.pos);
-            up64(Q.segq, PP.segq);
-            up64(Q.cpos, PP.cpos);
-            Q.part.alloc(std::max(1, PP.nseg), bytes_);
-            HIPCHECK(hipStreamSynchronize(s));
-            Q.cut = cut;
-            Q.ntop = PP.ntop;
-            Q.ntc = PP.ntc;
-            Q.nseg = PP.nseg;
-            Q.slice = PP.slice;
-            Q.nblock = PP.nblock;
-            Q.nq = PP.nq;
-            Q.ok = true;
-        }
-    }
+    build_pi(D, S, bytes_, s);
     // dependency-driven sweeps: level order lists children before parents
     D.flow_ok = S.nsn > 0 && S.max_c <= kFrontB;
     D.use_flow = D.flow_ok && !D.force_levels;
@@ -2477,7 +2538,7 @@ void launch_meas(Instance::Impl& D, int robust, double gamma, int unit_weights, 
     MeasArgs A;
     A.m = D.m;
     A.n = D.n;
-    A.slack = D.slack;
+    A.pth = D.pth.p;
     A.robust = robust;
     A.unit_weights = unit_weights;
     A.want_jac = want_jac;
@@ -2585,12 +2646,15 @@ SolveArgs<T> solve_args(Instance::Impl& D) {
     A.F = B.F.p;
     A.x = B.x.p;
     A.ubuf = B.ubuf.p;
+    A.M = B.M.p;
+    A.moff = D.moff.p;
     return A;
 }
 
 PiArgs pi_args(const Instance::Impl& D) {
     const Instance::Impl::PiDev& P = D.pi;
-    return PiArgs{P.sn.p, P.plen.p, P.boff.p, P.poff.p, P.ploc.p, P.parent.p, P.ntop, P.slice};
+    return PiArgs{P.sn.p, P.plen.p, P.boff.p, P.poff.p, P.ploc.p, P.parent.p, P.tcol.p, P.colsn.p, P.ntop, P.ntc,
+                  P.slice};
 }
 
 bool pi_active(const Instance::Impl& D) { return D.use_pi && D.pi.ok; }
@@ -2612,6 +2676,10 @@ void launch_pi_inverse(Instance::Impl& D) {
 template <typename T>
 void launch_factor(Instance::Impl& D, T piv_tol) {
     launch_factor_sweeps<T>(D, piv_tol);
+    const int nsn = (int)D.sn_first.n;
+    if (nsn && D.max_c <= kFrontB)  // the LDS task solves (every front fits) read M
+        k_front_m<T><<<(nsn + kBlock / kWave - 1) / (kBlock / kWave), kBlock, 0, D.stream>>>(nsn, solve_args<T>(D),
+                                                                                          bufs<T>(D).M.p);
     launch_pi_inverse<T>(D);
 }
 
@@ -2637,12 +2705,12 @@ void launch_trisolve(Instance::Impl& D, Prof* pf = nullptr) {
                                                                    A.ubuf, B.ctop.p);
             k_pi_fwd<T><<<(PI.nseg + nw - 1) / nw, kBlock, 0, D.stream>>>(PI.nseg, PI.segq.p, PI.src.p,
                                                                            B.pinv_rows.p, B.ctop.p, PI.part.p);
-            k_pi_rowsum<T><<<nblk(PI.ntc), kBlock, 0, D.stream>>>(PI.ntc, PI.segptr.p, PI.part.p, B.ytop.p);
+            k_pi_rowsum<T><<<(PI.ntc + nw - 1) / nw, kBlock, 0, D.stream>>>(PI.ntc, PI.segptr.p, PI.part.p, B.ytop.p);
         }
         prof_end(pf, PH_FWD, (PB.n_bottom ? 1 : 0) + (PI.ntc ? 3 : 0));
         prof_begin(pf, PH_BWD);
         if (PI.ntc)
-            k_pi_bwd<T><<<(PI.ntop + nw - 1) / nw, kBlock, 0, D.stream>>>(pi_args(D), A, B.pinv.p, B.ytop.p);
+            k_pi_bwd<T><<<(PI.ntc + nw - 1) / nw, kBlock, 0, D.stream>>>(pi_args(D), A, B.pinv.p, B.ytop.p);
         if (PB.n_bottom)
             k_bwd_lds<T><<<PB.n_bottom, kWave * D.pi_waves, PB.bottom_bytes, D.stream>>>(LA, A, 0);
         prof_end(pf, PH_BWD, (PB.n_bottom ? 1 : 0) + (PI.ntc ? 1 : 0));
@@ -2728,7 +2796,9 @@ void launch_trisolve(Instance::Impl& D, Prof* pf = nullptr) {
 
 template <typename T>
 void launch_rhs(Instance::Impl& D) {
-    if (D.ns) k_rhs_sum<T><<<nblk(D.ns), kBlock, 0, D.stream>>>(D.ns, D.tptr.p, D.tout.p, D.hr.p, bufs<T>(D).x.p);
+    if (D.ns)
+        k_rhs_sum<T><<<nblk((int64_t)D.ns * kRhsLanes), kBlock, 0, D.stream>>>(D.ns, D.tptr.p, D.tout.p, D.hr.p,
+                                                                               bufs<T>(D).x.p);
 }
 
 // dx source: the factor-precision solve vector, or the FP64 CG solution
@@ -3215,6 +3285,82 @@ std::string Instance::sweep_plan() const {
     }
     return out;
 }
+
+namespace {
+// Average time of one forward + backward solve with the current plan (the
+// right-hand side is restored before every repetition).
+template <typename T>
+double time_solves(Instance::Impl& D, size_t& bytes, int reps) {
+    PrecBufs<T>& B = bufs<T>(D);
+    B.alloc_pi(D.pi.sizes(), bytes);
+    launch_pi_inverse<T>(D);
+    DBuf<T> save;
+    size_t tmp = 0;
+    save.alloc(B.x.n, tmp);
+    HIPCHECK(hipMemcpyAsync(save.p, B.x.p, B.x.n * sizeof(T), hipMemcpyDeviceToDevice, D.stream));
+    launch_trisolve<T>(D);  // warm-up
+    HIPCHECK(hipEventRecord(D.ev0, D.stream));
+    for (int r = 0; r < reps; ++r) {
+        HIPCHECK(hipMemcpyAsync(B.x.p, save.p, B.x.n * sizeof(T), hipMemcpyDeviceToDevice, D.stream));
+        launch_trisolve<T>(D);
+    }
+    HIPCHECK(hipEventRecord(D.ev1, D.stream));
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipEventSynchronize(D.ev1));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, D.ev0, D.ev1));
+    HIPCHECK(hipMemcpyAsync(B.x.p, save.p, B.x.n * sizeof(T), hipMemcpyDeviceToDevice, D.stream));
+    HIPCHECK(hipStreamSynchronize(D.stream));
+    save.release(tmp);
+    return ms / std::max(1, reps);
+}
+}  // namespace
+
+std::string Instance::autotune_solve(int reps) {
+    HIPCHECK(hipSetDevice(device_));
+    Impl& D = *d_;
+    const Symbolic& S = *S_;
+    if (!factor_valid_) throw std::invalid_argument("autotune needs a factored gain matrix: solve first");
+    if (!D.use_pi || D.use_flow) return "partitioned inverse not in use";
+    const bool f64 = precision_ == 1 || fp64_factor_active_;
+    const int cuts[] = {16, 32, 64, 128};
+    const int waves[] = {2, 4, 8};
+    int bc = D.pi_cut, bw = D.pi_waves;
+    double best = 1e300;
+    std::string log;
+    char buf[96];
+    for (int cut : cuts)
+        for (int w : waves) {
+            D.pi_cut = cut;
+            D.pi_waves = w;
+            build_pi(D, S, bytes_, D.stream);
+            if (!D.pi.ok) continue;
+            const double ms = f64 ? time_solves<double>(D, bytes_, reps) : time_solves<float>(D, bytes_, reps);
+            snprintf(buf, sizeof(buf), "%s%d/%d:%.3f", log.empty() ? "" : " ", cut, w, ms);
+            log += buf;
+            if (ms < best) {
+                best = ms;
+                bc = cut;
+                bw = w;
+            }
+        }
+    D.pi_cut = bc;
+    D.pi_waves = bw;
+    build_pi(D, S, bytes_, D.stream);
+    if (D.f32.ready) D.f32.alloc_pi(D.pi.sizes(), bytes_);
+    if (D.f64.ready) D.f64.alloc_pi(D.pi.sizes(), bytes_);
+    if (f64)
+        launch_pi_inverse<double>(D);
+    else
+        launch_pi_inverse<float>(D);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(D.stream));
+    snprintf(buf, sizeof(buf), "chosen cut=%d waves=%d (%.3f ms per solve); ", bc, bw, best);
+    return buf + std::string("cut/waves:ms ") + log;
+}
+
+int Instance::partitioned_inverse_cut() const { return d_->pi_cut > 0 ? d_->pi_cut : kPiCutDefault; }
+int Instance::partitioned_inverse_waves() const { return d_->pi_waves; }
 
 int Instance::staged_count(int slot) const {
     if (slot < 0 || slot >= (int)d_->staged.size()) return -1;

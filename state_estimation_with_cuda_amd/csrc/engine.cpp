@@ -787,6 +787,16 @@ int Engine::solve(const SolverConfigIn& sc, ResultOut& out) {
     return ST_OK;
 }
 
+int Engine::autotune(int reps, std::string& report) {
+    if (!initialized_) return ST_NOT_INITIALIZED;
+    if (!inst_ || model_dirty_ || !inst_->factor_valid()) return ST_INVALID_ARGUMENT;  // needs a solved model
+    report = inst_->autotune_solve(std::max(1, reps));
+    tune_report_ = report;
+    opt_.pinv_cut = inst_->partitioned_inverse_cut();
+    opt_.pinv_waves = inst_->partitioned_inverse_waves();
+    return ST_OK;
+}
+
 int Engine::check_observability(int& observable, int& islands) {
     // Topological check (sle_engine.cu:684-753): union-find islands over closed
     // branches, then numerical observability through the gain factor pivots.

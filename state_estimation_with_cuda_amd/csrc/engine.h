@@ -198,6 +198,9 @@ class Engine {
 
     // introspection for parity tests
     Instance* instance() { return inst_.get(); }
+    // Autotune the solve plan on the current factor (solves once first if needed).
+    int autotune(int reps, std::string& report);
+    const std::string& autotune_report() const { return tune_report_; }
     std::shared_ptr<const Symbolic> symbolic() const { return sym_; }
     int ensure_ready();     // upload/rebuild as needed without solving
     int ensure_symbolic();  // host symbolic analysis only (no device needed)
@@ -209,6 +212,7 @@ class Engine {
     int rebuild();
 
     EngineConfig cfg_;
+    std::string tune_report_;
     EngineOptions opt_;
     bool initialized_ = false;
     // buses

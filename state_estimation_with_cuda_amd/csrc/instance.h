@@ -131,6 +131,12 @@ class Instance {
     // waves per workgroup of the bottom-subtree tasks and of the top chunks (takes effect at setup)
     void set_lds_waves(int bottom, int tail);
     std::string sweep_plan() const;
+    // Autotuner of the solve plan: times the forward + backward solves of the
+    // current factor for each candidate (bottom-subtree size x waves per
+    // workgroup) and keeps the fastest; returns what it measured.
+    std::string autotune_solve(int reps);
+    int partitioned_inverse_cut() const;
+    int partitioned_inverse_waves() const;
 
     // Device-resident telemetry batches: stage (host -> HBM) ahead of time,
     // apply (scatter into z on the device) right before a solve.

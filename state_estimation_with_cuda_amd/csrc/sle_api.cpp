@@ -869,11 +869,22 @@ SLE_API const char* SLE_CALL sle_ext_Describe(SLE_Handle handle) {
         std::string d = e.config().use_double_precision ? "plain FP64" : "mixed FP32 factor + FP64 refinement";
         d += "; pcg_steps=" + std::to_string(e.config().use_double_precision ? 0 : e.options().pcg_steps);
         if (e.instance()) d += "; sweeps=" + e.instance()->sweep_plan();
+        if (!e.autotune_report().empty()) d += "; autotune: " + e.autotune_report();
         g_str_cache = d;
     } catch (...) {
         g_str_cache = "unavailable";
     }
     return g_str_cache.c_str();
+}
+
+SLE_API SLE_StatusCode SLE_CALL sle_ext_AutotuneSolve(SLE_Handle handle, int32_t repetitions) {
+    if (!handle) return fail(SLE_ERROR_INVALID_HANDLE, "NULL handle");
+    return guarded([&]() -> SLE_StatusCode {
+        std::string report;
+        int rc = E(handle).autotune(repetitions, report);
+        if (rc == sle::ST_INVALID_ARGUMENT) return fail(SLE_ERROR_INVALID_ARGUMENT, "autotune needs a solved model");
+        return from_int(rc, "autotune");
+    });
 }
 
 SLE_API SLE_StatusCode SLE_CALL sle_ext_SetOption(SLE_Handle handle, const char* key, double value) {

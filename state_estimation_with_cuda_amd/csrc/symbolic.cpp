@@ -11,8 +11,36 @@
 namespace sle {
 namespace {
 
-inline int theta_state(int bus, int slack) { return bus == slack ? -1 : (bus < slack ? bus : bus - 1); }
-inline int v_state(int bus, int n) { return n - 1 + bus; }
+// Angle references: in every island of closed branches the SLACK-typed bus of
+// highest index (the reference keeps the last SLACK bus added,
+// network_model.cu:85-87).  States: the angles of all other buses in bus order,
+// then all magnitudes (sparse_matrix.cu:1677-1685 for a single island).
+void angle_references(const ModelView& mv, Symbolic& S) {
+    std::vector<int32_t> up(mv.n);
+    std::iota(up.begin(), up.end(), 0);
+    auto find = [&](int a) {
+        while (up[a] != a) a = up[a] = up[up[a]];
+        return a;
+    };
+    for (int k = 0; k < mv.nb; ++k)
+        if (mv.status[k]) {
+            int a = find(mv.from[k]), b = find(mv.to[k]);
+            if (a != b) up[std::max(a, b)] = std::min(a, b);
+        }
+    std::vector<int32_t> ref(mv.n, -1);
+    for (int b = 0; b < mv.n; ++b)
+        if (mv.bus_type[b] == 2) ref[find(b)] = b;
+    S.th_state.assign(mv.n, -1);
+    S.nth = 0;
+    S.n_refs = 0;
+    for (int b = 0; b < mv.n; ++b) {
+        if (ref[find(b)] != b)
+            S.th_state[b] = S.nth++;
+        else
+            S.n_refs++;
+    }
+    S.ns = S.nth + mv.n;
+}
 
 // Ybus pattern and per-entry contribution lists.  Entry (i,i) collects the
 // from-end (ff) or to-end (tt) self admittance of each closed incident branch,
@@ -75,21 +103,24 @@ void build_ybus(const ModelView& mv, Symbolic& S) {
 
 // Exact H row structure in natural state order, sorted.
 void build_h(const ModelView& mv, Symbolic& S) {
-    const int n = mv.n, slack = mv.slack;
+    const std::vector<int32_t>& ts = S.th_state;
+    auto theta_state = [&](int b) { return ts[b]; };
+    auto v_state = [&](int b) { return S.nth + b; };
+    auto has_th = [&](int b) { return ts[b] >= 0; };
     S.hrow.assign(mv.m + 1, 0);
     auto row_len = [&](int i) -> int {
         if (!mv.active[i]) return 0;
         int t = mv.mtype[i], l = mv.mloc[i];
         if (t == MT_V_MAG) return 1;
-        if (t == MT_V_ANGLE) return l == slack ? 0 : 1;
+        if (t == MT_V_ANGLE) return has_th(l) ? 1 : 0;
         if (is_bus_measurement(t)) {
-            int d = S.yrow[l + 1] - S.yrow[l];
-            bool has_slack = std::binary_search(S.ycol.begin() + S.yrow[l], S.ycol.begin() + S.yrow[l + 1], slack);
-            return 2 * d - (has_slack ? 1 : 0);
+            int len = 0;
+            for (int e = S.yrow[l]; e < S.yrow[l + 1]; ++e) len += has_th(S.ycol[e]) ? 2 : 1;
+            return len;
         }
         int a = mv.from[l], b = mv.to[l];
-        if (a == b) return a == slack ? 1 : 2;
-        return 4 - ((a == slack || b == slack) ? 1 : 0);
+        if (a == b) return has_th(a) ? 2 : 1;
+        return 2 + (has_th(a) ? 1 : 0) + (has_th(b) ? 1 : 0);
     };
     for (int i = 0; i < mv.m; ++i) S.hrow[i + 1] = S.hrow[i] + row_len(i);
     S.hcol.resize(S.hrow[mv.m]);
@@ -98,19 +129,19 @@ void build_h(const ModelView& mv, Symbolic& S) {
         if (q == S.hrow[i + 1]) continue;
         int t = mv.mtype[i], l = mv.mloc[i];
         if (t == MT_V_MAG) {
-            S.hcol[q] = v_state(l, n);
+            S.hcol[q] = v_state(l);
         } else if (t == MT_V_ANGLE) {
-            S.hcol[q] = theta_state(l, slack);
+            S.hcol[q] = theta_state(l);
         } else if (is_bus_measurement(t)) {
             for (int e = S.yrow[l]; e < S.yrow[l + 1]; ++e)
-                if (S.ycol[e] != slack) S.hcol[q++] = theta_state(S.ycol[e], slack);
-            for (int e = S.yrow[l]; e < S.yrow[l + 1]; ++e) S.hcol[q++] = v_state(S.ycol[e], n);
+                if (has_th(S.ycol[e])) S.hcol[q++] = theta_state(S.ycol[e]);
+            for (int e = S.yrow[l]; e < S.yrow[l + 1]; ++e) S.hcol[q++] = v_state(S.ycol[e]);
         } else {
             int a = std::min(mv.from[l], mv.to[l]), b = std::max(mv.from[l], mv.to[l]);
-            if (a != slack) S.hcol[q++] = theta_state(a, slack);
-            if (b != a && b != slack) S.hcol[q++] = theta_state(b, slack);
-            S.hcol[q++] = v_state(a, n);
-            if (b != a) S.hcol[q++] = v_state(b, n);
+            if (has_th(a)) S.hcol[q++] = theta_state(a);
+            if (b != a && has_th(b)) S.hcol[q++] = theta_state(b);
+            S.hcol[q++] = v_state(a);
+            if (b != a) S.hcol[q++] = v_state(b);
         }
     }
 }
@@ -227,7 +258,7 @@ void analyze(const ModelView& mv, Symbolic& S) {
     S.nb = mv.nb;
     S.m = mv.m;
     S.slack = mv.slack;
-    S.ns = 2 * mv.n - 1;
+    angle_references(mv, S);
     const int n = mv.n, ns = S.ns;
 
     build_ybus(mv, S);
@@ -240,16 +271,16 @@ void analyze(const ModelView& mv, Symbolic& S) {
     S.perm.clear();
     S.perm.reserve(ns);
     for (int b : S.bus_perm) {
-        if (b != mv.slack) S.perm.push_back(theta_state(b, mv.slack));
-        S.perm.push_back(v_state(b, n));
+        if (S.th_state[b] >= 0) S.perm.push_back(S.th_state[b]);
+        S.perm.push_back(S.nth + b);
     }
     S.iperm.assign(ns, 0);
     for (int k = 0; k < ns; ++k) S.iperm[S.perm[k]] = k;
     S.bus_pth.assign(n, -1);
     S.bus_pv.assign(n, -1);
     for (int b = 0; b < n; ++b) {
-        if (b != mv.slack) S.bus_pth[b] = S.iperm[theta_state(b, mv.slack)];
-        S.bus_pv[b] = S.iperm[v_state(b, n)];
+        if (S.th_state[b] >= 0) S.bus_pth[b] = S.iperm[S.th_state[b]];
+        S.bus_pv[b] = S.iperm[S.nth + b];
     }
 
     // ---- H^T in permuted state order -------------------------------------
@@ -469,15 +500,18 @@ void analyze(const ModelView& mv, Symbolic& S) {
     }
     // front / update-vector / relative-map offsets
     S.sn_foff.assign(nsn + 1, 0);
+    S.sn_moff.assign(nsn + 1, 0);
     S.sn_uoff.assign(nsn + 1, 0);
     S.sn_rmoff.assign(nsn + 1, 0);
     for (int s = 0; s < nsn; ++s) {
         int64_t c = S.sn_c[s], u = S.sn_c[s] - S.sn_k[s];
         S.sn_foff[s + 1] = S.sn_foff[s] + c * c;
+        S.sn_moff[s + 1] = S.sn_moff[s] + c * S.sn_k[s];
         S.sn_uoff[s + 1] = S.sn_uoff[s] + u;
         S.sn_rmoff[s + 1] = S.sn_rmoff[s] + u;
     }
     S.ftotal = S.sn_foff[nsn];
+    S.mtotal = S.sn_moff[nsn];
     S.utotal = S.sn_uoff[nsn];
     S.rmap.assign(S.sn_rmoff[nsn], 0);
     for (int s = 0; s < nsn; ++s) {

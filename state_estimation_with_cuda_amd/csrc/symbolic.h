@@ -43,6 +43,10 @@ inline bool is_bus_measurement(int t) {
 
 struct Symbolic {
     int n = 0, nb = 0, m = 0, slack = 0, ns = 0;
+    // angle references (one per island of closed branches) and the state layout:
+    // th_state[b] = angle state of bus b (-1 for a reference), V_b = nth + b
+    std::vector<int32_t> th_state;
+    int nth = 0, n_refs = 0;
 
     // Ybus (CSR, columns sorted, parallel branches merged; closed branches only)
     std::vector<int32_t> yrow, ycol, ydiag;
@@ -66,6 +70,7 @@ struct Symbolic {
     std::vector<int32_t> sn_parent;
     std::vector<int32_t> sn_child_ptr, sn_child;
     std::vector<int64_t> sn_foff;  // dense front (c x c, column-major) offsets; total ftotal
+    std::vector<int64_t> sn_moff;  // solve matrix (c x k, column-major) offsets; total mtotal
     std::vector<int64_t> sn_uoff;  // update-vector offsets for the triangular solve; total utotal
     std::vector<int64_t> sn_rmoff; // relative map (child below-rows -> parent front rows), length c-k
     std::vector<int32_t> rmap;
@@ -85,7 +90,7 @@ struct Symbolic {
     // H^T map in permuted state order (right-hand side H^T W r)
     std::vector<int32_t> tptr, tpos, trow;
 
-    int64_t nnz_l = 0, ftotal = 0, utotal = 0, flops = 0;
+    int64_t nnz_l = 0, ftotal = 0, mtotal = 0, utotal = 0, flops = 0;
     int max_c = 0, n_levels = 0;
     double seconds = 0;
 };

@@ -72,7 +72,8 @@ class Grid:
 
     @property
     def ns(self) -> int:
-        return 2 * self.n - 1
+        # one angle reference per island; generated grids have one SLACK bus per island
+        return 2 * self.n - int(np.count_nonzero(self.bus_type == BUS_SLACK))
 
 
 # ---------------------------------------------------------------------------
@@ -346,9 +347,31 @@ def workload_area(n: int, seed: int) -> Grid:
 
 
 def hot_start(g: Grid, V: np.ndarray, th: np.ndarray, sigma: float = 0.01, seed: int = 0):
-    """Previous solution perturbed by N(0, sigma^2) per state element (slack angle fixed)."""
+    """Previous solution perturbed by N(0, sigma^2) per state element (reference angles fixed)."""
     rng = np.random.default_rng(seed)
     V2 = V + rng.normal(0, sigma, V.shape)
     th2 = th + rng.normal(0, sigma, th.shape)
-    th2[g.slack] = th[g.slack]
+    ref = g.bus_type == BUS_SLACK
+    th2[ref] = th[ref]
     return V2, th2
+
+
+def concat(grids, name: str = "") -> Grid:
+    """Independent areas side by side as one model (block-diagonal gain matrix):
+    buses, branches and measurements of area k follow those of areas < k; each
+    area keeps its own SLACK bus, which is the angle reference of its island."""
+    bo = np.cumsum([0] + [g.n for g in grids])
+    ko = np.cumsum([0] + [g.nb for g in grids])
+    cat = lambda f: np.concatenate([getattr(g, f) for g in grids])
+    m_loc = []
+    for g, b0, k0 in zip(grids, bo, ko):
+        bus_meas = np.isin(g.m_type, [V_MAG, V_ANGLE, P_INJ, Q_INJ, P_PSEUDO, Q_PSEUDO])
+        m_loc.append(np.where(bus_meas, g.m_loc + b0, g.m_loc + k0).astype(np.int32))
+    return Grid(
+        n=int(bo[-1]), slack=int(grids[0].slack), bus_type=cat("bus_type"), vset=cat("vset"),
+        br_from=np.concatenate([g.br_from + b0 for g, b0 in zip(grids, bo)]).astype(np.int32),
+        br_to=np.concatenate([g.br_to + b0 for g, b0 in zip(grids, bo)]).astype(np.int32),
+        r=cat("r"), x=cat("x"), b=cat("b"), tap=cat("tap"), shift=cat("shift"), is_xfmr=cat("is_xfmr"),
+        status=cat("status"), m_type=cat("m_type"), m_loc=np.concatenate(m_loc), m_end=cat("m_end"),
+        sigma=cat("sigma"), z=cat("z"), V_true=cat("V_true"), th_true=cat("th_true"), z_true=cat("z_true"),
+        name=name or f"concat{len(grids)}")

@@ -143,7 +143,7 @@ EXT_FUNCTIONS = [
     "sle_ext_GetHPattern", "sle_ext_GetGPattern", "sle_ext_GetOrdering", "sle_ext_Evaluate",
     "sle_ext_JacobianGain", "sle_ext_Factor", "sle_ext_SolveFactor", "sle_ext_StageTelemetry",
     "sle_ext_ApplyStagedTelemetry", "sle_ext_GetKernelTimes", "sle_ext_ResetKernelTimes", "sle_ext_SetOption",
-    "sle_ext_Describe",
+    "sle_ext_Describe", "sle_ext_AutotuneSolve",
 ]
 KERNEL_CLASSES = ["power", "meas", "spgemm", "factor", "rhs", "fwd", "bwd", "update", "pcg"]
 
@@ -269,6 +269,7 @@ class Library:
             "sle_ext_ResetKernelTimes": (C.c_int, [H]),
             "sle_ext_SetOption": (C.c_int, [H, C.c_char_p, C.c_double]),
             "sle_ext_Describe": (C.c_char_p, [H]),
+            "sle_ext_AutotuneSolve": (C.c_int, [H, i32]),
         }
         missing = [n for n in API_FUNCTIONS + EXT_FUNCTIONS if n not in sig]
         assert not missing, missing
@@ -371,6 +372,12 @@ class Engine:
         ``switches`` every branch gets a switching device "S<k>"."""
         real = self.lib.np_real
         n, nb, m = g.n, g.nb, g.m
+        cfg = self.get_config()
+        need = {"max_buses": n, "max_branches": nb, "max_measurements": m,
+                "max_switching_devices": nb if switches else 0}
+        grow = {k: v for k, v in need.items() if v > cfg[k]}
+        if grow:
+            self.set_config(**grow)  # capacity limits of SLE_Config
         bid, bptr = _id_buffer("B", n)
         lid, lptr = _id_buffer("L", nb)
         mid, mptr = _id_buffer("M", m)
@@ -514,6 +521,11 @@ class Engine:
     def describe(self) -> str:
         d = self.L.sle_ext_Describe(self.h)
         return d.decode() if d else ""
+
+    def autotune(self, repetitions=5):
+        """Time the solve plan candidates on the current factor and keep the fastest."""
+        self._check(self.L.sle_ext_AutotuneSolve(self.h, int(repetitions)), "sle_ext_AutotuneSolve")
+        return self.describe()
 
     def set_option(self, key, value):
         self._check(self.L.sle_ext_SetOption(self.h, key.encode(), float(value)), "sle_ext_SetOption")

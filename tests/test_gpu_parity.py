@@ -201,3 +201,34 @@ def test_breaker_toggle_redoes_symbolic():
     ogc, ogr = o.g_pattern()
     assert np.array_equal(gc, ogc) and np.array_equal(gr, ogr)
     e.close()
+
+
+@pytest.mark.parametrize("fp64", [True, False])
+def test_batched_areas_match_oracle_and_solo_solves(fp64):
+    """Independent areas solved as one model (one launch sequence for all):
+    each area's state matches the oracle and the same area solved alone."""
+    areas = [gg.geometric(1500, seed=41, flow_meas=4000), gg.grid14(4), gg.geometric(800, seed=42, flow_meas="all"),
+             gg.geometric(3000, seed=43, flow_meas=9000)]
+    g = gg.concat(areas)
+    print(f"[areas fp64={fp64}] n={g.n} m={g.m} areas={len(areas)}", flush=True)
+    o = Oracle(g)
+    o.solve(mode=1, max_iter=50, tol=1e-11, flat_start=True)
+    oV, oth = o.state()
+    oJ = o.solve(mode=1, max_iter=1, tol=1.0)["objective"]
+    e = engine_for(g, double_precision=fp64)
+    r = e.solve(mode=1, max_iterations=50, tolerance=1e-8, flat_start=True)
+    assert r["status"] == 0, r
+    V, th = e.state()
+    assert np.abs(V - oV).max() < STATE_TOL and np.abs(th - oth).max() < STATE_TOL
+    assert abs(r["objective"] - oJ) / oJ < J_TOL
+    e.close()
+    b0 = 0
+    for a in areas:
+        ea = engine_for(a, double_precision=fp64)
+        ra = ea.solve(mode=1, max_iterations=50, tolerance=1e-8, flat_start=True)
+        assert ra["status"] == 0, ra
+        Va, tha = ea.state()
+        assert np.abs(V[b0:b0 + a.n] - Va).max() < STATE_TOL
+        assert np.abs(th[b0:b0 + a.n] - tha).max() < STATE_TOL
+        ea.close()
+        b0 += a.n

@@ -368,3 +368,28 @@ def test_realtime_mode_reuses_factor_and_reaches_same_optimum():
     V2, t2 = o2.state()
     assert np.abs(V1 - V2).max() < 1e-9 and np.abs(t1 - t2).max() < 1e-9
     del r0, V0, t0
+
+
+def test_independent_areas_solve_like_separate_models():
+    """Areas side by side (one SLACK bus each, no branch between them) must give
+    each area's own WLS optimum: a wrong angle-reference rule (a single global
+    reference, a shifted state index) makes the gain singular or moves the
+    angles of the later areas."""
+    areas = [gg.geometric(300, seed=31, flow_meas=700), gg.grid14(3), gg.geometric(200, seed=32, flow_meas="all")]
+    g = gg.concat(areas)
+    o = Oracle(g)
+    r = o.solve(mode=1, tol=1e-11, flat_start=True)
+    assert r["status"] == 0
+    V, th = o.state()
+    J = 0.0
+    b0 = 0
+    for a in areas:
+        oa = Oracle(a)
+        ra = oa.solve(mode=1, tol=1e-11, flat_start=True)
+        Va, tha = oa.state()
+        assert np.abs(V[b0:b0 + a.n] - Va).max() < 1e-9
+        assert np.abs(th[b0:b0 + a.n] - tha).max() < 1e-9
+        assert th[b0 + a.slack] == 0.0
+        J += ra["objective"]
+        b0 += a.n
+    assert abs(r["objective"] - J) < 1e-10 * J

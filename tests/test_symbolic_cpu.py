@@ -24,6 +24,9 @@ CASES = {
     "grid14_seed5": lambda: gg.grid14(5),
     "geo500_partial_flows": lambda: gg.geometric(500, seed=4, flow_meas=700),
     "geo3000": lambda: gg.geometric(3000, seed=9, flow_meas="all", dup_meas=500),
+    # independent areas in one model: one angle reference per island
+    "areas3": lambda: gg.concat([gg.geometric(300, seed=21, flow_meas=600), gg.grid14(2),
+                                 gg.geometric(500, seed=22, flow_meas="all")]),
 }
 
 
@@ -42,6 +45,7 @@ def test_patterns_and_ordering_match_oracle(name):
     for k in ("bus_perm", "perm", "parent", "colcount", "snode_ptr"):
         assert np.array_equal(es[k], os_[k]), k
     sz = e.sizes()
+    assert sz["ns"] == g.ns == 2 * g.n - int(np.count_nonzero(g.bus_type == gg.BUS_SLACK))
     assert sz["nnz_l"] == o.nnz_l
     assert sz["n_snodes"] == o.n_snodes
     # Ybus pattern (values need the device)
