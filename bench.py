@@ -37,6 +37,7 @@ METRIC = "p99 update latency + solves/s, 200k-bus WLS, mixed-prec vs FP64 at 1e-
 HBM_PEAK_GBS = 8000.0
 TOL = 1e-6
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_200k.md")
+AREA_MAX_ITER = 25  # a batch iterates until its slowest area converges
 COPY_BYTES = 512 << 20
 
 
@@ -249,12 +250,12 @@ def run_areas(args):
     starts = [gridgen.hot_start(g, V0, th0, 0.01, seed=50 + k) for k in range(args.warmup + args.steps)]
     for k in range(args.warmup):
         eb.set_state(*starts[k])
-        eb.solve(mode=0, max_iterations=10, tolerance=TOL)
+        eb.solve(mode=0, max_iterations=AREA_MAX_ITER, tolerance=TOL)
     tb, iters, ok = 0.0, [], 0
     for k in range(args.warmup, args.warmup + args.steps):
         eb.set_state(*starts[k])  # host upload of the perturbed start, outside the timed solve
         ts = time.perf_counter()
-        r = eb.solve(mode=0, max_iterations=10, tolerance=TOL)
+        r = eb.solve(mode=0, max_iterations=AREA_MAX_ITER, tolerance=TOL)
         tb += time.perf_counter() - ts
         iters.append(r["iterations"])
         ok += r["status"] == 0
@@ -268,7 +269,7 @@ def run_areas(args):
     for e in engines:
         e.solve(mode=1, max_iterations=30, tolerance=TOL, flat_start=True)
     log(f"per-area engines: setup + first solves {time.perf_counter() - t0:.1f}s")
-    tp = 0.0
+    tp, p_iters, p_ok = 0.0, [], 0
     worst = {"V": 0.0, "theta": 0.0}
     ksteps = min(args.steps, 3)
     for k in range(args.warmup + args.steps - ksteps, args.warmup + args.steps):  # ends on the batch's last start
@@ -276,8 +277,10 @@ def run_areas(args):
         for a, e, b0 in zip(areas, engines, offs):
             e.set_state(V[b0:b0 + a.n], th[b0:b0 + a.n])
             ts = time.perf_counter()
-            e.solve(mode=0, max_iterations=10, tolerance=TOL)
+            ra = e.solve(mode=0, max_iterations=AREA_MAX_ITER, tolerance=TOL)
             tp += time.perf_counter() - ts
+            p_iters.append(ra["iterations"])
+            p_ok += ra["status"] == 0
     # the same last start solved per area vs inside the batch (tolerance 1e-6)
     for a, e, b0 in zip(areas, engines, offs):
         Va, tha = e.state()
@@ -303,12 +306,162 @@ def run_areas(args):
                                "hot start = previous solution + N(0, 0.01); REALTIME to 1e-6",
                    "form": args.form},
         "per_instance_solves_per_s": round(per_rate, 1),
+        "per_instance_mean_iterations": float(np.mean(p_iters)),
+        "per_instance_converged": f"{p_ok}/{len(p_iters)}",
         "batched_vs_per_instance": round(batched_rate / per_rate, 2),
         "mean_iterations": float(np.mean(iters)),
         "converged_steps": ok,
         "plan": plan,
         "autotune": tune,
         "batched_vs_alone_state_diff": worst,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def non_bridge_branches(g, count, rng):
+    """Branches whose opening keeps the grid connected."""
+    import scipy.sparse as sp
+    import scipy.sparse.csgraph as cg
+
+    out = []
+    for k in rng.permutation(g.nb):
+        keep = np.ones(g.nb, bool)
+        keep[k] = False
+        A = sp.coo_matrix((np.ones(keep.sum()), (g.br_from[keep], g.br_to[keep])), shape=(g.n, g.n))
+        if cg.connected_components(A, directed=False)[0] == 1:
+            out.append(int(k))
+            if len(out) == count:
+                return out
+    return out
+
+
+def run_serve(args):
+    """Serving loop on the resident 200k-bus model: Poisson arrivals of
+    incremental updates (10 % of z re-measured), a breaker operation after every
+    50th update (alternately opening and closing a non-bridge branch), and full
+    solves of new 5k-50k-bus areas admitted by an HBM budget; latency is
+    arrival to result."""
+    from state_estimation_with_cuda_amd import gridgen, serve, sle
+
+    rng = np.random.default_rng(4)
+    g = gridgen.workload_200k()
+    fp64 = args.form == "fp64"
+
+    def engine(grid):
+        e = sle.Engine(double=True)
+        e.load_grid(grid)
+        e.set_config(use_double_precision=int(fp64), enable_profiling=0)
+        e.set_option("pcg_steps", 0 if fp64 else args.pcg_steps)
+        e.upload()
+        e.set_telemetry_f64(grid.z)
+        return e
+
+    main_e = engine(g)
+    r = main_e.solve(mode=1, max_iterations=30, tolerance=TOL, flat_start=True)
+    log(f"serve: resident model ready, first solve status {r['status']}")
+    # areas: a pool of new area models, log-uniform 5k-50k buses
+    pool_n = np.exp(rng.uniform(np.log(5_000), np.log(50_000), args.area_pool)).astype(int)
+    pool = [gridgen.workload_area(int(n), seed=7000 + k) for k, n in enumerate(pool_n)]
+    cal = engine(pool[0])
+    bytes_per_bus = 1.25 * cal.gpu_bytes() / pool[0].n
+    cal.close()
+    log(f"serve: {len(pool)} area models, {bytes_per_bus:.0f} B/bus estimate")
+    # request stream
+    t_upd = serve.poisson_arrivals(args.rate, args.duration, rng)
+    t_area = serve.poisson_arrivals(args.area_rate, args.duration, rng) if args.area_rate > 0 else np.zeros(0)
+    brks = non_bridge_branches(g, max(1, len(t_upd) // 100 + 1), rng)
+    k = max(1, int(round(0.10 * g.m)))
+    z = g.z.copy()
+    status = {}
+    reqs, replay = [], {}
+    open_brk = None
+    for q, t in enumerate(t_upd):
+        idx = np.sort(rng.choice(g.m, k, replace=False)).astype(np.int32)
+        val = g.z[idx] + rng.normal(0.0, 1.0, k) * g.sigma[idx]
+        z[idx] = val
+        rec = q % args.record_every == args.record_every // 2
+        reqs.append(serve.Request(serve.UPDATE, float(t), (idx, val), rid=q, record=rec))
+        if rec and len(replay) < args.replays:
+            replay[q] = (z.copy(), dict(status))
+        if (q + 1) % 50 == 0 and brks:
+            if open_brk is None:
+                open_brk = brks[(q // 100) % len(brks)]
+                op = (open_brk, 0)
+            else:
+                op = (open_brk, 1)
+                open_brk = None
+            status[op[0]] = op[1]
+            reqs.append(serve.Request(serve.TOGGLE, float(t) + 1e-6, op, rid=100_000 + q))
+    for j, t in enumerate(t_area):
+        reqs.append(serve.Request(serve.AREA, float(t), pool[j % len(pool)], rid=200_000 + j, record=j < 3))
+    srv = serve.Server(main_e, engine, hbm_budget=int(args.hbm_budget_gib * (1 << 30)), area_slots=args.area_slots,
+                       bytes_per_bus=bytes_per_bus, tol=TOL, max_iterations=10)
+    t0 = time.perf_counter()
+    done = serve.run(srv, reqs)
+    wall = time.perf_counter() - t0
+    summ = serve.summarize(done, warmup=args.serve_warmup)
+    ok = [r for r in done if r.error is None and r.status == 0 and r.arrival >= args.serve_warmup]
+    rate = len(ok) / max(wall - args.serve_warmup, 1e-9)
+    # isolation: the same request alone gives the same result
+    iso = {"update": 0.0, "area": 0.0}
+    for r in done:
+        if r.kind == serve.UPDATE and r.rid in replay and r.result is not None and r.error is None:
+            zr, st = replay[r.rid]
+            for kb, sv in status.items():  # breakers back to their state at that request
+                main_e.set_branch_status(kb, st.get(kb, 1))
+            main_e.set_telemetry_f64(zr)
+            main_e.set_state(*r.before[:2])
+            main_e.solve(mode=0, max_iterations=10, tolerance=TOL)
+            V, th = main_e.state()
+            iso["update"] = max(iso["update"], float(np.abs(V - r.result[0]).max()),
+                                float(np.abs(th - r.result[1]).max()))
+    for r in done:
+        if r.kind == serve.AREA and r.record and r.result is not None:
+            e = engine(r.payload)
+            e.solve(mode=1, max_iterations=50, tolerance=TOL, flat_start=True)
+            V, th = e.state()
+            e.close()
+            iso["area"] = max(iso["area"], float(np.abs(V - r.result[0]).max()), float(np.abs(th - r.result[1]).max()))
+    err = None
+    if not args.no_cpu_baseline and replay:
+        from oracle.oracle import Oracle
+
+        q0 = min(replay)
+        r0 = [r for r in done if r.kind == serve.UPDATE and r.rid == q0][0]
+        zr, st = replay[q0]
+        g2 = gridgen.Grid(**{**g.__dict__})
+        g2.z = zr
+        g2.status = g.status.copy()
+        for kb, sv in st.items():
+            g2.status[kb] = sv
+        o = Oracle(g2)
+        o.solve(mode=1, max_iter=30, tol=1e-10, flat_start=True)
+        oV, oth = o.state()
+        err = {"V": float(np.abs(r0.result[0] - oV).max()), "theta": float(np.abs(r0.result[1] - oth).max())}
+    main_e.close()
+    u = summ["update"]
+    out = {
+        "metric": "serving: p99 incremental-update latency (arrival to result) on the resident 200k-bus model",
+        "value": round(u["p99_ms"], 3) if u["p99_ms"] is not None else None,
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": len(done),
+        "warmup": args.serve_warmup,
+        "ms_per_step": round(u["p50_ms"], 3) if u["p50_ms"] is not None else None,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if fp64 else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"Poisson {args.rate}/s updates (10% of z) for {args.duration}s, breaker toggle per 50 "
+                               f"updates, new 5k-50k-bus areas at {args.area_rate}/s within {args.hbm_budget_gib} GiB",
+                   "form": args.form},
+        "latency": summ,
+        "solves_per_s": round(rate, 1),
+        "hbm_peak_gib": round(srv.peak_used / (1 << 30), 2),
+        "estimate_misses": srv.estimate_misses,
+        "isolation_max_diff": iso,
+        "error_vs_oracle": err,
     }
     print(json.dumps(out), flush=True)
 
@@ -326,8 +479,17 @@ def main():
     ap.add_argument("--skip-mixed", action="store_true", help="time the plain FP64 form only")
     ap.add_argument("--pcg-steps", type=int, default=2, help="mixed form: CG steps per Newton step")
     ap.add_argument("--copy-calibration", action="store_true", help="run a plain device copy first")
-    ap.add_argument("--workload", choices=["update200k", "areas"], default="update200k",
-                    help="update200k: the headline metric; areas: batched uneven areas")
+    ap.add_argument("--workload", choices=["update200k", "areas", "serve"], default="update200k",
+                    help="update200k: the headline metric; areas: batched uneven areas; serve: serving loop")
+    ap.add_argument("--rate", type=float, default=200.0, help="serve: update arrivals per second")
+    ap.add_argument("--duration", type=float, default=30.0, help="serve: seconds of arrivals")
+    ap.add_argument("--serve-warmup", type=float, default=5.0, help="serve: seconds excluded from the statistics")
+    ap.add_argument("--area-rate", type=float, default=1.0, help="serve: new areas per second")
+    ap.add_argument("--area-pool", type=int, default=6)
+    ap.add_argument("--area-slots", type=int, default=4)
+    ap.add_argument("--hbm-budget-gib", type=float, default=64.0)
+    ap.add_argument("--record-every", type=int, default=500)
+    ap.add_argument("--replays", type=int, default=4)
     ap.add_argument("--areas", type=int, default=256)
     ap.add_argument("--form", choices=["fp64", "mixed"], default="fp64", help="areas workload: precision form")
     ap.add_argument("--autotune", action="store_true", help="areas workload: autotune the solve plan first")
@@ -342,6 +504,9 @@ def main():
         copy_calibration()
     if args.workload == "areas":
         run_areas(args)
+        return
+    if args.workload == "serve":
+        run_serve(args)
         return
     t0 = time.perf_counter()
     if args.buses == 200_000:

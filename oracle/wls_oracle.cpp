@@ -147,15 +147,17 @@ void recompute_active(Case& c) {
     }
 }
 
-// Ybus: rows sorted by column, parallel branches merged, closed branches only.
+// Ybus: rows sorted by column, parallel branches merged.  The pattern holds
+// every branch, open or closed; an open branch contributes zero admittance, so
+// a breaker operation changes values only.
 void build_ybus(Case& c) {
     std::vector<std::vector<std::pair<int, std::pair<double, double>>>> rows(c.n);
     for (int i = 0; i < c.n; ++i) rows[i].push_back({i, {0.0, 0.0}});
     for (int k = 0; k < c.nb; ++k) {
-        if (!c.status[k]) continue;
+        const double on = c.status[k] ? 1.0 : 0.0;
         int i = c.from[k], j = c.to[k];
-        double g = c.gs[k], b = c.bs[k], a = c.tap[k], ph = c.shift[k];
-        double hb = 0.5 * c.bc[k];
+        double g = on * c.gs[k], b = on * c.bs[k], a = c.tap[k], ph = c.shift[k];
+        double hb = on * 0.5 * c.bc[k];
         double a2 = a * a;
         double cp = std::cos(ph), sp = std::sin(ph);
         // Yff = (g + j(b+hb))/a^2 ; Ytt = g + j(b+hb)
@@ -254,10 +256,10 @@ void eval_h(const Case& c, const double* V, const double* th, const double* P, c
     }
 }
 
-// Structural H row (natural state order, sorted).
+// Structural H row (natural state order, sorted); a measurement on an open
+// branch keeps its row (its values and weight are zero).
 void h_row_pattern(const Case& c, int i, std::vector<int>& cols) {
     cols.clear();
-    if (!c.active[i]) return;
     int t = c.mtype[i], l = c.mloc[i];
     if (t == V_MAG) {
         cols.push_back(v_idx(c, l));
@@ -296,6 +298,10 @@ void eval_jacobian(const Case& c, const double* V, const double* th, const doubl
     for (int i = 0; i < c.m; ++i) {
         int r0 = c.hrow[i], r1 = c.hrow[i + 1];
         if (r0 == r1) continue;
+        if (!c.active[i]) {  // inactive rows are zero
+            for (int q = r0; q < r1; ++q) hv[q] = 0;
+            continue;
+        }
         int t = c.mtype[i], l = c.mloc[i];
         auto put = [&](int col, double v) {
             for (int q = r0; q < r1; ++q)

@@ -326,7 +326,7 @@ __global__ __launch_bounds__(kBlock) void k_meas(MeasArgs A, double* __restrict_
             if (ar / sg > A.gamma) w = A.gamma / (sg * ar);
         }
         if (!act) w = 0.0;
-        const double sw = A.unit_weights ? 1.0 : sqrt(w);
+        const double sw = A.unit_weights ? (act ? 1.0 : 0.0) : sqrt(w);
         rsi = sw * r;
         A.rs[i] = rsi;
         if (A.hout) {
@@ -1801,6 +1801,7 @@ struct Instance::Impl {
     DBuf<double> V, th, P, Q, Pf, Qf, Pt, Qt;
     // measurements
     DBuf<uint8_t> mcode;
+    std::vector<uint8_t> mcode_h;  // host copy (device row order)
     DBuf<int32_t> mloc, hrow;
     DBuf<double> z, sigma, rs, hx, r;
     DBuf<double> hs;  // sqrt(w) * Jacobian values in the H pattern (FP64 in both forms)
@@ -2173,6 +2174,7 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
             hr[p + 1] = hr[p] + (S.hrow[i + 1] - S.hrow[i]);
         }
         D.mcode.upload(code, bytes_, s);
+        D.mcode_h = code;
         D.mloc.upload(loc, bytes_, s);
         D.sigma.upload(sg, bytes_, s);
         D.z.upload(zz, bytes_, s);
@@ -2435,6 +2437,10 @@ void Instance::update_branches(const HostArrays& h) {
     D.tap.upload(take(h.tap, D.kl), bytes_, s);
     D.shift.upload(take(h.shift, D.kl), bytes_, s);
     D.status.upload(take(h.status, D.kl), bytes_, s);
+    // measurement activity (rows of open branches weigh zero)
+    for (int p = 0; p < D.m; ++p)
+        D.mcode_h[p] = (uint8_t)((D.mcode_h[p] & ~(1u << 5)) | ((h.active[D.ml[p]] ? 1u : 0u) << 5));
+    D.mcode.upload(D.mcode_h, bytes_, s);
     HIPCHECK(hipStreamSynchronize(s));
     size_t nY = S_->ycol.size();
     if (nY) k_ybus<<<nblk(nY), kBlock, 0, s>>>((int)nY, D.ycptr.p, D.ycon.p, D.gs.p, D.bs.p, D.bc.p, D.tap.p,
@@ -2442,6 +2448,7 @@ void Instance::update_branches(const HostArrays& h) {
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(s));
     factor_valid_ = false;
+    D.results_valid = false;
 }
 
 // z arrives in insertion order; the device holds it in row layout order

@@ -505,19 +505,42 @@ void Engine::build_host_arrays(HostArrays& h) const {
     h.tap = br_tap_;
     h.shift = br_shift_;
     h.status.resize(h.nb);
-    for (int k = 0; k < h.nb; ++k) h.status[k] = (br_status_[k] && !br_removed_[k]) ? 1 : 0;
+    h.present.resize(h.nb);
+    for (int k = 0; k < h.nb; ++k) {
+        h.status[k] = (br_status_[k] && !br_removed_[k]) ? 1 : 0;
+        h.present[k] = br_removed_[k] ? 0 : 1;
+    }
     h.mtype = m_type_;
     h.mloc = m_loc_;
     h.mend = m_end_;
     h.active.resize(h.m);
-    for (int i = 0; i < h.m; ++i) h.active[i] = meas_active(i) ? 1 : 0;
+    h.m_present.resize(h.m);
+    for (int i = 0; i < h.m; ++i) {
+        h.active[i] = meas_active(i) ? 1 : 0;
+        const int l = m_loc_[i];
+        h.m_present[i] = !m_removed_[i] && (is_bus_meas(m_type_[i]) ? !bus_removed_[l] : !br_removed_[l]);
+    }
     h.sigma = m_sigma_;
     h.z = m_z_;
 }
 
 int Engine::ensure_symbolic() {
     if (bus_id_.empty() || slack_ < 0) return ST_MODEL_INCOMPLETE;
-    if (sym_ && !model_dirty_ && !sym_dirty_) return ST_OK;
+    if (sym_ && topo_dirty_ && !sym_dirty_) {
+        // breaker operations keep the structure unless an island's angle reference changed
+        HostArrays h;
+        build_host_arrays(h);
+        ModelView mv;
+        mv.n = h.n;
+        mv.nb = h.nb;
+        mv.slack = h.slack;
+        mv.bus_type = h.bus_type.data();
+        mv.from = h.from.data();
+        mv.to = h.to.data();
+        mv.status = h.status.data();
+        if (angle_states(mv) != sym_->th_state) sym_dirty_ = true;
+    }
+    if (sym_ && !sym_dirty_) return ST_OK;
     HostArrays h;
     build_host_arrays(h);
     ModelView mv;
@@ -533,6 +556,8 @@ int Engine::ensure_symbolic() {
     mv.mloc = h.mloc.data();
     mv.mend = h.mend.data();
     mv.active = h.active.data();
+    mv.present = h.present.data();
+    mv.m_present = h.m_present.data();
     auto S = std::make_shared<Symbolic>();
     analyze(mv, *S);
     sym_ = S;
@@ -572,6 +597,7 @@ int Engine::rebuild() {
 
 int Engine::ensure_ready() {
     if (!initialized_) return ST_NOT_INITIALIZED;
+    apply_topology_values();
     if (model_dirty_ || !inst_) {
         int rc = rebuild();
         if (rc != ST_OK) return rc;
@@ -708,10 +734,41 @@ int Engine::apply_switch_changes() {
         int k = sd_branch_[s];
         if (!br_removed_[k]) {
             br_status_[k] = (uint8_t)st;
-            model_dirty_ = true;
-            sym_dirty_ = true;
+            topo_dirty_ = true;
         }
     }
+    return ST_OK;
+}
+
+// A breaker operation changes values only (the structure holds open branches)
+// unless it changes which bus is the angle reference of some island: then the
+// state layout, and with it the symbolic phase, is redone.
+int Engine::apply_topology_values() {
+    if (!topo_dirty_) return ST_OK;
+    topo_dirty_ = false;
+    if (model_dirty_ || !inst_ || !sym_) {
+        model_dirty_ = true;
+        sym_dirty_ = true;
+        return ST_OK;
+    }
+    HostArrays h;
+    build_host_arrays(h);
+    ModelView mv;
+    mv.n = h.n;
+    mv.nb = h.nb;
+    mv.m = h.m;
+    mv.slack = h.slack;
+    mv.bus_type = h.bus_type.data();
+    mv.from = h.from.data();
+    mv.to = h.to.data();
+    mv.status = h.status.data();
+    if (angle_states(mv) != sym_->th_state) {
+        model_dirty_ = true;
+        sym_dirty_ = true;
+        return ST_OK;
+    }
+    inst_->update_branches(h);
+    results_dirty_ = true;
     return ST_OK;
 }
 
@@ -722,8 +779,7 @@ int Engine::set_branch_status_index(int k, int status) {
     if (br_status_[k] != st) {
         br_status_[k] = st;
         if (br_sd_[k] >= 0) sd_status_[br_sd_[k]] = st;
-        model_dirty_ = true;
-        sym_dirty_ = true;
+        topo_dirty_ = true;
     }
     return ST_OK;
 }

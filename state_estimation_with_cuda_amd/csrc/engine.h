@@ -142,6 +142,7 @@ class Engine {
     int set_switch_status(const char* sd_id, int status);
     int get_switch_status(const char* sd_id) const;
     int apply_switch_changes();
+    int apply_topology_values();
     int set_branch_status_index(int k, int status);
 
     // solve
@@ -212,6 +213,7 @@ class Engine {
     int rebuild();
 
     EngineConfig cfg_;
+    bool topo_dirty_ = false;  // breaker status changed since the last upload to the device
     std::string tune_report_;
     EngineOptions opt_;
     bool initialized_ = false;

@@ -25,9 +25,11 @@ struct HostArrays {
     std::vector<double> vset;
     std::vector<int32_t> from, to;
     std::vector<double> gs, bs, bc, tap, shift;  // series g, b; total charging; tap; phase shift
-    std::vector<uint8_t> status;
+    std::vector<uint8_t> status;   // 1 = closed
+    std::vector<uint8_t> present;  // 1 = branch in the structure (not removed)
     std::vector<int32_t> mtype, mloc, mend;
-    std::vector<uint8_t> active;
+    std::vector<uint8_t> active;     // 1 = measurement used now (its branch closed)
+    std::vector<uint8_t> m_present;  // 1 = measurement in the structure (not removed)
     std::vector<double> sigma, z;
 };
 
@@ -84,6 +86,8 @@ class Instance {
     // Allocate and upload everything for this symbolic analysis.
     void setup(const HostArrays& h, std::shared_ptr<const Symbolic> S);
     // Re-upload branch parameters + status and rebuild Ybus values (pattern unchanged).
+    // Branch parameters and breaker status (values only: the structure holds open
+    // branches too); refreshes Ybus and the measurement activity.
     void update_branches(const HostArrays& h);
     void set_z(const double* z);
     void set_z_indexed(const int32_t* idx, const double* val, int count);

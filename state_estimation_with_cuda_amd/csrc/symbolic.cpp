@@ -15,7 +15,7 @@ namespace {
 // highest index (the reference keeps the last SLACK bus added,
 // network_model.cu:85-87).  States: the angles of all other buses in bus order,
 // then all magnitudes (sparse_matrix.cu:1677-1685 for a single island).
-void angle_references(const ModelView& mv, Symbolic& S) {
+std::vector<int32_t> island_reference(const ModelView& mv) {
     std::vector<int32_t> up(mv.n);
     std::iota(up.begin(), up.end(), 0);
     auto find = [&](int a) {
@@ -30,11 +30,18 @@ void angle_references(const ModelView& mv, Symbolic& S) {
     std::vector<int32_t> ref(mv.n, -1);
     for (int b = 0; b < mv.n; ++b)
         if (mv.bus_type[b] == 2) ref[find(b)] = b;
+    std::vector<int32_t> out(mv.n);
+    for (int b = 0; b < mv.n; ++b) out[b] = ref[find(b)];
+    return out;  // reference bus of each bus's island (-1: none)
+}
+
+void angle_references(const ModelView& mv, Symbolic& S) {
+    const std::vector<int32_t> ref = island_reference(mv);
     S.th_state.assign(mv.n, -1);
     S.nth = 0;
     S.n_refs = 0;
     for (int b = 0; b < mv.n; ++b) {
-        if (ref[find(b)] != b)
+        if (ref[b] != b)
             S.th_state[b] = S.nth++;
         else
             S.n_refs++;
@@ -51,7 +58,7 @@ void build_ybus(const ModelView& mv, Symbolic& S) {
     std::vector<int32_t> cnt(n + 1, 0);
     for (int i = 0; i < n; ++i) cnt[i + 1]++;  // diagonal marker
     for (int k = 0; k < mv.nb; ++k) {
-        if (!mv.status[k]) continue;
+        if (mv.present && !mv.present[k]) continue;
         int i = mv.from[k], j = mv.to[k];
         if (i == j) {
             cnt[i + 1] += 2;
@@ -66,7 +73,7 @@ void build_ybus(const ModelView& mv, Symbolic& S) {
     std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
     for (int i = 0; i < n; ++i) trip[fill[i]++] = {i, -1};
     for (int k = 0; k < mv.nb; ++k) {
-        if (!mv.status[k]) continue;
+        if (mv.present && !mv.present[k]) continue;
         int i = mv.from[k], j = mv.to[k];
         if (i == j) {
             trip[fill[i]++] = {i, k * 4 + 0};
@@ -109,7 +116,7 @@ void build_h(const ModelView& mv, Symbolic& S) {
     auto has_th = [&](int b) { return ts[b] >= 0; };
     S.hrow.assign(mv.m + 1, 0);
     auto row_len = [&](int i) -> int {
-        if (!mv.active[i]) return 0;
+        if (mv.m_present && !mv.m_present[i]) return 0;
         int t = mv.mtype[i], l = mv.mloc[i];
         if (t == MT_V_MAG) return 1;
         if (t == MT_V_ANGLE) return has_th(l) ? 1 : 0;
@@ -249,6 +256,12 @@ void minimum_degree(int n, const std::vector<int32_t>& aptr, const std::vector<i
 }
 
 }  // namespace
+
+std::vector<int32_t> angle_states(const ModelView& mv) {
+    Symbolic S;
+    angle_references(mv, S);
+    return S.th_state;
+}
 
 void analyze(const ModelView& mv, Symbolic& S) {
     auto t0 = std::chrono::steady_clock::now();

@@ -28,7 +28,12 @@ struct ModelView {
     const int32_t* bus_type = nullptr;
     const int32_t* from = nullptr;
     const int32_t* to = nullptr;
-    const uint8_t* status = nullptr;  // 1 = closed
+    const uint8_t* status = nullptr;  // 1 = closed (islands and angle references)
+    // Structure: the patterns hold every present branch and measurement, open
+    // or closed, so that a breaker operation changes values only (an open
+    // branch contributes zero admittance, its measurements zero rows).
+    const uint8_t* present = nullptr;    // per branch; nullptr = all present
+    const uint8_t* m_present = nullptr;  // per measurement; nullptr = all present
     const int32_t* mtype = nullptr;
     const int32_t* mloc = nullptr;
     const int32_t* mend = nullptr;
@@ -97,6 +102,11 @@ struct Symbolic {
 
 // Full symbolic analysis.  Throws std::runtime_error on inconsistent input.
 void analyze(const ModelView& mv, Symbolic& S);
+
+// Angle state of every bus (-1 for the reference of its island) for the
+// current breaker status; equal to Symbolic::th_state while the islands keep
+// their references.
+std::vector<int32_t> angle_states(const ModelView& mv);
 
 // Natural-order G pattern (CSC, lower), for reporting and parity checks.
 void natural_gain_pattern(const Symbolic& S, std::vector<int32_t>& colptr, std::vector<int32_t>& row,

@@ -1,0 +1,227 @@
+"""Serving loop over the sle_* C-ABI: requests admitted as they arrive.
+
+One resident model (its gain factor and hot-start state stay in HBM) takes
+incremental telemetry updates and breaker operations in arrival order; full
+solves of new areas run beside it in a pool of slots, each on its own engine
+(its own HIP stream), admitted first-come first-served while their estimated
+HBM footprint fits the budget.  Work of the resident model and of the areas
+overlaps on the device.  Every request records arrival, start and completion
+times; latency is completion minus arrival.
+
+The reference engine has no serving layer (its ABI is synchronous and one
+handle is not to be used from two threads, sle_api.h:60-63); this module keeps
+that contract -- each handle is driven by exactly one thread -- and adds the
+admission and overlap around it.
+"""
+from __future__ import annotations
+
+import collections
+import dataclasses
+import threading
+import time
+from typing import Any, Callable, Optional
+
+import numpy as np
+
+UPDATE, TOGGLE, AREA = "update", "toggle", "area"
+
+
+@dataclasses.dataclass
+class Request:
+    kind: str
+    arrival: float  # seconds after the start of the run
+    payload: Any = None  # update: (idx, values); toggle: (branch, status); area: grid
+    rid: int = 0
+    record: bool = False  # keep the result (and, for updates, the starting point) for replay
+    start: Optional[float] = None
+    done: Optional[float] = None
+    status: Optional[int] = None
+    iterations: int = 0
+    error: Optional[str] = None
+    result: Any = None
+    before: Any = None  # updates with record: (V, theta, breaker status) before the request
+
+    @property
+    def latency(self) -> float:
+        return (self.done - self.arrival) if self.done is not None else float("nan")
+
+
+class Server:
+    """Continuous admission of requests.
+
+    main: engine of the resident model (sle.Engine-like: set_telemetry_indexed,
+          set_branch_status, solve, state).
+    make_area_engine(grid) -> engine with the area loaded, uploaded and its
+          telemetry set; the server closes it after the solve.
+    hbm_budget: bytes available to area slots; an area is admitted when its
+          estimate fits beside the areas already running.
+    """
+
+    def __init__(self, main, make_area_engine: Callable, hbm_budget: int, area_slots: int = 4,
+                 bytes_per_bus: float = 16384.0, tol: float = 1e-6, max_iterations: int = 10,
+                 clock: Callable[[], float] = time.perf_counter):
+        self.main = main
+        self.make_area_engine = make_area_engine
+        self.budget = int(hbm_budget)
+        self.slots = int(area_slots)
+        self.bytes_per_bus = float(bytes_per_bus)
+        self.tol = tol
+        self.max_iterations = max_iterations
+        self.clock = clock
+        self._cv = threading.Condition()
+        self._main_q: collections.deque = collections.deque()
+        self._area_q: collections.deque = collections.deque()
+        self._used = 0
+        self.peak_used = 0
+        self.estimate_misses = 0
+        self._running_areas = 0
+        self._closing = False
+        self._t0 = 0.0
+        self.completed: list = []
+        self._breakers: dict = {}
+        self._threads = [threading.Thread(target=self._main_loop, name="sle-main", daemon=True)]
+        self._threads += [threading.Thread(target=self._area_loop, name=f"sle-area{k}", daemon=True)
+                          for k in range(self.slots)]
+
+    # -- control -----------------------------------------------------------
+    def start(self):
+        self._t0 = self.clock()
+        for t in self._threads:
+            t.start()
+        return self._t0
+
+    def now(self) -> float:
+        return self.clock() - self._t0
+
+    def estimate_bytes(self, grid) -> int:
+        return int(self.bytes_per_bus * grid.n)
+
+    def submit(self, req: Request):
+        with self._cv:
+            if req.kind == AREA:
+                if self.estimate_bytes(req.payload) > self.budget:
+                    req.error = "area larger than the HBM budget"
+                    req.done = self.now()
+                    self.completed.append(req)
+                    return
+                self._area_q.append(req)
+            else:
+                self._main_q.append(req)
+            self._cv.notify_all()
+
+    def close(self, timeout: float = 600.0):
+        """Wait until every submitted request is done, then stop the workers."""
+        with self._cv:
+            self._closing = True
+            self._cv.notify_all()
+        for t in self._threads:
+            t.join(timeout)
+
+    # -- workers -----------------------------------------------------------
+    def _main_loop(self):
+        while True:
+            with self._cv:
+                while not self._main_q and not self._closing:
+                    self._cv.wait()
+                if not self._main_q:
+                    return
+                req = self._main_q.popleft()
+            req.start = self.now()
+            try:
+                if req.kind == UPDATE:
+                    idx, val = req.payload
+                    if req.record:
+                        V, th = self.main.state()
+                        req.before = (V, th, dict(self._breakers))
+                    self.main.set_telemetry_indexed(idx, val)
+                else:
+                    k, st = req.payload
+                    self.main.set_branch_status(k, st)
+                    self._breakers[k] = st
+                r = self.main.solve(mode=0, max_iterations=self.max_iterations, tolerance=self.tol)
+                req.status, req.iterations = r["status"], r["iterations"]
+                if req.record:
+                    req.result = self.main.state()
+            except Exception as exc:  # the request fails, the loop goes on
+                req.error = f"{type(exc).__name__}: {exc}"
+            req.done = self.now()
+            with self._cv:
+                self.completed.append(req)
+
+    def _area_loop(self):
+        while True:
+            with self._cv:
+                while True:
+                    if self._area_q:
+                        need = self.estimate_bytes(self._area_q[0].payload)
+                        if self._used + need <= self.budget:
+                            break
+                    elif self._closing:
+                        return
+                    self._cv.wait()
+                req = self._area_q.popleft()  # first come, first served
+                need = self.estimate_bytes(req.payload)
+                self._used += need
+                self.peak_used = max(self.peak_used, self._used)
+                self._running_areas += 1
+            req.start = self.now()
+            e = None
+            try:
+                e = self.make_area_engine(req.payload)
+                r = e.solve(mode=1, max_iterations=50, tolerance=self.tol, flat_start=True)
+                req.status, req.iterations = r["status"], r["iterations"]
+                req.result = e.state() if req.record else None
+                used = e.gpu_bytes() if hasattr(e, "gpu_bytes") else 0
+                if used > need:  # learn from the overrun: later admissions use the larger figure
+                    with self._cv:
+                        self.estimate_misses += 1
+                        self.bytes_per_bus = max(self.bytes_per_bus, 1.1 * used / max(req.payload.n, 1))
+            except Exception as exc:
+                req.error = f"{type(exc).__name__}: {exc}"
+            finally:
+                if e is not None:
+                    e.close()
+            req.done = self.now()
+            with self._cv:
+                self._used -= need
+                self._running_areas -= 1
+                self.completed.append(req)
+                self._cv.notify_all()
+
+
+def poisson_arrivals(rate: float, duration: float, rng: np.random.Generator) -> np.ndarray:
+    t, out = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= duration:
+            return np.array(out)
+        out.append(t)
+
+
+def run(server: Server, requests: list, sleep: Callable[[float], None] = time.sleep) -> list:
+    """Release each request at its arrival time, then wait for all of them."""
+    requests = sorted(requests, key=lambda r: r.arrival)
+    server.start()
+    for req in requests:
+        wait = req.arrival - server.now()
+        if wait > 0:
+            sleep(wait)
+        server.submit(req)
+    server.close()
+    return server.completed
+
+
+def summarize(done: list, warmup: float = 0.0) -> dict:
+    out = {}
+    for kind in (UPDATE, TOGGLE, AREA):
+        rs = [r for r in done if r.kind == kind and r.arrival >= warmup and r.done is not None]
+        lat = np.array([r.latency for r in rs if r.error is None]) * 1e3
+        out[kind] = {
+            "count": len(rs),
+            "failed": sum(1 for r in rs if r.error is not None),
+            "converged": sum(1 for r in rs if r.status == 0),
+            "p50_ms": float(np.percentile(lat, 50)) if lat.size else None,
+            "p99_ms": float(np.percentile(lat, 99)) if lat.size else None,
+            "max_ms": float(lat.max()) if lat.size else None,
+        }
+    return out

@@ -522,6 +522,10 @@ class Engine:
         d = self.L.sle_ext_Describe(self.h)
         return d.decode() if d else ""
 
+    def gpu_bytes(self) -> int:
+        """Device memory held by this engine (sle_GetGPUMemoryUsage)."""
+        return int(self.L.sle_GetGPUMemoryUsage(self.h))
+
     def autotune(self, repetitions=5):
         """Time the solve plan candidates on the current factor and keep the fastest."""
         self._check(self.L.sle_ext_AutotuneSolve(self.h, int(repetitions)), "sle_ext_AutotuneSolve")

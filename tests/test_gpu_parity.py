@@ -159,10 +159,13 @@ def test_huber_matches_oracle():
         e.close()
 
 
-def test_breaker_toggle_redoes_symbolic():
+def test_breaker_toggle_matches_oracle():
+    """Opening and closing a breaker inside an island: values change, the
+    structure does not (no symbolic redo); states match the oracle."""
     g = GRIDS["geo2k"]()
     e = engine_for(g, double_precision=True)
     e.solve(mode=1, max_iterations=50, tolerance=1e-8, flat_start=True)
+    before = e.ordering()
     # open a branch whose removal keeps the grid connected: one on a cycle
     o = Oracle(g)
     k = int(np.argmax(g.x))  # any branch; connectivity checked below
@@ -189,6 +192,11 @@ def test_breaker_toggle_redoes_symbolic():
     es, os_ = e.ordering(), o.symbolic()
     for key in ("bus_perm", "perm", "parent", "snode_ptr"):
         assert np.array_equal(es[key], os_[key]), key
+        assert np.array_equal(es[key], before[key]), key
+    # the open branch's admittance is gone from Ybus (its entries stay, at zero)
+    _, _, G, B = e.ybus()
+    _, _, oG, oB = o.ybus()
+    assert rel_err(G, oG) < 1e-12 and rel_err(B, oB) < 1e-12
     o.solve(mode=1, max_iter=50, tol=1e-11, flat_start=True)
     oV, oth = o.state()
     V, th = e.state()

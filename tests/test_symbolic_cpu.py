@@ -55,9 +55,12 @@ def test_patterns_and_ordering_match_oracle(name):
     e.close()
 
 
-def test_open_branch_changes_patterns_like_oracle():
+def test_open_branch_keeps_structure_like_oracle():
+    """Opening a breaker inside an island changes values only: the patterns,
+    ordering and supernodes stay those of the closed network (the open branch
+    contributes zero admittance, its flow rows zero weight), in the library and
+    in the oracle alike."""
     g = gg.geometric(400, seed=8, flow_meas="all")
-    # open a branch on a cycle (keep the grid connected)
     import scipy.sparse as sp
     import scipy.sparse.csgraph as cg
 
@@ -71,18 +74,49 @@ def test_open_branch_changes_patterns_like_oracle():
             break
     assert k is not None
     e = host_engine(g)
+    before = (e.h_pattern(), e.g_pattern(), e.ordering())
     e.set_branch_status(k, 0)
     o = Oracle(g)
     o.set_branch_status(k, 0)
     rp, col = e.h_pattern()
     orp, ocol = o.h_pattern()
     assert np.array_equal(rp, orp) and np.array_equal(col, ocol)
-    # flow rows of the open branch are empty
-    flows_k = np.nonzero((g.m_loc == k) & np.isin(g.m_type, (4, 5)))[0]
-    assert flows_k.size and np.all(rp[flows_k + 1] == rp[flows_k])
+    assert np.array_equal(rp, before[0][0]) and np.array_equal(col, before[0][1])
+    gc, gr = e.g_pattern()
+    assert np.array_equal(gc, before[1][0]) and np.array_equal(gr, before[1][1])
     es, os_ = e.ordering(), o.symbolic()
     for key in ("bus_perm", "perm", "parent", "colcount", "snode_ptr"):
         assert np.array_equal(es[key], os_[key]), key
+        assert np.array_equal(es[key], before[2][key]), key
+    e.close()
+
+
+def test_splitting_an_island_moves_the_angle_reference_like_oracle():
+    """Two areas joined by one tie branch form one island (one angle reference,
+    the SLACK bus of higher index); opening the tie splits them and each keeps
+    its own reference: one state less, and a new symbolic structure."""
+    a, b = gg.geometric(200, seed=51, flow_meas="all"), gg.geometric(150, seed=52, flow_meas="all")
+    g = gg.concat([a, b])
+    # tie branch between bus 0 of area b and bus 1 of area a
+    g.br_from = np.append(g.br_from, a.n).astype(np.int32)
+    g.br_to = np.append(g.br_to, 1).astype(np.int32)
+    for f, v in (("r", 0.02), ("x", 0.1), ("b", 0.0), ("tap", 1.0), ("shift", 0.0)):
+        setattr(g, f, np.append(getattr(g, f), v))
+    g.is_xfmr = np.append(g.is_xfmr, 0).astype(np.int32)
+    g.status = np.append(g.status, 1).astype(np.uint8)
+    e = host_engine(g)
+    o = Oracle(g)
+    assert e.sizes()["ns"] == 2 * g.n - 1
+    tie = g.nb - 1
+    e.set_branch_status(tie, 0)
+    o.set_branch_status(tie, 0)
+    assert e.sizes()["ns"] == 2 * g.n - 2 == o.ns
+    es, os_ = e.ordering(), o.symbolic()
+    for key in ("bus_perm", "perm", "parent", "colcount", "snode_ptr"):
+        assert np.array_equal(es[key], os_[key]), key
+    rp, col = e.h_pattern()
+    orp, ocol = o.h_pattern()
+    assert np.array_equal(rp, orp) and np.array_equal(col, ocol)
     e.close()
 
 
