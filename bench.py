@@ -38,6 +38,7 @@ HBM_PEAK_GBS = 8000.0
 TOL = 1e-6
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_200k.md")
 AREA_MAX_ITER = 25  # a batch iterates until its slowest area converges
+AREA_MODE = 1  # PRECISION: a perturbed start refactors (the cached factor of the old point may not contract)
 COPY_BYTES = 512 << 20
 
 
@@ -220,7 +221,7 @@ def run_areas(args):
     log-uniform in [500, 20000], redundancy ~3) solved as one model -- every
     kernel launch covers all areas -- against one engine per area solved one
     after the other.  A step re-solves every area from its previous solution
-    perturbed by N(0, 0.01) per state element (REALTIME, cached factor, tol 1e-6)."""
+    perturbed by N(0, 0.01) per state element (PRECISION: refactor every iteration, tol 1e-6)."""
     from state_estimation_with_cuda_amd import gridgen, sle
 
     t0 = time.perf_counter()
@@ -250,12 +251,12 @@ def run_areas(args):
     starts = [gridgen.hot_start(g, V0, th0, 0.01, seed=50 + k) for k in range(args.warmup + args.steps)]
     for k in range(args.warmup):
         eb.set_state(*starts[k])
-        eb.solve(mode=0, max_iterations=AREA_MAX_ITER, tolerance=TOL)
+        eb.solve(mode=AREA_MODE, max_iterations=AREA_MAX_ITER, tolerance=TOL)
     tb, iters, ok = 0.0, [], 0
     for k in range(args.warmup, args.warmup + args.steps):
         eb.set_state(*starts[k])  # host upload of the perturbed start, outside the timed solve
         ts = time.perf_counter()
-        r = eb.solve(mode=0, max_iterations=AREA_MAX_ITER, tolerance=TOL)
+        r = eb.solve(mode=AREA_MODE, max_iterations=AREA_MAX_ITER, tolerance=TOL)
         tb += time.perf_counter() - ts
         iters.append(r["iterations"])
         ok += r["status"] == 0
@@ -277,7 +278,7 @@ def run_areas(args):
         for a, e, b0 in zip(areas, engines, offs):
             e.set_state(V[b0:b0 + a.n], th[b0:b0 + a.n])
             ts = time.perf_counter()
-            ra = e.solve(mode=0, max_iterations=AREA_MAX_ITER, tolerance=TOL)
+            ra = e.solve(mode=AREA_MODE, max_iterations=AREA_MAX_ITER, tolerance=TOL)
             tp += time.perf_counter() - ts
             p_iters.append(ra["iterations"])
             p_ok += ra["status"] == 0
@@ -303,7 +304,7 @@ def run_areas(args):
         "dtype": "f64" if fp64 else "f32+f64",
         "data": "synthetic",
         "config": {"workload": f"{len(areas)} independent areas, n log-uniform [500, 20000] (sum {g.n}), m {g.m}; "
-                               "hot start = previous solution + N(0, 0.01); REALTIME to 1e-6",
+                               "hot start = previous solution + N(0, 0.01); PRECISION to 1e-6",
                    "form": args.form},
         "per_instance_solves_per_s": round(per_rate, 1),
         "per_instance_mean_iterations": float(np.mean(p_iters)),
@@ -343,6 +344,7 @@ def run_serve(args):
     arrival to result."""
     from state_estimation_with_cuda_amd import gridgen, serve, sle
 
+    sys.setswitchinterval(1e-4)  # the request threads hand the interpreter over quickly
     rng = np.random.default_rng(4)
     g = gridgen.workload_200k()
     fp64 = args.form == "fp64"
@@ -383,7 +385,7 @@ def run_serve(args):
         reqs.append(serve.Request(serve.UPDATE, float(t), (idx, val), rid=q, record=rec))
         if rec and len(replay) < args.replays:
             replay[q] = (z.copy(), dict(status))
-        if (q + 1) % 50 == 0 and brks:
+        if args.toggle_every and (q + 1) % args.toggle_every == 0 and brks:
             if open_brk is None:
                 open_brk = brks[(q // 100) % len(brks)]
                 op = (open_brk, 0)
@@ -400,6 +402,8 @@ def run_serve(args):
     done = serve.run(srv, reqs)
     wall = time.perf_counter() - t0
     summ = serve.summarize(done, warmup=args.serve_warmup)
+    bad = sorted(r.arrival for r in done if r.kind == serve.UPDATE and r.status != 0)
+    summ["first_unconverged_update_s"] = round(bad[0], 3) if bad else None
     ok = [r for r in done if r.error is None and r.status == 0 and r.arrival >= args.serve_warmup]
     rate = len(ok) / max(wall - args.serve_warmup, 1e-9)
     # isolation: the same request alone gives the same result
@@ -489,6 +493,7 @@ def main():
     ap.add_argument("--area-slots", type=int, default=4)
     ap.add_argument("--hbm-budget-gib", type=float, default=64.0)
     ap.add_argument("--record-every", type=int, default=500)
+    ap.add_argument("--toggle-every", type=int, default=50, help="serve: updates between breaker operations (0: none)")
     ap.add_argument("--replays", type=int, default=4)
     ap.add_argument("--areas", type=int, default=256)
     ap.add_argument("--form", choices=["fp64", "mixed"], default="fp64", help="areas workload: precision form")

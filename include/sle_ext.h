@@ -50,6 +50,11 @@ typedef struct SLE_ExtResult {
  *                     with every factorization) instead of level sweeps
  *   "pinv_cut"        fronts per bottom subtree under it (0: default, 32)
  *   "pinv_waves"      waves per bottom-subtree workgroup under it (default 4)
+ *   "huber_newton"    1 (default): Huber estimation takes IRLS gain weights
+ *                     (the reference's, kernels.cu:634-671) until the steps fall
+ *                     below 1e-2, then the Newton gain weights psi'(r/sigma)/sigma^2
+ *                     (outliers leave the gain, not the right-hand side): the
+ *                     same minimiser in far fewer iterations; 0: IRLS throughout
  *   "level_schedule"  1 (default): factor and solve level by level (kernel
  *                     boundaries order the fronts); 0: one dependency-driven
  *                     launch per sweep with release/acquire flags per front */

@@ -229,7 +229,8 @@ class Oracle:
         return lp, lr, lv
 
     def solve(self, mode=1, max_iter=100, tol=1e-6, robust=False, gamma=1.5, flat_start=False, vmin=0.5, vmax=1.5):
-        p = SolveParams(int(mode), int(max_iter), float(tol), int(bool(robust)), float(gamma), int(bool(flat_start)),
+        # robust: False/0 WLS, True/1 Huber IRLS (the reference's weights), 2 Huber with Newton gain weights
+        p = SolveParams(int(mode), int(max_iter), float(tol), int(robust), float(gamma), int(bool(flat_start)),
                         float(vmin), float(vmax))
         r = SolveResult()
         lib().or_solve(self.h, C.byref(p), C.byref(r))

@@ -51,6 +51,7 @@
 namespace {
 
 constexpr double kEps = 1e-12;
+constexpr double kNewtonSwitch = 1e-2;  // Huber-Newton: step size below which the gain drops outliers
 enum { V_MAG = 0, V_ANGLE = 1, P_INJ = 2, Q_INJ = 3, P_FLOW = 4, Q_FLOW = 5, I_MAG = 6, P_PSEUDO = 7, Q_PSEUDO = 8 };
 
 struct Case {
@@ -829,7 +830,7 @@ void or_solve(void* hh, const OracleSolveParams* prm, OracleSolveResult* out) {
     auto t0 = std::chrono::steady_clock::now();
     OracleSolveResult R{};
     R.status = 5;
-    std::vector<double> hx(c.m), P(c.n), Q(c.n), Pf(c.nb), Qf(c.nb), Pt(c.nb), Qt(c.nb), r(c.m), w(c.m);
+    std::vector<double> hx(c.m), P(c.n), Q(c.n), Pf(c.nb), Qf(c.nb), Pt(c.nb), Qt(c.nb), r(c.m), w(c.m), wg(c.m);
     std::vector<double> hv(c.hcol.size()), b(c.ns), dx, g;
     std::vector<int> tptr, trow, tpos;
     h_transpose(c, tptr, trow, tpos);
@@ -850,7 +851,10 @@ void or_solve(void* hh, const OracleSolveParams* prm, OracleSolveResult* out) {
         }
         bool converged = false, diverged = false, singular = false;
         R.iterations = 0;
+        double last_step = 1e300;
         for (int it = 0; it < prm->max_iter; ++it) {
+            // robust == 2: IRLS gain weights until the steps are small, then Newton's
+            const bool newton = prm->robust == 2 && last_step < kNewtonSwitch;
             eval_power(c, c.V.data(), c.th.data(), P.data(), Q.data(), Pf.data(), Qf.data(), Pt.data(), Qt.data());
             eval_h(c, c.V.data(), c.th.data(), P.data(), Q.data(), Pf.data(), Qf.data(), Pt.data(), Qt.data(),
                    hx.data());
@@ -863,12 +867,17 @@ void or_solve(void* hh, const OracleSolveParams* prm, OracleSolveResult* out) {
                     if (u > prm->gamma) wi = prm->gamma / (s * std::fabs(r[i]));
                 }
                 w[i] = c.active[i] ? wi : 0.0;
+                // robust == 2: Newton steps on the Huber objective -- the gain takes
+                // the curvature psi'(u)/sigma^2 (zero beyond gamma), the right-hand
+                // side stays H^T psi(u)/sigma; same minimiser as the IRLS form
+                wg[i] = w[i];
+                if (newton && std::fabs(r[i]) / s > prm->gamma) wg[i] = 0.0;
             }
             eval_jacobian(c, c.V.data(), c.th.data(), P.data(), Q.data(), Pf.data(), Qf.data(), Pt.data(),
                           Qt.data(), hv.data());
             bool refactor = prm->robust || !c.factor_valid || prm->mode == 1;
             if (refactor) {
-                gain(c, hv.data(), w.data(), &g);
+                gain(c, hv.data(), wg.data(), &g);
                 if (!factor(c, g)) {
                     singular = true;
                     c.factor_valid = false;
@@ -891,6 +900,7 @@ void or_solve(void* hh, const OracleSolveParams* prm, OracleSolveResult* out) {
             }
             R.iterations = it + 1;
             R.max_mismatch = mx;
+            last_step = mx;
             if (!finite) {
                 diverged = true;
                 break;

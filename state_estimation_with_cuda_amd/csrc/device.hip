@@ -56,6 +56,8 @@ constexpr int kFrontA = 32;    // fronts with c <= 32: LDS front of 32x32
 constexpr int kFrontB = 64;    // 32 < c <= 64: LDS front of 64x64
 constexpr int kVecMax = 2048;  // triangular-solve LDS vector bound for large fronts
 constexpr int kSpinLimit = 1 << 22;  // polls before a dependency wait gives up (~0.1 s)
+constexpr double kNewtonSwitch = 1e-2;  // Huber-Newton: step size below which the gain drops outliers
+constexpr double kRefreshRatio = 0.25;  // REALTIME: step ratio above which the cached factor is refreshed
 
 // measurement code byte: type (4 bits) | end << 4 | active << 5 | from-first << 6
 // (from-first: the branch's from bus precedes its to bus in the natural state
@@ -208,6 +210,7 @@ __global__ __launch_bounds__(kBlock) void k_flows(int nb, const int32_t* __restr
 struct MeasArgs {
     int m, n;
     int robust, unit_weights, want_jac;
+    int newton;  // Huber with Newton gain weights: outliers leave the gain (not the RHS)
     const int32_t* pth;  // angle state of each bus (-1 for an angle reference)
     double gamma;
     const uint8_t* mcode;
@@ -244,11 +247,12 @@ struct MeasArgs {
 
 __global__ __launch_bounds__(kBlock) void k_meas(MeasArgs A, double* __restrict__ hs) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
-    double rsi = 0.0;
-    // one Jacobian entry: row-major value and/or its product with sqrt(w) r
-    auto put = [&](int q, double v) {
-        if (A.want_jac) hs[q] = v;
-        if (A.hr) A.hr[A.hpos[q]] = v * rsi;
+    double swg = 0.0, wr = 0.0;
+    // one Jacobian entry d: sqrt(w_gain) d into the row-major Jacobian (for the
+    // gain) and/or d w r into the H^T-ordered right-hand-side products
+    auto put = [&](int q, double d) {
+        if (A.want_jac) hs[q] = swg * d;
+        if (A.hr) A.hr[A.hpos[q]] = d * wr;
     };
     double jw = 0.0, rn = -1.0;
     int ri = -1;
@@ -321,14 +325,20 @@ __global__ __launch_bounds__(kBlock) void k_meas(MeasArgs A, double* __restrict_
         const double sg = A.sigma[i];
         const double wb = 1.0 / (sg * sg);
         double w = wb;
+        bool outlier = false;
         if (A.robust) {
             double ar = fabs(r);
-            if (ar / sg > A.gamma) w = A.gamma / (sg * ar);
+            if (ar / sg > A.gamma) {
+                w = A.gamma / (sg * ar);
+                outlier = true;
+            }
         }
         if (!act) w = 0.0;
-        const double sw = A.unit_weights ? (act ? 1.0 : 0.0) : sqrt(w);
-        rsi = sw * r;
-        A.rs[i] = rsi;
+        const double unit = act ? 1.0 : 0.0;
+        const double sw = A.unit_weights ? unit : sqrt(w);
+        swg = A.unit_weights ? unit : (A.newton && outlier ? 0.0 : sw);
+        wr = A.unit_weights ? unit * r : w * r;
+        A.rs[i] = sw * r;
         if (A.hout) {
             A.hout[i] = act ? hv : 0.0;
             A.rout[i] = r;
@@ -340,7 +350,7 @@ __global__ __launch_bounds__(kBlock) void k_meas(MeasArgs A, double* __restrict_
         }
         if ((A.want_jac || A.hr) && q1 > q0) {
             if (t == 0 || t == 1) {
-                put(q0, sw);
+                put(q0, 1.0);
             } else if (t == 2 || t == 3 || t == 7 || t == 8) {
                 const bool isP = (t == 2 || t == 7);
                 const int e0 = A.yrow[l], e1 = A.yrow[l + 1];
@@ -369,24 +379,24 @@ __global__ __launch_bounds__(kBlock) void k_meas(MeasArgs A, double* __restrict_
                             dv = qe / Vj;
                         }
                     }
-                    if (A.pth[j] >= 0) put(q0 + kth++, sw * dth);
-                    put(q0 + nth + (e - e0), sw * dv);
+                    if (A.pth[j] >= 0) put(q0 + kth++, dth);
+                    put(q0 + nth + (e - e0), dv);
                 }
             } else {
                 // sorted columns: theta_min, theta_max, V_min, V_max (reference angles absent)
                 int q = q0;
                 if (fb == tb) {
-                    if (A.pth[fb] >= 0) put(q++, sw * (d[0] + d[1]));
-                    put(q, sw * (d[2] + d[3]));
+                    if (A.pth[fb] >= 0) put(q++, d[0] + d[1]);
+                    put(q, d[2] + d[3]);
                 } else {
                     const bool fl = (code >> 6) & 1;
                     int lo = fl ? fb : tb, hi = fl ? tb : fb;
                     double dlo_t = fl ? d[0] : d[1], dhi_t = fl ? d[1] : d[0];
                     double dlo_v = fl ? d[2] : d[3], dhi_v = fl ? d[3] : d[2];
-                    if (A.pth[lo] >= 0) put(q++, sw * dlo_t);
-                    if (A.pth[hi] >= 0) put(q++, sw * dhi_t);
-                    put(q++, sw * dlo_v);
-                    put(q, sw * dhi_v);
+                    if (A.pth[lo] >= 0) put(q++, dlo_t);
+                    if (A.pth[hi] >= 0) put(q++, dhi_t);
+                    put(q++, dlo_v);
+                    put(q, dhi_v);
                 }
             }
         }
@@ -1354,6 +1364,14 @@ __global__ __launch_bounds__(kBlock) void k_scatter_z(int count, const int32_t* 
     if (q < count) z[idx[q]] = val[q];
 }
 
+// z[row of measurement idx[q]] = val[q] (idx in insertion order, mapped on the device)
+__global__ __launch_bounds__(kBlock) void k_scatter_z_mapped(int count, const int32_t* __restrict__ idx,
+                                                             const double* __restrict__ val,
+                                                             const int32_t* __restrict__ lm, double* __restrict__ z) {
+    int q = blockIdx.x * kBlock + threadIdx.x;
+    if (q < count) z[lm[idx[q]]] = val[q];
+}
+
 // natural-order vector <-> permuted solve vector (debug solve only)
 template <typename T>
 __global__ void k_permute_in(int ns, const int32_t* __restrict__ perm, const double* __restrict__ b,
@@ -1802,6 +1820,10 @@ struct Instance::Impl {
     // measurements
     DBuf<uint8_t> mcode;
     std::vector<uint8_t> mcode_h;  // host copy (device row order)
+    std::vector<uint8_t> status_h;  // host copy of the breaker status (device branch order)
+    std::vector<int32_t> br_rows_ptr, br_rows;  // device rows of the measurements on each branch
+    std::vector<uint8_t> m_present_h;           // measurement in the structure (device row order)
+    bool topo_staged = false;
     DBuf<int32_t> mloc, hrow;
     DBuf<double> z, sigma, rs, hx, r;
     DBuf<double> hs;  // sqrt(w) * Jacobian values in the H pattern (FP64 in both forms)
@@ -1822,7 +1844,10 @@ struct Instance::Impl {
     std::vector<int32_t> lb, bl, lk, kl, lm, ml;
     std::vector<int32_t> qmap;    // natural H position -> device H position
     std::vector<int32_t> yrow_d;  // device Ybus row pointers
-    DBuf<int32_t> ml_d;
+    DBuf<int32_t> ml_d, lm_d;
+    int32_t* pin_idx = nullptr;  // pinned staging of indexed telemetry
+    double* pin_val = nullptr;
+    size_t pin_cap = 0;
     DBuf<double> ztmp;
     DBuf<int2> gpair;
     DBuf<int32_t> sn_first, sn_k, sn_c, child_ptr, child, rmap, rows, perm, pth, pv, lists;
@@ -1996,7 +2021,9 @@ Instance::~Instance() {
     rel(d_->hs); rel(d_->hcolp); rel(d_->cg_b); rel(d_->cg_x); rel(d_->cg_r); rel(d_->cg_p); rel(d_->cg_q);
     rel(d_->cg_z); rel(d_->cg_t); rel(d_->cg_parts); rel(d_->cg_s); rel(d_->z); rel(d_->sigma); rel(d_->rs); rel(d_->hx); rel(d_->r); rel(d_->jpart); rel(d_->rpart);
     rel(d_->mpart); rel(d_->ipart); rel(d_->zidx); rel(d_->zval); rel(d_->tptr); rel(d_->tpos); rel(d_->trow);
-    rel(d_->tout); rel(d_->ml_d); rel(d_->ztmp); rel(d_->hpos); rel(d_->hr);
+    rel(d_->tout); rel(d_->ml_d); rel(d_->lm_d); rel(d_->ztmp);
+    if (d_->pin_idx) (void)hipHostFree(d_->pin_idx);
+    if (d_->pin_val) (void)hipHostFree(d_->pin_val); rel(d_->hpos); rel(d_->hr);
     rel(d_->gptr); rel(d_->gloc); rel(d_->gdiag); rel(d_->g_sn_ptr); rel(d_->gpair); rel(d_->sn_first);
     rel(d_->sn_k); rel(d_->sn_c); rel(d_->child_ptr); rel(d_->child); rel(d_->rmap); rel(d_->rows);
     rel(d_->perm); rel(d_->pth); rel(d_->pv); rel(d_->lists); rel(d_->foff); rel(d_->rmoff); rel(d_->uoff);
@@ -2112,8 +2139,23 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
     D.bc.upload(br_perm(h.bc), bytes_, s);
     D.tap.upload(br_perm(h.tap), bytes_, s);
     D.shift.upload(br_perm(h.shift), bytes_, s);
-    D.status.upload(br_perm(h.status), bytes_, s);
+    D.status_h = br_perm(h.status);
+    D.status.upload(D.status_h, bytes_, s);
     HIPCHECK(hipStreamSynchronize(s));
+    {
+        D.br_rows_ptr.assign(nb + 1, 0);
+        for (int i = 0; i < m; ++i)
+            if (!is_bus_measurement(h.mtype[i])) D.br_rows_ptr[h.mloc[i] + 1]++;
+        for (int k = 0; k < nb; ++k) D.br_rows_ptr[k + 1] += D.br_rows_ptr[k];
+        D.br_rows.assign(D.br_rows_ptr[nb], 0);
+        std::vector<int32_t> f(D.br_rows_ptr.begin(), D.br_rows_ptr.end() - 1);
+        for (int i = 0; i < m; ++i)
+            if (!is_bus_measurement(h.mtype[i])) D.br_rows[f[h.mloc[i]]++] = D.lm[i];
+        D.m_present_h.assign(m, 1);
+        if (!h.m_present.empty())
+            for (int i = 0; i < m; ++i) D.m_present_h[D.lm[i]] = h.m_present[i];
+        D.topo_staged = false;
+    }
     {
         // Ybus rows in bus layout order; entries keep their natural order (it
         // fixes the column order of injection rows of H)
@@ -2180,6 +2222,7 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
         D.z.upload(zz, bytes_, s);
         D.hrow.upload(hr, bytes_, s);
         D.ml_d.upload(D.ml, bytes_, s);
+        D.lm_d.upload(D.lm, bytes_, s);
         HIPCHECK(hipStreamSynchronize(s));
         D.qmap.assign(S.hcol.size(), 0);
         for (int i = 0; i < m; ++i)
@@ -2431,12 +2474,16 @@ void Instance::update_branches(const HostArrays& h) {
     HIPCHECK(hipSetDevice(device_));
     Impl& D = *d_;
     hipStream_t s = D.stream;
-    D.gs.upload(take(h.gs, D.kl), bytes_, s);
-    D.bs.upload(take(h.bs, D.kl), bytes_, s);
-    D.bc.upload(take(h.bc, D.kl), bytes_, s);
-    D.tap.upload(take(h.tap, D.kl), bytes_, s);
-    D.shift.upload(take(h.shift, D.kl), bytes_, s);
-    D.status.upload(take(h.status, D.kl), bytes_, s);
+    // host sources stay alive until the stream has consumed them
+    const std::vector<double> gs = take(h.gs, D.kl), bs = take(h.bs, D.kl), bc = take(h.bc, D.kl),
+                              tap = take(h.tap, D.kl), shift = take(h.shift, D.kl);
+    D.status_h = take(h.status, D.kl);
+    D.gs.upload(gs, bytes_, s);
+    D.bs.upload(bs, bytes_, s);
+    D.bc.upload(bc, bytes_, s);
+    D.tap.upload(tap, bytes_, s);
+    D.shift.upload(shift, bytes_, s);
+    D.status.upload(D.status_h, bytes_, s);
     // measurement activity (rows of open branches weigh zero)
     for (int p = 0; p < D.m; ++p)
         D.mcode_h[p] = (uint8_t)((D.mcode_h[p] & ~(1u << 5)) | ((h.active[D.ml[p]] ? 1u : 0u) << 5));
@@ -2447,6 +2494,37 @@ void Instance::update_branches(const HostArrays& h) {
                                                 D.shift.p, D.status.p, D.yG.p, D.yB.p);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(s));
+    factor_valid_ = false;
+    D.results_valid = false;
+}
+
+// One breaker operation: branch k (insertion index) closed or open.  Staged on
+// the host; refresh_topology() applies all staged operations at once.
+void Instance::set_branch_closed(int k, bool closed) {
+    Impl& D = *d_;
+    if (k < 0 || k >= D.nb) throw std::invalid_argument("branch index out of range");
+    D.status_h[D.lk[k]] = closed ? 1 : 0;
+    for (int q = D.br_rows_ptr[k]; q < D.br_rows_ptr[k + 1]; ++q) {
+        const int p = D.br_rows[q];
+        const unsigned on = closed && D.m_present_h[p] ? 1u : 0u;
+        D.mcode_h[p] = (uint8_t)((D.mcode_h[p] & ~(1u << 5)) | (on << 5));
+    }
+    D.topo_staged = true;
+}
+
+void Instance::refresh_topology() {
+    Impl& D = *d_;
+    if (!D.topo_staged) return;
+    HIPCHECK(hipSetDevice(device_));
+    hipStream_t s = D.stream;
+    D.status.upload(D.status_h, bytes_, s);
+    D.mcode.upload(D.mcode_h, bytes_, s);
+    const size_t nY = S_->ycol.size();
+    if (nY) k_ybus<<<nblk(nY), kBlock, 0, s>>>((int)nY, D.ycptr.p, D.ycon.p, D.gs.p, D.bs.p, D.bc.p, D.tap.p,
+                                                D.shift.p, D.status.p, D.yG.p, D.yB.p);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(s));
+    D.topo_staged = false;
     factor_valid_ = false;
     D.results_valid = false;
 }
@@ -2471,13 +2549,23 @@ void Instance::set_z_indexed(const int32_t* idx, const double* val, int count) {
         D.zidx.alloc(count, bytes_);
         D.zval.alloc(count, bytes_);
     }
-    std::vector<int32_t> di(count);
-    for (int q = 0; q < count; ++q) di[q] = D.lm[idx[q]];
-    D.zidx.upload(di.data(), count, D.stream);
-    D.zval.upload(val, count, D.stream);
-    k_scatter_z<<<nblk(count), kBlock, 0, D.stream>>>(count, D.zidx.p, D.zval.p, D.z.p);
+    // through pinned staging (a pageable copy of this size costs several times more)
+    if (D.pin_cap < (size_t)count) {
+        if (D.pin_idx) HIPCHECK(hipHostFree(D.pin_idx));
+        if (D.pin_val) HIPCHECK(hipHostFree(D.pin_val));
+        D.pin_idx = nullptr;
+        D.pin_val = nullptr;
+        HIPCHECK(hipHostMalloc((void**)&D.pin_idx, (size_t)count * sizeof(int32_t), hipHostMallocDefault));
+        HIPCHECK(hipHostMalloc((void**)&D.pin_val, (size_t)count * sizeof(double), hipHostMallocDefault));
+        D.pin_cap = count;
+    }
+    std::memcpy(D.pin_idx, idx, (size_t)count * sizeof(int32_t));
+    std::memcpy(D.pin_val, val, (size_t)count * sizeof(double));
+    HIPCHECK(hipMemcpyAsync(D.zidx.p, D.pin_idx, (size_t)count * sizeof(int32_t), hipMemcpyHostToDevice, D.stream));
+    HIPCHECK(hipMemcpyAsync(D.zval.p, D.pin_val, (size_t)count * sizeof(double), hipMemcpyHostToDevice, D.stream));
+    k_scatter_z_mapped<<<nblk(count), kBlock, 0, D.stream>>>(count, D.zidx.p, D.zval.p, D.lm_d.p, D.z.p);
     HIPCHECK(hipGetLastError());
-    HIPCHECK(hipStreamSynchronize(D.stream));
+    HIPCHECK(hipStreamSynchronize(D.stream));  // the staging buffers are reused by the next call
 }
 
 void Instance::set_state(const double* V, const double* th) {
@@ -2541,7 +2629,7 @@ void launch_power(Instance::Impl& D) {
 }
 
 void launch_meas(Instance::Impl& D, int robust, double gamma, int unit_weights, int want_jac, bool want_out,
-                 bool want_rhs = false) {
+                 bool want_rhs = false, bool newton = false) {
     MeasArgs A;
     A.m = D.m;
     A.n = D.n;
@@ -2549,6 +2637,7 @@ void launch_meas(Instance::Impl& D, int robust, double gamma, int unit_weights, 
     A.robust = robust;
     A.unit_weights = unit_weights;
     A.want_jac = want_jac;
+    A.newton = newton ? 1 : 0;
     A.gamma = gamma;
     A.mcode = D.mcode.p;
     A.mloc = D.mloc.p;
@@ -2837,7 +2926,10 @@ template <typename T>
 void launch_pcg(Instance::Impl& D, int steps) {
     const int ns = D.ns, nb = nblk(ns);
     CgScalars* cs = D.cg_s.p;
-    k_rhs<double><<<nb, kBlock, 0, D.stream>>>(ns, D.tptr.p, D.tpos.p, D.trow.p, D.tout.p, D.hs.p, D.rs.p, D.cg_b.p);
+    // b = H^T W r from the products of the measurement pass (the gain weights
+    // of the CG operator may differ from W under the Huber-Newton form)
+    k_rhs_sum<double><<<nblk((int64_t)ns * kRhsLanes), kBlock, 0, D.stream>>>(ns, D.tptr.p, D.tout.p, D.hr.p,
+                                                                              D.cg_b.p);
     k_cg_init<<<nb, kBlock, 0, D.stream>>>(ns, D.cg_b.p, D.cg_x.p, D.cg_r.p);
     launch_precond<T>(D, D.cg_r.p, D.cg_z.p);
     launch_dot(D, D.cg_r.p, D.cg_z.p, &cs->rz[0]);
@@ -2902,7 +2994,8 @@ void prof_end(Prof* pf, int ph, int launches) {
 }
 
 template <typename T>
-void iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p, bool refactor, T piv_tol, Prof& pf) {
+void iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p, bool refactor, T piv_tol, Prof& pf,
+               bool newton) {
     pf.begin(PH_POWER);
     launch_power(D);
     pf.end(PH_POWER, 1);
@@ -2910,7 +3003,7 @@ void iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p, bool 
     // row-major Jacobian only where G or the CG product needs it; the RHS
     // products always
     const bool direct = p.pcg_steps <= 0;
-    launch_meas(D, p.robust, p.gamma, 0, (refactor || !direct) ? 1 : 0, false, direct);
+    launch_meas(D, p.robust ? 1 : 0, p.gamma, 0, (refactor || !direct) ? 1 : 0, false, true, newton);
     pf.end(PH_MEAS, 1);
     if (refactor) {
         pf.begin(PH_SPGEMM);
@@ -2961,20 +3054,34 @@ SolveOutcome Instance::solve(const SolveParams& p) {
         int stalls = 0;
         out.iterations = 0;
         out.max_mismatch = 0;
+        double last_step = 1e300;
+        bool newton_off = false;
+        bool stale = false;  // REALTIME: the cached factor contracts too slowly, refresh it
         for (int it = 0; it < p.max_iter; ++it) {
             const bool fp64 = precision_ == 1 || fp64_factor_active_;
-            const bool refactor = p.robust || !factor_valid_ || p.mode == 1;
+            const bool refactor = p.robust || !factor_valid_ || p.mode == 1 || stale;
+            stale = false;
+            // Huber-Newton: the IRLS gain until the steps are small, then the
+            // gain without the outliers (same minimiser, faster contraction)
+            const bool newton = p.robust == 2 && !newton_off && last_step < kNewtonSwitch;
             if (fp64) {
                 if (!D.f64.ready) D.f64.alloc(S, bytes_, D.pi.sizes());
                 SolveParams pd = p;
                 pd.pcg_steps = 0;  // the FP64 factor solves the Newton system directly
-                iteration<double>(D, S, pd, refactor, 1e-13, pf);
+                iteration<double>(D, S, pd, refactor, 1e-13, pf, newton);
             } else {
-                iteration<float>(D, S, p, refactor, 1e-5f, pf);
+                iteration<float>(D, S, p, refactor, 1e-5f, pf, newton);
             }
             HIPCHECK(hipGetLastError());
             read_status(D);
             pf.collect(ktimes_);
+            if (D.hst->piv && newton) {
+                // the gain without the outliers lost rank: IRLS weights from here on
+                newton_off = true;
+                factor_valid_ = false;
+                out.newton_fallbacks++;
+                continue;
+            }
             if (D.hst->flow_err) {
                 // a dependency wait timed out: redo this iteration level by level
                 D.use_flow = false;
@@ -2999,6 +3106,10 @@ SolveOutcome Instance::solve(const SolveParams& p) {
             out.iterations = it + 1;
             const double prev = out.max_mismatch;
             out.max_mismatch = D.hst->maxdx;
+            last_step = D.hst->maxdx;
+            // a cached factor (REALTIME) taken at a point the state has since left
+            // contracts slowly: refresh it at the current point for the next step
+            if (!refactor && it > 0 && D.hst->maxdx > kRefreshRatio * prev && D.hst->maxdx >= p.tol) stale = true;
             if (!fp64 && it > 0 && (D.hst->nan || D.hst->maxdx > 0.9 * prev)) {
                 // the FP32 factor stopped contracting: continue with an FP64 factor
                 if (++stalls >= 2 || D.hst->nan) {

@@ -156,6 +156,8 @@ int Engine::set_option(const char* key, double value) {
         if (value < 1 || value > 16) return ST_INVALID_ARGUMENT;
         (k == "lds_waves_bottom" ? opt_.lds_waves_bottom : opt_.lds_waves_tail) = (int)value;
         model_dirty_ = true;  // the plan is rebuilt at the next upload
+    } else if (k == "huber_newton") {
+        opt_.huber_newton = value != 0;
     } else if (k == "pinv") {
         opt_.pinv = value != 0;
         if (inst_) inst_->set_partitioned_inverse(opt_.pinv, -1, -1);
@@ -735,6 +737,7 @@ int Engine::apply_switch_changes() {
         if (!br_removed_[k]) {
             br_status_[k] = (uint8_t)st;
             topo_dirty_ = true;
+            topo_changed_.push_back(k);
         }
     }
     return ST_OK;
@@ -746,28 +749,30 @@ int Engine::apply_switch_changes() {
 int Engine::apply_topology_values() {
     if (!topo_dirty_) return ST_OK;
     topo_dirty_ = false;
-    if (model_dirty_ || !inst_ || !sym_) {
-        model_dirty_ = true;
-        sym_dirty_ = true;
-        return ST_OK;
-    }
-    HostArrays h;
-    build_host_arrays(h);
+    std::vector<uint8_t> status(n_branch());
+    for (int k = 0; k < n_branch(); ++k) status[k] = (br_status_[k] && !br_removed_[k]) ? 1 : 0;
     ModelView mv;
-    mv.n = h.n;
-    mv.nb = h.nb;
-    mv.m = h.m;
-    mv.slack = h.slack;
-    mv.bus_type = h.bus_type.data();
-    mv.from = h.from.data();
-    mv.to = h.to.data();
-    mv.status = h.status.data();
-    if (angle_states(mv) != sym_->th_state) {
+    mv.n = n_bus();
+    mv.nb = n_branch();
+    mv.m = n_meas();
+    mv.slack = slack_;
+    mv.bus_type = bus_type_.data();
+    mv.from = br_from_.data();
+    mv.to = br_to_.data();
+    mv.status = status.data();
+    if (!sym_ || angle_states(mv) != sym_->th_state) {  // new state layout: symbolic redo
         model_dirty_ = true;
         sym_dirty_ = true;
+        topo_changed_.clear();
         return ST_OK;
     }
-    inst_->update_branches(h);
+    if (model_dirty_ || !inst_) {  // the next upload carries the status anyway
+        topo_changed_.clear();
+        return ST_OK;
+    }
+    for (int k : topo_changed_) inst_->set_branch_closed(k, status[k] != 0);
+    topo_changed_.clear();
+    inst_->refresh_topology();
     results_dirty_ = true;
     return ST_OK;
 }
@@ -780,6 +785,7 @@ int Engine::set_branch_status_index(int k, int status) {
         br_status_[k] = st;
         if (br_sd_[k] >= 0) sd_status_[br_sd_[k]] = st;
         topo_dirty_ = true;
+        topo_changed_.push_back(k);
     }
     return ST_OK;
 }
@@ -816,7 +822,7 @@ int Engine::solve(const SolverConfigIn& sc, ResultOut& out) {
     p.mode = sc.mode;
     p.max_iter = std::max(1, sc.max_iterations);
     p.tol = sc.tolerance;
-    p.robust = sc.use_robust ? 1 : 0;
+    p.robust = sc.use_robust ? (opt_.huber_newton ? 2 : 1) : 0;
     p.gamma = sc.huber_gamma;
     p.flat_start = sc.use_flat_start ? 1 : 0;
     p.vmin = cfg_.voltage_min_pu;

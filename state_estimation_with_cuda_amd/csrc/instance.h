@@ -37,7 +37,7 @@ struct SolveParams {
     int mode = 1;          // 0 REALTIME (reuse a valid factor), 1 PRECISION (refactor every iteration)
     int max_iter = 100;
     double tol = 1e-4;
-    int robust = 0;
+    int robust = 0;        // 0 WLS, 1 Huber IRLS (the reference's weights), 2 Huber with Newton gain weights
     double gamma = 1.5;
     int flat_start = 0;
     double vmin = 0.5, vmax = 1.5;
@@ -58,6 +58,7 @@ struct SolveOutcome {
     int largest_residual_idx = -1;
     double ms = 0;
     int flags = 0;  // bit 0: dependency-driven sweeps; bit 1: FP64 factor active (mixed form fell back)
+    int newton_fallbacks = 0;  // Huber-Newton gains that lost rank (IRLS used instead)
 };
 
 // Kernel classes timed with HIP events on the instance stream (profiling mode).
@@ -89,6 +90,10 @@ class Instance {
     // Branch parameters and breaker status (values only: the structure holds open
     // branches too); refreshes Ybus and the measurement activity.
     void update_branches(const HostArrays& h);
+    // Breaker operations without a full upload: stage any number of them, then
+    // refresh_topology() uploads status and activity and recomputes Ybus.
+    void set_branch_closed(int k, bool closed);
+    void refresh_topology();
     void set_z(const double* z);
     void set_z_indexed(const int32_t* idx, const double* val, int count);
     void set_state(const double* V, const double* th);

@@ -216,10 +216,14 @@ def summarize(done: list, warmup: float = 0.0) -> dict:
     for kind in (UPDATE, TOGGLE, AREA):
         rs = [r for r in done if r.kind == kind and r.arrival >= warmup and r.done is not None]
         lat = np.array([r.latency for r in rs if r.error is None]) * 1e3
+        its = np.array([r.iterations for r in rs if r.error is None])
         out[kind] = {
             "count": len(rs),
             "failed": sum(1 for r in rs if r.error is not None),
             "converged": sum(1 for r in rs if r.status == 0),
+            "status_counts": {str(k): int(v) for k, v in collections.Counter(r.status for r in rs).items()},
+            "mean_iterations": float(its.mean()) if its.size else None,
+            "max_iterations": int(its.max()) if its.size else None,
             "p50_ms": float(np.percentile(lat, 50)) if lat.size else None,
             "p99_ms": float(np.percentile(lat, 99)) if lat.size else None,
             "max_ms": float(lat.max()) if lat.size else None,

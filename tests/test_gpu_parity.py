@@ -240,3 +240,46 @@ def test_batched_areas_match_oracle_and_solo_solves(fp64):
         assert np.abs(th[b0:b0 + a.n] - tha).max() < STATE_TOL
         ea.close()
         b0 += a.n
+
+
+def test_realtime_updates_across_breaker_operations():
+    """REALTIME updates (cached factor) interleaved with breaker operations on
+    branches inside the island: every solve converges and matches the oracle
+    for the same telemetry and breaker status."""
+    import scipy.sparse.csgraph as cg
+
+    g = GRIDS["geo2k"]()
+    e = engine_for(g, double_precision=True)
+    assert e.solve(mode=1, max_iterations=50, tolerance=1e-8, flat_start=True)["status"] == 0
+    rng = np.random.default_rng(9)
+    cands = []
+    for k in rng.permutation(g.nb):
+        keep = np.ones(g.nb, bool)
+        keep[k] = False
+        A = sp.coo_matrix((np.ones(keep.sum()), (g.br_from[keep], g.br_to[keep])), shape=(g.n, g.n))
+        if cg.connected_components(A, directed=False)[0] == 1:
+            cands.append(int(k))
+            if len(cands) == 3:
+                break
+    z = g.z.copy()
+    status = g.status.copy()
+    for step in range(12):
+        if step % 4 == 3:
+            k = cands[(step // 4) % len(cands)]
+            status[k] = 1 - status[k]
+            e.set_branch_status(k, int(status[k]))
+        idx = np.sort(rng.choice(g.m, g.m // 10, replace=False)).astype(np.int32)
+        z[idx] = g.z[idx] + rng.normal(0, 1, idx.size) * g.sigma[idx]
+        e.set_telemetry_indexed(idx, z[idx])
+        r = e.solve(mode=0, max_iterations=20, tolerance=1e-8)
+        print(f"[step {step}] status={r['status']} iters={r['iterations']} refactors={r['refactors']}", flush=True)
+        assert r["status"] == 0, (step, r)
+        if step % 4 == 3:
+            g2 = gg.Grid(**{**g.__dict__})
+            g2.z, g2.status = z.copy(), status.copy()
+            o = Oracle(g2)
+            o.solve(mode=1, max_iter=50, tol=1e-11, flat_start=True)
+            oV, oth = o.state()
+            V, th = e.state()
+            assert np.abs(V - oV).max() < STATE_TOL and np.abs(th - oth).max() < STATE_TOL
+    e.close()
