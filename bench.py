@@ -404,6 +404,14 @@ def run_serve(args):
     summ = serve.summarize(done, warmup=args.serve_warmup)
     bad = sorted(r.arrival for r in done if r.kind == serve.UPDATE and r.status != 0)
     summ["first_unconverged_update_s"] = round(bad[0], 3) if bad else None
+    slow = sorted((r for r in done if r.kind != serve.AREA and r.done is not None), key=lambda r: -r.latency)[:8]
+    summ["long_service"] = [{"kind": r.kind, "rid": r.rid, "arrival_s": round(r.arrival, 4),
+                             "service_ms": round((r.done - r.start) * 1e3, 2), "iterations": r.iterations,
+                             "status": r.status} for r in done
+                            if r.kind != serve.AREA and r.done is not None and r.done - r.start > 0.05][:10]
+    summ["slowest"] = [{"kind": r.kind, "rid": r.rid, "arrival_s": round(r.arrival, 4),
+                        "wait_ms": round((r.start - r.arrival) * 1e3, 2), "service_ms": round((r.done - r.start) * 1e3, 2),
+                        "iterations": r.iterations} for r in slow]
     ok = [r for r in done if r.error is None and r.status == 0 and r.arrival >= args.serve_warmup]
     rate = len(ok) / max(wall - args.serve_warmup, 1e-9)
     # isolation: the same request alone gives the same result

@@ -52,9 +52,11 @@ typedef struct SLE_ExtResult {
  *   "pinv_waves"      waves per bottom-subtree workgroup under it (default 4)
  *   "huber_newton"    1 (default): Huber estimation takes IRLS gain weights
  *                     (the reference's, kernels.cu:634-671) until the steps fall
- *                     below 1e-2, then the Newton gain weights psi'(r/sigma)/sigma^2
- *                     (outliers leave the gain, not the right-hand side): the
- *                     same minimiser in far fewer iterations; 0: IRLS throughout
+ *                     below 1e-2, then gain weights gamma^3/(sigma^2 |u|^3) for the
+ *                     outliers (|u| = |r|/sigma > gamma), nearer the curvature of
+ *                     the Huber loss; the right-hand side keeps the IRLS weights,
+ *                     so the minimiser is the same, in fewer iterations; 0: IRLS
+ *                     throughout
  *   "level_schedule"  1 (default): factor and solve level by level (kernel
  *                     boundaries order the fronts); 0: one dependency-driven
  *                     launch per sweep with release/acquire flags per front */

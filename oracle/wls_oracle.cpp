@@ -51,7 +51,7 @@
 namespace {
 
 constexpr double kEps = 1e-12;
-constexpr double kNewtonSwitch = 1e-2;  // Huber-Newton: step size below which the gain drops outliers
+constexpr double kNewtonSwitch = 1e-2;  // Huber: step size below which the outliers' gain weights drop
 enum { V_MAG = 0, V_ANGLE = 1, P_INJ = 2, Q_INJ = 3, P_FLOW = 4, Q_FLOW = 5, I_MAG = 6, P_PSEUDO = 7, Q_PSEUDO = 8 };
 
 struct Case {
@@ -853,7 +853,7 @@ void or_solve(void* hh, const OracleSolveParams* prm, OracleSolveResult* out) {
         R.iterations = 0;
         double last_step = 1e300;
         for (int it = 0; it < prm->max_iter; ++it) {
-            // robust == 2: IRLS gain weights until the steps are small, then Newton's
+            // robust == 2: IRLS gain weights until the steps are small, then reduced ones
             const bool newton = prm->robust == 2 && last_step < kNewtonSwitch;
             eval_power(c, c.V.data(), c.th.data(), P.data(), Q.data(), Pf.data(), Qf.data(), Pt.data(), Qt.data());
             eval_h(c, c.V.data(), c.th.data(), P.data(), Q.data(), Pf.data(), Qf.data(), Pt.data(), Qt.data(),
@@ -867,11 +867,16 @@ void or_solve(void* hh, const OracleSolveParams* prm, OracleSolveResult* out) {
                     if (u > prm->gamma) wi = prm->gamma / (s * std::fabs(r[i]));
                 }
                 w[i] = c.active[i] ? wi : 0.0;
-                // robust == 2: Newton steps on the Huber objective -- the gain takes
-                // the curvature psi'(u)/sigma^2 (zero beyond gamma), the right-hand
-                // side stays H^T psi(u)/sigma; same minimiser as the IRLS form
+                // robust == 2: near the solution the gain weights of the outliers
+                // fall from the IRLS weight gamma/(sigma^2 |u|) to
+                // gamma^3/(sigma^2 |u|^3), closer to the curvature of the Huber
+                // loss (zero beyond gamma) yet continuous at gamma; the right-hand
+                // side stays H^T psi(u)/sigma, so the minimiser is the IRLS one
                 wg[i] = w[i];
-                if (newton && std::fabs(r[i]) / s > prm->gamma) wg[i] = 0.0;
+                if (newton) {
+                    const double u = std::fabs(r[i]) / s;
+                    if (u > prm->gamma) wg[i] = w[i] * (prm->gamma / u) * (prm->gamma / u);
+                }
             }
             eval_jacobian(c, c.V.data(), c.th.data(), P.data(), Q.data(), Pf.data(), Qf.data(), Pt.data(),
                           Qt.data(), hv.data());

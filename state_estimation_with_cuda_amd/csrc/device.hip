@@ -56,7 +56,7 @@ constexpr int kFrontA = 32;    // fronts with c <= 32: LDS front of 32x32
 constexpr int kFrontB = 64;    // 32 < c <= 64: LDS front of 64x64
 constexpr int kVecMax = 2048;  // triangular-solve LDS vector bound for large fronts
 constexpr int kSpinLimit = 1 << 22;  // polls before a dependency wait gives up (~0.1 s)
-constexpr double kNewtonSwitch = 1e-2;  // Huber-Newton: step size below which the gain drops outliers
+constexpr double kNewtonSwitch = 1e-2;  // Huber: step size below which the outliers' gain weights drop
 constexpr double kRefreshRatio = 0.25;  // REALTIME: step ratio above which the cached factor is refreshed
 
 // measurement code byte: type (4 bits) | end << 4 | active << 5 | from-first << 6
@@ -210,7 +210,7 @@ __global__ __launch_bounds__(kBlock) void k_flows(int nb, const int32_t* __restr
 struct MeasArgs {
     int m, n;
     int robust, unit_weights, want_jac;
-    int newton;  // Huber with Newton gain weights: outliers leave the gain (not the RHS)
+    int newton;  // Huber near the solution: reduced gain weights of the outliers (not in the RHS)
     const int32_t* pth;  // angle state of each bus (-1 for an angle reference)
     double gamma;
     const uint8_t* mcode;
@@ -336,7 +336,8 @@ __global__ __launch_bounds__(kBlock) void k_meas(MeasArgs A, double* __restrict_
         if (!act) w = 0.0;
         const double unit = act ? 1.0 : 0.0;
         const double sw = A.unit_weights ? unit : sqrt(w);
-        swg = A.unit_weights ? unit : (A.newton && outlier ? 0.0 : sw);
+        // gain weight: IRLS, or near the solution gamma^3/(sigma^2 |u|^3) for outliers
+        swg = A.unit_weights ? unit : (A.newton && outlier ? sw * (A.gamma * sg / fabs(r)) : sw);
         wr = A.unit_weights ? unit * r : w * r;
         A.rs[i] = sw * r;
         if (A.hout) {
@@ -3061,8 +3062,8 @@ SolveOutcome Instance::solve(const SolveParams& p) {
             const bool fp64 = precision_ == 1 || fp64_factor_active_;
             const bool refactor = p.robust || !factor_valid_ || p.mode == 1 || stale;
             stale = false;
-            // Huber-Newton: the IRLS gain until the steps are small, then the
-            // gain without the outliers (same minimiser, faster contraction)
+            // Huber: the IRLS gain until the steps are small, then reduced gain
+            // weights of the outliers (same minimiser, faster contraction)
             const bool newton = p.robust == 2 && !newton_off && last_step < kNewtonSwitch;
             if (fp64) {
                 if (!D.f64.ready) D.f64.alloc(S, bytes_, D.pi.sizes());
@@ -3076,7 +3077,7 @@ SolveOutcome Instance::solve(const SolveParams& p) {
             read_status(D);
             pf.collect(ktimes_);
             if (D.hst->piv && newton) {
-                // the gain without the outliers lost rank: IRLS weights from here on
+                // the gain with reduced outlier weights lost rank: IRLS weights from here on
                 newton_off = true;
                 factor_valid_ = false;
                 out.newton_fallbacks++;

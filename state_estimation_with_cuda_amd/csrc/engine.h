@@ -87,7 +87,7 @@ struct EngineOptions {
     bool lds_solve = true;        // LDS-resident task triangular solves
     int lds_waves_bottom = 16, lds_waves_tail = 16;
     bool pinv = true;             // partitioned-inverse solve of the top of the supernodal tree
-    bool huber_newton = true;     // Huber: Newton gain weights near the solution (IRLS weights before)
+    bool huber_newton = true;     // Huber: reduced outlier gain weights near the solution (IRLS before)
     int pinv_cut = 0;             // fronts per bottom subtree under it (0: default)
     int pinv_waves = 4;           // waves per bottom-subtree workgroup under it
 };

@@ -138,7 +138,10 @@ class Server:
                     k, st = req.payload
                     self.main.set_branch_status(k, st)
                     self._breakers[k] = st
-                r = self.main.solve(mode=0, max_iterations=self.max_iterations, tolerance=self.tol)
+                # a breaker operation moves the optimum: refactor at every step of that
+                # solve (PRECISION); updates reuse the cached factor (REALTIME)
+                mode = 0 if req.kind == UPDATE else 1
+                r = self.main.solve(mode=mode, max_iterations=self.max_iterations, tolerance=self.tol)
                 req.status, req.iterations = r["status"], r["iterations"]
                 if req.record:
                     req.result = self.main.state()
