@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import collections
 import dataclasses
+import gc
 import threading
 import time
 from typing import Any, Callable, Optional
@@ -201,16 +202,27 @@ def poisson_arrivals(rate: float, duration: float, rng: np.random.Generator) -> 
         out.append(t)
 
 
-def run(server: Server, requests: list, sleep: Callable[[float], None] = time.sleep) -> list:
-    """Release each request at its arrival time, then wait for all of them."""
+def run(server: Server, requests: list, sleep: Callable[[float], None] = time.sleep, freeze_gc: bool = True) -> list:
+    """Release each request at its arrival time, then wait for all of them.
+
+    freeze_gc: move every object alive at the start out of the cyclic garbage
+    collector's reach for the run (a full collection over a large model's
+    Python objects stops all threads for a noticeable time)."""
     requests = sorted(requests, key=lambda r: r.arrival)
-    server.start()
-    for req in requests:
-        wait = req.arrival - server.now()
-        if wait > 0:
-            sleep(wait)
-        server.submit(req)
-    server.close()
+    if freeze_gc:
+        gc.collect()
+        gc.freeze()
+    try:
+        server.start()
+        for req in requests:
+            wait = req.arrival - server.now()
+            if wait > 0:
+                sleep(wait)
+            server.submit(req)
+        server.close()
+    finally:
+        if freeze_gc:
+            gc.unfreeze()
     return server.completed
 
 
