@@ -23,6 +23,7 @@ give the HBM traffic per launch recorded in profiles/hbm_traffic_200k.md.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -373,13 +374,17 @@ def run_serve(args):
     t_area = serve.poisson_arrivals(args.area_rate, args.duration, rng) if args.area_rate > 0 else np.zeros(0)
     brks = non_bridge_branches(g, max(1, len(t_upd) // 100 + 1), rng)
     k = max(1, int(round(0.10 * g.m)))
-    z = g.z.copy()
+    z = g.z.copy()  # the telemetry as the resident model sees it after each request
+    zb = g.z.copy()  # the telemetry base for the current breaker status: h(x_true) + the original noise
+    noise = g.z - g.z_true
+    inj = np.isin(g.m_type, [gridgen.P_INJ, gridgen.Q_INJ, gridgen.P_PSEUDO, gridgen.Q_PSEUDO])
+    status_arr = g.status.copy()
     status = {}
     reqs, replay = [], {}
     open_brk = None
     for q, t in enumerate(t_upd):
         idx = np.sort(rng.choice(g.m, k, replace=False)).astype(np.int32)
-        val = g.z[idx] + rng.normal(0.0, 1.0, k) * g.sigma[idx]
+        val = zb[idx] + rng.normal(0.0, 1.0, k) * g.sigma[idx]
         z[idx] = val
         rec = q % args.record_every == args.record_every // 2
         reqs.append(serve.Request(serve.UPDATE, float(t), (idx, val), rid=q, record=rec))
@@ -393,7 +398,16 @@ def run_serve(args):
                 op = (open_brk, 1)
                 open_brk = None
             status[op[0]] = op[1]
-            reqs.append(serve.Request(serve.TOGGLE, float(t) + 1e-6, op, rid=100_000 + q))
+            # the injections at the branch's ends follow the new topology (the network
+            # state stays x_true): the operation carries those telemetry values
+            status_arr[op[0]] = op[1]
+            gt = dataclasses.replace(g, status=status_arr)
+            ends = (g.br_from[op[0]], g.br_to[op[0]])
+            rows = np.nonzero(inj & np.isin(g.m_loc, ends))[0].astype(np.int32)
+            zb[rows] = gridgen.measurement_function(gt, g.V_true, g.th_true, m_type=g.m_type[rows],
+                                                    m_loc=g.m_loc[rows], m_end=g.m_end[rows]) + noise[rows]
+            z[rows] = zb[rows]
+            reqs.append(serve.Request(serve.TOGGLE, float(t) + 1e-6, op + (rows, zb[rows].copy()), rid=100_000 + q))
     for j, t in enumerate(t_area):
         reqs.append(serve.Request(serve.AREA, float(t), pool[j % len(pool)], rid=200_000 + j, record=j < 3))
     srv = serve.Server(main_e, engine, hbm_budget=int(args.hbm_budget_gib * (1 << 30)), area_slots=args.area_slots,
@@ -409,6 +423,12 @@ def run_serve(args):
                              "service_ms": round((r.done - r.start) * 1e3, 2), "iterations": r.iterations,
                              "status": r.status} for r in done
                             if r.kind != serve.AREA and r.done is not None and r.done - r.start > 0.05][:10]
+    worst = max((r for r in done if r.kind != serve.AREA and r.done is not None), key=lambda r: r.latency, default=None)
+    if worst is not None:  # the neighbourhood of the slowest request, for diagnosis
+        near = sorted((r for r in done if r.kind != serve.AREA and abs(r.arrival - worst.arrival) < 0.5),
+                      key=lambda r: r.arrival)
+        summ["around_slowest"] = [[r.kind[0], r.rid, round(r.arrival, 4), round(r.submitted, 4), round(r.start, 4),
+                                   round(r.done, 4)] for r in near]
     summ["slowest"] = [{"kind": r.kind, "rid": r.rid, "arrival_s": round(r.arrival, 4),
                         "wait_ms": round((r.start - r.arrival) * 1e3, 2), "service_ms": round((r.done - r.start) * 1e3, 2),
                         "iterations": r.iterations} for r in slow]
@@ -489,7 +509,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true", help="time the mixed form only")
     ap.add_argument("--skip-mixed", action="store_true", help="time the plain FP64 form only")
-    ap.add_argument("--pcg-steps", type=int, default=2, help="mixed form: CG steps per Newton step")
+    ap.add_argument("--pcg-steps", type=int, default=0, help="mixed form: CG steps per Newton step (engine default 0)")
     ap.add_argument("--copy-calibration", action="store_true", help="run a plain device copy first")
     ap.add_argument("--workload", choices=["update200k", "areas", "serve"], default="update200k",
                     help="update200k: the headline metric; areas: batched uneven areas; serve: serving loop")

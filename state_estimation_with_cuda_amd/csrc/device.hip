@@ -58,6 +58,7 @@ constexpr int kVecMax = 2048;  // triangular-solve LDS vector bound for large fr
 constexpr int kSpinLimit = 1 << 22;  // polls before a dependency wait gives up (~0.1 s)
 constexpr double kNewtonSwitch = 1e-2;  // Huber: step size below which the outliers' gain weights drop
 constexpr double kRefreshRatio = 0.25;  // REALTIME: step ratio above which the cached factor is refreshed
+constexpr double kLinearRegime = 1e-3;  // steps below this: Gauss-Newton contracts fast with a sound factor
 
 // measurement code byte: type (4 bits) | end << 4 | active << 5 | from-first << 6
 // (from-first: the branch's from bus precedes its to bus in the natural state
@@ -964,8 +965,10 @@ struct PiArgs {
 };
 
 // One wave per top supernode s: the k_s unit right-hand sides of its columns are
-// eliminated front by front along the path (extend-add through rmap, as in the
-// forward solve), and each front's solved rows are the block rows of L_TT^-1.
+// carried up the path front by front.  At front p the block F (c_p x k_s) becomes
+// M_p F_top + [0; F_bot] with the front's solve matrix M_p = [L11^-1; -L21 L11^-1]:
+// its first k_p rows are the block rows of L_TT^-1 for p's columns, the rest is
+// extend-added (through rmap) into the parent's block.
 template <typename T>
 __global__ __launch_bounds__(256) void k_pi_inv(PiArgs A, SolveArgs<T> SA, T* __restrict__ B) {
     extern __shared__ unsigned char smem_raw[];
@@ -974,7 +977,7 @@ __global__ __launch_bounds__(256) void k_pi_inv(PiArgs A, SolveArgs<T> SA, T* __
     if (u >= A.ntop) return;  // the whole wave leaves; no workgroup barrier below
     T* F0 = reinterpret_cast<T*>(smem_raw) + (size_t)w * 3 * A.slice;
     T* F1 = F0 + A.slice;
-    T* Lp = F1 + A.slice;
+    T* Mp = F1 + A.slice;
     const int s0 = A.sn[u];
     const int ks = SA.sn_k[s0];
     const int Ps = A.plen[u];
@@ -989,35 +992,24 @@ __global__ __launch_bounds__(256) void k_pi_inv(PiArgs A, SolveArgs<T> SA, T* __
     int ro = 0;
     for (;;) {
         const int k = SA.sn_k[p];
-        const T* Lg = SA.F + SA.foff[p];
-        for (int e = lane; e < c * k; e += kWave) Lp[e] = Lg[e];  // first k columns, col-major
-        wave_sync();
-        for (int j = 0; j < k; ++j) {
-            const T d = Lp[j * c + j];
-            for (int jj = lane; jj < ks; jj += kWave) F0[j * ks + jj] /= d;
-            wave_sync();
-            const int rest = (c - j - 1) * ks;
-            for (int e = lane; e < rest; e += kWave) {
-                const int i = j + 1 + e / ks, jj = e % ks;
-                F0[i * ks + jj] -= Lp[j * c + i] * F0[j * ks + jj];
-            }
-            wave_sync();
-        }
-        for (int e = lane; e < k * ks; e += kWave) {
-            const int j = e / ks, jj = e - j * ks;
-            Bs[(ro + j) + (int64_t)Ps * jj] = F0[e];
-        }
-        ro += k;
         const int q = A.parent[p];
-        if (q < 0) break;
-        const int cq = SA.sn_c[q];
+        const int cq = q >= 0 ? SA.sn_c[q] : 0;
+        const T* Mg = SA.M + SA.moff[p];
+        for (int e = lane; e < c * k; e += kWave) Mp[e] = Mg[e];  // c x k, col-major
         for (int e = lane; e < cq * ks; e += kWave) F1[e] = T(0);
         wave_sync();
         const int32_t* rm = SA.rmap + SA.rmoff[p];
-        for (int e = lane; e < (c - k) * ks; e += kWave) {
-            const int a = e / ks, jj = e - a * ks;
-            F1[rm[a] * ks + jj] = F0[(k + a) * ks + jj];
+        for (int e = lane; e < c * ks; e += kWave) {
+            const int i = e / ks, jj = e - i * ks;
+            T acc = i < k ? T(0) : F0[e];
+            for (int j = 0; j < k; ++j) acc += Mp[j * c + i] * F0[j * ks + jj];
+            if (i < k)
+                Bs[(ro + i) + (int64_t)Ps * jj] = acc;
+            else
+                F1[rm[i - k] * ks + jj] = acc;
         }
+        ro += k;
+        if (q < 0) break;
         wave_sync();
         T* t = F0;
         F0 = F1;
@@ -3111,9 +3103,13 @@ SolveOutcome Instance::solve(const SolveParams& p) {
             // a cached factor (REALTIME) taken at a point the state has since left
             // contracts slowly: refresh it at the current point for the next step
             if (!refactor && it > 0 && D.hst->maxdx > kRefreshRatio * prev && D.hst->maxdx >= p.tol) stale = true;
-            if (!fp64 && it > 0 && (D.hst->nan || D.hst->maxdx > 0.9 * prev)) {
-                // the FP32 factor stopped contracting: continue with an FP64 factor
-                if (++stalls >= 2 || D.hst->nan) {
+            // the FP32 factor is too coarse for this gain: a factor fresh at this
+            // point (or two steps in a row) should contract far better near the
+            // solution, so continue with an FP64 factor
+            const bool coarse = refactor && it > 0 && D.hst->maxdx < kLinearRegime &&
+                                D.hst->maxdx > kRefreshRatio * prev && D.hst->maxdx >= p.tol;
+            if (!fp64 && it > 0 && (D.hst->nan || D.hst->maxdx > 0.9 * prev || coarse)) {
+                if (coarse || ++stalls >= 2 || D.hst->nan) {
                     fp64_factor_active_ = true;
                     factor_valid_ = false;
                     out.fp64_refactors++;

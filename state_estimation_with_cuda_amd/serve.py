@@ -31,9 +31,10 @@ UPDATE, TOGGLE, AREA = "update", "toggle", "area"
 class Request:
     kind: str
     arrival: float  # seconds after the start of the run
-    payload: Any = None  # update: (idx, values); toggle: (branch, status); area: grid
+    payload: Any = None  # update: (idx, values); toggle: (branch, status[, idx, values]); area: grid
     rid: int = 0
     record: bool = False  # keep the result (and, for updates, the starting point) for replay
+    submitted: Optional[float] = None
     start: Optional[float] = None
     done: Optional[float] = None
     status: Optional[int] = None
@@ -98,6 +99,7 @@ class Server:
         return int(self.bytes_per_bus * grid.n)
 
     def submit(self, req: Request):
+        req.submitted = self.now()
         with self._cv:
             if req.kind == AREA:
                 if self.estimate_bytes(req.payload) > self.budget:
@@ -135,8 +137,10 @@ class Server:
                         V, th = self.main.state()
                         req.before = (V, th, dict(self._breakers))
                     self.main.set_telemetry_indexed(idx, val)
-                else:
-                    k, st = req.payload
+                else:  # (branch, status[, telemetry indices, values that change with it])
+                    k, st = req.payload[:2]
+                    if len(req.payload) > 2 and len(req.payload[2]):
+                        self.main.set_telemetry_indexed(req.payload[2], req.payload[3])
                     self.main.set_branch_status(k, st)
                     self._breakers[k] = st
                 # a breaker operation moves the optimum: refactor at every step of that
