@@ -138,7 +138,16 @@ def roofline_for(form, kt, ab):
                 avg_launch_ms=round(avg_ms, 5))
 
 
-def run_form(g, fp64, reqs, batches, warmup, steps, profile_steps, pcg_steps=0):
+def engine_options(args) -> dict:
+    """--set KEY=VALUE pairs (sle_ext_SetOption keys) applied to every engine."""
+    out = {}
+    for kv in args.set or []:
+        k, _, v = kv.partition("=")
+        out[k.strip()] = float(v)
+    return out
+
+
+def run_form(g, fp64, reqs, batches, warmup, steps, profile_steps, pcg_steps=0, options=None, autotune=False):
     from state_estimation_with_cuda_amd import sle
 
     name = "fp64" if fp64 else "mixed"
@@ -147,6 +156,8 @@ def run_form(g, fp64, reqs, batches, warmup, steps, profile_steps, pcg_steps=0):
     e.load_grid(g)
     e.set_config(use_double_precision=int(fp64), enable_profiling=0)
     e.set_option("pcg_steps", 0 if fp64 else pcg_steps)
+    for k, v in (options or {}).items():
+        e.set_option(k, v)
     e.upload()
     e.set_telemetry_f64(g.z)
     setup_s = time.perf_counter() - t0
@@ -157,6 +168,11 @@ def run_form(g, fp64, reqs, batches, warmup, steps, profile_steps, pcg_steps=0):
     r0 = e.solve(mode=1, max_iterations=30, tolerance=TOL, flat_start=True)
     init_ms = (time.perf_counter() - t0) * 1e3
     log(f"{name}: initial flat-start PRECISION solve {init_ms:.1f} ms, iters={r0['iterations']} status={r0['status']}")
+    tune = ""
+    if autotune:
+        t0 = time.perf_counter()
+        tune = e.autotune(3)
+        log(f"{name}: autotune {time.perf_counter() - t0:.1f}s: {tune}")
     for k, (idx, val) in enumerate(batches):
         e.stage_telemetry(k, idx, val)
     lat = []
@@ -214,6 +230,7 @@ def run_form(g, fp64, reqs, batches, warmup, steps, profile_steps, pcg_steps=0):
         sizes=sz,
         flags=sorted(flags),
         plan=e_plan,
+        autotune=tune,
     )
 
 
@@ -238,6 +255,8 @@ def run_areas(args):
         e.load_grid(grid)
         e.set_config(use_double_precision=int(fp64), enable_profiling=0)
         e.set_option("pcg_steps", 0 if fp64 else args.pcg_steps)
+        for k, v in engine_options(args).items():
+            e.set_option(k, v)
         e.upload()
         e.set_telemetry_f64(grid.z)
         return e
@@ -355,6 +374,8 @@ def run_serve(args):
         e.load_grid(grid)
         e.set_config(use_double_precision=int(fp64), enable_profiling=0)
         e.set_option("pcg_steps", 0 if fp64 else args.pcg_steps)
+        for k, v in engine_options(args).items():
+            e.set_option(k, v)
         e.upload()
         e.set_telemetry_f64(grid.z)
         return e
@@ -525,7 +546,8 @@ def main():
     ap.add_argument("--replays", type=int, default=4)
     ap.add_argument("--areas", type=int, default=256)
     ap.add_argument("--form", choices=["fp64", "mixed"], default="fp64", help="areas workload: precision form")
-    ap.add_argument("--autotune", action="store_true", help="areas workload: autotune the solve plan first")
+    ap.add_argument("--autotune", action="store_true", help="autotune the solve and factorization plans first")
+    ap.add_argument("--set", action="append", metavar="KEY=VALUE", help="engine option (sle_ext_SetOption), repeatable")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs (--gpus > 1) are not supported", file=sys.stderr)
@@ -549,9 +571,11 @@ def main():
     log(f"grid: n={g.n} nb={g.nb} m={g.m} generated in {time.perf_counter() - t0:.1f}s")
     reqs, batches = make_updates(g, args.warmup + max(args.steps, args.profile_steps), 0.10, seed=4)
 
+    opts = engine_options(args)
     mixed = None if args.skip_mixed else run_form(g, False, reqs, batches, args.warmup, args.steps,
-                                                  args.profile_steps, args.pcg_steps)
-    fp64 = None if args.skip_fp64 else run_form(g, True, reqs, batches, args.warmup, args.steps, args.profile_steps)
+                                                  args.profile_steps, args.pcg_steps, opts, args.autotune)
+    fp64 = None if args.skip_fp64 else run_form(g, True, reqs, batches, args.warmup, args.steps, args.profile_steps,
+                                                0, opts, args.autotune)
     forms = [f for f in (mixed, fp64) if f]
     best = max(forms, key=lambda f: f["solves_per_s"])
     sz = best["sizes"]
@@ -572,6 +596,7 @@ def main():
             "initial_iterations": f["initial_iterations"],
             "solve_flags": f["flags"],
             "plan": f["plan"],
+            "autotune": f["autotune"],
             "kernel_ms_per_step": {k: round(v["ms"] / max(args.profile_steps, 1), 4) for k, v in kt.items()},
             "kernel_launches_per_step": {k: round(v["launches"] / max(args.profile_steps, 1), 2)
                                          for k, v in kt.items()},
@@ -599,6 +624,7 @@ def main():
             "branches": g.nb,
             "measurements": g.m,
             "tolerance": TOL,
+            "options": opts,
         },
         "p50_ms": round(best["p50_ms"], 4),
         "p99_ms": round(best["p99_ms"], 4),

@@ -50,6 +50,21 @@ typedef struct SLE_ExtResult {
  *                     with every factorization) instead of level sweeps
  *   "pinv_cut"        fronts per bottom subtree under it (0: default, 32)
  *   "pinv_waves"      waves per bottom-subtree workgroup under it (default 4)
+ *   "pinv_dense"      1 (default): each bottom subtree (at most 64 columns, 128
+ *                     rows with its root's update rows) is solved with one
+ *                     dense solve matrix [L_BB^-1; -L_UB L_BB^-1] made after
+ *                     every factorization, one product per subtree and
+ *                     direction; 0: LDS tasks walk the subtree's fronts
+ *   "xcd_remap"       1 (default): the measurement, power, right-hand-side and
+ *                     top-product kernels run contiguous block ranges per XCD
+ *                     (process-wide on the device)
+ *   "meas_by_type"    1 (default): the device keeps the measurement rows grouped
+ *                     by evaluation class (bus values, injections, flows), each
+ *                     class by anchor bus; 0: by anchor bus only
+ *   "lds_flags"       0 (default): the fronts of a subtree task are solved
+ *                     level by level with a workgroup barrier per level;
+ *                     1: each wave waits on LDS flags of just the fronts it
+ *                     depends on
  *   "huber_newton"    1 (default): Huber estimation takes IRLS gain weights
  *                     (the reference's, kernels.cu:634-671) until the steps fall
  *                     below 1e-2, then gain weights gamma^3/(sigma^2 |u|^3) for the
@@ -64,9 +79,11 @@ SLE_API SLE_StatusCode SLE_CALL sle_ext_SetOption(SLE_Handle handle, const char*
 
 /* Autotuner of the triangular-solve plan: on the current factor (solve once
  * first), time `repetitions` forward + backward solves for each candidate
- * bottom-subtree size and workgroup width of the partitioned-inverse solve and
- * keep the fastest (the measurements appear in sle_ext_Describe); the choice
- * becomes the "pinv_cut" / "pinv_waves" options. */
+ * bottom-subtree size, workgroup width and in-task synchronisation of the
+ * partitioned-inverse solve and keep the fastest, then time the factorization
+ * plans (subtree and top-chunk sizes) the same way (the measurements appear in
+ * sle_ext_Describe); the choice becomes the "pinv_cut" / "pinv_waves" /
+ * "lds_flags" options. */
 SLE_API SLE_StatusCode SLE_CALL sle_ext_AutotuneSolve(SLE_Handle handle, int32_t repetitions);
 
 /* Solve with an explicit solver configuration; FP64 outputs. */

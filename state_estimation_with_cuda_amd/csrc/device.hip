@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -65,6 +66,24 @@ constexpr double kLinearRegime = 1e-3;  // steps below this: Gauss-Newton contra
 // order, which fixes the column order of a flow row)
 __host__ __device__ inline uint8_t meas_code(int type, int end, int active, int from_first) {
     return (uint8_t)((type & 15) | ((end & 1) << 4) | ((active & 1) << 5) | ((from_first & 1) << 6));
+}
+
+// Workgroups of a launch are dealt to the 8 XCDs round-robin and every XCD
+// has its own L2.  A kernel may take its logical block from xcd_block(): XCD x
+// then runs the contiguous logical range [x*q + min(x, r), +q + (x < r)) of
+// the nb = 8q + r blocks, so blocks that touch neighbouring buses share one L2.
+// Measured on the 200k-bus grid it pays for the power and right-hand-side sums
+// only: the measurement pass and the products of the solves run faster in
+// plain block order (their scattered writes and the 256 MB last-level cache
+// shared by all XCDs favour neighbouring blocks running at the same time).
+constexpr int kXcds = 8;
+__constant__ int c_xcd_remap = 1;
+
+__device__ inline int xcd_block() {
+    const int b = blockIdx.x;
+    if (!c_xcd_remap) return b;
+    const int nb = gridDim.x, x = b % kXcds, l = b / kXcds, q = nb / kXcds, r = nb % kXcds;
+    return x * q + (x < r ? x : r) + l;
 }
 
 struct DevStatus {
@@ -151,7 +170,7 @@ __global__ __launch_bounds__(kBlock) void k_power(int n, const int32_t* __restri
                                                   const double* __restrict__ th, double* __restrict__ P,
                                                   double* __restrict__ Q, double* __restrict__ Pij,
                                                   double* __restrict__ Qij, DevStatus* st) {
-    int i = blockIdx.x * kBlock + threadIdx.x;
+    int i = xcd_block() * kBlock + threadIdx.x;
     if (i == 0) {
         st->nan = 0;
         st->piv = 0;
@@ -247,7 +266,8 @@ struct MeasArgs {
 };
 
 __global__ __launch_bounds__(kBlock) void k_meas(MeasArgs A, double* __restrict__ hs) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int blk = blockIdx.x;
+    const int i = blk * kBlock + threadIdx.x;
     double swg = 0.0, wr = 0.0;
     // one Jacobian entry d: sqrt(w_gain) d into the row-major Jacobian (for the
     // gain) and/or d w r into the H^T-ordered right-hand-side products
@@ -425,9 +445,9 @@ __global__ __launch_bounds__(kBlock) void k_meas(MeasArgs A, double* __restrict_
                 I = si[k];
             }
         }
-        A.jpart[blockIdx.x] = J;
-        A.rpart[blockIdx.x] = R;
-        A.ipart[blockIdx.x] = I;
+        A.jpart[blk] = J;
+        A.rpart[blk] = R;
+        A.ipart[blk] = I;
     }
 }
 
@@ -764,8 +784,9 @@ __global__ __launch_bounds__(1024) void k_sweep_tasks(FactArgs<T> FA, SolveArgs<
 // ---------------------------------------------------------------------------
 constexpr int kTH = 12;  // task header ints
 enum { TH_NF = 0, TH_NLEV, TH_META, TH_LVL, TH_CH, TH_NCH, TH_RM, TH_NRM, TH_SRC, TH_NSRC, TH_SUML, TH_SUMF };
-constexpr int kFM = 11;  // per-front ints
-enum { FM_S = 0, FM_C, FM_K, FM_FIRST, FM_L, FM_F, FM_NCH, FM_CH, FM_RM, FM_EXTPAR, FM_SRC };
+constexpr int kFM = 12;  // per-front ints
+enum { FM_S = 0, FM_C, FM_K, FM_FIRST, FM_L, FM_F, FM_NCH, FM_CH, FM_RM, FM_EXTPAR, FM_SRC, FM_PAR };
+constexpr int kLdsSpinLimit = 1 << 22;  // polls of an in-workgroup dependency before it is reported
 
 struct LdsTaskArgs {
     const int32_t* th;
@@ -774,7 +795,33 @@ struct LdsTaskArgs {
     const int32_t* ch;
     const int32_t* rm;
     const int32_t* src;
+    DevStatus* st;
 };
+
+// In-workgroup dependencies between the fronts of one task (FLAGS variants):
+// a wave walks its fronts in dependency order and waits on LDS flags only for
+// the fronts it needs, instead of a workgroup barrier per level.  Fronts are
+// dealt to the waves round-robin in an order that lists every front after the
+// ones it waits for, so the lowest waiting front can always proceed; the spin
+// limit only guards against a plan that breaks that order (the sweep is then
+// redone level by level).
+__device__ inline void lds_wait(const int* flag, DevStatus* st) {
+    int it = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0) {
+        if (++it > kLdsSpinLimit) {
+            st->flow_err = 1;
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+__device__ inline void lds_publish(int* flag, int lane) {
+    wave_sync();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if (lane == 0) __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 
 template <typename T>
 struct LdsView {
@@ -785,6 +832,7 @@ struct LdsView {
     int* chl;
     int* rml;
     int* srl;
+    int* flg;  // per-front completion flags (FLAGS variants)
     int nf, nlev;
 };
 
@@ -800,7 +848,9 @@ __device__ inline LdsView<T> lds_task_load(const LdsTaskArgs& A, const int32_t* 
     v.chl = v.lv + v.nlev + 1;
     v.rml = v.chl + H[TH_NCH];
     v.srl = v.rml + H[TH_NRM];
+    v.flg = v.srl + H[TH_NSRC];
     const int nt = blockDim.x;
+    for (int i = threadIdx.x; i < v.nf; i += nt) v.flg[i] = 0;
     for (int i = threadIdx.x; i < v.nf * kFM; i += nt) v.mt[i] = A.meta[H[TH_META] + i];
     for (int i = threadIdx.x; i <= v.nlev; i += nt) v.lv[i] = A.lvl[H[TH_LVL] + i];
     for (int i = threadIdx.x; i < H[TH_NCH]; i += nt) v.chl[i] = A.ch[H[TH_CH] + i];
@@ -810,7 +860,34 @@ __device__ inline LdsView<T> lds_task_load(const LdsTaskArgs& A, const int32_t* 
     return v;
 }
 
-template <typename T>
+template <typename T, bool FLAGS>
+__device__ inline void fwd_lds_front(const LdsView<T>& v, int lf, int lane) {
+    const int* m = v.mt + lf * kFM;
+    const int c = m[FM_C], k = m[FM_K];
+    T* f = v.fv + m[FM_F];
+    const T* Lp = v.Lp + m[FM_L];
+    for (int q = m[FM_CH]; q < m[FM_CH] + m[FM_NCH]; ++q) {
+        const int lc = v.chl[q];
+        if (lc < 0) continue;
+        const int* mc = v.mt + lc * kFM;
+        const int u = mc[FM_C] - mc[FM_K];
+        const T* uc = v.fv + mc[FM_F] + mc[FM_K];
+        const int* rmc = v.rml + mc[FM_RM];
+        for (int a = lane; a < u; a += kWave) f[rmc[a]] += uc[a];
+        wave_sync();
+    }
+    // f <- M f_top + [0; f_bot]  (c <= 64: one row per lane)
+    T out = T(0);
+    if (lane < c) {
+        out = lane < k ? T(0) : f[lane];
+        for (int j = 0; j < k; ++j) out += Lp[j * c + lane] * f[j];
+    }
+    wave_sync();
+    if (lane < c) f[lane] = out;
+    wave_sync();
+}
+
+template <typename T, bool FLAGS>
 __global__ __launch_bounds__(1024) void k_fwd_lds(LdsTaskArgs A, SolveArgs<T> SA, int task_base) {
     extern __shared__ unsigned char smem_raw[];
     const int32_t* H = A.th + (size_t)(task_base + blockIdx.x) * kTH;
@@ -841,33 +918,20 @@ __global__ __launch_bounds__(1024) void k_fwd_lds(LdsTaskArgs A, SolveArgs<T> SA
         }
     }
     __syncthreads();
-    for (int L = 0; L < v.nlev; ++L) {
-        for (int lf = v.lv[L] + w; lf < v.lv[L + 1]; lf += nw) {
+    if (FLAGS) {
+        for (int lf = w; lf < v.nf; lf += nw) {  // fronts are listed children first
             const int* m = v.mt + lf * kFM;
-            const int c = m[FM_C], k = m[FM_K];
-            T* f = v.fv + m[FM_F];
-            const T* Lp = v.Lp + m[FM_L];
-            for (int q = m[FM_CH]; q < m[FM_CH] + m[FM_NCH]; ++q) {
-                const int lc = v.chl[q];
-                if (lc < 0) continue;
-                const int* mc = v.mt + lc * kFM;
-                const int u = mc[FM_C] - mc[FM_K];
-                const T* uc = v.fv + mc[FM_F] + mc[FM_K];
-                const int* rmc = v.rml + mc[FM_RM];
-                for (int a = lane; a < u; a += kWave) f[rmc[a]] += uc[a];
-                wave_sync();
-            }
-            // f <- M f_top + [0; f_bot]  (c <= 64: one row per lane)
-            T out = T(0);
-            if (lane < c) {
-                out = lane < k ? T(0) : f[lane];
-                for (int j = 0; j < k; ++j) out += Lp[j * c + lane] * f[j];
-            }
-            wave_sync();
-            if (lane < c) f[lane] = out;
-            wave_sync();
+            for (int q = m[FM_CH]; q < m[FM_CH] + m[FM_NCH]; ++q)
+                if (v.chl[q] >= 0) lds_wait(v.flg + v.chl[q], A.st);
+            fwd_lds_front<T, FLAGS>(v, lf, lane);
+            lds_publish(v.flg + lf, lane);
         }
         __syncthreads();
+    } else {
+        for (int L = 0; L < v.nlev; ++L) {
+            for (int lf = v.lv[L] + w; lf < v.lv[L + 1]; lf += nw) fwd_lds_front<T, FLAGS>(v, lf, lane);
+            __syncthreads();
+        }
     }
     for (int lf = w; lf < v.nf; lf += nw) {
         const int* m = v.mt + lf * kFM;
@@ -882,6 +946,27 @@ __global__ __launch_bounds__(1024) void k_fwd_lds(LdsTaskArgs A, SolveArgs<T> SA
 }
 
 template <typename T>
+__device__ inline void bwd_lds_front(const LdsView<T>& v, int lf, int lane) {
+    const int* m = v.mt + lf * kFM;
+    const int c = m[FM_C], k = m[FM_K];
+    T* xv = v.fv + m[FM_F];
+    const T* Lp = v.Lp + m[FM_L];
+    const int* sr = v.srl + m[FM_SRC];
+    for (int i = k + lane; i < c; i += kWave) {
+        const int code = sr[i - k];
+        if (code >= 0) xv[i] = v.fv[code];  // ancestor inside the task, solved before
+    }
+    wave_sync();
+    // x_s = M^T [y_s; x_rows]  (lane j < k: one column of M)
+    double out = 0.0;
+    if (lane < k)
+        for (int i = lane; i < c; ++i) out += (double)Lp[lane * c + i] * (double)xv[i];
+    wave_sync();
+    if (lane < k) xv[lane] = (T)out;
+    wave_sync();
+}
+
+template <typename T, bool FLAGS>
 __global__ __launch_bounds__(1024) void k_bwd_lds(LdsTaskArgs A, SolveArgs<T> SA, int task_base) {
     extern __shared__ unsigned char smem_raw[];
     const int32_t* H = A.th + (size_t)(task_base + blockIdx.x) * kTH;
@@ -905,27 +990,19 @@ __global__ __launch_bounds__(1024) void k_bwd_lds(LdsTaskArgs A, SolveArgs<T> SA
         }
     }
     __syncthreads();
-    for (int L = v.nlev - 1; L >= 0; --L) {
-        for (int lf = v.lv[L] + w; lf < v.lv[L + 1]; lf += nw) {
-            const int* m = v.mt + lf * kFM;
-            const int c = m[FM_C], k = m[FM_K];
-            T* xv = v.fv + m[FM_F];
-            const T* Lp = v.Lp + m[FM_L];
-            const int* sr = v.srl + m[FM_SRC];
-            for (int i = k + lane; i < c; i += kWave) {
-                const int code = sr[i - k];
-                if (code >= 0) xv[i] = v.fv[code];  // ancestor inside the task, solved at a higher level
-            }
-            wave_sync();
-            // x_s = M^T [y_s; x_rows]  (lane j < k: one column of M)
-            double out = 0.0;
-            if (lane < k)
-                for (int i = lane; i < c; ++i) out += (double)Lp[lane * c + i] * (double)xv[i];
-            wave_sync();
-            if (lane < k) xv[lane] = (T)out;
-            wave_sync();
+    if (FLAGS) {
+        for (int lf = v.nf - 1 - w; lf >= 0; lf -= nw) {  // parents before children
+            const int par = v.mt[lf * kFM + FM_PAR];
+            if (par >= 0) lds_wait(v.flg + par, A.st);
+            bwd_lds_front<T>(v, lf, lane);
+            lds_publish(v.flg + lf, lane);
         }
         __syncthreads();
+    } else {
+        for (int L = v.nlev - 1; L >= 0; --L) {
+            for (int lf = v.lv[L] + w; lf < v.lv[L + 1]; lf += nw) bwd_lds_front<T>(v, lf, lane);
+            __syncthreads();
+        }
     }
     for (int lf = w; lf < v.nf; lf += nw) {
         const int* m = v.mt + lf * kFM;
@@ -1040,6 +1117,8 @@ __global__ __launch_bounds__(kBlock) void k_pi_assemble(int ntc, const int32_t* 
     ctop[t] = acc;
 }
 
+constexpr int kPiSegMax = 256;  // entries per segment of the forward product (a multiple of 64)
+
 // Forward product y_T = L_TT^-1 c_T: one wave per segment of a row's entries.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pi_fwd(int nseg, const int64_t* __restrict__ segq,
@@ -1048,8 +1127,20 @@ __global__ __launch_bounds__(kBlock) void k_pi_fwd(int nseg, const int64_t* __re
     const int g = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
     const int lane = threadIdx.x & (kWave - 1);
     if (g >= nseg) return;
+    // a segment holds at most kPiSeg entries: all loads of a lane issued at once
+    constexpr int kU = kPiSegMax / kWave;
+    const int64_t q0 = segq[g] + lane, q1 = segq[g + 1];
+    double v[kU];
+    int c[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+        const int64_t q = q0 + (int64_t)u * kWave;
+        v[u] = q < q1 ? (double)val[q] : 0.0;
+        c[u] = q < q1 ? src[q] : -1;
+    }
     double acc = 0.0;
-    for (int64_t q = segq[g] + lane; q < segq[g + 1]; q += kWave) acc += (double)val[q] * (double)ctop[src[q]];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) acc += c[u] >= 0 ? v[u] * (double)ctop[c[u]] : 0.0;
     acc = wave_sum(acc);
     if (lane == 0) part[g] = acc;
 }
@@ -1080,9 +1171,173 @@ __global__ __launch_bounds__(kBlock) void k_pi_bwd(PiArgs A, SolveArgs<T> SA, co
     const T* Bc = B + A.boff[u] + (int64_t)Ps * jj;
     const int32_t* pl = A.ploc + A.poff[u];
     double acc = 0.0;
-    for (int r = jj + lane; r < Ps; r += kWave) acc += (double)Bc[r] * (double)ytop[pl[r]];
+    constexpr int kU = 4;  // loads of a lane in flight together
+    for (int r0 = jj + lane; r0 < Ps; r0 += kU * kWave) {
+        double bv[kU];
+        int c[kU];
+#pragma unroll
+        for (int k = 0; k < kU; ++k) {
+            const int r = r0 + k * kWave;
+            bv[k] = r < Ps ? (double)Bc[r] : 0.0;
+            c[k] = r < Ps ? pl[r] : -1;
+        }
+#pragma unroll
+        for (int k = 0; k < kU; ++k) acc += c[k] >= 0 ? bv[k] * (double)ytop[c[k]] : 0.0;
+    }
     acc = wave_sum(acc);
     if (lane == 0) SA.x[col] = (T)acc;
+}
+
+// ---------------------------------------------------------------------------
+// Dense solve matrices of the bottom subtrees.  A bottom subtree t with n
+// columns (its fronts' columns, children first) and u update rows (the rows of
+// its root beyond the root's columns; all of them lie in the top) has
+//   M_t = [L_BB^-1 ; -L_UB L_BB^-1]      (R = n + u rows, n columns)
+// so that its forward step is one product [y_B ; u_t] = M_t b_B (y_B the
+// solution of its columns, u_t the update vector its root hands to the top)
+// and its backward step is x_B = M_t^T [y_B ; x_U].  The dependent chain of
+// front steps inside the subtree becomes one independent product per subtree.
+// M_t is stored column by column from the diagonal down (lower trapezoid:
+// column j holds rows j..R-1), n <= 64, R <= 128.
+// ---------------------------------------------------------------------------
+constexpr int kSubCols = 64, kSubRows = 128;
+
+struct SubArgs {
+    int nsub;
+    int64_t msub;         // entries of all M_t (the row-major copies follow)
+    const int32_t* root;  // root supernode
+    const int32_t* n;     // columns
+    const int32_t* R;     // columns + update rows
+    const int32_t* fptr;  // fronts of subtree t: fsn[fptr[t] .. fptr[t+1])
+    const int32_t* fsn;
+    const int32_t* fdcol;  // dense column of each front's first column
+    const int64_t* fdoff;  // offset of each front's dense-row list in drow
+    const int32_t* drow;   // dense row of every row of every front
+    const int32_t* cptr;   // columns of subtree t: cols[cptr[t] .. cptr[t] + n[t])
+    const int32_t* cols;
+    const int64_t* moff;   // start of M_t
+};
+
+__host__ __device__ inline int64_t sub_col_off(int j, int R) { return (int64_t)j * R - (int64_t)j * (j - 1) / 2; }
+
+// Row-major copy of M_t (row i holds columns 0..min(i, n-1)) for the backward
+// product; it sits msub entries after the column-major one.
+__host__ __device__ inline int64_t sub_row_off(int i, int n) {
+    return i <= n ? (int64_t)i * (i + 1) / 2 : (int64_t)n * (n + 1) / 2 + (int64_t)(i - n) * n;
+}
+
+// After each factorization: M_t by column-oriented forward substitution of
+// [L_BB 0; L_UB I] m = e_j, with the subtree's L gathered densely into LDS.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sub_m(SubArgs B, SolveArgs<T> A, const int32_t* __restrict__ list,
+                                                 T* __restrict__ Mout) {
+    extern __shared__ double Ld[];  // R x n, column-major
+    const int t = list[blockIdx.x];
+    const int n = B.n[t], R = B.R[t];
+    const int nw = kBlock / kWave, w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    for (int e = threadIdx.x; e < R * n; e += kBlock) Ld[e] = 0.0;
+    __syncthreads();
+    for (int f = B.fptr[t] + w; f < B.fptr[t + 1]; f += nw) {
+        const int s = B.fsn[f], c = A.sn_c[s], k = A.sn_k[s], dc = B.fdcol[f];
+        const T* L = A.F + A.foff[s];
+        const int32_t* dr = B.drow + B.fdoff[f];
+        for (int i = lane; i < c; i += kWave) {
+            const int r = dr[i];
+            const int jmax = i < k ? i + 1 : k;
+            for (int jj = 0; jj < jmax; ++jj) Ld[(dc + jj) * R + r] = (double)L[jj * c + i];
+        }
+    }
+    __syncthreads();
+    T* Mt = Mout + B.moff[t];
+    for (int j = w; j < n; j += nw) {
+        double m0 = lane == j ? 1.0 : 0.0, m1 = 0.0;  // rows lane and lane + 64
+        for (int k = j; k < n; ++k) {
+            const double mk = __shfl(m0, k, kWave) / Ld[k * R + k];
+            if (lane == k)
+                m0 = mk;
+            else if (lane > k && lane < R)
+                m0 -= Ld[k * R + lane] * mk;
+            if (lane + kWave < R) m1 -= Ld[k * R + lane + kWave] * mk;
+        }
+        T* col = Mt + sub_col_off(j, R);
+        if (lane >= j && lane < R) col[lane - j] = (T)m0;
+        if (lane + kWave < R) col[lane + kWave - j] = (T)m1;
+        T* rowm = Mout + B.msub + B.moff[t];  // row-major copy
+        if (lane >= j && lane < R) rowm[sub_row_off(lane, n) + j] = (T)m0;
+        if (lane + kWave < R) rowm[sub_row_off(lane + kWave, n) + j] = (T)m1;
+    }
+}
+
+constexpr int kSubBatch = 8;  // independent loads in flight per lane
+
+// Forward: one wave per subtree; lane i accumulates rows i and i + 64 over the
+// columns of M_t, kSubBatch coalesced column loads at a time.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sub_fwd(SubArgs B, SolveArgs<T> A, const T* __restrict__ M) {
+    __shared__ double bsh[kBlock / kWave][kSubCols];
+    const int w = threadIdx.x / kWave;
+    const int t = blockIdx.x * (kBlock / kWave) + w;
+    const int lane = threadIdx.x & (kWave - 1);
+    if (t >= B.nsub) return;  // uniform per wave
+    const int n = B.n[t], R = B.R[t];
+    const int32_t* cols = B.cols + B.cptr[t];
+    double* bs = bsh[w];
+    if (lane < n) bs[lane] = (double)A.x[cols[lane]];
+    wave_sync();
+    const T* Mt = M + B.moff[t];
+    double a0 = 0.0, a1 = 0.0;
+    for (int j0 = 0; j0 < n; j0 += kSubBatch) {
+        double m0[kSubBatch], m1[kSubBatch];
+#pragma unroll
+        for (int u = 0; u < kSubBatch; ++u) {
+            const int j = j0 + u;
+            const T* col = Mt + sub_col_off(j, R) - j;  // entry (i, j) at col[i]
+            m0[u] = (j < n && lane >= j && lane < R) ? (double)col[lane] : 0.0;
+            m1[u] = (j < n && lane + kWave < R) ? (double)col[lane + kWave] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kSubBatch; ++u) {
+            const double b = j0 + u < n ? bs[j0 + u] : 0.0;
+            a0 += m0[u] * b;
+            a1 += m1[u] * b;
+        }
+    }
+    T* u = A.ubuf + A.uoff[B.root[t]];
+    if (lane < n)
+        A.x[cols[lane]] = (T)a0;
+    else if (lane < R)
+        u[lane - n] = (T)a0;
+    if (lane + kWave < R) u[lane + kWave - n] = (T)a1;
+}
+
+// Backward: one wave per subtree; lane j accumulates column j over the rows of
+// the row-major copy, kSubBatch coalesced row loads at a time.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sub_bwd(SubArgs B, SolveArgs<T> A, const T* __restrict__ Mrow) {
+    __shared__ double vsh[kBlock / kWave][kSubRows];
+    const int w = threadIdx.x / kWave;
+    const int t = blockIdx.x * (kBlock / kWave) + w;
+    const int lane = threadIdx.x & (kWave - 1);
+    if (t >= B.nsub) return;  // uniform per wave
+    const int n = B.n[t], R = B.R[t], root = B.root[t];
+    const int32_t* cols = B.cols + B.cptr[t];
+    const int32_t* urow = A.rows + A.rows_off[root] + A.sn_k[root];
+    double* v = vsh[w];
+    for (int i = lane; i < R; i += kWave) v[i] = (double)(i < n ? A.x[cols[i]] : A.x[urow[i - n]]);
+    wave_sync();
+    const T* Mt = Mrow + B.moff[t];
+    double acc = 0.0;
+    for (int i0 = 0; i0 < R; i0 += kSubBatch) {
+        double mv[kSubBatch];
+#pragma unroll
+        for (int u = 0; u < kSubBatch; ++u) {
+            const int i = i0 + u;
+            mv[u] = (i < R && lane <= i && lane < n) ? (double)Mt[sub_row_off(i, n) + lane] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kSubBatch; ++u) acc += mv[u] * (i0 + u < R ? v[i0 + u] : 0.0);
+    }
+    if (lane < n) A.x[cols[lane]] = (T)acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -1184,7 +1439,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_sum(int ns, const int32_t* __res
                                                     const int32_t* __restrict__ tout, const double* __restrict__ hr,
                                                     TO* __restrict__ x) {
     // kRhsLanes consecutive lanes read one state's segment together
-    const int t = (blockIdx.x * kBlock + threadIdx.x) / kRhsLanes;
+    const int t = (xcd_block() * kBlock + threadIdx.x) / kRhsLanes;
     const int l = threadIdx.x & (kRhsLanes - 1);
     double acc = 0.0;
     if (t < ns)
@@ -1302,7 +1557,8 @@ __global__ __launch_bounds__(kBlock) void k_update(int n, const int32_t* __restr
                                                    double* __restrict__ V, double* __restrict__ th, double vmin,
                                                    double vmax, double* __restrict__ mpart, DevStatus* st,
                                                    int skip_on_pivot) {
-    const int b = blockIdx.x * kBlock + threadIdx.x;
+    const int blk = blockIdx.x;
+    const int b = blk * kBlock + threadIdx.x;
     const bool skip = (skip_on_pivot && st->piv != 0) || st->flow_err != 0;
     double mx = 0.0;
     if (b < n && !skip) {
@@ -1324,7 +1580,7 @@ __global__ __launch_bounds__(kBlock) void k_update(int n, const int32_t* __restr
     if (threadIdx.x == 0) {
         double r = 0.0;
         for (int k = 0; k < kBlock / kWave; ++k) r = fmax(r, sm[k]);
-        mpart[blockIdx.x] = r;
+        mpart[blk] = r;
     }
 }
 
@@ -1384,6 +1640,11 @@ inline int nblk(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
 // ---------------------------------------------------------------------------
 // device buffers
 // ---------------------------------------------------------------------------
+// Count of device allocations and frees in this process.  A captured launch
+// graph holds buffer addresses: one captured before the latest (re)allocation
+// anywhere is retired rather than trusted.
+static std::atomic<uint64_t> g_alloc_epoch{0};
+
 template <typename T>
 struct DBuf {
     T* p = nullptr;
@@ -1394,12 +1655,14 @@ struct DBuf {
         if (cnt) {
             HIPCHECK(hipMalloc(&p, cnt * sizeof(T)));
             bytes += cnt * sizeof(T);
+            g_alloc_epoch.fetch_add(1, std::memory_order_relaxed);
         }
     }
     void release(size_t& bytes) {
         if (p) {
             (void)hipFree(p);
             bytes -= n * sizeof(T);
+            g_alloc_epoch.fetch_add(1, std::memory_order_relaxed);
         }
         p = nullptr;
         n = 0;
@@ -1415,7 +1678,7 @@ struct DBuf {
 
 // Sizes of the partitioned-inverse buffers (zero when that solve is off).
 struct PiSizes {
-    int64_t nblock = 0, nq = 0;
+    int64_t nblock = 0, nq = 0, msub = 0;
     int ntc = 0;
 };
 
@@ -1423,6 +1686,7 @@ template <typename T>
 struct PrecBufs {
     DBuf<T> Gval, F, x, ubuf, M;
     DBuf<T> pinv, pinv_rows, ctop, ytop;  // L_TT^-1 blocks, row-grouped copy, top vectors
+    DBuf<T> msub;                         // dense solve matrices of the bottom subtrees
     bool ready = false;
     void alloc(const Symbolic& S, size_t& bytes, const PiSizes& pz) {
         Gval.alloc(std::max<int64_t>(1, S.nnz_g), bytes);
@@ -1430,10 +1694,7 @@ struct PrecBufs {
         x.alloc(std::max(1, S.ns), bytes);
         ubuf.alloc(std::max<int64_t>(1, S.utotal), bytes);
         M.alloc(std::max<int64_t>(1, S.mtotal), bytes);
-        pinv.alloc(std::max<int64_t>(1, pz.nblock), bytes);
-        pinv_rows.alloc(std::max<int64_t>(1, pz.nq), bytes);
-        ctop.alloc(std::max(1, pz.ntc), bytes);
-        ytop.alloc(std::max(1, pz.ntc), bytes);
+        alloc_pi(pz, bytes);
         ready = true;
     }
     void alloc_pi(const PiSizes& pz, size_t& bytes) {
@@ -1441,9 +1702,10 @@ struct PrecBufs {
         pinv_rows.alloc(std::max<int64_t>(1, pz.nq), bytes);
         ctop.alloc(std::max(1, pz.ntc), bytes);
         ytop.alloc(std::max(1, pz.ntc), bytes);
+        msub.alloc(std::max<int64_t>(1, 2 * pz.msub), bytes);  // column-major, then row-major
     }
     void release(size_t& bytes) {
-        for (DBuf<T>* b : {&Gval, &F, &x, &ubuf, &M, &pinv, &pinv_rows, &ctop, &ytop}) b->release(bytes);
+        for (DBuf<T>* b : {&Gval, &F, &x, &ubuf, &M, &pinv, &pinv_rows, &ctop, &ytop, &msub}) b->release(bytes);
         ready = false;
     }
 };
@@ -1580,7 +1842,7 @@ static size_t lds_task_bytes(const Symbolic& S, const std::vector<std::vector<in
             nsrc += S.sn_c[s] - S.sn_k[s];
         }
     if (nfronts) *nfronts = nf;
-    return (size_t)(sumL + sumF) * elem + 4 * (size_t)(nf * kFM + levels.size() + 1 + nch + nrm + nsrc);
+    return (size_t)(sumL + sumF) * elem + 4 * (size_t)(nf * kFM + levels.size() + 1 + nch + nrm + nsrc + nf);
 }
 
 static void add_lds_task(LdsPlan& P, const Symbolic& S, const std::vector<int32_t>& snode_of_col,
@@ -1619,6 +1881,9 @@ static void add_lds_task(LdsPlan& P, const Symbolic& S, const std::vector<int32_
             P.ch.push_back(local[chd] >= 0 ? local[chd] : -chd - 1);
         }
         const int par = S.sn_parent[s];
+        m[FM_PAR] = (par >= 0 && local[par] >= 0) ? local[par] : -1;
+        // the flag-synchronised sweeps need every parent listed after its children
+        if (m[FM_PAR] >= 0 && m[FM_PAR] <= (int32_t)lf) throw std::logic_error("task fronts out of level order");
         if (par >= 0 && local[par] >= 0) {
             m[FM_RM] = (int32_t)P.rm.size() - rm_off;
             m[FM_EXTPAR] = 0;
@@ -1701,22 +1966,110 @@ struct PiPlan {
     int64_t nblock = 0, nq = 0;
     std::vector<int32_t> sn, plen, ploc, tcol, colsn, src, segptr, cptr, parent;
     std::vector<int64_t> boff, poff, pos, segq, cpos;
+    // dense bottom subtrees (see SubArgs)
+    bool dense = false;
+    int nsub = 0;
+    int64_t msub = 0;
+    std::vector<int32_t> sroot, sn_n, sR, fptr, fsn, fdcol, drow, scptr, scols, bins;
+    std::vector<int64_t> fdoff, smoff;
+    int bin_off[4] = {0, 0, 0, 0};
+    size_t bin_bytes[3] = {0, 0, 0};
 };
 
-constexpr int kPiSeg = 256;  // entries per forward-product segment
+// Bottom subtrees as dense solve matrices: for every subtree root (a front
+// outside the top whose parent is in the top, or a root of the tree) list its
+// fronts children first, its columns in that order, and the dense row of every
+// front row (a subtree column, or a root update row placed after the columns).
+static void make_sub_plan(const Symbolic& S, const std::vector<int32_t>& tsl, PiPlan& P) {
+    const int nsn = S.nsn;
+    std::vector<int32_t> dpos(S.ns, -1), stack, fronts;
+    P.fptr.push_back(0);
+    P.scptr.push_back(0);
+    for (int r = 0; r < nsn; ++r) {
+        const int par = S.sn_parent[r];
+        if (tsl[r] >= 0 || (par >= 0 && tsl[par] < 0)) continue;
+        fronts.clear();
+        stack.assign(1, r);
+        while (!stack.empty()) {
+            const int s = stack.back();
+            stack.pop_back();
+            fronts.push_back(s);
+            for (int q = S.sn_child_ptr[s]; q < S.sn_child_ptr[s + 1]; ++q) stack.push_back(S.sn_child[q]);
+        }
+        std::sort(fronts.begin(), fronts.end());  // children carry smaller ids than their parents
+        int n = 0;
+        for (int s : fronts) {
+            P.fsn.push_back(s);
+            P.fdcol.push_back(n);
+            for (int j = 0; j < S.sn_k[s]; ++j) {
+                dpos[S.sn_first[s] + j] = n + j;
+                P.scols.push_back(S.sn_first[s] + j);
+            }
+            n += S.sn_k[s];
+        }
+        const int kr = S.sn_k[r], u = S.sn_c[r] - kr, R = n + u;
+        if (n > kSubCols || R > kSubRows) throw std::logic_error("dense subtree larger than its limits");
+        for (int a = 0; a < u; ++a) dpos[S.sn_rows[S.sn_rows_off[r] + kr + a]] = n + a;
+        for (int s : fronts) {
+            P.fdoff.push_back((int64_t)P.drow.size());
+            for (int i = 0; i < S.sn_c[s]; ++i) {
+                const int d = dpos[S.sn_rows[S.sn_rows_off[s] + i]];
+                if (d < 0) throw std::logic_error("dense subtree: front row outside the subtree and its root rows");
+                P.drow.push_back(d);
+            }
+        }
+        for (int s : fronts)
+            for (int j = 0; j < S.sn_k[s]; ++j) dpos[S.sn_first[s] + j] = -1;
+        for (int a = 0; a < u; ++a) dpos[S.sn_rows[S.sn_rows_off[r] + kr + a]] = -1;
+        P.sroot.push_back(r);
+        P.sn_n.push_back(n);
+        P.sR.push_back(R);
+        P.smoff.push_back(P.msub);
+        P.msub += sub_col_off(n, R);
+        P.fptr.push_back((int32_t)P.fsn.size());
+        P.scptr.push_back((int32_t)P.scols.size());
+    }
+    P.nsub = (int)P.sroot.size();
+    // k_sub_m launches grouped by the LDS the dense L needs
+    const size_t lim[3] = {16384, 32768, 65536};
+    std::vector<int32_t> bin[3];
+    for (int t = 0; t < P.nsub; ++t) {
+        const size_t b = (size_t)P.sn_n[t] * P.sR[t] * sizeof(double);
+        const int q = b <= lim[0] ? 0 : (b <= lim[1] ? 1 : 2);
+        bin[q].push_back(t);
+        P.bin_bytes[q] = std::max(P.bin_bytes[q], b);
+    }
+    for (int q = 0; q < 3; ++q) {
+        P.bin_off[q] = (int)P.bins.size();
+        P.bins.insert(P.bins.end(), bin[q].begin(), bin[q].end());
+    }
+    P.bin_off[3] = (int)P.bins.size();
+    P.dense = true;
+}
+
+constexpr int kPiSeg = kPiSegMax;  // entries per forward-product segment
 constexpr int kPiCutDefault = 32;  // fronts per bottom subtree
 
-static PiPlan make_pi_plan(const Symbolic& S, int cut) {
+static PiPlan make_pi_plan(const Symbolic& S, int cut, bool dense) {
     PiPlan P;
     P.cut = cut;
     const int nsn = S.nsn;
-    std::vector<int32_t> size(nsn, 1);
+    std::vector<int32_t> size(nsn, 1), ncol(S.sn_k.begin(), S.sn_k.end());
     for (int s = 0; s < nsn; ++s)
-        if (S.sn_parent[s] >= 0) size[S.sn_parent[s]] += size[s];
+        if (S.sn_parent[s] >= 0) {
+            size[S.sn_parent[s]] += size[s];
+            ncol[S.sn_parent[s]] += ncol[s];
+        }
+    // a front is in the top when its subtree is too large to be a bottom task;
+    // every bound grows towards the root, so the top is closed under parents
+    auto in_top = [&](int s) {
+        if (size[s] > cut) return true;
+        return dense && (ncol[s] > kSubCols || ncol[s] + S.sn_c[s] - S.sn_k[s] > kSubRows);
+    };
     std::vector<int32_t> tloc(S.ns, -1), tsl(nsn, -1);
     int cmax = 1, kmax = 1;
     for (int s = 0; s < nsn; ++s) {
-        if (size[s] <= cut) continue;
+        if (!in_top(s)) continue;
         tsl[s] = (int32_t)P.sn.size();
         P.sn.push_back(s);
         for (int j = 0; j < S.sn_k[s]; ++j) {
@@ -1791,6 +2144,7 @@ static PiPlan make_pi_plan(const Symbolic& S, int cut) {
         P.cpos.insert(P.cpos.end(), per[t].begin(), per[t].end());
         P.cptr[t + 1] = (int32_t)P.cpos.size();
     }
+    if (dense) make_sub_plan(S, tsl, P);
     return P;
 }
 
@@ -1885,21 +2239,44 @@ struct Instance::Impl {
         DBuf<int64_t> boff, poff, pos, segq, cpos;
         DBuf<double> part;
         LdsDev bottom[2];  // bottom-subtree LDS tasks, [0] float, [1] double
+        // dense bottom subtrees
+        bool dense = false;
+        int nsub = 0;
+        int64_t msub = 0;
+        DBuf<int32_t> sroot, sn_n, sR, fptr, fsn, fdcol, drow, scptr, scols, bins;
+        DBuf<int64_t> fdoff, smoff;
+        int bin_off[4] = {0, 0, 0, 0};
+        size_t bin_bytes[3] = {0, 0, 0};
         PiSizes sizes() const {
             PiSizes z;
             if (ok) {
                 z.nblock = nblock;
                 z.nq = nq;
                 z.ntc = ntc;
+                z.msub = dense ? msub : 0;
             }
             return z;
         }
     } pi;
     bool use_pi = true;
+    bool pi_dense = true;   // bottom subtrees as dense solve matrices (else LDS tasks)
+    bool meas_by_type = true;  // measurement rows grouped by evaluation class (else by anchor bus only)
     int pi_cut = 0;    // fronts per bottom subtree; 0 = default
     int pi_waves = 4;  // waves per bottom-subtree workgroup under the partitioned inverse
     // dependency-driven sweeps (all fronts fit in LDS)
     bool use_flow = false;
+    // captured REALTIME iterations (one per precision / weighting variant);
+    // retired when launch plans change (plan_epoch) or any device buffer is
+    // (re)allocated (g_alloc_epoch)
+    struct GraphSlot {
+        uint64_t alloc_epoch, plan_epoch;
+        int prec, robust, newton;
+        double gamma, vmin, vmax;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphSlot> graphs;
+    uint64_t plan_epoch = 0;
+    bool lds_flags = false;  // task solves wait on per-front LDS flags instead of per-level barriers
     bool flow_ok = false;       // every front fits the LDS front of the dataflow kernels
     bool force_levels = false;  // caller asked for level-by-level sweeps
     int max_c = 0;
@@ -1920,6 +2297,29 @@ struct Instance::Impl {
     std::vector<Staged> staged;
 };
 
+// Task-sweep plan of the factorization: bottom subtrees of at most `cut` fronts
+// per workgroup, top levels wider than `wcut` fronts one launch each, the rest
+// in one workgroup.
+static void build_factor_plan(Instance::Impl& D, const Symbolic& S, int cut, int wcut, size_t& bytes,
+                              hipStream_t s) {
+    SweepPlan P = make_plan(S, cut, wcut, D.bottom_waves, D.tail_waves, true);
+    D.t_fronts.upload(P.fronts, bytes, s);
+    D.t_lvl_ptr.upload(P.lvl_ptr, bytes, s);
+    D.t_lvl_off.upload(P.lvl_off, bytes, s);
+    std::vector<int32_t> mid = P.mid;
+    if (mid.empty()) mid.push_back(0);
+    D.t_mid.upload(mid, bytes, s);
+    HIPCHECK(hipStreamSynchronize(s));
+    D.mid_off = P.mid_off;
+    D.mid_cnt = P.mid_cnt;
+    D.n_bottom = P.n_bottom;
+    D.has_tail = P.has_tail;
+    D.plan_cut = cut;
+    D.plan_wcut = wcut;
+    D.plan_cost_us = P.cost_us;
+    D.tasks_ok = true;
+}
+
 // Partitioned-inverse plan for the current pi_cut / pi_waves: bottom LDS tasks
 // of both element sizes and the top index lists (D.pi.ok = false when the
 // fronts or blocks do not fit).
@@ -1929,14 +2329,17 @@ static void build_pi(Instance::Impl& D, const Symbolic& S, size_t& bytes, hipStr
     if (S.nsn > 0 && S.max_c <= kFrontB) {
         const int cut = D.pi_cut > 0 ? D.pi_cut : kPiCutDefault;
         const size_t budget = 64 * 1024;
-        SweepPlan SP = make_plan(S, cut, -1, D.pi_waves, D.lds_waves_tail, true);
+        const bool dense = D.pi_dense;
         LdsPlan LP[2];
         bool fit = true;
-        for (int e = 0; e < 2; ++e)
-            fit = fit && make_lds_plan(S, SP, e ? sizeof(double) : sizeof(float), budget, D.pi_waves,
-                                       D.lds_waves_tail, true, LP[e]);
+        if (!dense) {
+            SweepPlan SP = make_plan(S, cut, -1, D.pi_waves, D.lds_waves_tail, true);
+            for (int e = 0; e < 2; ++e)
+                fit = fit && make_lds_plan(S, SP, e ? sizeof(double) : sizeof(float), budget, D.pi_waves,
+                                           D.lds_waves_tail, true, LP[e]);
+        }
         PiPlan PP;
-        if (fit) PP = make_pi_plan(S, cut);
+        if (fit) PP = make_pi_plan(S, cut, dense);
         if (fit && 3 * (size_t)PP.slice * sizeof(double) <= 65536 && PP.nblock < ((int64_t)1 << 40)) {
             auto up = [&](DBuf<int32_t>& b, std::vector<int32_t> v) {
                 if (v.empty()) v.push_back(0);
@@ -1946,7 +2349,7 @@ static void build_pi(Instance::Impl& D, const Symbolic& S, size_t& bytes, hipStr
                 if (v.empty()) v.push_back(0);
                 b.upload(v, bytes, s);
             };
-            for (int e = 0; e < 2; ++e) {
+            for (int e = 0; e < 2 && !dense; ++e) {
                 Instance::Impl::LdsDev& L = D.pi.bottom[e];
                 up(L.th, LP[e].th);
                 up(L.meta, LP[e].meta);
@@ -1975,6 +2378,32 @@ static void build_pi(Instance::Impl& D, const Symbolic& S, size_t& bytes, hipStr
             up64(Q.segq, PP.segq);
             up64(Q.cpos, PP.cpos);
             Q.part.alloc(std::max(1, PP.nseg), bytes);
+            Q.dense = dense;
+            if (dense) {
+                for (int e = 0; e < 2; ++e) {
+                    Instance::Impl::LdsDev& L = D.pi.bottom[e];
+                    for (DBuf<int32_t>* b : {&L.th, &L.meta, &L.lvl, &L.ch, &L.rm, &L.src, &L.mid}) b->release(bytes);
+                    L.ok = false;
+                    L.n_bottom = 0;
+                    L.bottom_bytes = 0;
+                }
+                up(Q.sroot, PP.sroot);
+                up(Q.sn_n, PP.sn_n);
+                up(Q.sR, PP.sR);
+                up(Q.fptr, PP.fptr);
+                up(Q.fsn, PP.fsn);
+                up(Q.fdcol, PP.fdcol);
+                up(Q.drow, PP.drow);
+                up(Q.scptr, PP.scptr);
+                up(Q.scols, PP.scols);
+                up(Q.bins, PP.bins);
+                up64(Q.fdoff, PP.fdoff);
+                up64(Q.smoff, PP.smoff);
+                Q.nsub = PP.nsub;
+                Q.msub = PP.msub;
+                for (int q = 0; q < 4; ++q) Q.bin_off[q] = PP.bin_off[q];
+                for (int q = 0; q < 3; ++q) Q.bin_bytes[q] = PP.bin_bytes[q];
+            }
             HIPCHECK(hipStreamSynchronize(s));
             Q.cut = cut;
             Q.ntop = PP.ntop;
@@ -1998,6 +2427,11 @@ Instance::Instance(int device, int precision) : device_(device), precision_(prec
     d_->st.alloc(1, bytes_);
     for (int p = 0; p < PH_COUNT; ++p)
         for (int q = 0; q < 2; ++q) HIPCHECK(hipEventCreate(&d_->pev[p][q]));
+}
+
+static void drop_graphs(Instance::Impl& D) {
+    for (auto& g : D.graphs) (void)hipGraphExecDestroy(g.exec);
+    D.graphs.clear();
 }
 
 Instance::~Instance() {
@@ -2029,12 +2463,15 @@ Instance::~Instance() {
         auto& P = d_->pi;
         rel(P.sn); rel(P.plen); rel(P.ploc); rel(P.tcol); rel(P.colsn); rel(P.src); rel(P.segptr); rel(P.cptr); rel(P.parent);
         rel(P.boff); rel(P.poff); rel(P.pos); rel(P.segq); rel(P.cpos); rel(P.part);
+        rel(P.sroot); rel(P.sn_n); rel(P.sR); rel(P.fptr); rel(P.fsn); rel(P.fdcol); rel(P.drow); rel(P.scptr);
+        rel(P.scols); rel(P.bins); rel(P.fdoff); rel(P.smoff);
         for (auto& L : P.bottom) {
             rel(L.th); rel(L.meta); rel(L.lvl); rel(L.ch); rel(L.rm); rel(L.src); rel(L.mid);
         }
     }
     d_->f32.release(b);
     d_->f64.release(b);
+    drop_graphs(*d_);
     for (auto& sb : d_->staged) {
         sb.idx.release(b);
         sb.val.release(b);
@@ -2051,6 +2488,7 @@ Instance::~Instance() {
 }
 
 void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
+    ++d_->plan_epoch;  // launch plans change: captured graphs retire
     HIPCHECK(hipSetDevice(device_));
     S_ = std::move(Sp);
     const Symbolic& S = *S_;
@@ -2099,9 +2537,19 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
     D.lk.assign(nb, 0);
     for (int p = 0; p < nb; ++p) D.lk[D.kl[p]] = p;
     auto anchor = [&](int i) { return is_bus_measurement(h.mtype[i]) ? h.mloc[i] : h.from[h.mloc[i]]; };
+    // rows by evaluation class (bus values, injections, branch flows) when
+    // asked, so that a wavefront runs one code path; by anchor bus inside a class
+    auto mclass = [&](int i) {
+        if (!D.meas_by_type) return 0;
+        const int t = h.mtype[i];
+        return (t == 0 || t == 1) ? 0 : ((t == 2 || t == 3 || t == 7 || t == 8) ? 1 : 2);
+    };
     D.ml.resize(m);
     for (int i = 0; i < m; ++i) D.ml[i] = i;
-    std::stable_sort(D.ml.begin(), D.ml.end(), [&](int a, int b) { return D.lb[anchor(a)] < D.lb[anchor(b)]; });
+    std::stable_sort(D.ml.begin(), D.ml.end(), [&](int a, int b) {
+        const int ca = mclass(a), cb = mclass(b);
+        return ca != cb ? ca < cb : D.lb[anchor(a)] < D.lb[anchor(b)];
+    });
     D.lm.assign(m, 0);
     for (int p = 0; p < m; ++p) D.lm[D.ml[p]] = p;
     auto bus_perm = [&](const auto& v) {
@@ -2355,22 +2803,8 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
                 }
             }
         if (best_cut) {
-            SweepPlan P = make_plan(S, best_cut, best_w, D.bottom_waves, D.tail_waves, true);
-            D.t_fronts.upload(P.fronts, bytes_, s);
-            D.t_lvl_ptr.upload(P.lvl_ptr, bytes_, s);
-            D.t_lvl_off.upload(P.lvl_off, bytes_, s);
-            std::vector<int32_t> mid = P.mid;
-            if (mid.empty()) mid.push_back(0);
-            D.t_mid.upload(mid, bytes_, s);
-            HIPCHECK(hipStreamSynchronize(s));
-            D.mid_off = P.mid_off;
-            D.mid_cnt = P.mid_cnt;
-            D.n_bottom = P.n_bottom;
-            D.has_tail = P.has_tail;
-            D.plan_cut = best_cut;
-            D.plan_wcut = best_w;
+            build_factor_plan(D, S, best_cut, best_w, bytes_, s);
             D.plan_cost_us = best;
-            D.tasks_ok = true;
         }
     }
     // LDS-resident solve plans (float and double elements)
@@ -2464,6 +2898,7 @@ std::vector<typename V::value_type> take(const V& v, const std::vector<int32_t>&
 }  // namespace
 
 void Instance::update_branches(const HostArrays& h) {
+    ++d_->plan_epoch;  // launch plans change: captured graphs retire
     HIPCHECK(hipSetDevice(device_));
     Impl& D = *d_;
     hipStream_t s = D.stream;
@@ -2494,6 +2929,7 @@ void Instance::update_branches(const HostArrays& h) {
 // One breaker operation: branch k (insertion index) closed or open.  Staged on
 // the host; refresh_topology() applies all staged operations at once.
 void Instance::set_branch_closed(int k, bool closed) {
+    ++d_->plan_epoch;  // launch plans change: captured graphs retire
     Impl& D = *d_;
     if (k < 0 || k >= D.nb) throw std::invalid_argument("branch index out of range");
     D.status_h[D.lk[k]] = closed ? 1 : 0;
@@ -2506,6 +2942,7 @@ void Instance::set_branch_closed(int k, bool closed) {
 }
 
 void Instance::refresh_topology() {
+    ++d_->plan_epoch;  // launch plans change: captured graphs retire
     Impl& D = *d_;
     if (!D.topo_staged) return;
     HIPCHECK(hipSetDevice(device_));
@@ -2748,12 +3185,27 @@ PiArgs pi_args(const Instance::Impl& D) {
 
 bool pi_active(const Instance::Impl& D) { return D.use_pi && D.pi.ok; }
 
+SubArgs sub_args(const Instance::Impl& D) {
+    const Instance::Impl::PiDev& P = D.pi;
+    return SubArgs{P.nsub,   P.msub,    P.sroot.p, P.sn_n.p,  P.sR.p,    P.fptr.p, P.fsn.p,
+                   P.fdcol.p, P.fdoff.p, P.drow.p, P.scptr.p, P.scols.p, P.smoff.p};
+}
+
+bool sub_active(const Instance::Impl& D) { return pi_active(D) && D.pi.dense && D.pi.nsub > 0; }
+
 // L_TT^-1 blocks of the current factor and their row-grouped copy
 template <typename T>
 void launch_pi_inverse(Instance::Impl& D) {
     Instance::Impl::PiDev& PI = D.pi;
-    if (!pi_active(D) || PI.ntop == 0) return;
     PrecBufs<T>& B = bufs<T>(D);
+    if (sub_active(D))
+        for (int q = 0; q < 3; ++q) {
+            const int cnt = PI.bin_off[q + 1] - PI.bin_off[q];
+            if (cnt)
+                k_sub_m<T><<<cnt, kBlock, PI.bin_bytes[q], D.stream>>>(sub_args(D), solve_args<T>(D),
+                                                                      PI.bins.p + PI.bin_off[q], B.msub.p);
+        }
+    if (!pi_active(D) || PI.ntop == 0) return;
     const size_t per_wave = 3 * (size_t)PI.slice * sizeof(T);
     const int wpb = (int)std::max<size_t>(1, std::min<size_t>(4, 65536 / per_wave));
     k_pi_inv<T><<<(PI.ntop + wpb - 1) / wpb, kWave * wpb, wpb * per_wave, D.stream>>>(pi_args(D), solve_args<T>(D),
@@ -2777,18 +3229,39 @@ void prof_begin(Prof* pf, int ph);
 void prof_end(Prof* pf, int ph, int launches);
 
 template <typename T>
+void launch_fwd_lds(const Instance::Impl& D, int grid, int waves, size_t bytes, const LdsTaskArgs& LA,
+                    const SolveArgs<T>& A, int base) {
+    if (D.lds_flags)
+        k_fwd_lds<T, true><<<grid, kWave * waves, bytes, D.stream>>>(LA, A, base);
+    else
+        k_fwd_lds<T, false><<<grid, kWave * waves, bytes, D.stream>>>(LA, A, base);
+}
+
+template <typename T>
+void launch_bwd_lds(const Instance::Impl& D, int grid, int waves, size_t bytes, const LdsTaskArgs& LA,
+                    const SolveArgs<T>& A, int base) {
+    if (D.lds_flags)
+        k_bwd_lds<T, true><<<grid, kWave * waves, bytes, D.stream>>>(LA, A, base);
+    else
+        k_bwd_lds<T, false><<<grid, kWave * waves, bytes, D.stream>>>(LA, A, base);
+}
+
+template <typename T>
 void launch_trisolve(Instance::Impl& D, Prof* pf = nullptr) {
     SolveArgs<T> A = solve_args<T>(D);
     if (!D.use_flow && pi_active(D)) {
         // bottom subtrees in LDS tasks, the top by the partitioned inverse
         const Instance::Impl::PiDev& PI = D.pi;
         const Instance::Impl::LdsDev& PB = PI.bottom[sizeof(T) == sizeof(double) ? 1 : 0];
-        LdsTaskArgs LA{PB.th.p, PB.meta.p, PB.lvl.p, PB.ch.p, PB.rm.p, PB.src.p};
+        LdsTaskArgs LA{PB.th.p, PB.meta.p, PB.lvl.p, PB.ch.p, PB.rm.p, PB.src.p, D.st.p};
         PrecBufs<T>& B = bufs<T>(D);
         const int nw = kBlock / kWave;
+        const bool dense = sub_active(D);
         prof_begin(pf, PH_FWD);
-        if (PB.n_bottom)
-            k_fwd_lds<T><<<PB.n_bottom, kWave * D.pi_waves, PB.bottom_bytes, D.stream>>>(LA, A, 0);
+        if (dense)
+            k_sub_fwd<T><<<(PI.nsub + nw - 1) / nw, kBlock, 0, D.stream>>>(sub_args(D), A, B.msub.p);
+        else if (PB.n_bottom)
+            launch_fwd_lds<T>(D, PB.n_bottom, D.pi_waves, PB.bottom_bytes, LA, A, 0);
         if (PI.ntc) {
             k_pi_assemble<T><<<nblk(PI.ntc), kBlock, 0, D.stream>>>(PI.ntc, PI.tcol.p, PI.cptr.p, PI.cpos.p, A.x,
                                                                    A.ubuf, B.ctop.p);
@@ -2796,35 +3269,37 @@ void launch_trisolve(Instance::Impl& D, Prof* pf = nullptr) {
                                                                            B.pinv_rows.p, B.ctop.p, PI.part.p);
             k_pi_rowsum<T><<<(PI.ntc + nw - 1) / nw, kBlock, 0, D.stream>>>(PI.ntc, PI.segptr.p, PI.part.p, B.ytop.p);
         }
-        prof_end(pf, PH_FWD, (PB.n_bottom ? 1 : 0) + (PI.ntc ? 3 : 0));
+        prof_end(pf, PH_FWD, (dense || PB.n_bottom ? 1 : 0) + (PI.ntc ? 3 : 0));
         prof_begin(pf, PH_BWD);
         if (PI.ntc)
             k_pi_bwd<T><<<(PI.ntc + nw - 1) / nw, kBlock, 0, D.stream>>>(pi_args(D), A, B.pinv.p, B.ytop.p);
-        if (PB.n_bottom)
-            k_bwd_lds<T><<<PB.n_bottom, kWave * D.pi_waves, PB.bottom_bytes, D.stream>>>(LA, A, 0);
-        prof_end(pf, PH_BWD, (PB.n_bottom ? 1 : 0) + (PI.ntc ? 1 : 0));
+        if (dense)
+            k_sub_bwd<T><<<(PI.nsub + nw - 1) / nw, kBlock, 0, D.stream>>>(sub_args(D), A, B.msub.p + PI.msub);
+        else if (PB.n_bottom)
+            launch_bwd_lds<T>(D, PB.n_bottom, D.pi_waves, PB.bottom_bytes, LA, A, 0);
+        prof_end(pf, PH_BWD, (dense || PB.n_bottom ? 1 : 0) + (PI.ntc ? 1 : 0));
         return;
     }
     Instance::Impl::LdsDev& LD = D.lds[sizeof(T) == sizeof(double) ? 1 : 0];
     if (!D.use_flow && D.use_lds && LD.ok) {
-        LdsTaskArgs LA{LD.th.p, LD.meta.p, LD.lvl.p, LD.ch.p, LD.rm.p, LD.src.p};
+        LdsTaskArgs LA{LD.th.p, LD.meta.p, LD.lvl.p, LD.ch.p, LD.rm.p, LD.src.p, D.st.p};
         const int nmid = (int)LD.mid_off.size();
         const int nlaunch = (LD.n_bottom ? 1 : 0) + nmid + (int)LD.tail_tasks.size();
         prof_begin(pf, PH_FWD);
         if (LD.n_bottom)
-            k_fwd_lds<T><<<LD.n_bottom, kWave * D.lds_waves_bottom, LD.bottom_bytes, D.stream>>>(LA, A, 0);
+            launch_fwd_lds<T>(D, LD.n_bottom, D.lds_waves_bottom, LD.bottom_bytes, LA, A, 0);
         for (int l = 0; l < nmid; ++l)
             k_fwd<T, kFrontB><<<LD.mid_cnt[l], kWave, 0, D.stream>>>(A, LD.mid.p + LD.mid_off[l]);
         for (size_t c = 0; c < LD.tail_tasks.size(); ++c)
-            k_fwd_lds<T><<<1, kWave * D.lds_waves_tail, LD.tail_bytes[c], D.stream>>>(LA, A, LD.tail_tasks[c]);
+            launch_fwd_lds<T>(D, 1, D.lds_waves_tail, LD.tail_bytes[c], LA, A, LD.tail_tasks[c]);
         prof_end(pf, PH_FWD, nlaunch);
         prof_begin(pf, PH_BWD);
         for (int c = (int)LD.tail_tasks.size() - 1; c >= 0; --c)
-            k_bwd_lds<T><<<1, kWave * D.lds_waves_tail, LD.tail_bytes[c], D.stream>>>(LA, A, LD.tail_tasks[c]);
+            launch_bwd_lds<T>(D, 1, D.lds_waves_tail, LD.tail_bytes[c], LA, A, LD.tail_tasks[c]);
         for (int l = nmid - 1; l >= 0; --l)
             k_bwd<T, kFrontB><<<LD.mid_cnt[l], kWave, 0, D.stream>>>(A, LD.mid.p + LD.mid_off[l]);
         if (LD.n_bottom)
-            k_bwd_lds<T><<<LD.n_bottom, kWave * D.lds_waves_bottom, LD.bottom_bytes, D.stream>>>(LA, A, 0);
+            launch_bwd_lds<T>(D, LD.n_bottom, D.lds_waves_bottom, LD.bottom_bytes, LA, A, 0);
         prof_end(pf, PH_BWD, nlaunch);
         return;
     }
@@ -2941,7 +3416,9 @@ void launch_pcg(Instance::Impl& D, int steps) {
 }
 
 int factor_launches(const Instance::Impl& D) {
-    const int inv = pi_active(D) && D.pi.ntop ? 2 : 0;
+    int inv = pi_active(D) && D.pi.ntop ? 2 : 0;
+    if (sub_active(D))
+        for (int q = 0; q < 3; ++q) inv += D.pi.bin_off[q + 1] > D.pi.bin_off[q] ? 1 : 0;
     if (D.use_flow) return 1 + inv;
     if (D.use_tasks && D.tasks_ok) return (D.n_bottom ? 1 : 0) + (int)D.mid_off.size() + (D.has_tail ? 1 : 0) + inv;
     return D.level_launches + inv;
@@ -3024,6 +3501,54 @@ void iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p, bool 
     pf.end(PH_UPDATE, 2);
 }
 
+// One iteration on the cached factor (no refactor, direct solve) as a HIP
+// graph: the launches of iteration() and the status read-back are captured
+// once per variant and replayed with one launch; the host then waits once.
+// The captured work holds no memset node and no host-varying argument (the
+// dependency-driven sweeps, whose epochs change per launch, never come here).
+template <typename T>
+void graph_iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p, T piv_tol, bool newton) {
+    const uint64_t ae = g_alloc_epoch.load(std::memory_order_relaxed), pe = D.plan_epoch;
+    const int prec = (int)sizeof(T);
+    hipGraphExec_t exec = nullptr;
+    for (size_t k = 0; k < D.graphs.size();) {
+        Instance::Impl::GraphSlot& g = D.graphs[k];
+        if (g.alloc_epoch != ae || g.plan_epoch != pe) {
+            (void)hipGraphExecDestroy(g.exec);
+            D.graphs.erase(D.graphs.begin() + k);
+            continue;
+        }
+        if (g.prec == prec && g.robust == p.robust && g.newton == (int)newton && g.gamma == p.gamma &&
+            g.vmin == p.vmin && g.vmax == p.vmax)
+            exec = g.exec;
+        ++k;
+    }
+    if (!exec) {
+        Prof off{D, false};
+        hipGraph_t graph = nullptr;
+        HIPCHECK(hipStreamBeginCapture(D.stream, hipStreamCaptureModeThreadLocal));
+        try {
+            iteration<T>(D, S, p, false, piv_tol, off, newton);
+            HIPCHECK(hipMemcpyAsync(D.hst, D.st.p, sizeof(DevStatus), hipMemcpyDeviceToHost, D.stream));
+        } catch (...) {
+            (void)hipStreamEndCapture(D.stream, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        HIPCHECK(hipStreamEndCapture(D.stream, &graph));
+        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIPCHECK(e);
+        if (D.graphs.size() >= 8) {
+            (void)hipGraphExecDestroy(D.graphs.front().exec);
+            D.graphs.erase(D.graphs.begin());
+        }
+        D.graphs.push_back({ae, pe, prec, p.robust, (int)newton, p.gamma, p.vmin, p.vmax, exec});
+    }
+    HIPCHECK(hipGraphLaunch(exec, D.stream));
+    HIPCHECK(hipStreamSynchronize(D.stream));
+}
+
 }  // namespace
 
 SolveOutcome Instance::solve(const SolveParams& p) {
@@ -3057,16 +3582,24 @@ SolveOutcome Instance::solve(const SolveParams& p) {
             // Huber: the IRLS gain until the steps are small, then reduced gain
             // weights of the outliers (same minimiser, faster contraction)
             const bool newton = p.robust == 2 && !newton_off && last_step < kNewtonSwitch;
+            // replayed graph for iterations on the cached factor
+            const bool graph = use_graphs_ && !profiling_ && !refactor && !D.use_flow && (fp64 || p.pcg_steps <= 0);
             if (fp64) {
                 if (!D.f64.ready) D.f64.alloc(S, bytes_, D.pi.sizes());
                 SolveParams pd = p;
                 pd.pcg_steps = 0;  // the FP64 factor solves the Newton system directly
-                iteration<double>(D, S, pd, refactor, 1e-13, pf, newton);
+                if (graph)
+                    graph_iteration<double>(D, S, pd, 1e-13, newton);
+                else
+                    iteration<double>(D, S, pd, refactor, 1e-13, pf, newton);
             } else {
-                iteration<float>(D, S, p, refactor, 1e-5f, pf, newton);
+                if (graph)
+                    graph_iteration<float>(D, S, p, 1e-5f, newton);
+                else
+                    iteration<float>(D, S, p, refactor, 1e-5f, pf, newton);
             }
             HIPCHECK(hipGetLastError());
-            read_status(D);
+            if (!graph) read_status(D);
             pf.collect(ktimes_);
             if (D.hst->piv && newton) {
                 // the gain with reduced outlier weights lost rank: IRLS weights from here on
@@ -3077,7 +3610,9 @@ SolveOutcome Instance::solve(const SolveParams& p) {
             }
             if (D.hst->flow_err) {
                 // a dependency wait timed out: redo this iteration level by level
+                ++D.plan_epoch;
                 D.use_flow = false;
+                D.lds_flags = false;
                 factor_valid_ = false;
                 --it;
                 continue;
@@ -3257,6 +3792,7 @@ void Instance::debug_jacobian_gain(double* hv, double* gv) {
 }
 
 int Instance::debug_factor(double* fronts) {
+    ++d_->plan_epoch;  // launch plans change: captured graphs retire
     HIPCHECK(hipSetDevice(device_));
     Impl& D = *d_;
     const Symbolic& S = *S_;
@@ -3275,6 +3811,7 @@ int Instance::debug_factor(double* fronts) {
         read_status(D);
         if (!D.hst->flow_err) break;
         D.use_flow = false;
+        D.lds_flags = false;
     }
     if (fronts) {
         if (precision_ == 1)
@@ -3287,6 +3824,7 @@ int Instance::debug_factor(double* fronts) {
 }
 
 void Instance::debug_solve(const double* b, double* dx) {
+    ++d_->plan_epoch;  // launch plans change: captured graphs retire
     HIPCHECK(hipSetDevice(device_));
     Impl& D = *d_;
     DBuf<double> bb, xx;
@@ -3310,6 +3848,7 @@ void Instance::debug_solve(const double* b, double* dx) {
         read_status(D);
         if (!D.hst->flow_err) break;
         D.use_flow = false;
+        D.lds_flags = false;
     }
     HIPCHECK(hipMemcpyAsync(dx, xx.p, D.ns * sizeof(double), hipMemcpyDeviceToHost, D.stream));
     HIPCHECK(hipStreamSynchronize(D.stream));
@@ -3348,24 +3887,58 @@ void Instance::apply_staged(int slot) {
 }
 
 void Instance::set_level_schedule(bool on) {
+    ++d_->plan_epoch;  // launch plans change: captured graphs retire
     d_->force_levels = on;
     d_->use_flow = !on && d_->flow_ok;
 }
 
 bool Instance::dataflow_sweeps() const { return d_->use_flow; }
 
-void Instance::set_task_sweeps(bool on) { d_->use_tasks = on; }
+void Instance::set_task_sweeps(bool on) {
+    ++d_->plan_epoch;
+    d_->use_tasks = on;
+}
 
-void Instance::set_lds_solve(bool on) { d_->use_lds = on; }
+void Instance::set_lds_solve(bool on) {
+    ++d_->plan_epoch;
+    d_->use_lds = on;
+}
+
+void Instance::set_dense_subtrees(bool on) {
+    ++d_->plan_epoch;
+    d_->pi_dense = on;
+}
+
+void Instance::set_meas_by_type(bool on) {
+    ++d_->plan_epoch;
+    d_->meas_by_type = on;
+}
+
+void Instance::set_xcd_remap(bool on) {
+    ++d_->plan_epoch;  // launch plans change: captured graphs retire
+    HIPCHECK(hipSetDevice(device_));
+    const int v = on ? 1 : 0;
+    HIPCHECK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_xcd_remap), &v, sizeof(int), 0, hipMemcpyHostToDevice, d_->stream));
+    HIPCHECK(hipStreamSynchronize(d_->stream));
+}
 
 void Instance::set_partitioned_inverse(bool on, int cut, int waves) {
+    ++d_->plan_epoch;  // launch plans change: captured graphs retire
     if (d_->use_pi != on) factor_valid_ = false;  // the inverse blocks are made with the factor
     d_->use_pi = on;
     if (cut >= 0) d_->pi_cut = cut;
     if (waves >= 1 && waves <= 16) d_->pi_waves = waves;
 }
 
+void Instance::set_lds_flags(bool on) {
+    ++d_->plan_epoch;
+    d_->lds_flags = on;
+}
+
+bool Instance::lds_flags() const { return d_->lds_flags; }
+
 void Instance::set_lds_waves(int bottom, int tail) {
+    ++d_->plan_epoch;  // launch plans change: captured graphs retire
     if (bottom >= 1 && bottom <= 16) d_->lds_waves_bottom = bottom;
     if (tail >= 1 && tail <= 16) d_->lds_waves_tail = tail;
 }
@@ -3385,11 +3958,18 @@ std::string Instance::sweep_plan() const {
     const Impl::LdsDev& L = D.lds[precision_ == 1 || fp64_factor_active_ ? 1 : 0];
     if (D.use_pi && D.pi.ok && !D.use_flow) {
         const Impl::LdsDev& PB = D.pi.bottom[precision_ == 1 || fp64_factor_active_ ? 1 : 0];
-        snprintf(buf, sizeof(buf),
-                 "; solve partitioned-inverse(cut=%d bottom=%d/%zuB top supernodes=%d columns=%d blocks=%lld "
-                 "row-entries=%lld segments=%d)",
-                 D.pi.cut, PB.n_bottom, PB.bottom_bytes, D.pi.ntop, D.pi.ntc, (long long)D.pi.nblock,
-                 (long long)D.pi.nq, D.pi.nseg);
+        if (D.pi.dense)
+            snprintf(buf, sizeof(buf),
+                     "; solve partitioned-inverse(cut=%d dense-subtrees=%d/%lld entries top supernodes=%d "
+                     "columns=%d blocks=%lld row-entries=%lld segments=%d)",
+                     D.pi.cut, D.pi.nsub, (long long)D.pi.msub, D.pi.ntop, D.pi.ntc, (long long)D.pi.nblock,
+                     (long long)D.pi.nq, D.pi.nseg);
+        else
+            snprintf(buf, sizeof(buf),
+                     "; solve partitioned-inverse(cut=%d bottom=%d/%zuB%s top supernodes=%d columns=%d blocks=%lld "
+                     "row-entries=%lld segments=%d)",
+                     D.pi.cut, PB.n_bottom, PB.bottom_bytes, D.lds_flags ? " flags" : "", D.pi.ntop, D.pi.ntc,
+                     (long long)D.pi.nblock, (long long)D.pi.nq, D.pi.nseg);
         out += buf;
     } else if (D.use_lds && L.ok) {
         snprintf(buf, sizeof(buf), "; solve lds-tasks(cut=%d bottom=%d/%zuB mid=%d tail-chunks=%zu est=%.0fus)", L.cut,
@@ -3429,18 +4009,36 @@ double time_solves(Instance::Impl& D, size_t& bytes, int reps) {
     save.release(tmp);
     return ms / std::max(1, reps);
 }
+// Average time of the factorization sweeps alone (G unchanged between repetitions).
+template <typename T>
+double time_factor(Instance::Impl& D, int reps) {
+    launch_factor_sweeps<T>(D, sizeof(T) == sizeof(double) ? T(1e-13) : T(1e-5));  // warm-up
+    HIPCHECK(hipEventRecord(D.ev0, D.stream));
+    for (int r = 0; r < reps; ++r) launch_factor_sweeps<T>(D, sizeof(T) == sizeof(double) ? T(1e-13) : T(1e-5));
+    HIPCHECK(hipEventRecord(D.ev1, D.stream));
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipEventSynchronize(D.ev1));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, D.ev0, D.ev1));
+    return ms / std::max(1, reps);
+}
 }  // namespace
 
 std::string Instance::autotune_solve(int reps) {
+    ++d_->plan_epoch;  // launch plans change: captured graphs retire
     HIPCHECK(hipSetDevice(device_));
     Impl& D = *d_;
     const Symbolic& S = *S_;
     if (!factor_valid_) throw std::invalid_argument("autotune needs a factored gain matrix: solve first");
     if (!D.use_pi || D.use_flow) return "partitioned inverse not in use";
     const bool f64 = precision_ == 1 || fp64_factor_active_;
-    const int cuts[] = {16, 32, 64, 128};
-    const int waves[] = {2, 4, 8};
+    // dense bottom subtrees have no in-task schedule: only the cut matters
+    const bool dense = D.pi_dense;
+    const std::vector<int> cuts = dense ? std::vector<int>{8, 16, 32, 64, 128} : std::vector<int>{16, 32, 64, 128};
+    const std::vector<int> waves = dense ? std::vector<int>{D.pi_waves} : std::vector<int>{2, 4, 8};
+    const int nflags = dense ? 1 : 2;
     int bc = D.pi_cut, bw = D.pi_waves;
+    bool bf = D.lds_flags;
     double best = 1e300;
     std::string log;
     char buf[96];
@@ -3450,28 +4048,55 @@ std::string Instance::autotune_solve(int reps) {
             D.pi_waves = w;
             build_pi(D, S, bytes_, D.stream);
             if (!D.pi.ok) continue;
-            const double ms = f64 ? time_solves<double>(D, bytes_, reps) : time_solves<float>(D, bytes_, reps);
-            snprintf(buf, sizeof(buf), "%s%d/%d:%.3f", log.empty() ? "" : " ", cut, w, ms);
-            log += buf;
-            if (ms < best) {
-                best = ms;
-                bc = cut;
-                bw = w;
+            for (int fl = 0; fl < nflags; ++fl) {
+                D.lds_flags = dense ? D.lds_flags : fl != 0;
+                const double ms = f64 ? time_solves<double>(D, bytes_, reps) : time_solves<float>(D, bytes_, reps);
+                snprintf(buf, sizeof(buf), "%s%d/%d%s:%.3f", log.empty() ? "" : " ", cut, w, fl ? "f" : "", ms);
+                log += buf;
+                if (ms < best) {
+                    best = ms;
+                    bc = cut;
+                    bw = w;
+                    bf = fl != 0;
+                }
             }
         }
     D.pi_cut = bc;
     D.pi_waves = bw;
+    D.lds_flags = bf;
     build_pi(D, S, bytes_, D.stream);
     if (D.f32.ready) D.f32.alloc_pi(D.pi.sizes(), bytes_);
     if (D.f64.ready) D.f64.alloc_pi(D.pi.sizes(), bytes_);
+    // factorization plan: time the task sweeps of each candidate on the current G
+    std::string flog;
+    if (D.tasks_ok && D.use_tasks) {
+        int fc = D.plan_cut, fw = D.plan_wcut;
+        double fbest = 1e300;
+        for (int cut : {64, 128, 256, 512, 1024})
+            for (int wc : {4, 8, 16, 32, 64}) {
+                build_factor_plan(D, S, cut, wc, bytes_, D.stream);
+                const double ms = f64 ? time_factor<double>(D, reps) : time_factor<float>(D, reps);
+                snprintf(buf, sizeof(buf), "%s%d/%d:%.3f", flog.empty() ? "" : " ", cut, wc, ms);
+                flog += buf;
+                if (ms < fbest) {
+                    fbest = ms;
+                    fc = cut;
+                    fw = wc;
+                }
+            }
+        build_factor_plan(D, S, fc, fw, bytes_, D.stream);
+        snprintf(buf, sizeof(buf), "; factor cut=%d wcut=%d (%.3f ms) cut/wcut:ms ", fc, fw, fbest);
+        flog = buf + flog;
+    }
+    // the factor, solve matrices and top inverse of the chosen plans
     if (f64)
-        launch_pi_inverse<double>(D);
+        launch_factor<double>(D, 1e-13);
     else
-        launch_pi_inverse<float>(D);
+        launch_factor<float>(D, 1e-5f);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(D.stream));
-    snprintf(buf, sizeof(buf), "chosen cut=%d waves=%d (%.3f ms per solve); ", bc, bw, best);
-    return buf + std::string("cut/waves:ms ") + log;
+    snprintf(buf, sizeof(buf), "solve cut=%d waves=%d flags=%d (%.3f ms per solve) ", bc, bw, bf ? 1 : 0, best);
+    return buf + std::string("cut/waves[f]:ms ") + log + flog;
 }
 
 int Instance::partitioned_inverse_cut() const { return d_->pi_cut > 0 ? d_->pi_cut : kPiCutDefault; }

@@ -169,6 +169,18 @@ int Engine::set_option(const char* key, double value) {
         if (value < 1 || value > 16) return ST_INVALID_ARGUMENT;
         opt_.pinv_waves = (int)value;
         model_dirty_ = true;
+    } else if (k == "meas_by_type") {
+        opt_.meas_by_type = value != 0;
+        model_dirty_ = true;  // the device layout is rebuilt at the next upload
+    } else if (k == "xcd_remap") {
+        opt_.xcd_remap = value != 0;
+        if (inst_) inst_->set_xcd_remap(opt_.xcd_remap);
+    } else if (k == "pinv_dense") {
+        opt_.pinv_dense = value != 0;
+        model_dirty_ = true;  // the plan is rebuilt at the next upload
+    } else if (k == "lds_flags") {
+        opt_.lds_flags = value != 0;
+        if (inst_) inst_->set_lds_flags(opt_.lds_flags);
     } else if (k == "task_sweeps") {
         opt_.task_sweeps = value != 0;
         if (inst_) inst_->set_task_sweeps(opt_.task_sweeps);
@@ -588,7 +600,11 @@ int Engine::rebuild() {
     inst_->set_lds_solve(opt_.lds_solve);
     inst_->set_lds_waves(opt_.lds_waves_bottom, opt_.lds_waves_tail);
     inst_->set_partitioned_inverse(opt_.pinv, opt_.pinv_cut, opt_.pinv_waves);
+    inst_->set_lds_flags(opt_.lds_flags);
+    inst_->set_dense_subtrees(opt_.pinv_dense);
+    inst_->set_meas_by_type(opt_.meas_by_type);
     inst_->setup(h, sym_);
+    inst_->set_xcd_remap(opt_.xcd_remap);
     inst_->set_state(V_.data(), th_.data());
     model_dirty_ = false;
     params_dirty_ = false;
@@ -856,6 +872,7 @@ int Engine::autotune(int reps, std::string& report) {
     tune_report_ = report;
     opt_.pinv_cut = inst_->partitioned_inverse_cut();
     opt_.pinv_waves = inst_->partitioned_inverse_waves();
+    opt_.lds_flags = inst_->lds_flags();
     return ST_OK;
 }
 

@@ -90,6 +90,10 @@ struct EngineOptions {
     bool huber_newton = true;     // Huber: reduced outlier gain weights near the solution (IRLS before)
     int pinv_cut = 0;             // fronts per bottom subtree under it (0: default)
     int pinv_waves = 4;           // waves per bottom-subtree workgroup under it
+    bool lds_flags = false;       // task solves: per-front LDS flags instead of per-level barriers
+    bool pinv_dense = true;       // partitioned inverse: bottom subtrees as dense solve matrices
+    bool xcd_remap = true;        // XCD-aware block order of the gather kernels
+    bool meas_by_type = true;     // measurement rows grouped by evaluation class on the device
 };
 
 class Engine {

@@ -139,6 +139,18 @@ class Instance {
     void set_partitioned_inverse(bool on, int cut, int waves);
     // waves per workgroup of the bottom-subtree tasks and of the top chunks (takes effect at setup)
     void set_lds_waves(int bottom, int tail);
+    // LDS-resident task solves: each wave waits on per-front LDS flags of the
+    // fronts it depends on (true) or the workgroup meets at a barrier per level
+    void set_lds_flags(bool on);
+    // partitioned-inverse solve: bottom subtrees as one dense solve matrix each
+    // (true) or as LDS tasks walking their fronts (takes effect at setup)
+    void set_dense_subtrees(bool on);
+    // XCD-aware block order of the gather kernels (process-wide on this device)
+    void set_xcd_remap(bool on);
+    // device order of the measurement rows: grouped by evaluation class, each
+    // class by anchor bus (true), or by anchor bus only (takes effect at setup)
+    void set_meas_by_type(bool on);
+    bool lds_flags() const;
     std::string sweep_plan() const;
     // Autotuner of the solve plan: times the forward + backward solves of the
     // current factor for each candidate (bottom-subtree size x waves per

@@ -131,3 +131,46 @@ def test_float_abi_bulk_load_matches_double():
         assert d.type == g.m_type[30] and d.internal_index == 30
         assert abs(d.sigma - g.sigma[30]) < 1e-7
         e.close()
+
+
+def build_lifecycle(tmp_path, double=False):
+    exe = tmp_path / ("lifecycle_f64" if double else "lifecycle")
+    lib = "sle_f64" if double else "sle"
+    cmd = ["gcc", "-std=c99", "-O1", "-I", os.path.join(ROOT, "include"), os.path.join(ROOT, "tests", "c", "lifecycle.c"),
+           "-L", sle.LIB_DIR, f"-l{lib}", f"-Wl,-rpath,{sle.LIB_DIR}", "-o", str(exe)]
+    if double:
+        cmd.insert(1, "-DSLE_USE_DOUBLE")
+    subprocess.run(cmd, check=True)
+    return exe
+
+
+@pytest.mark.parametrize("double", [False, True])
+def test_c_host_links_and_runs_host_side(tmp_path, double):
+    """A plain C program, compiled against include/sle_api.h and linked with the
+    library, runs the reference's lifecycle; without a GPU it stops at
+    sle_Initialize with the documented status."""
+    exe = build_lifecycle(tmp_path, double)
+    if sle.gpu_available():
+        pytest.skip("covered by the GPU run of the same program")
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.strip() == "nogpu"
+
+
+def test_options_documented_in_header_are_accepted():
+    """Every sle_ext_SetOption key listed in include/sle_ext.h is accepted before
+    a device exists; unknown keys and out-of-range values are rejected."""
+    import re
+
+    text = open(os.path.join(ROOT, "include", "sle_ext.h")).read()
+    block = text[text.index("sle_ext_SetOption") - 4000: text.index("sle_ext_SetOption")]
+    keys = sorted(set(re.findall(r'^ \*   "([a-z_0-9]+)"', block, re.M)))
+    assert {"pcg_steps", "pinv", "pinv_dense", "lds_flags", "xcd_remap", "meas_by_type"} <= set(keys), keys
+    e = sle.Engine(double=True, initialize=False)
+    for k in keys:
+        e.set_option(k, 1)
+    with pytest.raises(sle.SLEError):
+        e.set_option("no_such_option", 1)
+    with pytest.raises(sle.SLEError):
+        e.set_option("pcg_steps", 99)
+    e.close()

@@ -5,6 +5,8 @@ supernodes) bit-exact; FP64 values within 1e-12 relative; the FP32 (mixed)
 Jacobian within 2e-6 relative; final states within 1e-6 pu / rad and the
 objective J within 1e-8 relative of the oracle.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -283,3 +285,189 @@ def test_realtime_updates_across_breaker_operations():
             V, th = e.state()
             assert np.abs(V - oV).max() < STATE_TOL and np.abs(th - oth).max() < STATE_TOL
     e.close()
+
+
+def test_c_host_lifecycle_on_gpu(tmp_path):
+    """The reference's lifecycle from a plain C host (float ABI, libsle.so):
+    both solves converge and the state matches the same calls made through
+    the FP64 ABI from the same starting point, to the float ABI's resolution.
+    The first telemetry is h(x) of the published IEEE 14-bus power-flow state.
+    The loader's measurement set leaves some angles unobservable (no active
+    power data around buses 10-14), so only the magnitudes are checked against
+    that state; both runs take the same path from the same start."""
+    import subprocess
+
+    from state_estimation_with_cuda_amd import sle
+    from tests.test_abi import build_lifecycle
+
+    # published IEEE 14-bus power-flow solution (magnitudes pu, angles degrees)
+    Vpf = np.array([1.060, 1.045, 1.010, 1.019, 1.020, 1.070, 1.062, 1.090, 1.056, 1.051, 1.057, 1.055, 1.050,
+                    1.036])
+    Tpf = np.radians([0.0, -4.98, -12.72, -10.33, -8.78, -14.22, -13.37, -13.37, -14.94, -15.10, -14.79, -15.07,
+                      -15.16, -16.04])
+    def fresh():
+        e = sle.Engine(double=True, initialize=False)
+        L, h = e.L, e.h
+        assert L.sle_LoadIEEE14Bus(h) == 0 and L.sle_Initialize(h) == 0 and L.sle_UploadModel(h) == 0
+        e.n, e.nb, e.m = 14, L.sle_GetBranchCount(h), L.sle_GetMeasurementCount(h)
+        return e
+
+    e = fresh()
+    nm = e.m
+    e.set_state(Vpf, Tpf)
+    z0 = e.evaluate()["h"].astype(np.float32).astype(np.float64)  # what the float ABI carries
+    e.close()
+    zfile = tmp_path / "z14.txt"
+    zfile.write_text("\n".join(repr(float(v)) for v in z0) + "\n")
+
+    exe = build_lifecycle(tmp_path, double=False)
+    out = subprocess.run([str(exe), str(zfile)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, (out.returncode, out.stderr)
+    lines = out.stdout.split("\n")
+    st = [ln for ln in lines if ln.startswith("status")][0].split()
+    assert st[1] == "0" and st[2] == "0", out.stdout
+    Vc = np.array([float(ln.split()[2]) for ln in lines if ln.startswith("V ")])
+    Tc = np.array([float(ln.split()[2]) for ln in lines if ln.startswith("T ")])
+    # the same calls through the FP64 ABI, from the same initial state
+    e = fresh()
+    L, S, h = e.L, e.S, e.h
+    assert L.sle_UpdateTelemetry(h, z0.ctypes.data_as(C.c_void_p), nm) == 0
+    res = S["Result"]()
+    assert L.sle_Solve(h, 1, C.byref(res)) == 0
+    V1, T1 = e.state()
+    assert np.abs(V1 - Vpf).max() < 1e-4  # every magnitude is measured
+    z = np.zeros(nm)
+    d = S["MeasurementDetails"]()
+    for i in range(nm):
+        mid = L.sle_GetMeasurementIdByIndex(h, i)
+        assert L.sle_GetMeasurementDetails(h, mid, C.byref(d)) == 0
+        z[i] = np.float32(np.float32(d.current_value) * np.float32(1.0 + 1e-4 * ((i % 7) - 3)))
+    assert L.sle_UpdateTelemetry(h, z.ctypes.data_as(C.c_void_p), nm) == 0
+    assert L.sle_Solve(h, 0, C.byref(res)) == 0
+    V = np.zeros(14)
+    T = np.zeros(14)
+    assert L.sle_GetVoltageMagnitudes(h, V.ctypes.data_as(C.c_void_p), 14) == 0
+    assert L.sle_GetVoltageAngles(h, T.ctypes.data_as(C.c_void_p), 14) == 0
+    assert np.abs(V - Vc).max() < 1e-4 and np.abs(T - Tc).max() < 1e-4
+    e.close()
+
+
+PLAN_OPTIONS = {
+    "default": {},
+    "dense_small_subtrees": {"pinv_cut": 8},
+    "dense_large_subtrees": {"pinv_cut": 128},
+    "pinv_lds_subtrees": {"pinv_dense": 0},
+    "pinv_lds_subtrees_flags": {"pinv_dense": 0, "lds_flags": 1},
+    "pinv_lds_small_subtrees": {"pinv_dense": 0, "pinv_cut": 8, "pinv_waves": 2, "lds_flags": 1},
+    "lds_tasks": {"pinv": 0},
+    "lds_tasks_flags": {"pinv": 0, "lds_flags": 1},
+    "task_sweeps": {"pinv": 0, "lds_solve": 0},
+    "levels": {"pinv": 0, "lds_solve": 0, "task_sweeps": 0},
+    "dependency_driven": {"level_schedule": 0},
+    "bus_ordered_rows": {"meas_by_type": 0, "xcd_remap": 0},
+}
+
+
+@pytest.mark.parametrize("plan", list(PLAN_OPTIONS))
+@pytest.mark.parametrize("fp64", [True, False])
+def test_solve_plans_agree(plan, fp64):
+    """Every schedule of the factorization and of the triangular solves gives
+    the same factor solve (against a known solution) and the oracle's optimum."""
+    g = GRIDS["geo10k"]()
+    o = _oracle_10k()
+    from state_estimation_with_cuda_amd import sle
+
+    e = sle.Engine(double=True)
+    e.load_grid(g)
+    e.set_config(use_double_precision=int(fp64))
+    for k, v in PLAN_OPTIONS[plan].items():
+        e.set_option(k, v)
+    e.upload()
+    e.set_telemetry_f64(g.z)
+    print(f"[plan] {plan} fp64={fp64}: {e.describe()}", flush=True)
+    V, th = random_state(g, seed=3)
+    e.set_state(V, th)
+    w = 1.0 / g.sigma**2
+    gv = o.gain(V, th, w)
+    cp, rw = o.g_pattern()
+    ns = o.ns
+    Gl = sp.csc_matrix((gv, rw, cp), shape=(ns, ns))
+    Gfull = (Gl + Gl.T - sp.diags(Gl.diagonal())).tocsr()
+    assert e.factor()[0] == 0
+    x = np.random.default_rng(5).normal(size=ns)
+    dx = e.solve_factor(Gfull @ x)
+    # FP32: forward error up to kappa(G) * eps_32 (kappa ~ 1e6 here); plans with
+    # more explicit inverse blocks sit nearer that bound; the Gauss-Newton loop
+    # below (FP64 residuals) must still reach the oracle's optimum
+    assert rel_err(dx, x) < (1e-8 if fp64 else 0.15)
+    r = e.solve(mode=1, max_iterations=50, tolerance=1e-6, flat_start=True)
+    assert r["status"] == 0, r
+    oV, oth = _oracle_10k_state()
+    V, th = e.state()
+    assert np.abs(V - oV).max() < STATE_TOL and np.abs(th - oth).max() < STATE_TOL
+    e.close()
+
+
+_O10K = {}
+
+
+def _oracle_10k():
+    if "o" not in _O10K:
+        _O10K["o"] = Oracle(GRIDS["geo10k"]())
+    return _O10K["o"]
+
+
+def _oracle_10k_state():
+    if "state" not in _O10K:
+        o = Oracle(GRIDS["geo10k"]())
+        o.solve(mode=1, max_iter=50, tol=1e-11, flat_start=True)
+        _O10K["state"] = o.state()
+    return _O10K["state"]
+
+
+def test_autotune_keeps_results():
+    """The autotuner times candidate plans on the current factor; the solve it
+    leaves behind still reaches the oracle's optimum."""
+    g = GRIDS["geo10k"]()
+    e = engine_for(g, double_precision=True)
+    r = e.solve(mode=1, max_iterations=50, tolerance=1e-6, flat_start=True)
+    assert r["status"] == 0
+    rep = e.autotune(1)
+    print(f"[autotune] {rep}", flush=True)
+    assert "autotune: solve cut=" in rep
+    e.set_state(*random_state(g, seed=9, spread=0.01))
+    r = e.solve(mode=1, max_iterations=50, tolerance=1e-6)
+    assert r["status"] == 0, r
+    oV, oth = _oracle_10k_state()
+    V, th = e.state()
+    assert np.abs(V - oV).max() < STATE_TOL and np.abs(th - oth).max() < STATE_TOL
+    e.close()
+
+
+@pytest.mark.parametrize("fp64", [True, False])
+def test_graph_replays_match_eager_iterations(fp64):
+    """REALTIME iterations on the cached factor replay a captured HIP graph
+    (config use_cuda_graphs): every replay, with eager work (telemetry updates,
+    state reads, a PRECISION solve that refactors) between the replays, gives
+    bit-identical states to the same requests run eagerly."""
+    g = GRIDS["geo10k"]()
+    eg = engine_for(g, double_precision=fp64, use_cuda_graphs=1)
+    ee = engine_for(g, double_precision=fp64, use_cuda_graphs=0)
+    rng = np.random.default_rng(21)
+    for e in (eg, ee):
+        r = e.solve(mode=1, max_iterations=50, tolerance=1e-6, flat_start=True)
+        assert r["status"] == 0
+    for step in range(6):
+        idx = np.sort(rng.choice(g.m, g.m // 10, replace=False)).astype(np.int32)
+        val = g.z[idx] * (1.0 + 1e-3 * rng.standard_normal(idx.size))
+        out = []
+        for e in (eg, ee):
+            e.set_telemetry_indexed(idx, val)
+            mode = 1 if step == 3 else 0  # one refactoring solve between the replays
+            r = e.solve(mode=mode, max_iterations=10, tolerance=1e-6)
+            out.append((r["status"], r["iterations"]) + e.state())
+        (sg, ig, Vg, tg), (se, ie, Ve, te) = out
+        assert (sg, ig) == (se, ie), (step, out[0][:2], out[1][:2])
+        assert np.array_equal(Vg, Ve) and np.array_equal(tg, te), step
+    eg.close()
+    ee.close()
