@@ -139,12 +139,24 @@ def roofline_for(form, kt, ab):
 
 
 def engine_options(args) -> dict:
-    """--set KEY=VALUE pairs (sle_ext_SetOption keys) applied to every engine."""
+    """--set KEY=VALUE pairs (sle_ext_SetOption keys, or use_cuda_graphs) applied to every engine."""
     out = {}
     for kv in args.set or []:
         k, _, v = kv.partition("=")
         out[k.strip()] = float(v)
     return out
+
+
+CONFIG_KEYS = ("use_cuda_graphs",)  # --set keys that are SLE_Config fields rather than engine options
+
+
+def configure(e, fp64, pcg_steps, options):
+    e.set_config(use_double_precision=int(fp64), enable_profiling=0,
+                 **{k: int(v) for k, v in (options or {}).items() if k in CONFIG_KEYS})
+    e.set_option("pcg_steps", 0 if fp64 else pcg_steps)
+    for k, v in (options or {}).items():
+        if k not in CONFIG_KEYS:
+            e.set_option(k, v)
 
 
 def run_form(g, fp64, reqs, batches, warmup, steps, profile_steps, pcg_steps=0, options=None, autotune=False):
@@ -154,10 +166,7 @@ def run_form(g, fp64, reqs, batches, warmup, steps, profile_steps, pcg_steps=0, 
     t0 = time.perf_counter()
     e = sle.Engine(double=True)
     e.load_grid(g)
-    e.set_config(use_double_precision=int(fp64), enable_profiling=0)
-    e.set_option("pcg_steps", 0 if fp64 else pcg_steps)
-    for k, v in (options or {}).items():
-        e.set_option(k, v)
+    configure(e, fp64, pcg_steps, options)
     e.upload()
     e.set_telemetry_f64(g.z)
     setup_s = time.perf_counter() - t0
@@ -253,10 +262,7 @@ def run_areas(args):
     def engine(grid):
         e = sle.Engine(double=True)
         e.load_grid(grid)
-        e.set_config(use_double_precision=int(fp64), enable_profiling=0)
-        e.set_option("pcg_steps", 0 if fp64 else args.pcg_steps)
-        for k, v in engine_options(args).items():
-            e.set_option(k, v)
+        configure(e, fp64, args.pcg_steps, engine_options(args))
         e.upload()
         e.set_telemetry_f64(grid.z)
         return e
@@ -372,10 +378,7 @@ def run_serve(args):
     def engine(grid):
         e = sle.Engine(double=True)
         e.load_grid(grid)
-        e.set_config(use_double_precision=int(fp64), enable_profiling=0)
-        e.set_option("pcg_steps", 0 if fp64 else args.pcg_steps)
-        for k, v in engine_options(args).items():
-            e.set_option(k, v)
+        configure(e, fp64, args.pcg_steps, engine_options(args))
         e.upload()
         e.set_telemetry_f64(grid.z)
         return e

@@ -219,7 +219,7 @@ typedef struct SLE_Config {
     SLE_Real realtime_time_limit_ms;
     int32_t use_flat_start_default;
     int32_t enable_profiling;
-    int32_t use_cuda_graphs; /* capture the per-iteration chain in a HIP graph */
+    int32_t use_cuda_graphs; /* replay iterations on the cached factor as a HIP graph */
     int32_t block_size_standard;
     int32_t block_size_reduction;
     int32_t use_double_precision; /* 1: plain FP64 form; 0: mixed FP32 factor + FP64 refinement */

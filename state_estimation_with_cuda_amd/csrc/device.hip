@@ -568,16 +568,40 @@ __device__ inline void factor_front(const FactArgs<T>& A, int s, T* F) {
         const int cc = A.sn_c[ch], kc = A.sn_k[ch], u = cc - kc;
         const T* Fc = A.F + A.foff[ch];
         const int32_t* rm = A.rmap + A.rmoff[ch];
-        for (int idx = tid; idx < u * u; idx += NT) {
-            int a = idx % u, b = idx / u;
+        // the u x u square walked NT entries at a time, (a, b) carried along
+        // without a division per entry; the lower triangle is added
+        int a = tid, b = 0;
+        while (a >= u && b < u) {
+            a -= u;
+            ++b;
+        }
+        while (b < u) {
             if (a >= b) F[rm[b] * c + rm[a]] += Fc[(int64_t)(kc + b) * cc + kc + a];
+            a += NT;
+            while (a >= u && b < u) {
+                a -= u;
+                ++b;
+            }
         }
         Tm::sync();
     }
+    // the pivot thresholds' G_jj, loaded together before the pivot loop (one
+    // lane per pivot) instead of one dependent load pair per pivot
+    const bool pre = NT == kWave && k <= kWave;
+    T gdl = T(0);
+    if (pre && tid < k) {
+        const int ge = A.gdiag[first + tid];
+        gdl = ge >= 0 ? A.Gval[ge] : T(0);
+    }
     for (int j = 0; j < k; ++j) {
         T dd = F[j * c + j];
-        const int ge = A.gdiag[first + j];
-        const T gd = ge >= 0 ? A.Gval[ge] : T(0);
+        T gd;
+        if (pre) {
+            gd = __shfl(gdl, j, kWave);
+        } else {
+            const int ge = A.gdiag[first + j];
+            gd = ge >= 0 ? A.Gval[ge] : T(0);
+        }
         Tm::sync();
         if (!(dd > A.piv_tol * gd) || !(dd > T(0))) {
             // below threshold: flag it and substitute threshold * G_jj, so the
@@ -589,13 +613,11 @@ __device__ inline void factor_front(const FactArgs<T>& A, int s, T* F) {
         if (tid == 0) F[j * c + j] = dj;
         for (int i = j + 1 + tid; i < c; i += NT) F[j * c + i] = F[j * c + i] / dj;
         Tm::sync();
-        const int w = c - j - 1;
-        for (int idx = tid; idx < w * w; idx += NT) {
-            int a = idx % w, b = idx / w;
-            if (a >= b) {
-                int aa = j + 1 + a, bb = j + 1 + b;
-                F[bb * c + aa] -= F[j * c + aa] * F[j * c + bb];
-            }
+        // rank-1 update of the trailing lower triangle: each thread owns rows
+        // aa and walks their columns bb <= aa (column j read once per row)
+        for (int aa = j + 1 + tid; aa < c; aa += NT) {
+            const T la = F[j * c + aa];
+            for (int bb = j + 1; bb <= aa; ++bb) F[bb * c + aa] -= la * F[j * c + bb];
         }
         Tm::sync();
     }
@@ -695,10 +717,12 @@ __device__ inline void bwd_front(const SolveArgs<T>& A, int s, T* xs) {
 // step x_s = M^T [y_s; x_rows]: one parallel product instead of k dependent
 // pivot steps.  One wave per front; lane jj builds column jj.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_front_m(int nsn, SolveArgs<T> A, T* __restrict__ M) {
-    const int s = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+__global__ __launch_bounds__(kBlock) void k_front_m(int nsn, SolveArgs<T> A, T* __restrict__ M,
+                                                    const int32_t* __restrict__ list) {
+    const int q = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
     const int lane = threadIdx.x & (kWave - 1);
-    if (s >= nsn) return;  // uniform per wave
+    if (q >= nsn) return;  // uniform per wave
+    const int s = list ? list[q] : q;
     const int c = A.sn_c[s], k = A.sn_k[s];
     const T* L = A.F + A.foff[s];
     T* Ms = M + A.moff[s];
@@ -1117,17 +1141,21 @@ __global__ __launch_bounds__(kBlock) void k_pi_assemble(int ntc, const int32_t* 
     ctop[t] = acc;
 }
 
-constexpr int kPiSegMax = 256;  // entries per segment of the forward product (a multiple of 64)
+constexpr int kPiSegMax = 1024;  // entries per segment of the forward product (a multiple of 64)
 
-// Forward product y_T = L_TT^-1 c_T: one wave per segment of a row's entries.
+// Forward product y_T = L_TT^-1 c_T: one wave per segment of a row's entries,
+// all sixteen loads of a lane issued together.  A row held by one segment
+// (most rows) is written directly; longer rows leave partial sums for
+// k_pi_rowsum.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pi_fwd(int nseg, const int64_t* __restrict__ segq,
-                                                  const int32_t* __restrict__ src, const T* __restrict__ val,
-                                                  const T* __restrict__ ctop, double* __restrict__ part) {
+                                                  const int32_t* __restrict__ segrow,
+                                                  const int32_t* __restrict__ segptr, const int32_t* __restrict__ src,
+                                                  const T* __restrict__ val, const T* __restrict__ ctop,
+                                                  double* __restrict__ part, T* __restrict__ ytop) {
     const int g = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
     const int lane = threadIdx.x & (kWave - 1);
     if (g >= nseg) return;
-    // a segment holds at most kPiSeg entries: all loads of a lane issued at once
     constexpr int kU = kPiSegMax / kWave;
     const int64_t q0 = segq[g] + lane, q1 = segq[g + 1];
     double v[kU];
@@ -1142,16 +1170,24 @@ __global__ __launch_bounds__(kBlock) void k_pi_fwd(int nseg, const int64_t* __re
 #pragma unroll
     for (int u = 0; u < kU; ++u) acc += c[u] >= 0 ? v[u] * (double)ctop[c[u]] : 0.0;
     acc = wave_sum(acc);
-    if (lane == 0) part[g] = acc;
+    if (lane == 0) {
+        const int t = segrow[g];
+        if (segptr[t + 1] - segptr[t] == 1)
+            ytop[t] = (T)acc;
+        else
+            part[g] = acc;
+    }
 }
 
-// One wave per top column (rows near the root have hundreds of segments).
+// Rows spread over several segments: one wave per such row.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pi_rowsum(int ntc, const int32_t* __restrict__ segptr,
+__global__ __launch_bounds__(kBlock) void k_pi_rowsum(int nrows, const int32_t* __restrict__ rows,
+                                                     const int32_t* __restrict__ segptr,
                                                      const double* __restrict__ part, T* __restrict__ ytop) {
-    const int t = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    const int q = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
     const int lane = threadIdx.x & (kWave - 1);
-    if (t >= ntc) return;  // uniform per wave
+    if (q >= nrows) return;  // uniform per wave
+    const int t = rows[q];
     double acc = 0.0;
     for (int g = segptr[t] + lane; g < segptr[t + 1]; g += kWave) acc += part[g];
     acc = wave_sum(acc);
@@ -1188,6 +1224,39 @@ __global__ __launch_bounds__(kBlock) void k_pi_bwd(PiArgs A, SolveArgs<T> SA, co
     if (lane == 0) SA.x[col] = (T)acc;
 }
 
+// Backward product by top supernode: one wave reads the path's y_T once and
+// carries all k_s columns of the block (k_s <= kPiBwdK) with every load of a
+// 64-row chunk in flight together.
+constexpr int kPiBwdK = 24;
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pi_bwd_sn(PiArgs A, SolveArgs<T> SA, const T* __restrict__ B,
+                                                     const T* __restrict__ ytop) {
+    const int u = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    if (u >= A.ntop) return;  // uniform per wave
+    const int s = A.sn[u], ks = SA.sn_k[s], Ps = A.plen[u];
+    const T* Bu = B + A.boff[u];
+    const int32_t* pl = A.ploc + A.poff[u];
+    double acc[kPiBwdK];
+#pragma unroll
+    for (int jj = 0; jj < kPiBwdK; ++jj) acc[jj] = 0.0;
+    for (int r0 = 0; r0 < Ps; r0 += kWave) {
+        const int r = r0 + lane;
+        const double yv = r < Ps ? (double)ytop[pl[r]] : 0.0;
+#pragma unroll
+        for (int jj = 0; jj < kPiBwdK; ++jj)
+            if (jj < ks && r < Ps && r >= jj) acc[jj] += (double)Bu[r + (int64_t)Ps * jj] * yv;
+    }
+    const int first = SA.sn_first[s];
+#pragma unroll
+    for (int jj = 0; jj < kPiBwdK; ++jj)
+        if (jj < ks) {
+            const double v = wave_sum(acc[jj]);
+            if (lane == 0) SA.x[first + jj] = (T)v;
+        }
+}
+
 // ---------------------------------------------------------------------------
 // Dense solve matrices of the bottom subtrees.  A bottom subtree t with n
 // columns (its fronts' columns, children first) and u update rows (the rows of
@@ -1201,6 +1270,7 @@ __global__ __launch_bounds__(kBlock) void k_pi_bwd(PiArgs A, SolveArgs<T> SA, co
 // column j holds rows j..R-1), n <= 64, R <= 128.
 // ---------------------------------------------------------------------------
 constexpr int kSubCols = 64, kSubRows = 128;
+constexpr int kSubBins = 4;  // launch groups of the subtree-matrix kernels
 
 struct SubArgs {
     int nsub;
@@ -1268,7 +1338,65 @@ __global__ __launch_bounds__(kBlock) void k_sub_m(SubArgs B, SolveArgs<T> A, con
     }
 }
 
-constexpr int kSubBatch = 8;  // independent loads in flight per lane
+// The same M_t with one wave per subtree and all columns at once: lane j
+// carries column j through the row-oriented substitution
+//   X[i][j] = (delta_ij - sum_{k<i} L[i][k] X[k][j]) / L[i][i],   i < n,
+//   M[n + r][j] = -sum_{k<n} L[n + r][k] X[k][j],
+// whose inner sums are independent LDS loads (broadcast L, conflict-free X)
+// rather than a chain of cross-lane steps.  LDS: the dense L (R x n) and X
+// (n x n), both FP64.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sub_m_rows(SubArgs B, SolveArgs<T> A, const int32_t* __restrict__ list,
+                                                      T* __restrict__ Mout) {
+    extern __shared__ double sm[];
+    const int t = list[blockIdx.x];
+    const int n = B.n[t], R = B.R[t];
+    const int nw = kBlock / kWave, w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    double* Ld = sm;         // R x n, column-major
+    double* X = sm + R * n;  // X[k * n + j]
+    for (int e = threadIdx.x; e < R * n; e += kBlock) Ld[e] = 0.0;
+    __syncthreads();
+    // the gather spreads the fronts over the waves (its global loads dominate)
+    for (int f = B.fptr[t] + w; f < B.fptr[t + 1]; f += nw) {
+        const int s = B.fsn[f], c = A.sn_c[s], k = A.sn_k[s], dc = B.fdcol[f];
+        const T* L = A.F + A.foff[s];
+        const int32_t* dr = B.drow + B.fdoff[f];
+        for (int i = lane; i < c; i += kWave) {
+            const int r = dr[i];
+            const int jmax = i < k ? i + 1 : k;
+            for (int jj = 0; jj < jmax; ++jj) Ld[(dc + jj) * R + r] = (double)L[jj * c + i];
+        }
+    }
+    __syncthreads();
+    if (w != 0) return;  // one wave carries all columns through the substitution
+    T* Mt = Mout + B.moff[t];
+    T* Mr = Mout + B.msub + B.moff[t];
+    const int j = lane;
+    const int jc = j < n ? j : 0;  // lanes beyond the columns read column 0 and discard it
+    for (int i = 0; i < R; ++i) {
+        const int kend = i < n ? i : n;
+        // eight independent partial sums keep sixteen LDS loads in flight
+        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        int k = 0;
+        for (; k + 8 <= kend; k += 8) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc[u] -= Ld[(k + u) * R + i] * X[(k + u) * n + jc];
+        }
+        for (; k < kend; ++k) acc[0] -= Ld[k * R + i] * X[k * n + jc];
+        double v = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+        if (i == j) v += 1.0;
+        if (i < n) {
+            v = j <= i ? v / Ld[i * R + i] : 0.0;
+            if (j < n) X[i * n + j] = v;
+        }
+        if (j < n && j <= i) {
+            Mt[sub_col_off(j, R) + i - j] = (T)v;
+            Mr[sub_row_off(i, n) + j] = (T)v;
+        }
+    }
+}
+
+constexpr int kSubBatch = 16;  // independent loads in flight per lane
 
 // Forward: one wave per subtree; lane i accumulates rows i and i + 64 over the
 // columns of M_t, kSubBatch coalesced column loads at a time.
@@ -1964,16 +2092,17 @@ struct PiPlan {
     int cut = 0;
     int ntop = 0, ntc = 0, nseg = 0, slice = 0;
     int64_t nblock = 0, nq = 0;
-    std::vector<int32_t> sn, plen, ploc, tcol, colsn, src, segptr, cptr, parent;
+    std::vector<int32_t> sn, plen, ploc, tcol, colsn, src, segptr, cptr, parent, segrow, mrows;
     std::vector<int64_t> boff, poff, pos, segq, cpos;
+    int kmax = 0;  // largest k over the top supernodes
     // dense bottom subtrees (see SubArgs)
     bool dense = false;
     int nsub = 0;
     int64_t msub = 0;
     std::vector<int32_t> sroot, sn_n, sR, fptr, fsn, fdcol, drow, scptr, scols, bins;
     std::vector<int64_t> fdoff, smoff;
-    int bin_off[4] = {0, 0, 0, 0};
-    size_t bin_bytes[3] = {0, 0, 0};
+    int bin_off[kSubBins + 1] = {0, 0, 0, 0, 0};
+    size_t bin_bytes[kSubBins] = {0, 0, 0, 0};
 };
 
 // Bottom subtrees as dense solve matrices: for every subtree root (a front
@@ -2030,20 +2159,23 @@ static void make_sub_plan(const Symbolic& S, const std::vector<int32_t>& tsl, Pi
         P.scptr.push_back((int32_t)P.scols.size());
     }
     P.nsub = (int)P.sroot.size();
-    // k_sub_m launches grouped by the LDS the dense L needs
+    // launches of the subtree-matrix kernels grouped by the LDS they need:
+    // k_sub_m_rows (dense L and X) in bins 0-2, k_sub_m (dense L only) for the
+    // few subtrees whose L and X together pass 64 KiB
     const size_t lim[3] = {16384, 32768, 65536};
-    std::vector<int32_t> bin[3];
+    std::vector<int32_t> bin[kSubBins];
     for (int t = 0; t < P.nsub; ++t) {
-        const size_t b = (size_t)P.sn_n[t] * P.sR[t] * sizeof(double);
-        const int q = b <= lim[0] ? 0 : (b <= lim[1] ? 1 : 2);
+        const size_t n = P.sn_n[t], R = P.sR[t];
+        const size_t b = (R * n + n * n) * sizeof(double);
+        const int q = b <= lim[0] ? 0 : (b <= lim[1] ? 1 : (b <= lim[2] ? 2 : 3));
         bin[q].push_back(t);
-        P.bin_bytes[q] = std::max(P.bin_bytes[q], b);
+        P.bin_bytes[q] = std::max(P.bin_bytes[q], q < 3 ? b : R * n * sizeof(double));
     }
-    for (int q = 0; q < 3; ++q) {
+    for (int q = 0; q < kSubBins; ++q) {
         P.bin_off[q] = (int)P.bins.size();
         P.bins.insert(P.bins.end(), bin[q].begin(), bin[q].end());
     }
-    P.bin_off[3] = (int)P.bins.size();
+    P.bin_off[kSubBins] = (int)P.bins.size();
     P.dense = true;
 }
 
@@ -2083,6 +2215,7 @@ static PiPlan make_pi_plan(const Symbolic& S, int cut, bool dense) {
     P.ntop = (int)P.sn.size();
     P.ntc = (int)P.tcol.size();
     P.slice = cmax * kmax;
+    P.kmax = kmax;
     P.parent = S.sn_parent;
     // blocks and path rows
     P.plen.resize(P.ntop);
@@ -2122,8 +2255,12 @@ static PiPlan make_pi_plan(const Symbolic& S, int cut, bool dense) {
     }
     P.segptr.assign(P.ntc + 1, 0);
     for (int t = 0; t < P.ntc; ++t) {
-        for (int64_t q = rcnt[t]; q < rcnt[t + 1]; q += kPiSeg) P.segq.push_back(q);
+        for (int64_t q = rcnt[t]; q < rcnt[t + 1]; q += kPiSeg) {
+            P.segq.push_back(q);
+            P.segrow.push_back(t);
+        }
         P.segptr[t + 1] = (int32_t)P.segq.size();
+        if (P.segptr[t + 1] - P.segptr[t] != 1) P.mrows.push_back(t);  // several segments (or none)
     }
     P.nseg = (int)P.segq.size();
     P.segq.push_back(P.nq);
@@ -2235,8 +2372,9 @@ struct Instance::Impl {
         bool ok = false;
         int cut = 0, ntop = 0, ntc = 0, nseg = 0, slice = 0;
         int64_t nblock = 0, nq = 0;
-        DBuf<int32_t> sn, plen, ploc, tcol, colsn, src, segptr, cptr, parent;
+        DBuf<int32_t> sn, plen, ploc, tcol, colsn, src, segptr, cptr, parent, segrow, mrows;
         DBuf<int64_t> boff, poff, pos, segq, cpos;
+        int nmrows = 0, kmax = 0;
         DBuf<double> part;
         LdsDev bottom[2];  // bottom-subtree LDS tasks, [0] float, [1] double
         // dense bottom subtrees
@@ -2245,8 +2383,8 @@ struct Instance::Impl {
         int64_t msub = 0;
         DBuf<int32_t> sroot, sn_n, sR, fptr, fsn, fdcol, drow, scptr, scols, bins;
         DBuf<int64_t> fdoff, smoff;
-        int bin_off[4] = {0, 0, 0, 0};
-        size_t bin_bytes[3] = {0, 0, 0};
+        int bin_off[kSubBins + 1] = {0, 0, 0, 0, 0};
+        size_t bin_bytes[kSubBins] = {0, 0, 0, 0};
         PiSizes sizes() const {
             PiSizes z;
             if (ok) {
@@ -2370,6 +2508,10 @@ static void build_pi(Instance::Impl& D, const Symbolic& S, size_t& bytes, hipStr
             up(Q.colsn, PP.colsn);
             up(Q.src, PP.src);
             up(Q.segptr, PP.segptr);
+            up(Q.segrow, PP.segrow);
+            up(Q.mrows, PP.mrows);
+            Q.nmrows = (int)PP.mrows.size();
+            Q.kmax = PP.kmax;
             up(Q.cptr, PP.cptr);
             up(Q.parent, PP.parent);
             up64(Q.boff, PP.boff);
@@ -2401,8 +2543,8 @@ static void build_pi(Instance::Impl& D, const Symbolic& S, size_t& bytes, hipStr
                 up64(Q.smoff, PP.smoff);
                 Q.nsub = PP.nsub;
                 Q.msub = PP.msub;
-                for (int q = 0; q < 4; ++q) Q.bin_off[q] = PP.bin_off[q];
-                for (int q = 0; q < 3; ++q) Q.bin_bytes[q] = PP.bin_bytes[q];
+                for (int q = 0; q <= kSubBins; ++q) Q.bin_off[q] = PP.bin_off[q];
+                for (int q = 0; q < kSubBins; ++q) Q.bin_bytes[q] = PP.bin_bytes[q];
             }
             HIPCHECK(hipStreamSynchronize(s));
             Q.cut = cut;
@@ -2462,6 +2604,7 @@ Instance::~Instance() {
     {
         auto& P = d_->pi;
         rel(P.sn); rel(P.plen); rel(P.ploc); rel(P.tcol); rel(P.colsn); rel(P.src); rel(P.segptr); rel(P.cptr); rel(P.parent);
+        rel(P.segrow); rel(P.mrows);
         rel(P.boff); rel(P.poff); rel(P.pos); rel(P.segq); rel(P.cpos); rel(P.part);
         rel(P.sroot); rel(P.sn_n); rel(P.sR); rel(P.fptr); rel(P.fsn); rel(P.fdcol); rel(P.drow); rel(P.scptr);
         rel(P.scols); rel(P.bins); rel(P.fdoff); rel(P.smoff);
@@ -3199,9 +3342,13 @@ void launch_pi_inverse(Instance::Impl& D) {
     Instance::Impl::PiDev& PI = D.pi;
     PrecBufs<T>& B = bufs<T>(D);
     if (sub_active(D))
-        for (int q = 0; q < 3; ++q) {
+        for (int q = 0; q < kSubBins; ++q) {
             const int cnt = PI.bin_off[q + 1] - PI.bin_off[q];
-            if (cnt)
+            if (!cnt) continue;
+            if (q < kSubBins - 1)
+                k_sub_m_rows<T><<<cnt, kBlock, PI.bin_bytes[q], D.stream>>>(sub_args(D), solve_args<T>(D),
+                                                                           PI.bins.p + PI.bin_off[q], B.msub.p);
+            else
                 k_sub_m<T><<<cnt, kBlock, PI.bin_bytes[q], D.stream>>>(sub_args(D), solve_args<T>(D),
                                                                       PI.bins.p + PI.bin_off[q], B.msub.p);
         }
@@ -3214,14 +3361,25 @@ void launch_pi_inverse(Instance::Impl& D) {
                                                                                        B.pinv_rows.p);
 }
 
+// Everything the solves need from a factor: per-front solve matrices, the
+// dense subtree matrices and the top inverse blocks (after a factorization,
+// or after the solve plan changed on the same factor).
+template <typename T>
+void launch_solve_prep(Instance::Impl& D) {
+    // the LDS task solves (every front fits) read M of every front; with dense
+    // bottom subtrees only the top inverse blocks need M (of the top fronts)
+    const bool top_only = sub_active(D);
+    const int nsn = top_only ? D.pi.ntop : (int)D.sn_first.n;
+    if (nsn && D.max_c <= kFrontB)
+        k_front_m<T><<<(nsn + kBlock / kWave - 1) / (kBlock / kWave), kBlock, 0, D.stream>>>(
+            nsn, solve_args<T>(D), bufs<T>(D).M.p, top_only ? D.pi.sn.p : nullptr);
+    launch_pi_inverse<T>(D);
+}
+
 template <typename T>
 void launch_factor(Instance::Impl& D, T piv_tol) {
     launch_factor_sweeps<T>(D, piv_tol);
-    const int nsn = (int)D.sn_first.n;
-    if (nsn && D.max_c <= kFrontB)  // the LDS task solves (every front fits) read M
-        k_front_m<T><<<(nsn + kBlock / kWave - 1) / (kBlock / kWave), kBlock, 0, D.stream>>>(nsn, solve_args<T>(D),
-                                                                                          bufs<T>(D).M.p);
-    launch_pi_inverse<T>(D);
+    launch_solve_prep<T>(D);
 }
 
 struct Prof;
@@ -3265,14 +3423,20 @@ void launch_trisolve(Instance::Impl& D, Prof* pf = nullptr) {
         if (PI.ntc) {
             k_pi_assemble<T><<<nblk(PI.ntc), kBlock, 0, D.stream>>>(PI.ntc, PI.tcol.p, PI.cptr.p, PI.cpos.p, A.x,
                                                                    A.ubuf, B.ctop.p);
-            k_pi_fwd<T><<<(PI.nseg + nw - 1) / nw, kBlock, 0, D.stream>>>(PI.nseg, PI.segq.p, PI.src.p,
-                                                                           B.pinv_rows.p, B.ctop.p, PI.part.p);
-            k_pi_rowsum<T><<<(PI.ntc + nw - 1) / nw, kBlock, 0, D.stream>>>(PI.ntc, PI.segptr.p, PI.part.p, B.ytop.p);
+            k_pi_fwd<T><<<(PI.nseg + nw - 1) / nw, kBlock, 0, D.stream>>>(
+                PI.nseg, PI.segq.p, PI.segrow.p, PI.segptr.p, PI.src.p, B.pinv_rows.p, B.ctop.p, PI.part.p, B.ytop.p);
+            if (PI.nmrows)
+                k_pi_rowsum<T><<<(PI.nmrows + nw - 1) / nw, kBlock, 0, D.stream>>>(PI.nmrows, PI.mrows.p,
+                                                                                   PI.segptr.p, PI.part.p, B.ytop.p);
         }
-        prof_end(pf, PH_FWD, (dense || PB.n_bottom ? 1 : 0) + (PI.ntc ? 3 : 0));
+        prof_end(pf, PH_FWD, (dense || PB.n_bottom ? 1 : 0) + (PI.ntc ? 2 + (PI.nmrows ? 1 : 0) : 0));
         prof_begin(pf, PH_BWD);
-        if (PI.ntc)
-            k_pi_bwd<T><<<(PI.ntc + nw - 1) / nw, kBlock, 0, D.stream>>>(pi_args(D), A, B.pinv.p, B.ytop.p);
+        if (PI.ntc) {
+            if (PI.kmax <= kPiBwdK)
+                k_pi_bwd_sn<T><<<(PI.ntop + nw - 1) / nw, kBlock, 0, D.stream>>>(pi_args(D), A, B.pinv.p, B.ytop.p);
+            else
+                k_pi_bwd<T><<<(PI.ntc + nw - 1) / nw, kBlock, 0, D.stream>>>(pi_args(D), A, B.pinv.p, B.ytop.p);
+        }
         if (dense)
             k_sub_bwd<T><<<(PI.nsub + nw - 1) / nw, kBlock, 0, D.stream>>>(sub_args(D), A, B.msub.p + PI.msub);
         else if (PB.n_bottom)
@@ -3418,7 +3582,7 @@ void launch_pcg(Instance::Impl& D, int steps) {
 int factor_launches(const Instance::Impl& D) {
     int inv = pi_active(D) && D.pi.ntop ? 2 : 0;
     if (sub_active(D))
-        for (int q = 0; q < 3; ++q) inv += D.pi.bin_off[q + 1] > D.pi.bin_off[q] ? 1 : 0;
+        for (int q = 0; q < kSubBins; ++q) inv += D.pi.bin_off[q + 1] > D.pi.bin_off[q] ? 1 : 0;
     if (D.use_flow) return 1 + inv;
     if (D.use_tasks && D.tasks_ok) return (D.n_bottom ? 1 : 0) + (int)D.mid_off.size() + (D.has_tail ? 1 : 0) + inv;
     return D.level_launches + inv;
@@ -3502,8 +3666,8 @@ void iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p, bool 
 }
 
 // One iteration on the cached factor (no refactor, direct solve) as a HIP
-// graph: the launches of iteration() and the status read-back are captured
-// once per variant and replayed with one launch; the host then waits once.
+// graph: the launches of iteration() are captured once per variant and
+// replayed with one launch; the status read-back follows it on the stream.
 // The captured work holds no memset node and no host-varying argument (the
 // dependency-driven sweeps, whose epochs change per launch, never come here).
 template <typename T>
@@ -3529,7 +3693,6 @@ void graph_iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p,
         HIPCHECK(hipStreamBeginCapture(D.stream, hipStreamCaptureModeThreadLocal));
         try {
             iteration<T>(D, S, p, false, piv_tol, off, newton);
-            HIPCHECK(hipMemcpyAsync(D.hst, D.st.p, sizeof(DevStatus), hipMemcpyDeviceToHost, D.stream));
         } catch (...) {
             (void)hipStreamEndCapture(D.stream, &graph);
             if (graph) (void)hipGraphDestroy(graph);
@@ -3546,7 +3709,7 @@ void graph_iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p,
         D.graphs.push_back({ae, pe, prec, p.robust, (int)newton, p.gamma, p.vmin, p.vmax, exec});
     }
     HIPCHECK(hipGraphLaunch(exec, D.stream));
-    HIPCHECK(hipStreamSynchronize(D.stream));
+    read_status(D);
 }
 
 }  // namespace
@@ -3599,7 +3762,7 @@ SolveOutcome Instance::solve(const SolveParams& p) {
                     iteration<float>(D, S, p, refactor, 1e-5f, pf, newton);
             }
             HIPCHECK(hipGetLastError());
-            if (!graph) read_status(D);
+            if (!graph) read_status(D);  // graph_iteration reads it itself
             pf.collect(ktimes_);
             if (D.hst->piv && newton) {
                 // the gain with reduced outlier weights lost rank: IRLS weights from here on
@@ -3988,7 +4151,7 @@ template <typename T>
 double time_solves(Instance::Impl& D, size_t& bytes, int reps) {
     PrecBufs<T>& B = bufs<T>(D);
     B.alloc_pi(D.pi.sizes(), bytes);
-    launch_pi_inverse<T>(D);
+    launch_solve_prep<T>(D);
     DBuf<T> save;
     size_t tmp = 0;
     save.alloc(B.x.n, tmp);
