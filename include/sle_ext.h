@@ -61,6 +61,10 @@ typedef struct SLE_ExtResult {
  *   "meas_by_type"    1 (default): the device keeps the measurement rows grouped
  *                     by evaluation class (bus values, injections, flows), each
  *                     class by anchor bus; 0: by anchor bus only
+ *   "rhs_by_location" 1: iterations on a cached factor form H^T W r from the
+ *                     summed weighted residuals of each bus and branch end,
+ *                     gathered per state (no per-entry products); needs
+ *                     "meas_by_type"; 0 (default): per Jacobian entry
  *   "lds_flags"       0 (default): the fronts of a subtree task are solved
  *                     level by level with a workgroup barrier per level;
  *                     1: each wave waits on LDS flags of just the fronts it

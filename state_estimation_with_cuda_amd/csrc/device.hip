@@ -28,6 +28,9 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <map>
+#include <mutex>
+#include <vector>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -451,6 +454,197 @@ __global__ __launch_bounds__(kBlock) void k_meas(MeasArgs A, double* __restrict_
     }
 }
 
+// ---------------------------------------------------------------------------
+// Right-hand side H^T W r by location, for the iterations on a cached factor
+// (no Jacobian values needed).  Every H entry of an injection row is a value
+// of its bus's Ybus row (k_power's per-entry terms), and every entry of a flow
+// row is one of four partials of its branch end; so the rows of one location
+// collapse to their summed weighted residuals, and each state gathers its RHS
+// from the locations it touches:
+//   k_meas_loc: one thread per bus (its V, theta and injection rows) and per
+//               branch (the flow rows at both ends: one sincos per branch),
+//               giving per bus the sums of w r by kind and per branch the
+//               summed w r d for (theta_from, theta_to, V_from, V_to);
+//   k_rhs_loc:  one thread per bus j, over its Ybus row (through the
+//               transpose entry, the partial of the neighbour's injection)
+//               and its incident branches, writing both of its states.
+// The measurement rows must be grouped by location (device order with
+// measurement classes): V/theta rows by bus, injection rows by bus, flow rows
+// by branch end.  Same weights as k_meas (IRLS / Huber), different summation
+// order.
+// ---------------------------------------------------------------------------
+struct LocArgs {
+    int n, nb;
+    int robust;
+    double gamma;
+    const uint8_t* mcode;
+    const double* z;
+    const double* sigma;
+    const int32_t* bv_ptr;  // per device bus: its V / theta rows
+    const int32_t* bi_ptr;  // per device bus: its injection rows
+    const int32_t* br_ptr;  // per device branch end (2k + end): its flow rows
+    const double* P;
+    const double* Q;
+    const double* V;
+    const double* th;
+    const int32_t* from;
+    const int32_t* to;
+    const double* gs;
+    const double* bs;
+    const double* bc;
+    const double* tap;
+    const double* shift;
+    double* sbus;  // per bus: sum of w r over its V, theta, P, Q rows
+    double* sbr;   // per branch: sum of w r d over its flow rows, for theta_f, theta_t, V_f, V_t
+};
+
+__device__ inline double loc_wr(const LocArgs& A, int i, double hv) {
+    if (!((A.mcode[i] >> 5) & 1)) return 0.0;  // inactive: weight 0
+    const double r = A.z[i] - hv;
+    const double sg = A.sigma[i];
+    double w = 1.0 / (sg * sg);
+    if (A.robust) {
+        const double ar = fabs(r);
+        if (ar / sg > A.gamma) w = A.gamma / (sg * ar);
+    }
+    return w * r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_meas_loc(LocArgs A) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t < A.n) {
+        const int b = t;
+        double sV = 0.0, sT = 0.0, sP = 0.0, sQ = 0.0;
+        for (int i = A.bv_ptr[b]; i < A.bv_ptr[b + 1]; ++i) {
+            if ((A.mcode[i] & 15) == 0)
+                sV += loc_wr(A, i, A.V[b]);
+            else
+                sT += loc_wr(A, i, A.th[b]);
+        }
+        for (int i = A.bi_ptr[b]; i < A.bi_ptr[b + 1]; ++i) {
+            const int ty = A.mcode[i] & 15;
+            if (ty == 2 || ty == 7)
+                sP += loc_wr(A, i, A.P[b]);
+            else
+                sQ += loc_wr(A, i, A.Q[b]);
+        }
+        double* o = A.sbus + 4 * (int64_t)b;
+        o[0] = sV;
+        o[1] = sT;
+        o[2] = sP;
+        o[3] = sQ;
+    } else if (t < A.n + A.nb) {
+        const int l = t - A.n;
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        if (A.br_ptr[2 * l + 2] > A.br_ptr[2 * l]) {
+            const int fb = A.from[l], tb = A.to[l];
+            const double g = A.gs[l], b = A.bs[l], a = A.tap[l], bsh = b + 0.5 * A.bc[l];
+            double s, c;
+            sincos(A.th[fb] - A.th[tb] - A.shift[l], &s, &c);
+            const double Vi = A.V[fb], Vj = A.V[tb], AA = Vi * Vj / a;
+            for (int end = 0; end < 2; ++end) {
+                const int i0 = A.br_ptr[2 * l + end], i1 = A.br_ptr[2 * l + end + 1];
+                if (i1 == i0) continue;
+                double p, q, dP[4], dQ[4];
+                if (end == 0) {
+                    p = Vi * Vi * g / (a * a) - AA * (g * c + b * s);
+                    q = -Vi * Vi * bsh / (a * a) - AA * (g * s - b * c);
+                    dP[0] = AA * (g * s - b * c);
+                    dP[1] = -dP[0];
+                    dP[2] = 2.0 * Vi * g / (a * a) - (Vj / a) * (g * c + b * s);
+                    dP[3] = -(Vi / a) * (g * c + b * s);
+                    dQ[0] = -AA * (g * c + b * s);
+                    dQ[1] = -dQ[0];
+                    dQ[2] = -2.0 * Vi * bsh / (a * a) - (Vj / a) * (g * s - b * c);
+                    dQ[3] = -(Vi / a) * (g * s - b * c);
+                } else {
+                    p = Vj * Vj * g - AA * (g * c - b * s);
+                    q = -Vj * Vj * bsh + AA * (g * s + b * c);
+                    dP[0] = AA * (g * s + b * c);
+                    dP[1] = -dP[0];
+                    dP[2] = -(Vj / a) * (g * c - b * s);
+                    dP[3] = 2.0 * Vj * g - (Vi / a) * (g * c - b * s);
+                    dQ[0] = AA * (g * c - b * s);
+                    dQ[1] = -dQ[0];
+                    dQ[2] = (Vj / a) * (g * s + b * c);
+                    dQ[3] = -2.0 * Vj * bsh + (Vi / a) * (g * s + b * c);
+                }
+                // the current-magnitude partials, formed only if such a row exists
+                const double ve = end ? Vj : Vi;
+                const double S = sqrt(p * p + q * q);
+                const double inv = S > 1e-12 ? 1.0 / (S * ve) : 0.0;
+                for (int i = i0; i < i1; ++i) {
+                    const int ty = A.mcode[i] & 15;
+                    if (ty == 4) {
+                        const double wr = loc_wr(A, i, p);
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) acc[k] += wr * dP[k];
+                    } else if (ty == 5) {
+                        const double wr = loc_wr(A, i, q);
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) acc[k] += wr * dQ[k];
+                    } else {  // I_MAG = |S| / V_end
+                        const double wr = loc_wr(A, i, S / ve);
+                        double d[4];
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) d[k] = (p * dP[k] + q * dQ[k]) * inv;
+                        d[end ? 3 : 2] -= S / (ve * ve);
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) acc[k] += wr * d[k];
+                    }
+                }
+            }
+        }
+        double* o = A.sbr + 4 * (int64_t)l;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) o[k] = acc[k];
+    }
+}
+
+template <typename TO>
+__global__ __launch_bounds__(kBlock) void k_rhs_loc(int n, const int32_t* __restrict__ yrow,
+                                                    const int32_t* __restrict__ ycol, const int32_t* __restrict__ ytr,
+                                                    const double* __restrict__ yG, const double* __restrict__ yB,
+                                                    const double* __restrict__ Pij, const double* __restrict__ Qij,
+                                                    const double* __restrict__ P, const double* __restrict__ Q,
+                                                    const double* __restrict__ V, const double* __restrict__ sbus,
+                                                    const double* __restrict__ sbr, const int32_t* __restrict__ inc_ptr,
+                                                    const int32_t* __restrict__ inc, const int32_t* __restrict__ pth,
+                                                    const int32_t* __restrict__ pv, TO* __restrict__ x) {
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= n) return;
+    const double Vj = V[j];
+    double at = sbus[4 * (int64_t)j + 1], av = sbus[4 * (int64_t)j];
+    for (int e = yrow[j]; e < yrow[j + 1]; ++e) {
+        const int l = ycol[e];
+        const double sp = sbus[4 * (int64_t)l + 2], sq = sbus[4 * (int64_t)l + 3];
+        if (l == j) {
+            const double G = yG[e], Bv = yB[e];
+            at += sp * (-Q[j] - Vj * Vj * Bv) + sq * (P[j] - Vj * Vj * G);
+            av += sp * (P[j] / Vj + Vj * G) + sq * (Q[j] / Vj - Vj * Bv);
+        } else {
+            // partials of bus l's injections wrt this bus: the terms of entry (l, j)
+            const int e2 = ytr[e];
+            const double pe = Pij[e2], qe = Qij[e2];
+            at += sp * qe - sq * pe;
+            av += (sp * pe + sq * qe) / Vj;
+        }
+    }
+    for (int cix = inc_ptr[j]; cix < inc_ptr[j + 1]; ++cix) {
+        const int code = inc[cix];
+        const double* o = sbr + 4 * (int64_t)(code >> 1);
+        if (code & 1) {  // this bus is the branch's to end
+            at += o[1];
+            av += o[3];
+        } else {
+            at += o[0];
+            av += o[2];
+        }
+    }
+    if (pth[j] >= 0) x[pth[j]] = (TO)at;
+    x[pv[j]] = (TO)av;
+}
+
 // Single-block finish of the J / largest-residual partials.
 __global__ __launch_bounds__(kBlock) void k_jfinish(int nparts, const double* __restrict__ jpart,
                                                     const double* __restrict__ rpart, const int* __restrict__ ipart,
@@ -554,8 +748,107 @@ struct Team<true> {
     static __device__ void sync() { wave_sync(); }
 };
 
+// One wave factors one front with its dependent global loads gathered into
+// few round trips: the front's and its children's metadata, the G entries and
+// the pivot thresholds are loaded lane-parallel up front, the children's
+// update matrices then one child at a time.  (The factor sweeps are bound by
+// these round trips: fronts have at most a few dozen rows.)
+template <typename T>
+__device__ inline void factor_front_wave(const FactArgs<T>& A, int s, T* F) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int c = A.sn_c[s], k = A.sn_k[s], first = A.sn_first[s];
+    const int g0 = A.g_sn_ptr[s], g1 = A.g_sn_ptr[s + 1], p0 = A.child_ptr[s], p1 = A.child_ptr[s + 1];
+    const int nch = p1 - p0;
+    int ch_l = lane < nch ? A.child[p0 + lane] : -1;
+    const int ge_l = lane < k ? A.gdiag[first + lane] : -1;
+    for (int idx = lane; idx < c * c; idx += kWave) F[idx] = T(0);
+    wave_sync();
+    for (int e = g0 + lane; e < g1; e += kWave) F[A.gloc[e]] = A.Gval[e];
+    int cc_l = 0, kc_l = 0;
+    int64_t fo_l = 0, ro_l = 0;
+    if (ch_l >= 0) {
+        cc_l = A.sn_c[ch_l];
+        kc_l = A.sn_k[ch_l];
+        fo_l = A.foff[ch_l];
+        ro_l = A.rmoff[ch_l];
+    }
+    const T gdl = ge_l >= 0 ? A.Gval[ge_l] : T(0);
+    wave_sync();
+    for (int p = 0; p < nch; ++p) {
+        int cc, kc;
+        int64_t fo, ro;
+        if (p < kWave) {
+            cc = __shfl(cc_l, p, kWave);
+            kc = __shfl(kc_l, p, kWave);
+            fo = __shfl(fo_l, p, kWave);
+            ro = __shfl(ro_l, p, kWave);
+        } else {  // more than 64 children: the rest one by one
+            const int ch = A.child[p0 + p];
+            cc = A.sn_c[ch];
+            kc = A.sn_k[ch];
+            fo = A.foff[ch];
+            ro = A.rmoff[ch];
+        }
+        const int u = cc - kc;
+        const T* Fc = A.F + fo;
+        const int32_t* rm = A.rmap + ro;
+        if (u > 0) {
+            const int da = kWave % u, db = kWave / u;
+            int b = lane / u, a = lane - b * u;
+            while (b < u) {
+                if (a >= b) F[rm[b] * c + rm[a]] += Fc[(int64_t)(kc + b) * cc + kc + a];
+                a += da;
+                b += db;
+                if (a >= u) {
+                    a -= u;
+                    ++b;
+                }
+            }
+        }
+        wave_sync();
+    }
+    for (int j = 0; j < k; ++j) {
+        T dd = F[j * c + j];
+        const T gd = j < kWave ? __shfl(gdl, j, kWave) : [&] {
+            const int ge = A.gdiag[first + j];
+            return ge >= 0 ? A.Gval[ge] : T(0);
+        }();
+        wave_sync();
+        if (!(dd > A.piv_tol * gd) || !(dd > T(0))) {
+            if (lane == 0) A.st->piv = 1;
+            dd = gd > T(0) ? A.piv_tol * gd : T(1);
+        }
+        const T dj = sqrt(dd);
+        if (lane == 0) F[j * c + j] = dj;
+        for (int i = j + 1 + lane; i < c; i += kWave) F[j * c + i] = F[j * c + i] / dj;
+        wave_sync();
+        const int w = c - j - 1;
+        if (w > 0) {
+            const int da = kWave % w, db = kWave / w;
+            int b = lane / w, a = lane - b * w;
+            while (b < w) {
+                if (a >= b) {
+                    const int aa = j + 1 + a, bb = j + 1 + b;
+                    F[bb * c + aa] -= F[j * c + aa] * F[j * c + bb];
+                }
+                a += da;
+                b += db;
+                if (a >= w) {
+                    a -= w;
+                    ++b;
+                }
+            }
+        }
+        wave_sync();
+    }
+}
+
 template <typename T, int NT, bool WAVE = false>
 __device__ inline void factor_front(const FactArgs<T>& A, int s, T* F) {
+    if (NT == kWave) {
+        factor_front_wave<T>(A, s, F);
+        return;
+    }
     using Tm = Team<WAVE>;
     const int tid = Tm::id();
     const int c = A.sn_c[s], k = A.sn_k[s], first = A.sn_first[s];
@@ -568,19 +861,19 @@ __device__ inline void factor_front(const FactArgs<T>& A, int s, T* F) {
         const int cc = A.sn_c[ch], kc = A.sn_k[ch], u = cc - kc;
         const T* Fc = A.F + A.foff[ch];
         const int32_t* rm = A.rmap + A.rmoff[ch];
-        // the u x u square walked NT entries at a time, (a, b) carried along
-        // without a division per entry; the lower triangle is added
-        int a = tid, b = 0;
-        while (a >= u && b < u) {
-            a -= u;
-            ++b;
-        }
-        while (b < u) {
-            if (a >= b) F[rm[b] * c + rm[a]] += Fc[(int64_t)(kc + b) * cc + kc + a];
-            a += NT;
-            while (a >= u && b < u) {
-                a -= u;
-                ++b;
+        // the u x u square walked NT entries at a time; (a, b) advances by the
+        // precomputed (NT % u, NT / u) instead of a division per entry
+        if (u > 0) {
+            const int da = NT % u, db = NT / u;
+            int b = tid / u, a = tid - b * u;
+            while (b < u) {
+                if (a >= b) F[rm[b] * c + rm[a]] += Fc[(int64_t)(kc + b) * cc + kc + a];
+                a += da;
+                b += db;
+                if (a >= u) {
+                    a -= u;
+                    ++b;
+                }
             }
         }
         Tm::sync();
@@ -613,11 +906,24 @@ __device__ inline void factor_front(const FactArgs<T>& A, int s, T* F) {
         if (tid == 0) F[j * c + j] = dj;
         for (int i = j + 1 + tid; i < c; i += NT) F[j * c + i] = F[j * c + i] / dj;
         Tm::sync();
-        // rank-1 update of the trailing lower triangle: each thread owns rows
-        // aa and walks their columns bb <= aa (column j read once per row)
-        for (int aa = j + 1 + tid; aa < c; aa += NT) {
-            const T la = F[j * c + aa];
-            for (int bb = j + 1; bb <= aa; ++bb) F[bb * c + aa] -= la * F[j * c + bb];
+        // rank-1 update of the trailing lower triangle over the w x w square,
+        // NT entries at a time with the same division-free stepping
+        const int w = c - j - 1;
+        if (w > 0) {
+            const int da = NT % w, db = NT / w;
+            int b = tid / w, a = tid - b * w;
+            while (b < w) {
+                if (a >= b) {
+                    const int aa = j + 1 + a, bb = j + 1 + b;
+                    F[bb * c + aa] -= F[j * c + aa] * F[j * c + bb];
+                }
+                a += da;
+                b += db;
+                if (a >= w) {
+                    a -= w;
+                    ++b;
+                }
+            }
         }
         Tm::sync();
     }
@@ -1063,6 +1369,11 @@ struct PiArgs {
     int ntop;               // top supernodes
     int ntc;                // top columns
     int slice;              // LDS elements of one per-wave buffer (cmax * kmax)
+    // per top column: start of its block column, of its path rows, path length, column in its supernode
+    const int64_t* cboff;
+    const int64_t* cpoff;
+    const int32_t* cplen;
+    const int32_t* cjj;
 };
 
 // One wave per top supernode s: the k_s unit right-hand sides of its columns are
@@ -1141,10 +1452,12 @@ __global__ __launch_bounds__(kBlock) void k_pi_assemble(int ntc, const int32_t* 
     ctop[t] = acc;
 }
 
-constexpr int kPiSegMax = 1024;  // entries per segment of the forward product (a multiple of 64)
+// entries per segment of the forward product (a multiple of 64): 256 keeps
+// enough waves in flight (1024 measured 35% slower at 200k buses)
+constexpr int kPiSegMax = 256;
 
 // Forward product y_T = L_TT^-1 c_T: one wave per segment of a row's entries,
-// all sixteen loads of a lane issued together.  A row held by one segment
+// all loads of a lane issued together.  A row held by one segment
 // (most rows) is written directly; longer rows leave partial sums for
 // k_pi_rowsum.
 template <typename T>
@@ -1202,10 +1515,10 @@ __global__ __launch_bounds__(kBlock) void k_pi_bwd(PiArgs A, SolveArgs<T> SA, co
     const int t = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
     const int lane = threadIdx.x & (kWave - 1);
     if (t >= A.ntc) return;  // uniform per wave
-    const int u = A.colsn[t], col = A.tcol[t];
-    const int jj = col - SA.sn_first[A.sn[u]], Ps = A.plen[u];
-    const T* Bc = B + A.boff[u] + (int64_t)Ps * jj;
-    const int32_t* pl = A.ploc + A.poff[u];
+    // per-column metadata, one round trip: block column, path rows, path length
+    const int col = A.tcol[t], jj = A.cjj[t], Ps = A.cplen[t];
+    const T* Bc = B + A.cboff[t];
+    const int32_t* pl = A.ploc + A.cpoff[t];
     double acc = 0.0;
     constexpr int kU = 4;  // loads of a lane in flight together
     for (int r0 = jj + lane; r0 < Ps; r0 += kU * kWave) {
@@ -1222,39 +1535,6 @@ __global__ __launch_bounds__(kBlock) void k_pi_bwd(PiArgs A, SolveArgs<T> SA, co
     }
     acc = wave_sum(acc);
     if (lane == 0) SA.x[col] = (T)acc;
-}
-
-// Backward product by top supernode: one wave reads the path's y_T once and
-// carries all k_s columns of the block (k_s <= kPiBwdK) with every load of a
-// 64-row chunk in flight together.
-constexpr int kPiBwdK = 24;
-
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pi_bwd_sn(PiArgs A, SolveArgs<T> SA, const T* __restrict__ B,
-                                                     const T* __restrict__ ytop) {
-    const int u = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
-    const int lane = threadIdx.x & (kWave - 1);
-    if (u >= A.ntop) return;  // uniform per wave
-    const int s = A.sn[u], ks = SA.sn_k[s], Ps = A.plen[u];
-    const T* Bu = B + A.boff[u];
-    const int32_t* pl = A.ploc + A.poff[u];
-    double acc[kPiBwdK];
-#pragma unroll
-    for (int jj = 0; jj < kPiBwdK; ++jj) acc[jj] = 0.0;
-    for (int r0 = 0; r0 < Ps; r0 += kWave) {
-        const int r = r0 + lane;
-        const double yv = r < Ps ? (double)ytop[pl[r]] : 0.0;
-#pragma unroll
-        for (int jj = 0; jj < kPiBwdK; ++jj)
-            if (jj < ks && r < Ps && r >= jj) acc[jj] += (double)Bu[r + (int64_t)Ps * jj] * yv;
-    }
-    const int first = SA.sn_first[s];
-#pragma unroll
-    for (int jj = 0; jj < kPiBwdK; ++jj)
-        if (jj < ks) {
-            const double v = wave_sum(acc[jj]);
-            if (lane == 0) SA.x[first + jj] = (T)v;
-        }
 }
 
 // ---------------------------------------------------------------------------
@@ -1396,7 +1676,7 @@ __global__ __launch_bounds__(kBlock) void k_sub_m_rows(SubArgs B, SolveArgs<T> A
     }
 }
 
-constexpr int kSubBatch = 16;  // independent loads in flight per lane
+constexpr int kSubBatch = 8;  // independent loads in flight per lane (16 measured slower)
 
 // Forward: one wave per subtree; lane i accumulates rows i and i + 64 over the
 // columns of M_t, kSubBatch coalesced column loads at a time.
@@ -1409,13 +1689,11 @@ __global__ __launch_bounds__(kBlock) void k_sub_fwd(SubArgs B, SolveArgs<T> A, c
     if (t >= B.nsub) return;  // uniform per wave
     const int n = B.n[t], R = B.R[t];
     const int32_t* cols = B.cols + B.cptr[t];
-    double* bs = bsh[w];
-    if (lane < n) bs[lane] = (double)A.x[cols[lane]];
-    wave_sync();
     const T* Mt = M + B.moff[t];
-    double a0 = 0.0, a1 = 0.0;
-    for (int j0 = 0; j0 < n; j0 += kSubBatch) {
-        double m0[kSubBatch], m1[kSubBatch];
+    double m0[kSubBatch], m1[kSubBatch];
+    // column loads of batch j0 (they need only the header: the first batch is
+    // in flight while the right-hand side is gathered)
+    auto load = [&](int j0) {
 #pragma unroll
         for (int u = 0; u < kSubBatch; ++u) {
             const int j = j0 + u;
@@ -1423,6 +1701,14 @@ __global__ __launch_bounds__(kBlock) void k_sub_fwd(SubArgs B, SolveArgs<T> A, c
             m0[u] = (j < n && lane >= j && lane < R) ? (double)col[lane] : 0.0;
             m1[u] = (j < n && lane + kWave < R) ? (double)col[lane + kWave] : 0.0;
         }
+    };
+    load(0);
+    double* bs = bsh[w];
+    if (lane < n) bs[lane] = (double)A.x[cols[lane]];
+    wave_sync();
+    double a0 = 0.0, a1 = 0.0;
+    for (int j0 = 0; j0 < n; j0 += kSubBatch) {
+        if (j0) load(j0);
 #pragma unroll
         for (int u = 0; u < kSubBatch; ++u) {
             const double b = j0 + u < n ? bs[j0 + u] : 0.0;
@@ -1450,18 +1736,22 @@ __global__ __launch_bounds__(kBlock) void k_sub_bwd(SubArgs B, SolveArgs<T> A, c
     const int n = B.n[t], R = B.R[t], root = B.root[t];
     const int32_t* cols = B.cols + B.cptr[t];
     const int32_t* urow = A.rows + A.rows_off[root] + A.sn_k[root];
-    double* v = vsh[w];
-    for (int i = lane; i < R; i += kWave) v[i] = (double)(i < n ? A.x[cols[i]] : A.x[urow[i - n]]);
-    wave_sync();
     const T* Mt = Mrow + B.moff[t];
-    double acc = 0.0;
-    for (int i0 = 0; i0 < R; i0 += kSubBatch) {
-        double mv[kSubBatch];
+    double mv[kSubBatch];
+    auto load = [&](int i0) {  // row loads of batch i0 (the first one overlaps the gather below)
 #pragma unroll
         for (int u = 0; u < kSubBatch; ++u) {
             const int i = i0 + u;
             mv[u] = (i < R && lane <= i && lane < n) ? (double)Mt[sub_row_off(i, n) + lane] : 0.0;
         }
+    };
+    load(0);
+    double* v = vsh[w];
+    for (int i = lane; i < R; i += kWave) v[i] = (double)(i < n ? A.x[cols[i]] : A.x[urow[i - n]]);
+    wave_sync();
+    double acc = 0.0;
+    for (int i0 = 0; i0 < R; i0 += kSubBatch) {
+        if (i0) load(i0);
 #pragma unroll
         for (int u = 0; u < kSubBatch; ++u) acc += mv[u] * (i0 + u < R ? v[i0 + u] : 0.0);
     }
@@ -1712,7 +2002,11 @@ __global__ __launch_bounds__(kBlock) void k_update(int n, const int32_t* __restr
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_maxfinish(int nparts, const double* __restrict__ mpart, DevStatus* st) {
+// Also hands the iteration's status to the host: a copy into the pinned status
+// block through its device mapping, so the host waits on the stream and reads
+// it without a separate copy (no copy engine, no blit launch per iteration).
+__global__ __launch_bounds__(kBlock) void k_maxfinish(int nparts, const double* __restrict__ mpart, DevStatus* st,
+                                                      DevStatus* host_out) {
     double r = 0.0;
     for (int p = threadIdx.x; p < nparts; p += kBlock) r = fmax(r, mpart[p]);
     __shared__ double sm[kBlock / kWave];
@@ -1723,6 +2017,16 @@ __global__ __launch_bounds__(kBlock) void k_maxfinish(int nparts, const double* 
         double q = 0.0;
         for (int k = 0; k < kBlock / kWave; ++k) q = fmax(q, sm[k]);
         st->maxdx = q;
+        if (host_out) {
+            host_out->maxdx = q;
+            host_out->J = st->J;
+            host_out->rmax = st->rmax;
+            host_out->ridx = st->ridx;
+            host_out->nan = st->nan;
+            host_out->piv = st->piv;
+            host_out->flow_err = st->flow_err;
+            __threadfence_system();
+        }
     }
 }
 
@@ -1773,6 +2077,72 @@ inline int nblk(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
 // anywhere is retired rather than trusted.
 static std::atomic<uint64_t> g_alloc_epoch{0};
 
+// Process-wide cache of device blocks.  hipFree waits for the whole device,
+// so an engine torn down while another serves requests would stall that
+// engine's stream; freed blocks are kept per device and size class (four
+// classes per power of two, at most 25% slack) and handed to the next
+// allocation of that class instead.  The cache holds the peak of what was in
+// use; it is returned to the runtime when the process ends.
+class BlockCache {
+   public:
+    static size_t size_class(size_t bytes) {
+        if (bytes <= 4096) return 4096;
+        int e = 63 - __builtin_clzll((unsigned long long)bytes);  // 2^e <= bytes < 2^(e+1)
+        const size_t q = (size_t)1 << (e - 2);                   // quarter steps
+        return (bytes + q - 1) / q * q;
+    }
+    static void* get(size_t bytes) {
+        int dev = 0;
+        HIPCHECK(hipGetDevice(&dev));
+        const size_t cls = size_class(bytes);
+        {
+            std::lock_guard<std::mutex> lk(mu());
+            auto& fl = free_lists()[{dev, cls}];
+            if (!fl.empty()) {
+                void* p = fl.back();
+                fl.pop_back();
+                return p;
+            }
+        }
+        void* p = nullptr;
+        HIPCHECK(hipMalloc(&p, cls));
+        return p;
+    }
+    // The releasing instance may still have work queued that reads the block:
+    // its stream (set by StreamScope for the duration of an instance call) is
+    // drained before the block is offered to any other stream.
+    static void put(void* p, size_t bytes) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return;
+        if (release_stream()) (void)hipStreamSynchronize(release_stream());
+        std::lock_guard<std::mutex> lk(mu());
+        free_lists()[{dev, size_class(bytes)}].push_back(p);
+    }
+
+    static hipStream_t& release_stream() {
+        static thread_local hipStream_t s = nullptr;
+        return s;
+    }
+
+   private:
+    static std::mutex& mu() {
+        static std::mutex m;
+        return m;
+    }
+    static std::map<std::pair<int, size_t>, std::vector<void*>>& free_lists() {
+        static auto* f = new std::map<std::pair<int, size_t>, std::vector<void*>>();  // lives to process end
+        return *f;
+    }
+};
+
+// The stream whose queued work may still use the blocks released during an
+// instance call.
+struct StreamScope {
+    hipStream_t prev;
+    explicit StreamScope(hipStream_t s) : prev(BlockCache::release_stream()) { BlockCache::release_stream() = s; }
+    ~StreamScope() { BlockCache::release_stream() = prev; }
+};
+
 template <typename T>
 struct DBuf {
     T* p = nullptr;
@@ -1781,14 +2151,14 @@ struct DBuf {
         release(bytes);
         n = cnt;
         if (cnt) {
-            HIPCHECK(hipMalloc(&p, cnt * sizeof(T)));
+            p = static_cast<T*>(BlockCache::get(cnt * sizeof(T)));
             bytes += cnt * sizeof(T);
             g_alloc_epoch.fetch_add(1, std::memory_order_relaxed);
         }
     }
     void release(size_t& bytes) {
         if (p) {
-            (void)hipFree(p);
+            BlockCache::put(p, n * sizeof(T));
             bytes -= n * sizeof(T);
             g_alloc_epoch.fetch_add(1, std::memory_order_relaxed);
         }
@@ -2092,9 +2462,8 @@ struct PiPlan {
     int cut = 0;
     int ntop = 0, ntc = 0, nseg = 0, slice = 0;
     int64_t nblock = 0, nq = 0;
-    std::vector<int32_t> sn, plen, ploc, tcol, colsn, src, segptr, cptr, parent, segrow, mrows;
-    std::vector<int64_t> boff, poff, pos, segq, cpos;
-    int kmax = 0;  // largest k over the top supernodes
+    std::vector<int32_t> sn, plen, ploc, tcol, colsn, src, segptr, cptr, parent, segrow, mrows, cplen, cjj;
+    std::vector<int64_t> boff, poff, pos, segq, cpos, cboff, cpoff;
     // dense bottom subtrees (see SubArgs)
     bool dense = false;
     int nsub = 0;
@@ -2215,7 +2584,6 @@ static PiPlan make_pi_plan(const Symbolic& S, int cut, bool dense) {
     P.ntop = (int)P.sn.size();
     P.ntc = (int)P.tcol.size();
     P.slice = cmax * kmax;
-    P.kmax = kmax;
     P.parent = S.sn_parent;
     // blocks and path rows
     P.plen.resize(P.ntop);
@@ -2229,6 +2597,12 @@ static PiPlan make_pi_plan(const Symbolic& S, int cut, bool dense) {
         for (int p = s0; p >= 0; p = S.sn_parent[p])
             for (int j = 0; j < S.sn_k[p]; ++j, ++len) P.ploc.push_back(tloc[S.sn_first[p] + j]);
         P.plen[u] = len;
+        for (int j = 0; j < S.sn_k[s0]; ++j) {
+            P.cboff.push_back(P.boff[u] + (int64_t)len * j);
+            P.cpoff.push_back(P.poff[u]);
+            P.cplen.push_back(len);
+            P.cjj.push_back(j);
+        }
         P.nblock += (int64_t)len * S.sn_k[s0];
     }
     // forward product grouped by row: entry (u, r, jj) lands in row ploc[r]
@@ -2317,8 +2691,6 @@ struct Instance::Impl {
     DBuf<CgScalars> cg_s;
     DBuf<double> jpart, rpart, mpart;
     DBuf<int> ipart;
-    DBuf<int32_t> zidx;
-    DBuf<double> zval;
     // symbolic structures
     DBuf<int32_t> tptr, tpos, trow, tout, gptr, gloc, gdiag, g_sn_ptr;
     DBuf<int32_t> hpos;  // H^T position of each device H entry
@@ -2332,12 +2704,15 @@ struct Instance::Impl {
     int32_t* pin_idx = nullptr;  // pinned staging of indexed telemetry
     double* pin_val = nullptr;
     size_t pin_cap = 0;
+    int32_t* pin_idx_dev = nullptr;  // device mappings of the pinned staging
+    double* pin_val_dev = nullptr;
     DBuf<double> ztmp;
     DBuf<int2> gpair;
     DBuf<int32_t> sn_first, sn_k, sn_c, child_ptr, child, rmap, rows, perm, pth, pv, lists;
     DBuf<int64_t> foff, rmoff, uoff, rows_off, moff;
     DBuf<DevStatus> st;
-    DevStatus* hst = nullptr;  // pinned
+    DevStatus* hst = nullptr;      // pinned
+    DevStatus* hst_dev = nullptr;  // its device mapping (k_maxfinish writes the iteration status there)
     PrecBufs<float> f32;
     PrecBufs<double> f64;
     // per-level launch lists: [level][class] -> (offset into lists, count)
@@ -2372,9 +2747,9 @@ struct Instance::Impl {
         bool ok = false;
         int cut = 0, ntop = 0, ntc = 0, nseg = 0, slice = 0;
         int64_t nblock = 0, nq = 0;
-        DBuf<int32_t> sn, plen, ploc, tcol, colsn, src, segptr, cptr, parent, segrow, mrows;
-        DBuf<int64_t> boff, poff, pos, segq, cpos;
-        int nmrows = 0, kmax = 0;
+        DBuf<int32_t> sn, plen, ploc, tcol, colsn, src, segptr, cptr, parent, segrow, mrows, cplen, cjj;
+        DBuf<int64_t> boff, poff, pos, segq, cpos, cboff, cpoff;
+        int nmrows = 0;
         DBuf<double> part;
         LdsDev bottom[2];  // bottom-subtree LDS tasks, [0] float, [1] double
         // dense bottom subtrees
@@ -2399,6 +2774,11 @@ struct Instance::Impl {
     bool use_pi = true;
     bool pi_dense = true;   // bottom subtrees as dense solve matrices (else LDS tasks)
     bool meas_by_type = true;  // measurement rows grouped by evaluation class (else by anchor bus only)
+    // right-hand side by location (needs the class grouping)
+    bool rhs_loc = false;  // asked for
+    bool use_loc = false;  // in use (asked for, and the rows are grouped by class)
+    DBuf<int32_t> bv_ptr, bi_ptr, br_ptr, ytr, inc_ptr, inc;
+    DBuf<double> sbus, sbr;
     int pi_cut = 0;    // fronts per bottom subtree; 0 = default
     int pi_waves = 4;  // waves per bottom-subtree workgroup under the partitioned inverse
     // dependency-driven sweeps (all fronts fit in LDS)
@@ -2509,9 +2889,12 @@ static void build_pi(Instance::Impl& D, const Symbolic& S, size_t& bytes, hipStr
             up(Q.src, PP.src);
             up(Q.segptr, PP.segptr);
             up(Q.segrow, PP.segrow);
+            up(Q.cplen, PP.cplen);
+            up(Q.cjj, PP.cjj);
+            up64(Q.cboff, PP.cboff);
+            up64(Q.cpoff, PP.cpoff);
             up(Q.mrows, PP.mrows);
             Q.nmrows = (int)PP.mrows.size();
-            Q.kmax = PP.kmax;
             up(Q.cptr, PP.cptr);
             up(Q.parent, PP.parent);
             up64(Q.boff, PP.boff);
@@ -2566,6 +2949,7 @@ Instance::Instance(int device, int precision) : device_(device), precision_(prec
     HIPCHECK(hipEventCreate(&d_->ev0));
     HIPCHECK(hipEventCreate(&d_->ev1));
     HIPCHECK(hipHostMalloc((void**)&d_->hst, sizeof(DevStatus), hipHostMallocDefault));
+    HIPCHECK(hipHostGetDevicePointer((void**)&d_->hst_dev, d_->hst, 0));
     d_->st.alloc(1, bytes_);
     for (int p = 0; p < PH_COUNT; ++p)
         for (int q = 0; q < 2; ++q) HIPCHECK(hipEventCreate(&d_->pev[p][q]));
@@ -2589,8 +2973,10 @@ Instance::~Instance() {
     rel(d_->Pf); rel(d_->Qf); rel(d_->Pt); rel(d_->Qt); rel(d_->mcode); rel(d_->mloc); rel(d_->hrow);
     rel(d_->hs); rel(d_->hcolp); rel(d_->cg_b); rel(d_->cg_x); rel(d_->cg_r); rel(d_->cg_p); rel(d_->cg_q);
     rel(d_->cg_z); rel(d_->cg_t); rel(d_->cg_parts); rel(d_->cg_s); rel(d_->z); rel(d_->sigma); rel(d_->rs); rel(d_->hx); rel(d_->r); rel(d_->jpart); rel(d_->rpart);
-    rel(d_->mpart); rel(d_->ipart); rel(d_->zidx); rel(d_->zval); rel(d_->tptr); rel(d_->tpos); rel(d_->trow);
+    rel(d_->mpart); rel(d_->ipart); rel(d_->tptr); rel(d_->tpos); rel(d_->trow);
     rel(d_->tout); rel(d_->ml_d); rel(d_->lm_d); rel(d_->ztmp);
+    rel(d_->bv_ptr); rel(d_->bi_ptr); rel(d_->br_ptr); rel(d_->ytr); rel(d_->inc_ptr); rel(d_->inc); rel(d_->sbus);
+    rel(d_->sbr);
     if (d_->pin_idx) (void)hipHostFree(d_->pin_idx);
     if (d_->pin_val) (void)hipHostFree(d_->pin_val); rel(d_->hpos); rel(d_->hr);
     rel(d_->gptr); rel(d_->gloc); rel(d_->gdiag); rel(d_->g_sn_ptr); rel(d_->gpair); rel(d_->sn_first);
@@ -2604,7 +2990,7 @@ Instance::~Instance() {
     {
         auto& P = d_->pi;
         rel(P.sn); rel(P.plen); rel(P.ploc); rel(P.tcol); rel(P.colsn); rel(P.src); rel(P.segptr); rel(P.cptr); rel(P.parent);
-        rel(P.segrow); rel(P.mrows);
+        rel(P.segrow); rel(P.mrows); rel(P.cplen); rel(P.cjj); rel(P.cboff); rel(P.cpoff);
         rel(P.boff); rel(P.poff); rel(P.pos); rel(P.segq); rel(P.cpos); rel(P.part);
         rel(P.sroot); rel(P.sn_n); rel(P.sR); rel(P.fptr); rel(P.fsn); rel(P.fdcol); rel(P.drow); rel(P.scptr);
         rel(P.scols); rel(P.bins); rel(P.fdoff); rel(P.smoff);
@@ -2633,6 +3019,7 @@ Instance::~Instance() {
 void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
     ++d_->plan_epoch;  // launch plans change: captured graphs retire
     HIPCHECK(hipSetDevice(device_));
+    StreamScope scope_(d_->stream);
     S_ = std::move(Sp);
     const Symbolic& S = *S_;
     Impl& D = *d_;
@@ -2687,11 +3074,17 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
         const int t = h.mtype[i];
         return (t == 0 || t == 1) ? 0 : ((t == 2 || t == 3 || t == 7 || t == 8) ? 1 : 2);
     };
+    // inside the class of branch flows: by branch (device order) and end, so
+    // that the rows of one branch end are contiguous
+    auto mkey = [&](int i) -> int64_t {
+        if (D.meas_by_type && mclass(i) == 2) return (int64_t)D.lk[h.mloc[i]] * 2 + h.mend[i];
+        return D.lb[anchor(i)];
+    };
     D.ml.resize(m);
     for (int i = 0; i < m; ++i) D.ml[i] = i;
     std::stable_sort(D.ml.begin(), D.ml.end(), [&](int a, int b) {
         const int ca = mclass(a), cb = mclass(b);
-        return ca != cb ? ca < cb : D.lb[anchor(a)] < D.lb[anchor(b)];
+        return ca != cb ? ca < cb : mkey(a) < mkey(b);
     });
     D.lm.assign(m, 0);
     for (int p = 0; p < m; ++p) D.lm[D.ml[p]] = p;
@@ -2760,6 +3153,69 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
         D.ycol.upload(yc, bytes_, s);
         D.ycptr.upload(cp, bytes_, s);
         D.ycon.upload(cn, bytes_, s);
+        // location layout of the right-hand side (rows grouped by class, then
+        // by bus / branch end; see k_meas_loc)
+        D.use_loc = D.rhs_loc && D.meas_by_type;
+        if (D.use_loc) {
+            std::vector<int32_t> tr(std::max<size_t>(1, yc.size()), 0);
+            for (int p = 0; p < n; ++p)
+                for (int e = yr[p]; e < yr[p + 1]; ++e) {
+                    const int q = yc[e];
+                    int f = -1;
+                    for (int e2 = yr[q]; e2 < yr[q + 1]; ++e2)
+                        if (yc[e2] == p) f = e2;
+                    if (f < 0) throw std::logic_error("Ybus pattern not symmetric");
+                    tr[e] = f;
+                }
+            std::vector<int32_t> ip(n + 1, 0), inc(std::max(1, 2 * nb), 0);
+            for (int k = 0; k < nb; ++k) {
+                ip[D.lb[h.from[D.kl[k]]] + 1]++;
+                ip[D.lb[h.to[D.kl[k]]] + 1]++;
+            }
+            for (int p = 0; p < n; ++p) ip[p + 1] += ip[p];
+            std::vector<int32_t> fill(ip.begin(), ip.end() - 1);
+            for (int k = 0; k < nb; ++k) {
+                inc[fill[D.lb[h.from[D.kl[k]]]]++] = 2 * k;
+                inc[fill[D.lb[h.to[D.kl[k]]]]++] = 2 * k + 1;
+            }
+            // row ranges per location, in device row order
+            std::vector<int32_t> bv(n + 1, 0), bi(n + 1, 0), br(2 * (size_t)nb + 1, 0);
+            int n0 = 0, n1 = 0;
+            for (int i = 0; i < m; ++i) {
+                const int c = mclass(i);
+                if (c == 0) {
+                    bv[D.lb[h.mloc[i]] + 1]++;
+                    ++n0;
+                } else if (c == 1) {
+                    bi[D.lb[h.mloc[i]] + 1]++;
+                    ++n1;
+                } else {
+                    br[2 * (size_t)D.lk[h.mloc[i]] + h.mend[i] + 1]++;
+                }
+            }
+            bi[0] = n0;
+            br[0] = n0 + n1;
+            for (int p = 0; p < n; ++p) {
+                bv[p + 1] += bv[p];
+                bi[p + 1] += bi[p];
+            }
+            for (size_t q = 0; q < 2 * (size_t)nb; ++q) br[q + 1] += br[q];
+            // every row must sit inside its location's range
+            for (int r = 0; r < m; ++r) {
+                const int i = D.ml[r], c = mclass(i);
+                const size_t loc = c == 2 ? 2 * (size_t)D.lk[h.mloc[i]] + h.mend[i] : (size_t)D.lb[h.mloc[i]];
+                const std::vector<int32_t>& ptr = c == 0 ? bv : (c == 1 ? bi : br);
+                if (r < ptr[loc] || r >= ptr[loc + 1]) throw std::logic_error("measurement rows not grouped by location");
+            }
+            D.ytr.upload(tr, bytes_, s);
+            D.inc_ptr.upload(ip, bytes_, s);
+            D.inc.upload(inc, bytes_, s);
+            D.bv_ptr.upload(bv, bytes_, s);
+            D.bi_ptr.upload(bi, bytes_, s);
+            D.br_ptr.upload(br, bytes_, s);
+            D.sbus.alloc(4 * (size_t)std::max(1, n), bytes_);
+            D.sbr.alloc(4 * (size_t)std::max(1, nb), bytes_);
+        }
         HIPCHECK(hipStreamSynchronize(s));
     }
     size_t nY = S.ycol.size();
@@ -3043,6 +3499,7 @@ std::vector<typename V::value_type> take(const V& v, const std::vector<int32_t>&
 void Instance::update_branches(const HostArrays& h) {
     ++d_->plan_epoch;  // launch plans change: captured graphs retire
     HIPCHECK(hipSetDevice(device_));
+    StreamScope scope_(d_->stream);
     Impl& D = *d_;
     hipStream_t s = D.stream;
     // host sources stay alive until the stream has consumed them
@@ -3089,6 +3546,7 @@ void Instance::refresh_topology() {
     Impl& D = *d_;
     if (!D.topo_staged) return;
     HIPCHECK(hipSetDevice(device_));
+    StreamScope scope_(d_->stream);
     hipStream_t s = D.stream;
     D.status.upload(D.status_h, bytes_, s);
     D.mcode.upload(D.mcode_h, bytes_, s);
@@ -3105,6 +3563,7 @@ void Instance::refresh_topology() {
 // z arrives in insertion order; the device holds it in row layout order
 void Instance::set_z(const double* z) {
     HIPCHECK(hipSetDevice(device_));
+    StreamScope scope_(d_->stream);
     Impl& D = *d_;
     if (!D.m) return;
     if (D.ztmp.n != (size_t)D.m) D.ztmp.alloc(D.m, bytes_);
@@ -3117,12 +3576,10 @@ void Instance::set_z(const double* z) {
 void Instance::set_z_indexed(const int32_t* idx, const double* val, int count) {
     if (count <= 0) return;
     HIPCHECK(hipSetDevice(device_));
+    StreamScope scope_(d_->stream);
     Impl& D = *d_;
-    if (D.zidx.n < (size_t)count) {
-        D.zidx.alloc(count, bytes_);
-        D.zval.alloc(count, bytes_);
-    }
-    // through pinned staging (a pageable copy of this size costs several times more)
+    // through pinned staging that the scatter kernel reads in place (no copy
+    // engine: a large copy of another engine on the device cannot delay it)
     if (D.pin_cap < (size_t)count) {
         if (D.pin_idx) HIPCHECK(hipHostFree(D.pin_idx));
         if (D.pin_val) HIPCHECK(hipHostFree(D.pin_val));
@@ -3130,19 +3587,20 @@ void Instance::set_z_indexed(const int32_t* idx, const double* val, int count) {
         D.pin_val = nullptr;
         HIPCHECK(hipHostMalloc((void**)&D.pin_idx, (size_t)count * sizeof(int32_t), hipHostMallocDefault));
         HIPCHECK(hipHostMalloc((void**)&D.pin_val, (size_t)count * sizeof(double), hipHostMallocDefault));
+        HIPCHECK(hipHostGetDevicePointer((void**)&D.pin_idx_dev, D.pin_idx, 0));
+        HIPCHECK(hipHostGetDevicePointer((void**)&D.pin_val_dev, D.pin_val, 0));
         D.pin_cap = count;
     }
     std::memcpy(D.pin_idx, idx, (size_t)count * sizeof(int32_t));
     std::memcpy(D.pin_val, val, (size_t)count * sizeof(double));
-    HIPCHECK(hipMemcpyAsync(D.zidx.p, D.pin_idx, (size_t)count * sizeof(int32_t), hipMemcpyHostToDevice, D.stream));
-    HIPCHECK(hipMemcpyAsync(D.zval.p, D.pin_val, (size_t)count * sizeof(double), hipMemcpyHostToDevice, D.stream));
-    k_scatter_z_mapped<<<nblk(count), kBlock, 0, D.stream>>>(count, D.zidx.p, D.zval.p, D.lm_d.p, D.z.p);
+    k_scatter_z_mapped<<<nblk(count), kBlock, 0, D.stream>>>(count, D.pin_idx_dev, D.pin_val_dev, D.lm_d.p, D.z.p);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(D.stream));  // the staging buffers are reused by the next call
 }
 
 void Instance::set_state(const double* V, const double* th) {
     HIPCHECK(hipSetDevice(device_));
+    StreamScope scope_(d_->stream);
     Impl& D = *d_;
     std::vector<double> v(D.n), t(D.n);
     for (int p = 0; p < D.n; ++p) {
@@ -3167,12 +3625,14 @@ void fetch_unpermute(const DBuf<double>& src, const std::vector<int32_t>& pos_of
 
 void Instance::get_state(double* V, double* th) {
     HIPCHECK(hipSetDevice(device_));
+    StreamScope scope_(d_->stream);
     fetch_unpermute(d_->V, d_->lb, V, d_->stream);
     fetch_unpermute(d_->th, d_->lb, th, d_->stream);
 }
 
 void Instance::flat_start() {
     HIPCHECK(hipSetDevice(device_));
+    StreamScope scope_(d_->stream);
     k_flat<<<nblk(d_->n), kBlock, 0, d_->stream>>>(d_->n, d_->bus_type.p, d_->vset.p, d_->V.p, d_->th.p);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(d_->stream));
@@ -3199,6 +3659,43 @@ namespace {
 void launch_power(Instance::Impl& D) {
     k_power<<<nblk(D.n), kBlock, 0, D.stream>>>(D.n, D.yrow.p, D.ycol.p, D.yG.p, D.yB.p, D.V.p, D.th.p, D.P.p,
                                                 D.Q.p, D.Pij.p, D.Qij.p, D.st.p);
+}
+
+void launch_meas_loc(Instance::Impl& D, int robust, double gamma) {
+    LocArgs A;
+    A.n = D.n;
+    A.nb = D.nb;
+    A.robust = robust;
+    A.gamma = gamma;
+    A.mcode = D.mcode.p;
+    A.z = D.z.p;
+    A.sigma = D.sigma.p;
+    A.bv_ptr = D.bv_ptr.p;
+    A.bi_ptr = D.bi_ptr.p;
+    A.br_ptr = D.br_ptr.p;
+    A.P = D.P.p;
+    A.Q = D.Q.p;
+    A.V = D.V.p;
+    A.th = D.th.p;
+    A.from = D.from.p;
+    A.to = D.to.p;
+    A.gs = D.gs.p;
+    A.bs = D.bs.p;
+    A.bc = D.bc.p;
+    A.tap = D.tap.p;
+    A.shift = D.shift.p;
+    A.sbus = D.sbus.p;
+    A.sbr = D.sbr.p;
+    const int64_t nt = (int64_t)D.n + D.nb;
+    if (nt) k_meas_loc<<<nblk(nt), kBlock, 0, D.stream>>>(A);
+}
+
+template <typename T>
+void launch_rhs_loc(Instance::Impl& D) {
+    if (D.n)
+        k_rhs_loc<T><<<nblk(D.n), kBlock, 0, D.stream>>>(D.n, D.yrow.p, D.ycol.p, D.ytr.p, D.yG.p, D.yB.p, D.Pij.p,
+                                                         D.Qij.p, D.P.p, D.Q.p, D.V.p, D.sbus.p, D.sbr.p, D.inc_ptr.p,
+                                                         D.inc.p, D.pth.p, D.pv.p, bufs<T>(D).x.p);
 }
 
 void launch_meas(Instance::Impl& D, int robust, double gamma, int unit_weights, int want_jac, bool want_out,
@@ -3322,8 +3819,8 @@ SolveArgs<T> solve_args(Instance::Impl& D) {
 
 PiArgs pi_args(const Instance::Impl& D) {
     const Instance::Impl::PiDev& P = D.pi;
-    return PiArgs{P.sn.p, P.plen.p, P.boff.p, P.poff.p, P.ploc.p, P.parent.p, P.tcol.p, P.colsn.p, P.ntop, P.ntc,
-                  P.slice};
+    return PiArgs{P.sn.p,   P.plen.p,  P.boff.p,  P.poff.p,  P.ploc.p, P.parent.p, P.tcol.p, P.colsn.p,
+                  P.ntop,   P.ntc,     P.slice,   P.cboff.p, P.cpoff.p, P.cplen.p,  P.cjj.p};
 }
 
 bool pi_active(const Instance::Impl& D) { return D.use_pi && D.pi.ok; }
@@ -3431,12 +3928,8 @@ void launch_trisolve(Instance::Impl& D, Prof* pf = nullptr) {
         }
         prof_end(pf, PH_FWD, (dense || PB.n_bottom ? 1 : 0) + (PI.ntc ? 2 + (PI.nmrows ? 1 : 0) : 0));
         prof_begin(pf, PH_BWD);
-        if (PI.ntc) {
-            if (PI.kmax <= kPiBwdK)
-                k_pi_bwd_sn<T><<<(PI.ntop + nw - 1) / nw, kBlock, 0, D.stream>>>(pi_args(D), A, B.pinv.p, B.ytop.p);
-            else
-                k_pi_bwd<T><<<(PI.ntc + nw - 1) / nw, kBlock, 0, D.stream>>>(pi_args(D), A, B.pinv.p, B.ytop.p);
-        }
+        if (PI.ntc)
+            k_pi_bwd<T><<<(PI.ntc + nw - 1) / nw, kBlock, 0, D.stream>>>(pi_args(D), A, B.pinv.p, B.ytop.p);
         if (dense)
             k_sub_bwd<T><<<(PI.nsub + nw - 1) / nw, kBlock, 0, D.stream>>>(sub_args(D), A, B.msub.p + PI.msub);
         else if (PB.n_bottom)
@@ -3535,7 +4028,7 @@ void launch_update(Instance::Impl& D, double vmin, double vmax, const T* dx, int
     int nbk = nblk(D.n);
     k_update<T><<<nbk, kBlock, 0, D.stream>>>(D.n, D.pth.p, D.pv.p, dx, D.V.p, D.th.p, vmin, vmax, D.mpart.p, D.st.p,
                                               skip_on_pivot);
-    k_maxfinish<<<1, kBlock, 0, D.stream>>>(nbk, D.mpart.p, D.st.p);
+    k_maxfinish<<<1, kBlock, 0, D.stream>>>(nbk, D.mpart.p, D.st.p, D.hst_dev);
 }
 
 void launch_dot(Instance::Impl& D, const double* a, const double* b, double* out) {
@@ -3635,9 +4128,13 @@ void iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p, bool 
     pf.end(PH_POWER, 1);
     pf.begin(PH_MEAS);
     // row-major Jacobian only where G or the CG product needs it; the RHS
-    // products always
+    // products always (by location when no Jacobian values are needed)
     const bool direct = p.pcg_steps <= 0;
-    launch_meas(D, p.robust ? 1 : 0, p.gamma, 0, (refactor || !direct) ? 1 : 0, false, true, newton);
+    const bool loc = D.use_loc && !refactor && direct;
+    if (loc)
+        launch_meas_loc(D, p.robust ? 1 : 0, p.gamma);
+    else
+        launch_meas(D, p.robust ? 1 : 0, p.gamma, 0, (refactor || !direct) ? 1 : 0, false, true, newton);
     pf.end(PH_MEAS, 1);
     if (refactor) {
         pf.begin(PH_SPGEMM);
@@ -3657,7 +4154,10 @@ void iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p, bool 
         return;
     }
     pf.begin(PH_RHS);
-    launch_rhs<T>(D);
+    if (loc)
+        launch_rhs_loc<T>(D);
+    else
+        launch_rhs<T>(D);
     pf.end(PH_RHS, 1);
     launch_trisolve<T>(D, &pf);
     pf.begin(PH_UPDATE);
@@ -3709,13 +4209,14 @@ void graph_iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p,
         D.graphs.push_back({ae, pe, prec, p.robust, (int)newton, p.gamma, p.vmin, p.vmax, exec});
     }
     HIPCHECK(hipGraphLaunch(exec, D.stream));
-    read_status(D);
+    HIPCHECK(hipStreamSynchronize(D.stream));  // k_maxfinish wrote the status into D.hst
 }
 
 }  // namespace
 
 SolveOutcome Instance::solve(const SolveParams& p) {
     HIPCHECK(hipSetDevice(device_));
+    StreamScope scope_(d_->stream);
     Impl& D = *d_;
     const Symbolic& S = *S_;
     SolveOutcome out;
@@ -3762,7 +4263,7 @@ SolveOutcome Instance::solve(const SolveParams& p) {
                     iteration<float>(D, S, p, refactor, 1e-5f, pf, newton);
             }
             HIPCHECK(hipGetLastError());
-            if (!graph) read_status(D);  // graph_iteration reads it itself
+            if (!graph) HIPCHECK(hipStreamSynchronize(D.stream));  // status in D.hst (k_maxfinish)
             pf.collect(ktimes_);
             if (D.hst->piv && newton) {
                 // the gain with reduced outlier weights lost rank: IRLS weights from here on
@@ -3875,6 +4376,7 @@ SolveOutcome Instance::solve(const SolveParams& p) {
 void Instance::get_results(double* P, double* Q, double* Pf, double* Qf, double* Pt, double* Qt, double* hx,
                            double* r) {
     HIPCHECK(hipSetDevice(device_));
+    StreamScope scope_(d_->stream);
     Impl& D = *d_;
     fetch_unpermute(D.P, D.lb, P, D.stream);
     fetch_unpermute(D.Q, D.lb, Q, D.stream);
@@ -3888,6 +4390,7 @@ void Instance::get_results(double* P, double* Q, double* Pf, double* Qf, double*
 
 void Instance::debug_eval(double* hx, double* P, double* Q, double* Pf, double* Qf, double* Pt, double* Qt) {
     HIPCHECK(hipSetDevice(device_));
+    StreamScope scope_(d_->stream);
     Impl& D = *d_;
     launch_power(D);
     if (D.nb)
@@ -3901,6 +4404,7 @@ void Instance::debug_eval(double* hx, double* P, double* Q, double* Pf, double* 
 
 void Instance::debug_ybus(double* G, double* B) {
     HIPCHECK(hipSetDevice(device_));
+    StreamScope scope_(d_->stream);
     Impl& D = *d_;
     const Symbolic& S = *S_;
     const size_t nY = S.ycol.size();
@@ -3928,6 +4432,7 @@ void fetch_as_double(const DBuf<T>& src, size_t cnt, double* dst, hipStream_t s)
 
 void Instance::debug_jacobian_gain(double* hv, double* gv) {
     HIPCHECK(hipSetDevice(device_));
+    StreamScope scope_(d_->stream);
     Impl& D = *d_;
     const Symbolic& S = *S_;
     launch_power(D);
@@ -3957,6 +4462,7 @@ void Instance::debug_jacobian_gain(double* hv, double* gv) {
 int Instance::debug_factor(double* fronts) {
     ++d_->plan_epoch;  // launch plans change: captured graphs retire
     HIPCHECK(hipSetDevice(device_));
+    StreamScope scope_(d_->stream);
     Impl& D = *d_;
     const Symbolic& S = *S_;
     for (int attempt = 0; attempt < 2; ++attempt) {
@@ -3989,6 +4495,7 @@ int Instance::debug_factor(double* fronts) {
 void Instance::debug_solve(const double* b, double* dx) {
     ++d_->plan_epoch;  // launch plans change: captured graphs retire
     HIPCHECK(hipSetDevice(device_));
+    StreamScope scope_(d_->stream);
     Impl& D = *d_;
     DBuf<double> bb, xx;
     size_t tmp = 0;
@@ -4024,6 +4531,7 @@ void Instance::stage_telemetry(int slot, const int32_t* idx, const double* val, 
     for (int q = 0; q < count; ++q)
         if (idx[q] < 0 || idx[q] >= d_->m) throw std::invalid_argument("staged measurement index out of range");
     HIPCHECK(hipSetDevice(device_));
+    StreamScope scope_(d_->stream);
     Impl& D = *d_;
     if ((int)D.staged.size() <= slot) D.staged.resize(slot + 1);
     Impl::Staged& sb = D.staged[slot];
@@ -4043,6 +4551,7 @@ void Instance::apply_staged(int slot) {
     Impl& D = *d_;
     if (slot < 0 || slot >= (int)D.staged.size()) throw std::invalid_argument("unknown staging slot");
     HIPCHECK(hipSetDevice(device_));
+    StreamScope scope_(d_->stream);
     const Impl::Staged& sb = D.staged[slot];
     if (sb.count)
         k_scatter_z<<<nblk(sb.count), kBlock, 0, D.stream>>>(sb.count, sb.idx.p, sb.val.p, D.z.p);
@@ -4072,6 +4581,11 @@ void Instance::set_dense_subtrees(bool on) {
     d_->pi_dense = on;
 }
 
+void Instance::set_rhs_by_location(bool on) {
+    ++d_->plan_epoch;
+    d_->rhs_loc = on;
+}
+
 void Instance::set_meas_by_type(bool on) {
     ++d_->plan_epoch;
     d_->meas_by_type = on;
@@ -4080,6 +4594,7 @@ void Instance::set_meas_by_type(bool on) {
 void Instance::set_xcd_remap(bool on) {
     ++d_->plan_epoch;  // launch plans change: captured graphs retire
     HIPCHECK(hipSetDevice(device_));
+    StreamScope scope_(d_->stream);
     const int v = on ? 1 : 0;
     HIPCHECK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_xcd_remap), &v, sizeof(int), 0, hipMemcpyHostToDevice, d_->stream));
     HIPCHECK(hipStreamSynchronize(d_->stream));
@@ -4190,6 +4705,7 @@ double time_factor(Instance::Impl& D, int reps) {
 std::string Instance::autotune_solve(int reps) {
     ++d_->plan_epoch;  // launch plans change: captured graphs retire
     HIPCHECK(hipSetDevice(device_));
+    StreamScope scope_(d_->stream);
     Impl& D = *d_;
     const Symbolic& S = *S_;
     if (!factor_valid_) throw std::invalid_argument("autotune needs a factored gain matrix: solve first");

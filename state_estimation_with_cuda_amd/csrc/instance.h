@@ -150,6 +150,10 @@ class Instance {
     // device order of the measurement rows: grouped by evaluation class, each
     // class by anchor bus (true), or by anchor bus only (takes effect at setup)
     void set_meas_by_type(bool on);
+    // iterations on a cached factor form the right-hand side by location
+    // (summed weighted residuals per bus and branch end, gathered per state)
+    // instead of per Jacobian entry (takes effect at setup; needs the class order)
+    void set_rhs_by_location(bool on);
     bool lds_flags() const;
     std::string sweep_plan() const;
     // Autotuner of the solve plan: times the forward + backward solves of the

@@ -365,6 +365,7 @@ PLAN_OPTIONS = {
     "levels": {"pinv": 0, "lds_solve": 0, "task_sweeps": 0},
     "dependency_driven": {"level_schedule": 0},
     "bus_ordered_rows": {"meas_by_type": 0, "xcd_remap": 0},
+    "rhs_by_location": {"rhs_by_location": 1},
 }
 
 
@@ -470,4 +471,49 @@ def test_graph_replays_match_eager_iterations(fp64):
         assert (sg, ig) == (se, ie), (step, out[0][:2], out[1][:2])
         assert np.array_equal(Vg, Ve) and np.array_equal(tg, te), step
     eg.close()
+    ee.close()
+
+
+@pytest.mark.parametrize("fp64", [True, False])
+def test_rhs_by_location_matches_entry_products(fp64):
+    """REALTIME iterations with the right-hand side formed by location (option
+    rhs_by_location) follow the per-entry form: the same iteration counts and
+    states within 1e-8 after each update, and the oracle's optimum for the
+    last telemetry."""
+    from state_estimation_with_cuda_amd import sle
+
+    g = GRIDS["geo10k"]()
+
+    def make(loc):
+        e = sle.Engine(double=True)
+        e.load_grid(g)
+        e.set_config(use_double_precision=int(fp64))
+        e.set_option("rhs_by_location", loc)
+        e.upload()
+        e.set_telemetry_f64(g.z)
+        assert e.solve(mode=1, max_iterations=50, tolerance=1e-8, flat_start=True)["status"] == 0
+        return e
+
+    el, ee = make(1), make(0)
+    rng = np.random.default_rng(33)
+    z = g.z.copy()
+    for step in range(4):
+        idx = np.sort(rng.choice(g.m, g.m // 10, replace=False)).astype(np.int32)
+        val = g.z[idx] * (1.0 + 1e-3 * rng.standard_normal(idx.size))
+        z[idx] = val
+        res = []
+        for e in (el, ee):
+            e.set_telemetry_indexed(idx, val)
+            r = e.solve(mode=0, max_iterations=10, tolerance=1e-6)
+            assert r["status"] == 0, r
+            res.append(e.state())
+        assert np.abs(res[0][0] - res[1][0]).max() < 1e-8 and np.abs(res[0][1] - res[1][1]).max() < 1e-8, step
+    g2 = gg.Grid(**{**g.__dict__})
+    g2.z = z
+    o = Oracle(g2)
+    o.solve(mode=1, max_iter=50, tol=1e-11, flat_start=True)
+    oV, oth = o.state()
+    V, th = el.state()
+    assert np.abs(V - oV).max() < STATE_TOL and np.abs(th - oth).max() < STATE_TOL
+    el.close()
     ee.close()
