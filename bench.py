@@ -75,18 +75,20 @@ def request_z(g, req):
     return z
 
 
-def algorithmic_bytes(sz, g, prec_bytes, direct=True):
+def algorithmic_bytes(sz, g, prec_bytes, direct=True, plan=None):
     """Compulsory HBM bytes per launch of each kernel class (each element once),
     for an iteration that reuses the factor.  prec_bytes is the factor/solve
-    vector width (4 mixed, 8 plain FP64).  direct: the measurement pass writes
-    the RHS products (FP64, in H^T order, through the position map) instead of
-    the row-major Jacobian (FP64) that the CG refinement reads."""
+    vector width (4 mixed, 8 plain FP64).  direct: no CG refinement.  plan: the
+    engine's solve-plan sizes (sle_ext_GetSolvePlanSizes): the products of the
+    partitioned-inverse solve and, when the right-hand side is formed by
+    location, the bytes of those two kernels."""
     n, nb, m, ns = g.n, g.nb, g.m, sz["ns"]
     nnz_y, nnz_h = sz["nnz_y"], sz["nnz_h"]
     s = prec_bytes
+    plan = plan or {}
     ybus = (n + 1) * 4 + nnz_y * (4 + 8 * 4)  # yrow, ycol, yG, yB, Pij, Qij
     jac = nnz_h * (4 + 8) if direct else nnz_h * 8
-    return {
+    out = {
         # V, th read; P, Q write
         "power": n * 8 * 4 + ybus,
         # per row: code (1 B), loc, z, sigma read, scaled residual write, hrow;
@@ -97,6 +99,28 @@ def algorithmic_bytes(sz, g, prec_bytes, direct=True):
         # update: dx read, V/theta read+write, pth/pv
         "update": ns * s + n * (8 * 4 + 8),
     }
+    if plan.get("rhs_by_location") and direct:
+        # k_meas_loc: per row code, z, sigma; row ranges; bus V/th/P/Q; branch
+        # data; per-bus and per-branch sums written
+        out["meas"] = m * (1 + 8 + 8) + 2 * (n + 1) * 4 + (2 * nb + 1) * 4 + n * 8 * 4 + nb * (4 + 4 + 8 * 5) \
+            + 32 * n + 32 * nb
+        # k_rhs_loc: Ybus rows with transpose map and per-entry terms, self G/B,
+        # bus V/P/Q and sums, incident branches and their sums, state maps, b
+        out["rhs"] = (n + 1) * 4 + nnz_y * (4 + 4 + 8 + 8) + n * 16 + n * 8 * 3 + 32 * n + (n + 1) * 4 \
+            + 2 * nb * 4 + 32 * nb + 8 * n + ns * s
+    if plan.get("subtrees"):
+        nbc = ns - plan["top_columns"]  # bottom columns
+        ent, us, nsub = plan["subtree_entries"], plan["subtree_update_rows"], plan["subtrees"]
+        # M_t once; column map; right-hand side in, solution / update rows out; headers
+        out["sub_fwd"] = s * ent + 4 * nbc + s * nbc + s * nbc + s * us + 28 * nsub
+        out["sub_bwd"] = s * ent + 4 * nbc + s * (nbc + us) + 4 * us + s * nbc + 28 * nsub
+    if plan.get("top_columns"):
+        ntc, nq, nseg = plan["top_columns"], plan["top_row_entries"], plan["segments"]
+        # row-grouped L_TT^-1 entries and their column indices, c_T once, y_T out, segments
+        out["pi_fwd"] = (s + 4) * nq + 2 * s * ntc + 28 * nseg
+        # block entries, path rows once, y_T once, per-column metadata, x_T out
+        out["pi_bwd"] = s * plan["top_block_entries"] + 4 * plan["top_path_rows"] + 2 * s * ntc + 24 * ntc
+    return out
 
 
 def measured_traffic(form, kernel):
@@ -207,6 +231,7 @@ def run_form(g, fp64, reqs, batches, warmup, steps, profile_steps, pcg_steps=0, 
     total = time.perf_counter() - t_start
     V, th = e.state()
     e_plan = e.describe()
+    plan_sizes = e.solve_plan_sizes()
     last_z = request_z(g, reqs[warmup + steps - 1])
     # profiled pass (separate, so the events do not perturb the timed loop)
     kt = None
@@ -237,6 +262,7 @@ def run_form(g, fp64, reqs, batches, warmup, steps, profile_steps, pcg_steps=0, 
         objective=objective,
         kernel_times=kt,
         sizes=sz,
+        plan_sizes=plan_sizes,
         flags=sorted(flags),
         plan=e_plan,
         autotune=tune,
@@ -435,7 +461,8 @@ def run_serve(args):
     for j, t in enumerate(t_area):
         reqs.append(serve.Request(serve.AREA, float(t), pool[j % len(pool)], rid=200_000 + j, record=j < 3))
     srv = serve.Server(main_e, engine, hbm_budget=int(args.hbm_budget_gib * (1 << 30)), area_slots=args.area_slots,
-                       bytes_per_bus=bytes_per_bus, tol=TOL, max_iterations=10)
+                       bytes_per_bus=bytes_per_bus, tol=TOL, max_iterations=10,
+                       toggle_mode=args.toggle_mode)
     t0 = time.perf_counter()
     done = serve.run(srv, reqs)
     wall = time.perf_counter() - t0
@@ -509,9 +536,10 @@ def run_serve(args):
         "vs_baseline": None,
         "dtype": "f64" if fp64 else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"Poisson {args.rate}/s updates (10% of z) for {args.duration}s, breaker toggle per 50 "
-                               f"updates, new 5k-50k-bus areas at {args.area_rate}/s within {args.hbm_budget_gib} GiB",
-                   "form": args.form},
+        "config": {"workload": f"Poisson {args.rate}/s updates (10% of z) for {args.duration}s, breaker toggle per "
+                               f"{args.toggle_every} updates, new 5k-50k-bus areas at {args.area_rate}/s within "
+                               f"{args.hbm_budget_gib} GiB",
+                   "form": args.form, "toggle_mode": "REALTIME" if args.toggle_mode == 0 else "PRECISION"},
         "latency": summ,
         "solves_per_s": round(rate, 1),
         "hbm_peak_gib": round(srv.peak_used / (1 << 30), 2),
@@ -546,6 +574,8 @@ def main():
     ap.add_argument("--hbm-budget-gib", type=float, default=64.0)
     ap.add_argument("--record-every", type=int, default=500)
     ap.add_argument("--toggle-every", type=int, default=50, help="serve: updates between breaker operations (0: none)")
+    ap.add_argument("--toggle-mode", type=int, default=0, choices=[0, 1],
+                    help="serve: solve mode after a breaker operation (1 PRECISION, 0 REALTIME)")
     ap.add_argument("--replays", type=int, default=4)
     ap.add_argument("--areas", type=int, default=256)
     ap.add_argument("--form", choices=["fp64", "mixed"], default="fp64", help="areas workload: precision form")
@@ -585,8 +615,10 @@ def main():
 
     def summary(f):
         kt = f["kernel_times"] or {}
-        ab = algorithmic_bytes(f["sizes"], g, 8 if f["name"] == "fp64" else 4,
-                               direct=f["name"] == "fp64" or args.pcg_steps == 0)
+        # the mixed form solves with an FP64 factor once it switched to one (flag 2)
+        wide = f["name"] == "fp64" or any(fl & 2 for fl in f["flags"])
+        ab = algorithmic_bytes(f["sizes"], g, 8 if wide else 4, direct=f["name"] == "fp64" or args.pcg_steps == 0,
+                               plan=f["plan_sizes"])
         return {
             "solves_per_s": round(f["solves_per_s"], 2),
             "ms_per_step": round(f["ms_per_step"], 4),
@@ -599,6 +631,7 @@ def main():
             "initial_iterations": f["initial_iterations"],
             "solve_flags": f["flags"],
             "plan": f["plan"],
+            "plan_sizes": f["plan_sizes"],
             "autotune": f["autotune"],
             "kernel_ms_per_step": {k: round(v["ms"] / max(args.profile_steps, 1), 4) for k, v in kt.items()},
             "kernel_launches_per_step": {k: round(v["launches"] / max(args.profile_steps, 1), 2)

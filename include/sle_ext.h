@@ -61,10 +61,10 @@ typedef struct SLE_ExtResult {
  *   "meas_by_type"    1 (default): the device keeps the measurement rows grouped
  *                     by evaluation class (bus values, injections, flows), each
  *                     class by anchor bus; 0: by anchor bus only
- *   "rhs_by_location" 1: iterations on a cached factor form H^T W r from the
- *                     summed weighted residuals of each bus and branch end,
- *                     gathered per state (no per-entry products); needs
- *                     "meas_by_type"; 0 (default): per Jacobian entry
+ *   "rhs_by_location" 1 (default): iterations on a cached factor form H^T W r
+ *                     from the summed weighted residuals of each bus and
+ *                     branch end, gathered per state (no per-entry products);
+ *                     needs "meas_by_type"; 0: per Jacobian entry
  *   "lds_flags"       0 (default): the fronts of a subtree task are solved
  *                     level by level with a workgroup barrier per level;
  *                     1: each wave waits on LDS flags of just the fronts it
@@ -121,12 +121,22 @@ SLE_API SLE_StatusCode SLE_CALL sle_ext_ApplyStagedTelemetry(SLE_Handle handle, 
 
 /* Per-kernel-class device times (HIP events on the engine's stream), recorded
  * while SLE_Config.enable_profiling is set.  Classes, in order: power
- * injections, fused measurement/Jacobian pass, gain SpGEMM, supernodal factor,
- * right-hand side, forward solve, backward solve, state update.  ms[c] is the
- * accumulated time, launches[c] the kernel launches, calls[c] the phase runs. */
+ * injections, measurement pass, gain SpGEMM, supernodal factor, right-hand
+ * side, forward solve, backward solve, state update, CG refinement, and inside
+ * the two solves: bottom-subtree forward product, top forward product, top
+ * backward product, bottom-subtree backward product.  ms[c] is the accumulated
+ * time, launches[c] the kernel launches, calls[c] the phase runs. */
 SLE_API SLE_StatusCode SLE_CALL sle_ext_GetKernelTimes(SLE_Handle handle, double* ms, int64_t* launches,
                                                        int64_t* calls, int32_t count);
 SLE_API SLE_StatusCode SLE_CALL sle_ext_ResetKernelTimes(SLE_Handle handle);
+
+/* Sizes of the current solve plan (count entries at most, after a solve):
+ * [0] bottom subtrees, [1] entries of their solve matrices (one copy),
+ * [2] sum of their root update rows, [3] top supernodes, [4] top columns,
+ * [5] top block entries, [6] top row entries (forward product),
+ * [7] forward segments, [8] path rows of the top, [9] 1 when the right-hand
+ * side is formed by location.  No counterpart in the reference. */
+SLE_API SLE_StatusCode SLE_CALL sle_ext_GetSolvePlanSizes(SLE_Handle handle, int64_t* sizes, int32_t count);
 
 /* One-line description of the engine's current execution plan (precision
  * form, factor/solve sweep schedule); valid until the next call on the thread. */

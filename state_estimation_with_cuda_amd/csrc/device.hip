@@ -483,8 +483,6 @@ struct LocArgs {
     const int32_t* bv_ptr;  // per device bus: its V / theta rows
     const int32_t* bi_ptr;  // per device bus: its injection rows
     const int32_t* br_ptr;  // per device branch end (2k + end): its flow rows
-    const double* P;
-    const double* Q;
     const double* V;
     const double* th;
     const int32_t* from;
@@ -496,6 +494,16 @@ struct LocArgs {
     const double* shift;
     double* sbus;  // per bus: sum of w r over its V, theta, P, Q rows
     double* sbr;   // per branch: sum of w r d over its flow rows, for theta_f, theta_t, V_f, V_t
+    // the injections, computed by the bus threads first (k_power's work, fused)
+    const int32_t* yrow;
+    const int32_t* ycol;
+    const double* yG;
+    const double* yB;
+    double* Pout;
+    double* Qout;
+    double* Pij;
+    double* Qij;
+    DevStatus* st;
 };
 
 __device__ inline double loc_wr(const LocArgs& A, int i, double hv) {
@@ -512,8 +520,29 @@ __device__ inline double loc_wr(const LocArgs& A, int i, double hv) {
 
 __global__ __launch_bounds__(kBlock) void k_meas_loc(LocArgs A) {
     const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t == 0) {  // the iteration's status starts here (as in k_power)
+        A.st->nan = 0;
+        A.st->piv = 0;
+        A.st->flow_err = 0;
+    }
     if (t < A.n) {
         const int b = t;
+        // injections of bus b and the per-entry terms of its Ybus row
+        const double vi = A.V[b], ti = A.th[b];
+        double pb = 0.0, qb = 0.0;
+        for (int e = A.yrow[b]; e < A.yrow[b + 1]; ++e) {
+            const int j = A.ycol[e];
+            double sn, cs;
+            sincos(ti - A.th[j], &sn, &cs);
+            const double vv = vi * A.V[j], G = A.yG[e], Bv = A.yB[e];
+            const double pe = vv * (G * cs + Bv * sn), qe = vv * (G * sn - Bv * cs);
+            A.Pij[e] = pe;
+            A.Qij[e] = qe;
+            pb += pe;
+            qb += qe;
+        }
+        A.Pout[b] = pb;
+        A.Qout[b] = qb;
         double sV = 0.0, sT = 0.0, sP = 0.0, sQ = 0.0;
         for (int i = A.bv_ptr[b]; i < A.bv_ptr[b + 1]; ++i) {
             if ((A.mcode[i] & 15) == 0)
@@ -524,9 +553,9 @@ __global__ __launch_bounds__(kBlock) void k_meas_loc(LocArgs A) {
         for (int i = A.bi_ptr[b]; i < A.bi_ptr[b + 1]; ++i) {
             const int ty = A.mcode[i] & 15;
             if (ty == 2 || ty == 7)
-                sP += loc_wr(A, i, A.P[b]);
+                sP += loc_wr(A, i, pb);
             else
-                sQ += loc_wr(A, i, A.Q[b]);
+                sQ += loc_wr(A, i, qb);
         }
         double* o = A.sbus + 4 * (int64_t)b;
         o[0] = sV;
@@ -2750,6 +2779,7 @@ struct Instance::Impl {
         DBuf<int32_t> sn, plen, ploc, tcol, colsn, src, segptr, cptr, parent, segrow, mrows, cplen, cjj;
         DBuf<int64_t> boff, poff, pos, segq, cpos, cboff, cpoff;
         int nmrows = 0;
+        int64_t usum = 0, npath = 0;  // sum of subtree root update rows; path rows of the top
         DBuf<double> part;
         LdsDev bottom[2];  // bottom-subtree LDS tasks, [0] float, [1] double
         // dense bottom subtrees
@@ -2775,7 +2805,7 @@ struct Instance::Impl {
     bool pi_dense = true;   // bottom subtrees as dense solve matrices (else LDS tasks)
     bool meas_by_type = true;  // measurement rows grouped by evaluation class (else by anchor bus only)
     // right-hand side by location (needs the class grouping)
-    bool rhs_loc = false;  // asked for
+    bool rhs_loc = true;  // asked for
     bool use_loc = false;  // in use (asked for, and the rows are grouped by class)
     DBuf<int32_t> bv_ptr, bi_ptr, br_ptr, ytr, inc_ptr, inc;
     DBuf<double> sbus, sbr;
@@ -2889,6 +2919,9 @@ static void build_pi(Instance::Impl& D, const Symbolic& S, size_t& bytes, hipStr
             up(Q.src, PP.src);
             up(Q.segptr, PP.segptr);
             up(Q.segrow, PP.segrow);
+            Q.npath = (int64_t)PP.ploc.size();
+            Q.usum = 0;
+            for (int t = 0; t < PP.nsub; ++t) Q.usum += PP.sR[t] - PP.sn_n[t];
             up(Q.cplen, PP.cplen);
             up(Q.cjj, PP.cjj);
             up64(Q.cboff, PP.cboff);
@@ -3673,8 +3706,6 @@ void launch_meas_loc(Instance::Impl& D, int robust, double gamma) {
     A.bv_ptr = D.bv_ptr.p;
     A.bi_ptr = D.bi_ptr.p;
     A.br_ptr = D.br_ptr.p;
-    A.P = D.P.p;
-    A.Q = D.Q.p;
     A.V = D.V.p;
     A.th = D.th.p;
     A.from = D.from.p;
@@ -3686,6 +3717,15 @@ void launch_meas_loc(Instance::Impl& D, int robust, double gamma) {
     A.shift = D.shift.p;
     A.sbus = D.sbus.p;
     A.sbr = D.sbr.p;
+    A.yrow = D.yrow.p;
+    A.ycol = D.ycol.p;
+    A.yG = D.yG.p;
+    A.yB = D.yB.p;
+    A.Pout = D.P.p;
+    A.Qout = D.Q.p;
+    A.Pij = D.Pij.p;
+    A.Qij = D.Qij.p;
+    A.st = D.st.p;
     const int64_t nt = (int64_t)D.n + D.nb;
     if (nt) k_meas_loc<<<nblk(nt), kBlock, 0, D.stream>>>(A);
 }
@@ -3913,25 +3953,36 @@ void launch_trisolve(Instance::Impl& D, Prof* pf = nullptr) {
         const int nw = kBlock / kWave;
         const bool dense = sub_active(D);
         prof_begin(pf, PH_FWD);
-        if (dense)
+        if (dense) {
+            prof_begin(pf, PH_SUB_FWD);
             k_sub_fwd<T><<<(PI.nsub + nw - 1) / nw, kBlock, 0, D.stream>>>(sub_args(D), A, B.msub.p);
-        else if (PB.n_bottom)
+            prof_end(pf, PH_SUB_FWD, 1);
+        } else if (PB.n_bottom) {
             launch_fwd_lds<T>(D, PB.n_bottom, D.pi_waves, PB.bottom_bytes, LA, A, 0);
+        }
         if (PI.ntc) {
             k_pi_assemble<T><<<nblk(PI.ntc), kBlock, 0, D.stream>>>(PI.ntc, PI.tcol.p, PI.cptr.p, PI.cpos.p, A.x,
                                                                    A.ubuf, B.ctop.p);
+            prof_begin(pf, PH_PI_FWD);
             k_pi_fwd<T><<<(PI.nseg + nw - 1) / nw, kBlock, 0, D.stream>>>(
                 PI.nseg, PI.segq.p, PI.segrow.p, PI.segptr.p, PI.src.p, B.pinv_rows.p, B.ctop.p, PI.part.p, B.ytop.p);
+            prof_end(pf, PH_PI_FWD, 1);
             if (PI.nmrows)
                 k_pi_rowsum<T><<<(PI.nmrows + nw - 1) / nw, kBlock, 0, D.stream>>>(PI.nmrows, PI.mrows.p,
                                                                                    PI.segptr.p, PI.part.p, B.ytop.p);
         }
         prof_end(pf, PH_FWD, (dense || PB.n_bottom ? 1 : 0) + (PI.ntc ? 2 + (PI.nmrows ? 1 : 0) : 0));
         prof_begin(pf, PH_BWD);
-        if (PI.ntc)
+        if (PI.ntc) {
+            prof_begin(pf, PH_PI_BWD);
             k_pi_bwd<T><<<(PI.ntc + nw - 1) / nw, kBlock, 0, D.stream>>>(pi_args(D), A, B.pinv.p, B.ytop.p);
-        if (dense)
+            prof_end(pf, PH_PI_BWD, 1);
+        }
+        if (dense) {
+            prof_begin(pf, PH_SUB_BWD);
             k_sub_bwd<T><<<(PI.nsub + nw - 1) / nw, kBlock, 0, D.stream>>>(sub_args(D), A, B.msub.p + PI.msub);
+            prof_end(pf, PH_SUB_BWD, 1);
+        }
         else if (PB.n_bottom)
             launch_bwd_lds<T>(D, PB.n_bottom, D.pi_waves, PB.bottom_bytes, LA, A, 0);
         prof_end(pf, PH_BWD, (dense || PB.n_bottom ? 1 : 0) + (PI.ntc ? 1 : 0));
@@ -4123,14 +4174,17 @@ void prof_end(Prof* pf, int ph, int launches) {
 template <typename T>
 void iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p, bool refactor, T piv_tol, Prof& pf,
                bool newton) {
-    pf.begin(PH_POWER);
-    launch_power(D);
-    pf.end(PH_POWER, 1);
-    pf.begin(PH_MEAS);
     // row-major Jacobian only where G or the CG product needs it; the RHS
-    // products always (by location when no Jacobian values are needed)
+    // products always (by location when no Jacobian values are needed: the
+    // injections are then computed inside k_meas_loc)
     const bool direct = p.pcg_steps <= 0;
     const bool loc = D.use_loc && !refactor && direct;
+    if (!loc) {
+        pf.begin(PH_POWER);
+        launch_power(D);
+        pf.end(PH_POWER, 1);
+    }
+    pf.begin(PH_MEAS);
     if (loc)
         launch_meas_loc(D, p.robust ? 1 : 0, p.gamma);
     else
@@ -4579,6 +4633,17 @@ void Instance::set_lds_solve(bool on) {
 void Instance::set_dense_subtrees(bool on) {
     ++d_->plan_epoch;
     d_->pi_dense = on;
+}
+
+void Instance::solve_plan_sizes(int64_t* out, int count) const {
+    const Impl& D = *d_;
+    const Impl::PiDev& P = D.pi;
+    const bool on = D.use_pi && P.ok;
+    const int64_t v[10] = {on && P.dense ? P.nsub : 0, on && P.dense ? P.msub : 0, on && P.dense ? P.usum : 0,
+                           on ? P.ntop : 0,           on ? P.ntc : 0,            on ? P.nblock : 0,
+                           on ? P.nq : 0,             on ? P.nseg : 0,           on ? P.npath : 0,
+                           D.use_loc ? 1 : 0};
+    for (int i = 0; i < count && i < 10; ++i) out[i] = v[i];
 }
 
 void Instance::set_rhs_by_location(bool on) {

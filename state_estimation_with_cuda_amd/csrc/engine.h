@@ -94,7 +94,7 @@ struct EngineOptions {
     bool pinv_dense = true;       // partitioned inverse: bottom subtrees as dense solve matrices
     bool xcd_remap = true;        // XCD-aware block order of the gather kernels
     bool meas_by_type = true;     // measurement rows grouped by evaluation class on the device
-    bool rhs_by_location = false; // cached-factor iterations: RHS from per-location sums
+    bool rhs_by_location = true;  // cached-factor iterations: RHS from per-location sums
 };
 
 class Engine {

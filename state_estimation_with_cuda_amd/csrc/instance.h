@@ -62,7 +62,24 @@ struct SolveOutcome {
 };
 
 // Kernel classes timed with HIP events on the instance stream (profiling mode).
-enum Phase : int { PH_POWER = 0, PH_MEAS, PH_SPGEMM, PH_FACTOR, PH_RHS, PH_FWD, PH_BWD, PH_UPDATE, PH_PCG, PH_COUNT };
+// PH_FWD / PH_BWD span a whole solve direction; PH_SUB_* and PH_PI_* time the
+// bottom-subtree and top products inside them (partitioned-inverse solve)
+enum Phase : int {
+    PH_POWER = 0,
+    PH_MEAS,
+    PH_SPGEMM,
+    PH_FACTOR,
+    PH_RHS,
+    PH_FWD,
+    PH_BWD,
+    PH_UPDATE,
+    PH_PCG,
+    PH_SUB_FWD,
+    PH_PI_FWD,
+    PH_PI_BWD,
+    PH_SUB_BWD,
+    PH_COUNT
+};
 
 struct KernelTimes {
     double ms[PH_COUNT] = {0};       // accumulated device time between the phase's first launch and last completion
@@ -154,6 +171,12 @@ class Instance {
     // (summed weighted residuals per bus and branch end, gathered per state)
     // instead of per Jacobian entry (takes effect at setup; needs the class order)
     void set_rhs_by_location(bool on);
+    // sizes of the current solve plan: [0] bottom subtrees, [1] entries of their
+    // solve matrices (one copy), [2] sum of their root update rows, [3] top
+    // supernodes, [4] top columns, [5] top block entries, [6] top row entries
+    // (forward product), [7] forward segments, [8] path rows of the top
+    // (sum of P_s), [9] 1 if the right-hand side is formed by location
+    void solve_plan_sizes(int64_t* out, int count) const;
     bool lds_flags() const;
     std::string sweep_plan() const;
     // Autotuner of the solve plan: times the forward + backward solves of the

@@ -976,6 +976,15 @@ SLE_API SLE_StatusCode SLE_CALL sle_ext_GetKernelTimes(SLE_Handle handle, double
     return SLE_OK;
 }
 
+SLE_API SLE_StatusCode SLE_CALL sle_ext_GetSolvePlanSizes(SLE_Handle handle, int64_t* sizes, int32_t count) {
+    if (!handle) return fail(SLE_ERROR_INVALID_HANDLE, "NULL handle");
+    if (!sizes || count < 0) return fail(SLE_ERROR_INVALID_ARGUMENT, "NULL sizes");
+    for (int32_t i = 0; i < count; ++i) sizes[i] = 0;
+    sle::Instance* inst = E(handle).instance();
+    if (inst) inst->solve_plan_sizes(sizes, count);
+    return SLE_OK;
+}
+
 SLE_API SLE_StatusCode SLE_CALL sle_ext_ResetKernelTimes(SLE_Handle handle) {
     if (!handle) return fail(SLE_ERROR_INVALID_HANDLE, "NULL handle");
     if (E(handle).instance()) E(handle).instance()->reset_kernel_times();

@@ -61,7 +61,7 @@ class Server:
 
     def __init__(self, main, make_area_engine: Callable, hbm_budget: int, area_slots: int = 4,
                  bytes_per_bus: float = 16384.0, tol: float = 1e-6, max_iterations: int = 10,
-                 clock: Callable[[], float] = time.perf_counter):
+                 clock: Callable[[], float] = time.perf_counter, toggle_mode: int = 0):
         self.main = main
         self.make_area_engine = make_area_engine
         self.budget = int(hbm_budget)
@@ -70,6 +70,10 @@ class Server:
         self.tol = tol
         self.max_iterations = max_iterations
         self.clock = clock
+        # solve mode after a breaker operation: 1 PRECISION (refactor every
+        # step), 0 REALTIME (the invalidated factor is rebuilt once, then reused
+        # and refreshed only if it contracts poorly)
+        self.toggle_mode = toggle_mode
         self._cv = threading.Condition()
         self._main_q: collections.deque = collections.deque()
         self._area_q: collections.deque = collections.deque()
@@ -143,9 +147,9 @@ class Server:
                         self.main.set_telemetry_indexed(req.payload[2], req.payload[3])
                     self.main.set_branch_status(k, st)
                     self._breakers[k] = st
-                # a breaker operation moves the optimum: refactor at every step of that
-                # solve (PRECISION); updates reuse the cached factor (REALTIME)
-                mode = 0 if req.kind == UPDATE else 1
+                # updates reuse the cached factor (REALTIME); a breaker operation
+                # moves the optimum and invalidates the factor (toggle_mode)
+                mode = 0 if req.kind == UPDATE else self.toggle_mode
                 r = self.main.solve(mode=mode, max_iterations=self.max_iterations, tolerance=self.tol)
                 req.status, req.iterations = r["status"], r["iterations"]
                 if req.record:

@@ -143,9 +143,12 @@ EXT_FUNCTIONS = [
     "sle_ext_GetHPattern", "sle_ext_GetGPattern", "sle_ext_GetOrdering", "sle_ext_Evaluate",
     "sle_ext_JacobianGain", "sle_ext_Factor", "sle_ext_SolveFactor", "sle_ext_StageTelemetry",
     "sle_ext_ApplyStagedTelemetry", "sle_ext_GetKernelTimes", "sle_ext_ResetKernelTimes", "sle_ext_SetOption",
-    "sle_ext_Describe", "sle_ext_AutotuneSolve",
+    "sle_ext_Describe", "sle_ext_AutotuneSolve", "sle_ext_GetSolvePlanSizes",
 ]
-KERNEL_CLASSES = ["power", "meas", "spgemm", "factor", "rhs", "fwd", "bwd", "update", "pcg"]
+KERNEL_CLASSES = ["power", "meas", "spgemm", "factor", "rhs", "fwd", "bwd", "update", "pcg",
+                  "sub_fwd", "pi_fwd", "pi_bwd", "sub_bwd"]
+PLAN_SIZES = ["subtrees", "subtree_entries", "subtree_update_rows", "top_supernodes", "top_columns",
+              "top_block_entries", "top_row_entries", "segments", "top_path_rows", "rhs_by_location"]
 
 
 class Library:
@@ -270,6 +273,7 @@ class Library:
             "sle_ext_SetOption": (C.c_int, [H, C.c_char_p, C.c_double]),
             "sle_ext_Describe": (C.c_char_p, [H]),
             "sle_ext_AutotuneSolve": (C.c_int, [H, i32]),
+            "sle_ext_GetSolvePlanSizes": (C.c_int, [H, lp, i32]),
         }
         missing = [n for n in API_FUNCTIONS + EXT_FUNCTIONS if n not in sig]
         assert not missing, missing
@@ -550,6 +554,11 @@ class Engine:
         ca = np.zeros(n, np.int64)
         self._check(self.L.sle_ext_GetKernelTimes(self.h, ms, la, ca, n), "sle_ext_GetKernelTimes")
         return {k: dict(ms=float(ms[i]), launches=int(la[i]), calls=int(ca[i])) for i, k in enumerate(KERNEL_CLASSES)}
+
+    def solve_plan_sizes(self):
+        out = np.zeros(len(PLAN_SIZES), np.int64)
+        self._check(self.L.sle_ext_GetSolvePlanSizes(self.h, out, len(PLAN_SIZES)), "sle_ext_GetSolvePlanSizes")
+        return {k: int(out[i]) for i, k in enumerate(PLAN_SIZES)}
 
     def reset_kernel_times(self):
         self._check(self.L.sle_ext_ResetKernelTimes(self.h), "sle_ext_ResetKernelTimes")
