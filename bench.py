@@ -100,20 +100,32 @@ def algorithmic_bytes(sz, g, prec_bytes, direct=True, plan=None):
         "update": ns * s + n * (8 * 4 + 8),
     }
     if plan.get("rhs_by_location") and direct:
-        # k_meas_loc: per row code, z, sigma; row ranges; bus V/th/P/Q; branch
+        # k_meas_loc (with the injections): Ybus rows, per-entry terms written;
+        # bus V/th read, P/Q written; per row code, z, sigma; row ranges; branch
         # data; per-bus and per-branch sums written
-        out["meas"] = m * (1 + 8 + 8) + 2 * (n + 1) * 4 + (2 * nb + 1) * 4 + n * 8 * 4 + nb * (4 + 4 + 8 * 5) \
-            + 32 * n + 32 * nb
-        # k_rhs_loc: Ybus rows with transpose map and per-entry terms, self G/B,
-        # bus V/P/Q and sums, incident branches and their sums, state maps, b
+        out["meas"] = (n + 1) * 4 + nnz_y * (4 + 8 + 8 + 8 + 8) + n * 8 * 4 + m * (1 + 8 + 8) \
+            + 2 * (n + 1) * 4 + (2 * nb + 1) * 4 + nb * (4 + 4 + 8 * 5) + 32 * n + 32 * nb
+        out["power"] = 0  # fused into the measurement pass on this path
+        # k_rhs_loc: Ybus rows with the transpose map and per-entry terms, self
+        # G/B, bus V/P/Q and sums, incident branches and their sums, state maps, b
         out["rhs"] = (n + 1) * 4 + nnz_y * (4 + 4 + 8 + 8) + n * 16 + n * 8 * 3 + 32 * n + (n + 1) * 4 \
             + 2 * nb * 4 + 32 * nb + 8 * n + ns * s
     if plan.get("subtrees"):
-        nbc = ns - plan["top_columns"]  # bottom columns
-        ent, us, nsub = plan["subtree_entries"], plan["subtree_update_rows"], plan["subtrees"]
-        # M_t once; column map; right-hand side in, solution / update rows out; headers
-        out["sub_fwd"] = s * ent + 4 * nbc + s * nbc + s * nbc + s * us + 28 * nsub
-        out["sub_bwd"] = s * ent + 4 * nbc + s * (nbc + us) + 4 * us + s * nbc + 28 * nsub
+        def sub(ent, us, nsub, nc):
+            # M_t once; column map; right-hand side in, solution / update rows out; headers
+            fwd = s * ent + 4 * nc + s * nc + s * nc + s * us + 28 * nsub
+            bwd = s * ent + 4 * nc + s * (nc + us) + 4 * us + s * nc + 28 * nsub
+            return fwd, bwd
+        nc_all = ns - plan["top_columns"]  # columns of all dense subtrees
+        out["sub_fwd"], out["sub_bwd"] = sub(plan["bottom_entries"], plan["bottom_update_rows"],
+                                             plan["bottom_subtrees"], plan["bottom_columns"])
+        nsub1 = plan["subtrees"] - plan["bottom_subtrees"]
+        if nsub1:
+            nc1 = nc_all - plan["bottom_columns"]
+            f1, b1 = sub(plan["subtree_entries"] - plan["bottom_entries"],
+                         plan["subtree_update_rows"] - plan["bottom_update_rows"], nsub1, nc1)
+            out["mid_fwd"] = f1 + nc1 * (2 * s + 8)  # with the in-place assembly of the middle columns
+            out["mid_bwd"] = b1
     if plan.get("top_columns"):
         ntc, nq, nseg = plan["top_columns"], plan["top_row_entries"], plan["segments"]
         # row-grouped L_TT^-1 entries and their column indices, c_T once, y_T out, segments
@@ -155,7 +167,9 @@ def roofline_for(form, kt, ab):
     if not cand:
         return None
     dom = max(cand, key=lambda k: cand[k]["ms"])
-    avg_ms = cand[dom]["ms"] / cand[dom]["launches"]
+    # the byte models are per run of a class (one launch, but for the middle
+    # forward class two: assembly and product)
+    avg_ms = cand[dom]["ms"] / max(cand[dom]["calls"], 1)
     achieved = ab[dom] / (avg_ms * 1e-3) / 1e9
     return dict(kernel=dom, bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(form, dom), bytes_per_launch=int(ab[dom]),
@@ -636,7 +650,8 @@ def main():
             "kernel_ms_per_step": {k: round(v["ms"] / max(args.profile_steps, 1), 4) for k, v in kt.items()},
             "kernel_launches_per_step": {k: round(v["launches"] / max(args.profile_steps, 1), 2)
                                          for k, v in kt.items()},
-            "roofline": roofline_for(f["name"], kt, ab),
+            # the mixed form on the FP64 factor runs the FP64 kernels: their traffic
+            "roofline": roofline_for("fp64" if wide else f["name"], kt, ab),
         }
 
     out = {

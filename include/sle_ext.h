@@ -55,6 +55,10 @@ typedef struct SLE_ExtResult {
  *                     dense solve matrix [L_BB^-1; -L_UB L_BB^-1] made after
  *                     every factorization, one product per subtree and
  *                     direction; 0: LDS tasks walk the subtree's fronts
+ *   "pinv_mid"        1 (default, with "pinv_dense"): the forest left above the
+ *                     bottom subtrees is cut the same way once more (middle
+ *                     subtrees, same limits), so that only the rest is solved
+ *                     through L_TT^-1 blocks; 0: two levels
  *   "xcd_remap"       1 (default): the measurement, power, right-hand-side and
  *                     top-product kernels run contiguous block ranges per XCD
  *                     (process-wide on the device)
@@ -124,7 +128,8 @@ SLE_API SLE_StatusCode SLE_CALL sle_ext_ApplyStagedTelemetry(SLE_Handle handle, 
  * injections, measurement pass, gain SpGEMM, supernodal factor, right-hand
  * side, forward solve, backward solve, state update, CG refinement, and inside
  * the two solves: bottom-subtree forward product, top forward product, top
- * backward product, bottom-subtree backward product.  ms[c] is the accumulated
+ * backward product, bottom-subtree backward product, middle-subtree forward
+ * (assembly and product) and backward products.  ms[c] is the accumulated
  * time, launches[c] the kernel launches, calls[c] the phase runs. */
 SLE_API SLE_StatusCode SLE_CALL sle_ext_GetKernelTimes(SLE_Handle handle, double* ms, int64_t* launches,
                                                        int64_t* calls, int32_t count);
@@ -135,7 +140,9 @@ SLE_API SLE_StatusCode SLE_CALL sle_ext_ResetKernelTimes(SLE_Handle handle);
  * [2] sum of their root update rows, [3] top supernodes, [4] top columns,
  * [5] top block entries, [6] top row entries (forward product),
  * [7] forward segments, [8] path rows of the top, [9] 1 when the right-hand
- * side is formed by location.  No counterpart in the reference. */
+ * side is formed by location; [0]-[2] cover bottom and middle subtrees, and
+ * [10]-[13] the bottom ones alone: count, entries, update rows, columns.  No
+ * counterpart in the reference. */
 SLE_API SLE_StatusCode SLE_CALL sle_ext_GetSolvePlanSizes(SLE_Handle handle, int64_t* sizes, int32_t count);
 
 /* One-line description of the engine's current execution plan (precision

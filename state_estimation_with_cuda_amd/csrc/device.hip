@@ -1595,6 +1595,8 @@ struct SubArgs {
     const int32_t* cptr;   // columns of subtree t: cols[cptr[t] .. cptr[t] + n[t])
     const int32_t* cols;
     const int64_t* moff;   // start of M_t
+    const int32_t* acptr;  // per subtree column: the update positions that add into its right-hand side
+    const int64_t* acpos;
 };
 
 __host__ __device__ inline int64_t sub_col_off(int j, int R) { return (int64_t)j * R - (int64_t)j * (j - 1) / 2; }
@@ -1709,19 +1711,22 @@ constexpr int kSubBatch = 8;  // independent loads in flight per lane (16 measur
 
 // Forward: one wave per subtree; lane i accumulates rows i and i + 64 over the
 // columns of M_t, kSubBatch coalesced column loads at a time.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_sub_fwd(SubArgs B, SolveArgs<T> A, const T* __restrict__ M) {
+// LEVEL 0: bottom subtrees, 1: middle subtrees (one instantiation each, so
+// that profiles tell the two launches apart)
+template <typename T, int LEVEL>
+__global__ __launch_bounds__(kBlock) void k_sub_fwd(SubArgs B, SolveArgs<T> A, const T* __restrict__ M, int t0,
+                                                   int t1) {
     __shared__ double bsh[kBlock / kWave][kSubCols];
     const int w = threadIdx.x / kWave;
-    const int t = blockIdx.x * (kBlock / kWave) + w;
+    const int t = t0 + blockIdx.x * (kBlock / kWave) + w;
     const int lane = threadIdx.x & (kWave - 1);
-    if (t >= B.nsub) return;  // uniform per wave
+    if (t >= t1) return;  // uniform per wave
     const int n = B.n[t], R = B.R[t];
     const int32_t* cols = B.cols + B.cptr[t];
     const T* Mt = M + B.moff[t];
     double m0[kSubBatch], m1[kSubBatch];
     // column loads of batch j0 (they need only the header: the first batch is
-    // in flight while the right-hand side is gathered)
+    // in flight while the right-hand side is gathered); rows lane and lane + 64
     auto load = [&](int j0) {
 #pragma unroll
         for (int u = 0; u < kSubBatch; ++u) {
@@ -1733,7 +1738,14 @@ __global__ __launch_bounds__(kBlock) void k_sub_fwd(SubArgs B, SolveArgs<T> A, c
     };
     load(0);
     double* bs = bsh[w];
-    if (lane < n) bs[lane] = (double)A.x[cols[lane]];
+    if (lane < n) {
+        double b = (double)A.x[cols[lane]];
+        if (LEVEL == 1) {  // a middle subtree: the bottom roots' update rows add in first
+            const int c = B.cptr[t] + lane;
+            for (int q = B.acptr[c]; q < B.acptr[c + 1]; ++q) b += (double)A.ubuf[B.acpos[q]];
+        }
+        bs[lane] = b;
+    }
     wave_sync();
     double a0 = 0.0, a1 = 0.0;
     for (int j0 = 0; j0 < n; j0 += kSubBatch) {
@@ -1755,13 +1767,14 @@ __global__ __launch_bounds__(kBlock) void k_sub_fwd(SubArgs B, SolveArgs<T> A, c
 
 // Backward: one wave per subtree; lane j accumulates column j over the rows of
 // the row-major copy, kSubBatch coalesced row loads at a time.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_sub_bwd(SubArgs B, SolveArgs<T> A, const T* __restrict__ Mrow) {
+template <typename T, int LEVEL>
+__global__ __launch_bounds__(kBlock) void k_sub_bwd(SubArgs B, SolveArgs<T> A, const T* __restrict__ Mrow, int t0,
+                                                   int t1) {
     __shared__ double vsh[kBlock / kWave][kSubRows];
     const int w = threadIdx.x / kWave;
-    const int t = blockIdx.x * (kBlock / kWave) + w;
+    const int t = t0 + blockIdx.x * (kBlock / kWave) + w;
     const int lane = threadIdx.x & (kWave - 1);
-    if (t >= B.nsub) return;  // uniform per wave
+    if (t >= t1) return;  // uniform per wave
     const int n = B.n[t], R = B.R[t], root = B.root[t];
     const int32_t* cols = B.cols + B.cptr[t];
     const int32_t* urow = A.rows + A.rows_off[root] + A.sn_k[root];
@@ -2495,9 +2508,13 @@ struct PiPlan {
     std::vector<int64_t> boff, poff, pos, segq, cpos, cboff, cpoff;
     // dense bottom subtrees (see SubArgs)
     bool dense = false;
-    int nsub = 0;
+    int nsub = 0, nsub0 = 0;  // subtrees; of them bottom ones (listed first)
     int64_t msub = 0;
     std::vector<int32_t> sroot, sn_n, sR, fptr, fsn, fdcol, drow, scptr, scols, bins;
+    // middle columns; and per subtree column (in scols order) the bottom roots'
+    // update positions that add into it (empty for bottom subtree columns)
+    std::vector<int32_t> mcol, acptr;
+    std::vector<int64_t> acpos;
     std::vector<int64_t> fdoff, smoff;
     int bin_off[kSubBins + 1] = {0, 0, 0, 0, 0};
     size_t bin_bytes[kSubBins] = {0, 0, 0, 0};
@@ -2507,21 +2524,26 @@ struct PiPlan {
 // outside the top whose parent is in the top, or a root of the tree) list its
 // fronts children first, its columns in that order, and the dense row of every
 // front row (a subtree column, or a root update row placed after the columns).
-static void make_sub_plan(const Symbolic& S, const std::vector<int32_t>& tsl, PiPlan& P) {
+static void make_sub_plan(const Symbolic& S, const std::vector<int8_t>& lev, PiPlan& P) {
     const int nsn = S.nsn;
     std::vector<int32_t> dpos(S.ns, -1), stack, fronts;
     P.fptr.push_back(0);
     P.scptr.push_back(0);
+    // the bottom subtrees (level 0) first, then the middle ones (level 1): a
+    // subtree is its root's descendants of the same level
+    for (int level = 0; level < 2; ++level)
     for (int r = 0; r < nsn; ++r) {
         const int par = S.sn_parent[r];
-        if (tsl[r] >= 0 || (par >= 0 && tsl[par] < 0)) continue;
+        if (lev[r] != level || (par >= 0 && lev[par] == level)) continue;
+        if (level == 0) P.nsub0 = (int)P.sroot.size() + 1;
         fronts.clear();
         stack.assign(1, r);
         while (!stack.empty()) {
             const int s = stack.back();
             stack.pop_back();
             fronts.push_back(s);
-            for (int q = S.sn_child_ptr[s]; q < S.sn_child_ptr[s + 1]; ++q) stack.push_back(S.sn_child[q]);
+            for (int q = S.sn_child_ptr[s]; q < S.sn_child_ptr[s + 1]; ++q)
+                if (lev[S.sn_child[q]] == level) stack.push_back(S.sn_child[q]);
         }
         std::sort(fronts.begin(), fronts.end());  // children carry smaller ids than their parents
         int n = 0;
@@ -2580,7 +2602,7 @@ static void make_sub_plan(const Symbolic& S, const std::vector<int32_t>& tsl, Pi
 constexpr int kPiSeg = kPiSegMax;  // entries per forward-product segment
 constexpr int kPiCutDefault = 32;  // fronts per bottom subtree
 
-static PiPlan make_pi_plan(const Symbolic& S, int cut, bool dense) {
+static PiPlan make_pi_plan(const Symbolic& S, int cut, bool dense, bool mid) {
     PiPlan P;
     P.cut = cut;
     const int nsn = S.nsn;
@@ -2596,10 +2618,33 @@ static PiPlan make_pi_plan(const Symbolic& S, int cut, bool dense) {
         if (size[s] > cut) return true;
         return dense && (ncol[s] > kSubCols || ncol[s] + S.sn_c[s] - S.sn_k[s] > kSubRows);
     };
-    std::vector<int32_t> tloc(S.ns, -1), tsl(nsn, -1);
+    // level of each front: 0 bottom subtree, 1 middle subtree (dense subtrees
+    // of the forest left above the bottom ones, same limits), 2 top
+    std::vector<int8_t> lev(nsn, 0);
+    for (int s = 0; s < nsn; ++s) lev[s] = in_top(s) ? 2 : 0;
+    if (dense && mid) {
+        std::vector<int32_t> nc2(nsn, 0);
+        for (int s = 0; s < nsn; ++s) {
+            if (lev[s] != 2) continue;
+            nc2[s] += S.sn_k[s];
+            const int p = S.sn_parent[s];
+            if (p >= 0 && lev[p] == 2) nc2[p] += nc2[s];
+        }
+        // both bounds grow towards the root: the middle subtrees are closed
+        // downwards inside the top forest, what stays at level 2 upwards
+        for (int s = 0; s < nsn; ++s)
+            if (lev[s] == 2 && nc2[s] <= kSubCols && nc2[s] + S.sn_c[s] - S.sn_k[s] <= kSubRows) lev[s] = 1;
+    }
+    std::vector<int32_t> tloc(S.ns, -1), tsl(nsn, -1), mloc(S.ns, -1);
+    for (int s = 0; s < nsn; ++s)
+        if (lev[s] == 1)
+            for (int j = 0; j < S.sn_k[s]; ++j) {
+                mloc[S.sn_first[s] + j] = (int32_t)P.mcol.size();
+                P.mcol.push_back(S.sn_first[s] + j);
+            }
     int cmax = 1, kmax = 1;
     for (int s = 0; s < nsn; ++s) {
-        if (!in_top(s)) continue;
+        if (lev[s] != 2) continue;
         tsl[s] = (int32_t)P.sn.size();
         P.sn.push_back(s);
         for (int j = 0; j < S.sn_k[s]; ++j) {
@@ -2667,16 +2712,21 @@ static PiPlan make_pi_plan(const Symbolic& S, int cut, bool dense) {
     }
     P.nseg = (int)P.segq.size();
     P.segq.push_back(P.nq);
-    // assembly: update rows of the bottom subtree roots, per top column
-    std::vector<std::vector<int64_t>> per(P.ntc);
+    // assembly: the update rows of every subtree root (bottom or middle), per
+    // middle column and per top column
+    std::vector<std::vector<int64_t>> per(P.ntc), perm(P.mcol.size());
     for (int s = 0; s < nsn; ++s) {
         const int par = S.sn_parent[s];
-        if (tsl[s] >= 0 || par < 0 || tsl[par] < 0) continue;
+        if (lev[s] == 2 || par < 0 || lev[par] == lev[s]) continue;  // not a subtree root with a parent
         const int c = S.sn_c[s], k = S.sn_k[s];
         for (int a = 0; a < c - k; ++a) {
-            const int t = tloc[S.sn_rows[S.sn_rows_off[s] + k + a]];
-            if (t < 0) throw std::logic_error("partitioned inverse: update row outside the top");
-            per[t].push_back(S.sn_uoff[s] + a);
+            const int row = S.sn_rows[S.sn_rows_off[s] + k + a];
+            if (tloc[row] >= 0)
+                per[tloc[row]].push_back(S.sn_uoff[s] + a);
+            else if (lev[s] == 0 && mloc[row] >= 0)
+                perm[mloc[row]].push_back(S.sn_uoff[s] + a);
+            else
+                throw std::logic_error("partitioned inverse: update row below its subtree's level");
         }
     }
     P.cptr.assign(P.ntc + 1, 0);
@@ -2684,7 +2734,15 @@ static PiPlan make_pi_plan(const Symbolic& S, int cut, bool dense) {
         P.cpos.insert(P.cpos.end(), per[t].begin(), per[t].end());
         P.cptr[t + 1] = (int32_t)P.cpos.size();
     }
-    if (dense) make_sub_plan(S, tsl, P);
+    if (dense) {
+        make_sub_plan(S, lev, P);
+        P.acptr.assign(P.scols.size() + 1, 0);
+        for (size_t c = 0; c < P.scols.size(); ++c) {
+            const int ml = mloc[P.scols[c]];
+            if (ml >= 0) P.acpos.insert(P.acpos.end(), perm[ml].begin(), perm[ml].end());
+            P.acptr[c + 1] = (int32_t)P.acpos.size();
+        }
+    }
     return P;
 }
 
@@ -2777,14 +2835,17 @@ struct Instance::Impl {
         int cut = 0, ntop = 0, ntc = 0, nseg = 0, slice = 0;
         int64_t nblock = 0, nq = 0;
         DBuf<int32_t> sn, plen, ploc, tcol, colsn, src, segptr, cptr, parent, segrow, mrows, cplen, cjj;
-        DBuf<int64_t> boff, poff, pos, segq, cpos, cboff, cpoff;
         int nmrows = 0;
+        DBuf<int64_t> boff, poff, pos, segq, cpos, cboff, cpoff;
         int64_t usum = 0, npath = 0;  // sum of subtree root update rows; path rows of the top
+        int64_t msub0 = 0, usum0 = 0, ncol0 = 0;  // of the bottom subtrees: entries, update rows, columns
         DBuf<double> part;
         LdsDev bottom[2];  // bottom-subtree LDS tasks, [0] float, [1] double
         // dense bottom subtrees
         bool dense = false;
-        int nsub = 0;
+        int nsub = 0, nsub0 = 0;  // subtrees (bottom ones first)
+        DBuf<int32_t> acptr;
+        DBuf<int64_t> acpos;
         int64_t msub = 0;
         DBuf<int32_t> sroot, sn_n, sR, fptr, fsn, fdcol, drow, scptr, scols, bins;
         DBuf<int64_t> fdoff, smoff;
@@ -2803,6 +2864,7 @@ struct Instance::Impl {
     } pi;
     bool use_pi = true;
     bool pi_dense = true;   // bottom subtrees as dense solve matrices (else LDS tasks)
+    bool pi_mid = true;     // a second level of dense subtrees between the bottom ones and the top
     bool meas_by_type = true;  // measurement rows grouped by evaluation class (else by anchor bus only)
     // right-hand side by location (needs the class grouping)
     bool rhs_loc = true;  // asked for
@@ -2887,7 +2949,7 @@ static void build_pi(Instance::Impl& D, const Symbolic& S, size_t& bytes, hipStr
                                            D.lds_waves_tail, true, LP[e]);
         }
         PiPlan PP;
-        if (fit) PP = make_pi_plan(S, cut, dense);
+        if (fit) PP = make_pi_plan(S, cut, dense, D.pi_mid);
         if (fit && 3 * (size_t)PP.slice * sizeof(double) <= 65536 && PP.nblock < ((int64_t)1 << 40)) {
             auto up = [&](DBuf<int32_t>& b, std::vector<int32_t> v) {
                 if (v.empty()) v.push_back(0);
@@ -2919,15 +2981,22 @@ static void build_pi(Instance::Impl& D, const Symbolic& S, size_t& bytes, hipStr
             up(Q.src, PP.src);
             up(Q.segptr, PP.segptr);
             up(Q.segrow, PP.segrow);
+            up(Q.mrows, PP.mrows);
+            Q.nmrows = (int)PP.mrows.size();
             Q.npath = (int64_t)PP.ploc.size();
-            Q.usum = 0;
-            for (int t = 0; t < PP.nsub; ++t) Q.usum += PP.sR[t] - PP.sn_n[t];
+            Q.usum = Q.usum0 = Q.ncol0 = 0;
+            for (int t = 0; t < PP.nsub; ++t) {
+                Q.usum += PP.sR[t] - PP.sn_n[t];
+                if (t < PP.nsub0) {
+                    Q.usum0 += PP.sR[t] - PP.sn_n[t];
+                    Q.ncol0 += PP.sn_n[t];
+                }
+            }
+            Q.msub0 = PP.nsub0 < PP.nsub ? PP.smoff[PP.nsub0] : PP.msub;
             up(Q.cplen, PP.cplen);
             up(Q.cjj, PP.cjj);
             up64(Q.cboff, PP.cboff);
             up64(Q.cpoff, PP.cpoff);
-            up(Q.mrows, PP.mrows);
-            Q.nmrows = (int)PP.mrows.size();
             up(Q.cptr, PP.cptr);
             up(Q.parent, PP.parent);
             up64(Q.boff, PP.boff);
@@ -2958,7 +3027,10 @@ static void build_pi(Instance::Impl& D, const Symbolic& S, size_t& bytes, hipStr
                 up64(Q.fdoff, PP.fdoff);
                 up64(Q.smoff, PP.smoff);
                 Q.nsub = PP.nsub;
+                Q.nsub0 = PP.nsub0;
                 Q.msub = PP.msub;
+                up(Q.acptr, PP.acptr);
+                up64(Q.acpos, PP.acpos);
                 for (int q = 0; q <= kSubBins; ++q) Q.bin_off[q] = PP.bin_off[q];
                 for (int q = 0; q < kSubBins; ++q) Q.bin_bytes[q] = PP.bin_bytes[q];
             }
@@ -3026,7 +3098,7 @@ Instance::~Instance() {
         rel(P.segrow); rel(P.mrows); rel(P.cplen); rel(P.cjj); rel(P.cboff); rel(P.cpoff);
         rel(P.boff); rel(P.poff); rel(P.pos); rel(P.segq); rel(P.cpos); rel(P.part);
         rel(P.sroot); rel(P.sn_n); rel(P.sR); rel(P.fptr); rel(P.fsn); rel(P.fdcol); rel(P.drow); rel(P.scptr);
-        rel(P.scols); rel(P.bins); rel(P.fdoff); rel(P.smoff);
+        rel(P.scols); rel(P.bins); rel(P.fdoff); rel(P.smoff); rel(P.acptr); rel(P.acpos);
         for (auto& L : P.bottom) {
             rel(L.th); rel(L.meta); rel(L.lvl); rel(L.ch); rel(L.rm); rel(L.src); rel(L.mid);
         }
@@ -3867,8 +3939,8 @@ bool pi_active(const Instance::Impl& D) { return D.use_pi && D.pi.ok; }
 
 SubArgs sub_args(const Instance::Impl& D) {
     const Instance::Impl::PiDev& P = D.pi;
-    return SubArgs{P.nsub,   P.msub,    P.sroot.p, P.sn_n.p,  P.sR.p,    P.fptr.p, P.fsn.p,
-                   P.fdcol.p, P.fdoff.p, P.drow.p, P.scptr.p, P.scols.p, P.smoff.p};
+    return SubArgs{P.nsub,   P.msub,    P.sroot.p, P.sn_n.p,  P.sR.p,    P.fptr.p,  P.fsn.p, P.fdcol.p,
+                   P.fdoff.p, P.drow.p, P.scptr.p, P.scols.p, P.smoff.p, P.acptr.p, P.acpos.p};
 }
 
 bool sub_active(const Instance::Impl& D) { return pi_active(D) && D.pi.dense && D.pi.nsub > 0; }
@@ -3954,9 +4026,18 @@ void launch_trisolve(Instance::Impl& D, Prof* pf = nullptr) {
         const bool dense = sub_active(D);
         prof_begin(pf, PH_FWD);
         if (dense) {
-            prof_begin(pf, PH_SUB_FWD);
-            k_sub_fwd<T><<<(PI.nsub + nw - 1) / nw, kBlock, 0, D.stream>>>(sub_args(D), A, B.msub.p);
-            prof_end(pf, PH_SUB_FWD, 1);
+            // bottom subtrees, then the middle ones on the assembled right-hand side
+            const int n0 = PI.nsub0, n1 = PI.nsub - PI.nsub0;
+            if (n0) {
+                prof_begin(pf, PH_SUB_FWD);
+                k_sub_fwd<T, 0><<<(n0 + nw - 1) / nw, kBlock, 0, D.stream>>>(sub_args(D), A, B.msub.p, 0, n0);
+                prof_end(pf, PH_SUB_FWD, 1);
+            }
+            if (n1) {
+                prof_begin(pf, PH_MID_FWD);
+                k_sub_fwd<T, 1><<<(n1 + nw - 1) / nw, kBlock, 0, D.stream>>>(sub_args(D), A, B.msub.p, n0, PI.nsub);
+                prof_end(pf, PH_MID_FWD, 1);
+            }
         } else if (PB.n_bottom) {
             launch_fwd_lds<T>(D, PB.n_bottom, D.pi_waves, PB.bottom_bytes, LA, A, 0);
         }
@@ -3979,9 +4060,20 @@ void launch_trisolve(Instance::Impl& D, Prof* pf = nullptr) {
             prof_end(pf, PH_PI_BWD, 1);
         }
         if (dense) {
-            prof_begin(pf, PH_SUB_BWD);
-            k_sub_bwd<T><<<(PI.nsub + nw - 1) / nw, kBlock, 0, D.stream>>>(sub_args(D), A, B.msub.p + PI.msub);
-            prof_end(pf, PH_SUB_BWD, 1);
+            // middle subtrees (their update rows are top columns), then the bottom ones
+            const int n0 = PI.nsub0, n1 = PI.nsub - PI.nsub0;
+            if (n1) {
+                prof_begin(pf, PH_MID_BWD);
+                k_sub_bwd<T, 1><<<(n1 + nw - 1) / nw, kBlock, 0, D.stream>>>(sub_args(D), A, B.msub.p + PI.msub, n0,
+                                                                             PI.nsub);
+                prof_end(pf, PH_MID_BWD, 1);
+            }
+            if (n0) {
+                prof_begin(pf, PH_SUB_BWD);
+                k_sub_bwd<T, 0><<<(n0 + nw - 1) / nw, kBlock, 0, D.stream>>>(sub_args(D), A, B.msub.p + PI.msub, 0,
+                                                                             n0);
+                prof_end(pf, PH_SUB_BWD, 1);
+            }
         }
         else if (PB.n_bottom)
             launch_bwd_lds<T>(D, PB.n_bottom, D.pi_waves, PB.bottom_bytes, LA, A, 0);
@@ -4635,15 +4727,20 @@ void Instance::set_dense_subtrees(bool on) {
     d_->pi_dense = on;
 }
 
+void Instance::set_middle_subtrees(bool on) {
+    ++d_->plan_epoch;
+    d_->pi_mid = on;
+}
+
 void Instance::solve_plan_sizes(int64_t* out, int count) const {
     const Impl& D = *d_;
     const Impl::PiDev& P = D.pi;
     const bool on = D.use_pi && P.ok;
-    const int64_t v[10] = {on && P.dense ? P.nsub : 0, on && P.dense ? P.msub : 0, on && P.dense ? P.usum : 0,
-                           on ? P.ntop : 0,           on ? P.ntc : 0,            on ? P.nblock : 0,
-                           on ? P.nq : 0,             on ? P.nseg : 0,           on ? P.npath : 0,
-                           D.use_loc ? 1 : 0};
-    for (int i = 0; i < count && i < 10; ++i) out[i] = v[i];
+    const bool d = on && P.dense;
+    const int64_t v[14] = {d ? P.nsub : 0,    d ? P.msub : 0, d ? P.usum : 0, on ? P.ntop : 0,  on ? P.ntc : 0,
+                           on ? P.nblock : 0, on ? P.nq : 0,  on ? P.nseg : 0, on ? P.npath : 0, D.use_loc ? 1 : 0,
+                           d ? P.nsub0 : 0,   d ? P.msub0 : 0, d ? P.usum0 : 0, d ? P.ncol0 : 0};
+    for (int i = 0; i < count && i < 14; ++i) out[i] = v[i];
 }
 
 void Instance::set_rhs_by_location(bool on) {
@@ -4703,10 +4800,10 @@ std::string Instance::sweep_plan() const {
         const Impl::LdsDev& PB = D.pi.bottom[precision_ == 1 || fp64_factor_active_ ? 1 : 0];
         if (D.pi.dense)
             snprintf(buf, sizeof(buf),
-                     "; solve partitioned-inverse(cut=%d dense-subtrees=%d/%lld entries top supernodes=%d "
+                     "; solve partitioned-inverse(cut=%d dense-subtrees=%d+%d/%lld entries top supernodes=%d "
                      "columns=%d blocks=%lld row-entries=%lld segments=%d)",
-                     D.pi.cut, D.pi.nsub, (long long)D.pi.msub, D.pi.ntop, D.pi.ntc, (long long)D.pi.nblock,
-                     (long long)D.pi.nq, D.pi.nseg);
+                     D.pi.cut, D.pi.nsub0, D.pi.nsub - D.pi.nsub0, (long long)D.pi.msub, D.pi.ntop, D.pi.ntc,
+                     (long long)D.pi.nblock, (long long)D.pi.nq, D.pi.nseg);
         else
             snprintf(buf, sizeof(buf),
                      "; solve partitioned-inverse(cut=%d bottom=%d/%zuB%s top supernodes=%d columns=%d blocks=%lld "

@@ -178,6 +178,9 @@ int Engine::set_option(const char* key, double value) {
     } else if (k == "xcd_remap") {
         opt_.xcd_remap = value != 0;
         if (inst_) inst_->set_xcd_remap(opt_.xcd_remap);
+    } else if (k == "pinv_mid") {
+        opt_.pinv_mid = value != 0;
+        model_dirty_ = true;  // the plan is rebuilt at the next upload
     } else if (k == "pinv_dense") {
         opt_.pinv_dense = value != 0;
         model_dirty_ = true;  // the plan is rebuilt at the next upload
@@ -605,6 +608,7 @@ int Engine::rebuild() {
     inst_->set_partitioned_inverse(opt_.pinv, opt_.pinv_cut, opt_.pinv_waves);
     inst_->set_lds_flags(opt_.lds_flags);
     inst_->set_dense_subtrees(opt_.pinv_dense);
+    inst_->set_middle_subtrees(opt_.pinv_mid);
     inst_->set_meas_by_type(opt_.meas_by_type);
     inst_->set_rhs_by_location(opt_.rhs_by_location);
     inst_->setup(h, sym_);

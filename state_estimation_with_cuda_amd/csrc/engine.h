@@ -92,6 +92,7 @@ struct EngineOptions {
     int pinv_waves = 4;           // waves per bottom-subtree workgroup under it
     bool lds_flags = false;       // task solves: per-front LDS flags instead of per-level barriers
     bool pinv_dense = true;       // partitioned inverse: bottom subtrees as dense solve matrices
+    bool pinv_mid = true;         // and a middle level of dense subtrees under a smaller top
     bool xcd_remap = true;        // XCD-aware block order of the gather kernels
     bool meas_by_type = true;     // measurement rows grouped by evaluation class on the device
     bool rhs_by_location = true;  // cached-factor iterations: RHS from per-location sums

@@ -78,6 +78,8 @@ enum Phase : int {
     PH_PI_FWD,
     PH_PI_BWD,
     PH_SUB_BWD,
+    PH_MID_FWD,
+    PH_MID_BWD,
     PH_COUNT
 };
 
@@ -162,6 +164,9 @@ class Instance {
     // partitioned-inverse solve: bottom subtrees as one dense solve matrix each
     // (true) or as LDS tasks walking their fronts (takes effect at setup)
     void set_dense_subtrees(bool on);
+    // with dense subtrees: a second level of them inside the forest above the
+    // bottom ones, leaving a smaller top to the partitioned inverse (setup)
+    void set_middle_subtrees(bool on);
     // XCD-aware block order of the gather kernels (process-wide on this device)
     void set_xcd_remap(bool on);
     // device order of the measurement rows: grouped by evaluation class, each
@@ -175,7 +180,9 @@ class Instance {
     // solve matrices (one copy), [2] sum of their root update rows, [3] top
     // supernodes, [4] top columns, [5] top block entries, [6] top row entries
     // (forward product), [7] forward segments, [8] path rows of the top
-    // (sum of P_s), [9] 1 if the right-hand side is formed by location
+    // (sum of P_s), [9] 1 if the right-hand side is formed by location, and of
+    // the bottom subtrees alone: [10] count, [11] entries, [12] update rows,
+    // [13] columns
     void solve_plan_sizes(int64_t* out, int count) const;
     bool lds_flags() const;
     std::string sweep_plan() const;

@@ -146,9 +146,10 @@ EXT_FUNCTIONS = [
     "sle_ext_Describe", "sle_ext_AutotuneSolve", "sle_ext_GetSolvePlanSizes",
 ]
 KERNEL_CLASSES = ["power", "meas", "spgemm", "factor", "rhs", "fwd", "bwd", "update", "pcg",
-                  "sub_fwd", "pi_fwd", "pi_bwd", "sub_bwd"]
+                  "sub_fwd", "pi_fwd", "pi_bwd", "sub_bwd", "mid_fwd", "mid_bwd"]
 PLAN_SIZES = ["subtrees", "subtree_entries", "subtree_update_rows", "top_supernodes", "top_columns",
-              "top_block_entries", "top_row_entries", "segments", "top_path_rows", "rhs_by_location"]
+              "top_block_entries", "top_row_entries", "segments", "top_path_rows", "rhs_by_location",
+              "bottom_subtrees", "bottom_entries", "bottom_update_rows", "bottom_columns"]
 
 
 class Library:

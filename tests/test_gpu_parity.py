@@ -366,6 +366,7 @@ PLAN_OPTIONS = {
     "dependency_driven": {"level_schedule": 0},
     "bus_ordered_rows": {"meas_by_type": 0, "xcd_remap": 0},
     "rhs_by_location": {"rhs_by_location": 1},
+    "two_levels": {"pinv_mid": 0},
 }
 
 
