@@ -506,20 +506,52 @@ struct LocArgs {
     DevStatus* st;
 };
 
-__device__ inline double loc_wr(const LocArgs& A, int i, double hv) {
-    if (!((A.mcode[i] >> 5) & 1)) return 0.0;  // inactive: weight 0
-    const double r = A.z[i] - hv;
-    const double sg = A.sigma[i];
-    double w = 1.0 / (sg * sg);
-    if (A.robust) {
-        const double ar = fabs(r);
-        if (ar / sg > A.gamma) w = A.gamma / (sg * ar);
+// w r of one row from its code, value and sigma at the evaluated h
+struct LocWeight {
+    int robust;
+    double gamma;
+    __device__ double operator()(uint8_t code, double z, double sg, double hv) const {
+        if (!((code >> 5) & 1)) return 0.0;  // inactive: weight 0
+        const double r = z - hv;
+        double w = 1.0 / (sg * sg);
+        if (robust) {
+            const double ar = fabs(r);
+            if (ar / sg > gamma) w = gamma / (sg * ar);
+        }
+        return w * r;
     }
-    return w * r;
+};
+
+// Entries per chunk of a thread's Ybus-row and measurement-row walks: the loads
+// of a chunk are issued together (indices clamped to the last entry), so a walk
+// waits for memory once per chunk rather than once per entry.
+constexpr int kLocChunk = 4;
+
+template <typename F>
+__device__ inline void loc_rows(const uint8_t* __restrict__ mcode, const double* __restrict__ zv,
+                                const double* __restrict__ sigma, int i0, int i1, F f) {
+    for (int i = i0; i < i1; i += kLocChunk) {
+        uint8_t c[kLocChunk];
+        double z[kLocChunk], sg[kLocChunk];
+#pragma unroll
+        for (int q = 0; q < kLocChunk; ++q) {
+            const int ii = min(i + q, i1 - 1);
+            c[q] = mcode[ii];
+            z[q] = zv[ii];
+            sg[q] = sigma[ii];
+        }
+#pragma unroll
+        for (int q = 0; q < kLocChunk; ++q)
+            if (i + q < i1) f(c[q], z[q], sg[q]);
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_meas_loc(LocArgs A) {
     const int t = blockIdx.x * kBlock + threadIdx.x;
+    const LocWeight wr_of{A.robust, A.gamma};
+    const uint8_t* mcode = A.mcode;
+    const double* zv = A.z;
+    const double* sigma = A.sigma;
     if (t == 0) {  // the iteration's status starts here (as in k_power)
         A.st->nan = 0;
         A.st->piv = 0;
@@ -530,33 +562,60 @@ __global__ __launch_bounds__(kBlock) void k_meas_loc(LocArgs A) {
         // injections of bus b and the per-entry terms of its Ybus row
         const double vi = A.V[b], ti = A.th[b];
         double pb = 0.0, qb = 0.0;
-        for (int e = A.yrow[b]; e < A.yrow[b + 1]; ++e) {
-            const int j = A.ycol[e];
-            double sn, cs;
-            sincos(ti - A.th[j], &sn, &cs);
-            const double vv = vi * A.V[j], G = A.yG[e], Bv = A.yB[e];
-            const double pe = vv * (G * cs + Bv * sn), qe = vv * (G * sn - Bv * cs);
-            A.Pij[e] = pe;
-            A.Qij[e] = qe;
-            pb += pe;
-            qb += qe;
+        const int e1 = A.yrow[b + 1];  // every row holds its diagonal entry
+        for (int e = A.yrow[b]; e < e1; e += kLocChunk) {
+            int jj[kLocChunk];
+            double G[kLocChunk], Bq[kLocChunk], Vq[kLocChunk], tq[kLocChunk];
+#pragma unroll
+            for (int q = 0; q < kLocChunk; ++q) {
+                const int ee = min(e + q, e1 - 1);
+                jj[q] = A.ycol[ee];
+                G[q] = A.yG[ee];
+                Bq[q] = A.yB[ee];
+            }
+#pragma unroll
+            for (int q = 0; q < kLocChunk; ++q) {
+                Vq[q] = A.V[jj[q]];
+                tq[q] = A.th[jj[q]];
+            }
+            // one sincos in the loop body: the chunk shifts down a register per entry
+            const int cnt = min(kLocChunk, e1 - e);
+            for (int q = 0; q < cnt; ++q) {
+                double sn, cs;
+                sincos(ti - tq[0], &sn, &cs);
+                const double vv = vi * Vq[0];
+                const double pe = vv * (G[0] * cs + Bq[0] * sn), qe = vv * (G[0] * sn - Bq[0] * cs);
+                A.Pij[e + q] = pe;
+                A.Qij[e + q] = qe;
+                pb += pe;
+                qb += qe;
+#pragma unroll
+                for (int k = 0; k + 1 < kLocChunk; ++k) {
+                    G[k] = G[k + 1];
+                    Bq[k] = Bq[k + 1];
+                    Vq[k] = Vq[k + 1];
+                    tq[k] = tq[k + 1];
+                }
+            }
         }
         A.Pout[b] = pb;
         A.Qout[b] = qb;
         double sV = 0.0, sT = 0.0, sP = 0.0, sQ = 0.0;
-        for (int i = A.bv_ptr[b]; i < A.bv_ptr[b + 1]; ++i) {
-            if ((A.mcode[i] & 15) == 0)
-                sV += loc_wr(A, i, A.V[b]);
-            else
-                sT += loc_wr(A, i, A.th[b]);
-        }
-        for (int i = A.bi_ptr[b]; i < A.bi_ptr[b + 1]; ++i) {
-            const int ty = A.mcode[i] & 15;
-            if (ty == 2 || ty == 7)
-                sP += loc_wr(A, i, pb);
-            else
-                sQ += loc_wr(A, i, qb);
-        }
+        // (selects rather than branches: a branch per sum lets the compiler
+        // address the sums through a pointer, which moves them to scratch)
+        loc_rows(mcode, zv, sigma, A.bv_ptr[b], A.bv_ptr[b + 1], [&](uint8_t c, double z, double sg) {
+            const bool isv = (c & 15) == 0;
+            const double wr = wr_of(c, z, sg, isv ? vi : ti);
+            sV += isv ? wr : 0.0;
+            sT += isv ? 0.0 : wr;
+        });
+        loc_rows(mcode, zv, sigma, A.bi_ptr[b], A.bi_ptr[b + 1], [&](uint8_t c, double z, double sg) {
+            const int ty = c & 15;
+            const bool isp = ty == 2 || ty == 7;
+            const double wr = wr_of(c, z, sg, isp ? pb : qb);
+            sP += isp ? wr : 0.0;
+            sQ += isp ? 0.0 : wr;
+        });
         double* o = A.sbus + 4 * (int64_t)b;
         o[0] = sV;
         o[1] = sT;
@@ -574,54 +633,55 @@ __global__ __launch_bounds__(kBlock) void k_meas_loc(LocArgs A) {
             for (int end = 0; end < 2; ++end) {
                 const int i0 = A.br_ptr[2 * l + end], i1 = A.br_ptr[2 * l + end + 1];
                 if (i1 == i0) continue;
-                double p, q, dP[4], dQ[4];
-                if (end == 0) {
-                    p = Vi * Vi * g / (a * a) - AA * (g * c + b * s);
-                    q = -Vi * Vi * bsh / (a * a) - AA * (g * s - b * c);
-                    dP[0] = AA * (g * s - b * c);
-                    dP[1] = -dP[0];
-                    dP[2] = 2.0 * Vi * g / (a * a) - (Vj / a) * (g * c + b * s);
-                    dP[3] = -(Vi / a) * (g * c + b * s);
-                    dQ[0] = -AA * (g * c + b * s);
-                    dQ[1] = -dQ[0];
-                    dQ[2] = -2.0 * Vi * bsh / (a * a) - (Vj / a) * (g * s - b * c);
-                    dQ[3] = -(Vi / a) * (g * s - b * c);
-                } else {
-                    p = Vj * Vj * g - AA * (g * c - b * s);
-                    q = -Vj * Vj * bsh + AA * (g * s + b * c);
-                    dP[0] = AA * (g * s + b * c);
-                    dP[1] = -dP[0];
-                    dP[2] = -(Vj / a) * (g * c - b * s);
-                    dP[3] = 2.0 * Vj * g - (Vi / a) * (g * c - b * s);
-                    dQ[0] = AA * (g * c - b * s);
-                    dQ[1] = -dQ[0];
-                    dQ[2] = (Vj / a) * (g * s + b * c);
-                    dQ[3] = -2.0 * Vj * bsh + (Vi / a) * (g * s + b * c);
-                }
-                // the current-magnitude partials, formed only if such a row exists
+                // the flow at this end; its partials enter only through the row sums
+                // cP = sum of w r over P rows (and I rows times p / (|S| V)), cQ
+                // alike, cV = the I rows' extra V_end term
+                const double cgb = end ? g * c - b * s : g * c + b * s;
+                const double sgb = end ? g * s + b * c : g * s - b * c;
+                const double p = end ? Vj * Vj * g - AA * cgb : Vi * Vi * g / (a * a) - AA * cgb;
+                const double q = end ? -Vj * Vj * bsh + AA * sgb : -Vi * Vi * bsh / (a * a) - AA * sgb;
                 const double ve = end ? Vj : Vi;
-                const double S = sqrt(p * p + q * q);
-                const double inv = S > 1e-12 ? 1.0 / (S * ve) : 0.0;
-                for (int i = i0; i < i1; ++i) {
-                    const int ty = A.mcode[i] & 15;
-                    if (ty == 4) {
-                        const double wr = loc_wr(A, i, p);
-#pragma unroll
-                        for (int k = 0; k < 4; ++k) acc[k] += wr * dP[k];
-                    } else if (ty == 5) {
-                        const double wr = loc_wr(A, i, q);
-#pragma unroll
-                        for (int k = 0; k < 4; ++k) acc[k] += wr * dQ[k];
+                double cP = 0.0, cQ = 0.0, cV = 0.0;
+                loc_rows(mcode, zv, sigma, i0, i1, [&](uint8_t cd, double z, double sg) {
+                    const int ty = cd & 15;
+                    if (ty == 4 || ty == 5) {
+                        const double wr = wr_of(cd, z, sg, ty == 4 ? p : q);
+                        cP += ty == 4 ? wr : 0.0;
+                        cQ += ty == 4 ? 0.0 : wr;
                     } else {  // I_MAG = |S| / V_end
-                        const double wr = loc_wr(A, i, S / ve);
-                        double d[4];
-#pragma unroll
-                        for (int k = 0; k < 4; ++k) d[k] = (p * dP[k] + q * dQ[k]) * inv;
-                        d[end ? 3 : 2] -= S / (ve * ve);
-#pragma unroll
-                        for (int k = 0; k < 4; ++k) acc[k] += wr * d[k];
+                        const double S = sqrt(p * p + q * q);
+                        const double inv = S > 1e-12 ? 1.0 / (S * ve) : 0.0;
+                        const double wr = wr_of(cd, z, sg, S / ve);
+                        cP += wr * p * inv;
+                        cQ += wr * q * inv;
+                        cV -= wr * S / (ve * ve);
                     }
+                });
+                // partials wrt (theta_f, theta_t, V_f, V_t)
+                double dP[4], dQ[4];
+                if (end == 0) {
+                    dP[0] = AA * sgb;
+                    dP[2] = 2.0 * Vi * g / (a * a) - (Vj / a) * cgb;
+                    dP[3] = -(Vi / a) * cgb;
+                    dQ[0] = -AA * cgb;
+                    dQ[2] = -2.0 * Vi * bsh / (a * a) - (Vj / a) * sgb;
+                    dQ[3] = -(Vi / a) * sgb;
+                } else {
+                    dP[0] = AA * sgb;
+                    dP[2] = -(Vj / a) * cgb;
+                    dP[3] = 2.0 * Vj * g - (Vi / a) * cgb;
+                    dQ[0] = AA * cgb;
+                    dQ[2] = (Vj / a) * sgb;
+                    dQ[3] = -2.0 * Vj * bsh + (Vi / a) * sgb;
                 }
+                dP[1] = -dP[0];
+                dQ[1] = -dQ[0];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) acc[k] += cP * dP[k] + cQ * dQ[k];
+                if (end)  // constant indices keep acc in registers
+                    acc[3] += cV;
+                else
+                    acc[2] += cV;
             }
         }
         double* o = A.sbr + 4 * (int64_t)l;
@@ -633,7 +693,6 @@ __global__ __launch_bounds__(kBlock) void k_meas_loc(LocArgs A) {
 template <typename TO>
 __global__ __launch_bounds__(kBlock) void k_rhs_loc(int n, const int32_t* __restrict__ yrow,
                                                     const int32_t* __restrict__ ycol, const int32_t* __restrict__ ytr,
-                                                    const double* __restrict__ yG, const double* __restrict__ yB,
                                                     const double* __restrict__ Pij, const double* __restrict__ Qij,
                                                     const double* __restrict__ P, const double* __restrict__ Q,
                                                     const double* __restrict__ V, const double* __restrict__ sbus,
@@ -642,32 +701,57 @@ __global__ __launch_bounds__(kBlock) void k_rhs_loc(int n, const int32_t* __rest
                                                     const int32_t* __restrict__ pv, TO* __restrict__ x) {
     const int j = blockIdx.x * kBlock + threadIdx.x;
     if (j >= n) return;
-    const double Vj = V[j];
+    const double Vj = V[j], Pj = P[j], Qj = Q[j];
     double at = sbus[4 * (int64_t)j + 1], av = sbus[4 * (int64_t)j];
-    for (int e = yrow[j]; e < yrow[j + 1]; ++e) {
-        const int l = ycol[e];
-        const double sp = sbus[4 * (int64_t)l + 2], sq = sbus[4 * (int64_t)l + 3];
-        if (l == j) {
-            const double G = yG[e], Bv = yB[e];
-            at += sp * (-Q[j] - Vj * Vj * Bv) + sq * (P[j] - Vj * Vj * G);
-            av += sp * (P[j] / Vj + Vj * G) + sq * (Q[j] / Vj - Vj * Bv);
-        } else {
-            // partials of bus l's injections wrt this bus: the terms of entry (l, j)
-            const int e2 = ytr[e];
-            const double pe = Pij[e2], qe = Qij[e2];
-            at += sp * qe - sq * pe;
-            av += (sp * pe + sq * qe) / Vj;
+    // the Ybus row in chunks (see kLocChunk); the diagonal's own terms are
+    // Vj^2 G and -Vj^2 B, the self-admittance parts of the injection partials
+    const int e1 = yrow[j + 1];
+    for (int e = yrow[j]; e < e1; e += kLocChunk) {
+        int l[kLocChunk], e2[kLocChunk];
+#pragma unroll
+        for (int q = 0; q < kLocChunk; ++q) {
+            const int ee = min(e + q, e1 - 1);
+            l[q] = ycol[ee];
+            e2[q] = ytr[ee];
+        }
+        double sp[kLocChunk], sq[kLocChunk], pe[kLocChunk], qe[kLocChunk];
+#pragma unroll
+        for (int q = 0; q < kLocChunk; ++q) {
+            sp[q] = sbus[4 * (int64_t)l[q] + 2];
+            sq[q] = sbus[4 * (int64_t)l[q] + 3];
+            pe[q] = Pij[e2[q]];
+            qe[q] = Qij[e2[q]];
+        }
+#pragma unroll
+        for (int q = 0; q < kLocChunk; ++q) {
+            if (e + q >= e1) break;
+            if (l[q] == j) {
+                at += sp[q] * (qe[q] - Qj) + sq[q] * (Pj - pe[q]);
+                av += sp[q] * ((Pj + pe[q]) / Vj) + sq[q] * ((Qj + qe[q]) / Vj);
+            } else {
+                // partials of bus l's injections wrt this bus: the terms of entry (l, j)
+                at += sp[q] * qe[q] - sq[q] * pe[q];
+                av += (sp[q] * pe[q] + sq[q] * qe[q]) / Vj;
+            }
         }
     }
-    for (int cix = inc_ptr[j]; cix < inc_ptr[j + 1]; ++cix) {
-        const int code = inc[cix];
-        const double* o = sbr + 4 * (int64_t)(code >> 1);
-        if (code & 1) {  // this bus is the branch's to end
-            at += o[1];
-            av += o[3];
-        } else {
-            at += o[0];
-            av += o[2];
+    const int c1 = inc_ptr[j + 1];
+    for (int c = inc_ptr[j]; c < c1; c += kLocChunk) {
+        int code[kLocChunk];
+#pragma unroll
+        for (int q = 0; q < kLocChunk; ++q) code[q] = inc[min(c + q, c1 - 1)];
+        double o0[kLocChunk], o1[kLocChunk];
+#pragma unroll
+        for (int q = 0; q < kLocChunk; ++q) {  // (theta, V) sums of this bus's end of the branch
+            const double* o = sbr + 4 * (int64_t)(code[q] >> 1) + (code[q] & 1);
+            o0[q] = o[0];
+            o1[q] = o[2];
+        }
+#pragma unroll
+        for (int q = 0; q < kLocChunk; ++q) {
+            if (c + q >= c1) break;
+            at += o0[q];
+            av += o1[q];
         }
     }
     if (pth[j] >= 0) x[pth[j]] = (TO)at;
@@ -1597,6 +1681,8 @@ struct SubArgs {
     const int64_t* moff;   // start of M_t
     const int32_t* acptr;  // per subtree column: the update positions that add into its right-hand side
     const int64_t* acpos;
+    const int64_t* uoff;  // per subtree: its root's update rows in the update buffer
+    const int64_t* urow;  // and their row indices in the front row lists
 };
 
 __host__ __device__ inline int64_t sub_col_off(int j, int R) { return (int64_t)j * R - (int64_t)j * (j - 1) / 2; }
@@ -1605,6 +1691,13 @@ __host__ __device__ inline int64_t sub_col_off(int j, int R) { return (int64_t)j
 // product; it sits msub entries after the column-major one.
 __host__ __device__ inline int64_t sub_row_off(int i, int n) {
     return i <= n ? (int64_t)i * (i + 1) / 2 : (int64_t)n * (n + 1) / 2 + (int64_t)(i - n) * n;
+}
+
+// Load at a 32-bit element offset from a wave-uniform base: the address stays a
+// scalar base plus one 32-bit register instead of a 64-bit pair per load.
+template <typename T>
+__device__ inline T ld_off(const T* base, uint32_t i) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (size_t)(i * (uint32_t)sizeof(T)));
 }
 
 // After each factorization: M_t by column-oriented forward substitution of
@@ -1707,57 +1800,84 @@ __global__ __launch_bounds__(kBlock) void k_sub_m_rows(SubArgs B, SolveArgs<T> A
     }
 }
 
-constexpr int kSubBatch = 8;  // independent loads in flight per lane (16 measured slower)
+constexpr int kSubBatch = 8;    // column loads in flight per lane and row (16 measured slower)
+constexpr int kSubBatchB = 16;  // row loads in flight per lane, backward
 
 // Forward: one wave per subtree; lane i accumulates rows i and i + 64 over the
-// columns of M_t, kSubBatch coalesced column loads at a time.
+// columns of M_t, kSubBatch coalesced column loads at a time (2 kSubBatch when
+// every row fits one lane).
 // LEVEL 0: bottom subtrees, 1: middle subtrees (one instantiation each, so
 // that profiles tell the two launches apart)
 template <typename T, int LEVEL>
 __global__ __launch_bounds__(kBlock) void k_sub_fwd(SubArgs B, SolveArgs<T> A, const T* __restrict__ M, int t0,
                                                    int t1) {
     __shared__ double bsh[kBlock / kWave][kSubCols];
-    const int w = threadIdx.x / kWave;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int t = t0 + blockIdx.x * (kBlock / kWave) + w;
     const int lane = threadIdx.x & (kWave - 1);
     if (t >= t1) return;  // uniform per wave
     const int n = B.n[t], R = B.R[t];
     const int32_t* cols = B.cols + B.cptr[t];
     const T* Mt = M + B.moff[t];
-    double m0[kSubBatch], m1[kSubBatch];
-    // column loads of batch j0 (they need only the header: the first batch is
-    // in flight while the right-hand side is gathered); rows lane and lane + 64
-    auto load = [&](int j0) {
-#pragma unroll
-        for (int u = 0; u < kSubBatch; ++u) {
-            const int j = j0 + u;
-            const T* col = Mt + sub_col_off(j, R) - j;  // entry (i, j) at col[i]
-            m0[u] = (j < n && lane >= j && lane < R) ? (double)col[lane] : 0.0;
-            m1[u] = (j < n && lane + kWave < R) ? (double)col[lane + kWave] : 0.0;
-        }
-    };
-    load(0);
+    T* u = A.ubuf + B.uoff[t];
+    // entry (i, j) of the packed column-major M_t: j R - j (j - 1) / 2 + i - j
+    auto cidx = [&](int j, int i) { return (uint32_t)(j * R - j * (j - 1) / 2 + i - j); };
     double* bs = bsh[w];
-    if (lane < n) {
-        double b = (double)A.x[cols[lane]];
-        if (LEVEL == 1) {  // a middle subtree: the bottom roots' update rows add in first
-            const int c = B.cptr[t] + lane;
-            for (int q = B.acptr[c]; q < B.acptr[c + 1]; ++q) b += (double)A.ubuf[B.acpos[q]];
+    // the right-hand side of the subtree columns into LDS (issued after the
+    // first column loads, which need only the header)
+    auto gather = [&]() {
+        if (lane < n) {
+            double b = (double)A.x[cols[lane]];
+            if (LEVEL == 1) {  // a middle subtree: the bottom roots' update rows add in first
+                const int c = B.cptr[t] + lane;
+                for (int q = B.acptr[c]; q < B.acptr[c + 1]; ++q) b += (double)A.ubuf[B.acpos[q]];
+            }
+            bs[lane] = b;
         }
-        bs[lane] = b;
-    }
-    wave_sync();
+        wave_sync();
+    };
     double a0 = 0.0, a1 = 0.0;
-    for (int j0 = 0; j0 < n; j0 += kSubBatch) {
-        if (j0) load(j0);
+    if (R <= kWave) {
+        // one row per lane: twice the columns per batch in the same registers
+        constexpr int NB = 2 * kSubBatch;
+        double m[NB];
+        auto load = [&](int j0) {
 #pragma unroll
-        for (int u = 0; u < kSubBatch; ++u) {
-            const double b = j0 + u < n ? bs[j0 + u] : 0.0;
-            a0 += m0[u] * b;
-            a1 += m1[u] * b;
+            for (int q = 0; q < NB; ++q) {
+                const int j = j0 + q;
+                m[q] = (j < n && lane >= j && lane < R) ? (double)ld_off(Mt, cidx(j, lane)) : 0.0;
+            }
+        };
+        load(0);
+        gather();
+        for (int j0 = 0; j0 < n; j0 += NB) {
+            if (j0) load(j0);
+#pragma unroll
+            for (int q = 0; q < NB; ++q) a0 += m[q] * (j0 + q < n ? bs[j0 + q] : 0.0);
+        }
+    } else {
+        // rows lane and lane + 64
+        double m0[kSubBatch], m1[kSubBatch];
+        auto load = [&](int j0) {
+#pragma unroll
+            for (int q = 0; q < kSubBatch; ++q) {
+                const int j = j0 + q;
+                m0[q] = (j < n && lane >= j && lane < R) ? (double)ld_off(Mt, cidx(j, lane)) : 0.0;
+                m1[q] = (j < n && lane + kWave < R) ? (double)ld_off(Mt, cidx(j, lane + kWave)) : 0.0;
+            }
+        };
+        load(0);
+        gather();
+        for (int j0 = 0; j0 < n; j0 += kSubBatch) {
+            if (j0) load(j0);
+#pragma unroll
+            for (int q = 0; q < kSubBatch; ++q) {
+                const double b = j0 + q < n ? bs[j0 + q] : 0.0;
+                a0 += m0[q] * b;
+                a1 += m1[q] * b;
+            }
         }
     }
-    T* u = A.ubuf + A.uoff[B.root[t]];
     if (lane < n)
         A.x[cols[lane]] = (T)a0;
     else if (lane < R)
@@ -1766,25 +1886,26 @@ __global__ __launch_bounds__(kBlock) void k_sub_fwd(SubArgs B, SolveArgs<T> A, c
 }
 
 // Backward: one wave per subtree; lane j accumulates column j over the rows of
-// the row-major copy, kSubBatch coalesced row loads at a time.
+// the row-major copy, kSubBatchB coalesced row loads at a time.
 template <typename T, int LEVEL>
 __global__ __launch_bounds__(kBlock) void k_sub_bwd(SubArgs B, SolveArgs<T> A, const T* __restrict__ Mrow, int t0,
                                                    int t1) {
     __shared__ double vsh[kBlock / kWave][kSubRows];
-    const int w = threadIdx.x / kWave;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int t = t0 + blockIdx.x * (kBlock / kWave) + w;
     const int lane = threadIdx.x & (kWave - 1);
     if (t >= t1) return;  // uniform per wave
-    const int n = B.n[t], R = B.R[t], root = B.root[t];
+    const int n = B.n[t], R = B.R[t];
     const int32_t* cols = B.cols + B.cptr[t];
-    const int32_t* urow = A.rows + A.rows_off[root] + A.sn_k[root];
+    const int32_t* urow = A.rows + B.urow[t];
     const T* Mt = Mrow + B.moff[t];
-    double mv[kSubBatch];
+    double mv[kSubBatchB];
     auto load = [&](int i0) {  // row loads of batch i0 (the first one overlaps the gather below)
 #pragma unroll
-        for (int u = 0; u < kSubBatch; ++u) {
+        for (int u = 0; u < kSubBatchB; ++u) {
             const int i = i0 + u;
-            mv[u] = (i < R && lane <= i && lane < n) ? (double)Mt[sub_row_off(i, n) + lane] : 0.0;
+            const uint32_t ro = i <= n ? i * (i + 1) / 2 : n * (n + 1) / 2 + (i - n) * n;  // sub_row_off
+            mv[u] = (i < R && lane <= i && lane < n) ? (double)ld_off(Mt, ro + lane) : 0.0;
         }
     };
     load(0);
@@ -1792,10 +1913,10 @@ __global__ __launch_bounds__(kBlock) void k_sub_bwd(SubArgs B, SolveArgs<T> A, c
     for (int i = lane; i < R; i += kWave) v[i] = (double)(i < n ? A.x[cols[i]] : A.x[urow[i - n]]);
     wave_sync();
     double acc = 0.0;
-    for (int i0 = 0; i0 < R; i0 += kSubBatch) {
+    for (int i0 = 0; i0 < R; i0 += kSubBatchB) {
         if (i0) load(i0);
 #pragma unroll
-        for (int u = 0; u < kSubBatch; ++u) acc += mv[u] * (i0 + u < R ? v[i0 + u] : 0.0);
+        for (int u = 0; u < kSubBatchB; ++u) acc += mv[u] * (i0 + u < R ? v[i0 + u] : 0.0);
     }
     if (lane < n) A.x[cols[lane]] = (T)acc;
 }
@@ -2516,6 +2637,8 @@ struct PiPlan {
     std::vector<int32_t> mcol, acptr;
     std::vector<int64_t> acpos;
     std::vector<int64_t> fdoff, smoff;
+    // per subtree: its root's update rows in the update buffer and in the row lists
+    std::vector<int64_t> suoff, surow;
     int bin_off[kSubBins + 1] = {0, 0, 0, 0, 0};
     size_t bin_bytes[kSubBins] = {0, 0, 0, 0};
 };
@@ -2571,6 +2694,8 @@ static void make_sub_plan(const Symbolic& S, const std::vector<int8_t>& lev, PiP
             for (int j = 0; j < S.sn_k[s]; ++j) dpos[S.sn_first[s] + j] = -1;
         for (int a = 0; a < u; ++a) dpos[S.sn_rows[S.sn_rows_off[r] + kr + a]] = -1;
         P.sroot.push_back(r);
+        P.suoff.push_back(S.sn_uoff[r]);
+        P.surow.push_back((int64_t)S.sn_rows_off[r] + kr);
         P.sn_n.push_back(n);
         P.sR.push_back(R);
         P.smoff.push_back(P.msub);
@@ -2848,7 +2973,7 @@ struct Instance::Impl {
         DBuf<int64_t> acpos;
         int64_t msub = 0;
         DBuf<int32_t> sroot, sn_n, sR, fptr, fsn, fdcol, drow, scptr, scols, bins;
-        DBuf<int64_t> fdoff, smoff;
+        DBuf<int64_t> fdoff, smoff, suoff, surow;
         int bin_off[kSubBins + 1] = {0, 0, 0, 0, 0};
         size_t bin_bytes[kSubBins] = {0, 0, 0, 0};
         PiSizes sizes() const {
@@ -3026,6 +3151,8 @@ static void build_pi(Instance::Impl& D, const Symbolic& S, size_t& bytes, hipStr
                 up(Q.bins, PP.bins);
                 up64(Q.fdoff, PP.fdoff);
                 up64(Q.smoff, PP.smoff);
+                up64(Q.suoff, PP.suoff);
+                up64(Q.surow, PP.surow);
                 Q.nsub = PP.nsub;
                 Q.nsub0 = PP.nsub0;
                 Q.msub = PP.msub;
@@ -3098,7 +3225,7 @@ Instance::~Instance() {
         rel(P.segrow); rel(P.mrows); rel(P.cplen); rel(P.cjj); rel(P.cboff); rel(P.cpoff);
         rel(P.boff); rel(P.poff); rel(P.pos); rel(P.segq); rel(P.cpos); rel(P.part);
         rel(P.sroot); rel(P.sn_n); rel(P.sR); rel(P.fptr); rel(P.fsn); rel(P.fdcol); rel(P.drow); rel(P.scptr);
-        rel(P.scols); rel(P.bins); rel(P.fdoff); rel(P.smoff); rel(P.acptr); rel(P.acpos);
+        rel(P.scols); rel(P.bins); rel(P.fdoff); rel(P.smoff); rel(P.suoff); rel(P.surow); rel(P.acptr); rel(P.acpos);
         for (auto& L : P.bottom) {
             rel(L.th); rel(L.meta); rel(L.lvl); rel(L.ch); rel(L.rm); rel(L.src); rel(L.mid);
         }
@@ -3633,8 +3760,9 @@ void Instance::update_branches(const HostArrays& h) {
 
 // One breaker operation: branch k (insertion index) closed or open.  Staged on
 // the host; refresh_topology() applies all staged operations at once.
+// A breaker operation changes device values only (status, row activity, Ybus):
+// launches keep their arguments, so captured iteration graphs stay valid.
 void Instance::set_branch_closed(int k, bool closed) {
-    ++d_->plan_epoch;  // launch plans change: captured graphs retire
     Impl& D = *d_;
     if (k < 0 || k >= D.nb) throw std::invalid_argument("branch index out of range");
     D.status_h[D.lk[k]] = closed ? 1 : 0;
@@ -3647,7 +3775,6 @@ void Instance::set_branch_closed(int k, bool closed) {
 }
 
 void Instance::refresh_topology() {
-    ++d_->plan_epoch;  // launch plans change: captured graphs retire
     Impl& D = *d_;
     if (!D.topo_staged) return;
     HIPCHECK(hipSetDevice(device_));
@@ -3805,7 +3932,7 @@ void launch_meas_loc(Instance::Impl& D, int robust, double gamma) {
 template <typename T>
 void launch_rhs_loc(Instance::Impl& D) {
     if (D.n)
-        k_rhs_loc<T><<<nblk(D.n), kBlock, 0, D.stream>>>(D.n, D.yrow.p, D.ycol.p, D.ytr.p, D.yG.p, D.yB.p, D.Pij.p,
+        k_rhs_loc<T><<<nblk(D.n), kBlock, 0, D.stream>>>(D.n, D.yrow.p, D.ycol.p, D.ytr.p, D.Pij.p,
                                                          D.Qij.p, D.P.p, D.Q.p, D.V.p, D.sbus.p, D.sbr.p, D.inc_ptr.p,
                                                          D.inc.p, D.pth.p, D.pv.p, bufs<T>(D).x.p);
 }
@@ -3940,7 +4067,8 @@ bool pi_active(const Instance::Impl& D) { return D.use_pi && D.pi.ok; }
 SubArgs sub_args(const Instance::Impl& D) {
     const Instance::Impl::PiDev& P = D.pi;
     return SubArgs{P.nsub,   P.msub,    P.sroot.p, P.sn_n.p,  P.sR.p,    P.fptr.p,  P.fsn.p, P.fdcol.p,
-                   P.fdoff.p, P.drow.p, P.scptr.p, P.scols.p, P.smoff.p, P.acptr.p, P.acpos.p};
+                   P.fdoff.p, P.drow.p, P.scptr.p, P.scols.p, P.smoff.p, P.acptr.p, P.acpos.p,
+                   P.suoff.p, P.surow.p};
 }
 
 bool sub_active(const Instance::Impl& D) { return pi_active(D) && D.pi.dense && D.pi.nsub > 0; }

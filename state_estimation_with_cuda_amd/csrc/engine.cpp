@@ -558,7 +558,10 @@ int Engine::ensure_symbolic() {
         mv.from = h.from.data();
         mv.to = h.to.data();
         mv.status = h.status.data();
-        if (angle_states(mv) != sym_->th_state) sym_dirty_ = true;
+        if (!islands_kept(mv.status)) {
+            if (angle_states(mv) != sym_->th_state) sym_dirty_ = true;
+            set_islands(mv);
+        }
     }
     if (sym_ && !sym_dirty_) return ST_OK;
     HostArrays h;
@@ -582,7 +585,75 @@ int Engine::ensure_symbolic() {
     analyze(mv, *S);
     sym_ = S;
     sym_dirty_ = false;
+    set_islands(mv);
     return ST_OK;
+}
+
+void Engine::set_islands(const ModelView& mv) {
+    (void)angle_states(mv, &isl_);
+    // incident present branches per bus (open ones included: status decides at search time)
+    const int n = mv.n;
+    adj_ptr_.assign(n + 1, 0);
+    for (int k = 0; k < mv.nb; ++k) {
+        if (mv.present && !mv.present[k]) continue;
+        adj_ptr_[mv.from[k] + 1]++;
+        adj_ptr_[mv.to[k] + 1]++;
+    }
+    for (int b = 0; b < n; ++b) adj_ptr_[b + 1] += adj_ptr_[b];
+    adj_br_.resize(adj_ptr_[n]);
+    std::vector<int32_t> f(adj_ptr_.begin(), adj_ptr_.end() - 1);
+    for (int k = 0; k < mv.nb; ++k) {
+        if (mv.present && !mv.present[k]) continue;
+        adj_br_[f[mv.from[k]]++] = k;
+        adj_br_[f[mv.to[k]]++] = k;
+    }
+    isl_mark_.assign(n, 0);
+    isl_epoch_ = 0;
+}
+
+bool Engine::islands_kept(const uint8_t* status) {
+    if (isl_.size() != (size_t)n_bus() || adj_ptr_.size() != (size_t)n_bus() + 1) return false;
+    for (int k : topo_changed_) {
+        const int a = br_from_[k], b = br_to_[k];
+        if (status[k] ? isl_[a] != isl_[b] : !ends_connected(a, b, status)) return false;
+    }
+    return true;
+}
+
+// Whether buses a and b are joined by closed branches: two breadth-first
+// searches, one from each end, the smaller frontier advancing first, until
+// they meet or one side runs out.  In a meshed network an opened branch's ends
+// meet again within a few hops.
+bool Engine::ends_connected(int a, int b, const uint8_t* status) {
+    if (a == b) return true;
+    if (isl_epoch_ > 0xfffffff0u) {
+        std::fill(isl_mark_.begin(), isl_mark_.end(), 0u);
+        isl_epoch_ = 0;
+    }
+    const uint32_t ma = ++isl_epoch_, mb = ++isl_epoch_;
+    isl_q0_.assign(1, a);
+    isl_q1_.assign(1, b);
+    isl_mark_[a] = ma;
+    isl_mark_[b] = mb;
+    size_t h0 = 0, h1 = 0;
+    while (h0 < isl_q0_.size() && h1 < isl_q1_.size()) {
+        const bool first = isl_q0_.size() - h0 <= isl_q1_.size() - h1;
+        std::vector<int32_t>& q = first ? isl_q0_ : isl_q1_;
+        size_t& h = first ? h0 : h1;
+        const uint32_t me = first ? ma : mb, other = first ? mb : ma;
+        const int u = q[h++];
+        for (int p = adj_ptr_[u]; p < adj_ptr_[u + 1]; ++p) {
+            const int k = adj_br_[p];
+            if (!status[k]) continue;
+            const int v = br_from_[k] == u ? br_to_[k] : br_from_[k];
+            if (isl_mark_[v] == other) return true;
+            if (isl_mark_[v] != me) {
+                isl_mark_[v] = me;
+                q.push_back(v);
+            }
+        }
+    }
+    return false;
 }
 
 int Engine::rebuild() {
@@ -784,12 +855,23 @@ int Engine::apply_topology_values() {
     mv.from = br_from_.data();
     mv.to = br_to_.data();
     mv.status = status.data();
-    if (!sym_ || angle_states(mv) != sym_->th_state) {  // new state layout: symbolic redo
+    if (!sym_) {
         model_dirty_ = true;
         sym_dirty_ = true;
         topo_changed_.clear();
         return ST_OK;
     }
+    const std::vector<int> changed = topo_changed_;
+    if (!islands_kept(mv.status)) {
+        if (angle_states(mv) != sym_->th_state) {  // new state layout: symbolic redo
+            model_dirty_ = true;
+            sym_dirty_ = true;
+            topo_changed_.clear();
+            return ST_OK;
+        }
+        set_islands(mv);  // new islands, same references
+    }
+    topo_changed_ = changed;
     if (model_dirty_ || !inst_) {  // the next upload carries the status anyway
         topo_changed_.clear();
         return ST_OK;

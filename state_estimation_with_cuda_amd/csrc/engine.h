@@ -218,10 +218,22 @@ class Engine {
     void build_host_arrays(HostArrays& h) const;
     void admittance(int k, double& g, double& b) const;
     int rebuild();
+    // Breaker operations since the last island analysis that keep every island
+    // (a closed branch inside one island, an opened branch whose ends stay
+    // connected): then the angle references, and with them the state layout,
+    // are unchanged without a pass over the whole network.
+    bool islands_kept(const uint8_t* status);
+    bool ends_connected(int a, int b, const uint8_t* status);
+    void set_islands(const ModelView& mv);
 
     EngineConfig cfg_;
     bool topo_dirty_ = false;       // breaker status changed since the last upload to the device
     std::vector<int> topo_changed_;  // branches operated since then
+    // islands of the last full analysis: per bus its island's smallest bus; the
+    // present branches incident to each bus; search marks
+    std::vector<int32_t> isl_, adj_ptr_, adj_br_, isl_q0_, isl_q1_;
+    std::vector<uint32_t> isl_mark_;
+    uint32_t isl_epoch_ = 0;
     std::string tune_report_;
     EngineOptions opt_;
     bool initialized_ = false;

@@ -15,7 +15,7 @@ namespace {
 // highest index (the reference keeps the last SLACK bus added,
 // network_model.cu:85-87).  States: the angles of all other buses in bus order,
 // then all magnitudes (sparse_matrix.cu:1677-1685 for a single island).
-std::vector<int32_t> island_reference(const ModelView& mv) {
+std::vector<int32_t> island_reference(const ModelView& mv, std::vector<int32_t>* roots) {
     std::vector<int32_t> up(mv.n);
     std::iota(up.begin(), up.end(), 0);
     auto find = [&](int a) {
@@ -27,16 +27,18 @@ std::vector<int32_t> island_reference(const ModelView& mv) {
             int a = find(mv.from[k]), b = find(mv.to[k]);
             if (a != b) up[std::max(a, b)] = std::min(a, b);
         }
+    for (int b = 0; b < mv.n; ++b) up[b] = find(b);  // every bus points at its island's smallest bus
     std::vector<int32_t> ref(mv.n, -1);
     for (int b = 0; b < mv.n; ++b)
-        if (mv.bus_type[b] == 2) ref[find(b)] = b;
+        if (mv.bus_type[b] == 2) ref[up[b]] = b;
     std::vector<int32_t> out(mv.n);
-    for (int b = 0; b < mv.n; ++b) out[b] = ref[find(b)];
+    for (int b = 0; b < mv.n; ++b) out[b] = ref[up[b]];
+    if (roots) *roots = std::move(up);
     return out;  // reference bus of each bus's island (-1: none)
 }
 
-void angle_references(const ModelView& mv, Symbolic& S) {
-    const std::vector<int32_t> ref = island_reference(mv);
+void angle_references(const ModelView& mv, Symbolic& S, std::vector<int32_t>* roots = nullptr) {
+    const std::vector<int32_t> ref = island_reference(mv, roots);
     S.th_state.assign(mv.n, -1);
     S.nth = 0;
     S.n_refs = 0;
@@ -257,9 +259,9 @@ void minimum_degree(int n, const std::vector<int32_t>& aptr, const std::vector<i
 
 }  // namespace
 
-std::vector<int32_t> angle_states(const ModelView& mv) {
+std::vector<int32_t> angle_states(const ModelView& mv, std::vector<int32_t>* islands) {
     Symbolic S;
-    angle_references(mv, S);
+    angle_references(mv, S, islands);
     return S.th_state;
 }
 

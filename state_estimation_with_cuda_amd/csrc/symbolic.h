@@ -105,8 +105,9 @@ void analyze(const ModelView& mv, Symbolic& S);
 
 // Angle state of every bus (-1 for the reference of its island) for the
 // current breaker status; equal to Symbolic::th_state while the islands keep
-// their references.
-std::vector<int32_t> angle_states(const ModelView& mv);
+// their references.  islands (optional): per bus, the smallest bus index of
+// its island of closed branches.
+std::vector<int32_t> angle_states(const ModelView& mv, std::vector<int32_t>* islands = nullptr);
 
 // Natural-order G pattern (CSC, lower), for reporting and parity checks.
 void natural_gain_pattern(const Symbolic& S, std::vector<int32_t>& colptr, std::vector<int32_t>& row,

@@ -129,3 +129,46 @@ def test_symbolic_sizes_200k_scale_structure():
     assert sz["max_front"] <= 64
     assert sz["nnz_l"] < 3 * sz["nnz_g"]
     e.close()
+
+
+def test_breaker_sequences_track_islands_like_oracle():
+    """A run of breaker operations, each checked against the oracle's full
+    island analysis: splitting and re-joining two areas (one angle reference
+    each, then one for both), opening and closing a branch inside an island
+    (same structure), and cutting off a bus without a SLACK bus (new islands,
+    same references)."""
+    a, b = gg.geometric(200, seed=51, flow_meas="all"), gg.geometric(150, seed=52, flow_meas="all")
+    g = gg.concat([a, b])
+    g.br_from = np.append(g.br_from, a.n).astype(np.int32)
+    g.br_to = np.append(g.br_to, 1).astype(np.int32)
+    for f, v in (("r", 0.02), ("x", 0.1), ("b", 0.0), ("tap", 1.0), ("shift", 0.0)):
+        setattr(g, f, np.append(getattr(g, f), v))
+    g.is_xfmr = np.append(g.is_xfmr, 0).astype(np.int32)
+    g.status = np.append(g.status, 1).astype(np.uint8)
+    tie = g.nb - 1
+    import scipy.sparse as sp
+    import scipy.sparse.csgraph as cg
+
+    def parts(skip):
+        keep = np.ones(g.nb, bool)
+        keep[list(skip)] = False
+        A = sp.coo_matrix((np.ones(keep.sum()), (g.br_from[keep], g.br_to[keep])), shape=(g.n, g.n))
+        return cg.connected_components(A, directed=False)[0]
+
+    inner = next(k for k in range(a.nb) if parts([k]) == 1)  # a branch of area a on a cycle
+    deg = np.bincount(np.concatenate([g.br_from, g.br_to]), minlength=g.n)
+    leaf = next((k for k in range(a.nb) if (deg[g.br_from[k]] == 1 or deg[g.br_to[k]] == 1)
+                 and g.bus_type[g.br_from[k] if deg[g.br_from[k]] == 1 else g.br_to[k]] != 2), None)
+    ops = [(tie, 0), (tie, 1), (inner, 0), (inner, 1), (tie, 0), (inner, 0), (tie, 1), (inner, 1)]
+    if leaf is not None:
+        ops += [(leaf, 0), (inner, 0), (leaf, 1), (inner, 1)]
+    e = host_engine(g)
+    o = Oracle(g)
+    for k, st in ops:
+        e.set_branch_status(k, st)
+        o.set_branch_status(k, st)
+        assert e.sizes()["ns"] == o.ns, (k, st)
+        es, os_ = e.ordering(), o.symbolic()
+        for key in ("perm", "parent", "snode_ptr"):
+            assert np.array_equal(es[key], os_[key]), (k, st, key)
+    e.close()
