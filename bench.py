@@ -106,9 +106,9 @@ def algorithmic_bytes(sz, g, prec_bytes, direct=True, plan=None):
         out["meas"] = (n + 1) * 4 + nnz_y * (4 + 8 + 8 + 8 + 8) + n * 8 * 4 + m * (1 + 8 + 8) \
             + 2 * (n + 1) * 4 + (2 * nb + 1) * 4 + nb * (4 + 4 + 8 * 5) + 32 * n + 32 * nb
         out["power"] = 0  # fused into the measurement pass on this path
-        # k_rhs_loc: Ybus rows with the transpose map and per-entry terms, self
-        # G/B, bus V/P/Q and sums, incident branches and their sums, state maps, b
-        out["rhs"] = (n + 1) * 4 + nnz_y * (4 + 4 + 8 + 8) + n * 16 + n * 8 * 3 + 32 * n + (n + 1) * 4 \
+        # k_rhs_loc: Ybus rows with the transpose map and per-entry terms, bus
+        # V/P/Q and sums, incident branches and their sums, state maps, b
+        out["rhs"] = (n + 1) * 4 + nnz_y * (4 + 4 + 8 + 8) + n * 8 * 3 + 32 * n + (n + 1) * 4 \
             + 2 * nb * 4 + 32 * nb + 8 * n + ns * s
     if plan.get("subtrees"):
         def sub(ent, us, nsub, nc):
@@ -584,7 +584,8 @@ def main():
     ap.add_argument("--serve-warmup", type=float, default=5.0, help="serve: seconds excluded from the statistics")
     ap.add_argument("--area-rate", type=float, default=1.0, help="serve: new areas per second")
     ap.add_argument("--area-pool", type=int, default=6)
-    ap.add_argument("--area-slots", type=int, default=4)
+    ap.add_argument("--area-slots", type=int, default=3,
+                    help="serve: concurrent area engines (with the resident model at most 4 streams, one hardware queue each)")
     ap.add_argument("--hbm-budget-gib", type=float, default=64.0)
     ap.add_argument("--record-every", type=int, default=500)
     ap.add_argument("--toggle-every", type=int, default=50, help="serve: updates between breaker operations (0: none)")

@@ -2298,6 +2298,71 @@ class BlockCache {
     }
 };
 
+// Streams and pinned status blocks kept per device for the life of the
+// process.  HIP binds each new stream to one of a few hardware queues in turn
+// (GPU_MAX_HW_QUEUES), so engines created and destroyed beside a long-lived
+// one would, one creation in a few, land on its queue and serialize their
+// kernels with it; a reused stream keeps its queue, and nothing is destroyed or
+// freed (which can wait for the whole device) while other engines run.
+class EnginePool {
+   public:
+    static hipStream_t stream() {
+        int dev = 0;
+        HIPCHECK(hipGetDevice(&dev));
+        {
+            std::lock_guard<std::mutex> lk(mu());
+            auto& fl = streams()[dev];
+            if (!fl.empty()) {
+                hipStream_t st = fl.back();
+                fl.pop_back();
+                return st;
+            }
+        }
+        hipStream_t st = nullptr;
+        HIPCHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        return st;
+    }
+    // st must be drained (no queued work)
+    static void give_back(hipStream_t st) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return;
+        std::lock_guard<std::mutex> lk(mu());
+        streams()[dev].push_back(st);
+    }
+    static void* pinned(size_t bytes) {
+        {
+            std::lock_guard<std::mutex> lk(mu());
+            auto& fl = pins()[bytes];
+            if (!fl.empty()) {
+                void* p = fl.back();
+                fl.pop_back();
+                return p;
+            }
+        }
+        void* p = nullptr;
+        HIPCHECK(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+        return p;
+    }
+    static void give_back_pinned(void* p, size_t bytes) {
+        std::lock_guard<std::mutex> lk(mu());
+        pins()[bytes].push_back(p);
+    }
+
+   private:
+    static std::mutex& mu() {
+        static std::mutex m;
+        return m;
+    }
+    static std::map<int, std::vector<hipStream_t>>& streams() {
+        static auto* f = new std::map<int, std::vector<hipStream_t>>();  // lives to process end
+        return *f;
+    }
+    static std::map<size_t, std::vector<void*>>& pins() {
+        static auto* f = new std::map<size_t, std::vector<void*>>();
+        return *f;
+    }
+};
+
 // The stream whose queued work may still use the blocks released during an
 // instance call.
 struct StreamScope {
@@ -3177,10 +3242,10 @@ static void build_pi(Instance::Impl& D, const Symbolic& S, size_t& bytes, hipStr
 Instance::Instance(int device, int precision) : device_(device), precision_(precision) {
     HIPCHECK(hipSetDevice(device_));
     d_ = new Impl();
-    HIPCHECK(hipStreamCreateWithFlags(&d_->stream, hipStreamNonBlocking));
+    d_->stream = EnginePool::stream();
     HIPCHECK(hipEventCreate(&d_->ev0));
     HIPCHECK(hipEventCreate(&d_->ev1));
-    HIPCHECK(hipHostMalloc((void**)&d_->hst, sizeof(DevStatus), hipHostMallocDefault));
+    d_->hst = static_cast<DevStatus*>(EnginePool::pinned(sizeof(DevStatus)));
     HIPCHECK(hipHostGetDevicePointer((void**)&d_->hst_dev, d_->hst, 0));
     d_->st.alloc(1, bytes_);
     for (int p = 0; p < PH_COUNT; ++p)
@@ -3240,10 +3305,13 @@ Instance::~Instance() {
     for (int p = 0; p < PH_COUNT; ++p)
         for (int q = 0; q < 2; ++q)
             if (d_->pev[p][q]) (void)hipEventDestroy(d_->pev[p][q]);
-    if (d_->hst) (void)hipHostFree(d_->hst);
+    if (d_->hst) EnginePool::give_back_pinned(d_->hst, sizeof(DevStatus));
     if (d_->ev0) (void)hipEventDestroy(d_->ev0);
     if (d_->ev1) (void)hipEventDestroy(d_->ev1);
-    if (d_->stream) (void)hipStreamDestroy(d_->stream);
+    if (d_->stream) {
+        (void)hipStreamSynchronize(d_->stream);
+        EnginePool::give_back(d_->stream);
+    }
     delete d_;
     d_ = nullptr;
 }
