@@ -300,9 +300,23 @@ def library(double: bool = True) -> Library:
 
 
 def _id_buffer(prefix: str, n: int):
-    """Contiguous fixed-width NUL-terminated ids and the address of each."""
-    w = len(prefix) + len(str(max(n - 1, 0))) + 1
-    arr = np.char.add(prefix, np.arange(n).astype(str)).astype(f"S{w}")
+    """Contiguous fixed-width NUL-terminated ids "<prefix><i>" and the address of each.
+
+    The digits come from integer arithmetic on whole arrays (no string
+    operations per element), so a large model's ids take about a millisecond
+    and other Python threads, such as a serving loop's, are not held up."""
+    pb = prefix.encode()
+    nd = len(str(max(n - 1, 0)))
+    w = len(pb) + nd + 1
+    arr = np.zeros((max(n, 1), w), np.uint8)
+    arr[:, : len(pb)] = np.frombuffer(pb, np.uint8)
+    for d in range(1, nd + 1):  # the numbers with d digits form one range
+        lo, hi = (0 if d == 1 else 10 ** (d - 1)), min(10**d, n)
+        if lo >= hi:
+            continue
+        x = np.arange(lo, hi, dtype=np.int32)[:, None]
+        pw = (10 ** np.arange(d - 1, -1, -1)).astype(np.int32)[None, :]
+        arr[lo:hi, len(pb) : len(pb) + d] = (48 + (x // pw) % 10).astype(np.uint8)
     base = arr.ctypes.data
     return arr, base + np.arange(n, dtype=np.uint64) * np.uint64(w)
 
@@ -387,20 +401,20 @@ class Engine:
         lid, lptr = _id_buffer("L", nb)
         mid, mptr = _id_buffer("M", m)
         self._keep += [bid, lid, mid]
+        # the record arrays are numpy allocations passed by address (a ctypes
+        # array of that many structs is built with the interpreter lock held)
         B = self.S["BusInfo"]
-        bus = (B * n)()
-        arr = np.frombuffer(bus, dtype=self._dtype(B), count=n)
+        arr = self._records(B, n)
         arr["id"] = bptr
         arr["base_kv"] = 100.0
         arr["type"] = g.bus_type
         arr["v_setpoint"] = g.vset.astype(real)
         arr["p_scheduled"] = 0
         arr["q_scheduled"] = 0
-        self._check(self.L.sle_AddBuses(self.h, C.cast(bus, C.c_void_p), n), "sle_AddBuses")
+        self._check(self.L.sle_AddBuses(self.h, C.c_void_p(arr.ctypes.data), n), "sle_AddBuses")
         if nb:
             Br = self.S["BranchInfo"]
-            br = (Br * nb)()
-            a = np.frombuffer(br, dtype=self._dtype(Br), count=nb)
+            a = self._records(Br, nb)
             a["id"] = lptr
             a["from_bus"] = bptr[g.br_from]
             a["to_bus"] = bptr[g.br_to]
@@ -412,10 +426,9 @@ class Engine:
             a["phase_shift"] = g.shift
             a["is_transformer"] = g.is_xfmr
             a["sd_id"] = 0
-            self._check(self.L.sle_AddBranches(self.h, C.cast(br, C.c_void_p), nb), "sle_AddBranches")
+            self._check(self.L.sle_AddBranches(self.h, C.c_void_p(a.ctypes.data), nb), "sle_AddBranches")
         M = self.S["MeasurementInfo"]
-        me = (M * m)()
-        a = np.frombuffer(me, dtype=self._dtype(M), count=m)
+        a = self._records(M, m)
         a["id"] = mptr
         a["type"] = g.m_type
         bus_meas = np.isin(g.m_type, (0, 1, 2, 3, 7, 8))
@@ -426,7 +439,7 @@ class Engine:
         a["pt_ratio"] = 1
         a["ct_ratio"] = 1
         a["is_pseudo"] = np.isin(g.m_type, (7, 8)).astype(np.int32)
-        self._check(self.L.sle_AddMeasurements(self.h, C.cast(me, C.c_void_p), m), "sle_AddMeasurements")
+        self._check(self.L.sle_AddMeasurements(self.h, C.c_void_p(a.ctypes.data), m), "sle_AddMeasurements")
         if switches and nb:
             sid, sptr = _id_buffer("S", nb)
             self._keep.append(sid)
@@ -438,6 +451,12 @@ class Engine:
                 info.status = int(g.status[k])
                 self._check(self.L.sle_AddSwitch(self.h, C.byref(info)), "sle_AddSwitch")
         self.n, self.nb, self.m = n, nb, m
+
+    @classmethod
+    def _records(cls, struct, count):
+        """Zeroed array of `count` records laid out as the C struct (size and
+        field offsets of the ctypes mirror)."""
+        return np.zeros(max(count, 1), dtype=cls._dtype(struct))[:count]
 
     @staticmethod
     def _dtype(struct):
