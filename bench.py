@@ -170,9 +170,12 @@ def roofline_for(form, kt, ab):
     # the byte models are per run of a class (one launch, but for the middle
     # forward class two: assembly and product)
     avg_ms = cand[dom]["ms"] / max(cand[dom]["calls"], 1)
+    traffic = measured_traffic(form, dom)
+    if traffic is None and dom in ("meas", "power"):  # FP64 kernels in every form
+        traffic = measured_traffic("fp64", dom)
     achieved = ab[dom] / (avg_ms * 1e-3) / 1e9
     return dict(kernel=dom, bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(form, dom), bytes_per_launch=int(ab[dom]),
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, bytes_per_launch=int(ab[dom]),
                 avg_launch_ms=round(avg_ms, 5))
 
 

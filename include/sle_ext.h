@@ -27,7 +27,8 @@ typedef struct SLE_ExtResult {
     double objective;
     double largest_residual; /* max |z-h|/sigma over active rows */
     int32_t largest_residual_idx;
-    int32_t flags;       /* bit 0: dependency-driven factor/solve sweeps; bit 1: FP64 factor in use */
+    int32_t flags;       /* bit 0: dependency-driven factor/solve sweeps; bit 1: FP64 factor in use;
+                            bit 2: mixed form on the FP64 factor (FP32 solve data) */
     double gpu_ms;       /* device time of the solve */
     double host_ms;      /* wall time of the call */
     int32_t pivots_substituted; /* mixed form: iterations whose FP32 factor had pivots substituted */
@@ -65,6 +66,13 @@ typedef struct SLE_ExtResult {
  *   "meas_by_type"    1 (default): the device keeps the measurement rows grouped
  *                     by evaluation class (bus values, injections, flows), each
  *                     class by anchor bus; 0: by anchor bus only
+ *   "mixed_fp64_factor" 1 (default): the mixed form (use_double_precision 0)
+ *                     factors G in FP64 and rounds the factor's solve data
+ *                     (panels, subtree solve matrices, top inverse blocks) to
+ *                     FP32 once per factorization; the iterations on the cached
+ *                     factor then solve in FP32 with FP64 residuals, RHS and
+ *                     state; 0: FP32 factorization (switching to an FP64 factor
+ *                     when it contracts poorly)
  *   "rhs_by_location" 1 (default): iterations on a cached factor form H^T W r
  *                     from the summed weighted residuals of each bus and
  *                     branch end, gathered per state (no per-entry products);

@@ -1663,6 +1663,7 @@ __global__ __launch_bounds__(kBlock) void k_pi_bwd(PiArgs A, SolveArgs<T> SA, co
 // column j holds rows j..R-1), n <= 64, R <= 128.
 // ---------------------------------------------------------------------------
 constexpr int kSubCols = 64, kSubRows = 128;
+static_assert(kSubRows <= 2 * kWave, "the backward gather takes rows lane and lane + 64");
 constexpr int kSubBins = 4;  // launch groups of the subtree-matrix kernels
 
 struct SubArgs {
@@ -1823,11 +1824,20 @@ __global__ __launch_bounds__(kBlock) void k_sub_fwd(SubArgs B, SolveArgs<T> A, c
     // entry (i, j) of the packed column-major M_t: j R - j (j - 1) / 2 + i - j
     auto cidx = [&](int j, int i) { return (uint32_t)(j * R - j * (j - 1) / 2 + i - j); };
     double* bs = bsh[w];
+    // FP32 elements: unconditional loads (an entry outside the trapezoid reads
+    // entry 0) of raw values, selected and converted in the product; a load
+    // under a branch gets its conversion in the same branch and waits for it
+    // there, which serialized the batch.  The column index goes first, so the
+    // gather waits for it alone (loads complete in issue order).  FP64
+    // elements keep predicated loads and the gather after the first batch
+    // (measured faster for them).
+    constexpr bool clamp = sizeof(T) == 4;
+    int cl = clamp && lane < n ? cols[lane] : 0;
     // the right-hand side of the subtree columns into LDS (issued after the
     // first column loads, which need only the header)
     auto gather = [&]() {
         if (lane < n) {
-            double b = (double)A.x[cols[lane]];
+            double b = (double)A.x[cl];
             if (LEVEL == 1) {  // a middle subtree: the bottom roots' update rows add in first
                 const int c = B.cptr[t] + lane;
                 for (int q = B.acptr[c]; q < B.acptr[c + 1]; ++q) b += (double)A.ubuf[B.acpos[q]];
@@ -1840,46 +1850,52 @@ __global__ __launch_bounds__(kBlock) void k_sub_fwd(SubArgs B, SolveArgs<T> A, c
     if (R <= kWave) {
         // one row per lane: twice the columns per batch in the same registers
         constexpr int NB = 2 * kSubBatch;
-        double m[NB];
+        T m[NB];
+        auto ok = [&](int j, int i) { return j < n && i >= j && i < R; };
         auto load = [&](int j0) {
 #pragma unroll
-            for (int q = 0; q < NB; ++q) {
-                const int j = j0 + q;
-                m[q] = (j < n && lane >= j && lane < R) ? (double)ld_off(Mt, cidx(j, lane)) : 0.0;
-            }
+            for (int q = 0; q < NB; ++q)
+                m[q] = clamp ? ld_off(Mt, ok(j0 + q, lane) ? cidx(j0 + q, lane) : 0u)
+                             : (ok(j0 + q, lane) ? ld_off(Mt, cidx(j0 + q, lane)) : T(0));
         };
         load(0);
+        if (!clamp && lane < n) cl = cols[lane];
         gather();
         for (int j0 = 0; j0 < n; j0 += NB) {
             if (j0) load(j0);
 #pragma unroll
-            for (int q = 0; q < NB; ++q) a0 += m[q] * (j0 + q < n ? bs[j0 + q] : 0.0);
+            for (int q = 0; q < NB; ++q)
+                a0 += (!clamp || ok(j0 + q, lane) ? (double)m[q] : 0.0) * (j0 + q < n ? bs[j0 + q] : 0.0);
         }
     } else {
         // rows lane and lane + 64
-        double m0[kSubBatch], m1[kSubBatch];
+        T m0[kSubBatch], m1[kSubBatch];
+        auto ok = [&](int j, int i) { return j < n && i >= j && i < R; };
         auto load = [&](int j0) {
 #pragma unroll
             for (int q = 0; q < kSubBatch; ++q) {
                 const int j = j0 + q;
-                m0[q] = (j < n && lane >= j && lane < R) ? (double)ld_off(Mt, cidx(j, lane)) : 0.0;
-                m1[q] = (j < n && lane + kWave < R) ? (double)ld_off(Mt, cidx(j, lane + kWave)) : 0.0;
+                m0[q] = clamp ? ld_off(Mt, ok(j, lane) ? cidx(j, lane) : 0u)
+                              : (ok(j, lane) ? ld_off(Mt, cidx(j, lane)) : T(0));
+                m1[q] = clamp ? ld_off(Mt, ok(j, lane + kWave) ? cidx(j, lane + kWave) : 0u)
+                              : (ok(j, lane + kWave) ? ld_off(Mt, cidx(j, lane + kWave)) : T(0));
             }
         };
         load(0);
+        if (!clamp && lane < n) cl = cols[lane];
         gather();
         for (int j0 = 0; j0 < n; j0 += kSubBatch) {
             if (j0) load(j0);
 #pragma unroll
             for (int q = 0; q < kSubBatch; ++q) {
                 const double b = j0 + q < n ? bs[j0 + q] : 0.0;
-                a0 += m0[q] * b;
-                a1 += m1[q] * b;
+                a0 += (!clamp || ok(j0 + q, lane) ? (double)m0[q] : 0.0) * b;
+                a1 += (!clamp || ok(j0 + q, lane + kWave) ? (double)m1[q] : 0.0) * b;
             }
         }
     }
     if (lane < n)
-        A.x[cols[lane]] = (T)a0;
+        A.x[cl] = (T)a0;
     else if (lane < R)
         u[lane - n] = (T)a0;
     if (lane + kWave < R) u[lane + kWave - n] = (T)a1;
@@ -1899,24 +1915,41 @@ __global__ __launch_bounds__(kBlock) void k_sub_bwd(SubArgs B, SolveArgs<T> A, c
     const int32_t* cols = B.cols + B.cptr[t];
     const int32_t* urow = A.rows + B.urow[t];
     const T* Mt = Mrow + B.moff[t];
-    double mv[kSubBatchB];
+    T mv[kSubBatchB];  // raw values; FP32 loads unconditional (see k_sub_fwd)
+    constexpr bool clamp = sizeof(T) == 4;
     auto load = [&](int i0) {  // row loads of batch i0 (the first one overlaps the gather below)
 #pragma unroll
         for (int u = 0; u < kSubBatchB; ++u) {
             const int i = i0 + u;
             const uint32_t ro = i <= n ? i * (i + 1) / 2 : n * (n + 1) / 2 + (i - n) * n;  // sub_row_off
-            mv[u] = (i < R && lane <= i && lane < n) ? (double)ld_off(Mt, ro + lane) : 0.0;
+            const bool ok = i < R && lane <= i && lane < n;
+            mv[u] = clamp ? ld_off(Mt, ok ? ro + lane : 0u) : (ok ? ld_off(Mt, ro + lane) : T(0));
         }
     };
-    load(0);
+    // FP32: the gather's indices first (loads complete in issue order), then
+    // the first row batch, then the gather itself (R <= 2 * 64); FP64: the
+    // first row batch, then the gather
     double* v = vsh[w];
-    for (int i = lane; i < R; i += kWave) v[i] = (double)(i < n ? A.x[cols[i]] : A.x[urow[i - n]]);
+    if (clamp) {
+        const int ia = lane, ib = lane + kWave;
+        const int xa = ia < R ? (ia < n ? cols[ia] : urow[ia - n]) : 0;
+        const int xb = ib < R ? (ib < n ? cols[ib] : urow[ib - n]) : 0;
+        load(0);
+        if (ia < R) v[ia] = (double)A.x[xa];
+        if (ib < R) v[ib] = (double)A.x[xb];
+    } else {
+        load(0);
+        for (int i = lane; i < R; i += kWave) v[i] = (double)(i < n ? A.x[cols[i]] : A.x[urow[i - n]]);
+    }
     wave_sync();
     double acc = 0.0;
     for (int i0 = 0; i0 < R; i0 += kSubBatchB) {
         if (i0) load(i0);
 #pragma unroll
-        for (int u = 0; u < kSubBatchB; ++u) acc += mv[u] * (i0 + u < R ? v[i0 + u] : 0.0);
+        for (int u = 0; u < kSubBatchB; ++u) {
+            const int i = i0 + u;
+            acc += (!clamp || (i < R && lane <= i && lane < n) ? (double)mv[u] : 0.0) * (i < R ? v[i] : 0.0);
+        }
     }
     if (lane < n) A.x[cols[lane]] = (T)acc;
 }
@@ -3058,6 +3091,11 @@ struct Instance::Impl {
     bool meas_by_type = true;  // measurement rows grouped by evaluation class (else by anchor bus only)
     // right-hand side by location (needs the class grouping)
     bool rhs_loc = true;  // asked for
+    // mixed form: refactor in FP64, round the solve data to FP32 for the
+    // iterations on the cached factor (else an FP32 factorization)
+    bool mixed64 = true;
+    bool mixed64_off = false;  // this gain's FP64 factor had pivots below 1e-13 G_jj
+    DBuf<double> vsave, thsave;  // the state before a mixed-form FP64 refactor
     bool use_loc = false;  // in use (asked for, and the rows are grouped by class)
     DBuf<int32_t> bv_ptr, bi_ptr, br_ptr, ytr, inc_ptr, inc;
     DBuf<double> sbus, sbr;
@@ -3269,7 +3307,7 @@ Instance::~Instance() {
     rel(d_->yG); rel(d_->yB); rel(d_->Pij); rel(d_->Qij); rel(d_->V); rel(d_->th); rel(d_->P); rel(d_->Q);
     rel(d_->Pf); rel(d_->Qf); rel(d_->Pt); rel(d_->Qt); rel(d_->mcode); rel(d_->mloc); rel(d_->hrow);
     rel(d_->hs); rel(d_->hcolp); rel(d_->cg_b); rel(d_->cg_x); rel(d_->cg_r); rel(d_->cg_p); rel(d_->cg_q);
-    rel(d_->cg_z); rel(d_->cg_t); rel(d_->cg_parts); rel(d_->cg_s); rel(d_->z); rel(d_->sigma); rel(d_->rs); rel(d_->hx); rel(d_->r); rel(d_->jpart); rel(d_->rpart);
+    rel(d_->cg_z); rel(d_->cg_t); rel(d_->cg_parts); rel(d_->cg_s); rel(d_->z); rel(d_->sigma); rel(d_->rs); rel(d_->hx); rel(d_->r); rel(d_->jpart); rel(d_->rpart); rel(d_->vsave); rel(d_->thsave);
     rel(d_->mpart); rel(d_->ipart); rel(d_->tptr); rel(d_->tpos); rel(d_->trow);
     rel(d_->tout); rel(d_->ml_d); rel(d_->lm_d); rel(d_->ztmp);
     rel(d_->bv_ptr); rel(d_->bi_ptr); rel(d_->br_ptr); rel(d_->ytr); rel(d_->inc_ptr); rel(d_->inc); rel(d_->sbus);
@@ -3779,6 +3817,7 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
         d_->f32.alloc(S, bytes_, d_->pi.sizes());
     }
     fp64_factor_active_ = false;
+    d_->mixed64_off = false;
     factor_valid_ = false;
     d_->results_valid = false;
     if (nY) k_ybus<<<nblk(nY), kBlock, 0, s>>>((int)nY, D.ycptr.p, D.ycon.p, D.gs.p, D.bs.p, D.bc.p, D.tap.p,
@@ -4169,6 +4208,31 @@ void launch_pi_inverse(Instance::Impl& D) {
 // Everything the solves need from a factor: per-front solve matrices, the
 // dense subtree matrices and the top inverse blocks (after a factorization,
 // or after the solve plan changed on the same factor).
+__global__ __launch_bounds__(kBlock) void k_round_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        b[i] = (float)a[i];
+}
+
+// The mixed form on the FP64 factor: the FP32 solve data are the FP64 factor's
+// panels, subtree solve matrices and top inverse blocks rounded once per
+// factorization.  The solves then read half the bytes, and the iteration
+// operator is the FP64 one up to one rounding per entry (no FP32
+// factorization error, which grows with the gain's condition number).
+void round_solve_data(Instance::Impl& D, size_t& bytes) {
+    PrecBufs<double>& A = D.f64;
+    PrecBufs<float>& B = D.f32;
+    auto round = [&](const DBuf<double>& a, DBuf<float>& b) {
+        if (b.n != a.n) b.alloc(a.n, bytes);
+        if (a.n)
+            k_round_f32<<<(int)std::min<int64_t>(4096, nblk((int64_t)a.n)), kBlock, 0, D.stream>>>((int64_t)a.n, a.p,
+                                                                                                    b.p);
+    };
+    round(A.F, B.F);
+    round(A.msub, B.msub);
+    round(A.pinv, B.pinv);
+    round(A.pinv_rows, B.pinv_rows);
+}
+
 template <typename T>
 void launch_solve_prep(Instance::Impl& D) {
     // the LDS task solves (every front fits) read M of every front; with dense
@@ -4582,8 +4646,10 @@ SolveOutcome Instance::solve(const SolveParams& p) {
         bool newton_off = false;
         bool stale = false;  // REALTIME: the cached factor contracts too slowly, refresh it
         for (int it = 0; it < p.max_iter; ++it) {
-            const bool fp64 = precision_ == 1 || fp64_factor_active_;
             const bool refactor = p.robust || !factor_valid_ || p.mode == 1 || stale;
+            // mixed form on the FP64 factor: refactor iterations in FP64
+            const bool split = precision_ == 0 && !fp64_factor_active_ && D.mixed64 && !D.mixed64_off;
+            const bool fp64 = precision_ == 1 || fp64_factor_active_ || (split && refactor);
             stale = false;
             // Huber: the IRLS gain until the steps are small, then reduced gain
             // weights of the outliers (same minimiser, faster contraction)
@@ -4594,10 +4660,22 @@ SolveOutcome Instance::solve(const SolveParams& p) {
                 if (!D.f64.ready) D.f64.alloc(S, bytes_, D.pi.sizes());
                 SolveParams pd = p;
                 pd.pcg_steps = 0;  // the FP64 factor solves the Newton system directly
+                if (split) {  // kept in case this factor turns out singular (below)
+                    if (D.vsave.n != D.V.n) D.vsave.alloc(D.V.n, bytes_);
+                    if (D.thsave.n != D.th.n) D.thsave.alloc(D.th.n, bytes_);
+                    HIPCHECK(hipMemcpyAsync(D.vsave.p, D.V.p, D.V.n * sizeof(double), hipMemcpyDeviceToDevice,
+                                            D.stream));
+                    HIPCHECK(hipMemcpyAsync(D.thsave.p, D.th.p, D.th.n * sizeof(double), hipMemcpyDeviceToDevice,
+                                            D.stream));
+                }
                 if (graph)
                     graph_iteration<double>(D, S, pd, 1e-13, newton);
                 else
                     iteration<double>(D, S, pd, refactor, 1e-13, pf, newton);
+                if (split) {
+                    if (!D.f32.ready) D.f32.alloc(S, bytes_, D.pi.sizes());
+                    round_solve_data(D, bytes_);
+                }
             } else {
                 if (graph)
                     graph_iteration<float>(D, S, p, 1e-5f, newton);
@@ -4623,9 +4701,22 @@ SolveOutcome Instance::solve(const SolveParams& p) {
                 --it;
                 continue;
             }
+            if (D.hst->piv && split) {
+                // the FP64 factor of this gain has pivots below 1e-13 G_jj (an
+                // unobservable part): back to the state before the step, and the
+                // mixed form factors in FP32 from here, substituting such pivots
+                HIPCHECK(hipMemcpyAsync(D.V.p, D.vsave.p, D.V.n * sizeof(double), hipMemcpyDeviceToDevice, D.stream));
+                HIPCHECK(hipMemcpyAsync(D.th.p, D.thsave.p, D.th.n * sizeof(double), hipMemcpyDeviceToDevice,
+                                        D.stream));
+                HIPCHECK(hipStreamSynchronize(D.stream));
+                D.mixed64_off = true;
+                factor_valid_ = false;
+                --it;
+                continue;
+            }
             if (D.hst->piv && !fp64) {
-                // FP32 pivots below threshold were substituted (threshold * G_jj):
-                // the factor stays SPD; the FP64 residual / RHS of the next
+                // mixed-form pivots below threshold were substituted (threshold *
+                // G_jj): the factor stays SPD; the FP64 residual / RHS of the next
                 // iterations (or CG) correct the approximate Newton step
                 out.pivots_substituted++;
             } else if (D.hst->piv) {
@@ -4647,7 +4738,7 @@ SolveOutcome Instance::solve(const SolveParams& p) {
             // the FP32 factor is too coarse for this gain: a factor fresh at this
             // point (or two steps in a row) should contract far better near the
             // solution, so continue with an FP64 factor
-            const bool coarse = refactor && it > 0 && D.hst->maxdx < kLinearRegime &&
+            const bool coarse = refactor && !split && it > 0 && D.hst->maxdx < kLinearRegime &&
                                 D.hst->maxdx > kRefreshRatio * prev && D.hst->maxdx >= p.tol;
             if (!fp64 && it > 0 && (D.hst->nan || D.hst->maxdx > 0.9 * prev || coarse)) {
                 if (coarse || ++stalls >= 2 || D.hst->nan) {
@@ -4710,7 +4801,8 @@ SolveOutcome Instance::solve(const SolveParams& p) {
     float ms = 0;
     HIPCHECK(hipEventElapsedTime(&ms, D.ev0, D.ev1));
     out.ms = ms;
-    out.flags = (D.use_flow ? 1 : 0) | (fp64_factor_active_ ? 2 : 0);
+    out.flags = (D.use_flow ? 1 : 0) | (fp64_factor_active_ ? 2 : 0) |
+                (precision_ == 0 && !fp64_factor_active_ && D.mixed64 ? 4 : 0);
     D.results_valid = true;
     return out;
 }
@@ -4944,6 +5036,13 @@ void Instance::set_rhs_by_location(bool on) {
     d_->rhs_loc = on;
 }
 
+void Instance::set_mixed_fp64_factor(bool on) {
+    ++d_->plan_epoch;
+    if (d_->mixed64 != on) factor_valid_ = false;
+    d_->mixed64 = on;
+    d_->mixed64_off = false;
+}
+
 void Instance::set_meas_by_type(bool on) {
     ++d_->plan_epoch;
     d_->meas_by_type = on;
@@ -5069,6 +5168,8 @@ std::string Instance::autotune_solve(int reps) {
     if (!factor_valid_) throw std::invalid_argument("autotune needs a factored gain matrix: solve first");
     if (!D.use_pi || D.use_flow) return "partitioned inverse not in use";
     const bool f64 = precision_ == 1 || fp64_factor_active_;
+    // mixed form on the FP64 factor: FP32 solves, FP64 factorization
+    const bool split = !f64 && D.mixed64;
     // dense bottom subtrees have no in-task schedule: only the cut matters
     const bool dense = D.pi_dense;
     const std::vector<int> cuts = dense ? std::vector<int>{8, 16, 32, 64, 128} : std::vector<int>{16, 32, 64, 128};
@@ -5112,7 +5213,7 @@ std::string Instance::autotune_solve(int reps) {
         for (int cut : {64, 128, 256, 512, 1024})
             for (int wc : {4, 8, 16, 32, 64}) {
                 build_factor_plan(D, S, cut, wc, bytes_, D.stream);
-                const double ms = f64 ? time_factor<double>(D, reps) : time_factor<float>(D, reps);
+                const double ms = f64 || split ? time_factor<double>(D, reps) : time_factor<float>(D, reps);
                 snprintf(buf, sizeof(buf), "%s%d/%d:%.3f", flog.empty() ? "" : " ", cut, wc, ms);
                 flog += buf;
                 if (ms < fbest) {
@@ -5126,10 +5227,12 @@ std::string Instance::autotune_solve(int reps) {
         flog = buf + flog;
     }
     // the factor, solve matrices and top inverse of the chosen plans
-    if (f64)
+    if (f64 || split) {
         launch_factor<double>(D, 1e-13);
-    else
+        if (split) round_solve_data(D, bytes_);
+    } else {
         launch_factor<float>(D, 1e-5f);
+    }
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(D.stream));
     snprintf(buf, sizeof(buf), "solve cut=%d waves=%d flags=%d (%.3f ms per solve) ", bc, bw, bf ? 1 : 0, best);

@@ -169,6 +169,9 @@ int Engine::set_option(const char* key, double value) {
         if (value < 1 || value > 16) return ST_INVALID_ARGUMENT;
         opt_.pinv_waves = (int)value;
         model_dirty_ = true;
+    } else if (k == "mixed_fp64_factor") {
+        opt_.mixed_fp64_factor = value != 0;
+        if (inst_) inst_->set_mixed_fp64_factor(opt_.mixed_fp64_factor);
     } else if (k == "rhs_by_location") {
         opt_.rhs_by_location = value != 0;
         model_dirty_ = true;  // the device layout is rebuilt at the next upload
@@ -682,6 +685,7 @@ int Engine::rebuild() {
     inst_->set_middle_subtrees(opt_.pinv_mid);
     inst_->set_meas_by_type(opt_.meas_by_type);
     inst_->set_rhs_by_location(opt_.rhs_by_location);
+    inst_->set_mixed_fp64_factor(opt_.mixed_fp64_factor);
     inst_->setup(h, sym_);
     inst_->set_xcd_remap(opt_.xcd_remap);
     inst_->set_state(V_.data(), th_.data());

@@ -96,6 +96,7 @@ struct EngineOptions {
     bool xcd_remap = true;        // XCD-aware block order of the gather kernels
     bool meas_by_type = true;     // measurement rows grouped by evaluation class on the device
     bool rhs_by_location = true;  // cached-factor iterations: RHS from per-location sums
+    bool mixed_fp64_factor = true;  // mixed form: FP64 factorization, FP32 solve data
 };
 
 class Engine {

@@ -176,6 +176,9 @@ class Instance {
     // (summed weighted residuals per bus and branch end, gathered per state)
     // instead of per Jacobian entry (takes effect at setup; needs the class order)
     void set_rhs_by_location(bool on);
+    // Mixed form (precision 0): refactor in FP64 and round the solve data to
+    // FP32 for the iterations on the cached factor (on), or factor in FP32 (off).
+    void set_mixed_fp64_factor(bool on);
     // sizes of the current solve plan: [0] bottom subtrees, [1] entries of their
     // solve matrices (one copy), [2] sum of their root update rows, [3] top
     // supernodes, [4] top columns, [5] top block entries, [6] top row entries

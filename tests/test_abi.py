@@ -163,7 +163,7 @@ def test_options_documented_in_header_are_accepted():
     import re
 
     text = open(os.path.join(ROOT, "include", "sle_ext.h")).read()
-    block = text[text.index("sle_ext_SetOption") - 4000: text.index("sle_ext_SetOption")]
+    block = text[: text.index("sle_ext_SetOption")]
     keys = sorted(set(re.findall(r'^ \*   "([a-z_0-9]+)"', block, re.M)))
     assert {"pcg_steps", "pinv", "pinv_dense", "lds_flags", "xcd_remap", "meas_by_type"} <= set(keys), keys
     e = sle.Engine(double=True, initialize=False)

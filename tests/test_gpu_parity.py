@@ -518,3 +518,53 @@ def test_rhs_by_location_matches_entry_products(fp64):
     assert np.abs(V - oV).max() < STATE_TOL and np.abs(th - oth).max() < STATE_TOL
     el.close()
     ee.close()
+
+
+def test_mixed_form_on_fp64_factor():
+    """The mixed form factors in FP64 and solves with the factor's data rounded
+    to FP32 (option mixed_fp64_factor, the default): REALTIME updates keep the
+    cached factor (no refactor, flag bit 2, not the FP64 fallback of bit 1),
+    need at most one iteration more than the FP64 form, and reach its states
+    within 1e-6, and the oracle's optimum for the last telemetry."""
+    from state_estimation_with_cuda_amd import sle
+
+    g = GRIDS["geo10k"]()
+
+    def make(fp64, split=1):
+        e = sle.Engine(double=True)
+        e.load_grid(g)
+        e.set_config(use_double_precision=int(fp64))
+        e.set_option("mixed_fp64_factor", split)
+        e.upload()
+        e.set_telemetry_f64(g.z)
+        r = e.solve(mode=1, max_iterations=50, tolerance=1e-8, flat_start=True)
+        assert r["status"] == 0, r
+        return e
+
+    es, e64 = make(False), make(True)
+    rng = np.random.default_rng(35)
+    z = g.z.copy()
+    for step in range(4):
+        idx = np.sort(rng.choice(g.m, g.m // 10, replace=False)).astype(np.int32)
+        val = g.z[idx] + 0.5 * rng.standard_normal(idx.size) * g.sigma[idx]
+        z[idx] = val
+        out = []
+        for e in (es, e64):
+            e.set_telemetry_indexed(idx, val)
+            r = e.solve(mode=0, max_iterations=10, tolerance=1e-7)
+            assert r["status"] == 0, r
+            out.append((r, e.state()))
+        (rs, (Vs, ths)), (r64, (V64, th64)) = out
+        print(f"[mixed64 step {step}] iters {rs['iterations']} vs {r64['iterations']} flags {rs['flags']}", flush=True)
+        assert rs["refactors"] == 0 and rs["flags"] & 4 and not rs["flags"] & 2, rs
+        assert rs["iterations"] <= r64["iterations"] + 1
+        assert np.abs(Vs - V64).max() < STATE_TOL and np.abs(ths - th64).max() < STATE_TOL
+    g2 = gg.Grid(**{**g.__dict__})
+    g2.z = z
+    o = Oracle(g2)
+    o.solve(mode=1, max_iter=50, tol=1e-11, flat_start=True)
+    oV, oth = o.state()
+    V, th = es.state()
+    assert np.abs(V - oV).max() < STATE_TOL and np.abs(th - oth).max() < STATE_TOL
+    es.close()
+    e64.close()
