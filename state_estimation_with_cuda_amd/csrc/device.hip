@@ -526,22 +526,23 @@ struct LocWeight {
 // of a chunk are issued together (indices clamped to the last entry), so a walk
 // waits for memory once per chunk rather than once per entry.
 constexpr int kLocChunk = 4;
+constexpr int kLocChunkFlow = 2;  // flow rows of one branch end (a few at most)
 
-template <typename F>
+template <int K = kLocChunk, typename F>
 __device__ inline void loc_rows(const uint8_t* __restrict__ mcode, const double* __restrict__ zv,
                                 const double* __restrict__ sigma, int i0, int i1, F f) {
-    for (int i = i0; i < i1; i += kLocChunk) {
-        uint8_t c[kLocChunk];
-        double z[kLocChunk], sg[kLocChunk];
+    for (int i = i0; i < i1; i += K) {
+        uint8_t c[K];
+        double z[K], sg[K];
 #pragma unroll
-        for (int q = 0; q < kLocChunk; ++q) {
+        for (int q = 0; q < K; ++q) {
             const int ii = min(i + q, i1 - 1);
             c[q] = mcode[ii];
             z[q] = zv[ii];
             sg[q] = sigma[ii];
         }
 #pragma unroll
-        for (int q = 0; q < kLocChunk; ++q)
+        for (int q = 0; q < K; ++q)
             if (i + q < i1) f(c[q], z[q], sg[q]);
     }
 }
@@ -642,7 +643,7 @@ __global__ __launch_bounds__(kBlock) void k_meas_loc(LocArgs A) {
                 const double q = end ? -Vj * Vj * bsh + AA * sgb : -Vi * Vi * bsh / (a * a) - AA * sgb;
                 const double ve = end ? Vj : Vi;
                 double cP = 0.0, cQ = 0.0, cV = 0.0;
-                loc_rows(mcode, zv, sigma, i0, i1, [&](uint8_t cd, double z, double sg) {
+                loc_rows<kLocChunkFlow>(mcode, zv, sigma, i0, i1, [&](uint8_t cd, double z, double sg) {
                     const int ty = cd & 15;
                     if (ty == 4 || ty == 5) {
                         const double wr = wr_of(cd, z, sg, ty == 4 ? p : q);
