@@ -2752,11 +2752,29 @@ static void make_sub_plan(const Symbolic& S, const std::vector<int8_t>& lev, PiP
     P.fptr.push_back(0);
     P.scptr.push_back(0);
     // the bottom subtrees (level 0) first, then the middle ones (level 1): a
-    // subtree is its root's descendants of the same level
-    for (int level = 0; level < 2; ++level)
+    // subtree is its root's descendants of the same level.  Within a level the
+    // largest subtrees come first (most matrix entries), so that the waves of
+    // the solve products that start last are the short ones.
+    for (int level = 0; level < 2; ++level) {
+    std::vector<std::pair<int64_t, int>> order;
     for (int r = 0; r < nsn; ++r) {
         const int par = S.sn_parent[r];
         if (lev[r] != level || (par >= 0 && lev[par] == level)) continue;
+        int64_t n = 0;
+        stack.assign(1, r);
+        while (!stack.empty()) {
+            const int s = stack.back();
+            stack.pop_back();
+            n += S.sn_k[s];
+            for (int q = S.sn_child_ptr[s]; q < S.sn_child_ptr[s + 1]; ++q)
+                if (lev[S.sn_child[q]] == level) stack.push_back(S.sn_child[q]);
+        }
+        const int64_t R = n + S.sn_c[r] - S.sn_k[r];
+        order.emplace_back(-(n * R - n * (n - 1) / 2), r);
+    }
+    std::sort(order.begin(), order.end());
+    for (const auto& o : order) {
+        const int r = o.second;
         if (level == 0) P.nsub0 = (int)P.sroot.size() + 1;
         fronts.clear();
         stack.assign(1, r);
@@ -2801,6 +2819,7 @@ static void make_sub_plan(const Symbolic& S, const std::vector<int8_t>& lev, PiP
         P.msub += sub_col_off(n, R);
         P.fptr.push_back((int32_t)P.fsn.size());
         P.scptr.push_back((int32_t)P.scols.size());
+    }
     }
     P.nsub = (int)P.sroot.size();
     // launches of the subtree-matrix kernels grouped by the LDS they need:
