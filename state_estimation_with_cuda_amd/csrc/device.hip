@@ -2529,10 +2529,12 @@ static SweepPlan make_plan(const Symbolic& S, int cut, int wcut, int bottom_wave
     for (int s = 0; s < nsn; ++s)
         if (root_of[s] >= 0) tl[root_of[s]][S.sn_level[s]].push_back(s);
     double costA = 0;
+    std::vector<std::pair<double, int>> tcost(roots.size());
     for (size_t t = 0; t < roots.size(); ++t) {
         double c = 0;
         for (const auto& L : tl[t]) c += std::ceil((double)L.size() / bottom_waves) * kFrontUs;
         costA = std::max(costA, c);
+        tcost[t] = {-c, (int)t};
     }
     // top tree levels
     std::vector<int32_t> htop(nsn, 0);
@@ -2553,9 +2555,14 @@ static SweepPlan make_plan(const Symbolic& S, int cut, int wcut, int bottom_wave
     for (int l = Lstar; l < H; ++l) costC += std::ceil((double)top[l].size() / tail_waves) * kFrontUs;
     P.cost_us = (roots.empty() ? 0 : costA + kLaunchUs) + costB + (Lstar < H ? costC + kLaunchUs : 0);
     if (!build) return P;
-    for (size_t t = 0; t < roots.size(); ++t) append_task(P, tl[t]);
+    // bottom tasks longest first: the workgroups dispatched last are the short ones
+    std::sort(tcost.begin(), tcost.end());
+    std::vector<std::vector<std::vector<int32_t>>> tls;
+    tls.reserve(roots.size());
+    for (const auto& tc : tcost) tls.push_back(std::move(tl[tc.second]));
+    for (const auto& L : tls) append_task(P, L);
     P.n_bottom = (int)roots.size();
-    P.bottom_levels = tl;
+    P.bottom_levels = std::move(tls);
     for (int l = 0; l < Lstar; ++l) {
         P.mid_off.push_back((int)P.mid.size());
         P.mid_cnt.push_back((int)top[l].size());
@@ -2834,7 +2841,10 @@ static void make_sub_plan(const Symbolic& S, const std::vector<int8_t>& lev, PiP
         bin[q].push_back(t);
         P.bin_bytes[q] = std::max(P.bin_bytes[q], q < 3 ? b : R * n * sizeof(double));
     }
-    for (int q = 0; q < kSubBins; ++q) {
+    for (int q = 0; q < kSubBins; ++q) {  // largest first within a bin
+        std::stable_sort(bin[q].begin(), bin[q].end(), [&](int a, int b) {
+            return (int64_t)P.sR[a] * P.sn_n[a] > (int64_t)P.sR[b] * P.sn_n[b];
+        });
         P.bin_off[q] = (int)P.bins.size();
         P.bins.insert(P.bins.end(), bin[q].begin(), bin[q].end());
     }
