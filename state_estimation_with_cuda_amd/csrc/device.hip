@@ -2528,6 +2528,13 @@ static SweepPlan make_plan(const Symbolic& S, int cut, int wcut, int bottom_wave
     for (size_t t = 0; t < roots.size(); ++t) tl[t].resize(S.sn_level[roots[t]] + 1);
     for (int s = 0; s < nsn; ++s)
         if (root_of[s] >= 0) tl[root_of[s]][S.sn_level[s]].push_back(s);
+    // the fronts of a level go to the waves in turn: largest fronts first, so
+    // that the waves' shares of a level end close together
+    auto by_size = [&](std::vector<int32_t>& L) {
+        std::stable_sort(L.begin(), L.end(), [&](int a, int b) { return S.sn_c[a] > S.sn_c[b]; });
+    };
+    for (auto& T : tl)
+        for (auto& L : T) by_size(L);
     double costA = 0;
     std::vector<std::pair<double, int>> tcost(roots.size());
     for (size_t t = 0; t < roots.size(); ++t) {
@@ -2548,6 +2555,7 @@ static SweepPlan make_plan(const Symbolic& S, int cut, int wcut, int bottom_wave
     std::vector<std::vector<int32_t>> top(H);
     for (int s = 0; s < nsn; ++s)
         if (!inside(s)) top[htop[s]].push_back(s);
+    for (auto& L : top) by_size(L);
     int Lstar = H;
     while (Lstar > 0 && (int)top[Lstar - 1].size() <= wcut) --Lstar;
     double costB = Lstar * (kLaunchUs + kFrontUs);
