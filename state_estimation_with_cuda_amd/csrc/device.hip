@@ -97,6 +97,7 @@ struct DevStatus {
     int nan;
     int piv;
     int flow_err;
+    uint32_t seq;  // iterations finished (k_maxfinish); the host copy's is written last
 };
 
 // ---------------------------------------------------------------------------
@@ -2222,7 +2223,11 @@ __global__ __launch_bounds__(kBlock) void k_maxfinish(int nparts, const double* 
             host_out->nan = st->nan;
             host_out->piv = st->piv;
             host_out->flow_err = st->flow_err;
+            // the sequence number last, after the fields are visible to the host
+            const uint32_t sq = st->seq + 1;
+            st->seq = sq;
             __threadfence_system();
+            __hip_atomic_store(&host_out->seq, sq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         }
     }
 }
@@ -3324,6 +3329,10 @@ Instance::Instance(int device, int precision) : device_(device), precision_(prec
     d_->hst = static_cast<DevStatus*>(EnginePool::pinned(sizeof(DevStatus)));
     HIPCHECK(hipHostGetDevicePointer((void**)&d_->hst_dev, d_->hst, 0));
     d_->st.alloc(1, bytes_);
+    // the iteration sequence starts at zero on both sides (outside any capture)
+    HIPCHECK(hipMemsetAsync(d_->st.p, 0, sizeof(DevStatus), d_->stream));
+    std::memset(d_->hst, 0, sizeof(DevStatus));
+    HIPCHECK(hipStreamSynchronize(d_->stream));
     for (int p = 0; p < PH_COUNT; ++p)
         for (int q = 0; q < 2; ++q) HIPCHECK(hipEventCreate(&d_->pev[p][q]));
 }
@@ -4652,8 +4661,22 @@ void graph_iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p,
         }
         D.graphs.push_back({ae, pe, prec, p.robust, (int)newton, p.gamma, p.vmin, p.vmax, exec});
     }
+    // k_maxfinish, the graph's last kernel, writes the status into D.hst and
+    // then the next sequence number: the host waits for that number instead
+    // of a stream synchronization, which wakes it several microseconds later
+    // (the GPU idles between iterations meanwhile).  A status never seen within
+    // the spin budget falls back to the synchronization.
+    volatile uint32_t* seq = &D.hst->seq;
+    const uint32_t want = *seq + 1;
     HIPCHECK(hipGraphLaunch(exec, D.stream));
-    HIPCHECK(hipStreamSynchronize(D.stream));  // k_maxfinish wrote the status into D.hst
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t spin = 0; *seq != want; ++spin) {
+        if ((spin & 1023) == 1023 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(200)) {
+            HIPCHECK(hipStreamSynchronize(D.stream));
+            break;
+        }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
 }
 
 }  // namespace
