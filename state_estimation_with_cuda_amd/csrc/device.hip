@@ -3057,6 +3057,7 @@ struct Instance::Impl {
     int32_t* pin_idx = nullptr;  // pinned staging of indexed telemetry
     double* pin_val = nullptr;
     size_t pin_cap = 0;
+    bool pin_busy = false;  // a scatter queued on the stream may still read the staging buffers
     int32_t* pin_idx_dev = nullptr;  // device mappings of the pinned staging
     double* pin_val_dev = nullptr;
     DBuf<double> ztmp;
@@ -3965,7 +3966,14 @@ void Instance::set_z_indexed(const int32_t* idx, const double* val, int count) {
     StreamScope scope_(d_->stream);
     Impl& D = *d_;
     // through pinned staging that the scatter kernel reads in place (no copy
-    // engine: a large copy of another engine on the device cannot delay it)
+    // engine: a large copy of another engine on the device cannot delay it).
+    // The previous call's scatter must have read the buffers before they are
+    // refilled; the call itself returns without waiting (its scatter runs
+    // ahead of the next solve on the same stream).
+    if (D.pin_busy) {
+        HIPCHECK(hipStreamSynchronize(D.stream));
+        D.pin_busy = false;
+    }
     if (D.pin_cap < (size_t)count) {
         if (D.pin_idx) HIPCHECK(hipHostFree(D.pin_idx));
         if (D.pin_val) HIPCHECK(hipHostFree(D.pin_val));
@@ -3981,7 +3989,7 @@ void Instance::set_z_indexed(const int32_t* idx, const double* val, int count) {
     std::memcpy(D.pin_val, val, (size_t)count * sizeof(double));
     k_scatter_z_mapped<<<nblk(count), kBlock, 0, D.stream>>>(count, D.pin_idx_dev, D.pin_val_dev, D.lm_d.p, D.z.p);
     HIPCHECK(hipGetLastError());
-    HIPCHECK(hipStreamSynchronize(D.stream));  // the staging buffers are reused by the next call
+    D.pin_busy = true;
 }
 
 void Instance::set_state(const double* V, const double* th) {
