@@ -33,6 +33,10 @@ typedef struct SLE_ExtResult {
     double host_ms;      /* wall time of the call */
     int32_t pivots_substituted; /* mixed form: iterations whose FP32 factor had pivots substituted */
     int32_t pcg_steps;   /* conjugate-gradient steps per Newton step (0: direct solve) */
+    int32_t graph_captures; /* iteration graphs captured during this solve (0 when replays reused them) */
+    int32_t seq_fallbacks;  /* replayed iterations whose status the host waited for by a stream
+                               synchronization after its spin budget (expected 0) */
+    int32_t retries;        /* flat-start restarts after divergence (PRECISION; wls_solver.cu:260-334) */
 } SLE_ExtResult;
 
 /* Engine options outside SLE_Config:

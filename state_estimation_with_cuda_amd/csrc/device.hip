@@ -2274,10 +2274,6 @@ inline int nblk(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
 // ---------------------------------------------------------------------------
 // device buffers
 // ---------------------------------------------------------------------------
-// Count of device allocations and frees in this process.  A captured launch
-// graph holds buffer addresses: one captured before the latest (re)allocation
-// anywhere is retired rather than trusted.
-static std::atomic<uint64_t> g_alloc_epoch{0};
 
 // Process-wide cache of device blocks.  hipFree waits for the whole device,
 // so an engine torn down while another serves requests would stall that
@@ -2414,20 +2410,20 @@ template <typename T>
 struct DBuf {
     T* p = nullptr;
     size_t n = 0;
-    void alloc(size_t cnt, size_t& bytes) {
+    void alloc(size_t cnt, Ledger& bytes) {
         release(bytes);
         n = cnt;
         if (cnt) {
             p = static_cast<T*>(BlockCache::get(cnt * sizeof(T)));
-            bytes += cnt * sizeof(T);
-            g_alloc_epoch.fetch_add(1, std::memory_order_relaxed);
+            bytes.bytes += cnt * sizeof(T);
+            ++bytes.epoch;
         }
     }
-    void release(size_t& bytes) {
+    void release(Ledger& bytes) {
         if (p) {
             BlockCache::put(p, n * sizeof(T));
-            bytes -= n * sizeof(T);
-            g_alloc_epoch.fetch_add(1, std::memory_order_relaxed);
+            bytes.bytes -= n * sizeof(T);
+            ++bytes.epoch;
         }
         p = nullptr;
         n = 0;
@@ -2435,7 +2431,7 @@ struct DBuf {
     void upload(const T* h, size_t cnt, hipStream_t s) {
         if (cnt) HIPCHECK(hipMemcpyAsync(p, h, cnt * sizeof(T), hipMemcpyHostToDevice, s));
     }
-    void upload(const std::vector<T>& v, size_t& bytes, hipStream_t s) {
+    void upload(const std::vector<T>& v, Ledger& bytes, hipStream_t s) {
         if (n != v.size()) alloc(v.size(), bytes);
         upload(v.data(), v.size(), s);
     }
@@ -2453,7 +2449,7 @@ struct PrecBufs {
     DBuf<T> pinv, pinv_rows, ctop, ytop;  // L_TT^-1 blocks, row-grouped copy, top vectors
     DBuf<T> msub;                         // dense solve matrices of the bottom subtrees
     bool ready = false;
-    void alloc(const Symbolic& S, size_t& bytes, const PiSizes& pz) {
+    void alloc(const Symbolic& S, Ledger& bytes, const PiSizes& pz) {
         Gval.alloc(std::max<int64_t>(1, S.nnz_g), bytes);
         F.alloc(std::max<int64_t>(1, S.ftotal), bytes);
         x.alloc(std::max(1, S.ns), bytes);
@@ -2462,14 +2458,14 @@ struct PrecBufs {
         alloc_pi(pz, bytes);
         ready = true;
     }
-    void alloc_pi(const PiSizes& pz, size_t& bytes) {
+    void alloc_pi(const PiSizes& pz, Ledger& bytes) {
         pinv.alloc(std::max<int64_t>(1, pz.nblock), bytes);
         pinv_rows.alloc(std::max<int64_t>(1, pz.nq), bytes);
         ctop.alloc(std::max(1, pz.ntc), bytes);
         ytop.alloc(std::max(1, pz.ntc), bytes);
         msub.alloc(std::max<int64_t>(1, 2 * pz.msub), bytes);  // column-major, then row-major
     }
-    void release(size_t& bytes) {
+    void release(Ledger& bytes) {
         for (DBuf<T>* b : {&Gval, &F, &x, &ubuf, &M, &pinv, &pinv_rows, &ctop, &ytop, &msub}) b->release(bytes);
         ready = false;
     }
@@ -3148,8 +3144,8 @@ struct Instance::Impl {
     // dependency-driven sweeps (all fronts fit in LDS)
     bool use_flow = false;
     // captured REALTIME iterations (one per precision / weighting variant);
-    // retired when launch plans change (plan_epoch) or any device buffer is
-    // (re)allocated (g_alloc_epoch)
+    // retired when launch plans change (plan_epoch) or any device buffer of
+    // this instance is (re)allocated (its ledger's epoch)
     struct GraphSlot {
         uint64_t alloc_epoch, plan_epoch;
         int prec, robust, newton;
@@ -3182,7 +3178,7 @@ struct Instance::Impl {
 // Task-sweep plan of the factorization: bottom subtrees of at most `cut` fronts
 // per workgroup, top levels wider than `wcut` fronts one launch each, the rest
 // in one workgroup.
-static void build_factor_plan(Instance::Impl& D, const Symbolic& S, int cut, int wcut, size_t& bytes,
+static void build_factor_plan(Instance::Impl& D, const Symbolic& S, int cut, int wcut, Ledger& bytes,
                               hipStream_t s) {
     SweepPlan P = make_plan(S, cut, wcut, D.bottom_waves, D.tail_waves, true);
     D.t_fronts.upload(P.fronts, bytes, s);
@@ -3205,7 +3201,7 @@ static void build_factor_plan(Instance::Impl& D, const Symbolic& S, int cut, int
 // Partitioned-inverse plan for the current pi_cut / pi_waves: bottom LDS tasks
 // of both element sizes and the top index lists (D.pi.ok = false when the
 // fronts or blocks do not fit).
-static void build_pi(Instance::Impl& D, const Symbolic& S, size_t& bytes, hipStream_t s) {
+static void build_pi(Instance::Impl& D, const Symbolic& S, Ledger& bytes, hipStream_t s) {
     // partitioned-inverse top solve: bottom LDS tasks of at most pi_cut fronts
     D.pi.ok = false;
     if (S.nsn > 0 && S.max_c <= kFrontB) {
@@ -3347,7 +3343,7 @@ Instance::~Instance() {
     if (!d_) return;
     (void)hipSetDevice(device_);
     if (d_->stream) (void)hipStreamSynchronize(d_->stream);
-    size_t b = bytes_;
+    Ledger b = bytes_;
     // DBuf destructors do not free; release explicitly
     auto rel = [&](auto& buf) { buf.release(b); };
     rel(d_->bus_type); rel(d_->from); rel(d_->to); rel(d_->vset); rel(d_->gs); rel(d_->bs); rel(d_->bc);
@@ -4273,7 +4269,7 @@ __global__ __launch_bounds__(kBlock) void k_round_f32(int64_t n, const double* _
 // factorization.  The solves then read half the bytes, and the iteration
 // operator is the FP64 one up to one rounding per entry (no FP32
 // factorization error, which grows with the gain's condition number).
-void round_solve_data(Instance::Impl& D, size_t& bytes) {
+void round_solve_data(Instance::Impl& D, Ledger& bytes) {
     PrecBufs<double>& A = D.f64;
     PrecBufs<float>& B = D.f32;
     auto round = [&](const DBuf<double>& a, DBuf<float>& b) {
@@ -4632,8 +4628,9 @@ void iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p, bool 
 // The captured work holds no memset node and no host-varying argument (the
 // dependency-driven sweeps, whose epochs change per launch, never come here).
 template <typename T>
-void graph_iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p, T piv_tol, bool newton) {
-    const uint64_t ae = g_alloc_epoch.load(std::memory_order_relaxed), pe = D.plan_epoch;
+void graph_iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p, T piv_tol, bool newton,
+                     uint64_t ae, SolveOutcome& out) {
+    const uint64_t pe = D.plan_epoch;
     const int prec = (int)sizeof(T);
     hipGraphExec_t exec = nullptr;
     for (size_t k = 0; k < D.graphs.size();) {
@@ -4668,6 +4665,7 @@ void graph_iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p,
             D.graphs.erase(D.graphs.begin());
         }
         D.graphs.push_back({ae, pe, prec, p.robust, (int)newton, p.gamma, p.vmin, p.vmax, exec});
+        out.graph_captures++;
     }
     // k_maxfinish, the graph's last kernel, writes the status into D.hst and
     // then the next sequence number: the host waits for that number instead
@@ -4681,6 +4679,7 @@ void graph_iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p,
     for (uint32_t spin = 0; *seq != want; ++spin) {
         if ((spin & 1023) == 1023 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(200)) {
             HIPCHECK(hipStreamSynchronize(D.stream));
+            out.seq_fallbacks++;
             break;
         }
     }
@@ -4738,7 +4737,7 @@ SolveOutcome Instance::solve(const SolveParams& p) {
                                             D.stream));
                 }
                 if (graph)
-                    graph_iteration<double>(D, S, pd, 1e-13, newton);
+                    graph_iteration<double>(D, S, pd, 1e-13, newton, bytes_.epoch, out);
                 else
                     iteration<double>(D, S, pd, refactor, 1e-13, pf, newton);
                 if (split) {
@@ -4747,7 +4746,7 @@ SolveOutcome Instance::solve(const SolveParams& p) {
                 }
             } else {
                 if (graph)
-                    graph_iteration<float>(D, S, p, 1e-5f, newton);
+                    graph_iteration<float>(D, S, p, 1e-5f, newton, bytes_.epoch, out);
                 else
                     iteration<float>(D, S, p, refactor, 1e-5f, pf, newton);
             }
@@ -4853,6 +4852,7 @@ SolveOutcome Instance::solve(const SolveParams& p) {
             retries++;
         }
     } while (retry);
+    out.retries = retries;
     // quantities at the final state
     launch_power(D);
     if (D.nb)
@@ -5001,7 +5001,7 @@ void Instance::debug_solve(const double* b, double* dx) {
     StreamScope scope_(d_->stream);
     Impl& D = *d_;
     DBuf<double> bb, xx;
-    size_t tmp = 0;
+    Ledger tmp;
     bb.alloc(D.ns, tmp);
     xx.alloc(D.ns, tmp);
     bb.upload(b, D.ns, D.stream);
@@ -5189,12 +5189,12 @@ namespace {
 // Average time of one forward + backward solve with the current plan (the
 // right-hand side is restored before every repetition).
 template <typename T>
-double time_solves(Instance::Impl& D, size_t& bytes, int reps) {
+double time_solves(Instance::Impl& D, Ledger& bytes, int reps) {
     PrecBufs<T>& B = bufs<T>(D);
     B.alloc_pi(D.pi.sizes(), bytes);
     launch_solve_prep<T>(D);
     DBuf<T> save;
-    size_t tmp = 0;
+    Ledger tmp;
     save.alloc(B.x.n, tmp);
     HIPCHECK(hipMemcpyAsync(save.p, B.x.p, B.x.n * sizeof(T), hipMemcpyDeviceToDevice, D.stream));
     launch_trisolve<T>(D);  // warm-up

@@ -954,6 +954,9 @@ int Engine::solve(const SolverConfigIn& sc, ResultOut& out) {
     out.flags = o.flags;
     out.pivots_substituted = o.pivots_substituted;
     out.pcg_steps = p.pcg_steps;
+    out.graph_captures = o.graph_captures;
+    out.seq_fallbacks = o.seq_fallbacks;
+    out.retries = o.retries;
     out.bad_data_count = 0;
     last_ = out;
     return ST_OK;

@@ -77,6 +77,7 @@ struct ResultOut {
     double max_mismatch = 0, objective = 0, ms = 0, largest_residual = 0;
     int largest_residual_idx = -1, bad_data_count = 0;
     int refactors = 0, fp64_refactors = 0, flags = 0, pivots_substituted = 0, pcg_steps = 0;
+    int graph_captures = 0, seq_fallbacks = 0, retries = 0;
 };
 
 // Engine options beyond SLE_Config (sle_ext_SetOption).

@@ -59,6 +59,18 @@ struct SolveOutcome {
     double ms = 0;
     int flags = 0;  // bit 0: dependency-driven sweeps; bit 1: FP64 factor active (mixed form fell back)
     int newton_fallbacks = 0;  // Huber-Newton gains that lost rank (IRLS used instead)
+    int graph_captures = 0;    // iteration graphs captured and instantiated during the solve
+    int seq_fallbacks = 0;     // replayed iterations whose status the host did not see within the spin budget
+    int retries = 0;           // flat-start restarts after divergence (PRECISION only)
+};
+
+// Device bytes held by an instance, and a count of its allocations and
+// releases: a captured launch graph holds buffer addresses, so one captured
+// before the instance's latest (re)allocation is retired rather than trusted.
+// Other instances' allocations do not touch it.
+struct Ledger {
+    size_t bytes = 0;
+    uint64_t epoch = 0;
 };
 
 // Kernel classes timed with HIP events on the instance stream (profiling mode).
@@ -137,7 +149,7 @@ class Instance {
     // Solve G dx = b with the current factor (b, dx in natural state order).
     void debug_solve(const double* b, double* dx);
 
-    size_t device_bytes() const { return bytes_; }
+    size_t device_bytes() const { return bytes_.bytes; }
     int precision() const { return precision_; }
     const Symbolic& symbolic() const { return *S_; }
     KernelTimes& kernel_times() { return ktimes_; }
@@ -211,7 +223,7 @@ class Instance {
     bool fp64_factor_active_ = false;  // mixed form fell back to FP64 after a pivot trigger
     bool profiling_ = false;
     bool use_graphs_ = false;
-    size_t bytes_ = 0;
+    Ledger bytes_;
     KernelTimes ktimes_;
 };
 

@@ -854,6 +854,9 @@ SLE_API SLE_StatusCode SLE_CALL sle_ext_Solve(SLE_Handle handle, const SLE_Solve
             result->flags = r.flags;
             result->pivots_substituted = r.pivots_substituted;
             result->pcg_steps = r.pcg_steps;
+            result->graph_captures = r.graph_captures;
+            result->seq_fallbacks = r.seq_fallbacks;
+            result->retries = r.retries;
             result->gpu_ms = r.ms;
             result->host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         }

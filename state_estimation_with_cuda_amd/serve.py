@@ -6,7 +6,8 @@ solves of new areas run beside it in a pool of slots, each on its own engine
 (its own HIP stream), admitted first-come first-served while their estimated
 HBM footprint fits the budget.  Work of the resident model and of the areas
 overlaps on the device.  Every request records arrival, start and completion
-times; latency is completion minus arrival.
+times; latency is completion (the estimate read back to the host) minus
+arrival.
 
 The reference engine has no serving layer (its ABI is synchronous and one
 handle is not to be used from two threads, sle_api.h:60-63); this module keeps
@@ -152,8 +153,10 @@ class Server:
                 mode = 0 if req.kind == UPDATE else self.toggle_mode
                 r = self.main.solve(mode=mode, max_iterations=self.max_iterations, tolerance=self.tol)
                 req.status, req.iterations = r["status"], r["iterations"]
+                # the request is done when its estimate is on the host
+                state = self.main.state()
                 if req.record:
-                    req.result = self.main.state()
+                    req.result = state
             except Exception as exc:  # the request fails, the loop goes on
                 req.error = f"{type(exc).__name__}: {exc}"
             req.done = self.now()
@@ -166,6 +169,17 @@ class Server:
                 while True:
                     if self._area_q:
                         need = self.estimate_bytes(self._area_q[0].payload)
+                        if need > self.budget:
+                            # the estimate grew past the whole budget after this
+                            # request was queued (an overrun raised bytes_per_bus):
+                            # it can never be admitted, so it fails instead of
+                            # holding up every request behind it
+                            head = self._area_q.popleft()
+                            head.error = "area larger than the HBM budget"
+                            head.done = self.now()
+                            self.completed.append(head)
+                            self._cv.notify_all()
+                            continue
                         if self._used + need <= self.budget:
                             break
                     elif self._closing:
@@ -182,7 +196,8 @@ class Server:
                 e = self.make_area_engine(req.payload)
                 r = e.solve(mode=1, max_iterations=50, tolerance=self.tol, flat_start=True)
                 req.status, req.iterations = r["status"], r["iterations"]
-                req.result = e.state() if req.record else None
+                state = e.state()  # the request is done when its estimate is on the host
+                req.result = state if req.record else None
                 used = e.gpu_bytes() if hasattr(e, "gpu_bytes") else 0
                 if used > need:  # learn from the overrun: later admissions use the larger figure
                     with self._cv:

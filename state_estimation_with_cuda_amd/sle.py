@@ -114,7 +114,8 @@ class ExtResult(C.Structure):
                 ("fp64_refactors", C.c_int32), ("max_mismatch", C.c_double), ("objective", C.c_double),
                 ("largest_residual", C.c_double), ("largest_residual_idx", C.c_int32), ("flags", C.c_int32),
                 ("gpu_ms", C.c_double), ("host_ms", C.c_double), ("pivots_substituted", C.c_int32),
-                ("pcg_steps", C.c_int32)]
+                ("pcg_steps", C.c_int32), ("graph_captures", C.c_int32), ("seq_fallbacks", C.c_int32),
+                ("retries", C.c_int32)]
 
 
 # every function of the reference ABI (sle_api.h:447-1391) and the extensions
@@ -517,6 +518,7 @@ class Engine:
                     fp64_refactors=r.fp64_refactors, max_mismatch=r.max_mismatch, objective=r.objective,
                     largest_residual=r.largest_residual, largest_residual_idx=r.largest_residual_idx, flags=r.flags,
                     pivots_substituted=r.pivots_substituted, pcg_steps=r.pcg_steps,
+                    graph_captures=r.graph_captures, seq_fallbacks=r.seq_fallbacks, retries=r.retries,
                     gpu_ms=r.gpu_ms, host_ms=r.host_ms)
 
     def solve_mode(self, mode):

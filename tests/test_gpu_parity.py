@@ -135,6 +135,7 @@ def test_realtime_hot_start_reuses_factor():
     e.set_telemetry_indexed(idx, z[idx])
     r1 = e.solve(mode=0, max_iterations=20, tolerance=1e-7)
     assert r1["status"] == 0 and r1["refactors"] == 0, r1
+    assert r1["seq_fallbacks"] == 0, r1  # every replayed iteration's status seen without the fallback wait
     g2 = gg.Grid(**{**g.__dict__})
     g2.z = z
     o = Oracle(g2)

@@ -114,3 +114,58 @@ def test_oversized_area_is_refused_and_failures_are_isolated():
     bad = [r for r in done if r.kind == serve.UPDATE and r.rid == 5][0]
     assert bad.error and "bad telemetry" in bad.error
     assert sum(1 for r in done if r.error is None) == len(reqs) - 2
+
+
+def test_queued_area_whose_estimate_outgrows_the_budget_fails_instead_of_blocking():
+    """An area engine that uses more HBM than estimated raises bytes_per_bus; an
+    area queued meanwhile whose new estimate exceeds the whole budget must fail
+    rather than wait forever (and hold up every area behind it)."""
+    log = []
+    main = FakeEngine("main", log)
+    started = threading.Event()
+    release = threading.Event()
+
+    class Overrun(FakeEngine):
+        def __init__(self, grid):
+            super().__init__(f"area{grid.n}", log, delay=0.0)
+            self.n = grid.n
+            if grid.n == 100:  # the first area: hold it until the big one is queued
+                started.set()
+                release.wait(5.0)
+
+        def gpu_bytes(self):
+            return self.n * 50  # five times the 10 B/bus estimate
+
+    srv = serve.Server(main, Overrun, hbm_budget=2000, area_slots=1, bytes_per_bus=10.0)
+    srv.start()
+    srv.submit(serve.Request(serve.AREA, 0.0, FakeGrid(100), rid=1))
+    assert started.wait(5.0)
+    # 150 buses: 1,500 B estimated now (fits), 150 x 55 B after the first overrun (does not)
+    srv.submit(serve.Request(serve.AREA, 0.0, FakeGrid(150), rid=2))
+    srv.submit(serve.Request(serve.AREA, 0.0, FakeGrid(20), rid=3))
+    release.set()
+    t0 = time.time()
+    srv.close(timeout=10.0)
+    assert time.time() - t0 < 10.0
+    done = {r.rid: r for r in srv.completed}
+    assert set(done) == {1, 2, 3}
+    assert done[1].error is None and srv.estimate_misses >= 1
+    assert done[2].error and "budget" in done[2].error
+    assert done[3].error is None  # the area behind it still ran
+
+
+def test_updates_finish_with_the_estimate_on_the_host():
+    """Latency runs to the read-back of the state: every update reads it."""
+    calls = []
+
+    class Main(FakeEngine):
+        def state(self):
+            calls.append("state")
+            return np.zeros(2), np.zeros(2)
+
+    srv = serve.Server(Main("main", []), lambda g: None, hbm_budget=1, area_slots=1)
+    reqs = [serve.Request(serve.UPDATE, 0.001 * k, (np.array([k], np.int32), np.array([1.0])), rid=k)
+            for k in range(5)]
+    done = serve.run(srv, reqs, freeze_gc=False)
+    assert all(r.error is None for r in done)
+    assert calls.count("state") == 5
