@@ -110,6 +110,24 @@ SLE_API SLE_StatusCode SLE_CALL sle_ext_AutotuneSolve(SLE_Handle handle, int32_t
 SLE_API SLE_StatusCode SLE_CALL sle_ext_Solve(SLE_Handle handle, const SLE_SolverConfig* config,
                                               SLE_ExtResult* result);
 
+/* Allocation-free variant of sle_ext_Solve.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It runs the REALTIME Gauss-Newton iterations on the handle's cached gain
+ * factor exactly as sle_ext_Solve does, but makes no device allocation or
+ * free: the iteration status record and the per-block partials of the
+ * reductions live in a static device array of 1 MiB per GPU (enough for about
+ * 12 million measurement rows), and no HIP graph is instantiated.  Every such
+ * call on one GPU shares that array, so calls that overlap on one GPU (two
+ * handles on two threads) race; the array keeps the last call's contents and
+ * each call initializes what it reads.  Returns SLE_ERROR_INVALID_ARGUMENT,
+ * before launching anything, when the request would need an allocation: a
+ * mode other than SLE_MODE_REALTIME, use_robust, no cached factor (solve once
+ * with sle_ext_Solve first), pending model / parameter / breaker changes, or a
+ * model whose partials exceed the array. */
+SLE_API SLE_StatusCode SLE_CALL sle_ext_SolveNoAlloc(SLE_Handle handle, const SLE_SolverConfig* config,
+                                                     SLE_ExtResult* result);
+
 /* Telemetry: values[q] for measurement idx[q] (count may be any size). */
 SLE_API SLE_StatusCode SLE_CALL sle_ext_UpdateTelemetryIndexed(SLE_Handle handle, const int32_t* idx,
                                                                const double* values, int32_t count);

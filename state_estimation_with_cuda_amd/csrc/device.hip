@@ -2411,6 +2411,7 @@ struct DBuf {
     T* p = nullptr;
     size_t n = 0;
     void alloc(size_t cnt, Ledger& bytes) {
+        if (bytes.frozen) throw NoAllocError("the request needs a device allocation (allocation-free solve)");
         release(bytes);
         n = cnt;
         if (cnt) {
@@ -2420,6 +2421,7 @@ struct DBuf {
         }
     }
     void release(Ledger& bytes) {
+        if (p && bytes.frozen) throw NoAllocError("the request would release a device buffer (allocation-free solve)");
         if (p) {
             BlockCache::put(p, n * sizeof(T));
             bytes.bytes -= n * sizeof(T);
@@ -3318,6 +3320,7 @@ static void build_pi(Instance::Impl& D, const Symbolic& S, Ledger& bytes, hipStr
 }
 
 Instance::Instance(int device, int precision) : device_(device), precision_(precision) {
+    init_device_scratch(device_);
     HIPCHECK(hipSetDevice(device_));
     d_ = new Impl();
     d_->stream = EnginePool::stream();
@@ -4874,6 +4877,121 @@ SolveOutcome Instance::solve(const SolveParams& p) {
                 (precision_ == 0 && !fp64_factor_active_ && D.mixed64 ? 4 : 0);
     D.results_valid = true;
     return out;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free solve
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free solve: the iteration status record and the
+// per-block partials of the reductions (max |dx| per block of buses; objective,
+// largest scaled residual and its row per block of measurement rows).  Each
+// device holds its own copy until the process ends.  In host code `scratch`
+// names no device address: kernels get the address that hipGetSymbolAddress
+// returned for the current device (scratch_address).
+constexpr size_t kScratchBytes = size_t(1) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, unsigned char*>& scratch_table() {
+    static auto* t = new std::map<int, unsigned char*>();  // lives to process end
+    return *t;
+}
+
+unsigned char* scratch_address(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = scratch_table().find(device);
+    return it == scratch_table().end() ? nullptr : it->second;
+}
+
+// Offsets into the scratch array (each 256-byte aligned).
+struct ScratchLayout {
+    size_t st = 0, mpart = 0, jpart = 0, rpart = 0, ipart = 0, total = 0;
+};
+
+ScratchLayout scratch_layout(int n, int m) {
+    auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+    ScratchLayout L;
+    size_t off = 0;
+    L.st = off;
+    off += up(sizeof(DevStatus));
+    L.mpart = off;
+    off += up(sizeof(double) * (size_t)nblk(std::max(1, n)));  // k_update: one partial per block of buses
+    const size_t mp = (size_t)nblk(std::max(1, m));             // k_meas: one partial per block of rows
+    L.jpart = off;
+    off += up(sizeof(double) * mp);
+    L.rpart = off;
+    off += up(sizeof(double) * mp);
+    L.ipart = off;
+    off += up(sizeof(int) * mp);
+    L.total = off;
+    return L;
+}
+}  // namespace
+
+void init_device_scratch(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (scratch_table().count(device)) return;
+    HIPCHECK(hipSetDevice(device));
+    void* p = nullptr;
+    HIPCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    scratch_table()[device] = static_cast<unsigned char*>(p);
+}
+
+size_t device_scratch_bytes() { return kScratchBytes; }
+
+SolveOutcome Instance::solve_no_alloc(const SolveParams& p) {
+    if (p.mode != 0 || p.robust != 0 || p.pcg_steps > 0)
+        throw NoAllocError("the allocation-free solve runs REALTIME WLS iterations on the cached factor "
+                           "(no PRECISION mode, no Huber, no CG refinement)");
+    if (!factor_valid_) throw NoAllocError("no cached gain factor: solve once with sle_ext_Solve first");
+    Impl& D = *d_;
+    const ScratchLayout L = scratch_layout(D.n, D.m);
+    if (L.total > kScratchBytes)
+        throw NoAllocError("the model's reduction partials need " + std::to_string(L.total) +
+                           " bytes of scratch; the allocation-free solve has " + std::to_string(kScratchBytes));
+    unsigned char* base = scratch_address(device_);
+    if (!base) throw NoAllocError("the device's scratch address was not looked up (engine not initialized)");
+    HIPCHECK(hipSetDevice(device_));
+    // the kernels take the scratch addresses where the instance's own buffers
+    // go otherwise; the status record (iteration sequence included) is carried
+    // over in both directions so that later replays of the instance agree
+    DevStatus* own_st = D.st.p;
+    double* own_m = D.mpart.p;
+    double* own_j = D.jpart.p;
+    double* own_r = D.rpart.p;
+    int* own_i = D.ipart.p;
+    DevStatus* sst = reinterpret_cast<DevStatus*>(base + L.st);
+    HIPCHECK(hipMemcpyAsync(sst, own_st, sizeof(DevStatus), hipMemcpyDeviceToDevice, D.stream));
+    const bool graphs = use_graphs_;
+    struct Restore {
+        Instance& I;
+        Impl& D;
+        DevStatus *own_st, *sst;
+        double *own_m, *own_j, *own_r;
+        int* own_i;
+        bool graphs;
+        ~Restore() {
+            (void)hipMemcpyAsync(own_st, sst, sizeof(DevStatus), hipMemcpyDeviceToDevice, D.stream);
+            D.st.p = own_st;
+            D.mpart.p = own_m;
+            D.jpart.p = own_j;
+            D.rpart.p = own_r;
+            D.ipart.p = own_i;
+            I.use_graphs_ = graphs;
+            I.bytes_.frozen = false;
+        }
+    } restore{*this, D, own_st, sst, own_m, own_j, own_r, own_i, graphs};
+    D.st.p = sst;
+    D.mpart.p = reinterpret_cast<double*>(base + L.mpart);
+    if (D.m) {
+        D.jpart.p = reinterpret_cast<double*>(base + L.jpart);
+        D.rpart.p = reinterpret_cast<double*>(base + L.rpart);
+        D.ipart.p = reinterpret_cast<int*>(base + L.ipart);
+    }
+    use_graphs_ = false;
+    bytes_.frozen = true;
+    return solve(p);
 }
 
 void Instance::get_results(double* P, double* Q, double* Pf, double* Qf, double* Pt, double* Qt, double* hx,

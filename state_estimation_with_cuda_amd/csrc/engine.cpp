@@ -129,6 +129,7 @@ int Engine::initialize() {
     int nd = device_count();
     if (nd <= 0) return ST_GPU_FAILURE;
     if (cfg_.device_id < 0 || cfg_.device_id >= nd) return ST_INVALID_ARGUMENT;
+    init_device_scratch(cfg_.device_id);  // the allocation-free solve's scratch address on this device
     initialized_ = true;
     return ST_OK;
 }
@@ -928,6 +929,13 @@ int Engine::solve(const SolverConfigIn& sc, ResultOut& out) {
         out.status = 4;  // NOT_OBSERVABLE
         return rc;
     }
+    SolveParams p = solve_params(sc);
+    SolveOutcome o = inst_->solve(p);
+    take_outcome(o, p, out);
+    return ST_OK;
+}
+
+SolveParams Engine::solve_params(const SolverConfigIn& sc) const {
     SolveParams p;
     p.mode = sc.mode;
     p.max_iter = std::max(1, sc.max_iterations);
@@ -939,7 +947,33 @@ int Engine::solve(const SolverConfigIn& sc, ResultOut& out) {
     p.vmax = cfg_.voltage_max_pu;
     p.time_limit_ms = sc.mode == 0 ? sc.time_limit_ms : 0.0;
     p.pcg_steps = cfg_.use_double_precision ? 0 : opt_.pcg_steps;
-    SolveOutcome o = inst_->solve(p);
+    return p;
+}
+
+int Engine::solve_no_alloc(const SolverConfigIn& sc, ResultOut& out) {
+    out = ResultOut();
+    if (!initialized_) return ST_NOT_INITIALIZED;
+    bool pending = topo_dirty_ || !topo_changed_.empty();
+    for (int s : sd_pending_) pending = pending || s >= 0;
+    if (!inst_ || model_dirty_ || params_dirty_ || pending)
+        throw NoAllocError("the allocation-free solve needs the uploaded and solved model without pending model, "
+                           "parameter or breaker changes (solve with sle_ext_Solve first)");
+    struct Unfreeze {
+        Instance* i;
+        ~Unfreeze() { i->freeze_allocations(false); }
+    } unfreeze{inst_.get()};
+    inst_->freeze_allocations(true);
+    if (z_dirty_) {
+        inst_->set_z(m_z_.data());
+        z_dirty_ = false;
+    }
+    SolveParams p = solve_params(sc);
+    SolveOutcome o = inst_->solve_no_alloc(p);
+    take_outcome(o, p, out);
+    return ST_OK;
+}
+
+void Engine::take_outcome(const SolveOutcome& o, const SolveParams& p, ResultOut& out) {
     state_dirty_ = true;
     results_dirty_ = true;
     out.status = o.status;
@@ -959,7 +993,6 @@ int Engine::solve(const SolverConfigIn& sc, ResultOut& out) {
     out.retries = o.retries;
     out.bad_data_count = 0;
     last_ = out;
-    return ST_OK;
 }
 
 int Engine::autotune(int reps, std::string& report) {

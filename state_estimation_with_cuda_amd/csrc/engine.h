@@ -156,6 +156,9 @@ class Engine {
 
     // solve
     int solve(const SolverConfigIn& sc, ResultOut& out);
+    // The same REALTIME solve without any device allocation (Instance::solve_no_alloc);
+    // throws NoAllocError when the request or the engine's state needs one.
+    int solve_no_alloc(const SolverConfigIn& sc, ResultOut& out);
     SolverConfigIn mode_config(int mode) const;
     int check_observability(int& observable, int& islands);
     int apply_flat_start();
@@ -217,6 +220,8 @@ class Engine {
     void load_ieee14();
 
    private:
+    SolveParams solve_params(const SolverConfigIn& sc) const;
+    void take_outcome(const SolveOutcome& o, const SolveParams& p, ResultOut& out);
     void build_host_arrays(HostArrays& h) const;
     void admittance(int k, double& g, double& b) const;
     int rebuild();

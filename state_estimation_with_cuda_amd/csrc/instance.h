@@ -71,6 +71,7 @@ struct SolveOutcome {
 struct Ledger {
     size_t bytes = 0;
     uint64_t epoch = 0;
+    bool frozen = false;  // set during an allocation-free solve: any (re)allocation throws NoAllocError
 };
 
 // Kernel classes timed with HIP events on the instance stream (profiling mode).
@@ -106,6 +107,14 @@ struct GpuError : std::runtime_error {
     explicit GpuError(const std::string& s) : std::runtime_error(s) {}
 };
 
+// Raised when the allocation-free solve cannot run without allocating (no
+// cached factor, a mode that refactors, a model whose reduction partials
+// exceed the scratch array); mapped to SLE_ERROR_INVALID_ARGUMENT.  Thrown
+// before any device allocation is made.
+struct NoAllocError : std::runtime_error {
+    explicit NoAllocError(const std::string& s) : std::runtime_error(s) {}
+};
+
 class Instance {
    public:
     struct Impl;
@@ -134,6 +143,15 @@ class Instance {
     bool factor_valid() const { return factor_valid_; }
 
     SolveOutcome solve(const SolveParams& p);
+    // The same REALTIME iterations on the cached factor without any device
+    // allocation: the status record and the reductions' per-block partials
+    // live in the device's static scratch array (kScratchBytes), shared by all
+    // instances on the device, so two such calls that overlap on one device
+    // race.  No captured graph is used (instantiating one allocates).  Throws
+    // NoAllocError, before launching anything, when the request needs an
+    // allocation: a mode other than REALTIME WLS, no valid cached factor, or
+    // partials larger than the scratch.
+    SolveOutcome solve_no_alloc(const SolveParams& p);
 
     // Results of the last solve at the final state (downloaded on demand).
     void get_results(double* P, double* Q, double* Pf, double* Qf, double* Pt, double* Qt, double* hx, double* r);
@@ -150,6 +168,8 @@ class Instance {
     void debug_solve(const double* b, double* dx);
 
     size_t device_bytes() const { return bytes_.bytes; }
+    // while on, any device (re)allocation of this instance throws NoAllocError
+    void freeze_allocations(bool on) { bytes_.frozen = on; }
     int precision() const { return precision_; }
     const Symbolic& symbolic() const { return *S_; }
     KernelTimes& kernel_times() { return ktimes_; }
@@ -226,6 +246,12 @@ class Instance {
     Ledger bytes_;
     KernelTimes ktimes_;
 };
+
+// Looks up the address of the allocation-free solve's static scratch array on
+// `device` (once per device; called at engine initialization).
+void init_device_scratch(int device);
+// Size of that array in bytes (per device).
+size_t device_scratch_bytes();
 
 // Device helpers exposed to the engine.
 int device_count();

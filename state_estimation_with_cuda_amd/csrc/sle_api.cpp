@@ -52,6 +52,8 @@ SLE_StatusCode guarded(F&& f) {
         return f();
     } catch (const std::bad_alloc&) {
         return fail(SLE_ERROR_OUT_OF_MEMORY, "host or device allocation failed");
+    } catch (const sle::NoAllocError& e) {
+        return fail(SLE_ERROR_INVALID_ARGUMENT, e.what());
     } catch (const sle::GpuError& e) {
         return fail(SLE_ERROR_CUDA_FAILURE, e.what());
     } catch (const std::exception& e) {
@@ -834,13 +836,14 @@ SLE_API int32_t SLE_CALL sle_IsGPUAvailable(void) {
 // =========================================================================
 // Extensions (include/sle_ext.h)
 // =========================================================================
-SLE_API SLE_StatusCode SLE_CALL sle_ext_Solve(SLE_Handle handle, const SLE_SolverConfig* config,
-                                              SLE_ExtResult* result) {
+namespace {
+SLE_StatusCode ext_solve(SLE_Handle handle, const SLE_SolverConfig* config, SLE_ExtResult* result, bool no_alloc) {
     if (!handle || !config) return fail(SLE_ERROR_INVALID_ARGUMENT, "NULL argument");
     return guarded([&]() -> SLE_StatusCode {
         auto t0 = std::chrono::steady_clock::now();
         sle::ResultOut r;
-        int rc = E(handle).solve(from_solver_config(config), r);
+        int rc = no_alloc ? E(handle).solve_no_alloc(from_solver_config(config), r)
+                          : E(handle).solve(from_solver_config(config), r);
         if (result) {
             std::memset(result, 0, sizeof(*result));
             result->status = r.status;
@@ -863,6 +866,17 @@ SLE_API SLE_StatusCode SLE_CALL sle_ext_Solve(SLE_Handle handle, const SLE_Solve
         if (rc != sle::ST_OK) return from_int(rc, "solve");
         return map_status(r.status);
     });
+}
+}  // namespace
+
+SLE_API SLE_StatusCode SLE_CALL sle_ext_Solve(SLE_Handle handle, const SLE_SolverConfig* config,
+                                              SLE_ExtResult* result) {
+    return ext_solve(handle, config, result, false);
+}
+
+SLE_API SLE_StatusCode SLE_CALL sle_ext_SolveNoAlloc(SLE_Handle handle, const SLE_SolverConfig* config,
+                                                     SLE_ExtResult* result) {
+    return ext_solve(handle, config, result, true);
 }
 
 SLE_API const char* SLE_CALL sle_ext_Describe(SLE_Handle handle) {

@@ -144,7 +144,7 @@ EXT_FUNCTIONS = [
     "sle_ext_GetHPattern", "sle_ext_GetGPattern", "sle_ext_GetOrdering", "sle_ext_Evaluate",
     "sle_ext_JacobianGain", "sle_ext_Factor", "sle_ext_SolveFactor", "sle_ext_StageTelemetry",
     "sle_ext_ApplyStagedTelemetry", "sle_ext_GetKernelTimes", "sle_ext_ResetKernelTimes", "sle_ext_SetOption",
-    "sle_ext_Describe", "sle_ext_AutotuneSolve", "sle_ext_GetSolvePlanSizes",
+    "sle_ext_Describe", "sle_ext_AutotuneSolve", "sle_ext_GetSolvePlanSizes", "sle_ext_SolveNoAlloc",
 ]
 KERNEL_CLASSES = ["power", "meas", "spgemm", "factor", "rhs", "fwd", "bwd", "update", "pcg",
                   "sub_fwd", "pi_fwd", "pi_bwd", "sub_bwd", "mid_fwd", "mid_bwd"]
@@ -253,6 +253,7 @@ class Library:
             "sle_GetCUDAVersion": (i32, []),
             "sle_IsGPUAvailable": (i32, []),
             "sle_ext_Solve": (C.c_int, [H, P(S["SolverConfig"]), P(ExtResult)]),
+            "sle_ext_SolveNoAlloc": (C.c_int, [H, P(S["SolverConfig"]), P(ExtResult)]),
             "sle_ext_UpdateTelemetryIndexed": (C.c_int, [H, ip, dp, i32]),
             "sle_ext_UpdateTelemetryF64": (C.c_int, [H, dp, i32]),
             "sle_ext_SetState": (C.c_int, [H, dp, dp, i32]),
@@ -507,13 +508,15 @@ class Engine:
 
     # -- solve ---------------------------------------------------------------
     def solve(self, mode=MODE_PRECISION, max_iterations=100, tolerance=1e-6, robust=False, gamma=1.5,
-              flat_start=False, time_limit_ms=0.0, check=False):
+              flat_start=False, time_limit_ms=0.0, check=False, no_alloc=False):
+        """sle_ext_Solve, or with no_alloc its allocation-free variant sle_ext_SolveNoAlloc."""
         sc = self.S["SolverConfig"](int(mode), int(max_iterations), tolerance, gamma, int(bool(robust)),
                                     int(bool(flat_start)), time_limit_ms)
         r = ExtResult()
-        rc = self.L.sle_ext_Solve(self.h, C.byref(sc), C.byref(r))
+        fn = self.L.sle_ext_SolveNoAlloc if no_alloc else self.L.sle_ext_Solve
+        rc = fn(self.h, C.byref(sc), C.byref(r))
         if check:
-            self._check(rc, "sle_ext_Solve")
+            self._check(rc, "sle_ext_SolveNoAlloc" if no_alloc else "sle_ext_Solve")
         return dict(rc=rc, status=r.status, iterations=r.iterations, refactors=r.refactors,
                     fp64_refactors=r.fp64_refactors, max_mismatch=r.max_mismatch, objective=r.objective,
                     largest_residual=r.largest_residual, largest_residual_idx=r.largest_residual_idx, flags=r.flags,
