@@ -23,7 +23,6 @@ give the HBM traffic per launch recorded in profiles/hbm_traffic_200k.md.
 from __future__ import annotations
 
 import argparse
-import dataclasses
 import json
 import os
 import sys
@@ -388,23 +387,6 @@ def run_areas(args):
     print(json.dumps(out), flush=True)
 
 
-def non_bridge_branches(g, count, rng):
-    """Branches whose opening keeps the grid connected."""
-    import scipy.sparse as sp
-    import scipy.sparse.csgraph as cg
-
-    out = []
-    for k in rng.permutation(g.nb):
-        keep = np.ones(g.nb, bool)
-        keep[k] = False
-        A = sp.coo_matrix((np.ones(keep.sum()), (g.br_from[keep], g.br_to[keep])), shape=(g.n, g.n))
-        if cg.connected_components(A, directed=False)[0] == 1:
-            out.append(int(k))
-            if len(out) == count:
-                return out
-    return out
-
-
 def run_serve(args):
     """Serving loop on the resident 200k-bus model: Poisson arrivals of
     incremental updates (10 % of z re-measured), a breaker operation after every
@@ -437,46 +419,10 @@ def run_serve(args):
     cal.close()
     log(f"serve: {len(pool)} area models, {bytes_per_bus:.0f} B/bus estimate")
     # request stream
-    t_upd = serve.poisson_arrivals(args.rate, args.duration, rng)
-    t_area = serve.poisson_arrivals(args.area_rate, args.duration, rng) if args.area_rate > 0 else np.zeros(0)
-    brks = non_bridge_branches(g, max(1, len(t_upd) // 100 + 1), rng)
-    k = max(1, int(round(0.10 * g.m)))
-    z = g.z.copy()  # the telemetry as the resident model sees it after each request
-    zb = g.z.copy()  # the telemetry base for the current breaker status: h(x_true) + the original noise
-    noise = g.z - g.z_true
-    inj = np.isin(g.m_type, [gridgen.P_INJ, gridgen.Q_INJ, gridgen.P_PSEUDO, gridgen.Q_PSEUDO])
-    status_arr = g.status.copy()
-    status = {}
-    reqs, replay = [], {}
-    open_brk = None
-    for q, t in enumerate(t_upd):
-        idx = np.sort(rng.choice(g.m, k, replace=False)).astype(np.int32)
-        val = zb[idx] + rng.normal(0.0, 1.0, k) * g.sigma[idx]
-        z[idx] = val
-        rec = q % args.record_every == args.record_every // 2
-        reqs.append(serve.Request(serve.UPDATE, float(t), (idx, val), rid=q, record=rec))
-        if rec and len(replay) < args.replays:
-            replay[q] = (z.copy(), dict(status))
-        if args.toggle_every and (q + 1) % args.toggle_every == 0 and brks:
-            if open_brk is None:
-                open_brk = brks[(q // 100) % len(brks)]
-                op = (open_brk, 0)
-            else:
-                op = (open_brk, 1)
-                open_brk = None
-            status[op[0]] = op[1]
-            # the injections at the branch's ends follow the new topology (the network
-            # state stays x_true): the operation carries those telemetry values
-            status_arr[op[0]] = op[1]
-            gt = dataclasses.replace(g, status=status_arr)
-            ends = (g.br_from[op[0]], g.br_to[op[0]])
-            rows = np.nonzero(inj & np.isin(g.m_loc, ends))[0].astype(np.int32)
-            zb[rows] = gridgen.measurement_function(gt, g.V_true, g.th_true, m_type=g.m_type[rows],
-                                                    m_loc=g.m_loc[rows], m_end=g.m_end[rows]) + noise[rows]
-            z[rows] = zb[rows]
-            reqs.append(serve.Request(serve.TOGGLE, float(t) + 1e-6, op + (rows, zb[rows].copy()), rid=100_000 + q))
-    for j, t in enumerate(t_area):
-        reqs.append(serve.Request(serve.AREA, float(t), pool[j % len(pool)], rid=200_000 + j, record=j < 3))
+    wl = serve.make_workload(g, rng, args.rate, args.duration, toggle_every=args.toggle_every,
+                             area_rate=args.area_rate, area_pool=pool, record_every=args.record_every,
+                             replays=args.replays)
+    reqs, replay, status = wl.requests, wl.replay, wl.breakers
     srv = serve.Server(main_e, engine, hbm_budget=int(args.hbm_budget_gib * (1 << 30)), area_slots=args.area_slots,
                        bytes_per_bus=bytes_per_bus, tol=TOL, max_iterations=10,
                        toggle_mode=args.toggle_mode)

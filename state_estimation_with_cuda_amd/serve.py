@@ -216,6 +216,87 @@ class Server:
                 self._cv.notify_all()
 
 
+def non_bridge_branches(g, count: int, rng: np.random.Generator) -> list:
+    """Branches whose opening keeps the grid connected (a breaker operation on
+    one of them changes no island)."""
+    import scipy.sparse as sp
+    import scipy.sparse.csgraph as cg
+
+    out = []
+    for k in rng.permutation(g.nb):
+        keep = np.ones(g.nb, bool)
+        keep[k] = False
+        A = sp.coo_matrix((np.ones(keep.sum()), (g.br_from[keep], g.br_to[keep])), shape=(g.n, g.n))
+        if cg.connected_components(A, directed=False)[0] == 1:
+            out.append(int(k))
+            if len(out) == count:
+                return out
+    return out
+
+
+@dataclasses.dataclass
+class Workload:
+    """A request stream on a resident model, with what is needed to replay or
+    check any recorded request: the telemetry and breaker status it saw."""
+    requests: list
+    replay: dict  # update rid -> (z after the request, {branch: status} at that request)
+    breakers: dict  # every branch operated, with its last status
+
+
+def make_workload(g, rng: np.random.Generator, rate: float, duration: float, frac: float = 0.10,
+                  toggle_every: int = 50, area_rate: float = 0.0, area_pool=(), record_every: int = 500,
+                  replays: int = 4, record_areas: int = 3) -> Workload:
+    """Poisson arrivals of incremental updates (each re-measures a random `frac`
+    of z: the base value h(x_true) + noise plus fresh N(0, sigma^2)), a breaker
+    operation after every `toggle_every` updates (alternately opening and
+    closing a non-bridge branch; the injection readings at its ends follow the
+    new topology at the true state and travel with the operation), and full
+    solves of new areas drawn from `area_pool` at `area_rate` per second.
+    Every `record_every`-th update (offset by half) keeps its result, and the
+    first `replays` of them keep their telemetry and breaker status."""
+    from . import gridgen
+
+    t_upd = poisson_arrivals(rate, duration, rng)
+    t_area = poisson_arrivals(area_rate, duration, rng) if area_rate > 0 and len(area_pool) else np.zeros(0)
+    brks = non_bridge_branches(g, max(1, len(t_upd) // 100 + 1), rng) if toggle_every else []
+    k = max(1, int(round(frac * g.m)))
+    z = g.z.copy()  # the telemetry as the resident model sees it after each request
+    zb = g.z.copy()  # the telemetry base for the current breaker status: h(x_true) + the original noise
+    noise = g.z - g.z_true
+    inj = np.isin(g.m_type, [gridgen.P_INJ, gridgen.Q_INJ, gridgen.P_PSEUDO, gridgen.Q_PSEUDO])
+    status_arr = g.status.copy()
+    status: dict = {}
+    reqs, replay = [], {}
+    open_brk = None
+    for q, t in enumerate(t_upd):
+        idx = np.sort(rng.choice(g.m, k, replace=False)).astype(np.int32)
+        val = zb[idx] + rng.normal(0.0, 1.0, k) * g.sigma[idx]
+        z[idx] = val
+        rec = q % record_every == record_every // 2
+        reqs.append(Request(UPDATE, float(t), (idx, val), rid=q, record=rec))
+        if rec and len(replay) < replays:
+            replay[q] = (z.copy(), dict(status))
+        if toggle_every and (q + 1) % toggle_every == 0 and brks:
+            if open_brk is None:
+                open_brk = brks[(q // 100) % len(brks)]
+                op = (open_brk, 0)
+            else:
+                op = (open_brk, 1)
+                open_brk = None
+            status[op[0]] = op[1]
+            status_arr[op[0]] = op[1]
+            gt = dataclasses.replace(g, status=status_arr)
+            ends = (g.br_from[op[0]], g.br_to[op[0]])
+            rows = np.nonzero(inj & np.isin(g.m_loc, ends))[0].astype(np.int32)
+            zb[rows] = gridgen.measurement_function(gt, g.V_true, g.th_true, m_type=g.m_type[rows],
+                                                    m_loc=g.m_loc[rows], m_end=g.m_end[rows]) + noise[rows]
+            z[rows] = zb[rows]
+            reqs.append(Request(TOGGLE, float(t) + 1e-6, op + (rows, zb[rows].copy()), rid=100_000 + q))
+    for j, t in enumerate(t_area):
+        reqs.append(Request(AREA, float(t), area_pool[j % len(area_pool)], rid=200_000 + j, record=j < record_areas))
+    return Workload(reqs, replay, status)
+
+
 def poisson_arrivals(rate: float, duration: float, rng: np.random.Generator) -> np.ndarray:
     t, out = 0.0, []
     while True:
