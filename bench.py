@@ -1,21 +1,38 @@
 #!/usr/bin/env python3
-"""Incremental-update latency and throughput of the 200k-bus WLS state estimator.
+"""WLS state estimation on one MI355X: the 200k-bus incremental-update metric,
+with the Huber, batched-area and serving workloads beside it.
 
-Workload: the 200,000-bus / ~270,000-branch / 2,000,000-measurement seeded
-geometric grid (1 % gross errors), resident on one GPU.  One step is one
-incremental update as served in real time: 10 % of the measurement values are
-replaced (a telemetry batch already staged in HBM is scattered into z) and a
-REALTIME Gauss-Newton solve runs to ||dx||_inf < 1e-6 reusing the cached gain
-factor.  The mixed form (FP32 Jacobian / gain / factor, FP64 residual, RHS and
-state) and the plain FP64 form are timed on the same sequence of updates.
+Headline (`value`): the 200,000-bus / ~270,000-branch / 2,000,000-measurement
+seeded geometric grid (1 % gross errors), resident on the GPU.  One step is
+one incremental update: 10 % of the measurement values are replaced (a
+telemetry batch already staged in HBM is scattered into z on the device) and a
+REALTIME Gauss-Newton solve runs to ||dx||_inf < 1e-6 on the cached gain
+factor.  Two precision forms run the same sequence of updates: plain FP64, and
+the mixed form (FP64 h(x), residual, Jacobian, right-hand side, state and
+factorization; the cached factor's solve data rounded to FP32 once per
+factorization, so the iterations on it solve in FP32).  `value` is the mixed
+form's rate when it is the faster (config.form names the form), with p50/p99
+step latency; both forms are reported in full, with their errors against the
+FP64 CPU oracle, a roofline entry for the dominant kernel (HIP-event timed) and
+the oracle's CPU baseline.  Each form also reports `host_path`: the same
+updates pushed from host memory (sle_ext_UpdateTelemetryIndexed through pinned
+staging) with the estimate read back to the host inside the timed step.
 
-Prints ONE JSON line.  `value` is the solves/s of the faster of the two forms
-(named in config.form; on this workload the plain FP64 form, because the FP32
-factor of the 200k-bus gain matrix needs conjugate-gradient refinement to
-contract), with p50/p99 step latency of that form; both forms are reported in
-full beside it, with their errors against the FP64 CPU oracle, a roofline entry
-for the dominant kernel (HIP-event timed) and the oracle's CPU baseline.
+Beside the headline, unless --no-extras:
+  huber200k   flat-start Huber solve (gamma 1.5 sigma) of the same grid, both
+              forms: time, iterations, supernode-size histogram, error vs the
+              oracle (warm-started at the GPU estimate and iterated to 1e-9:
+              the Huber objective is convex, with one minimiser);
+  areas256    256 independent areas (500-20,000 buses) batched in one model,
+              each re-solved from its previous solution + N(0, 0.01), against
+              one engine per area;
+  serve200k   the serving loop on the resident 200k model, both forms: Poisson
+              updates at 200/s, a breaker toggle per 50 updates, new 5k-50k-bus
+              areas at 1/s within a 64 GiB budget; p50/p99/max latency from
+              arrival to the estimate on the host, and the error of a sampled
+              update against the oracle.
 
+--workload areas / serve run one of those alone with its own options.
 --copy-calibration runs a plain device copy of known size first; under
 `rocprofv3 --pmc FETCH_SIZE` / `WRITE_SIZE` it calibrates the counters that
 give the HBM traffic per launch recorded in profiles/hbm_traffic_200k.md.
@@ -38,6 +55,7 @@ HBM_PEAK_GBS = 8000.0
 TOL = 1e-6
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_200k.md")
 AREA_MAX_ITER = 25  # a batch iterates until its slowest area converges
+HUBER_GAMMA = 1.5
 AREA_MODE = 1  # PRECISION: a perturbed start refactors (the cached factor of the old point may not contract)
 COPY_BYTES = 512 << 20
 
@@ -199,7 +217,17 @@ def configure(e, fp64, pcg_steps, options):
             e.set_option(k, v)
 
 
-def run_form(g, fp64, reqs, batches, warmup, steps, profile_steps, pcg_steps=0, options=None, autotune=False):
+def supernode_histogram(e):
+    """Supernode sizes (columns per supernode) of the engine's symbolic factor."""
+    sizes = np.diff(e.ordering()["snode_ptr"])
+    edges = [1, 2, 3, 5, 9, 17, 33, 65]
+    top = max(int(sizes.max()) + 1, edges[-1] + 1)
+    counts = np.histogram(sizes, bins=edges + [top])[0]
+    return {f"{lo}-{hi - 1}": int(c) for lo, hi, c in zip(edges, edges[1:] + [top], counts) if c}
+
+
+def run_form(g, fp64, reqs, batches, warmup, steps, profile_steps, pcg_steps=0, options=None, autotune=False,
+             huber=False):
     from state_estimation_with_cuda_amd import sle
 
     name = "fp64" if fp64 else "mixed"
@@ -222,8 +250,8 @@ def run_form(g, fp64, reqs, batches, warmup, steps, profile_steps, pcg_steps=0, 
         t0 = time.perf_counter()
         tune = e.autotune(3)
         log(f"{name}: autotune {time.perf_counter() - t0:.1f}s: {tune}")
-    for k, (idx, val) in enumerate(batches):
-        e.stage_telemetry(k, idx, val)
+    for k in range(warmup + steps):
+        e.stage_telemetry(k, *batches[k])
     lat = []
     iters = []
     flags = set()
@@ -241,14 +269,24 @@ def run_form(g, fp64, reqs, batches, warmup, steps, profile_steps, pcg_steps=0, 
         r = step(k)
         lat.append((time.perf_counter() - ts) * 1e3)
         iters.append(r["iterations"])
-        objective = r["objective"]
         flags.add(r["flags"])
         ok += r["status"] == 0
     total = time.perf_counter() - t_start
-    V, th = e.state()
+    # the same kind of update from host memory, the estimate read back inside the step
+    hlat, h_ok = [], 0
+    t_start = time.perf_counter()
+    for k in range(warmup + steps, warmup + 2 * steps):
+        ts = time.perf_counter()
+        e.set_telemetry_indexed(*batches[k])
+        r = e.solve(mode=0, max_iterations=10, tolerance=TOL)
+        V, th = e.state()
+        hlat.append((time.perf_counter() - ts) * 1e3)
+        h_ok += r["status"] == 0
+    h_total = time.perf_counter() - t_start
+    objective = r["objective"]
     e_plan = e.describe()
     plan_sizes = e.solve_plan_sizes()
-    last_z = request_z(g, reqs[warmup + steps - 1])
+    last_z = request_z(g, reqs[warmup + 2 * steps - 1])
     # profiled pass (separate, so the events do not perturb the timed loop)
     kt = None
     if profile_steps:
@@ -258,8 +296,20 @@ def run_form(g, fp64, reqs, batches, warmup, steps, profile_steps, pcg_steps=0, 
             step(k)
         kt = e.kernel_times()
         e.set_config(enable_profiling=0)
+    hub = None
+    if huber:
+        e.set_telemetry_f64(g.z)
+        t0 = time.perf_counter()
+        rh = e.solve(mode=1, max_iterations=150, tolerance=TOL, robust=True, gamma=HUBER_GAMMA, flat_start=True)
+        wall = (time.perf_counter() - t0) * 1e3
+        hV, hth = e.state()
+        hub = dict(status=rh["status"], iterations=rh["iterations"], refactors=rh["refactors"],
+                   gpu_ms=round(rh["gpu_ms"], 2), wall_ms=round(wall, 2), objective=rh["objective"],
+                   supernodes=supernode_histogram(e), V=hV, th=hth)
+        log(f"{name}: Huber flat start {wall:.1f} ms, iters={rh['iterations']} status={rh['status']}")
     e.close()
     lat = np.array(lat)
+    hlat = np.array(hlat)
     return dict(
         name=name,
         solves_per_s=steps / total,
@@ -267,6 +317,9 @@ def run_form(g, fp64, reqs, batches, warmup, steps, profile_steps, pcg_steps=0, 
         p50_ms=float(np.percentile(lat, 50)),
         p99_ms=float(np.percentile(lat, 99)),
         max_ms=float(lat.max()),
+        host_path=dict(solves_per_s=round(steps / h_total, 2), p50_ms=round(float(np.percentile(hlat, 50)), 4),
+                       p99_ms=round(float(np.percentile(hlat, 99)), 4), max_ms=round(float(hlat.max()), 4),
+                       converged_steps=h_ok),
         mean_iterations=float(np.mean(iters)),
         converged=ok,
         initial_solve_ms=init_ms,
@@ -282,10 +335,11 @@ def run_form(g, fp64, reqs, batches, warmup, steps, profile_steps, pcg_steps=0, 
         flags=sorted(flags),
         plan=e_plan,
         autotune=tune,
+        huber=hub,
     )
 
 
-def run_areas(args):
+def areas_result(args, form, steps, warmup, per_instance_steps):
     """Batched uneven areas: `--areas` independent area estimators (bus counts
     log-uniform in [500, 20000], redundancy ~3) solved as one model -- every
     kernel launch covers all areas -- against one engine per area solved one
@@ -298,7 +352,7 @@ def run_areas(args):
     areas = [gridgen.workload_area(int(n), seed=1000 + k) for k, n in enumerate(sizes)]
     g = gridgen.concat(areas, name=f"areas{args.areas}")
     log(f"areas: {len(areas)} areas, n={g.n} nb={g.nb} m={g.m}, generated in {time.perf_counter() - t0:.1f}s")
-    fp64 = args.form == "fp64"
+    fp64 = form == "fp64"
     offs = np.cumsum([0] + [a.n for a in areas])
 
     def engine(grid):
@@ -316,12 +370,12 @@ def run_areas(args):
     V0, th0 = eb.state()
     log(f"batched: setup + first solve {time.perf_counter() - t0:.1f}s, status {r['status']} iters {r['iterations']}")
     tune = eb.autotune(3) if args.autotune else ""
-    starts = [gridgen.hot_start(g, V0, th0, 0.01, seed=50 + k) for k in range(args.warmup + args.steps)]
-    for k in range(args.warmup):
+    starts = [gridgen.hot_start(g, V0, th0, 0.01, seed=50 + k) for k in range(warmup + steps)]
+    for k in range(warmup):
         eb.set_state(*starts[k])
         eb.solve(mode=AREA_MODE, max_iterations=AREA_MAX_ITER, tolerance=TOL)
     tb, iters, ok = 0.0, [], 0
-    for k in range(args.warmup, args.warmup + args.steps):
+    for k in range(warmup, warmup + steps):
         eb.set_state(*starts[k])  # host upload of the perturbed start, outside the timed solve
         ts = time.perf_counter()
         r = eb.solve(mode=AREA_MODE, max_iterations=AREA_MAX_ITER, tolerance=TOL)
@@ -331,7 +385,7 @@ def run_areas(args):
     Vb, thb = eb.state()
     plan = eb.describe()
     eb.close()
-    batched_rate = len(areas) * args.steps / tb
+    batched_rate = len(areas) * steps / tb
     # per-instance: one engine per area, solved one after the other
     t0 = time.perf_counter()
     engines = [engine(a) for a in areas]
@@ -340,8 +394,8 @@ def run_areas(args):
     log(f"per-area engines: setup + first solves {time.perf_counter() - t0:.1f}s")
     tp, p_iters, p_ok = 0.0, [], 0
     worst = {"V": 0.0, "theta": 0.0}
-    ksteps = min(args.steps, 3)
-    for k in range(args.warmup + args.steps - ksteps, args.warmup + args.steps):  # ends on the batch's last start
+    ksteps = max(1, min(steps, per_instance_steps))
+    for k in range(warmup + steps - ksteps, warmup + steps):  # ends on the batch's last start
         V, th = starts[k]
         for a, e, b0 in zip(areas, engines, offs):
             e.set_state(V[b0:b0 + a.n], th[b0:b0 + a.n])
@@ -363,9 +417,9 @@ def run_areas(args):
         "value": round(batched_rate, 1),
         "unit": "area-solves/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(tb / args.steps * 1e3, 3),
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": round(tb / steps * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -373,7 +427,7 @@ def run_areas(args):
         "data": "synthetic",
         "config": {"workload": f"{len(areas)} independent areas, n log-uniform [500, 20000] (sum {g.n}), m {g.m}; "
                                "hot start = previous solution + N(0, 0.01); PRECISION to 1e-6",
-                   "form": args.form},
+                   "form": form},
         "per_instance_solves_per_s": round(per_rate, 1),
         "per_instance_mean_iterations": float(np.mean(p_iters)),
         "per_instance_converged": f"{p_ok}/{len(p_iters)}",
@@ -384,21 +438,25 @@ def run_areas(args):
         "autotune": tune,
         "batched_vs_alone_state_diff": worst,
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
-def run_serve(args):
+def run_areas(args):
+    print(json.dumps(areas_result(args, args.form, args.steps, args.warmup, 3)), flush=True)
+
+
+def serve_result(args, form, duration, g=None, oracle_cache=None):
     """Serving loop on the resident 200k-bus model: Poisson arrivals of
     incremental updates (10 % of z re-measured), a breaker operation after every
     50th update (alternately opening and closing a non-bridge branch), and full
     solves of new 5k-50k-bus areas admitted by an HBM budget; latency is
-    arrival to result."""
+    arrival to the estimate on the host."""
     from state_estimation_with_cuda_amd import gridgen, serve, sle
 
     sys.setswitchinterval(1e-4)  # the request threads hand the interpreter over quickly
     rng = np.random.default_rng(4)
-    g = gridgen.workload_200k()
-    fp64 = args.form == "fp64"
+    g = g if g is not None else gridgen.workload_200k()
+    fp64 = form == "fp64"
 
     def engine(grid):
         e = sle.Engine(double=True)
@@ -419,7 +477,7 @@ def run_serve(args):
     cal.close()
     log(f"serve: {len(pool)} area models, {bytes_per_bus:.0f} B/bus estimate")
     # request stream
-    wl = serve.make_workload(g, rng, args.rate, args.duration, toggle_every=args.toggle_every,
+    wl = serve.make_workload(g, rng, args.rate, duration, toggle_every=args.toggle_every,
                              area_rate=args.area_rate, area_pool=pool, record_every=args.record_every,
                              replays=args.replays)
     reqs, replay, status = wl.requests, wl.replay, wl.breakers
@@ -475,15 +533,24 @@ def run_serve(args):
         q0 = min(replay)
         r0 = [r for r in done if r.kind == serve.UPDATE and r.rid == q0][0]
         zr, st = replay[q0]
-        g2 = gridgen.Grid(**{**g.__dict__})
-        g2.z = zr
-        g2.status = g.status.copy()
-        for kb, sv in st.items():
-            g2.status[kb] = sv
-        o = Oracle(g2)
-        o.solve(mode=1, max_iter=30, tol=1e-10, flat_start=True)
-        oV, oth = o.state()
-        err = {"V": float(np.abs(r0.result[0] - oV).max()), "theta": float(np.abs(r0.result[1] - oth).max())}
+        key = (q0, tuple(sorted(st.items())), float(zr.sum()))
+        if oracle_cache is None or key not in oracle_cache:
+            g2 = gridgen.Grid(**{**g.__dict__})
+            g2.z = zr
+            g2.status = g.status.copy()
+            for kb, sv in st.items():
+                g2.status[kb] = sv
+            o = Oracle(g2)
+            o.solve(mode=1, max_iter=30, tol=1e-10, flat_start=True)
+            ref = o.state()
+            if oracle_cache is not None:
+                oracle_cache[key] = ref
+        else:
+            ref = oracle_cache[key]
+        oV, oth = ref
+        if r0.result is not None:
+            err = {"request": q0, "breakers": {str(k): v for k, v in st.items()},
+                   "V": float(np.abs(r0.result[0] - oV).max()), "theta": float(np.abs(r0.result[1] - oth).max())}
     main_e.close()
     u = summ["update"]
     out = {
@@ -499,10 +566,10 @@ def run_serve(args):
         "vs_baseline": None,
         "dtype": "f64" if fp64 else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"Poisson {args.rate}/s updates (10% of z) for {args.duration}s, breaker toggle per "
+        "config": {"workload": f"Poisson {args.rate}/s updates (10% of z) for {duration}s, breaker toggle per "
                                f"{args.toggle_every} updates, new 5k-50k-bus areas at {args.area_rate}/s within "
                                f"{args.hbm_budget_gib} GiB",
-                   "form": args.form, "toggle_mode": "REALTIME" if args.toggle_mode == 0 else "PRECISION"},
+                   "form": form, "toggle_mode": "REALTIME" if args.toggle_mode == 0 else "PRECISION"},
         "latency": summ,
         "solves_per_s": round(rate, 1),
         "hbm_peak_gib": round(srv.peak_used / (1 << 30), 2),
@@ -510,7 +577,11 @@ def run_serve(args):
         "isolation_max_diff": iso,
         "error_vs_oracle": err,
     }
-    print(json.dumps(out), flush=True)
+    return out
+
+
+def run_serve(args):
+    print(json.dumps(serve_result(args, args.form, args.duration)), flush=True)
 
 
 def main():
@@ -545,6 +616,12 @@ def main():
     ap.add_argument("--form", choices=["fp64", "mixed"], default="fp64", help="areas workload: precision form")
     ap.add_argument("--autotune", action="store_true", help="autotune the solve and factorization plans first")
     ap.add_argument("--set", action="append", metavar="KEY=VALUE", help="engine option (sle_ext_SetOption), repeatable")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="headline only: no Huber, batched-area or serving legs in the default line")
+    ap.add_argument("--extra-serve-duration", type=float, default=20.0,
+                    help="default line: seconds of arrivals of each form's serving leg")
+    ap.add_argument("--extra-serve-warmup", type=float, default=2.0,
+                    help="default line: seconds of each serving leg excluded from its statistics")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs (--gpus > 1) are not supported", file=sys.stderr)
@@ -566,13 +643,15 @@ def main():
     else:
         g = gridgen.geometric(args.buses, seed=2, flow_meas="all", dup_meas=int(1.6 * args.buses), gross_frac=0.01)
     log(f"grid: n={g.n} nb={g.nb} m={g.m} generated in {time.perf_counter() - t0:.1f}s")
-    reqs, batches = make_updates(g, args.warmup + max(args.steps, args.profile_steps), 0.10, seed=4)
+    # staged-HBM steps, then as many host-path steps on the following requests
+    reqs, batches = make_updates(g, args.warmup + max(2 * args.steps, args.profile_steps), 0.10, seed=4)
 
     opts = engine_options(args)
+    extras = not args.no_extras
     mixed = None if args.skip_mixed else run_form(g, False, reqs, batches, args.warmup, args.steps,
-                                                  args.profile_steps, args.pcg_steps, opts, args.autotune)
+                                                  args.profile_steps, args.pcg_steps, opts, args.autotune, extras)
     fp64 = None if args.skip_fp64 else run_form(g, True, reqs, batches, args.warmup, args.steps, args.profile_steps,
-                                                0, opts, args.autotune)
+                                                0, opts, args.autotune, extras)
     forms = [f for f in (mixed, fp64) if f]
     best = max(forms, key=lambda f: f["solves_per_s"])
     sz = best["sizes"]
@@ -594,6 +673,7 @@ def main():
             "initial_solve_ms": round(f["initial_solve_ms"], 2),
             "initial_iterations": f["initial_iterations"],
             "solve_flags": f["flags"],
+            "host_path": f["host_path"],
             "plan": f["plan"],
             "plan_sizes": f["plan_sizes"],
             "autotune": f["autotune"],
@@ -619,7 +699,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "incremental WLS update of a 200k-bus grid: 10% of z replaced (staged in HBM), REALTIME "
-                        "Gauss-Newton to ||dx||inf<1e-6 with the cached gain factor",
+                        "Gauss-Newton to ||dx||inf<1e-6 with the cached gain factor (host_path: telemetry from "
+                        "host memory and the estimate read back, inside the step)",
             "form": best["name"],
             "buses": g.n,
             "branches": g.nb,
@@ -642,18 +723,27 @@ def main():
     out["roofline"] = out[best["name"]]["roofline"]
 
     if not args.no_cpu_baseline:
-        from oracle.oracle import Oracle, num_threads
+        from oracle.oracle import Oracle, cpu_model, num_threads, set_num_threads
 
         t0 = time.perf_counter()
         o = Oracle(g)
         o.solve(mode=1, max_iter=30, tol=TOL, flat_start=True)
         log(f"oracle: setup + initial solve {time.perf_counter() - t0:.1f}s")
+        cores = num_threads()
         times = []
         for k in range(args.cpu_sample):
             o.set_z(request_z(g, reqs[args.warmup + args.steps - args.cpu_sample + k]))
             r = o.solve(mode=0, max_iter=10, tol=TOL)
             times.append(r["seconds"])
         cpu_rate = len(times) / sum(times)
+        # the same updates on one core (a smaller sample)
+        set_num_threads(1)
+        times1 = []
+        for k in range(min(3, args.cpu_sample)):
+            o.set_z(request_z(g, reqs[args.warmup + args.steps - args.cpu_sample + k]))
+            r = o.solve(mode=0, max_iter=10, tol=TOL)
+            times1.append(r["seconds"])
+        set_num_threads(cores)
         # reference optimum for the last request, then the GPU errors against it
         o.set_z(best["last_z"])
         o.solve(mode=1, max_iter=30, tol=1e-10)
@@ -668,11 +758,55 @@ def main():
         out["cpu_baseline"] = {
             "value": round(cpu_rate, 4),
             "unit": "solves/s",
-            "cores": num_threads(),
+            "cores": cores,
             "kind": "port",
             "sample": f"{len(times)} incremental updates (10% of z replaced, REALTIME WLS with cached factor, "
                       f"tol {TOL}) of the same 200k-bus model on the FP64 oracle: {sum(times):.1f}s",
+            "cpu_model": cpu_model(),
+            "one_core_solves_per_s": round(len(times1) / sum(times1), 4),
+            "one_core_sample": f"{len(times1)} of those updates on 1 thread: {sum(times1):.1f}s",
         }
+        # Huber: the oracle iterated from each form's estimate to 1e-9 (one minimiser)
+        for f in forms:
+            h = f["huber"]
+            if not h:
+                continue
+            o.set_z(g.z)
+            o.set_state(h["V"], h["th"])
+            t0 = time.perf_counter()
+            ro = o.solve(mode=1, max_iter=40, tol=1e-9, robust=2, gamma=HUBER_GAMMA)
+            hV, hth = o.state()
+            h["error_vs_oracle"] = {"V": float(np.abs(h["V"] - hV).max()), "theta": float(np.abs(h["th"] - hth).max()),
+                                    "J_rel": abs(h["objective"] - ro["objective"]) / abs(ro["objective"]),
+                                    "oracle_iterations": ro["iterations"], "oracle_status": ro["status"]}
+            log(f"oracle: Huber check of {f['name']} in {time.perf_counter() - t0:.1f}s, {ro['iterations']} iterations")
+        del o
+    for f in forms:
+        h = f["huber"]
+        if h:
+            out[f["name"]]["huber200k"] = {k: v for k, v in h.items() if k not in ("V", "th")}
+    if extras:
+        # the serving loop on the resident 200k model, both forms (the same request stream)
+        cache = {}
+        sargs = argparse.Namespace(**{**vars(args), "serve_warmup": args.extra_serve_warmup, "replays": 1})
+        serving = {}
+        for f in forms:
+            t0 = time.perf_counter()
+            res = serve_result(sargs, f["name"], args.extra_serve_duration, g=g, oracle_cache=cache)
+            log(f"serve {f['name']}: {time.perf_counter() - t0:.1f}s")
+            serving[f["name"]] = {"update": res["latency"]["update"], "toggle": res["latency"]["toggle"],
+                                  "area": res["latency"]["area"], "solves_per_s": res["solves_per_s"],
+                                  "hbm_peak_gib": res["hbm_peak_gib"], "isolation_max_diff": res["isolation_max_diff"],
+                                  "error_vs_oracle": res["error_vs_oracle"]}
+        out["serve200k"] = {"workload": res["config"]["workload"], "latency": "arrival to the estimate on the host",
+                            **serving}
+        # batched uneven areas (one model) against one engine per area
+        t0 = time.perf_counter()
+        ares = areas_result(args, "fp64", 5, 2, 1)
+        log(f"areas: {time.perf_counter() - t0:.1f}s")
+        out["areas256"] = {k: ares[k] for k in ("value", "unit", "ms_per_step", "config", "per_instance_solves_per_s",
+                                                "batched_vs_per_instance", "mean_iterations", "converged_steps",
+                                                "batched_vs_alone_state_diff")}
     print(json.dumps(out), flush=True)
 
 

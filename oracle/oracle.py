@@ -99,6 +99,7 @@ def lib():
         L.or_solve.argtypes = [C.c_void_p, C.POINTER(SolveParams), C.POINTER(SolveResult)]
         L.or_get_residuals.argtypes = [C.c_void_p, f64p]
         L.or_num_threads.restype = C.c_int32
+        L.or_set_num_threads.argtypes = [C.c_int32]
         _lib = L
     return _lib
 
@@ -251,3 +252,18 @@ class Oracle:
 
 def num_threads() -> int:
     return int(lib().or_num_threads())
+
+
+def set_num_threads(n: int) -> None:
+    lib().or_set_num_threads(int(n))
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"

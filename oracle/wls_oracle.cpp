@@ -963,4 +963,14 @@ int32_t or_num_threads(void) {
 #endif
 }
 
+// Threads of the following oracle calls (the CPU baseline is timed on all
+// host cores and on one).
+void or_set_num_threads(int32_t n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
 }  // extern "C"

@@ -61,8 +61,9 @@ def errors(V, th, J, ref):
 def supernode_histogram(e):
     sizes = np.diff(e.ordering()["snode_ptr"])
     edges = [1, 2, 3, 5, 9, 17, 33, 65]
-    h = np.histogram(sizes, bins=edges + [max(int(sizes.max()) + 1, 66)])[0]
-    return {f"{lo}-{hi - 1}": int(c) for lo, hi, c in zip(edges, edges[1:] + [int(sizes.max()) + 1], h)}
+    top = max(int(sizes.max()) + 1, edges[-1] + 1)
+    h = np.histogram(sizes, bins=edges + [top])[0]
+    return {f"{lo}-{hi - 1}": int(c) for lo, hi, c in zip(edges, edges[1:] + [top], h) if c}
 
 
 @pytest.mark.parametrize("fp64", [True, False])
