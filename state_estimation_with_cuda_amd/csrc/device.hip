@@ -2241,6 +2241,18 @@ __global__ __launch_bounds__(kBlock) void k_flat(int n, const int32_t* __restric
     th[b] = 0.0;
 }
 
+// The estimate in insertion (natural) bus order, written straight into pinned
+// host memory through its device mapping: no copy engine on the read-back.
+__global__ __launch_bounds__(kBlock) void k_state_out(int n, const int32_t* __restrict__ lb,
+                                                      const double* __restrict__ V, const double* __restrict__ th,
+                                                      double* __restrict__ out) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int p = lb[i];
+    out[i] = V[p];
+    out[n + i] = th[p];
+}
+
 __global__ __launch_bounds__(kBlock) void k_scatter_z(int count, const int32_t* __restrict__ idx,
                                                       const double* __restrict__ val, double* __restrict__ z) {
     int q = blockIdx.x * kBlock + threadIdx.x;
@@ -3052,6 +3064,10 @@ struct Instance::Impl {
     std::vector<int32_t> qmap;    // natural H position -> device H position
     std::vector<int32_t> yrow_d;  // device Ybus row pointers
     DBuf<int32_t> ml_d, lm_d;
+    DBuf<int32_t> lb_d;             // device position of each bus (insertion order)
+    double* state_pin = nullptr;    // pinned [V; theta] in insertion order (2n), read back by get_state
+    double* state_pin_dev = nullptr;
+    size_t state_pin_bytes = 0;
     int32_t* pin_idx = nullptr;  // pinned staging of indexed telemetry
     double* pin_val = nullptr;
     size_t pin_cap = 0;
@@ -3356,7 +3372,8 @@ Instance::~Instance() {
     rel(d_->hs); rel(d_->hcolp); rel(d_->cg_b); rel(d_->cg_x); rel(d_->cg_r); rel(d_->cg_p); rel(d_->cg_q);
     rel(d_->cg_z); rel(d_->cg_t); rel(d_->cg_parts); rel(d_->cg_s); rel(d_->z); rel(d_->sigma); rel(d_->rs); rel(d_->hx); rel(d_->r); rel(d_->jpart); rel(d_->rpart); rel(d_->vsave); rel(d_->thsave);
     rel(d_->mpart); rel(d_->ipart); rel(d_->tptr); rel(d_->tpos); rel(d_->trow);
-    rel(d_->tout); rel(d_->ml_d); rel(d_->lm_d); rel(d_->ztmp);
+    rel(d_->tout); rel(d_->ml_d); rel(d_->lm_d); rel(d_->lb_d); rel(d_->ztmp);
+    if (d_->state_pin) EnginePool::give_back_pinned(d_->state_pin, d_->state_pin_bytes);
     rel(d_->bv_ptr); rel(d_->bi_ptr); rel(d_->br_ptr); rel(d_->ytr); rel(d_->inc_ptr); rel(d_->inc); rel(d_->sbus);
     rel(d_->sbr);
     if (d_->pin_idx) (void)hipHostFree(d_->pin_idx);
@@ -3648,6 +3665,7 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
         D.hrow.upload(hr, bytes_, s);
         D.ml_d.upload(D.ml, bytes_, s);
         D.lm_d.upload(D.lm, bytes_, s);
+        D.lb_d.upload(D.lb, bytes_, s);
         HIPCHECK(hipStreamSynchronize(s));
         D.qmap.assign(S.hcol.size(), 0);
         for (int i = 0; i < m; ++i)
@@ -4019,8 +4037,23 @@ void fetch_unpermute(const DBuf<double>& src, const std::vector<int32_t>& pos_of
 void Instance::get_state(double* V, double* th) {
     HIPCHECK(hipSetDevice(device_));
     StreamScope scope_(d_->stream);
-    fetch_unpermute(d_->V, d_->lb, V, d_->stream);
-    fetch_unpermute(d_->th, d_->lb, th, d_->stream);
+    Impl& D = *d_;
+    if (!D.n) return;
+    const size_t need = 2 * (size_t)D.n * sizeof(double);
+    if (D.state_pin_bytes != need) {
+        if (D.state_pin) {
+            HIPCHECK(hipStreamSynchronize(D.stream));
+            EnginePool::give_back_pinned(D.state_pin, D.state_pin_bytes);
+        }
+        D.state_pin = static_cast<double*>(EnginePool::pinned(need));
+        HIPCHECK(hipHostGetDevicePointer((void**)&D.state_pin_dev, D.state_pin, 0));
+        D.state_pin_bytes = need;
+    }
+    k_state_out<<<nblk(D.n), kBlock, 0, D.stream>>>(D.n, D.lb_d.p, D.V.p, D.th.p, D.state_pin_dev);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(D.stream));
+    if (V) std::memcpy(V, D.state_pin, (size_t)D.n * sizeof(double));
+    if (th) std::memcpy(th, D.state_pin + D.n, (size_t)D.n * sizeof(double));
 }
 
 void Instance::flat_start() {

@@ -802,6 +802,17 @@ void Engine::ensure_state() {
     state_dirty_ = false;
 }
 
+void Engine::state_out(double* v, double* th, int count) {
+    const int n = std::min(count, n_bus());
+    if (state_dirty_ && inst_ && n == n_bus() && v && th) {
+        inst_->get_state(v, th);  // the host copy stays stale until a getter needs it
+        return;
+    }
+    ensure_state();
+    if (v) std::copy(V_.begin(), V_.begin() + n, v);
+    if (th) std::copy(th_.begin(), th_.begin() + n, th);
+}
+
 int Engine::sync_measurements() {
     if (!inst_ || model_dirty_) return ensure_ready();
     if (z_dirty_) {

@@ -173,6 +173,9 @@ class Engine {
         return th_;
     }
     void ensure_state();  // download the estimate after a solve (on first use)
+    // the estimate into caller arrays of count entries (fetched straight from
+    // the device after a solve when the caller takes all of it)
+    void state_out(double* v, double* th, int count);
     int set_state(const double* V, const double* th, int count);
     void ensure_results();
     std::vector<double> P_, Q_, Pf_, Qf_, Pt_, Qt_, h_, r_;

@@ -932,10 +932,10 @@ SLE_API SLE_StatusCode SLE_CALL sle_ext_SetState(SLE_Handle handle, const double
 
 SLE_API SLE_StatusCode SLE_CALL sle_ext_GetState(SLE_Handle handle, double* v, double* theta, int32_t count) {
     if (!handle || count <= 0) return fail(SLE_ERROR_INVALID_ARGUMENT, "bad argument");
-    int32_t n = std::min<int32_t>(count, E(handle).n_bus());
-    if (v) std::copy(E(handle).V().begin(), E(handle).V().begin() + n, v);
-    if (theta) std::copy(E(handle).th().begin(), E(handle).th().begin() + n, theta);
-    return SLE_OK;
+    return guarded([&]() -> SLE_StatusCode {
+        E(handle).state_out(v, theta, count);
+        return SLE_OK;
+    });
 }
 
 SLE_API SLE_StatusCode SLE_CALL sle_ext_GetResults(SLE_Handle handle, double* p_inj, double* q_inj, double* p_from,

@@ -86,6 +86,7 @@ class Server:
         self._t0 = 0.0
         self.completed: list = []
         self._breakers: dict = {}
+        self._state_buf = None  # the resident model's estimate, read back into the same arrays
         self._threads = [threading.Thread(target=self._main_loop, name="sle-main", daemon=True)]
         self._threads += [threading.Thread(target=self._area_loop, name=f"sle-area{k}", daemon=True)
                           for k in range(self.slots)]
@@ -154,9 +155,10 @@ class Server:
                 r = self.main.solve(mode=mode, max_iterations=self.max_iterations, tolerance=self.tol)
                 req.status, req.iterations = r["status"], r["iterations"]
                 # the request is done when its estimate is on the host
-                state = self.main.state()
                 if req.record:
-                    req.result = state
+                    req.result = self.main.state()
+                else:
+                    self._state_buf = self.main.state(out=self._state_buf)
             except Exception as exc:  # the request fails, the loop goes on
                 req.error = f"{type(exc).__name__}: {exc}"
             req.done = self.now()

@@ -529,9 +529,9 @@ class Engine:
         rc = self.L.sle_Solve(self.h, int(mode), C.byref(res))
         return rc, res
 
-    def state(self):
-        V = np.zeros(self.n)
-        th = np.zeros(self.n)
+    def state(self, out=None):
+        """(V, theta) in insertion order; out=(V, theta) fills caller arrays instead of new ones."""
+        V, th = out if out is not None else (np.empty(self.n), np.empty(self.n))
         self._check(self.L.sle_ext_GetState(self.h, V, th, self.n), "sle_ext_GetState")
         return V, th
 

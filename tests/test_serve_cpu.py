@@ -45,7 +45,7 @@ class FakeEngine:
         finally:
             self.lock.release()
 
-    def state(self):
+    def state(self, out=None):
         return np.zeros(2), np.zeros(2)
 
     def gpu_bytes(self):
@@ -159,7 +159,7 @@ def test_updates_finish_with_the_estimate_on_the_host():
     calls = []
 
     class Main(FakeEngine):
-        def state(self):
+        def state(self, out=None):
             calls.append("state")
             return np.zeros(2), np.zeros(2)
 
