@@ -275,6 +275,11 @@ def run_form(g, fp64, reqs, batches, warmup, steps, profile_steps, pcg_steps=0, 
     # the same kind of update from host memory, the estimate read back inside the step
     hlat, h_ok = [], 0
     V, th = np.empty(g.n), np.empty(g.n)  # the client's result arrays, reused across requests
+    # one untimed pass first (the pinned staging and read-back buffers are made on first use);
+    # it re-applies the last staged request, so z is unchanged
+    e.set_telemetry_indexed(*batches[warmup + steps - 1])
+    e.solve(mode=0, max_iterations=10, tolerance=TOL)
+    e.state(out=(V, th))
     t_start = time.perf_counter()
     for k in range(warmup + steps, warmup + 2 * steps):
         ts = time.perf_counter()

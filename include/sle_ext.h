@@ -106,6 +106,33 @@ SLE_API SLE_StatusCode SLE_CALL sle_ext_SetOption(SLE_Handle handle, const char*
  * "lds_flags" options. */
 SLE_API SLE_StatusCode SLE_CALL sle_ext_AutotuneSolve(SLE_Handle handle, int32_t repetitions);
 
+/* Largest-normalized-residual statistics at the current estimate (after a
+ * solve): rn[i] = |r_i| / sqrt(Omega_ii) and omega[i] = Omega_ii, where
+ * Omega = R - H G^-1 H^T is the covariance of the WLS residuals (R =
+ * diag(sigma^2), G = H^T R^-1 H at the estimate), insertion order, count
+ * entries at most; 0 for inactive measurements and for critical ones (Omega_ii
+ * ~ 0: their residual is structurally zero).  Omega_ii comes from the sparse
+ * inverse subset of G on the supernodal fronts, computed on the GPU from a fresh
+ * FP64 factorization at the estimate (which becomes the cached factor).  The
+ * reference has no counterpart: it tests |r|/sigma (sle_engine.cu:959-977,
+ * sle_api.cu:1295-1316).  Returns SLE_ERROR_SINGULAR_MATRIX for a singular gain.
+ * sle_IdentifyBadData uses rn (option "bad_data_lnr", default 1); with
+ * SLE_Config.auto_bad_data_rejection every converged solve repeats the
+ * largest-normalized-residual test, setting the measurement aside and
+ * re-estimating while rn exceeds bad_data_threshold (at most option
+ * "max_eliminations", default 10, per solve), and SLE_Result.largest_residual /
+ * largest_residual_idx / bad_data_count report the last test and the number set
+ * aside (without it -- unless option "lnr_in_result" is 1 -- largest_residual is
+ * max |r|/sigma and bad_data_count 0).  Each solve with auto rejection starts
+ * with every measurement active again. */
+SLE_API SLE_StatusCode SLE_CALL sle_ext_NormalizedResiduals(SLE_Handle handle, double* rn, double* omega,
+                                                           int32_t count);
+
+/* Set measurement `index` (insertion order) aside (active = 0) or back in use
+ * (active = 1): a value-only change (the structure is kept); the next solve
+ * refactors.  No counterpart in the reference. */
+SLE_API SLE_StatusCode SLE_CALL sle_ext_SetMeasurementActive(SLE_Handle handle, int32_t index, int32_t active);
+
 /* Solve with an explicit solver configuration; FP64 outputs. */
 SLE_API SLE_StatusCode SLE_CALL sle_ext_Solve(SLE_Handle handle, const SLE_SolverConfig* config,
                                               SLE_ExtResult* result);

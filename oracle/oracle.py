@@ -98,6 +98,8 @@ def lib():
         L.or_factor.argtypes = [C.c_void_p, f64p, i64p, i32p, f64p]
         L.or_solve.argtypes = [C.c_void_p, C.POINTER(SolveParams), C.POINTER(SolveResult)]
         L.or_get_residuals.argtypes = [C.c_void_p, f64p]
+        L.or_normalized_residuals.restype = C.c_int32
+        L.or_normalized_residuals.argtypes = [C.c_void_p, f64p, f64p]
         L.or_num_threads.restype = C.c_int32
         L.or_set_num_threads.argtypes = [C.c_int32]
         _lib = L
@@ -243,6 +245,15 @@ class Oracle:
             objective=r.objective,
             seconds=r.seconds,
         )
+
+    def normalized_residuals(self):
+        """(rn, omega) at the current state: rn = |r| / sqrt(Omega_ii), Omega = R - H G^-1 H^T."""
+        rn = np.zeros(self.m)
+        om = np.zeros(self.m)
+        rc = lib().or_normalized_residuals(self.h, rn, om)
+        if rc != 0:
+            raise np.linalg.LinAlgError("oracle normalized residuals: singular gain")
+        return rn, om
 
     def residuals(self):
         out = np.zeros(self.m)

@@ -29,6 +29,10 @@
 //   factor + solve     sparse_matrix.cu:793-895 (cusolverSp csrlsvchol is
 //                      replaced by a left-looking sparse Cholesky)
 //
+//   bad data (LNR)     sle_engine.cu:959-977 thresholds |r|/sigma; here the
+//                      normalized residual |r|/sqrt(Omega_ii) of Abur &
+//                      Exposito ch. 5, Omega from the sparse inverse subset
+//
 // Pins (tests/test_oracle_*.py): closed-form 2-bus cases, injection = sum of
 // branch flows, central finite differences of h(x) for H, dense H^T W H for G,
 // brute-force symbolic elimination for etree/column counts/supernodes, dense
@@ -630,6 +634,40 @@ void solve_factor(const Case& c, const std::vector<double>& b, std::vector<doubl
     for (int k = 0; k < ns; ++k) dx[c.perm[k]] = y[k];
 }
 
+// Sparse inverse subset Z = (P G P^T)^-1 on the structure of L (entries of Z
+// at every position of L's pattern), by the column recurrences of Takahashi,
+// Fagan and Chin (1973) in the form of Erisman and Tinney (1975), from the last
+// column to the first:
+//   Z_ij = -(1/L_jj) sum_{k in struct(j)} L_kj Z_ik   (i in struct(j))
+//   Z_jj = (1/L_jj) (1/L_jj - sum_{k in struct(j)} L_kj Z_kj)
+// struct(j) is a clique of the filled graph, so every Z_ik needed lies in the
+// pattern of an already finished column.
+double z_at(const Case& c, const std::vector<double>& zval, int i, int j) {
+    if (i < j) std::swap(i, j);
+    const int* b = c.lrow.data() + c.lptr[j];
+    const int* e = c.lrow.data() + c.lptr[j + 1];
+    const int* p = std::lower_bound(b, e, i);
+    if (p == e || *p != i) return std::nan("");  // outside the filled pattern: not in the subset
+    return zval[p - c.lrow.data()];
+}
+
+void selected_inverse(const Case& c, std::vector<double>& zval) {
+    zval.assign(c.lrow.size(), 0.0);
+    for (int j = c.ns - 1; j >= 0; --j) {
+        const int64_t d = c.lptr[j], e = c.lptr[j + 1];
+        const double ljj = c.lval[d];
+        for (int64_t q = d + 1; q < e; ++q) {
+            const int i = c.lrow[q];
+            double s = 0;
+            for (int64_t t = d + 1; t < e; ++t) s += c.lval[t] * z_at(c, zval, i, c.lrow[t]);
+            zval[q] = -s / ljj;
+        }
+        double s = 0;
+        for (int64_t t = d + 1; t < e; ++t) s += c.lval[t] * zval[t];
+        zval[d] = (1.0 / ljj - s) / ljj;
+    }
+}
+
 void prepare(Case& c) {
     compute_references(c);
     compute_admittances(c);
@@ -948,6 +986,48 @@ void or_solve(void* hh, const OracleSolveParams* prm, OracleSolveResult* out) {
     R.objective = J;
     R.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     *out = R;
+}
+
+// Largest-normalized-residual statistics at the current state (Abur &
+// Exposito 2004, ch. 5): with R = diag(sigma^2), G = H^T R^-1 H and the
+// residual covariance Omega = R - H G^-1 H^T, omega[i] = Omega_ii and
+// rn[i] = |r_i| / sqrt(Omega_ii).  Inactive rows and critical measurements
+// (Omega_ii <= 1e-10 sigma^2: the residual is structurally zero) get rn = 0.
+// The reference thresholds |r|/sigma instead (sle_engine.cu:959-977).
+// Returns 0, or 3 when the gain is singular.
+int32_t or_normalized_residuals(void* hh, double* rn, double* omega) {
+    Case& c = *(Case*)hh;
+    std::vector<double> hx(c.m), P(c.n), Q(c.n), Pf(c.nb), Qf(c.nb), Pt(c.nb), Qt(c.nb), w(c.m);
+    std::vector<double> hv(c.hcol.size()), g, zval;
+    eval_power(c, c.V.data(), c.th.data(), P.data(), Q.data(), Pf.data(), Qf.data(), Pt.data(), Qt.data());
+    eval_h(c, c.V.data(), c.th.data(), P.data(), Q.data(), Pf.data(), Qf.data(), Pt.data(), Qt.data(), hx.data());
+    eval_jacobian(c, c.V.data(), c.th.data(), P.data(), Q.data(), Pf.data(), Qf.data(), Pt.data(), Qt.data(),
+                  hv.data());
+    for (int i = 0; i < c.m; ++i) w[i] = c.active[i] ? 1.0 / (c.sigma[i] * c.sigma[i]) : 0.0;
+    gain(c, hv.data(), w.data(), &g);
+    if (!factor(c, g)) {
+        c.factor_valid = false;
+        return 3;
+    }
+    c.factor_valid = true;
+    selected_inverse(c, zval);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int i = 0; i < c.m; ++i) {
+        rn[i] = 0.0;
+        omega[i] = 0.0;
+        if (!c.active[i]) continue;
+        const double s2 = c.sigma[i] * c.sigma[i];
+        double lev = 0;  // h_i G^-1 h_i^T / sigma_i^2
+        for (int a = c.hrow[i]; a < c.hrow[i + 1]; ++a)
+            for (int b = c.hrow[i]; b < c.hrow[i + 1]; ++b)
+                lev += hv[a] * hv[b] * z_at(c, zval, c.iperm[c.hcol[a]], c.iperm[c.hcol[b]]);
+        lev /= s2;
+        const double om = s2 * (1.0 - lev);
+        omega[i] = om;
+        const double r = c.z[i] - hx[i];
+        rn[i] = om > 1e-10 * s2 ? std::fabs(r) / std::sqrt(om) : 0.0;
+    }
+    return 0;
 }
 
 void or_get_residuals(void* h, double* res) {

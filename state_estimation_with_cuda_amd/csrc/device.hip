@@ -3044,6 +3044,18 @@ struct Instance::Impl {
     std::vector<uint8_t> status_h;  // host copy of the breaker status (device branch order)
     std::vector<int32_t> br_rows_ptr, br_rows;  // device rows of the measurements on each branch
     std::vector<uint8_t> m_present_h;           // measurement in the structure (device row order)
+    std::vector<uint8_t> m_rejected_h;          // set aside as bad data (device row order)
+    std::vector<int32_t> row_branch_h;          // device branch of a flow row, -1 for a bus row (device row order)
+    // normalized residuals: per device H entry the local row of its column in
+    // the row's front, per device row that front; fronts by level (root levels
+    // first); the sparse inverse subset on the fronts; row leverages
+    struct LnrDev {
+        bool ok = false;
+        uint64_t plan_epoch = ~0ull;
+        DBuf<int32_t> hloc, hsn, parent, levels;
+        std::vector<int> lvl_off;
+        DBuf<double> Z, lev;
+    } lnr;
     bool topo_staged = false;
     DBuf<int32_t> mloc, hrow;
     DBuf<double> z, sigma, rs, hx, r;
@@ -3531,6 +3543,12 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
         for (int i = 0; i < m; ++i)
             if (!is_bus_measurement(h.mtype[i])) D.br_rows[f[h.mloc[i]]++] = D.lm[i];
         D.m_present_h.assign(m, 1);
+        D.m_rejected_h.assign(m, 0);
+        D.row_branch_h.assign(m, -1);
+        for (int i = 0; i < m; ++i) {
+            if (!h.rejected.empty()) D.m_rejected_h[D.lm[i]] = h.rejected[i];
+            if (!is_bus_measurement(h.mtype[i])) D.row_branch_h[D.lm[i]] = D.lk[h.mloc[i]];
+        }
         if (!h.m_present.empty())
             for (int i = 0; i < m; ++i) D.m_present_h[D.lm[i]] = h.m_present[i];
         D.topo_staged = false;
@@ -3940,9 +3958,21 @@ void Instance::set_branch_closed(int k, bool closed) {
     D.status_h[D.lk[k]] = closed ? 1 : 0;
     for (int q = D.br_rows_ptr[k]; q < D.br_rows_ptr[k + 1]; ++q) {
         const int p = D.br_rows[q];
-        const unsigned on = closed && D.m_present_h[p] ? 1u : 0u;
+        const unsigned on = closed && D.m_present_h[p] && !D.m_rejected_h[p] ? 1u : 0u;
         D.mcode_h[p] = (uint8_t)((D.mcode_h[p] & ~(1u << 5)) | (on << 5));
     }
+    D.topo_staged = true;
+}
+
+void Instance::set_measurement_rejected(int i, bool rejected) {
+    Impl& D = *d_;
+    if (i < 0 || i >= D.m) throw std::invalid_argument("measurement index out of range");
+    const int p = D.lm[i];
+    D.m_rejected_h[p] = rejected ? 1 : 0;
+    const int k = D.row_branch_h[p];
+    const bool closed = k < 0 || D.status_h[k];
+    const unsigned on = closed && D.m_present_h[p] && !rejected ? 1u : 0u;
+    D.mcode_h[p] = (uint8_t)((D.mcode_h[p] & ~(1u << 5)) | (on << 5));
     D.topo_staged = true;
 }
 
@@ -4913,6 +4943,98 @@ SolveOutcome Instance::solve(const SolveParams& p) {
 }
 
 // ---------------------------------------------------------------------------
+// Normalized residuals: sparse inverse subset of the gain on the fronts
+// ---------------------------------------------------------------------------
+// For a supernode with own columns J (k) and below-rows I (u = c - k), the
+// front's solve matrix M = [L_JJ^-1; -W] (W = L_IJ L_JJ^-1) and the inverse
+// Z = G^-1 restricted to the front's rows:
+//   Z_II  from the parent's front (I lies in the parent's rows: rmap)
+//   Z_IJ = Z_II (-W)
+//   Z_JJ = L_JJ^-T L_JJ^-1 + (-W)^T Z_IJ
+// (the block form of the Takahashi recurrences).  Fronts are processed from
+// the roots down, one workgroup per front; Z is stored like the fronts
+// (c x c, column-major, both triangles).
+struct SelInvArgs {
+    const int32_t* sn_k;
+    const int32_t* sn_c;
+    const int64_t* foff;
+    const int64_t* moff;
+    const int64_t* rmoff;
+    const int32_t* rmap;
+    const int32_t* parent;
+    const double* M;
+    double* Z;
+};
+
+__global__ __launch_bounds__(kBlock) void k_selinv(SelInvArgs A, const int32_t* __restrict__ list) {
+    const int s = list[blockIdx.x];
+    const int c = A.sn_c[s], k = A.sn_k[s], u = c - k;
+    double* Zs = A.Z + A.foff[s];
+    const double* Ms = A.M + A.moff[s];
+    const int tid = threadIdx.x;
+    if (u > 0) {
+        const int p = A.parent[s];
+        const int cp = A.sn_c[p];
+        const double* Zp = A.Z + A.foff[p];
+        const int32_t* rm = A.rmap + A.rmoff[s];
+        for (int e = tid; e < u * u; e += kBlock) {
+            const int a = e % u, b = e / u;
+            Zs[(int64_t)(k + b) * c + (k + a)] = Zp[(int64_t)rm[b] * cp + rm[a]];
+        }
+        __syncthreads();
+        for (int e = tid; e < u * k; e += kBlock) {
+            const int a = e % u, j = e / u;
+            double acc = 0.0;
+            for (int b = 0; b < u; ++b) acc += Zs[(int64_t)(k + b) * c + (k + a)] * Ms[(int64_t)j * c + (k + b)];
+            Zs[(int64_t)j * c + (k + a)] = acc;  // row k+a, column j
+        }
+        __syncthreads();
+        for (int e = tid; e < u * k; e += kBlock) {
+            const int a = e % u, j = e / u;
+            Zs[(int64_t)(k + a) * c + j] = Zs[(int64_t)j * c + (k + a)];
+        }
+    }
+    // Z_JJ reads Z_IJ (columns j < k, rows >= k) and writes rows and columns < k
+    for (int e = tid; e < k * k; e += kBlock) {
+        const int i = e % k, j = e / k;
+        if (i < j) continue;
+        double acc = 0.0;
+        for (int r = i; r < k; ++r) acc += Ms[(int64_t)i * c + r] * Ms[(int64_t)j * c + r];
+        for (int b = 0; b < u; ++b) acc += Ms[(int64_t)i * c + (k + b)] * Zs[(int64_t)j * c + (k + b)];
+        Zs[(int64_t)j * c + i] = acc;
+        Zs[(int64_t)i * c + j] = acc;
+    }
+}
+
+// Leverage of each measurement row: h_i Z h_i^T over the front holding the
+// row's columns (hs = sqrt(w) H with the base weights, so this is
+// h_i G^-1 h_i^T / sigma_i^2).  One thread per row.
+__global__ __launch_bounds__(kBlock) void k_leverage(int m, const int32_t* __restrict__ hrow,
+                                                     const double* __restrict__ hs, const int32_t* __restrict__ hloc,
+                                                     const int32_t* __restrict__ hsn, const int32_t* __restrict__ sn_c,
+                                                     const int64_t* __restrict__ foff, const double* __restrict__ Z,
+                                                     double* __restrict__ lev) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= m) return;
+    const int s = hsn[p];
+    if (s < 0) {
+        lev[p] = 0.0;
+        return;
+    }
+    const int c = sn_c[s];
+    const double* Zs = Z + foff[s];
+    const int q0 = hrow[p], q1 = hrow[p + 1];
+    double acc = 0.0;
+    for (int a = q0; a < q1; ++a) {
+        const int64_t col = (int64_t)hloc[a] * c;
+        double t = 0.0;
+        for (int b = q0; b < q1; ++b) t += hs[b] * Zs[col + hloc[b]];
+        acc += hs[a] * t;
+    }
+    lev[p] = acc;
+}
+
+// ---------------------------------------------------------------------------
 // Allocation-free solve
 // ---------------------------------------------------------------------------
 // Scratch of the allocation-free solve: the iteration status record and the
@@ -4972,6 +5094,120 @@ void init_device_scratch(int device) {
 }
 
 size_t device_scratch_bytes() { return kScratchBytes; }
+
+namespace {
+// Host plan of the normalized residuals for the current symbolic analysis.
+void build_lnr_plan(Instance::Impl& D, const Symbolic& S, Ledger& bytes, hipStream_t s) {
+    auto& L = D.lnr;
+    const int m = S.m;
+    // supernode of each permuted column
+    std::vector<int32_t> col_sn(S.ns);
+    for (int t = 0; t < S.nsn; ++t)
+        for (int j = S.sn_first[t]; j < S.sn_first[t] + S.sn_k[t]; ++j) col_sn[j] = t;
+    std::vector<int32_t> hloc(S.hcol.size()), hsn(std::max(1, m), -1);
+    for (int i = 0; i < m; ++i) {
+        const int q0 = S.hrow[i], q1 = S.hrow[i + 1];
+        if (q0 == q1) continue;
+        int j0 = S.ns;
+        for (int q = q0; q < q1; ++q) j0 = std::min(j0, S.iperm[S.hcol[q]]);
+        // the row's columns are a clique of the filled graph: all of them lie
+        // in the front of the supernode of the first eliminated one
+        const int t = col_sn[j0];
+        const int first = S.sn_first[t], k = S.sn_k[t], c = S.sn_c[t];
+        const int32_t* rows = S.sn_rows.data() + S.sn_rows_off[t];
+        for (int q = q0; q < q1; ++q) {
+            const int j = S.iperm[S.hcol[q]];
+            int loc;
+            if (j >= first && j < first + k) {
+                loc = j - first;
+            } else {
+                const int32_t* e = std::lower_bound(rows + k, rows + c, j);
+                if (e == rows + c || *e != j) throw std::logic_error("measurement row outside the front of its first column");
+                loc = (int)(e - rows);
+            }
+            hloc[D.qmap[q]] = loc;
+        }
+        hsn[D.lm[i]] = t;
+    }
+    // fronts by level, root levels first (a parent's level is above its children's)
+    std::vector<int32_t> lv;
+    L.lvl_off.assign(1, 0);
+    for (int l = S.n_levels - 1; l >= 0; --l) {
+        for (int q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) lv.push_back(S.level_sn[q]);
+        L.lvl_off.push_back((int)lv.size());
+    }
+    if (lv.empty()) lv.push_back(0);
+    std::vector<int32_t> par(S.sn_parent);
+    if (par.empty()) par.push_back(-1);
+    L.hloc.upload(hloc.empty() ? std::vector<int32_t>(1, 0) : hloc, bytes, s);
+    L.hsn.upload(hsn, bytes, s);
+    L.parent.upload(par, bytes, s);
+    L.levels.upload(lv, bytes, s);
+    if (L.Z.n != (size_t)std::max<int64_t>(1, S.ftotal)) L.Z.alloc(std::max<int64_t>(1, S.ftotal), bytes);
+    if (L.lev.n != (size_t)std::max(1, m)) L.lev.alloc(std::max(1, m), bytes);
+    HIPCHECK(hipStreamSynchronize(s));
+    L.plan_epoch = D.plan_epoch;
+    L.ok = true;
+}
+}  // namespace
+
+bool Instance::leverages(double* lev, double* r) {
+    HIPCHECK(hipSetDevice(device_));
+    StreamScope scope_(d_->stream);
+    Impl& D = *d_;
+    const Symbolic& S = *S_;
+    hipStream_t s = D.stream;
+    if (!D.lnr.ok || D.lnr.plan_epoch != D.plan_epoch) build_lnr_plan(D, S, bytes_, s);
+    if (!D.f64.ready) D.f64.alloc(S, bytes_, D.pi.sizes());
+    // the WLS gain at the current state (base weights), factored in FP64
+    launch_power(D);
+    launch_meas(D, 0, 1.5, 0, 1, true);
+    launch_spgemm<double>(D, S);
+    launch_factor<double>(D, 1e-13);
+    HIPCHECK(hipGetLastError());
+    read_status(D);
+    if (D.hst->piv) {
+        factor_valid_ = false;
+        return false;
+    }
+    // the factor is a valid cached factor at this state; the mixed form's FP32
+    // solve data follow it (an FP32-factorization mixed form refactors instead)
+    if (precision_ == 1 || fp64_factor_active_) {
+        factor_valid_ = true;
+    } else if (D.mixed64 && !D.mixed64_off) {
+        if (!D.f32.ready) D.f32.alloc(S, bytes_, D.pi.sizes());
+        round_solve_data(D, bytes_);
+        factor_valid_ = true;
+    } else {
+        factor_valid_ = false;
+    }
+    // M = [L_JJ^-1; -L_IJ L_JJ^-1] of every front (the top fronts' are the solve's own)
+    const int nsn = (int)S.nsn;
+    if (nsn)
+        k_front_m<double><<<(nsn + kBlock / kWave - 1) / (kBlock / kWave), kBlock, 0, s>>>(nsn, solve_args<double>(D),
+                                                                                          D.f64.M.p, nullptr);
+    SelInvArgs A{D.sn_k.p, D.sn_c.p, D.foff.p, D.moff.p, D.rmoff.p, D.rmap.p, D.lnr.parent.p, D.f64.M.p, D.lnr.Z.p};
+    for (size_t l = 0; l + 1 < D.lnr.lvl_off.size(); ++l) {
+        const int cnt = D.lnr.lvl_off[l + 1] - D.lnr.lvl_off[l];
+        if (cnt) k_selinv<<<cnt, kBlock, 0, s>>>(A, D.lnr.levels.p + D.lnr.lvl_off[l]);
+    }
+    if (D.m)
+        k_leverage<<<nblk(D.m), kBlock, 0, s>>>(D.m, D.hrow.p, D.hs.p, D.lnr.hloc.p, D.lnr.hsn.p, D.sn_c.p, D.foff.p,
+                                               D.lnr.Z.p, D.lnr.lev.p);
+    HIPCHECK(hipGetLastError());
+    std::vector<double> lv(D.m), rr(D.m);
+    if (D.m) {
+        HIPCHECK(hipMemcpyAsync(lv.data(), D.lnr.lev.p, D.m * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipMemcpyAsync(rr.data(), D.r.p, D.m * sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    HIPCHECK(hipStreamSynchronize(s));
+    for (int p = 0; p < D.m; ++p) {
+        if (lev) lev[D.ml[p]] = lv[p];
+        if (r) r[D.ml[p]] = rr[p];
+    }
+    D.results_valid = false;
+    return true;
+}
 
 SolveOutcome Instance::solve_no_alloc(const SolveParams& p) {
     if (p.mode != 0 || p.robust != 0 || p.pcg_steps > 0)

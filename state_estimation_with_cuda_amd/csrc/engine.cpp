@@ -197,6 +197,13 @@ int Engine::set_option(const char* key, double value) {
     } else if (k == "level_schedule") {
         opt_.level_schedule = value != 0;
         if (inst_) inst_->set_level_schedule(opt_.level_schedule);
+    } else if (k == "bad_data_lnr") {
+        opt_.bad_data_lnr = value != 0;
+    } else if (k == "lnr_in_result") {
+        opt_.lnr_in_result = value != 0;
+    } else if (k == "max_eliminations") {
+        if (value < 0 || value > 1e6) return ST_INVALID_ARGUMENT;
+        opt_.max_eliminations = (int)value;
     } else {
         return ST_INVALID_ARGUMENT;
     }
@@ -298,6 +305,7 @@ int Engine::add_measurement(const char* id, int type, const char* loc, int end, 
     m_pseudo_.push_back(pseudo);
     m_z_.push_back(0.0);
     m_removed_.push_back(0);
+    m_rejected_.push_back(0);
     model_dirty_ = true;
     sym_dirty_ = true;
     return ST_OK;
@@ -507,7 +515,7 @@ void Engine::admittance(int k, double& g, double& b) const {
 }
 
 bool Engine::meas_active(int i) const {
-    if (m_removed_[i]) return false;
+    if (m_removed_[i] || m_rejected_[i]) return false;
     int t = m_type_[i];
     if (is_bus_meas(t)) return !bus_removed_[m_loc_[i]];
     return br_status_[m_loc_[i]] != 0 && !br_removed_[m_loc_[i]];
@@ -544,6 +552,7 @@ void Engine::build_host_arrays(HostArrays& h) const {
         const int l = m_loc_[i];
         h.m_present[i] = !m_removed_[i] && (is_bus_meas(m_type_[i]) ? !bus_removed_[l] : !br_removed_[l]);
     }
+    h.rejected = m_rejected_;
     h.sigma = m_sigma_;
     h.z = m_z_;
 }
@@ -941,9 +950,52 @@ int Engine::solve(const SolverConfigIn& sc, ResultOut& out) {
         return rc;
     }
     SolveParams p = solve_params(sc);
+    if (cfg_.auto_bad_data_rejection) {  // every solve starts from the full telemetry
+        bool any = false;
+        for (int i = 0; i < n_meas(); ++i)
+            if (m_rejected_[i]) {
+                set_measurement_rejected(i, false, false);
+                any = true;
+            }
+        if (any && inst_ && !model_dirty_) inst_->refresh_topology();
+    }
     SolveOutcome o = inst_->solve(p);
     take_outcome(o, p, out);
+    if ((cfg_.auto_bad_data_rejection || opt_.lnr_in_result) && o.status == 0) lnr_result(sc, out);
     return ST_OK;
+}
+
+// The largest normalized residual of a converged estimate into the result;
+// with auto_bad_data_rejection, the largest-normalized-residual test repeated:
+// while it exceeds the threshold, that measurement is set aside and the state
+// re-estimated (at most max_eliminations per solve).
+void Engine::lnr_result(const SolverConfigIn& sc, ResultOut& out) {
+    const double th = cfg_.bad_data_threshold;
+    int eliminated = 0;
+    std::vector<double> rn, om;
+    for (;;) {
+        if (normalized_residuals(rn, om) != ST_OK) return;
+        int k = -1;
+        for (int i = 0; i < n_meas(); ++i)
+            if (rn[i] > 0 && (k < 0 || rn[i] > rn[k])) k = i;
+        out.largest_residual = k >= 0 ? rn[k] : 0.0;
+        out.largest_residual_idx = k;
+        out.bad_data_count = eliminated;
+        if (!cfg_.auto_bad_data_rejection || k < 0 || rn[k] <= th || eliminated >= opt_.max_eliminations) break;
+        set_measurement_rejected(k, true);
+        ++eliminated;
+        SolveParams p = solve_params(sc);
+        p.flat_start = 0;
+        SolveOutcome o = inst_->solve(p);
+        ResultOut again;
+        take_outcome(o, p, again);
+        again.iterations += out.iterations;
+        again.refactors += out.refactors;
+        out = again;
+        out.bad_data_count = eliminated;
+        if (o.status != 0) break;
+    }
+    last_ = out;
 }
 
 SolveParams Engine::solve_params(const SolverConfigIn& sc) const {
@@ -969,6 +1021,9 @@ int Engine::solve_no_alloc(const SolverConfigIn& sc, ResultOut& out) {
     if (!inst_ || model_dirty_ || params_dirty_ || pending)
         throw NoAllocError("the allocation-free solve needs the uploaded and solved model without pending model, "
                            "parameter or breaker changes (solve with sle_ext_Solve first)");
+    if (cfg_.auto_bad_data_rejection || opt_.lnr_in_result)
+        throw NoAllocError("the allocation-free solve does not run the normalized-residual test "
+                           "(auto_bad_data_rejection / lnr_in_result)");
     struct Unfreeze {
         Instance* i;
         ~Unfreeze() { i->freeze_allocations(false); }
@@ -1084,15 +1139,58 @@ void Engine::ensure_results() {
     results_dirty_ = false;
 }
 
-// sle_engine.cu:959-977: |r| / sigma above threshold over active rows
+// Active measurements whose normalized residual |r| / sqrt(Omega_ii) exceeds
+// the threshold (Abur & Exposito 2004, ch. 5), in index order; with option
+// bad_data_lnr = 0 the reference's test |r| / sigma > threshold
+// (sle_engine.cu:959-977).
 std::vector<int32_t> Engine::identify_bad_data(double threshold) {
-    ensure_results();
     double th = threshold > 0 ? threshold : cfg_.bad_data_threshold;
-    flush_staged_host();
     std::vector<int32_t> out;
+    if (opt_.bad_data_lnr) {
+        std::vector<double> rn, om;
+        if (normalized_residuals(rn, om) != ST_OK) return out;
+        for (int i = 0; i < n_meas(); ++i)
+            if (meas_active(i) && rn[i] > th) out.push_back(i);
+        return out;
+    }
+    ensure_results();
+    flush_staged_host();
     for (int i = 0; i < n_meas(); ++i)
         if (meas_active(i) && std::fabs(r_[i]) / m_sigma_[i] > th) out.push_back(i);
     return out;
+}
+
+int Engine::normalized_residuals(std::vector<double>& rn, std::vector<double>& omega) {
+    if (!initialized_) return ST_NOT_INITIALIZED;
+    int rc = ensure_ready();
+    if (rc != ST_OK) return rc;
+    const int m = n_meas();
+    std::vector<double> lev(m, 0.0), r(m, 0.0);
+    if (!inst_->leverages(lev.data(), r.data())) return ST_SINGULAR;
+    rn.assign(m, 0.0);
+    omega.assign(m, 0.0);
+    for (int i = 0; i < m; ++i) {
+        if (!meas_active(i)) continue;
+        const double s2 = m_sigma_[i] * m_sigma_[i];
+        const double om = s2 * (1.0 - lev[i]);
+        omega[i] = om;
+        // a critical measurement (Omega_ii ~ 0) has a structurally zero residual: not testable
+        rn[i] = om > 1e-10 * s2 ? std::fabs(r[i]) / std::sqrt(om) : 0.0;
+    }
+    results_dirty_ = true;
+    return ST_OK;
+}
+
+int Engine::set_measurement_rejected(int i, bool rejected, bool apply) {
+    if (i < 0 || i >= n_meas()) return ST_INVALID_ARGUMENT;
+    if ((m_rejected_[i] != 0) == rejected) return ST_OK;
+    m_rejected_[i] = rejected ? 1 : 0;
+    results_dirty_ = true;
+    if (inst_ && !model_dirty_) {
+        inst_->set_measurement_rejected(i, rejected);
+        if (apply) inst_->refresh_topology();
+    }
+    return ST_OK;
 }
 
 size_t Engine::gpu_bytes() const { return inst_ ? inst_->device_bytes() : 0; }

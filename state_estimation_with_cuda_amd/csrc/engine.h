@@ -98,6 +98,9 @@ struct EngineOptions {
     bool meas_by_type = true;     // measurement rows grouped by evaluation class on the device
     bool rhs_by_location = true;  // cached-factor iterations: RHS from per-location sums
     bool mixed_fp64_factor = true;  // mixed form: FP64 factorization, FP32 solve data
+    bool bad_data_lnr = true;     // bad data by normalized residuals |r|/sqrt(Omega_ii) (0: |r|/sigma, the reference's)
+    bool lnr_in_result = false;   // every solve reports the largest normalized residual (an extra refactor)
+    int max_eliminations = 10;    // auto_bad_data_rejection: measurements set aside per solve at most
 };
 
 class Engine {
@@ -183,6 +186,12 @@ class Engine {
     double last_solve_ms() const { return last_.ms; }
     const ResultOut& last_result() const { return last_; }
     std::vector<int32_t> identify_bad_data(double threshold);
+    // Normalized residuals rn_i = |r_i| / sqrt(Omega_ii) and Omega_ii at the
+    // current state (insertion order; 0 for inactive and critical rows).
+    int normalized_residuals(std::vector<double>& rn, std::vector<double>& omega);
+    // Set a measurement aside as bad data (inactive, structure unchanged) or back.
+    // apply = false: staged, applied by the next refresh of the device topology
+    int set_measurement_rejected(int i, bool rejected, bool apply = true);
 
     // lookups
     int bus_index(const char* id) const;
@@ -225,6 +234,7 @@ class Engine {
    private:
     SolveParams solve_params(const SolverConfigIn& sc) const;
     void take_outcome(const SolveOutcome& o, const SolveParams& p, ResultOut& out);
+    void lnr_result(const SolverConfigIn& sc, ResultOut& out);
     void build_host_arrays(HostArrays& h) const;
     void admittance(int k, double& g, double& b) const;
     int rebuild();
@@ -263,6 +273,7 @@ class Engine {
     std::vector<int32_t> m_type_, m_loc_, m_end_, m_pseudo_;
     std::vector<double> m_sigma_, m_pt_, m_ct_, m_z_;
     std::vector<uint8_t> m_removed_;
+    std::vector<uint8_t> m_rejected_;  // set aside as bad data (sle_ext_SetMeasurementActive, auto rejection)
     // switches
     std::vector<std::string> sd_id_;
     std::vector<int32_t> sd_branch_, sd_status_, sd_pending_;

@@ -30,6 +30,7 @@ struct HostArrays {
     std::vector<int32_t> mtype, mloc, mend;
     std::vector<uint8_t> active;     // 1 = measurement used now (its branch closed)
     std::vector<uint8_t> m_present;  // 1 = measurement in the structure (not removed)
+    std::vector<uint8_t> rejected;   // 1 = set aside as bad data (inactive; empty = none)
     std::vector<double> sigma, z;
 };
 
@@ -134,6 +135,18 @@ class Instance {
     // refresh_topology() uploads status and activity and recomputes Ybus.
     void set_branch_closed(int k, bool closed);
     void refresh_topology();
+    // Bad-data rejection of measurement i (insertion order): a value-only
+    // change of its activity, like a breaker operation; refresh_topology()
+    // applies it (and invalidates the factor).
+    void set_measurement_rejected(int i, bool rejected);
+    // Leverages s_i = h_i G^-1 h_i^T / sigma_i^2 and residuals r_i = z_i - h_i(x)
+    // of every measurement (insertion order) at the current state, with the
+    // WLS gain at that state: the gain is refactored in FP64, the sparse inverse
+    // subset Z = G^-1 is formed front by front from the root down, and each
+    // row's leverage comes from the one front holding all of its columns.  The
+    // factor left behind is the WLS gain's at this state (a cached factor for
+    // REALTIME iterations).  Returns false when the gain is singular.
+    bool leverages(double* lev, double* r);
     void set_z(const double* z);
     void set_z_indexed(const int32_t* idx, const double* val, int count);
     void set_state(const double* V, const double* th);

@@ -879,6 +879,28 @@ SLE_API SLE_StatusCode SLE_CALL sle_ext_SolveNoAlloc(SLE_Handle handle, const SL
     return ext_solve(handle, config, result, true);
 }
 
+SLE_API SLE_StatusCode SLE_CALL sle_ext_NormalizedResiduals(SLE_Handle handle, double* rn, double* omega,
+                                                           int32_t count) {
+    if (!handle) return fail(SLE_ERROR_INVALID_HANDLE, "NULL handle");
+    if (count < 0) return fail(SLE_ERROR_INVALID_ARGUMENT, "bad argument");
+    return guarded([&]() -> SLE_StatusCode {
+        std::vector<double> r, om;
+        int rc = E(handle).normalized_residuals(r, om);
+        if (rc != sle::ST_OK) return from_int(rc, "normalized residuals");
+        const size_t n = std::min<size_t>((size_t)count, r.size());
+        if (rn) std::copy(r.begin(), r.begin() + n, rn);
+        if (omega) std::copy(om.begin(), om.begin() + n, omega);
+        return SLE_OK;
+    });
+}
+
+SLE_API SLE_StatusCode SLE_CALL sle_ext_SetMeasurementActive(SLE_Handle handle, int32_t index, int32_t active) {
+    if (!handle) return fail(SLE_ERROR_INVALID_HANDLE, "NULL handle");
+    return guarded([&]() -> SLE_StatusCode {
+        return from_int(E(handle).set_measurement_rejected(index, active == 0), "set measurement active");
+    });
+}
+
 SLE_API const char* SLE_CALL sle_ext_Describe(SLE_Handle handle) {
     if (!handle) return nullptr;
     try {

@@ -145,6 +145,7 @@ EXT_FUNCTIONS = [
     "sle_ext_JacobianGain", "sle_ext_Factor", "sle_ext_SolveFactor", "sle_ext_StageTelemetry",
     "sle_ext_ApplyStagedTelemetry", "sle_ext_GetKernelTimes", "sle_ext_ResetKernelTimes", "sle_ext_SetOption",
     "sle_ext_Describe", "sle_ext_AutotuneSolve", "sle_ext_GetSolvePlanSizes", "sle_ext_SolveNoAlloc",
+    "sle_ext_NormalizedResiduals", "sle_ext_SetMeasurementActive",
 ]
 KERNEL_CLASSES = ["power", "meas", "spgemm", "factor", "rhs", "fwd", "bwd", "update", "pcg",
                   "sub_fwd", "pi_fwd", "pi_bwd", "sub_bwd", "mid_fwd", "mid_bwd"]
@@ -254,6 +255,8 @@ class Library:
             "sle_IsGPUAvailable": (i32, []),
             "sle_ext_Solve": (C.c_int, [H, P(S["SolverConfig"]), P(ExtResult)]),
             "sle_ext_SolveNoAlloc": (C.c_int, [H, P(S["SolverConfig"]), P(ExtResult)]),
+            "sle_ext_NormalizedResiduals": (C.c_int, [H, dp, dp, i32]),
+            "sle_ext_SetMeasurementActive": (C.c_int, [H, i32, i32]),
             "sle_ext_UpdateTelemetryIndexed": (C.c_int, [H, ip, dp, i32]),
             "sle_ext_UpdateTelemetryF64": (C.c_int, [H, dp, i32]),
             "sle_ext_SetState": (C.c_int, [H, dp, dp, i32]),
@@ -523,6 +526,25 @@ class Engine:
                     pivots_substituted=r.pivots_substituted, pcg_steps=r.pcg_steps,
                     graph_captures=r.graph_captures, seq_fallbacks=r.seq_fallbacks, retries=r.retries,
                     gpu_ms=r.gpu_ms, host_ms=r.host_ms)
+
+    def normalized_residuals(self):
+        """(rn, omega) per measurement: rn = |r| / sqrt(Omega_ii), Omega = R - H G^-1 H^T."""
+        rn = np.zeros(self.m)
+        om = np.zeros(self.m)
+        self._check(self.L.sle_ext_NormalizedResiduals(self.h, rn, om, self.m), "sle_ext_NormalizedResiduals")
+        return rn, om
+
+    def set_measurement_active(self, i, active):
+        self._check(self.L.sle_ext_SetMeasurementActive(self.h, int(i), int(bool(active))),
+                    "sle_ext_SetMeasurementActive")
+
+    def identify_bad_data(self, threshold=0.0, max_count=None):
+        n = self.m if max_count is None else int(max_count)
+        idx = np.zeros(max(n, 1), np.int32)
+        cnt = C.c_int32(0)
+        self._check(self.L.sle_IdentifyBadData(self.h, threshold, idx.ctypes.data_as(C.c_void_p), n,
+                                                C.byref(cnt)), "sle_IdentifyBadData")
+        return idx[:min(cnt.value, n)]
 
     def solve_mode(self, mode):
         res = self.S["Result"]()

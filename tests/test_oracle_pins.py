@@ -408,3 +408,47 @@ def test_huber_newton_reaches_the_irls_minimiser():
     assert ri["status"] == 0 and rn["status"] == 0
     assert np.abs(Vi - Vn).max() < 1e-9 and np.abs(ti - tn).max() < 1e-9
     assert rn["iterations"] < ri["iterations"]
+
+
+# ---------------------------------------------------------------------------
+# Largest normalized residual (bad-data identification)
+@pytest.mark.parametrize("maker", [lambda: gg.grid14(0), variant14, lambda: gg.geometric(60, seed=9, flow_meas=120)])
+def test_normalized_residuals_match_dense_covariance(maker):
+    """Omega = R - H G^-1 H^T computed densely at the oracle's optimum equals the
+    oracle's sparse-inverse-subset diagonal; rn = |r| / sqrt(Omega_ii); the
+    leverages 1 - Omega_ii / sigma_i^2 sum to the number of states."""
+    g = maker()
+    o = Oracle(g)
+    assert o.solve(mode=1, max_iter=50, tol=1e-12, flat_start=True)["status"] == 0
+    V, th = o.state()
+    rn, om = o.normalized_residuals()
+    rp, col = o.h_pattern()
+    H = dense_from_csr(rp, col, o.jacobian(V, th), (g.m, o.ns))
+    R = g.sigma ** 2
+    G = H.T @ (H / R[:, None])
+    Om = R - np.einsum("ij,ji->i", H, np.linalg.solve(G, H.T))
+    assert np.allclose(om, Om, rtol=1e-8, atol=1e-12 * R.max())
+    r = g.z - o.eval(V, th)["h"]
+    test = Om > 1e-10 * R
+    assert np.allclose(rn[test], np.abs(r[test]) / np.sqrt(Om[test]), rtol=1e-7)
+    assert np.all(rn[~test] == 0)
+    assert abs((1.0 - om / R).sum() - o.ns) < 1e-8 * o.ns
+
+
+def test_largest_normalized_residual_finds_a_gross_error():
+    """One reading corrupted by +20 sigma in a redundant grid: it has the
+    largest normalized residual, far above 3; without it none exceeds 4."""
+    g = gg.geometric(200, seed=12, flow_meas="all")
+    bad = 137
+    z = g.z.copy()
+    z[bad] += 20 * g.sigma[bad]
+    g2 = gg.Grid(**{**g.__dict__})
+    g2.z = z
+    o = Oracle(g2)
+    assert o.solve(mode=1, max_iter=50, tol=1e-12, flat_start=True)["status"] == 0
+    rn, _ = o.normalized_residuals()
+    assert int(np.argmax(rn)) == bad and rn[bad] > 10
+    o = Oracle(g)
+    assert o.solve(mode=1, max_iter=50, tol=1e-12, flat_start=True)["status"] == 0
+    rn, _ = o.normalized_residuals()
+    assert rn.max() < 4.5
