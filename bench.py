@@ -350,7 +350,10 @@ def areas_result(args, form, steps, warmup, per_instance_steps):
     log-uniform in [500, 20000], redundancy ~3) solved as one model -- every
     kernel launch covers all areas -- against one engine per area solved one
     after the other.  A step re-solves every area from its previous solution
-    perturbed by N(0, 0.01) per state element (PRECISION: refactor every iteration, tol 1e-6)."""
+    perturbed by N(0, 0.01) per state element (PRECISION: refactor every iteration, tol 1e-6).
+    Every iteration of the batch -- gain product, factorization, solve data,
+    solves and update for all areas -- is one replay of a captured HIP graph,
+    and the batch's solve and factorization plans are the autotuner's."""
     from state_estimation_with_cuda_amd import gridgen, sle
 
     t0 = time.perf_counter()
@@ -375,7 +378,11 @@ def areas_result(args, form, steps, warmup, per_instance_steps):
     r = eb.solve(mode=1, max_iterations=30, tolerance=TOL, flat_start=True)
     V0, th0 = eb.state()
     log(f"batched: setup + first solve {time.perf_counter() - t0:.1f}s, status {r['status']} iters {r['iterations']}")
-    tune = eb.autotune(3) if args.autotune else ""
+    # tile and supernode-block sizes of the batch chosen by the autotuner (timed on the device)
+    t0 = time.perf_counter()
+    tune = eb.autotune(3) if not args.no_autotune_areas else ""
+    if tune:
+        log(f"batched: autotune {time.perf_counter() - t0:.1f}s: {tune}")
     starts = [gridgen.hot_start(g, V0, th0, 0.01, seed=50 + k) for k in range(warmup + steps)]
     for k in range(warmup):
         eb.set_state(*starts[k])
@@ -621,6 +628,8 @@ def main():
     ap.add_argument("--areas", type=int, default=256)
     ap.add_argument("--form", choices=["fp64", "mixed"], default="fp64", help="areas workload: precision form")
     ap.add_argument("--autotune", action="store_true", help="autotune the solve and factorization plans first")
+    ap.add_argument("--no-autotune-areas", action="store_true",
+                    help="batched areas: keep the default plans instead of the autotuner's")
     ap.add_argument("--set", action="append", metavar="KEY=VALUE", help="engine option (sle_ext_SetOption), repeatable")
     ap.add_argument("--no-extras", action="store_true",
                     help="headline only: no Huber, batched-area or serving legs in the default line")

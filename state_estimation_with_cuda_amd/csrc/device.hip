@@ -3178,7 +3178,7 @@ struct Instance::Impl {
     // this instance is (re)allocated (its ledger's epoch)
     struct GraphSlot {
         uint64_t alloc_epoch, plan_epoch;
-        int prec, robust, newton;
+        int prec, robust, newton, refactor;
         double gamma, vmin, vmax;
         hipGraphExec_t exec;
     };
@@ -4688,14 +4688,16 @@ void iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p, bool 
     pf.end(PH_UPDATE, 2);
 }
 
-// One iteration on the cached factor (no refactor, direct solve) as a HIP
-// graph: the launches of iteration() are captured once per variant and
-// replayed with one launch; the status read-back follows it on the stream.
-// The captured work holds no memset node and no host-varying argument (the
-// dependency-driven sweeps, whose epochs change per launch, never come here).
+// One Gauss-Newton iteration (direct solve) as a HIP graph: the launches of
+// iteration() -- on the cached factor, or with the gain product, the
+// factorization and its solve data when it refactors (PRECISION, Huber, the
+// batched areas) -- are captured once per variant and replayed with one
+// launch; the status read-back follows it on the stream.  The captured work
+// holds no memset node and no host-varying argument (the dependency-driven
+// sweeps, whose epochs change per launch, never come here).
 template <typename T>
 void graph_iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p, T piv_tol, bool newton,
-                     uint64_t ae, SolveOutcome& out) {
+                     bool refactor, uint64_t ae, SolveOutcome& out) {
     const uint64_t pe = D.plan_epoch;
     const int prec = (int)sizeof(T);
     hipGraphExec_t exec = nullptr;
@@ -4706,8 +4708,8 @@ void graph_iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p,
             D.graphs.erase(D.graphs.begin() + k);
             continue;
         }
-        if (g.prec == prec && g.robust == p.robust && g.newton == (int)newton && g.gamma == p.gamma &&
-            g.vmin == p.vmin && g.vmax == p.vmax)
+        if (g.prec == prec && g.robust == p.robust && g.newton == (int)newton && g.refactor == (int)refactor &&
+            g.gamma == p.gamma && g.vmin == p.vmin && g.vmax == p.vmax)
             exec = g.exec;
         ++k;
     }
@@ -4716,7 +4718,7 @@ void graph_iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p,
         hipGraph_t graph = nullptr;
         HIPCHECK(hipStreamBeginCapture(D.stream, hipStreamCaptureModeThreadLocal));
         try {
-            iteration<T>(D, S, p, false, piv_tol, off, newton);
+            iteration<T>(D, S, p, refactor, piv_tol, off, newton);
         } catch (...) {
             (void)hipStreamEndCapture(D.stream, &graph);
             if (graph) (void)hipGraphDestroy(graph);
@@ -4726,11 +4728,11 @@ void graph_iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p,
         const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         HIPCHECK(e);
-        if (D.graphs.size() >= 8) {
+        if (D.graphs.size() >= 12) {
             (void)hipGraphExecDestroy(D.graphs.front().exec);
             D.graphs.erase(D.graphs.begin());
         }
-        D.graphs.push_back({ae, pe, prec, p.robust, (int)newton, p.gamma, p.vmin, p.vmax, exec});
+        D.graphs.push_back({ae, pe, prec, p.robust, (int)newton, (int)refactor, p.gamma, p.vmin, p.vmax, exec});
         out.graph_captures++;
     }
     // k_maxfinish, the graph's last kernel, writes the status into D.hst and
@@ -4788,8 +4790,8 @@ SolveOutcome Instance::solve(const SolveParams& p) {
             // Huber: the IRLS gain until the steps are small, then reduced gain
             // weights of the outliers (same minimiser, faster contraction)
             const bool newton = p.robust == 2 && !newton_off && last_step < kNewtonSwitch;
-            // replayed graph for iterations on the cached factor
-            const bool graph = use_graphs_ && !profiling_ && !refactor && !D.use_flow && (fp64 || p.pcg_steps <= 0);
+            // every direct-solve iteration replays a captured graph (one per variant)
+            const bool graph = use_graphs_ && !profiling_ && !D.use_flow && (fp64 || p.pcg_steps <= 0);
             if (fp64) {
                 if (!D.f64.ready) D.f64.alloc(S, bytes_, D.pi.sizes());
                 SolveParams pd = p;
@@ -4803,7 +4805,7 @@ SolveOutcome Instance::solve(const SolveParams& p) {
                                             D.stream));
                 }
                 if (graph)
-                    graph_iteration<double>(D, S, pd, 1e-13, newton, bytes_.epoch, out);
+                    graph_iteration<double>(D, S, pd, 1e-13, newton, refactor, bytes_.epoch, out);
                 else
                     iteration<double>(D, S, pd, refactor, 1e-13, pf, newton);
                 if (split) {
@@ -4812,7 +4814,7 @@ SolveOutcome Instance::solve(const SolveParams& p) {
                 }
             } else {
                 if (graph)
-                    graph_iteration<float>(D, S, p, 1e-5f, newton, bytes_.epoch, out);
+                    graph_iteration<float>(D, S, p, 1e-5f, newton, refactor, bytes_.epoch, out);
                 else
                     iteration<float>(D, S, p, refactor, 1e-5f, pf, newton);
             }

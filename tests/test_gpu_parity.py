@@ -569,3 +569,44 @@ def test_mixed_form_on_fp64_factor():
     assert np.abs(V - oV).max() < STATE_TOL and np.abs(th - oth).max() < STATE_TOL
     es.close()
     e64.close()
+
+
+@pytest.mark.parametrize("fp64", [True, False])
+def test_batched_precision_iterations_replay_graphs(fp64):
+    """Refactoring iterations (gain product, factorization, solve data, solves,
+    update) replay captured graphs too: a batched-area PRECISION solve with
+    graphs is bit-identical to the eager one, captures its graphs once and
+    reuses them on the next solve (a hot start from the perturbed solution),
+    and matches the oracle."""
+    areas = [gg.geometric(1500, seed=41, flow_meas=4000), gg.grid14(4), gg.geometric(800, seed=42, flow_meas="all"),
+             gg.geometric(3000, seed=43, flow_meas=9000)]
+    g = gg.concat(areas)
+    eg = engine_for(g, double_precision=fp64, use_cuda_graphs=1)
+    ee = engine_for(g, double_precision=fp64, use_cuda_graphs=0)
+    out = []
+    for e in (eg, ee):
+        r = e.solve(mode=1, max_iterations=50, tolerance=1e-8, flat_start=True)
+        assert r["status"] == 0, r
+        out.append((r, e.state()))
+    (rg, (Vg, tg)), (re_, (Ve, te)) = out
+    assert rg["graph_captures"] >= 1 and re_["graph_captures"] == 0
+    assert rg["iterations"] == re_["iterations"]
+    assert np.array_equal(Vg, Ve) and np.array_equal(tg, te)
+    start = gg.hot_start(g, Vg, tg, 0.01, seed=3)
+    out = []
+    for e in (eg, ee):
+        e.set_state(*start)
+        r = e.solve(mode=1, max_iterations=50, tolerance=1e-8)
+        assert r["status"] == 0, r
+        out.append((r, e.state()))
+    (rg, (Vg, tg)), (re_, (Ve, te)) = out
+    print(f"[batched graphs fp64={fp64}] iters={rg['iterations']} captures={rg['graph_captures']} "
+          f"fallbacks={rg['seq_fallbacks']}", flush=True)
+    assert rg["graph_captures"] == 0 and rg["seq_fallbacks"] == 0, rg
+    assert np.array_equal(Vg, Ve) and np.array_equal(tg, te)
+    o = Oracle(g)
+    o.solve(mode=1, max_iter=50, tol=1e-11, flat_start=True)
+    oV, oth = o.state()
+    assert np.abs(Vg - oV).max() < STATE_TOL and np.abs(tg - oth).max() < STATE_TOL
+    eg.close()
+    ee.close()
