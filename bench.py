@@ -290,6 +290,17 @@ def run_form(g, fp64, reqs, batches, warmup, steps, profile_steps, pcg_steps=0, 
         h_ok += r["status"] == 0
     h_total = time.perf_counter() - t_start
     objective = r["objective"]
+    # largest-normalized-residual test at this estimate: the first call builds the
+    # row-to-front plan on the host, the second is the steady-state cost
+    lnr = None
+    if huber:
+        e.normalized_residuals()
+        t0 = time.perf_counter()
+        rn, _ = e.normalized_residuals()
+        lnr = dict(ms=round((time.perf_counter() - t0) * 1e3, 2), max_rn=round(float(rn.max()), 3),
+                   above_3=int(np.sum(rn > 3.0)), above_6=int(np.sum(rn > 6.0)),
+                   gross_errors_planted=int(round(0.01 * g.m)))
+        log(f"{name}: normalized residuals {lnr}")
     e_plan = e.describe()
     plan_sizes = e.solve_plan_sizes()
     last_z = request_z(g, reqs[warmup + 2 * steps - 1])
@@ -342,6 +353,7 @@ def run_form(g, fp64, reqs, batches, warmup, steps, profile_steps, pcg_steps=0, 
         plan=e_plan,
         autotune=tune,
         huber=hub,
+        lnr=lnr,
     )
 
 
@@ -800,6 +812,8 @@ def main():
         h = f["huber"]
         if h:
             out[f["name"]]["huber200k"] = {k: v for k, v in h.items() if k not in ("V", "th")}
+        if f["lnr"]:
+            out[f["name"]]["lnr200k"] = f["lnr"]
     if extras:
         # the serving loop on the resident 200k model, both forms (the same request stream)
         cache = {}

@@ -87,12 +87,11 @@ typedef struct SLE_ExtResult {
  *                     depends on
  *   "huber_newton"    1 (default): Huber estimation takes IRLS gain weights
  *                     (the reference's, kernels.cu:634-671) until the steps fall
- *                     below 1e-2, then gain weights 1e-4/sigma^2 for the
- *                     outliers (|u| = |r|/sigma > gamma): the Huber loss has zero
- *                     curvature there, so this is a semismooth Newton step with
- *                     the gain kept positive definite; the right-hand side keeps
- *                     the IRLS weights, so the minimiser is the same, in far fewer
- *                     iterations; 0: IRLS throughout
+ *                     below 1e-2, then gain weights (gamma/|u|)^6 / sigma^2 for the
+ *                     outliers (|u| = |r|/sigma > gamma), nearer the Huber loss's
+ *                     zero curvature there yet continuous at gamma; the right-hand
+ *                     side keeps the IRLS weights, so the minimiser is the same,
+ *                     in fewer iterations; 0: IRLS throughout
  *   "level_schedule"  1 (default): factor and solve level by level (kernel
  *                     boundaries order the fronts); 0: one dependency-driven
  *                     launch per sweep with release/acquire flags per front */

@@ -56,7 +56,6 @@ namespace {
 
 constexpr double kEps = 1e-12;
 constexpr double kNewtonSwitch = 1e-2;  // Huber: step size below which the outliers' gain weights drop
-constexpr double kOutlierGain = 1e-4;    // Huber near the solution: outliers' gain weight / (1/sigma^2)
 enum { V_MAG = 0, V_ANGLE = 1, P_INJ = 2, Q_INJ = 3, P_FLOW = 4, Q_FLOW = 5, I_MAG = 6, P_PSEUDO = 7, Q_PSEUDO = 8 };
 
 struct Case {
@@ -908,14 +907,18 @@ void or_solve(void* hh, const OracleSolveParams* prm, OracleSolveResult* out) {
                 w[i] = c.active[i] ? wi : 0.0;
                 // robust == 2: near the solution the gain weights of the outliers
                 // fall from the IRLS weight gamma/(sigma^2 |u|) to
-                // kOutlierGain/sigma^2: the Huber loss has zero curvature beyond
-                // gamma, so this is (semismooth) Newton's gain kept positive
-                // definite; the right-hand side stays H^T psi(u)/sigma, so the
-                // minimiser is the IRLS one
+                // (gamma/|u|)^6 / sigma^2: nearer the Huber loss's zero curvature
+                // beyond gamma, yet continuous at gamma (a weight that jumps to
+                // ~0 there lets rows near the threshold flip in and out and the
+                // iteration cycle); the right-hand side stays H^T psi(u)/sigma,
+                // so the minimiser is the IRLS one
                 wg[i] = w[i];
                 if (newton) {
                     const double u = std::fabs(r[i]) / s;
-                    if (u > prm->gamma && c.active[i]) wg[i] = kOutlierGain / (s * s);
+                    if (u > prm->gamma && c.active[i]) {
+                        const double x = prm->gamma / u;
+                        wg[i] = x * x * x * x * x * x / (s * s);
+                    }
                 }
             }
             eval_jacobian(c, c.V.data(), c.th.data(), P.data(), Q.data(), Pf.data(), Qf.data(), Pt.data(),

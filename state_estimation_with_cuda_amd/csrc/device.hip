@@ -61,10 +61,6 @@ constexpr int kFrontB = 64;    // 32 < c <= 64: LDS front of 64x64
 constexpr int kVecMax = 2048;  // triangular-solve LDS vector bound for large fronts
 constexpr int kSpinLimit = 1 << 22;  // polls before a dependency wait gives up (~0.1 s)
 constexpr double kNewtonSwitch = 1e-2;  // Huber: step size below which the outliers' gain weights drop
-// Huber near the solution: an outlier's gain weight, as a fraction of its base
-// weight 1/sigma^2 (the Huber loss has zero curvature beyond gamma: exact Newton
-// would give it none; a small positive weight keeps the gain positive definite)
-constexpr double kOutlierGain = 1e-4;
 constexpr double kRefreshRatio = 0.25;  // REALTIME: step ratio above which the cached factor is refreshed
 constexpr double kLinearRegime = 1e-3;  // steps below this: Gauss-Newton contracts fast with a sound factor
 
@@ -238,7 +234,7 @@ __global__ __launch_bounds__(kBlock) void k_flows(int nb, const int32_t* __restr
 struct MeasArgs {
     int m, n;
     int robust, unit_weights, want_jac;
-    int newton;  // Huber near the solution: outliers' gain weight kOutlierGain / sigma^2 (not in the RHS)
+    int newton;  // Huber near the solution: outliers' gain weight (gamma/|u|)^6 / sigma^2 (not in the RHS)
     const int32_t* pth;  // angle state of each bus (-1 for an angle reference)
     double gamma;
     const uint8_t* mcode;
@@ -365,9 +361,10 @@ __global__ __launch_bounds__(kBlock) void k_meas(MeasArgs A, double* __restrict_
         if (!act) w = 0.0;
         const double unit = act ? 1.0 : 0.0;
         const double sw = A.unit_weights ? unit : sqrt(w);
-        // gain weight: IRLS, or near the solution kOutlierGain / sigma^2 for outliers
-        // (semismooth Newton on the Huber loss)
-        swg = A.unit_weights ? unit : (A.newton && outlier ? sqrt(kOutlierGain) / sg : sw);
+        // gain weight: IRLS, or near the solution (gamma/|u|)^6 / sigma^2 for the
+        // outliers (|u| = |r| / sigma > gamma)
+        const double xg = A.gamma * sg / fabs(r);
+        swg = A.unit_weights ? unit : (A.newton && outlier ? xg * xg * xg / sg : sw);
         wr = A.unit_weights ? unit * r : w * r;
         A.rs[i] = sw * r;
         if (A.hout) {

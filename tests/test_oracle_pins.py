@@ -396,9 +396,10 @@ def test_independent_areas_solve_like_separate_models():
 
 
 def test_huber_newton_reaches_the_irls_minimiser():
-    """Huber with semismooth-Newton gain weights (outliers at 1e-4 of their
-    weight in the gain once the steps are small) converges to the same
-    M-estimate as the reference's IRLS weights, in at most half the iterations."""
+    """Huber with reduced gain weights near the solution (outliers at
+    (gamma/|u|)^6 of their weight in the gain once the steps are small)
+    converges to the same M-estimate as the reference's IRLS weights, in at most
+    two thirds of the iterations."""
     g = gg.geometric(3000, seed=23, flow_meas="all", dup_meas=600, gross_frac=0.01)
     o = Oracle(g)
     ri = o.solve(mode=1, max_iter=400, tol=1e-11, robust=1, gamma=1.5, flat_start=True)
@@ -407,8 +408,8 @@ def test_huber_newton_reaches_the_irls_minimiser():
     Vn, tn = o.state()
     assert ri["status"] == 0 and rn["status"] == 0
     assert np.abs(Vi - Vn).max() < 1e-9 and np.abs(ti - tn).max() < 1e-9
-    print(f"IRLS {ri['iterations']} iterations, semismooth Newton {rn['iterations']}")
-    assert 2 * rn["iterations"] <= ri["iterations"]
+    print(f"IRLS {ri['iterations']} iterations, reduced gain weights {rn['iterations']}")
+    assert 3 * rn["iterations"] <= 2 * ri["iterations"]
 
 
 # ---------------------------------------------------------------------------
