@@ -19,10 +19,13 @@ updates pushed from host memory (sle_ext_UpdateTelemetryIndexed through pinned
 staging) with the estimate read back to the host inside the timed step.
 
 Beside the headline, unless --no-extras:
-  huber200k   flat-start Huber solve (gamma 1.5 sigma) of the same grid, both
-              forms: time, iterations, supernode-size histogram, error vs the
-              oracle (warm-started at the GPU estimate and iterated to 1e-9:
-              the Huber objective is convex, with one minimiser);
+  huber200k   flat-start Huber solve (gamma 1.5 sigma, to 1e-8) of the same
+              grid, both forms: time, iterations, supernode-size histogram,
+              error vs the oracle (warm-started at the GPU estimate and
+              iterated to 1e-10: the Huber objective is convex, with one
+              minimiser);
+  lnr200k     the largest-normalized-residual test at the estimate: time of
+              one pass (Omega_ii of all 2M rows), largest rn, rows above 3;
   areas256    256 independent areas (500-20,000 buses) batched in one model,
               each re-solved from its previous solution + N(0, 0.01), against
               one engine per area;
@@ -56,6 +59,7 @@ TOL = 1e-6
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_200k.md")
 AREA_MAX_ITER = 25  # a batch iterates until its slowest area converges
 HUBER_GAMMA = 1.5
+HUBER_TOL = 1e-8  # the Huber iteration contracts linearly: a 1e-6 step still leaves ~1e-6 to the minimiser
 AREA_MODE = 1  # PRECISION: a perturbed start refactors (the cached factor of the old point may not contract)
 COPY_BYTES = 512 << 20
 
@@ -317,10 +321,11 @@ def run_form(g, fp64, reqs, batches, warmup, steps, profile_steps, pcg_steps=0, 
     if huber:
         e.set_telemetry_f64(g.z)
         t0 = time.perf_counter()
-        rh = e.solve(mode=1, max_iterations=150, tolerance=TOL, robust=True, gamma=HUBER_GAMMA, flat_start=True)
+        rh = e.solve(mode=1, max_iterations=150, tolerance=HUBER_TOL, robust=True, gamma=HUBER_GAMMA,
+                     flat_start=True)
         wall = (time.perf_counter() - t0) * 1e3
         hV, hth = e.state()
-        hub = dict(status=rh["status"], iterations=rh["iterations"], refactors=rh["refactors"],
+        hub = dict(tolerance=HUBER_TOL, status=rh["status"], iterations=rh["iterations"], refactors=rh["refactors"],
                    gpu_ms=round(rh["gpu_ms"], 2), wall_ms=round(wall, 2), objective=rh["objective"],
                    supernodes=supernode_histogram(e), V=hV, th=hth)
         log(f"{name}: Huber flat start {wall:.1f} ms, iters={rh['iterations']} status={rh['status']}")
@@ -793,7 +798,7 @@ def main():
             "one_core_solves_per_s": round(len(times1) / sum(times1), 4),
             "one_core_sample": f"{len(times1)} of those updates on 1 thread: {sum(times1):.1f}s",
         }
-        # Huber: the oracle iterated from each form's estimate to 1e-9 (one minimiser)
+        # Huber: the oracle iterated from each form's estimate to 1e-10 (one minimiser)
         for f in forms:
             h = f["huber"]
             if not h:
@@ -801,7 +806,7 @@ def main():
             o.set_z(g.z)
             o.set_state(h["V"], h["th"])
             t0 = time.perf_counter()
-            ro = o.solve(mode=1, max_iter=40, tol=1e-9, robust=2, gamma=HUBER_GAMMA)
+            ro = o.solve(mode=1, max_iter=60, tol=1e-10, robust=2, gamma=HUBER_GAMMA)
             hV, hth = o.state()
             h["error_vs_oracle"] = {"V": float(np.abs(h["V"] - hV).max()), "theta": float(np.abs(h["th"] - hth).max()),
                                     "J_rel": abs(h["objective"] - ro["objective"]) / abs(ro["objective"]),
