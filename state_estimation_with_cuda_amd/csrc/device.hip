@@ -3105,6 +3105,7 @@ struct Instance::Impl {
     // narrow top tail in one workgroup
     bool tasks_ok = false;
     bool use_tasks = true;  // option: task sweeps when the plan exists
+    int force_cut = 0, force_wcut = 0;  // option: factorization plan (0: the cost model's)
     int n_bottom = 0, tail_waves = 0, bottom_waves = 4;
     bool has_tail = false;
     DBuf<int32_t> t_fronts, t_lvl_ptr, t_lvl_off, t_mid;
@@ -3815,8 +3816,18 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
         D.tail_waves = (int)std::max<size_t>(1, std::min<size_t>(16, 65536 / std::max<size_t>(1, slice_bytes)));
         int best_cut = 0, best_w = 0;
         double best = D.level_cost_us;
+        if (D.force_cut > 0) {  // a caller-chosen plan (option factor_cut / factor_wcut)
+            best_cut = D.force_cut;
+            best_w = D.force_wcut > 0 ? D.force_wcut : 4;
+            best = make_plan(S, best_cut, best_w, D.bottom_waves, D.tail_waves, false).cost_us;
+        } else
+        // the single-workgroup tail takes levels of at most 16 fronts: wider
+        // levels ran there one front per wave in turn, ~10 us a round, while a
+        // launch of their own costs one round (measured at 200k buses: tails of
+        // levels up to 2-16 fronts wide 2.32 ms per factorization, up to 64 wide
+        // 2.54 ms)
         for (int cut : {8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096})
-            for (int wc : {2, 4, 8, 16, 32, 64, 128}) {
+            for (int wc : {2, 4, 8, 16}) {
                 SweepPlan P = make_plan(S, cut, wc, D.bottom_waves, D.tail_waves, false);
                 if (P.cost_us < best) {
                     best = P.cost_us;
@@ -5459,6 +5470,12 @@ void Instance::set_level_schedule(bool on) {
 }
 
 bool Instance::dataflow_sweeps() const { return d_->use_flow; }
+
+void Instance::set_factor_plan(int cut, int wcut) {
+    ++d_->plan_epoch;
+    d_->force_cut = cut;
+    d_->force_wcut = wcut;
+}
 
 void Instance::set_task_sweeps(bool on) {
     ++d_->plan_epoch;

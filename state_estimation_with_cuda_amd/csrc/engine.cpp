@@ -197,6 +197,10 @@ int Engine::set_option(const char* key, double value) {
     } else if (k == "level_schedule") {
         opt_.level_schedule = value != 0;
         if (inst_) inst_->set_level_schedule(opt_.level_schedule);
+    } else if (k == "factor_cut" || k == "factor_wcut") {
+        if (value < 0 || value > 1e6) return ST_INVALID_ARGUMENT;
+        (k == "factor_cut" ? opt_.factor_cut : opt_.factor_wcut) = (int)value;
+        model_dirty_ = true;  // the plan is rebuilt at the next upload
     } else if (k == "bad_data_lnr") {
         opt_.bad_data_lnr = value != 0;
     } else if (k == "lnr_in_result") {
@@ -687,6 +691,7 @@ int Engine::rebuild() {
     inst_->set_use_graphs(cfg_.use_cuda_graphs);
     inst_->set_level_schedule(opt_.level_schedule);
     inst_->set_task_sweeps(opt_.task_sweeps);
+    inst_->set_factor_plan(opt_.factor_cut, opt_.factor_wcut);
     inst_->set_lds_solve(opt_.lds_solve);
     inst_->set_lds_waves(opt_.lds_waves_bottom, opt_.lds_waves_tail);
     inst_->set_partitioned_inverse(opt_.pinv, opt_.pinv_cut, opt_.pinv_waves);

@@ -101,6 +101,7 @@ struct EngineOptions {
     bool bad_data_lnr = true;     // bad data by normalized residuals |r|/sqrt(Omega_ii) (0: |r|/sigma, the reference's)
     bool lnr_in_result = false;   // every solve reports the largest normalized residual (an extra refactor)
     int max_eliminations = 10;    // auto_bad_data_rejection: measurements set aside per solve at most
+    int factor_cut = 0, factor_wcut = 0;  // factorization plan (0: the cost model's)
 };
 
 class Engine {

@@ -195,6 +195,10 @@ class Instance {
     bool dataflow_sweeps() const;
     // task sweeps (bottom subtrees per workgroup, narrow top in one workgroup)
     void set_task_sweeps(bool on);
+    // factorization plan: bottom subtrees of at most `cut` fronts per workgroup,
+    // top levels of at most `wcut` fronts in the single-workgroup tail (0: the
+    // cost model's choice; takes effect at setup)
+    void set_factor_plan(int cut, int wcut);
     // LDS-resident task triangular solves
     void set_lds_solve(bool on);
     // partitioned-inverse solve of the top of the tree; cut = fronts per bottom
