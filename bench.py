@@ -630,7 +630,7 @@ def main():
     ap.add_argument("--workload", choices=["update200k", "areas", "serve"], default="update200k",
                     help="update200k: the headline metric; areas: batched uneven areas; serve: serving loop")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: update arrivals per second")
-    ap.add_argument("--duration", type=float, default=30.0, help="serve: seconds of arrivals")
+    ap.add_argument("--duration", type=float, default=60.0, help="serve: seconds of arrivals")
     ap.add_argument("--serve-warmup", type=float, default=5.0, help="serve: seconds excluded from the statistics")
     ap.add_argument("--area-rate", type=float, default=1.0, help="serve: new areas per second")
     ap.add_argument("--area-pool", type=int, default=6)

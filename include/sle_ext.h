@@ -225,6 +225,13 @@ SLE_API SLE_StatusCode SLE_CALL sle_ext_GetOrdering(SLE_Handle handle, int32_t* 
                                                     int32_t* parent, int32_t* colcount, int32_t* snode_first);
 
 /* --- numeric introspection at the current state ------------------------- */
+/* The Gauss-Newton right-hand side b = H^T W r (WLS weights 1/sigma^2, r = z -
+ * h(x)) at the current state, natural state order (n_states entries), from the
+ * device kernels of the iterations on a cached factor (by_location = 1: summed
+ * weighted residuals per bus and branch end, gathered per state) or of the
+ * refactoring iterations (0: per Jacobian entry, summed per state).  Returns
+ * SLE_ERROR_INVALID_ARGUMENT for by_location = 1 when that form is off. */
+SLE_API SLE_StatusCode SLE_CALL sle_ext_GainRhs(SLE_Handle handle, int32_t by_location, double* b);
 SLE_API SLE_StatusCode SLE_CALL sle_ext_Evaluate(SLE_Handle handle, double* h, double* p_inj, double* q_inj,
                                                  double* p_from, double* q_from, double* p_to, double* q_to);
 /* H values in the H pattern order and G = H^T H (unit weights) in the natural G pattern order */

@@ -94,6 +94,7 @@ def lib():
         L.or_eval.argtypes = [C.c_void_p, f64p, f64p, f64p, f64p, f64p, f64p, f64p, f64p, f64p]
         L.or_jacobian.argtypes = [C.c_void_p, f64p, f64p, f64p]
         L.or_gain.argtypes = [C.c_void_p, f64p, f64p, f64p, f64p]
+        L.or_rhs.argtypes = [C.c_void_p, f64p, f64p, f64p]
         L.or_factor.restype = C.c_int32
         L.or_factor.argtypes = [C.c_void_p, f64p, i64p, i32p, f64p]
         L.or_solve.argtypes = [C.c_void_p, C.POINTER(SolveParams), C.POINTER(SolveResult)]
@@ -210,6 +211,12 @@ class Oracle:
         hv = np.zeros(self.nnz_h)
         lib().or_jacobian(self.h, np.ascontiguousarray(V, np.float64), np.ascontiguousarray(th, np.float64), hv)
         return hv
+
+    def rhs(self, V, th):
+        """Gauss-Newton right-hand side H^T W r at (V, th), natural state order."""
+        b = np.zeros(self.ns)
+        lib().or_rhs(self.h, np.ascontiguousarray(V, np.float64), np.ascontiguousarray(th, np.float64), b)
+        return b
 
     def gain(self, V, th, w):
         gv = np.zeros(self.nnz_g)

@@ -839,6 +839,29 @@ void or_jacobian(void* h, const double* V, const double* th, double* hv) {
     eval_jacobian(*c, V, th, P.data(), Q.data(), Pf.data(), Qf.data(), Pt.data(), Qt.data(), hv);
 }
 
+// Gauss-Newton right-hand side b = H^T W r at the given state (WLS weights,
+// natural state order), as in the GN loop (kernels.cu:823-862 semantics, sums
+// per state over the transposed H pattern).
+void or_rhs(void* h, const double* V, const double* th, double* b) {
+    Case* c = (Case*)h;
+    std::vector<double> P(c->n), Q(c->n), Pf(c->nb), Qf(c->nb), Pt(c->nb), Qt(c->nb), hx(c->m);
+    std::vector<double> hv(c->hcol.size());
+    eval_power(*c, V, th, P.data(), Q.data(), Pf.data(), Qf.data(), Pt.data(), Qt.data());
+    eval_h(*c, V, th, P.data(), Q.data(), Pf.data(), Qf.data(), Pt.data(), Qt.data(), hx.data());
+    eval_jacobian(*c, V, th, P.data(), Q.data(), Pf.data(), Qf.data(), Pt.data(), Qt.data(), hv.data());
+    std::vector<int> tptr, trow, tpos;
+    h_transpose(*c, tptr, trow, tpos);
+    for (int j = 0; j < c->ns; ++j) {
+        double s = 0;
+        for (int k = tptr[j]; k < tptr[j + 1]; ++k) {
+            const int i = trow[k];
+            if (!c->active[i]) continue;
+            s += hv[tpos[k]] * (c->z[i] - hx[i]) / (c->sigma[i] * c->sigma[i]);
+        }
+        b[j] = s;
+    }
+}
+
 // G lower values (pattern order of or_get_g_pattern) for state and weights w.
 void or_gain(void* h, const double* V, const double* th, const double* w, double* gv) {
     Case* c = (Case*)h;

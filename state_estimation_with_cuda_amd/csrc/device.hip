@@ -5364,6 +5364,33 @@ void Instance::debug_jacobian_gain(double* hv, double* gv) {
     }
 }
 
+bool Instance::debug_rhs(bool by_location, double* b) {
+    HIPCHECK(hipSetDevice(device_));
+    StreamScope scope_(d_->stream);
+    Impl& D = *d_;
+    const Symbolic& S = *S_;
+    if (by_location && !D.use_loc) return false;
+    if (!D.f64.ready) D.f64.alloc(S, bytes_, D.pi.sizes());
+    if (by_location) {
+        launch_meas_loc(D, 0, 1.5);
+        launch_rhs_loc<double>(D);
+    } else {
+        launch_power(D);
+        launch_meas(D, 0, 1.5, 0, 0, false, true);
+        launch_rhs<double>(D);
+    }
+    DBuf<double> out;
+    Ledger tmp;
+    out.alloc(std::max(1, D.ns), tmp);
+    if (D.ns) k_permute_out<double><<<nblk(D.ns), kBlock, 0, D.stream>>>(D.ns, D.perm.p, D.f64.x.p, out.p);
+    HIPCHECK(hipGetLastError());
+    if (D.ns) HIPCHECK(hipMemcpyAsync(b, out.p, D.ns * sizeof(double), hipMemcpyDeviceToHost, D.stream));
+    HIPCHECK(hipStreamSynchronize(D.stream));
+    out.release(tmp);
+    D.results_valid = false;
+    return true;
+}
+
 int Instance::debug_factor(double* fronts) {
     ++d_->plan_epoch;  // launch plans change: captured graphs retire
     HIPCHECK(hipSetDevice(device_));

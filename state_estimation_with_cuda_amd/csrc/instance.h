@@ -179,6 +179,11 @@ class Instance {
     int debug_factor(double* fronts);
     // Solve G dx = b with the current factor (b, dx in natural state order).
     void debug_solve(const double* b, double* dx);
+    // The Gauss-Newton right-hand side H^T W r at the current state (WLS
+    // weights, natural state order) from the device kernels of either form:
+    // by location (k_meas_loc + k_rhs_loc) or per Jacobian entry (k_meas +
+    // k_rhs_sum).  Returns false when the by-location form is not set up.
+    bool debug_rhs(bool by_location, double* b);
 
     size_t device_bytes() const { return bytes_.bytes; }
     // while on, any device (re)allocation of this instance throws NoAllocError

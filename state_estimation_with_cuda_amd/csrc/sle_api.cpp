@@ -1136,6 +1136,18 @@ SLE_API SLE_StatusCode SLE_CALL sle_ext_Evaluate(SLE_Handle handle, double* h, d
     });
 }
 
+SLE_API SLE_StatusCode SLE_CALL sle_ext_GainRhs(SLE_Handle handle, int32_t by_location, double* b) {
+    if (!handle || !b) return fail(SLE_ERROR_INVALID_ARGUMENT, "NULL argument");
+    return guarded([&]() -> SLE_StatusCode {
+        if (!E(handle).initialized()) return fail(SLE_ERROR_NOT_INITIALIZED, "engine not initialized");
+        int rc = E(handle).ensure_ready();
+        if (rc != sle::ST_OK) return from_int(rc, "gain rhs");
+        if (!E(handle).instance()->debug_rhs(by_location != 0, b))
+            return fail(SLE_ERROR_INVALID_ARGUMENT, "the right-hand side by location is off (option rhs_by_location)");
+        return SLE_OK;
+    });
+}
+
 SLE_API SLE_StatusCode SLE_CALL sle_ext_JacobianGain(SLE_Handle handle, double* h_values, double* g_values) {
     if (!handle) return fail(SLE_ERROR_INVALID_HANDLE, "NULL handle");
     return guarded([&]() -> SLE_StatusCode {

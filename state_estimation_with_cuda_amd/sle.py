@@ -145,7 +145,7 @@ EXT_FUNCTIONS = [
     "sle_ext_JacobianGain", "sle_ext_Factor", "sle_ext_SolveFactor", "sle_ext_StageTelemetry",
     "sle_ext_ApplyStagedTelemetry", "sle_ext_GetKernelTimes", "sle_ext_ResetKernelTimes", "sle_ext_SetOption",
     "sle_ext_Describe", "sle_ext_AutotuneSolve", "sle_ext_GetSolvePlanSizes", "sle_ext_SolveNoAlloc",
-    "sle_ext_NormalizedResiduals", "sle_ext_SetMeasurementActive",
+    "sle_ext_NormalizedResiduals", "sle_ext_SetMeasurementActive", "sle_ext_GainRhs",
 ]
 KERNEL_CLASSES = ["power", "meas", "spgemm", "factor", "rhs", "fwd", "bwd", "update", "pcg",
                   "sub_fwd", "pi_fwd", "pi_bwd", "sub_bwd", "mid_fwd", "mid_bwd"]
@@ -257,6 +257,7 @@ class Library:
             "sle_ext_SolveNoAlloc": (C.c_int, [H, P(S["SolverConfig"]), P(ExtResult)]),
             "sle_ext_NormalizedResiduals": (C.c_int, [H, dp, dp, i32]),
             "sle_ext_SetMeasurementActive": (C.c_int, [H, i32, i32]),
+            "sle_ext_GainRhs": (C.c_int, [H, i32, dp]),
             "sle_ext_UpdateTelemetryIndexed": (C.c_int, [H, ip, dp, i32]),
             "sle_ext_UpdateTelemetryF64": (C.c_int, [H, dp, i32]),
             "sle_ext_SetState": (C.c_int, [H, dp, dp, i32]),
@@ -664,6 +665,12 @@ class Engine:
         self._check(self.L.sle_ext_Evaluate(self.h, out["h"], out["P"], out["Q"], out["Pf"], out["Qf"], out["Pt"],
                                             out["Qt"]), "sle_ext_Evaluate")
         return out
+
+    def gain_rhs(self, by_location=True):
+        ns = self.sizes()["ns"]
+        b = np.zeros(ns)
+        self._check(self.L.sle_ext_GainRhs(self.h, int(bool(by_location)), b), "sle_ext_GainRhs")
+        return b
 
     def jacobian_gain(self):
         sz = self.sizes()

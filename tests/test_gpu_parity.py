@@ -610,3 +610,59 @@ def test_batched_precision_iterations_replay_graphs(fp64):
     assert np.abs(Vg - oV).max() < STATE_TOL and np.abs(tg - oth).max() < STATE_TOL
     eg.close()
     ee.close()
+
+
+RHS_CASES = {"grid14": GRIDS["grid14"], "geo2k": GRIDS["geo2k"], "geo10k": GRIDS["geo10k"]}
+
+
+@pytest.mark.parametrize("name", list(RHS_CASES))
+@pytest.mark.parametrize("fp64", [True, False])
+def test_rhs_vector_matches_oracle(name, fp64):
+    """The Gauss-Newton right-hand side H^T W r from both device forms (by
+    location, for iterations on a cached factor; per Jacobian entry, for
+    refactoring iterations) equals the oracle's at a perturbed state."""
+    g = RHS_CASES[name]()
+    o = Oracle(g)
+    e = engine_for(g, double_precision=fp64)
+    V, th = random_state(g, seed=12, spread=0.02)
+    e.set_state(V, th)
+    ob = o.rhs(V, th)
+    for loc in (True, False):
+        b = e.gain_rhs(by_location=loc)
+        err = rel_err(b, ob)
+        print(f"[rhs {name} fp64={fp64} loc={loc}] rel err {err:.2e}", flush=True)
+        assert err < 1e-12, err
+    e.close()
+
+
+def test_float_abi_solve_matches_oracle():
+    """libsle.so (SLE_Real = float, the reference's default ABI) driven from
+    Python: a flat-start PRECISION solve through sle_Solve, the estimate read
+    through the float getters, against the oracle on the same float-rounded
+    inputs."""
+    from state_estimation_with_cuda_amd import sle
+
+    g = GRIDS["geo2k"]()
+    f32 = lambda a: np.asarray(a, np.float32).astype(np.float64)
+    g32 = gg.Grid(**{**g.__dict__})
+    for f in ("vset", "r", "x", "b", "tap", "shift", "sigma", "z"):
+        setattr(g32, f, f32(getattr(g, f)))
+    e = sle.Engine(double=False)
+    e.load_grid(g)  # records carry float values: the engine sees g32
+    e.set_config(use_double_precision=0, convergence_tolerance=1e-7, max_precision_iterations=50)
+    e.upload()
+    e.set_telemetry(g.z)
+    res = e.S["Result"]()
+    rc = e.L.sle_Solve(e.h, 1, C.byref(res))
+    assert rc == 0 and res.status == 0, (rc, res.status)
+    V = np.zeros(g.n, np.float32)
+    T = np.zeros(g.n, np.float32)
+    assert e.L.sle_GetVoltageMagnitudes(e.h, V.ctypes.data_as(C.c_void_p), g.n) == 0
+    assert e.L.sle_GetVoltageAngles(e.h, T.ctypes.data_as(C.c_void_p), g.n) == 0
+    o = Oracle(g32)
+    assert o.solve(mode=1, max_iter=50, tol=1e-11, flat_start=True)["status"] == 0
+    oV, oth = o.state()
+    dV, dth = np.abs(V - oV).max(), np.abs(T - oth).max()
+    print(f"[float abi] iters={res.iterations} dV={dV:.2e} dth={dth:.2e}", flush=True)
+    assert dV < STATE_TOL and dth < STATE_TOL
+    e.close()

@@ -454,3 +454,36 @@ def test_largest_normalized_residual_finds_a_gross_error():
     assert o.solve(mode=1, max_iter=50, tol=1e-12, flat_start=True)["status"] == 0
     rn, _ = o.normalized_residuals()
     assert rn.max() < 4.5
+
+
+@pytest.mark.parametrize("maker", [lambda: gg.grid14(0), variant14])
+def test_rhs_is_half_the_negative_objective_gradient(maker):
+    """b = H^T W r equals -1/2 dJ/dx for J = sum w (z - h(x))^2, by central
+    differences of the oracle's h(x) (state layout: angles, then magnitudes)."""
+    g = maker()
+    o = Oracle(g)
+    rng = np.random.default_rng(6)
+    V = g.V_true + rng.normal(0, 0.02, g.n)
+    th = g.th_true + rng.normal(0, 0.02, g.n)
+    th[g.slack] = 0
+    b = o.rhs(V, th)
+    w = 1.0 / g.sigma ** 2
+
+    def J(Vx, tx):
+        r = g.z - o.eval(Vx, tx)["h"]
+        return float(np.sum(w * r * r))
+
+    n, s, step = g.n, g.slack, 1e-6
+    fd = np.zeros(o.ns)
+    for j in range(o.ns):
+        Vp, Vm, tp, tm = V.copy(), V.copy(), th.copy(), th.copy()
+        if j < n - 1:
+            bus = j if j < s else j + 1
+            tp[bus] += step
+            tm[bus] -= step
+        else:
+            bus = j - (n - 1)
+            Vp[bus] += step
+            Vm[bus] -= step
+        fd[j] = -0.5 * (J(Vp, tp) - J(Vm, tm)) / (2 * step)
+    assert np.max(np.abs(b - fd)) < 1e-5 * max(1.0, np.abs(b).max())
