@@ -222,7 +222,7 @@ typedef struct SLE_Config {
     int32_t use_cuda_graphs; /* replay iterations on the cached factor as a HIP graph */
     int32_t block_size_standard;
     int32_t block_size_reduction;
-    int32_t use_double_precision; /* 1: plain FP64 form; 0: mixed FP32 factor + FP64 refinement */
+    int32_t use_double_precision; /* 1: plain FP64 form; 0: mixed (FP64 factor, FP32 solves) */
     SLE_Real voltage_min_pu;
     SLE_Real voltage_max_pu;
     SLE_Real bad_data_threshold;

@@ -1,11 +1,15 @@
 // device.hip -- gfx950 kernels and the device-resident Gauss-Newton loop.
 //
 // Precision (the mixed form): h(x), residuals, Huber weights, the right-hand
-// side H^T W r, the state and the convergence test are FP64; the scaled
-// Jacobian sqrt(W) H, the gain entries and the factor are FP32 (T = float).
-// The plain form instantiates the same kernels with T = double.  A pivot that
-// falls below threshold in the FP32 factor switches the instance to the FP64
-// factor (pivot-triggered refactor).
+// side H^T W r, the state and the convergence test are FP64; the gain is
+// formed and factored in FP64 on the iterations that refactor, and the
+// factor's solve data (panels, subtree solve matrices, top inverse blocks) are
+// rounded once to FP32 for the iterations on the cached factor (T = float in
+// the solve kernels), which then read half the bytes.  The Gauss-Newton loop
+// corrects the rounding like any other inexact step.  The plain form
+// instantiates the same kernels with T = double.  Option: an FP32
+// factorization instead (mixed64 off), whose pivots below threshold switch the
+// instance to the FP64 factor.
 //
 // Kernel map (reference counterparts, replaced rather than translated):
 //   k_ybus        fillYbusKernel / initYbusDiagonalKernel, sparse_matrix.cu:389-492
@@ -1958,6 +1962,146 @@ __global__ __launch_bounds__(kBlock) void k_sub_bwd(SubArgs B, SolveArgs<T> A, c
     if (lane < n) A.x[cols[lane]] = (T)acc;
 }
 
+// Loads through a buffer resource at a wave-uniform byte offset (an SGPR) plus
+// a per-lane one, so that a batch of loads shares a single offset register.
+// gfx9 descriptor word 3 (32-bit data format); no range limit is used: callers
+// keep every access inside the allocation.
+__device__ inline __amdgpu_buffer_rsrc_t buf_rsrc(const void* p) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)0xffffffff, 0x00020000);
+}
+template <typename T>
+__device__ inline T buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff);
+template <>
+__device__ inline float buf_ld<float>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+template <>
+__device__ inline double buf_ld<double>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+
+// The same products with one workgroup per subtree, for the middle level: few
+// subtrees (about 1.6 per CU) and most of them near the 64 x 128 limit, so the
+// one-wave kernels above spend up to eight dependent batches per subtree with
+// little else to hide them.  Here each of the NW waves takes a share of the
+// work -- forward the columns w, w + NW, ..., backward the rows w, w + NW, ...
+// -- and issues all of its loads at once, ahead of the right-hand-side gather;
+// the partial sums then meet in LDS.  The chain per subtree is the gather
+// (index, then value) and one reduction.
+template <int NW>
+constexpr bool sub_split_ok() {
+    return NW * kWave >= kSubRows && kSubCols % NW == 0 && kSubRows % NW == 0 && kSubCols == kWave;
+}
+
+template <typename T, int NW>
+__global__ __launch_bounds__(NW* kWave) void k_sub_fwd_wg(SubArgs B, SolveArgs<T> A, const T* __restrict__ M,
+                                                          int t0) {
+    static_assert(sub_split_ok<NW>(), "subtree split");
+    constexpr int QW = kSubCols / NW;  // columns per wave
+    __shared__ double bs[kSubCols];
+    __shared__ double part[NW][kSubRows];
+    const int t = t0 + blockIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int n = B.n[t], R = B.R[t];
+    const int32_t* cols = B.cols + B.cptr[t];
+    const T* Mt = M + B.moff[t];
+    const int cl = threadIdx.x < n ? cols[threadIdx.x] : 0;  // issued first: the gather waits for it alone
+    const int64_t uo = B.uoff[t];
+    // Column j of the packed column-major M_t starts at j R - j (j - 1) / 2 and
+    // holds rows j..R-1, so row i sits at (that start - j) + i: a scalar base per
+    // column and the lane as the offset.  Columns past n reread column n - 1;
+    // rows outside column j read its neighbours (inside this subtree's copies,
+    // or the padding after them) and are discarded in the product.
+    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(Mt);
+    auto base = [&](int q) {
+        const int j = min(w + NW * q, n - 1);
+        return (uint32_t)((j * R - j * (j - 1) / 2 - j) * (int)sizeof(T));
+    };
+    T m0[QW], m1[QW];
+#pragma unroll
+    for (int q = 0; q < QW; ++q) m0[q] = buf_ld<T>(rs, lane * sizeof(T), base(q));
+    if (R > kWave) {
+#pragma unroll
+        for (int q = 0; q < QW; ++q) m1[q] = buf_ld<T>(rs, (lane + kWave) * sizeof(T), base(q));
+    }
+    if (threadIdx.x < n) {  // the bottom roots' update rows add into the right-hand side first
+        double b = (double)A.x[cl];
+        const int c = B.cptr[t] + threadIdx.x;
+        for (int q = B.acptr[c]; q < B.acptr[c + 1]; ++q) b += (double)A.ubuf[B.acpos[q]];
+        bs[threadIdx.x] = b;
+    }
+    __syncthreads();
+    auto ok = [&](int j, int i) { return j < n && i >= j && i < R; };
+    double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+    for (int q = 0; q < QW; ++q) {
+        const int j = w + NW * q;
+        const double b = j < n ? bs[j] : 0.0;
+        a0 += (ok(j, lane) ? (double)m0[q] : 0.0) * b;
+        if (R > kWave) a1 += (ok(j, lane + kWave) ? (double)m1[q] : 0.0) * b;
+    }
+    part[w][lane] = a0;
+    part[w][lane + kWave] = a1;
+    __syncthreads();
+    const int i = threadIdx.x;
+    if (i < R) {
+        double y = 0.0;
+#pragma unroll
+        for (int v = 0; v < NW; ++v) y += part[v][i];
+        if (i < n)
+            A.x[cl] = (T)y;
+        else
+            A.ubuf[uo + i - n] = (T)y;
+    }
+}
+
+template <typename T, int NW>
+__global__ __launch_bounds__(NW* kWave) void k_sub_bwd_wg(SubArgs B, SolveArgs<T> A, const T* __restrict__ Mrow,
+                                                          int t0) {
+    static_assert(sub_split_ok<NW>(), "subtree split");
+    constexpr int UW = kSubRows / NW;  // rows per wave
+    __shared__ double v[kSubRows];
+    __shared__ double part[NW][kSubCols];
+    const int t = t0 + blockIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int n = B.n[t], R = B.R[t];
+    const int32_t* cols = B.cols + B.cptr[t];
+    const int32_t* urow = A.rows + B.urow[t];
+    const T* Mt = Mrow + B.moff[t];
+    const int r = threadIdx.x;
+    const int xr = r < R ? (r < n ? cols[r] : urow[r - n]) : 0;  // issued first: the gather waits for it alone
+    // row i of the row-major copy starts at sub_row_off(i, n) and holds columns
+    // 0..min(i, n - 1): a scalar base per row and the lane as the offset.  Rows
+    // past R reread row R - 1; lanes past the row read the next one (or the
+    // padding after the copies) and are discarded in the product.
+    const __amdgpu_buffer_rsrc_t rs = buf_rsrc(Mt);
+    T mv[UW];
+#pragma unroll
+    for (int u = 0; u < UW; ++u) {
+        const int i = min(w + NW * u, R - 1);
+        const int ro = i <= n ? i * (i + 1) / 2 : n * (n + 1) / 2 + (i - n) * n;
+        mv[u] = buf_ld<T>(rs, lane * sizeof(T), (uint32_t)(ro * (int)sizeof(T)));
+    }
+    if (r < R) v[r] = (double)A.x[xr];
+    __syncthreads();
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < UW; ++u) {
+        const int i = w + NW * u;
+        acc += (i < R && lane <= i && lane < n ? (double)mv[u] : 0.0) * (i < R ? v[i] : 0.0);
+    }
+    part[w][lane] = acc;
+    __syncthreads();
+    if (r < n) {
+        double x = 0.0;
+#pragma unroll
+        for (int q = 0; q < NW; ++q) x += part[q][r];
+        A.x[xr] = (T)x;  // xr = cols[r] for r < n
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Dependency-driven sweeps: one launch covers the whole supernodal tree.
 // Block b handles front order[b]; `order` lists every child before its parent
@@ -2479,7 +2623,9 @@ struct PrecBufs {
         pinv_rows.alloc(std::max<int64_t>(1, pz.nq), bytes);
         ctop.alloc(std::max(1, pz.ntc), bytes);
         ytop.alloc(std::max(1, pz.ntc), bytes);
-        msub.alloc(std::max<int64_t>(1, 2 * pz.msub), bytes);  // column-major, then row-major
+        // column-major, then row-major; the workgroup products read up to a
+        // row's width (kSubCols) past the last row of the last subtree
+        msub.alloc(2 * pz.msub + kSubCols, bytes);
     }
     void release(Ledger& bytes) {
         for (DBuf<T>* b : {&Gval, &F, &x, &ubuf, &M, &pinv, &pinv_rows, &ctop, &ytop, &msub}) b->release(bytes);
@@ -3161,6 +3307,9 @@ struct Instance::Impl {
     bool use_pi = true;
     bool pi_dense = true;   // bottom subtrees as dense solve matrices (else LDS tasks)
     bool pi_mid = true;     // a second level of dense subtrees between the bottom ones and the top
+    // middle subtree products: waves per subtree (0: one wave each, k_sub_fwd /
+    // k_sub_bwd; 4, 8, 16: one workgroup each; -1: by precision), per direction
+    int sub_wg_fwd = -1, sub_wg_bwd = -1;
     bool meas_by_type = true;  // measurement rows grouped by evaluation class (else by anchor bus only)
     // right-hand side by location (needs the class grouping)
     bool rhs_loc = true;  // asked for
@@ -4403,6 +4552,29 @@ void launch_bwd_lds(const Instance::Impl& D, int grid, int waves, size_t bytes, 
         k_bwd_lds<T, false><<<grid, kWave * waves, bytes, D.stream>>>(LA, A, base);
 }
 
+// Middle-level products, nw waves per subtree (4, 8 or 16)
+template <typename T>
+void launch_sub_fwd_wg(const Instance::Impl& D, int nw, int nsub, const SubArgs& B, const SolveArgs<T>& A,
+                       const T* M, int t0) {
+    if (nw == 4)
+        k_sub_fwd_wg<T, 4><<<nsub, 4 * kWave, 0, D.stream>>>(B, A, M, t0);
+    else if (nw == 8)
+        k_sub_fwd_wg<T, 8><<<nsub, 8 * kWave, 0, D.stream>>>(B, A, M, t0);
+    else
+        k_sub_fwd_wg<T, 16><<<nsub, 16 * kWave, 0, D.stream>>>(B, A, M, t0);
+}
+
+template <typename T>
+void launch_sub_bwd_wg(const Instance::Impl& D, int nw, int nsub, const SubArgs& B, const SolveArgs<T>& A,
+                       const T* M, int t0) {
+    if (nw == 4)
+        k_sub_bwd_wg<T, 4><<<nsub, 4 * kWave, 0, D.stream>>>(B, A, M, t0);
+    else if (nw == 8)
+        k_sub_bwd_wg<T, 8><<<nsub, 8 * kWave, 0, D.stream>>>(B, A, M, t0);
+    else
+        k_sub_bwd_wg<T, 16><<<nsub, 16 * kWave, 0, D.stream>>>(B, A, M, t0);
+}
+
 template <typename T>
 void launch_trisolve(Instance::Impl& D, Prof* pf = nullptr) {
     SolveArgs<T> A = solve_args<T>(D);
@@ -4425,7 +4597,13 @@ void launch_trisolve(Instance::Impl& D, Prof* pf = nullptr) {
             }
             if (n1) {
                 prof_begin(pf, PH_MID_FWD);
-                k_sub_fwd<T, 1><<<(n1 + nw - 1) / nw, kBlock, 0, D.stream>>>(sub_args(D), A, B.msub.p, n0, PI.nsub);
+                // by precision (measured at 200k buses): four waves per subtree
+                const int wg = D.sub_wg_fwd >= 0 ? D.sub_wg_fwd : 4;
+                if (wg)
+                    launch_sub_fwd_wg<T>(D, wg, n1, sub_args(D), A, B.msub.p, n0);
+                else
+                    k_sub_fwd<T, 1><<<(n1 + nw - 1) / nw, kBlock, 0, D.stream>>>(sub_args(D), A, B.msub.p, n0,
+                                                                                 PI.nsub);
                 prof_end(pf, PH_MID_FWD, 1);
             }
         } else if (PB.n_bottom) {
@@ -4452,16 +4630,22 @@ void launch_trisolve(Instance::Impl& D, Prof* pf = nullptr) {
         if (dense) {
             // middle subtrees (their update rows are top columns), then the bottom ones
             const int n0 = PI.nsub0, n1 = PI.nsub - PI.nsub0;
+            const T* Mrow = B.msub.p + PI.msub;
             if (n1) {
                 prof_begin(pf, PH_MID_BWD);
-                k_sub_bwd<T, 1><<<(n1 + nw - 1) / nw, kBlock, 0, D.stream>>>(sub_args(D), A, B.msub.p + PI.msub, n0,
-                                                                             PI.nsub);
+                // by precision (measured at 200k buses): eight waves per subtree
+                // for FP32 data, one wave for FP64 (whose 64-wide row loads of the
+                // split kernel cost more than its shorter chain saves)
+                const int wg = D.sub_wg_bwd >= 0 ? D.sub_wg_bwd : (sizeof(T) == 4 ? 8 : 0);
+                if (wg)
+                    launch_sub_bwd_wg<T>(D, wg, n1, sub_args(D), A, Mrow, n0);
+                else
+                    k_sub_bwd<T, 1><<<(n1 + nw - 1) / nw, kBlock, 0, D.stream>>>(sub_args(D), A, Mrow, n0, PI.nsub);
                 prof_end(pf, PH_MID_BWD, 1);
             }
             if (n0) {
                 prof_begin(pf, PH_SUB_BWD);
-                k_sub_bwd<T, 0><<<(n0 + nw - 1) / nw, kBlock, 0, D.stream>>>(sub_args(D), A, B.msub.p + PI.msub, 0,
-                                                                             n0);
+                k_sub_bwd<T, 0><<<(n0 + nw - 1) / nw, kBlock, 0, D.stream>>>(sub_args(D), A, Mrow, 0, n0);
                 prof_end(pf, PH_SUB_BWD, 1);
             }
         }
@@ -5522,6 +5706,14 @@ void Instance::set_dense_subtrees(bool on) {
 void Instance::set_middle_subtrees(bool on) {
     ++d_->plan_epoch;
     d_->pi_mid = on;
+}
+
+void Instance::set_subtree_workgroups(int fwd, int bwd) {
+    auto ok = [](int w) { return w == -1 || w == 0 || w == 4 || w == 8 || w == 16; };
+    if (!ok(fwd) || !ok(bwd)) throw std::invalid_argument("waves per middle subtree: -1, 0, 4, 8 or 16");
+    ++d_->plan_epoch;
+    d_->sub_wg_fwd = fwd;
+    d_->sub_wg_bwd = bwd;
 }
 
 void Instance::solve_plan_sizes(int64_t* out, int count) const {

@@ -185,6 +185,10 @@ int Engine::set_option(const char* key, double value) {
     } else if (k == "pinv_mid") {
         opt_.pinv_mid = value != 0;
         model_dirty_ = true;  // the plan is rebuilt at the next upload
+    } else if (k == "pinv_wg_fwd" || k == "pinv_wg_bwd") {
+        if (value != -1 && value != 0 && value != 4 && value != 8 && value != 16) return false;
+        (k == "pinv_wg_fwd" ? opt_.pinv_wg_fwd : opt_.pinv_wg_bwd) = (int)value;
+        if (inst_) inst_->set_subtree_workgroups(opt_.pinv_wg_fwd, opt_.pinv_wg_bwd);
     } else if (k == "pinv_dense") {
         opt_.pinv_dense = value != 0;
         model_dirty_ = true;  // the plan is rebuilt at the next upload
@@ -698,6 +702,7 @@ int Engine::rebuild() {
     inst_->set_lds_flags(opt_.lds_flags);
     inst_->set_dense_subtrees(opt_.pinv_dense);
     inst_->set_middle_subtrees(opt_.pinv_mid);
+    inst_->set_subtree_workgroups(opt_.pinv_wg_fwd, opt_.pinv_wg_bwd);
     inst_->set_meas_by_type(opt_.meas_by_type);
     inst_->set_rhs_by_location(opt_.rhs_by_location);
     inst_->set_mixed_fp64_factor(opt_.mixed_fp64_factor);

@@ -905,7 +905,7 @@ SLE_API const char* SLE_CALL sle_ext_Describe(SLE_Handle handle) {
     if (!handle) return nullptr;
     try {
         sle::Engine& e = E(handle);
-        std::string d = e.config().use_double_precision ? "plain FP64" : "mixed FP32 factor + FP64 refinement";
+        std::string d = e.config().use_double_precision ? "plain FP64" : "mixed: FP64 factor, FP32 solve data, FP64 residuals and state";
         d += "; pcg_steps=" + std::to_string(e.config().use_double_precision ? 0 : e.options().pcg_steps);
         if (e.instance()) d += "; sweeps=" + e.instance()->sweep_plan();
         if (!e.autotune_report().empty()) d += "; autotune: " + e.autotune_report();
