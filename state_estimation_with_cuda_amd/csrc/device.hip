@@ -1479,7 +1479,7 @@ __global__ __launch_bounds__(1024) void k_bwd_lds(LdsTaskArgs A, SolveArgs<T> SA
 // -- become two independent gathers.
 // ---------------------------------------------------------------------------
 struct PiArgs {
-    const int32_t* sn;      // top supernodes, ascending (children before parents)
+    const int32_t* sn;      // top supernodes in a postorder of the top forest
     const int32_t* plen;    // P_s, path rows of each top supernode
     const int64_t* boff;    // offset of each supernode's block
     const int64_t* poff;    // offset of each supernode's path rows in ploc
@@ -1578,31 +1578,33 @@ __global__ __launch_bounds__(kBlock) void k_pi_assemble(int ntc, const int32_t* 
 constexpr int kPiSegMax = 256;
 
 // Forward product y_T = L_TT^-1 c_T: one wave per segment of a row's entries,
-// all loads of a lane issued together.  A row held by one segment
-// (most rows) is written directly; longer rows leave partial sums for
-// k_pi_rowsum.
+// all loads of a lane issued together.  The entries of a row are one
+// contiguous range of c_T (the top fronts are numbered in postorder), so entry
+// q multiplies c_T[segc + q]: the matrix values and c_T stream side by side,
+// with no column index per entry.  A row held by one segment (most rows) is
+// written directly; longer rows leave partial sums for k_pi_rowsum.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pi_fwd(int nseg, const int64_t* __restrict__ segq,
-                                                  const int32_t* __restrict__ segrow,
-                                                  const int32_t* __restrict__ segptr, const int32_t* __restrict__ src,
-                                                  const T* __restrict__ val, const T* __restrict__ ctop,
-                                                  double* __restrict__ part, T* __restrict__ ytop) {
+                                                  const int64_t* __restrict__ segc, const int32_t* __restrict__ segrow,
+                                                  const int32_t* __restrict__ segptr, const T* __restrict__ val,
+                                                  const T* __restrict__ ctop, double* __restrict__ part,
+                                                  T* __restrict__ ytop) {
     const int g = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
     const int lane = threadIdx.x & (kWave - 1);
     if (g >= nseg) return;
     constexpr int kU = kPiSegMax / kWave;
     const int64_t q0 = segq[g] + lane, q1 = segq[g + 1];
-    double v[kU];
-    int c[kU];
+    const T* cq = ctop + segc[g];  // c_T aligned with the entries
+    T v[kU], c[kU];
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
         const int64_t q = q0 + (int64_t)u * kWave;
-        v[u] = q < q1 ? (double)val[q] : 0.0;
-        c[u] = q < q1 ? src[q] : -1;
+        v[u] = q < q1 ? val[q] : T(0);
+        c[u] = q < q1 ? cq[q] : T(0);
     }
     double acc = 0.0;
 #pragma unroll
-    for (int u = 0; u < kU; ++u) acc += c[u] >= 0 ? v[u] * (double)ctop[c[u]] : 0.0;
+    for (int u = 0; u < kU; ++u) acc += (double)v[u] * (double)c[u];
     acc = wave_sum(acc);
     if (lane == 0) {
         const int t = segrow[g];
@@ -2903,7 +2905,7 @@ struct PiPlan {
     int ntop = 0, ntc = 0, nseg = 0, slice = 0;
     int64_t nblock = 0, nq = 0;
     std::vector<int32_t> sn, plen, ploc, tcol, colsn, src, segptr, cptr, parent, segrow, mrows, cplen, cjj;
-    std::vector<int64_t> boff, poff, pos, segq, cpos, cboff, cpoff;
+    std::vector<int64_t> boff, poff, pos, segq, segc, cpos, cboff, cpoff;
     // dense bottom subtrees (see SubArgs)
     bool dense = false;
     int nsub = 0, nsub0 = 0;  // subtrees; of them bottom ones (listed first)
@@ -3066,9 +3068,42 @@ static PiPlan make_pi_plan(const Symbolic& S, int cut, bool dense, bool mid) {
                 mloc[S.sn_first[s] + j] = (int32_t)P.mcol.size();
                 P.mcol.push_back(S.sn_first[s] + j);
             }
+    // the top fronts in a postorder of the top forest (children in ascending
+    // order): the descendants of a front, and so their columns, are then one
+    // contiguous range just before its own -- every row of L_TT^-1 reads one
+    // contiguous range of c_T in the forward product
+    std::vector<int32_t> top_order;
+    {
+        std::vector<int32_t> cptr_t(nsn + 1, 0), ch_t, roots, fill_t;
+        for (int s = 0; s < nsn; ++s)
+            if (lev[s] == 2 && S.sn_parent[s] >= 0) ++cptr_t[S.sn_parent[s] + 1];
+        for (int s = 0; s < nsn; ++s) cptr_t[s + 1] += cptr_t[s];
+        ch_t.resize(cptr_t[nsn]);
+        fill_t.assign(cptr_t.begin(), cptr_t.end() - 1);
+        for (int s = 0; s < nsn; ++s) {
+            if (lev[s] != 2) continue;
+            if (S.sn_parent[s] >= 0)
+                ch_t[fill_t[S.sn_parent[s]]++] = s;  // ascending within each parent
+            else
+                roots.push_back(s);
+        }
+        std::vector<std::pair<int32_t, int32_t>> st;  // (front, next child)
+        for (int r : roots) {
+            st.emplace_back(r, cptr_t[r]);
+            while (!st.empty()) {
+                auto& f = st.back();
+                if (f.second < cptr_t[f.first + 1]) {
+                    const int c = ch_t[f.second++];
+                    st.emplace_back(c, cptr_t[c]);
+                } else {
+                    top_order.push_back(f.first);
+                    st.pop_back();
+                }
+            }
+        }
+    }
     int cmax = 1, kmax = 1;
-    for (int s = 0; s < nsn; ++s) {
-        if (lev[s] != 2) continue;
+    for (int s : top_order) {
         tsl[s] = (int32_t)P.sn.size();
         P.sn.push_back(s);
         for (int j = 0; j < S.sn_k[s]; ++j) {
@@ -3125,17 +3160,26 @@ static PiPlan make_pi_plan(const Symbolic& S, int cut, bool dense, bool mid) {
             }
         }
     }
+    // each row's entries cover one contiguous ascending range of c_T (the
+    // postorder above): a segment needs only the column of its first entry
+    for (int t = 0; t < P.ntc; ++t) {
+        bool ok = rcnt[t + 1] > rcnt[t] && P.src[rcnt[t + 1] - 1] == t;  // ends at its diagonal
+        for (int64_t q = rcnt[t]; ok && q < rcnt[t + 1]; ++q) ok = P.src[q] == P.src[rcnt[t]] + (q - rcnt[t]);
+        if (!ok) throw std::logic_error("partitioned inverse: a row of L_TT^-1 is not one contiguous column range");
+    }
     P.segptr.assign(P.ntc + 1, 0);
     for (int t = 0; t < P.ntc; ++t) {
         for (int64_t q = rcnt[t]; q < rcnt[t + 1]; q += kPiSeg) {
             P.segq.push_back(q);
             P.segrow.push_back(t);
+            P.segc.push_back(P.src[q] - q);
         }
         P.segptr[t + 1] = (int32_t)P.segq.size();
         if (P.segptr[t + 1] - P.segptr[t] != 1) P.mrows.push_back(t);  // several segments (or none)
     }
     P.nseg = (int)P.segq.size();
     P.segq.push_back(P.nq);
+    std::vector<int32_t>().swap(P.src);  // carried by segc
     // assembly: the update rows of every subtree root (bottom or middle), per
     // middle column and per top column
     std::vector<std::vector<int64_t>> per(P.ntc), perm(P.mcol.size());
@@ -3276,9 +3320,9 @@ struct Instance::Impl {
         bool ok = false;
         int cut = 0, ntop = 0, ntc = 0, nseg = 0, slice = 0;
         int64_t nblock = 0, nq = 0;
-        DBuf<int32_t> sn, plen, ploc, tcol, colsn, src, segptr, cptr, parent, segrow, mrows, cplen, cjj;
+        DBuf<int32_t> sn, plen, ploc, tcol, colsn, segptr, cptr, parent, segrow, mrows, cplen, cjj;
         int nmrows = 0;
-        DBuf<int64_t> boff, poff, pos, segq, cpos, cboff, cpoff;
+        DBuf<int64_t> boff, poff, pos, segq, segc, cpos, cboff, cpoff;
         int64_t usum = 0, npath = 0;  // sum of subtree root update rows; path rows of the top
         int64_t msub0 = 0, usum0 = 0, ncol0 = 0;  // of the bottom subtrees: entries, update rows, columns
         DBuf<double> part;
@@ -3428,7 +3472,6 @@ static void build_pi(Instance::Impl& D, const Symbolic& S, Ledger& bytes, hipStr
             up(Q.ploc, PP.ploc);
             up(Q.tcol, PP.tcol);
             up(Q.colsn, PP.colsn);
-            up(Q.src, PP.src);
             up(Q.segptr, PP.segptr);
             up(Q.segrow, PP.segrow);
             up(Q.mrows, PP.mrows);
@@ -3453,6 +3496,7 @@ static void build_pi(Instance::Impl& D, const Symbolic& S, Ledger& bytes, hipStr
             up64(Q.poff, PP.poff);
             up64(Q.pos, PP.pos);
             up64(Q.segq, PP.segq);
+            up64(Q.segc, PP.segc);
             up64(Q.cpos, PP.cpos);
             Q.part.alloc(std::max(1, PP.nseg), bytes);
             Q.dense = dense;
@@ -3552,9 +3596,9 @@ Instance::~Instance() {
     }
     {
         auto& P = d_->pi;
-        rel(P.sn); rel(P.plen); rel(P.ploc); rel(P.tcol); rel(P.colsn); rel(P.src); rel(P.segptr); rel(P.cptr); rel(P.parent);
+        rel(P.sn); rel(P.plen); rel(P.ploc); rel(P.tcol); rel(P.colsn); rel(P.segptr); rel(P.cptr); rel(P.parent);
         rel(P.segrow); rel(P.mrows); rel(P.cplen); rel(P.cjj); rel(P.cboff); rel(P.cpoff);
-        rel(P.boff); rel(P.poff); rel(P.pos); rel(P.segq); rel(P.cpos); rel(P.part);
+        rel(P.boff); rel(P.poff); rel(P.pos); rel(P.segq); rel(P.segc); rel(P.cpos); rel(P.part);
         rel(P.sroot); rel(P.sn_n); rel(P.sR); rel(P.fptr); rel(P.fsn); rel(P.fdcol); rel(P.drow); rel(P.scptr);
         rel(P.scols); rel(P.bins); rel(P.fdoff); rel(P.smoff); rel(P.suoff); rel(P.surow); rel(P.acptr); rel(P.acpos);
         for (auto& L : P.bottom) {
@@ -4614,7 +4658,7 @@ void launch_trisolve(Instance::Impl& D, Prof* pf = nullptr) {
                                                                    A.ubuf, B.ctop.p);
             prof_begin(pf, PH_PI_FWD);
             k_pi_fwd<T><<<(PI.nseg + nw - 1) / nw, kBlock, 0, D.stream>>>(
-                PI.nseg, PI.segq.p, PI.segrow.p, PI.segptr.p, PI.src.p, B.pinv_rows.p, B.ctop.p, PI.part.p, B.ytop.p);
+                PI.nseg, PI.segq.p, PI.segc.p, PI.segrow.p, PI.segptr.p, B.pinv_rows.p, B.ctop.p, PI.part.p, B.ytop.p);
             prof_end(pf, PH_PI_FWD, 1);
             if (PI.nmrows)
                 k_pi_rowsum<T><<<(PI.nmrows + nw - 1) / nw, kBlock, 0, D.stream>>>(PI.nmrows, PI.mrows.p,
