@@ -121,15 +121,15 @@ def algorithmic_bytes(sz, g, prec_bytes, direct=True, plan=None):
         "update": ns * s + n * (8 * 4 + 8),
     }
     if plan.get("rhs_by_location") and direct:
-        # k_meas_loc (with the injections): Ybus rows, per-entry terms written;
-        # bus V/th read, P/Q written; per row code, z, sigma; row ranges; branch
-        # data; per-bus and per-branch sums written
-        out["meas"] = (n + 1) * 4 + nnz_y * (4 + 8 + 8 + 8 + 8) + n * 8 * 4 + m * (1 + 8 + 8) \
+        # k_meas_loc (with the injections): Ybus rows; bus V/th read, P/Q
+        # written; per row code, z, sigma; row ranges; branch data; per-bus and
+        # per-branch sums written
+        out["meas"] = (n + 1) * 4 + nnz_y * (4 + 8 + 8) + n * 8 * 4 + m * (1 + 8 + 8) \
             + 2 * (n + 1) * 4 + (2 * nb + 1) * 4 + nb * (4 + 4 + 8 * 5) + 32 * n + 32 * nb
         out["power"] = 0  # fused into the measurement pass on this path
-        # k_rhs_loc: Ybus rows with the transpose map and per-entry terms, bus
-        # V/P/Q and sums, incident branches and their sums, state maps, b
-        out["rhs"] = (n + 1) * 4 + nnz_y * (4 + 4 + 8 + 8) + n * 8 * 3 + 32 * n + (n + 1) * 4 \
+        # k_rhs_loc: Ybus rows with the transposed admittances, bus V/th/P/Q
+        # and sums, incident branches and their sums, state maps, b
+        out["rhs"] = (n + 1) * 4 + nnz_y * (4 + 8 + 8) + n * 8 * 4 + 32 * n + (n + 1) * 4 \
             + 2 * nb * 4 + 32 * nb + 8 * n + ns * s
     if plan.get("subtrees"):
         def sub(ent, us, nsub, nc):
@@ -149,8 +149,9 @@ def algorithmic_bytes(sz, g, prec_bytes, direct=True, plan=None):
             out["mid_bwd"] = b1
     if plan.get("top_columns"):
         ntc, nq, nseg = plan["top_columns"], plan["top_row_entries"], plan["segments"]
-        # row-grouped L_TT^-1 entries and their column indices, c_T once, y_T out, segments
-        out["pi_fwd"] = (s + 4) * nq + 2 * s * ntc + 28 * nseg
+        # row-grouped L_TT^-1 entries (each row one contiguous range of c_T), c_T
+        # once, y_T out, segment headers
+        out["pi_fwd"] = s * nq + 2 * s * ntc + 36 * nseg
         # block entries, path rows once, y_T once, per-column metadata, x_T out
         out["pi_bwd"] = s * plan["top_block_entries"] + 4 * plan["top_path_rows"] + 2 * s * ntc + 24 * ntc
     return out

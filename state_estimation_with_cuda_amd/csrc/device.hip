@@ -169,6 +169,18 @@ __global__ __launch_bounds__(kBlock) void k_ybus(int nY, const int32_t* __restri
     yB[e] = gi;
 }
 
+// Ybus values at the transposed positions: entry e of row j (column l) gets
+// Y_lj, read by k_rhs_loc in its row walk; refreshed after every k_ybus.
+__global__ __launch_bounds__(kBlock) void k_ybus_tr(int nY, const int32_t* __restrict__ ytr,
+                                                   const double* __restrict__ yG, const double* __restrict__ yB,
+                                                   double* __restrict__ yGt, double* __restrict__ yBt) {
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= nY) return;
+    const int f = ytr[e];
+    yGt[e] = yG[f];
+    yBt[e] = yB[f];
+}
+
 // ---------------------------------------------------------------------------
 // Bus injections; per Ybus entry P_ij, Q_ij terms kept for the Jacobian
 // ---------------------------------------------------------------------------
@@ -508,8 +520,6 @@ struct LocArgs {
     const double* yB;
     double* Pout;
     double* Qout;
-    double* Pij;
-    double* Qij;
     DevStatus* st;
 };
 
@@ -567,7 +577,7 @@ __global__ __launch_bounds__(kBlock) void k_meas_loc(LocArgs A) {
     }
     if (t < A.n) {
         const int b = t;
-        // injections of bus b and the per-entry terms of its Ybus row
+        // injections of bus b over its Ybus row
         const double vi = A.V[b], ti = A.th[b];
         double pb = 0.0, qb = 0.0;
         const int e1 = A.yrow[b + 1];  // every row holds its diagonal entry
@@ -592,11 +602,8 @@ __global__ __launch_bounds__(kBlock) void k_meas_loc(LocArgs A) {
                 double sn, cs;
                 sincos(ti - tq[0], &sn, &cs);
                 const double vv = vi * Vq[0];
-                const double pe = vv * (G[0] * cs + Bq[0] * sn), qe = vv * (G[0] * sn - Bq[0] * cs);
-                A.Pij[e + q] = pe;
-                A.Qij[e + q] = qe;
-                pb += pe;
-                qb += qe;
+                pb += vv * (G[0] * cs + Bq[0] * sn);
+                qb += vv * (G[0] * sn - Bq[0] * cs);
 #pragma unroll
                 for (int k = 0; k + 1 < kLocChunk; ++k) {
                     G[k] = G[k + 1];
@@ -700,46 +707,66 @@ __global__ __launch_bounds__(kBlock) void k_meas_loc(LocArgs A) {
 
 template <typename TO>
 __global__ __launch_bounds__(kBlock) void k_rhs_loc(int n, const int32_t* __restrict__ yrow,
-                                                    const int32_t* __restrict__ ycol, const int32_t* __restrict__ ytr,
-                                                    const double* __restrict__ Pij, const double* __restrict__ Qij,
-                                                    const double* __restrict__ P, const double* __restrict__ Q,
-                                                    const double* __restrict__ V, const double* __restrict__ sbus,
+                                                    const int32_t* __restrict__ ycol, const double* __restrict__ yGt,
+                                                    const double* __restrict__ yBt, const double* __restrict__ P,
+                                                    const double* __restrict__ Q, const double* __restrict__ V,
+                                                    const double* __restrict__ th, const double* __restrict__ sbus,
                                                     const double* __restrict__ sbr, const int32_t* __restrict__ inc_ptr,
                                                     const int32_t* __restrict__ inc, const int32_t* __restrict__ pth,
                                                     const int32_t* __restrict__ pv, TO* __restrict__ x) {
     const int j = blockIdx.x * kBlock + threadIdx.x;
     if (j >= n) return;
-    const double Vj = V[j], Pj = P[j], Qj = Q[j];
+    const double Vj = V[j], tj = th[j], Pj = P[j], Qj = Q[j];
     double at = sbus[4 * (int64_t)j + 1], av = sbus[4 * (int64_t)j];
-    // the Ybus row in chunks (see kLocChunk); the diagonal's own terms are
-    // Vj^2 G and -Vj^2 B, the self-admittance parts of the injection partials
+    // The Ybus row in chunks (see kLocChunk).  Entry (j, l) needs the terms of
+    // entry (l, j) of bus l's injection, V_l V_j (G_lj cos + B_lj sin)(th_l -
+    // th_j) and its Q twin: recomputed here from V_l, th_l and the transposed
+    // admittances, by the same operations as k_meas_loc's row of l (so the
+    // values are the ones that summed into P_l, Q_l).  The diagonal's own terms
+    // are Vj^2 G and -Vj^2 B, the self-admittance parts of the partials.
     const int e1 = yrow[j + 1];
     for (int e = yrow[j]; e < e1; e += kLocChunk) {
-        int l[kLocChunk], e2[kLocChunk];
+        int l[kLocChunk];
+        double G[kLocChunk], Bq[kLocChunk];
 #pragma unroll
         for (int q = 0; q < kLocChunk; ++q) {
             const int ee = min(e + q, e1 - 1);
             l[q] = ycol[ee];
-            e2[q] = ytr[ee];
+            G[q] = yGt[ee];
+            Bq[q] = yBt[ee];
         }
-        double sp[kLocChunk], sq[kLocChunk], pe[kLocChunk], qe[kLocChunk];
+        double sp[kLocChunk], sq[kLocChunk], Vl[kLocChunk], tl[kLocChunk];
 #pragma unroll
         for (int q = 0; q < kLocChunk; ++q) {
             sp[q] = sbus[4 * (int64_t)l[q] + 2];
             sq[q] = sbus[4 * (int64_t)l[q] + 3];
-            pe[q] = Pij[e2[q]];
-            qe[q] = Qij[e2[q]];
+            Vl[q] = V[l[q]];
+            tl[q] = th[l[q]];
         }
-#pragma unroll
-        for (int q = 0; q < kLocChunk; ++q) {
-            if (e + q >= e1) break;
-            if (l[q] == j) {
-                at += sp[q] * (qe[q] - Qj) + sq[q] * (Pj - pe[q]);
-                av += sp[q] * ((Pj + pe[q]) / Vj) + sq[q] * ((Qj + qe[q]) / Vj);
+        // one sincos in the loop body: the chunk shifts down a register per entry
+        const int cnt = min(kLocChunk, e1 - e);
+        for (int q = 0; q < cnt; ++q) {
+            double sn, cs;
+            sincos(tl[0] - tj, &sn, &cs);
+            const double vv = Vl[0] * Vj;
+            const double pe = vv * (G[0] * cs + Bq[0] * sn), qe = vv * (G[0] * sn - Bq[0] * cs);
+            if (l[0] == j) {
+                at += sp[0] * (qe - Qj) + sq[0] * (Pj - pe);
+                av += sp[0] * ((Pj + pe) / Vj) + sq[0] * ((Qj + qe) / Vj);
             } else {
                 // partials of bus l's injections wrt this bus: the terms of entry (l, j)
-                at += sp[q] * qe[q] - sq[q] * pe[q];
-                av += (sp[q] * pe[q] + sq[q] * qe[q]) / Vj;
+                at += sp[0] * qe - sq[0] * pe;
+                av += (sp[0] * pe + sq[0] * qe) / Vj;
+            }
+#pragma unroll
+            for (int k = 0; k + 1 < kLocChunk; ++k) {
+                l[k] = l[k + 1];
+                G[k] = G[k + 1];
+                Bq[k] = Bq[k + 1];
+                sp[k] = sp[k + 1];
+                sq[k] = sq[k + 1];
+                Vl[k] = Vl[k + 1];
+                tl[k] = tl[k + 1];
             }
         }
     }
@@ -3228,6 +3255,7 @@ struct Instance::Impl {
     // Ybus
     DBuf<int32_t> yrow, ycol, ycptr, ycon;
     DBuf<double> yG, yB, Pij, Qij;
+    DBuf<double> yGt, yBt;  // Ybus values at the transposed positions (right-hand side by location)
     // state & powers
     DBuf<double> V, th, P, Q, Pf, Qf, Pt, Qt;
     // measurements
@@ -3575,7 +3603,7 @@ Instance::~Instance() {
     auto rel = [&](auto& buf) { buf.release(b); };
     rel(d_->bus_type); rel(d_->from); rel(d_->to); rel(d_->vset); rel(d_->gs); rel(d_->bs); rel(d_->bc);
     rel(d_->tap); rel(d_->shift); rel(d_->status); rel(d_->yrow); rel(d_->ycol); rel(d_->ycptr); rel(d_->ycon);
-    rel(d_->yG); rel(d_->yB); rel(d_->Pij); rel(d_->Qij); rel(d_->V); rel(d_->th); rel(d_->P); rel(d_->Q);
+    rel(d_->yG); rel(d_->yB); rel(d_->yGt); rel(d_->yBt); rel(d_->Pij); rel(d_->Qij); rel(d_->V); rel(d_->th); rel(d_->P); rel(d_->Q);
     rel(d_->Pf); rel(d_->Qf); rel(d_->Pt); rel(d_->Qt); rel(d_->mcode); rel(d_->mloc); rel(d_->hrow);
     rel(d_->hs); rel(d_->hcolp); rel(d_->cg_b); rel(d_->cg_x); rel(d_->cg_r); rel(d_->cg_p); rel(d_->cg_q);
     rel(d_->cg_z); rel(d_->cg_t); rel(d_->cg_parts); rel(d_->cg_s); rel(d_->z); rel(d_->sigma); rel(d_->rs); rel(d_->hx); rel(d_->r); rel(d_->jpart); rel(d_->rpart); rel(d_->vsave); rel(d_->thsave);
@@ -3840,6 +3868,10 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
         D.yB.alloc(nY, bytes_);
         D.Pij.alloc(nY, bytes_);
         D.Qij.alloc(nY, bytes_);
+    }
+    if (D.use_loc && D.yGt.n != nY) {
+        D.yGt.alloc(nY, bytes_);
+        D.yBt.alloc(nY, bytes_);
     }
     if (D.V.n != (size_t)h.n) {
         std::vector<double> V0(h.n, 1.0), t0(h.n, 0.0);
@@ -4111,6 +4143,8 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
     d_->results_valid = false;
     if (nY) k_ybus<<<nblk(nY), kBlock, 0, s>>>((int)nY, D.ycptr.p, D.ycon.p, D.gs.p, D.bs.p, D.bc.p, D.tap.p,
                                                 D.shift.p, D.status.p, D.yG.p, D.yB.p);
+    if (nY && D.use_loc)
+        k_ybus_tr<<<nblk(nY), kBlock, 0, s>>>((int)nY, D.ytr.p, D.yG.p, D.yB.p, D.yGt.p, D.yBt.p);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(s));
 }
@@ -4148,6 +4182,8 @@ void Instance::update_branches(const HostArrays& h) {
     size_t nY = S_->ycol.size();
     if (nY) k_ybus<<<nblk(nY), kBlock, 0, s>>>((int)nY, D.ycptr.p, D.ycon.p, D.gs.p, D.bs.p, D.bc.p, D.tap.p,
                                                 D.shift.p, D.status.p, D.yG.p, D.yB.p);
+    if (nY && D.use_loc)
+        k_ybus_tr<<<nblk(nY), kBlock, 0, s>>>((int)nY, D.ytr.p, D.yG.p, D.yB.p, D.yGt.p, D.yBt.p);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(s));
     factor_valid_ = false;
@@ -4193,6 +4229,8 @@ void Instance::refresh_topology() {
     const size_t nY = S_->ycol.size();
     if (nY) k_ybus<<<nblk(nY), kBlock, 0, s>>>((int)nY, D.ycptr.p, D.ycon.p, D.gs.p, D.bs.p, D.bc.p, D.tap.p,
                                                 D.shift.p, D.status.p, D.yG.p, D.yB.p);
+    if (nY && D.use_loc)
+        k_ybus_tr<<<nblk(nY), kBlock, 0, s>>>((int)nY, D.ytr.p, D.yG.p, D.yB.p, D.yGt.p, D.yBt.p);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(s));
     D.topo_staged = false;
@@ -4352,8 +4390,6 @@ void launch_meas_loc(Instance::Impl& D, int robust, double gamma) {
     A.yB = D.yB.p;
     A.Pout = D.P.p;
     A.Qout = D.Q.p;
-    A.Pij = D.Pij.p;
-    A.Qij = D.Qij.p;
     A.st = D.st.p;
     const int64_t nt = (int64_t)D.n + D.nb;
     if (nt) k_meas_loc<<<nblk(nt), kBlock, 0, D.stream>>>(A);
@@ -4362,9 +4398,9 @@ void launch_meas_loc(Instance::Impl& D, int robust, double gamma) {
 template <typename T>
 void launch_rhs_loc(Instance::Impl& D) {
     if (D.n)
-        k_rhs_loc<T><<<nblk(D.n), kBlock, 0, D.stream>>>(D.n, D.yrow.p, D.ycol.p, D.ytr.p, D.Pij.p,
-                                                         D.Qij.p, D.P.p, D.Q.p, D.V.p, D.sbus.p, D.sbr.p, D.inc_ptr.p,
-                                                         D.inc.p, D.pth.p, D.pv.p, bufs<T>(D).x.p);
+        k_rhs_loc<T><<<nblk(D.n), kBlock, 0, D.stream>>>(D.n, D.yrow.p, D.ycol.p, D.yGt.p, D.yBt.p, D.P.p, D.Q.p,
+                                                         D.V.p, D.th.p, D.sbus.p, D.sbr.p, D.inc_ptr.p, D.inc.p,
+                                                         D.pth.p, D.pv.p, bufs<T>(D).x.p);
 }
 
 void launch_meas(Instance::Impl& D, int robust, double gamma, int unit_weights, int want_jac, bool want_out,
