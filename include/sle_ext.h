@@ -64,12 +64,12 @@ typedef struct SLE_ExtResult {
  *                     bottom subtrees is cut the same way once more (middle
  *                     subtrees, same limits), so that only the rest is solved
  *                     through L_TT^-1 blocks; 0: two levels
- *   "pinv_wg_fwd"     waves per middle subtree in the forward product: 0 (one
+ *   "pinv_wg_fwd"     waves per middle subtree in the forward product: 1 (one
  *                     wave per subtree), 4, 8 or 16 (one workgroup per subtree,
- *                     columns split over its waves); -1 (default): 4
+ *                     columns split over its waves); 0 (default): 4
  *   "pinv_wg_bwd"     the same for the backward product (rows split over the
- *                     waves); -1 (default): 8 for FP32 solve data, 0 for FP64
- *   "xcd_remap"      1 (default): the measurement, power, right-hand-side and
+ *                     waves); 0 (default): 8 for FP32 solve data, 1 for FP64
+ *   "xcd_remap"       1 (default): the measurement, power, right-hand-side and
  *                     top-product kernels run contiguous block ranges per XCD
  *                     (process-wide on the device)
  *   "meas_by_type"    1 (default): the device keeps the measurement rows grouped

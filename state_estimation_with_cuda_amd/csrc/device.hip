@@ -3379,9 +3379,9 @@ struct Instance::Impl {
     bool use_pi = true;
     bool pi_dense = true;   // bottom subtrees as dense solve matrices (else LDS tasks)
     bool pi_mid = true;     // a second level of dense subtrees between the bottom ones and the top
-    // middle subtree products: waves per subtree (0: one wave each, k_sub_fwd /
-    // k_sub_bwd; 4, 8, 16: one workgroup each; -1: by precision), per direction
-    int sub_wg_fwd = -1, sub_wg_bwd = -1;
+    // middle subtree products: waves per subtree (1: one wave each, k_sub_fwd /
+    // k_sub_bwd; 4, 8, 16: one workgroup each; 0: by precision), per direction
+    int sub_wg_fwd = 0, sub_wg_bwd = 0;
     bool meas_by_type = true;  // measurement rows grouped by evaluation class (else by anchor bus only)
     // right-hand side by location (needs the class grouping)
     bool rhs_loc = true;  // asked for
@@ -4678,8 +4678,8 @@ void launch_trisolve(Instance::Impl& D, Prof* pf = nullptr) {
             if (n1) {
                 prof_begin(pf, PH_MID_FWD);
                 // by precision (measured at 200k buses): four waves per subtree
-                const int wg = D.sub_wg_fwd >= 0 ? D.sub_wg_fwd : 4;
-                if (wg)
+                const int wg = D.sub_wg_fwd > 0 ? D.sub_wg_fwd : 4;
+                if (wg > 1)
                     launch_sub_fwd_wg<T>(D, wg, n1, sub_args(D), A, B.msub.p, n0);
                 else
                     k_sub_fwd<T, 1><<<(n1 + nw - 1) / nw, kBlock, 0, D.stream>>>(sub_args(D), A, B.msub.p, n0,
@@ -4716,8 +4716,8 @@ void launch_trisolve(Instance::Impl& D, Prof* pf = nullptr) {
                 // by precision (measured at 200k buses): eight waves per subtree
                 // for FP32 data, one wave for FP64 (whose 64-wide row loads of the
                 // split kernel cost more than its shorter chain saves)
-                const int wg = D.sub_wg_bwd >= 0 ? D.sub_wg_bwd : (sizeof(T) == 4 ? 8 : 0);
-                if (wg)
+                const int wg = D.sub_wg_bwd > 0 ? D.sub_wg_bwd : (sizeof(T) == 4 ? 8 : 1);
+                if (wg > 1)
                     launch_sub_bwd_wg<T>(D, wg, n1, sub_args(D), A, Mrow, n0);
                 else
                     k_sub_bwd<T, 1><<<(n1 + nw - 1) / nw, kBlock, 0, D.stream>>>(sub_args(D), A, Mrow, n0, PI.nsub);
@@ -5789,8 +5789,8 @@ void Instance::set_middle_subtrees(bool on) {
 }
 
 void Instance::set_subtree_workgroups(int fwd, int bwd) {
-    auto ok = [](int w) { return w == -1 || w == 0 || w == 4 || w == 8 || w == 16; };
-    if (!ok(fwd) || !ok(bwd)) throw std::invalid_argument("waves per middle subtree: -1, 0, 4, 8 or 16");
+    auto ok = [](int w) { return w == 0 || w == 1 || w == 4 || w == 8 || w == 16; };
+    if (!ok(fwd) || !ok(bwd)) throw std::invalid_argument("waves per middle subtree: 0, 1, 4, 8 or 16");
     ++d_->plan_epoch;
     d_->sub_wg_fwd = fwd;
     d_->sub_wg_bwd = bwd;

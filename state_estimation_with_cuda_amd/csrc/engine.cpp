@@ -186,7 +186,7 @@ int Engine::set_option(const char* key, double value) {
         opt_.pinv_mid = value != 0;
         model_dirty_ = true;  // the plan is rebuilt at the next upload
     } else if (k == "pinv_wg_fwd" || k == "pinv_wg_bwd") {
-        if (value != -1 && value != 0 && value != 4 && value != 8 && value != 16) return false;
+        if (value != 0 && value != 1 && value != 4 && value != 8 && value != 16) return ST_INVALID_ARGUMENT;
         (k == "pinv_wg_fwd" ? opt_.pinv_wg_fwd : opt_.pinv_wg_bwd) = (int)value;
         if (inst_) inst_->set_subtree_workgroups(opt_.pinv_wg_fwd, opt_.pinv_wg_bwd);
     } else if (k == "pinv_dense") {

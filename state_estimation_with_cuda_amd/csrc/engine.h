@@ -94,8 +94,8 @@ struct EngineOptions {
     bool lds_flags = false;       // task solves: per-front LDS flags instead of per-level barriers
     bool pinv_dense = true;       // partitioned inverse: bottom subtrees as dense solve matrices
     bool pinv_mid = true;         // and a middle level of dense subtrees under a smaller top
-    int pinv_wg_fwd = -1;         // middle subtree products: waves per subtree, forward (0: one wave
-    int pinv_wg_bwd = -1;         // each; 4, 8, 16; -1: by precision) and backward
+    int pinv_wg_fwd = 0;          // middle subtree products: waves per subtree, forward (1: one wave
+    int pinv_wg_bwd = 0;          // each; 4, 8, 16: a workgroup each; 0: by precision) and backward
     bool xcd_remap = true;        // XCD-aware block order of the gather kernels
     bool meas_by_type = true;     // measurement rows grouped by evaluation class on the device
     bool rhs_by_location = true;  // cached-factor iterations: RHS from per-location sums

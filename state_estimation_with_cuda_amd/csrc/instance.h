@@ -221,8 +221,8 @@ class Instance {
     // with dense subtrees: a second level of them inside the forest above the
     // bottom ones, leaving a smaller top to the partitioned inverse (setup)
     void set_middle_subtrees(bool on);
-    // middle subtree products: waves per subtree, forward and backward (0: one
-    // wave each; 4, 8 or 16: one workgroup each)
+    // middle subtree products: waves per subtree, forward and backward (1: one
+    // wave each; 4, 8 or 16: one workgroup each; 0: by precision)
     void set_subtree_workgroups(int fwd, int bwd);
     // XCD-aware block order of the gather kernels (process-wide on this device)
     void set_xcd_remap(bool on);

@@ -368,7 +368,7 @@ PLAN_OPTIONS = {
     "bus_ordered_rows": {"meas_by_type": 0, "xcd_remap": 0},
     "rhs_by_location": {"rhs_by_location": 1},
     "two_levels": {"pinv_mid": 0},
-    "middle_subtrees_one_wave": {"pinv_wg_fwd": 0, "pinv_wg_bwd": 0},
+    "middle_subtrees_one_wave": {"pinv_wg_fwd": 1, "pinv_wg_bwd": 1},
     "middle_subtrees_4_waves": {"pinv_wg_fwd": 4, "pinv_wg_bwd": 4},
     "middle_subtrees_8_waves": {"pinv_wg_fwd": 8, "pinv_wg_bwd": 8},
     "middle_subtrees_16_waves": {"pinv_wg_fwd": 16, "pinv_wg_bwd": 16},
