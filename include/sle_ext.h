@@ -29,7 +29,7 @@ typedef struct SLE_ExtResult {
     int32_t largest_residual_idx;
     int32_t flags;       /* bit 0: dependency-driven factor/solve sweeps; bit 1: FP64 factor in use;
                             bit 2: mixed form on the FP64 factor (FP32 solve data) */
-    double gpu_ms;       /* device time of the solve */
+    double gpu_ms;       /* time of the solve: first launch to the final status on the host */
     double host_ms;      /* wall time of the call */
     int32_t pivots_substituted; /* mixed form: iterations whose FP32 factor had pivots substituted */
     int32_t pcg_steps;   /* conjugate-gradient steps per Newton step (0: direct solve) */
@@ -37,6 +37,8 @@ typedef struct SLE_ExtResult {
     int32_t seq_fallbacks;  /* replayed iterations whose status the host waited for by a stream
                                synchronization after its spin budget (expected 0) */
     int32_t retries;        /* flat-start restarts after divergence (PRECISION; wls_solver.cu:260-334) */
+    int32_t ahead;          /* iterations launched before the previous iteration's status was seen */
+    int32_t ahead_idle;     /* of those, iterations that found the solve converged and changed nothing */
 } SLE_ExtResult;
 
 /* Engine options outside SLE_Config:
@@ -69,6 +71,10 @@ typedef struct SLE_ExtResult {
  *                     columns split over its waves); 0 (default): 4
  *   "pinv_wg_bwd"     the same for the backward product (rows split over the
  *                     waves); 0 (default): 8 for FP32 solve data, 1 for FP64
+ *   "pinv_pack"       1: the bottom subtree products run bottom subtrees of at
+ *                     most 32 rows two to a wave and of at most 16 rows four
+ *                     to a wave; 0: one subtree per wave; -1 (default): packed
+ *                     for FP32 solve data, one per wave for FP64
  *   "xcd_remap"       1 (default): the measurement, power, right-hand-side and
  *                     top-product kernels run contiguous block ranges per XCD
  *                     (process-wide on the device)
@@ -191,7 +197,11 @@ SLE_API SLE_StatusCode SLE_CALL sle_ext_ApplyStagedTelemetry(SLE_Handle handle, 
  * side, forward solve, backward solve, state update, CG refinement, and inside
  * the two solves: bottom-subtree forward product, top forward product, top
  * backward product, bottom-subtree backward product, middle-subtree forward
- * (assembly and product) and backward products.  ms[c] is the accumulated
+ * (assembly and product) and backward products; then the measurement pass
+ * that opens a solve's first iteration on the cached factor, and the status
+ * finish of iterations that a measurement pass closes (on the cached factor
+ * with the right-hand side by location, the measurement pass class times the
+ * closing pass).  ms[c] is the accumulated
  * time, launches[c] the kernel launches, calls[c] the phase runs. */
 SLE_API SLE_StatusCode SLE_CALL sle_ext_GetKernelTimes(SLE_Handle handle, double* ms, int64_t* launches,
                                                        int64_t* calls, int32_t count);

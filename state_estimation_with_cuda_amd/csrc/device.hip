@@ -67,6 +67,9 @@ constexpr int kSpinLimit = 1 << 22;  // polls before a dependency wait gives up 
 constexpr double kNewtonSwitch = 1e-2;  // Huber: step size below which the outliers' gain weights drop
 constexpr double kRefreshRatio = 0.25;  // REALTIME: step ratio above which the cached factor is refreshed
 constexpr double kLinearRegime = 1e-3;  // steps below this: Gauss-Newton contracts fast with a sound factor
+// an iteration on the cached factor is queued behind the running one unless
+// the running one's predicted step is below kAheadMargin * tol
+constexpr double kAheadMargin = 2.0;
 
 // measurement code byte: type (4 bits) | end << 4 | active << 5 | from-first << 6
 // (from-first: the branch's from bus precedes its to bus in the natural state
@@ -102,6 +105,11 @@ struct DevStatus {
     int piv;
     int flow_err;
     uint32_t seq;  // iterations finished (k_maxfinish); the host copy's is written last
+    // set by k_maxfinish when an iteration converged (or produced a NaN step)
+    // while a next iteration may already be queued behind it: that iteration
+    // then leaves the state and the status alone.  Cleared by the pass that
+    // opens a solve's first iteration (k_power, or k_meas_loc with reset).
+    int stop;
 };
 
 // ---------------------------------------------------------------------------
@@ -195,6 +203,7 @@ __global__ __launch_bounds__(kBlock) void k_power(int n, const int32_t* __restri
         st->nan = 0;
         st->piv = 0;
         st->flow_err = 0;
+        st->stop = 0;
     }
     if (i >= n) return;
     double vi = V[i], ti = th[i];
@@ -521,21 +530,34 @@ struct LocArgs {
     double* Pout;
     double* Qout;
     DevStatus* st;
+    int reset;  // clear the iteration's status flags (the pass that opens an iteration)
+    // optional: per block, J (base weights) and the largest |r| / sigma with its row
+    double* jpart;
+    double* rpart;
+    int* ipart;
 };
 
-// w r of one row from its code, value and sigma at the evaluated h
+// w r of one row from its code, value and sigma at the evaluated h; jt gets the
+// row's term of J, r^2 / sigma^2 with the base weight (0 for inactive rows)
 struct LocWeight {
     int robust;
     double gamma;
-    __device__ double operator()(uint8_t code, double z, double sg, double hv) const {
+    __device__ double operator()(uint8_t code, double z, double sg, double hv, double& jt) const {
+        jt = 0.0;
         if (!((code >> 5) & 1)) return 0.0;  // inactive: weight 0
         const double r = z - hv;
-        double w = 1.0 / (sg * sg);
+        const double wb = 1.0 / (sg * sg);
+        double w = wb;
         if (robust) {
             const double ar = fabs(r);
             if (ar / sg > gamma) w = gamma / (sg * ar);
         }
+        jt = wb * r * r;
         return w * r;
+    }
+    __device__ double operator()(uint8_t code, double z, double sg, double hv) const {
+        double jt;
+        return (*this)(code, z, sg, hv, jt);
     }
 };
 
@@ -560,7 +582,54 @@ __device__ inline void loc_rows(const uint8_t* __restrict__ mcode, const double*
         }
 #pragma unroll
         for (int q = 0; q < K; ++q)
-            if (i + q < i1) f(c[q], z[q], sg[q]);
+            if (i + q < i1) f(c[q], z[q], sg[q], i + q);
+    }
+}
+
+// J with the base weights and the largest |r| / sigma (ties to the lower row)
+// over the active rows a thread evaluates, from each row's J term r^2 / sigma^2
+// (its square root is |r| / sigma: one root per thread, at the end)
+struct LocStats {
+    double J = 0.0, best = -1.0;
+    int ri = -1;
+    __device__ void add(uint8_t code, double jt, int i) {
+        if (!((code >> 5) & 1)) return;
+        J += jt;
+        if (jt > best || (jt == best && (ri < 0 || i < ri))) {
+            best = jt;
+            ri = i;
+        }
+    }
+    __device__ double rn() const { return best < 0.0 ? -1.0 : sqrt(best); }
+};
+
+// Block partials of LocStats (fixed tree order: deterministic), one per block.
+__device__ inline void loc_stats_block(LocStats s, double* jpart, double* rpart, int* ipart) {
+    __shared__ double sj[kBlock / kWave], sr[kBlock / kWave];
+    __shared__ int si[kBlock / kWave];
+    double J = wave_sum(s.J), R = s.rn();
+    int I = s.ri;
+    wave_argmax(R, I);
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        sj[w] = J;
+        sr[w] = R;
+        si[w] = I;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double JJ = 0.0, RR = -1.0;
+        int II = -1;
+        for (int k = 0; k < kBlock / kWave; ++k) {
+            JJ += sj[k];
+            if (sr[k] > RR || (sr[k] == RR && si[k] >= 0 && (II < 0 || si[k] < II))) {
+                RR = sr[k];
+                II = si[k];
+            }
+        }
+        jpart[blockIdx.x] = JJ;
+        rpart[blockIdx.x] = RR;
+        ipart[blockIdx.x] = II;
     }
 }
 
@@ -570,11 +639,14 @@ __global__ __launch_bounds__(kBlock) void k_meas_loc(LocArgs A) {
     const uint8_t* mcode = A.mcode;
     const double* zv = A.z;
     const double* sigma = A.sigma;
-    if (t == 0) {  // the iteration's status starts here (as in k_power)
+    if (t == 0 && A.reset) {  // the iteration's status starts here (as in k_power)
         A.st->nan = 0;
         A.st->piv = 0;
         A.st->flow_err = 0;
+        A.st->stop = 0;
     }
+    const bool stats = A.jpart != nullptr;
+    LocStats ls;
     if (t < A.n) {
         const int b = t;
         // injections of bus b over its Ybus row
@@ -618,18 +690,22 @@ __global__ __launch_bounds__(kBlock) void k_meas_loc(LocArgs A) {
         double sV = 0.0, sT = 0.0, sP = 0.0, sQ = 0.0;
         // (selects rather than branches: a branch per sum lets the compiler
         // address the sums through a pointer, which moves them to scratch)
-        loc_rows(mcode, zv, sigma, A.bv_ptr[b], A.bv_ptr[b + 1], [&](uint8_t c, double z, double sg) {
+        loc_rows(mcode, zv, sigma, A.bv_ptr[b], A.bv_ptr[b + 1], [&](uint8_t c, double z, double sg, int i) {
             const bool isv = (c & 15) == 0;
-            const double wr = wr_of(c, z, sg, isv ? vi : ti);
+            double jt;
+            const double wr = wr_of(c, z, sg, isv ? vi : ti, jt);
             sV += isv ? wr : 0.0;
             sT += isv ? 0.0 : wr;
+            if (stats) ls.add(c, jt, i);
         });
-        loc_rows(mcode, zv, sigma, A.bi_ptr[b], A.bi_ptr[b + 1], [&](uint8_t c, double z, double sg) {
+        loc_rows(mcode, zv, sigma, A.bi_ptr[b], A.bi_ptr[b + 1], [&](uint8_t c, double z, double sg, int i) {
             const int ty = c & 15;
             const bool isp = ty == 2 || ty == 7;
-            const double wr = wr_of(c, z, sg, isp ? pb : qb);
+            double jt;
+            const double wr = wr_of(c, z, sg, isp ? pb : qb, jt);
             sP += isp ? wr : 0.0;
             sQ += isp ? 0.0 : wr;
+            if (stats) ls.add(c, jt, i);
         });
         double* o = A.sbus + 4 * (int64_t)b;
         o[0] = sV;
@@ -657,20 +733,22 @@ __global__ __launch_bounds__(kBlock) void k_meas_loc(LocArgs A) {
                 const double q = end ? -Vj * Vj * bsh + AA * sgb : -Vi * Vi * bsh / (a * a) - AA * sgb;
                 const double ve = end ? Vj : Vi;
                 double cP = 0.0, cQ = 0.0, cV = 0.0;
-                loc_rows<kLocChunkFlow>(mcode, zv, sigma, i0, i1, [&](uint8_t cd, double z, double sg) {
+                loc_rows<kLocChunkFlow>(mcode, zv, sigma, i0, i1, [&](uint8_t cd, double z, double sg, int i) {
                     const int ty = cd & 15;
+                    double jt;
                     if (ty == 4 || ty == 5) {
-                        const double wr = wr_of(cd, z, sg, ty == 4 ? p : q);
+                        const double wr = wr_of(cd, z, sg, ty == 4 ? p : q, jt);
                         cP += ty == 4 ? wr : 0.0;
                         cQ += ty == 4 ? 0.0 : wr;
                     } else {  // I_MAG = |S| / V_end
                         const double S = sqrt(p * p + q * q);
                         const double inv = S > 1e-12 ? 1.0 / (S * ve) : 0.0;
-                        const double wr = wr_of(cd, z, sg, S / ve);
+                        const double wr = wr_of(cd, z, sg, S / ve, jt);
                         cP += wr * p * inv;
                         cQ += wr * q * inv;
                         cV -= wr * S / (ve * ve);
                     }
+                    if (stats) ls.add(cd, jt, i);
                 });
                 // partials wrt (theta_f, theta_t, V_f, V_t)
                 double dP[4], dQ[4];
@@ -703,6 +781,7 @@ __global__ __launch_bounds__(kBlock) void k_meas_loc(LocArgs A) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) o[k] = acc[k];
     }
+    if (stats) loc_stats_block(ls, A.jpart, A.rpart, A.ipart);
 }
 
 template <typename TO>
@@ -713,9 +792,15 @@ __global__ __launch_bounds__(kBlock) void k_rhs_loc(int n, const int32_t* __rest
                                                     const double* __restrict__ th, const double* __restrict__ sbus,
                                                     const double* __restrict__ sbr, const int32_t* __restrict__ inc_ptr,
                                                     const int32_t* __restrict__ inc, const int32_t* __restrict__ pth,
-                                                    const int32_t* __restrict__ pv, TO* __restrict__ x) {
+                                                    const int32_t* __restrict__ pv, TO* __restrict__ x,
+                                                    DevStatus* st) {
     const int j = blockIdx.x * kBlock + threadIdx.x;
     if (j >= n) return;
+    if (j == 0) {  // opens an iteration whose measurement pass closed the previous one
+        st->nan = 0;
+        st->piv = 0;
+        st->flow_err = 0;
+    }
     const double Vj = V[j], tj = th[j], Pj = P[j], Qj = Q[j];
     double at = sbus[4 * (int64_t)j + 1], av = sbus[4 * (int64_t)j];
     // The Ybus row in chunks (see kLocChunk).  Entry (j, l) needs the terms of
@@ -793,21 +878,37 @@ __global__ __launch_bounds__(kBlock) void k_rhs_loc(int n, const int32_t* __rest
     x[pv[j]] = (TO)av;
 }
 
-// Single-block finish of the J / largest-residual partials.
-__global__ __launch_bounds__(kBlock) void k_jfinish(int nparts, const double* __restrict__ jpart,
-                                                    const double* __restrict__ rpart, const int* __restrict__ ipart,
-                                                    DevStatus* st) {
+// Single-block finish of the J / largest-residual partials into st (J, rmax,
+// ridx), NT threads.  Loads go four partials per thread at a time, so that the
+// block waits for memory once per four strides.
+template <int NT>
+__device__ inline void jfinish_block(int nparts, const double* __restrict__ jpart, const double* __restrict__ rpart,
+                                     const int* __restrict__ ipart, DevStatus* st) {
     double J = 0.0, R = -1.0;
     int I = -1;
-    for (int p = threadIdx.x; p < nparts; p += kBlock) {
-        J += jpart[p];
-        if (rpart[p] > R || (rpart[p] == R && ipart[p] >= 0 && (I < 0 || ipart[p] < I))) {
-            R = rpart[p];
-            I = ipart[p];
+    constexpr int U = 4;
+    for (int p0 = threadIdx.x; p0 < nparts; p0 += U * NT) {
+        double jv[U], rv[U];
+        int iv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int p = min(p0 + u * NT, nparts - 1);
+            jv[u] = jpart[p];
+            rv[u] = rpart[p];
+            iv[u] = ipart[p];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (p0 + u * NT >= nparts) break;
+            J += jv[u];
+            if (rv[u] > R || (rv[u] == R && iv[u] >= 0 && (I < 0 || iv[u] < I))) {
+                R = rv[u];
+                I = iv[u];
+            }
         }
     }
-    __shared__ double sj[kBlock / kWave], sr[kBlock / kWave];
-    __shared__ int si[kBlock / kWave];
+    __shared__ double sj[NT / kWave], sr[NT / kWave];
+    __shared__ int si[NT / kWave];
     J = wave_sum(J);
     wave_argmax(R, I);
     const int w = threadIdx.x / kWave;
@@ -820,7 +921,7 @@ __global__ __launch_bounds__(kBlock) void k_jfinish(int nparts, const double* __
     if (threadIdx.x == 0) {
         double JJ = 0.0, RR = -1.0;
         int II = -1;
-        for (int k = 0; k < kBlock / kWave; ++k) {
+        for (int k = 0; k < NT / kWave; ++k) {
             JJ += sj[k];
             if (sr[k] > RR || (sr[k] == RR && si[k] >= 0 && (II < 0 || si[k] < II))) {
                 RR = sr[k];
@@ -831,6 +932,12 @@ __global__ __launch_bounds__(kBlock) void k_jfinish(int nparts, const double* __
         st->rmax = RR;
         st->ridx = II;
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_jfinish(int nparts, const double* __restrict__ jpart,
+                                                    const double* __restrict__ rpart, const int* __restrict__ ipart,
+                                                    DevStatus* st) {
+    jfinish_block<kBlock>(nparts, jpart, rpart, ipart, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -1847,20 +1954,14 @@ constexpr int kSubBatchB = 16;  // row loads in flight per lane, backward
 // LEVEL 0: bottom subtrees, 1: middle subtrees (one instantiation each, so
 // that profiles tell the two launches apart)
 template <typename T, int LEVEL>
-__global__ __launch_bounds__(kBlock) void k_sub_fwd(SubArgs B, SolveArgs<T> A, const T* __restrict__ M, int t0,
-                                                   int t1) {
-    __shared__ double bsh[kBlock / kWave][kSubCols];
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const int t = t0 + blockIdx.x * (kBlock / kWave) + w;
-    const int lane = threadIdx.x & (kWave - 1);
-    if (t >= t1) return;  // uniform per wave
+__device__ __forceinline__ void sub_fwd_wave(const SubArgs& B, const SolveArgs<T>& A, const T* __restrict__ M,
+                                             const int t, const int lane, double* bs) {
     const int n = B.n[t], R = B.R[t];
     const int32_t* cols = B.cols + B.cptr[t];
     const T* Mt = M + B.moff[t];
     T* u = A.ubuf + B.uoff[t];
     // entry (i, j) of the packed column-major M_t: j R - j (j - 1) / 2 + i - j
     auto cidx = [&](int j, int i) { return (uint32_t)(j * R - j * (j - 1) / 2 + i - j); };
-    double* bs = bsh[w];
     // FP32 elements: unconditional loads (an entry outside the trapezoid reads
     // entry 0) of raw values, selected and converted in the product; a load
     // under a branch gets its conversion in the same branch and waits for it
@@ -1938,16 +2039,22 @@ __global__ __launch_bounds__(kBlock) void k_sub_fwd(SubArgs B, SolveArgs<T> A, c
     if (lane + kWave < R) u[lane + kWave - n] = (T)a1;
 }
 
-// Backward: one wave per subtree; lane j accumulates column j over the rows of
-// the row-major copy, kSubBatchB coalesced row loads at a time.
 template <typename T, int LEVEL>
-__global__ __launch_bounds__(kBlock) void k_sub_bwd(SubArgs B, SolveArgs<T> A, const T* __restrict__ Mrow, int t0,
+__global__ __launch_bounds__(kBlock) void k_sub_fwd(SubArgs B, SolveArgs<T> A, const T* __restrict__ M, int t0,
                                                    int t1) {
-    __shared__ double vsh[kBlock / kWave][kSubRows];
+    __shared__ double bsh[kBlock / kWave][kSubCols];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int t = t0 + blockIdx.x * (kBlock / kWave) + w;
     const int lane = threadIdx.x & (kWave - 1);
     if (t >= t1) return;  // uniform per wave
+    sub_fwd_wave<T, LEVEL>(B, A, M, t, lane, bsh[w]);
+}
+
+// Backward: one wave per subtree; lane j accumulates column j over the rows of
+// the row-major copy, kSubBatchB coalesced row loads at a time.
+template <typename T, int LEVEL>
+__device__ __forceinline__ void sub_bwd_wave(const SubArgs& B, const SolveArgs<T>& A, const T* __restrict__ Mrow,
+                                             const int t, const int lane, double* v) {
     const int n = B.n[t], R = B.R[t];
     const int32_t* cols = B.cols + B.cptr[t];
     const int32_t* urow = A.rows + B.urow[t];
@@ -1966,7 +2073,6 @@ __global__ __launch_bounds__(kBlock) void k_sub_bwd(SubArgs B, SolveArgs<T> A, c
     // FP32: the gather's indices first (loads complete in issue order), then
     // the first row batch, then the gather itself (R <= 2 * 64); FP64: the
     // first row batch, then the gather
-    double* v = vsh[w];
     if (clamp) {
         const int ia = lane, ib = lane + kWave;
         const int xa = ia < R ? (ia < n ? cols[ia] : urow[ia - n]) : 0;
@@ -1989,6 +2095,163 @@ __global__ __launch_bounds__(kBlock) void k_sub_bwd(SubArgs B, SolveArgs<T> A, c
         }
     }
     if (lane < n) A.x[cols[lane]] = (T)acc;
+}
+
+template <typename T, int LEVEL>
+__global__ __launch_bounds__(kBlock) void k_sub_bwd(SubArgs B, SolveArgs<T> A, const T* __restrict__ Mrow, int t0,
+                                                   int t1) {
+    __shared__ double vsh[kBlock / kWave][kSubRows];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int t = t0 + blockIdx.x * (kBlock / kWave) + w;
+    const int lane = threadIdx.x & (kWave - 1);
+    if (t >= t1) return;  // uniform per wave
+    sub_bwd_wave<T, LEVEL>(B, A, Mrow, t, lane, vsh[w]);
+}
+
+// Bottom-subtree products with the small subtrees packed: a schedule lists
+// the waves (largest first), each either one subtree (more than 32 rows: the
+// one-wave code above) or two subtrees of at most 32 rows, or four of at most
+// 16, side by side on G = 32 or 16 lanes.  58 % of the bottom subtrees of the
+// 200k-bus grid have at most 32 rows: packing them cuts the waves of a product
+// by a third (13,178 -> 8,400).
+struct SubPack {
+    int nw;               // waves
+    const int32_t* kind;  // per wave: 64 (one subtree), 32 or 16 (lanes per subtree)
+    const int32_t* sub;   // per wave: four subtree indices (-1: none)
+};
+
+constexpr int kPackBatch = 16;  // column (forward) or row (backward) loads in flight per lane
+
+// Forward product of subtree t (t < 0: none) on G lanes, lane l = row l
+// (R <= G); bs: the G doubles of LDS of this lane group.
+template <typename T, int G>
+__device__ __forceinline__ void sub_fwd_packed(const SubArgs& B, const SolveArgs<T>& A, const T* __restrict__ M,
+                                               const int t, const int l, double* bs) {
+    constexpr bool clamp = sizeof(T) == 4;  // unconditional FP32 loads, as in sub_fwd_wave
+    int n = 0, R = 0, cp = 0;
+    int64_t mo = 0, uo = 0;
+    if (t >= 0) {
+        n = B.n[t];
+        R = B.R[t];
+        cp = B.cptr[t];
+        mo = B.moff[t];
+        uo = B.uoff[t];
+    }
+    const T* Mt = M + mo;
+    const int cl = l < n ? B.cols[cp + l] : 0;  // issued first: the gather waits for it alone
+    auto ok = [&](int j) { return j < n && l >= j && l < R; };
+    auto cidx = [&](int j) { return (uint32_t)(j * R - j * (j - 1) / 2 + l - j); };
+    T m[kPackBatch];
+    auto load = [&](int j0) {
+#pragma unroll
+        for (int q = 0; q < kPackBatch; ++q) {
+            const int j = j0 + q;
+            m[q] = clamp ? ld_off(Mt, ok(j) ? cidx(j) : 0u) : (ok(j) ? ld_off(Mt, cidx(j)) : T(0));
+        }
+    };
+    load(0);
+    if (l < n) bs[l] = (double)A.x[cl];
+    wave_sync();
+    double a = 0.0;
+#pragma unroll
+    for (int q = 0; q < kPackBatch; ++q) a += (!clamp || ok(q) ? (double)m[q] : 0.0) * (q < n ? bs[q] : 0.0);
+    if (G > kPackBatch && __ballot(n > kPackBatch)) {
+        load(kPackBatch);
+#pragma unroll
+        for (int q = 0; q < kPackBatch; ++q) {
+            const int j = kPackBatch + q;
+            a += (!clamp || ok(j) ? (double)m[q] : 0.0) * (j < n ? bs[j] : 0.0);
+        }
+    }
+    if (l < n)
+        A.x[cl] = (T)a;
+    else if (l < R)
+        A.ubuf[uo + l - n] = (T)a;
+}
+
+// Backward product of subtree t (t < 0: none) on G lanes, lane l = column l,
+// over the rows of the row-major copy (R <= G); v: the G doubles of LDS of
+// this lane group.
+template <typename T, int G>
+__device__ __forceinline__ void sub_bwd_packed(const SubArgs& B, const SolveArgs<T>& A, const T* __restrict__ Mrow,
+                                               const int t, const int l, double* v) {
+    constexpr bool clamp = sizeof(T) == 4;
+    int n = 0, R = 0, cp = 0;
+    int64_t mo = 0, ur = 0;
+    if (t >= 0) {
+        n = B.n[t];
+        R = B.R[t];
+        cp = B.cptr[t];
+        mo = B.moff[t];
+        ur = B.urow[t];
+    }
+    const T* Mt = Mrow + mo;
+    // this lane's row value: a subtree column, or one of the root's update rows
+    const int xr = l < R ? (l < n ? B.cols[cp + l] : A.rows[ur + l - n]) : 0;
+    auto ok = [&](int i) { return i < R && l <= i && l < n; };
+    auto ro = [&](int i) { return (uint32_t)(i <= n ? i * (i + 1) / 2 : n * (n + 1) / 2 + (i - n) * n) + l; };
+    T mv[kPackBatch];
+    auto load = [&](int i0) {
+#pragma unroll
+        for (int u = 0; u < kPackBatch; ++u) {
+            const int i = i0 + u;
+            mv[u] = clamp ? ld_off(Mt, ok(i) ? ro(i) : 0u) : (ok(i) ? ld_off(Mt, ro(i)) : T(0));
+        }
+    };
+    load(0);
+    if (l < R) v[l] = (double)A.x[xr];
+    wave_sync();
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < kPackBatch; ++u) acc += (!clamp || ok(u) ? (double)mv[u] : 0.0) * (u < R ? v[u] : 0.0);
+    if (G > kPackBatch && __ballot(R > kPackBatch)) {
+        load(kPackBatch);
+#pragma unroll
+        for (int u = 0; u < kPackBatch; ++u) {
+            const int i = kPackBatch + u;
+            acc += (!clamp || ok(i) ? (double)mv[u] : 0.0) * (i < R ? v[i] : 0.0);
+        }
+    }
+    if (l < n) A.x[xr] = (T)acc;  // xr = cols[l] for l < n
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sub_fwd_pk(SubArgs B, SubPack P, SolveArgs<T> A, const T* __restrict__ M) {
+    __shared__ double bsh[kBlock / kWave][kSubCols];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int wv = blockIdx.x * (kBlock / kWave) + w;
+    const int lane = threadIdx.x & (kWave - 1);
+    if (wv >= P.nw) return;  // uniform per wave
+    const int kind = P.kind[wv];
+    if (kind == kWave) {
+        sub_fwd_wave<T, 0>(B, A, M, P.sub[4 * wv], lane, bsh[w]);
+    } else if (kind == 32) {
+        const int g = lane >> 5;
+        sub_fwd_packed<T, 32>(B, A, M, P.sub[4 * wv + g], lane & 31, bsh[w] + 32 * g);
+    } else {
+        const int g = lane >> 4;
+        sub_fwd_packed<T, 16>(B, A, M, P.sub[4 * wv + g], lane & 15, bsh[w] + 16 * g);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sub_bwd_pk(SubArgs B, SubPack P, SolveArgs<T> A,
+                                                       const T* __restrict__ Mrow) {
+    __shared__ double vsh[kBlock / kWave][kSubRows];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int wv = blockIdx.x * (kBlock / kWave) + w;
+    const int lane = threadIdx.x & (kWave - 1);
+    if (wv >= P.nw) return;  // uniform per wave
+    const int kind = P.kind[wv];
+    if (kind == kWave) {
+        sub_bwd_wave<T, 0>(B, A, Mrow, P.sub[4 * wv], lane, vsh[w]);
+    } else if (kind == 32) {
+        const int g = lane >> 5;
+        sub_bwd_packed<T, 32>(B, A, Mrow, P.sub[4 * wv + g], lane & 31, vsh[w] + 32 * g);
+    } else {
+        const int g = lane >> 4;
+        sub_bwd_packed<T, 16>(B, A, Mrow, P.sub[4 * wv + g], lane & 15, vsh[w] + 16 * g);
+    }
 }
 
 // Loads through a buffer resource at a wave-uniform byte offset (an SGPR) plus
@@ -2350,7 +2613,7 @@ __global__ __launch_bounds__(kBlock) void k_update(int n, const int32_t* __restr
                                                    int skip_on_pivot) {
     const int blk = blockIdx.x;
     const int b = blk * kBlock + threadIdx.x;
-    const bool skip = (skip_on_pivot && st->piv != 0) || st->flow_err != 0;
+    const bool skip = (skip_on_pivot && st->piv != 0) || st->flow_err != 0 || st->stop != 0;
     double mx = 0.0;
     if (b < n && !skip) {
         double dv = (double)x[pv[b]];
@@ -2378,32 +2641,88 @@ __global__ __launch_bounds__(kBlock) void k_update(int n, const int32_t* __restr
 // Also hands the iteration's status to the host: a copy into the pinned status
 // block through its device mapping, so the host waits on the stream and reads
 // it without a separate copy (no copy engine, no blit launch per iteration).
-__global__ __launch_bounds__(kBlock) void k_maxfinish(int nparts, const double* __restrict__ mpart, DevStatus* st,
-                                                      DevStatus* host_out) {
-    double r = 0.0;
-    for (int p = threadIdx.x; p < nparts; p += kBlock) r = fmax(r, mpart[p]);
-    __shared__ double sm[kBlock / kWave];
-    r = wave_max(r);
-    if ((threadIdx.x & (kWave - 1)) == 0) sm[threadIdx.x / kWave] = r;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double q = 0.0;
-        for (int k = 0; k < kBlock / kWave; ++k) q = fmax(q, sm[k]);
-        st->maxdx = q;
-        if (host_out) {
-            host_out->maxdx = q;
-            host_out->J = st->J;
-            host_out->rmax = st->rmax;
-            host_out->ridx = st->ridx;
-            host_out->nan = st->nan;
-            host_out->piv = st->piv;
-            host_out->flow_err = st->flow_err;
-            // the sequence number last, after the fields are visible to the host
-            const uint32_t sq = st->seq + 1;
-            st->seq = sq;
-            __threadfence_system();
-            __hip_atomic_store(&host_out->seq, sq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+// With nj > 0 it also finishes the J / largest-residual partials of a
+// measurement pass at the updated state (the pass that closes an iteration), so
+// that the status carries J at the state the iteration leaves.  One block of
+// kFinish threads; every load (the status flags, the max |dx| and J partials)
+// is issued before the first wait.
+//
+// stop_tol > 0: an iteration whose max |dx| falls below it (or that produced a
+// NaN step) sets st->stop, so that an iteration queued behind it changes
+// nothing; an iteration that finds st->stop set publishes no status.
+constexpr int kFinish = 1024;
+__global__ __launch_bounds__(kFinish) void k_maxfinish(int nparts, const double* __restrict__ mpart, DevStatus* st,
+                                                       DevStatus* host_out, int nj, const double* __restrict__ jpart,
+                                                       const double* __restrict__ rpart, const int* __restrict__ ipart,
+                                                       double stop_tol) {
+    const int tid = threadIdx.x;
+    const int stop = st->stop;
+    double mx = 0.0, J = 0.0, R = -1.0;
+    int I = -1;
+    const int np = max(nparts, nj);
+    for (int p0 = tid; p0 < np; p0 += 2 * kFinish) {
+        const int p1 = p0 + kFinish;
+        const double m0 = p0 < nparts ? mpart[p0] : 0.0, m1 = p1 < nparts ? mpart[p1] : 0.0;
+        const double j0 = p0 < nj ? jpart[p0] : 0.0, j1 = p1 < nj ? jpart[p1] : 0.0;
+        const double r0 = p0 < nj ? rpart[p0] : -1.0, r1 = p1 < nj ? rpart[p1] : -1.0;
+        const int i0 = p0 < nj ? ipart[p0] : -1, i1 = p1 < nj ? ipart[p1] : -1;
+        mx = fmax(mx, fmax(m0, m1));
+        J += j0 + j1;
+        if (r0 > R || (r0 == R && i0 >= 0 && (I < 0 || i0 < I))) {
+            R = r0;
+            I = i0;
         }
+        if (r1 > R || (r1 == R && i1 >= 0 && (I < 0 || i1 < I))) {
+            R = r1;
+            I = i1;
+        }
+    }
+    if (stop) return;  // uniform: this iteration was queued behind a finished one
+    mx = wave_max(mx);
+    J = wave_sum(J);
+    wave_argmax(R, I);
+    __shared__ double smx[kFinish / kWave], sj[kFinish / kWave], sr[kFinish / kWave];
+    __shared__ int si[kFinish / kWave];
+    const int w = tid / kWave;
+    if ((tid & (kWave - 1)) == 0) {
+        smx[w] = mx;
+        sj[w] = J;
+        sr[w] = R;
+        si[w] = I;
+    }
+    __syncthreads();
+    if (tid != 0) return;
+    double q = 0.0, JJ = 0.0, RR = -1.0;
+    int II = -1;
+    for (int k = 0; k < kFinish / kWave; ++k) {  // fixed order: deterministic
+        q = fmax(q, smx[k]);
+        JJ += sj[k];
+        if (sr[k] > RR || (sr[k] == RR && si[k] >= 0 && (II < 0 || si[k] < II))) {
+            RR = sr[k];
+            II = si[k];
+        }
+    }
+    st->maxdx = q;
+    if (nj > 0) {
+        st->J = JJ;
+        st->rmax = RR;
+        st->ridx = II;
+    }
+    const int nan = st->nan, piv = st->piv, ferr = st->flow_err;
+    if (stop_tol > 0.0 && !ferr && !piv && (q < stop_tol || nan)) st->stop = 1;
+    if (host_out) {
+        host_out->maxdx = q;
+        host_out->J = nj > 0 ? JJ : st->J;
+        host_out->rmax = nj > 0 ? RR : st->rmax;
+        host_out->ridx = nj > 0 ? II : st->ridx;
+        host_out->nan = nan;
+        host_out->piv = piv;
+        host_out->flow_err = ferr;
+        // the sequence number last, after the fields are visible to the host
+        const uint32_t sq = st->seq + 1;
+        st->seq = sq;
+        __threadfence_system();
+        __hip_atomic_store(&host_out->seq, sq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -2947,7 +3266,49 @@ struct PiPlan {
     std::vector<int64_t> suoff, surow;
     int bin_off[kSubBins + 1] = {0, 0, 0, 0, 0};
     size_t bin_bytes[kSubBins] = {0, 0, 0, 0};
+    // the bottom-subtree products' wave schedule (SubPack)
+    std::vector<int32_t> pkind, psub;
 };
+
+// The wave schedule of the bottom-subtree products: subtrees of at most 16
+// rows four to a wave, of at most 32 rows two to a wave, the others one to a
+// wave; the waves largest first (by the entries of their largest subtree).
+static void make_sub_pack(PiPlan& P) {
+    struct Wave {
+        int64_t cost;
+        int kind;
+        int32_t sub[4];
+    };
+    std::vector<Wave> waves;
+    Wave open[2] = {{0, 16, {-1, -1, -1, -1}}, {0, 32, {-1, -1, -1, -1}}};
+    int fill[2] = {0, 0};
+    const int cap[2] = {4, 2};
+    for (int t = 0; t < P.nsub0; ++t) {  // the plan lists them largest first
+        const int R = P.sR[t];
+        const int64_t e = sub_col_off(P.sn_n[t], R);
+        const int c = R <= 16 ? 0 : (R <= 32 ? 1 : -1);
+        if (c < 0) {
+            waves.push_back({e, kWave, {t, -1, -1, -1}});
+            continue;
+        }
+        if (fill[c] == 0) open[c].cost = e;
+        open[c].sub[fill[c]++] = t;
+        if (fill[c] == cap[c]) {
+            waves.push_back(open[c]);
+            open[c] = {0, c ? 32 : 16, {-1, -1, -1, -1}};
+            fill[c] = 0;
+        }
+    }
+    for (int c = 0; c < 2; ++c)
+        if (fill[c]) waves.push_back(open[c]);
+    std::stable_sort(waves.begin(), waves.end(), [](const Wave& a, const Wave& b) { return a.cost > b.cost; });
+    P.pkind.clear();
+    P.psub.clear();
+    for (const Wave& w : waves) {
+        P.pkind.push_back(w.kind);
+        P.psub.insert(P.psub.end(), w.sub, w.sub + 4);
+    }
+}
 
 // Bottom subtrees as dense solve matrices: for every subtree root (a front
 // outside the top whose parent is in the top, or a root of the tree) list its
@@ -3050,6 +3411,7 @@ static void make_sub_plan(const Symbolic& S, const std::vector<int8_t>& lev, PiP
     }
     P.bin_off[kSubBins] = (int)P.bins.size();
     P.dense = true;
+    make_sub_pack(P);
 }
 
 constexpr int kPiSeg = kPiSegMax;  // entries per forward-product segment
@@ -3365,6 +3727,12 @@ struct Instance::Impl {
         DBuf<int64_t> fdoff, smoff, suoff, surow;
         int bin_off[kSubBins + 1] = {0, 0, 0, 0, 0};
         size_t bin_bytes[kSubBins] = {0, 0, 0, 0};
+        // subtrees of each level (bottom, middle) by rows R (<= 16, 32, 64, 128)
+        // and by columns n (<= 8, 16, 32, 64), for the plan description
+        int rhist[2][4] = {}, nhist[2][4] = {};
+        // the bottom-subtree products' wave schedule (small subtrees packed)
+        DBuf<int32_t> pkind, psub;
+        int npw = 0;
         PiSizes sizes() const {
             PiSizes z;
             if (ok) {
@@ -3382,6 +3750,7 @@ struct Instance::Impl {
     // middle subtree products: waves per subtree (1: one wave each, k_sub_fwd /
     // k_sub_bwd; 4, 8, 16: one workgroup each; 0: by precision), per direction
     int sub_wg_fwd = 0, sub_wg_bwd = 0;
+    int sub_pack = -1;  // bottom-subtree products: small subtrees two or four to a wave (1), not (0), by precision (-1)
     bool meas_by_type = true;  // measurement rows grouped by evaluation class (else by anchor bus only)
     // right-hand side by location (needs the class grouping)
     bool rhs_loc = true;  // asked for
@@ -3402,8 +3771,8 @@ struct Instance::Impl {
     // this instance is (re)allocated (its ledger's epoch)
     struct GraphSlot {
         uint64_t alloc_epoch, plan_epoch;
-        int prec, robust, newton, refactor;
-        double gamma, vmin, vmax;
+        int prec, robust, newton, refactor, closing, lead;
+        double gamma, vmin, vmax, tol;
         hipGraphExec_t exec;
     };
     std::vector<GraphSlot> graphs;
@@ -3415,7 +3784,13 @@ struct Instance::Impl {
     uint32_t epoch = 0;
     DBuf<uint32_t> done;
     DBuf<int32_t> order_fwd, order_bwd, sn_parent;
-    bool results_valid = false;
+    // the last solve's injections, flows, h(x) and residuals are not evaluated
+    // yet (eval_outputs); zsnap holds z as that solve saw it once telemetry
+    // has changed since
+    double ahead_ratio = 0.05;  // last contraction of consecutive steps on the cached factor
+    bool outputs_pending = false;
+    bool zsnap_valid = false;
+    DBuf<double> zsnap;
     // profiling events per phase
     hipEvent_t pev[PH_COUNT][2] = {};
     bool pev_used[PH_COUNT] = {};
@@ -3458,6 +3833,9 @@ static void build_factor_plan(Instance::Impl& D, const Symbolic& S, int cut, int
 static void build_pi(Instance::Impl& D, const Symbolic& S, Ledger& bytes, hipStream_t s) {
     // partitioned-inverse top solve: bottom LDS tasks of at most pi_cut fronts
     D.pi.ok = false;
+    D.pi.npw = 0;
+    for (int lv = 0; lv < 2; ++lv)
+        for (int q = 0; q < 4; ++q) D.pi.rhist[lv][q] = D.pi.nhist[lv][q] = 0;
     if (S.nsn > 0 && S.max_c <= kFrontB) {
         const int cut = D.pi_cut > 0 ? D.pi_cut : kPiCutDefault;
         const size_t budget = 64 * 1024;
@@ -3553,8 +3931,17 @@ static void build_pi(Instance::Impl& D, const Symbolic& S, Ledger& bytes, hipStr
                 Q.nsub = PP.nsub;
                 Q.nsub0 = PP.nsub0;
                 Q.msub = PP.msub;
+                for (int t = 0; t < PP.nsub; ++t) {
+                    const int lv = t < PP.nsub0 ? 0 : 1;
+                    const int R = PP.sR[t], n = PP.sn_n[t];
+                    Q.rhist[lv][R <= 16 ? 0 : R <= 32 ? 1 : R <= 64 ? 2 : 3]++;
+                    Q.nhist[lv][n <= 8 ? 0 : n <= 16 ? 1 : n <= 32 ? 2 : 3]++;
+                }
                 up(Q.acptr, PP.acptr);
                 up64(Q.acpos, PP.acpos);
+                up(Q.pkind, PP.pkind);
+                up(Q.psub, PP.psub);
+                Q.npw = (int)PP.pkind.size();
                 for (int q = 0; q <= kSubBins; ++q) Q.bin_off[q] = PP.bin_off[q];
                 for (int q = 0; q < kSubBins; ++q) Q.bin_bytes[q] = PP.bin_bytes[q];
             }
@@ -3606,7 +3993,7 @@ Instance::~Instance() {
     rel(d_->yG); rel(d_->yB); rel(d_->yGt); rel(d_->yBt); rel(d_->Pij); rel(d_->Qij); rel(d_->V); rel(d_->th); rel(d_->P); rel(d_->Q);
     rel(d_->Pf); rel(d_->Qf); rel(d_->Pt); rel(d_->Qt); rel(d_->mcode); rel(d_->mloc); rel(d_->hrow);
     rel(d_->hs); rel(d_->hcolp); rel(d_->cg_b); rel(d_->cg_x); rel(d_->cg_r); rel(d_->cg_p); rel(d_->cg_q);
-    rel(d_->cg_z); rel(d_->cg_t); rel(d_->cg_parts); rel(d_->cg_s); rel(d_->z); rel(d_->sigma); rel(d_->rs); rel(d_->hx); rel(d_->r); rel(d_->jpart); rel(d_->rpart); rel(d_->vsave); rel(d_->thsave);
+    rel(d_->cg_z); rel(d_->cg_t); rel(d_->cg_parts); rel(d_->cg_s); rel(d_->z); rel(d_->sigma); rel(d_->rs); rel(d_->hx); rel(d_->r); rel(d_->zsnap); rel(d_->jpart); rel(d_->rpart); rel(d_->vsave); rel(d_->thsave);
     rel(d_->mpart); rel(d_->ipart); rel(d_->tptr); rel(d_->tpos); rel(d_->trow);
     rel(d_->tout); rel(d_->ml_d); rel(d_->lm_d); rel(d_->lb_d); rel(d_->ztmp);
     if (d_->state_pin) EnginePool::give_back_pinned(d_->state_pin, d_->state_pin_bytes);
@@ -3628,7 +4015,7 @@ Instance::~Instance() {
         rel(P.segrow); rel(P.mrows); rel(P.cplen); rel(P.cjj); rel(P.cboff); rel(P.cpoff);
         rel(P.boff); rel(P.poff); rel(P.pos); rel(P.segq); rel(P.segc); rel(P.cpos); rel(P.part);
         rel(P.sroot); rel(P.sn_n); rel(P.sR); rel(P.fptr); rel(P.fsn); rel(P.fdcol); rel(P.drow); rel(P.scptr);
-        rel(P.scols); rel(P.bins); rel(P.fdoff); rel(P.smoff); rel(P.suoff); rel(P.surow); rel(P.acptr); rel(P.acpos);
+        rel(P.scols); rel(P.bins); rel(P.fdoff); rel(P.smoff); rel(P.suoff); rel(P.surow); rel(P.acptr); rel(P.acpos); rel(P.pkind); rel(P.psub);
         for (auto& L : P.bottom) {
             rel(L.th); rel(L.meta); rel(L.lvl); rel(L.ch); rel(L.rm); rel(L.src); rel(L.mid);
         }
@@ -3921,10 +4308,16 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
         D.rs.alloc(std::max(1, h.m), bytes_);
         D.hx.alloc(std::max(1, h.m), bytes_);
         D.r.alloc(std::max(1, h.m), bytes_);
-        int nparts = nblk(std::max(1, h.m));
-        D.jpart.alloc(nparts, bytes_);
-        D.rpart.alloc(nparts, bytes_);
-        D.ipart.alloc(nparts, bytes_);
+        D.zsnap.alloc(std::max(1, h.m), bytes_);
+    }
+    {
+        // J partials: one per block of rows (k_meas) or of buses and branches (k_meas_loc)
+        const size_t nparts = std::max(nblk(std::max(1, h.m)), nblk((int64_t)h.n + h.nb));
+        if (D.jpart.n != nparts) {
+            D.jpart.alloc(nparts, bytes_);
+            D.rpart.alloc(nparts, bytes_);
+            D.ipart.alloc(nparts, bytes_);
+        }
     }
     D.mpart.alloc(nblk(std::max(1, h.n)), bytes_);
     D.hs.alloc(std::max<size_t>(1, S.hcol.size()), bytes_);
@@ -4140,7 +4533,8 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
     fp64_factor_active_ = false;
     d_->mixed64_off = false;
     factor_valid_ = false;
-    d_->results_valid = false;
+    d_->outputs_pending = false;
+    d_->zsnap_valid = false;
     if (nY) k_ybus<<<nblk(nY), kBlock, 0, s>>>((int)nY, D.ycptr.p, D.ycon.p, D.gs.p, D.bs.p, D.bc.p, D.tap.p,
                                                 D.shift.p, D.status.p, D.yG.p, D.yB.p);
     if (nY && D.use_loc)
@@ -4158,12 +4552,18 @@ std::vector<typename V::value_type> take(const V& v, const std::vector<int32_t>&
 }
 }  // namespace
 
+namespace {
+void eval_outputs(Instance::Impl& D);
+void keep_z(Instance::Impl& D);
+}  // namespace
+
 void Instance::update_branches(const HostArrays& h) {
     ++d_->plan_epoch;  // launch plans change: captured graphs retire
     HIPCHECK(hipSetDevice(device_));
     StreamScope scope_(d_->stream);
     Impl& D = *d_;
     hipStream_t s = D.stream;
+    eval_outputs(D);
     // host sources stay alive until the stream has consumed them
     const std::vector<double> gs = take(h.gs, D.kl), bs = take(h.bs, D.kl), bc = take(h.bc, D.kl),
                               tap = take(h.tap, D.kl), shift = take(h.shift, D.kl);
@@ -4187,7 +4587,6 @@ void Instance::update_branches(const HostArrays& h) {
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(s));
     factor_valid_ = false;
-    D.results_valid = false;
 }
 
 // One breaker operation: branch k (insertion index) closed or open.  Staged on
@@ -4224,6 +4623,7 @@ void Instance::refresh_topology() {
     HIPCHECK(hipSetDevice(device_));
     StreamScope scope_(d_->stream);
     hipStream_t s = D.stream;
+    eval_outputs(D);
     D.status.upload(D.status_h, bytes_, s);
     D.mcode.upload(D.mcode_h, bytes_, s);
     const size_t nY = S_->ycol.size();
@@ -4235,7 +4635,6 @@ void Instance::refresh_topology() {
     HIPCHECK(hipStreamSynchronize(s));
     D.topo_staged = false;
     factor_valid_ = false;
-    D.results_valid = false;
 }
 
 // z arrives in insertion order; the device holds it in row layout order
@@ -4245,6 +4644,7 @@ void Instance::set_z(const double* z) {
     Impl& D = *d_;
     if (!D.m) return;
     if (D.ztmp.n != (size_t)D.m) D.ztmp.alloc(D.m, bytes_);
+    keep_z(D);
     D.ztmp.upload(z, D.m, D.stream);
     k_gather<<<nblk(D.m), kBlock, 0, D.stream>>>(D.m, D.ml_d.p, D.ztmp.p, D.z.p);
     HIPCHECK(hipGetLastError());
@@ -4278,6 +4678,7 @@ void Instance::set_z_indexed(const int32_t* idx, const double* val, int count) {
     }
     std::memcpy(D.pin_idx, idx, (size_t)count * sizeof(int32_t));
     std::memcpy(D.pin_val, val, (size_t)count * sizeof(double));
+    keep_z(D);
     k_scatter_z_mapped<<<nblk(count), kBlock, 0, D.stream>>>(count, D.pin_idx_dev, D.pin_val_dev, D.lm_d.p, D.z.p);
     HIPCHECK(hipGetLastError());
     D.pin_busy = true;
@@ -4292,6 +4693,7 @@ void Instance::set_state(const double* V, const double* th) {
         v[p] = V[D.bl[p]];
         t[p] = th[D.bl[p]];
     }
+    eval_outputs(D);
     D.V.upload(v.data(), D.n, D.stream);
     D.th.upload(t.data(), D.n, D.stream);
     HIPCHECK(hipStreamSynchronize(D.stream));
@@ -4333,6 +4735,7 @@ void Instance::get_state(double* V, double* th) {
 void Instance::flat_start() {
     HIPCHECK(hipSetDevice(device_));
     StreamScope scope_(d_->stream);
+    eval_outputs(*d_);
     k_flat<<<nblk(d_->n), kBlock, 0, d_->stream>>>(d_->n, d_->bus_type.p, d_->vset.p, d_->V.p, d_->th.p);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(d_->stream));
@@ -4361,7 +4764,9 @@ void launch_power(Instance::Impl& D) {
                                                 D.Q.p, D.Pij.p, D.Qij.p, D.st.p);
 }
 
-void launch_meas_loc(Instance::Impl& D, int robust, double gamma) {
+// reset: the pass opens an iteration (clears its status flags); stats: J and
+// largest-residual partials per block into jpart / rpart / ipart
+void launch_meas_loc(Instance::Impl& D, int robust, double gamma, bool reset = true, bool stats = false) {
     LocArgs A;
     A.n = D.n;
     A.nb = D.nb;
@@ -4391,6 +4796,10 @@ void launch_meas_loc(Instance::Impl& D, int robust, double gamma) {
     A.Pout = D.P.p;
     A.Qout = D.Q.p;
     A.st = D.st.p;
+    A.reset = reset ? 1 : 0;
+    A.jpart = stats ? D.jpart.p : nullptr;
+    A.rpart = stats ? D.rpart.p : nullptr;
+    A.ipart = stats ? D.ipart.p : nullptr;
     const int64_t nt = (int64_t)D.n + D.nb;
     if (nt) k_meas_loc<<<nblk(nt), kBlock, 0, D.stream>>>(A);
 }
@@ -4400,11 +4809,11 @@ void launch_rhs_loc(Instance::Impl& D) {
     if (D.n)
         k_rhs_loc<T><<<nblk(D.n), kBlock, 0, D.stream>>>(D.n, D.yrow.p, D.ycol.p, D.yGt.p, D.yBt.p, D.P.p, D.Q.p,
                                                          D.V.p, D.th.p, D.sbus.p, D.sbr.p, D.inc_ptr.p, D.inc.p,
-                                                         D.pth.p, D.pv.p, bufs<T>(D).x.p);
+                                                         D.pth.p, D.pv.p, bufs<T>(D).x.p, D.st.p);
 }
 
 void launch_meas(Instance::Impl& D, int robust, double gamma, int unit_weights, int want_jac, bool want_out,
-                 bool want_rhs = false, bool newton = false) {
+                 bool want_rhs = false, bool newton = false, const double* z = nullptr) {
     MeasArgs A;
     A.m = D.m;
     A.n = D.n;
@@ -4416,7 +4825,7 @@ void launch_meas(Instance::Impl& D, int robust, double gamma, int unit_weights, 
     A.gamma = gamma;
     A.mcode = D.mcode.p;
     A.mloc = D.mloc.p;
-    A.z = D.z.p;
+    A.z = z ? z : D.z.p;
     A.sigma = D.sigma.p;
     A.hrow = D.hrow.p;
     A.yrow = D.yrow.p;
@@ -4445,6 +4854,30 @@ void launch_meas(Instance::Impl& D, int robust, double gamma, int unit_weights, 
     A.rpart = D.rpart.p;
     A.ipart = D.ipart.p;
     if (D.m) k_meas<<<nblk(D.m), kBlock, 0, D.stream>>>(A, D.hs.p);
+}
+
+// The last solve's outputs at its state and telemetry: injections, branch
+// flows, each row's h(x) and residual.  Evaluated on first request, or before
+// the state, the topology or the row activity changes.
+void eval_outputs(Instance::Impl& D) {
+    if (!D.outputs_pending) return;
+    launch_power(D);
+    if (D.nb)
+        k_flows<<<nblk(D.nb), kBlock, 0, D.stream>>>(D.nb, D.from.p, D.to.p, D.gs.p, D.bs.p, D.bc.p, D.tap.p,
+                                                     D.shift.p, D.status.p, D.V.p, D.th.p, D.Pf.p, D.Qf.p, D.Pt.p,
+                                                     D.Qt.p);
+    launch_meas(D, 0, 1.5, 0, 0, true, false, false, D.zsnap_valid ? D.zsnap.p : nullptr);
+    HIPCHECK(hipGetLastError());
+    D.outputs_pending = false;
+    D.zsnap_valid = false;
+}
+
+// Before z changes while the last solve's outputs are unevaluated: keep z as
+// that solve saw it (one device copy, only for the first change).
+void keep_z(Instance::Impl& D) {
+    if (!D.outputs_pending || D.zsnap_valid || !D.m) return;
+    HIPCHECK(hipMemcpyAsync(D.zsnap.p, D.z.p, (size_t)D.m * sizeof(double), hipMemcpyDeviceToDevice, D.stream));
+    D.zsnap_valid = true;
 }
 
 template <typename T>
@@ -4535,6 +4968,19 @@ SubArgs sub_args(const Instance::Impl& D) {
     return SubArgs{P.nsub,   P.msub,    P.sroot.p, P.sn_n.p,  P.sR.p,    P.fptr.p,  P.fsn.p, P.fdcol.p,
                    P.fdoff.p, P.drow.p, P.scptr.p, P.scols.p, P.smoff.p, P.acptr.p, P.acpos.p,
                    P.suoff.p, P.surow.p};
+}
+
+// Packed bottom-subtree products: by option, else by precision (measured at
+// 200k buses: FP32 solve data 20.2 -> 19.2 us forward, 20.2 -> 18.7 us
+// backward; FP64 slower packed, 26.5 -> 31.9 us backward)
+template <typename T>
+bool sub_packed(const Instance::Impl& D) {
+    if (!D.pi.npw) return false;
+    return D.sub_pack >= 0 ? D.sub_pack != 0 : sizeof(T) == sizeof(float);
+}
+
+SubPack sub_pack(const Instance::Impl& D) {
+    return SubPack{D.pi.npw, D.pi.pkind.p, D.pi.psub.p};
 }
 
 bool sub_active(const Instance::Impl& D) { return pi_active(D) && D.pi.dense && D.pi.nsub > 0; }
@@ -4672,7 +5118,11 @@ void launch_trisolve(Instance::Impl& D, Prof* pf = nullptr) {
             const int n0 = PI.nsub0, n1 = PI.nsub - PI.nsub0;
             if (n0) {
                 prof_begin(pf, PH_SUB_FWD);
-                k_sub_fwd<T, 0><<<(n0 + nw - 1) / nw, kBlock, 0, D.stream>>>(sub_args(D), A, B.msub.p, 0, n0);
+                if (sub_packed<T>(D))
+                    k_sub_fwd_pk<T><<<(PI.npw + nw - 1) / nw, kBlock, 0, D.stream>>>(sub_args(D), sub_pack(D), A,
+                                                                                     B.msub.p);
+                else
+                    k_sub_fwd<T, 0><<<(n0 + nw - 1) / nw, kBlock, 0, D.stream>>>(sub_args(D), A, B.msub.p, 0, n0);
                 prof_end(pf, PH_SUB_FWD, 1);
             }
             if (n1) {
@@ -4725,7 +5175,11 @@ void launch_trisolve(Instance::Impl& D, Prof* pf = nullptr) {
             }
             if (n0) {
                 prof_begin(pf, PH_SUB_BWD);
-                k_sub_bwd<T, 0><<<(n0 + nw - 1) / nw, kBlock, 0, D.stream>>>(sub_args(D), A, Mrow, 0, n0);
+                if (sub_packed<T>(D))
+                    k_sub_bwd_pk<T><<<(PI.npw + nw - 1) / nw, kBlock, 0, D.stream>>>(sub_args(D), sub_pack(D), A,
+                                                                                     Mrow);
+                else
+                    k_sub_bwd<T, 0><<<(n0 + nw - 1) / nw, kBlock, 0, D.stream>>>(sub_args(D), A, Mrow, 0, n0);
                 prof_end(pf, PH_SUB_BWD, 1);
             }
         }
@@ -4821,11 +5275,25 @@ void launch_rhs(Instance::Impl& D) {
 
 // dx source: the factor-precision solve vector, or the FP64 CG solution
 template <typename T>
+void launch_state_update(Instance::Impl& D, double vmin, double vmax, const T* dx, int skip_on_pivot) {
+    k_update<T><<<nblk(D.n), kBlock, 0, D.stream>>>(D.n, D.pth.p, D.pv.p, dx, D.V.p, D.th.p, vmin, vmax, D.mpart.p,
+                                                    D.st.p, skip_on_pivot);
+}
+
+// The iteration's status: max |dx| of the update and, with stats, J and the
+// largest residual from the measurement pass at the updated state.  stop_tol:
+// see k_maxfinish.
+void launch_finish(Instance::Impl& D, bool stats, double stop_tol = -1.0) {
+    const int nj = stats ? nblk((int64_t)D.n + D.nb) : 0;
+    k_maxfinish<<<1, kFinish, 0, D.stream>>>(nblk(D.n), D.mpart.p, D.st.p, D.hst_dev, nj, D.jpart.p, D.rpart.p,
+                                             D.ipart.p, stop_tol);
+}
+
+// dx source: the factor-precision solve vector, or the FP64 CG solution
+template <typename T>
 void launch_update(Instance::Impl& D, double vmin, double vmax, const T* dx, int skip_on_pivot) {
-    int nbk = nblk(D.n);
-    k_update<T><<<nbk, kBlock, 0, D.stream>>>(D.n, D.pth.p, D.pv.p, dx, D.V.p, D.th.p, vmin, vmax, D.mpart.p, D.st.p,
-                                              skip_on_pivot);
-    k_maxfinish<<<1, kBlock, 0, D.stream>>>(nbk, D.mpart.p, D.st.p, D.hst_dev);
+    launch_state_update<T>(D, vmin, vmax, dx, skip_on_pivot);
+    launch_finish(D, false);
 }
 
 void launch_dot(Instance::Impl& D, const double* a, const double* b, double* out) {
@@ -4917,25 +5385,37 @@ void prof_end(Prof* pf, int ph, int launches) {
     if (pf) pf->end(ph, launches);
 }
 
+// One Gauss-Newton iteration.  closing (iterations on the cached factor with
+// the right-hand side by location): the iteration ends with the measurement
+// pass at the state it leaves -- the residuals and right-hand-side sums of the
+// next iteration, and J and the largest residual of that state, reduced into
+// the status -- so the next iteration starts at its right-hand side, and a
+// converged solve needs no separate pass for J.  lead: the sums at the current
+// state are not there yet (first such iteration of a solve, or after any
+// other kind of iteration): the iteration opens with the pass as well.
 template <typename T>
 void iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p, bool refactor, T piv_tol, Prof& pf,
-               bool newton) {
+               bool newton, bool closing = false, bool lead = true) {
     // row-major Jacobian only where G or the CG product needs it; the RHS
     // products always (by location when no Jacobian values are needed: the
     // injections are then computed inside k_meas_loc)
     const bool direct = p.pcg_steps <= 0;
     const bool loc = D.use_loc && !refactor && direct;
+    closing = closing && loc;
     if (!loc) {
         pf.begin(PH_POWER);
         launch_power(D);
         pf.end(PH_POWER, 1);
     }
-    pf.begin(PH_MEAS);
-    if (loc)
-        launch_meas_loc(D, p.robust ? 1 : 0, p.gamma);
-    else
-        launch_meas(D, p.robust ? 1 : 0, p.gamma, 0, (refactor || !direct) ? 1 : 0, false, true, newton);
-    pf.end(PH_MEAS, 1);
+    if (!closing || lead) {
+        const int ph = closing ? PH_MEAS_LEAD : PH_MEAS;
+        pf.begin(ph);
+        if (loc)
+            launch_meas_loc(D, p.robust ? 1 : 0, p.gamma);
+        else
+            launch_meas(D, p.robust ? 1 : 0, p.gamma, 0, (refactor || !direct) ? 1 : 0, false, true, newton);
+        pf.end(ph, 1);
+    }
     if (refactor) {
         pf.begin(PH_SPGEMM);
         launch_spgemm<T>(D, S);
@@ -4960,21 +5440,35 @@ void iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p, bool 
         launch_rhs<T>(D);
     pf.end(PH_RHS, 1);
     launch_trisolve<T>(D, &pf);
+    const int skip_on_pivot = sizeof(T) == sizeof(double) ? 1 : 0;
+    if (!closing) {
+        pf.begin(PH_UPDATE);
+        launch_update<T>(D, p.vmin, p.vmax, bufs<T>(D).x.p, skip_on_pivot);
+        pf.end(PH_UPDATE, 2);
+        return;
+    }
     pf.begin(PH_UPDATE);
-    launch_update<T>(D, p.vmin, p.vmax, bufs<T>(D).x.p, sizeof(T) == sizeof(double) ? 1 : 0);
-    pf.end(PH_UPDATE, 2);
+    launch_state_update<T>(D, p.vmin, p.vmax, bufs<T>(D).x.p, skip_on_pivot);
+    pf.end(PH_UPDATE, 1);
+    pf.begin(PH_MEAS);
+    launch_meas_loc(D, p.robust ? 1 : 0, p.gamma, false, true);
+    pf.end(PH_MEAS, 1);
+    pf.begin(PH_FINISH);
+    launch_finish(D, true, p.tol);
+    pf.end(PH_FINISH, 1);
 }
 
 // One Gauss-Newton iteration (direct solve) as a HIP graph: the launches of
 // iteration() -- on the cached factor, or with the gain product, the
 // factorization and its solve data when it refactors (PRECISION, Huber, the
 // batched areas) -- are captured once per variant and replayed with one
-// launch; the status read-back follows it on the stream.  The captured work
-// holds no memset node and no host-varying argument (the dependency-driven
-// sweeps, whose epochs change per launch, never come here).
+// launch.  The captured work holds no memset node and no host-varying argument
+// (the dependency-driven sweeps, whose epochs change per launch, never come
+// here).
 template <typename T>
-void graph_iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p, T piv_tol, bool newton,
-                     bool refactor, uint64_t ae, SolveOutcome& out) {
+hipGraphExec_t iteration_graph(Instance::Impl& D, const Symbolic& S, const SolveParams& p, T piv_tol, bool newton,
+                               bool refactor, uint64_t ae, SolveOutcome& out, bool closing, bool lead,
+                               hipGraphExec_t running = nullptr) {
     const uint64_t pe = D.plan_epoch;
     const int prec = (int)sizeof(T);
     hipGraphExec_t exec = nullptr;
@@ -4986,7 +5480,8 @@ void graph_iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p,
             continue;
         }
         if (g.prec == prec && g.robust == p.robust && g.newton == (int)newton && g.refactor == (int)refactor &&
-            g.gamma == p.gamma && g.vmin == p.vmin && g.vmax == p.vmax)
+            g.closing == (int)closing && g.lead == (int)lead && g.gamma == p.gamma && g.vmin == p.vmin &&
+            g.vmax == p.vmax && g.tol == p.tol)
             exec = g.exec;
         ++k;
     }
@@ -4995,7 +5490,7 @@ void graph_iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p,
         hipGraph_t graph = nullptr;
         HIPCHECK(hipStreamBeginCapture(D.stream, hipStreamCaptureModeThreadLocal));
         try {
-            iteration<T>(D, S, p, refactor, piv_tol, off, newton);
+            iteration<T>(D, S, p, refactor, piv_tol, off, newton, closing, lead);
         } catch (...) {
             (void)hipStreamEndCapture(D.stream, &graph);
             if (graph) (void)hipGraphDestroy(graph);
@@ -5005,23 +5500,31 @@ void graph_iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p,
         const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         HIPCHECK(e);
-        if (D.graphs.size() >= 12) {
-            (void)hipGraphExecDestroy(D.graphs.front().exec);
-            D.graphs.erase(D.graphs.begin());
+        if (D.graphs.size() >= 12) {  // the oldest one not queued on the stream right now goes
+            const size_t k = D.graphs.front().exec == running ? 1 : 0;
+            (void)hipGraphExecDestroy(D.graphs[k].exec);
+            D.graphs.erase(D.graphs.begin() + k);
         }
-        D.graphs.push_back({ae, pe, prec, p.robust, (int)newton, (int)refactor, p.gamma, p.vmin, p.vmax, exec});
+        D.graphs.push_back({ae, pe, prec, p.robust, (int)newton, (int)refactor, (int)closing, (int)lead, p.gamma,
+                            p.vmin, p.vmax, p.tol, exec});
         out.graph_captures++;
     }
-    // k_maxfinish, the graph's last kernel, writes the status into D.hst and
-    // then the next sequence number: the host waits for that number instead
-    // of a stream synchronization, which wakes it several microseconds later
-    // (the GPU idles between iterations meanwhile).  A status never seen within
-    // the spin budget falls back to the synchronization.
+    return exec;
+}
+
+// k_maxfinish, a graph's last kernel, writes the status into D.hst and then
+// the next sequence number: the host waits for that number instead of a
+// stream synchronization, which wakes it several microseconds later (the GPU
+// idles meanwhile unless the next iteration is already queued).  A status
+// never seen within the spin budget falls back to the synchronization; the
+// return value is the sequence number then found (want, or beyond it when an
+// iteration queued behind has published too).
+uint32_t published_seq(const Instance::Impl& D) { return *(volatile const uint32_t*)&D.hst->seq; }
+
+uint32_t wait_seq(Instance::Impl& D, uint32_t want, SolveOutcome& out) {
     volatile uint32_t* seq = &D.hst->seq;
-    const uint32_t want = *seq + 1;
-    HIPCHECK(hipGraphLaunch(exec, D.stream));
     const auto t0 = std::chrono::steady_clock::now();
-    for (uint32_t spin = 0; *seq != want; ++spin) {
+    for (uint32_t spin = 0; (int32_t)(*seq - want) < 0; ++spin) {
         if ((spin & 1023) == 1023 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(200)) {
             HIPCHECK(hipStreamSynchronize(D.stream));
             out.seq_fallbacks++;
@@ -5029,6 +5532,7 @@ void graph_iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p,
         }
     }
     std::atomic_thread_fence(std::memory_order_acquire);
+    return *seq;
 }
 
 }  // namespace
@@ -5041,62 +5545,146 @@ SolveOutcome Instance::solve(const SolveParams& p) {
     SolveOutcome out;
     Prof pf{D, profiling_};
     auto t0 = std::chrono::steady_clock::now();
-    HIPCHECK(hipEventRecord(D.ev0, D.stream));
+    // a new estimate supersedes the previous solve's unevaluated outputs
+    D.outputs_pending = false;
+    D.zsnap_valid = false;
     if (p.flat_start) k_flat<<<nblk(D.n), kBlock, 0, D.stream>>>(D.n, D.bus_type.p, D.vset.p, D.V.p, D.th.p);
     int retries = 0;
     bool retry = false;
+    // the last iteration was closed by a measurement pass at the current state:
+    // the next one on the cached factor can start at its right-hand side, and
+    // the status holds J and the largest residual of this state
+    bool closed = false;
     do {
         if (retry) {
             k_flat<<<nblk(D.n), kBlock, 0, D.stream>>>(D.n, D.bus_type.p, D.vset.p, D.V.p, D.th.p);
             factor_valid_ = false;
             retry = false;
+            closed = false;
         }
         bool converged = false, diverged = false, singular = false, timed_out = false;
         int stalls = 0;
         out.iterations = 0;
         out.max_mismatch = 0;
         double last_step = 1e300;
+        double prev_step = 0.0;  // the step before last_step, 0 when unknown
         bool newton_off = false;
         bool stale = false;  // REALTIME: the cached factor contracts too slowly, refresh it
+        // Iterations on the cached factor that close with the measurement pass
+        // are launched one ahead: while iteration it runs, it + 1 is queued
+        // behind it unless it is predicted to converge, so that the GPU does
+        // not wait for the host between them.  If it does converge, the queued
+        // iteration finds the device's stop flag and changes nothing.
+        bool ahead = false;   // iteration it was launched by the previous pass
+        bool ahead_fp64 = false;
+        hipGraphExec_t ahead_exec = nullptr;
+        uint32_t ahead_want = 0;
         for (int it = 0; it < p.max_iter; ++it) {
-            const bool refactor = p.robust || !factor_valid_ || p.mode == 1 || stale;
-            // mixed form on the FP64 factor: refactor iterations in FP64
             const bool split = precision_ == 0 && !fp64_factor_active_ && D.mixed64 && !D.mixed64_off;
-            const bool fp64 = precision_ == 1 || fp64_factor_active_ || (split && refactor);
-            stale = false;
-            // Huber: the IRLS gain until the steps are small, then reduced gain
-            // weights of the outliers (same minimiser, faster contraction)
-            const bool newton = p.robust == 2 && !newton_off && last_step < kNewtonSwitch;
-            // every direct-solve iteration replays a captured graph (one per variant)
-            const bool graph = use_graphs_ && !profiling_ && !D.use_flow && (fp64 || p.pcg_steps <= 0);
-            if (fp64) {
-                if (!D.f64.ready) D.f64.alloc(S, bytes_, D.pi.sizes());
-                SolveParams pd = p;
-                pd.pcg_steps = 0;  // the FP64 factor solves the Newton system directly
-                if (split) {  // kept in case this factor turns out singular (below)
-                    if (D.vsave.n != D.V.n) D.vsave.alloc(D.V.n, bytes_);
-                    if (D.thsave.n != D.th.n) D.thsave.alloc(D.th.n, bytes_);
-                    HIPCHECK(hipMemcpyAsync(D.vsave.p, D.V.p, D.V.n * sizeof(double), hipMemcpyDeviceToDevice,
-                                            D.stream));
-                    HIPCHECK(hipMemcpyAsync(D.thsave.p, D.th.p, D.th.n * sizeof(double), hipMemcpyDeviceToDevice,
-                                            D.stream));
+            bool refactor, fp64, newton, graph, closing, lead;
+            uint32_t want = 0;  // sequence number of this graph iteration's status
+            hipGraphExec_t running = ahead ? ahead_exec : nullptr;
+            if (ahead) {
+                refactor = false;
+                fp64 = ahead_fp64;
+                newton = false;
+                graph = closing = true;
+                lead = false;
+                want = ahead_want;
+                ahead = false;
+            } else {
+                refactor = p.robust || !factor_valid_ || p.mode == 1 || stale;
+                // mixed form on the FP64 factor: refactor iterations in FP64
+                fp64 = precision_ == 1 || fp64_factor_active_ || (split && refactor);
+                stale = false;
+                // Huber: the IRLS gain until the steps are small, then reduced gain
+                // weights of the outliers (same minimiser, faster contraction)
+                newton = p.robust == 2 && !newton_off && last_step < kNewtonSwitch;
+                // every direct-solve iteration replays a captured graph (one per variant)
+                graph = use_graphs_ && !profiling_ && !D.use_flow && (fp64 || p.pcg_steps <= 0);
+                // iterations on the cached factor with the right-hand side by
+                // location close with the measurement pass at the state they leave
+                closing = D.use_loc && !refactor && (fp64 || p.pcg_steps <= 0);
+                lead = !closed;
+                if (fp64) {
+                    if (!D.f64.ready) D.f64.alloc(S, bytes_, D.pi.sizes());
+                    SolveParams pd = p;
+                    pd.pcg_steps = 0;  // the FP64 factor solves the Newton system directly
+                    if (split) {  // kept in case this factor turns out singular (below)
+                        if (D.vsave.n != D.V.n) D.vsave.alloc(D.V.n, bytes_);
+                        if (D.thsave.n != D.th.n) D.thsave.alloc(D.th.n, bytes_);
+                        HIPCHECK(hipMemcpyAsync(D.vsave.p, D.V.p, D.V.n * sizeof(double), hipMemcpyDeviceToDevice,
+                                                D.stream));
+                        HIPCHECK(hipMemcpyAsync(D.thsave.p, D.th.p, D.th.n * sizeof(double),
+                                                hipMemcpyDeviceToDevice, D.stream));
+                    }
+                    if (graph) {
+                        hipGraphExec_t g = iteration_graph<double>(D, S, pd, 1e-13, newton, refactor, bytes_.epoch,
+                                                                   out, closing, lead);
+                        running = g;
+                        want = published_seq(D) + 1;
+                        HIPCHECK(hipGraphLaunch(g, D.stream));
+                    } else {
+                        iteration<double>(D, S, pd, refactor, 1e-13, pf, newton, closing, lead);
+                    }
+                    if (split) {
+                        if (!D.f32.ready) D.f32.alloc(S, bytes_, D.pi.sizes());
+                        round_solve_data(D, bytes_);
+                    }
+                } else if (graph) {
+                    hipGraphExec_t g =
+                        iteration_graph<float>(D, S, p, 1e-5f, newton, refactor, bytes_.epoch, out, closing, lead);
+                    running = g;
+                    want = published_seq(D) + 1;
+                    HIPCHECK(hipGraphLaunch(g, D.stream));
+                } else {
+                    iteration<float>(D, S, p, refactor, 1e-5f, pf, newton, closing, lead);
                 }
-                if (graph)
-                    graph_iteration<double>(D, S, pd, 1e-13, newton, refactor, bytes_.epoch, out);
-                else
-                    iteration<double>(D, S, pd, refactor, 1e-13, pf, newton);
-                if (split) {
-                    if (!D.f32.ready) D.f32.alloc(S, bytes_, D.pi.sizes());
-                    round_solve_data(D, bytes_);
+            }
+            closed = closing;
+            HIPCHECK(hipGetLastError());
+            // queue the next iteration behind this one when it would be another
+            // closing graph iteration on the cached factor and this one is not
+            // predicted to converge (its step from the last two known steps,
+            // kAheadMargin tol at least)
+            if (graph && closing && it + 1 < p.max_iter && p.mode == 0 && !p.robust && factor_valid_ && !stale) {
+                const bool nfp64 = precision_ == 1 || fp64_factor_active_;
+                const bool ngraph = use_graphs_ && !profiling_ && !D.use_flow && (nfp64 || p.pcg_steps <= 0);
+                const bool nclosing = D.use_loc && (nfp64 || p.pcg_steps <= 0);
+                const double ratio = prev_step > 0.0 ? last_step / prev_step : D.ahead_ratio;
+                const bool first = it == 0 || last_step >= 1e299;
+                const double predicted = first ? 1e300 : last_step * std::min(ratio, 1.0);
+                bool in_time = true;
+                if (p.time_limit_ms > 0)
+                    in_time = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                                  .count() < p.time_limit_ms;
+                if (nfp64 == fp64 && ngraph && nclosing && in_time && predicted >= kAheadMargin * p.tol) {
+                    SolveParams pd = p;
+                    if (fp64) pd.pcg_steps = 0;
+                    hipGraphExec_t g =
+                        fp64 ? iteration_graph<double>(D, S, pd, 1e-13, false, false, bytes_.epoch, out, true, false,
+                                                       running)
+                             : iteration_graph<float>(D, S, pd, 1e-5f, false, false, bytes_.epoch, out, true, false,
+                                                      running);
+                    HIPCHECK(hipGraphLaunch(g, D.stream));
+                    ahead = true;
+                    ahead_exec = g;
+                    ahead_fp64 = fp64;
+                    ahead_want = want + 1;
+                    out.ahead++;
+                }
+            }
+            if (graph) {
+                const uint32_t seen = wait_seq(D, want, out);
+                if (ahead && (int32_t)(seen - ahead_want) >= 0) {
+                    // a fallback synchronization found the iteration queued behind
+                    // this one finished too: its status is the one in D.hst
+                    ahead = false;
+                    ++it;
                 }
             } else {
-                if (graph)
-                    graph_iteration<float>(D, S, p, 1e-5f, newton, refactor, bytes_.epoch, out);
-                else
-                    iteration<float>(D, S, p, refactor, 1e-5f, pf, newton);
+                HIPCHECK(hipStreamSynchronize(D.stream));  // status in D.hst (k_maxfinish)
             }
-            HIPCHECK(hipGetLastError());
-            if (!graph) HIPCHECK(hipStreamSynchronize(D.stream));  // status in D.hst (k_maxfinish)
             pf.collect(ktimes_);
             if (D.hst->piv && newton) {
                 // the gain with reduced outlier weights lost rank: IRLS weights from here on
@@ -5122,6 +5710,7 @@ SolveOutcome Instance::solve(const SolveParams& p) {
                 HIPCHECK(hipMemcpyAsync(D.th.p, D.thsave.p, D.th.n * sizeof(double), hipMemcpyDeviceToDevice,
                                         D.stream));
                 HIPCHECK(hipStreamSynchronize(D.stream));
+                closed = false;
                 D.mixed64_off = true;
                 factor_valid_ = false;
                 --it;
@@ -5144,6 +5733,9 @@ SolveOutcome Instance::solve(const SolveParams& p) {
             out.iterations = it + 1;
             const double prev = out.max_mismatch;
             out.max_mismatch = D.hst->maxdx;
+            // contraction of consecutive steps on the cached factor (the look-ahead's predictor)
+            if (!refactor && it > 0 && prev > 0.0) D.ahead_ratio = D.hst->maxdx / prev;
+            prev_step = (!refactor && it > 0) ? prev : 0.0;
             last_step = D.hst->maxdx;
             // a cached factor (REALTIME) taken at a point the state has since left
             // contracts slowly: refresh it at the current point for the next step
@@ -5175,7 +5767,7 @@ SolveOutcome Instance::solve(const SolveParams& p) {
                 converged = true;
                 break;
             }
-            if (p.mode == 0 && p.time_limit_ms > 0) {
+            if (p.mode == 0 && p.time_limit_ms > 0 && !ahead) {
                 double el = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
                 if (el > p.time_limit_ms) {
                     timed_out = true;
@@ -5183,6 +5775,9 @@ SolveOutcome Instance::solve(const SolveParams& p) {
                 }
             }
         }
+        // an iteration still queued here follows a converged or diverged one:
+        // the device's stop flag keeps it from changing anything
+        if (ahead) out.ahead_idle++;
         if (singular)
             out.status = 3;
         else if (diverged)
@@ -5198,26 +5793,30 @@ SolveOutcome Instance::solve(const SolveParams& p) {
         }
     } while (retry);
     out.retries = retries;
-    // quantities at the final state
-    launch_power(D);
-    if (D.nb)
-        k_flows<<<nblk(D.nb), kBlock, 0, D.stream>>>(D.nb, D.from.p, D.to.p, D.gs.p, D.bs.p, D.bc.p, D.tap.p,
-                                                     D.shift.p, D.status.p, D.V.p, D.th.p, D.Pf.p, D.Qf.p, D.Pt.p,
-                                                     D.Qt.p);
-    launch_meas(D, 0, p.gamma, 0, 0, true);
-    if (D.m) k_jfinish<<<1, kBlock, 0, D.stream>>>(nblk(D.m), D.jpart.p, D.rpart.p, D.ipart.p, D.st.p);
-    HIPCHECK(hipEventRecord(D.ev1, D.stream));
-    HIPCHECK(hipGetLastError());
-    read_status(D);
+    // J and the largest residual at the final state: from the status of an
+    // iteration that a measurement pass closed, else one pass now.  The rest
+    // of the outputs (injections, flows, h(x), residuals) are evaluated when
+    // first asked for (eval_outputs).
+    if (!closed) {
+        if (D.use_loc) {
+            launch_meas_loc(D, 0, p.gamma, true, true);
+            const int nj = nblk((int64_t)D.n + D.nb);
+            k_jfinish<<<1, kBlock, 0, D.stream>>>(nj, D.jpart.p, D.rpart.p, D.ipart.p, D.st.p);
+        } else {
+            launch_power(D);
+            launch_meas(D, 0, p.gamma, 0, 0, false);
+            if (D.m) k_jfinish<<<1, kBlock, 0, D.stream>>>(nblk(D.m), D.jpart.p, D.rpart.p, D.ipart.p, D.st.p);
+        }
+        HIPCHECK(hipGetLastError());
+        read_status(D);
+    }
+    D.outputs_pending = true;
     out.objective = D.m ? D.hst->J : 0.0;
     out.largest_residual = D.m ? D.hst->rmax : 0.0;
     out.largest_residual_idx = (D.m && D.hst->ridx >= 0) ? D.ml[D.hst->ridx] : -1;
-    float ms = 0;
-    HIPCHECK(hipEventElapsedTime(&ms, D.ev0, D.ev1));
-    out.ms = ms;
+    out.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     out.flags = (D.use_flow ? 1 : 0) | (fp64_factor_active_ ? 2 : 0) |
                 (precision_ == 0 && !fp64_factor_active_ && D.mixed64 ? 4 : 0);
-    D.results_valid = true;
     return out;
 }
 
@@ -5343,7 +5942,7 @@ struct ScratchLayout {
     size_t st = 0, mpart = 0, jpart = 0, rpart = 0, ipart = 0, total = 0;
 };
 
-ScratchLayout scratch_layout(int n, int m) {
+ScratchLayout scratch_layout(int n, int m, int nb) {
     auto up = [](size_t x) { return (x + 255) / 256 * 256; };
     ScratchLayout L;
     size_t off = 0;
@@ -5351,7 +5950,8 @@ ScratchLayout scratch_layout(int n, int m) {
     off += up(sizeof(DevStatus));
     L.mpart = off;
     off += up(sizeof(double) * (size_t)nblk(std::max(1, n)));  // k_update: one partial per block of buses
-    const size_t mp = (size_t)nblk(std::max(1, m));             // k_meas: one partial per block of rows
+    // J partials: one per block of rows (k_meas) or of buses and branches (k_meas_loc)
+    const size_t mp = (size_t)std::max(nblk(std::max(1, m)), nblk((int64_t)n + nb));
     L.jpart = off;
     off += up(sizeof(double) * mp);
     L.rpart = off;
@@ -5484,7 +6084,6 @@ bool Instance::leverages(double* lev, double* r) {
         if (lev) lev[D.ml[p]] = lv[p];
         if (r) r[D.ml[p]] = rr[p];
     }
-    D.results_valid = false;
     return true;
 }
 
@@ -5494,7 +6093,7 @@ SolveOutcome Instance::solve_no_alloc(const SolveParams& p) {
                            "(no PRECISION mode, no Huber, no CG refinement)");
     if (!factor_valid_) throw NoAllocError("no cached gain factor: solve once with sle_ext_Solve first");
     Impl& D = *d_;
-    const ScratchLayout L = scratch_layout(D.n, D.m);
+    const ScratchLayout L = scratch_layout(D.n, D.m, D.nb);
     if (L.total > kScratchBytes)
         throw NoAllocError("the model's reduction partials need " + std::to_string(L.total) +
                            " bytes of scratch; the allocation-free solve has " + std::to_string(kScratchBytes));
@@ -5547,6 +6146,7 @@ void Instance::get_results(double* P, double* Q, double* Pf, double* Qf, double*
     HIPCHECK(hipSetDevice(device_));
     StreamScope scope_(d_->stream);
     Impl& D = *d_;
+    eval_outputs(D);
     fetch_unpermute(D.P, D.lb, P, D.stream);
     fetch_unpermute(D.Q, D.lb, Q, D.stream);
     fetch_unpermute(D.Pf, D.lk, Pf, D.stream);
@@ -5651,7 +6251,6 @@ bool Instance::debug_rhs(bool by_location, double* b) {
     if (D.ns) HIPCHECK(hipMemcpyAsync(b, out.p, D.ns * sizeof(double), hipMemcpyDeviceToHost, D.stream));
     HIPCHECK(hipStreamSynchronize(D.stream));
     out.release(tmp);
-    D.results_valid = false;
     return true;
 }
 
@@ -5749,6 +6348,7 @@ void Instance::apply_staged(int slot) {
     HIPCHECK(hipSetDevice(device_));
     StreamScope scope_(d_->stream);
     const Impl::Staged& sb = D.staged[slot];
+    if (sb.count) keep_z(D);
     if (sb.count)
         k_scatter_z<<<nblk(sb.count), kBlock, 0, D.stream>>>(sb.count, sb.idx.p, sb.val.p, D.z.p);
     HIPCHECK(hipGetLastError());
@@ -5786,6 +6386,12 @@ void Instance::set_dense_subtrees(bool on) {
 void Instance::set_middle_subtrees(bool on) {
     ++d_->plan_epoch;
     d_->pi_mid = on;
+}
+
+void Instance::set_subtree_packing(int mode) {
+    if (mode < -1 || mode > 1) throw std::invalid_argument("subtree packing: -1 (by precision), 0 or 1");
+    ++d_->plan_epoch;
+    d_->sub_pack = mode;
 }
 
 void Instance::set_subtree_workgroups(int fwd, int bwd) {
@@ -5856,7 +6462,7 @@ void Instance::set_lds_waves(int bottom, int tail) {
 
 std::string Instance::sweep_plan() const {
     const Impl& D = *d_;
-    char buf[256];
+    char buf[512];
     if (D.use_flow) return "dependency-driven";
     std::string out;
     if (!(D.use_tasks && D.tasks_ok)) {
@@ -5872,9 +6478,13 @@ std::string Instance::sweep_plan() const {
         if (D.pi.dense)
             snprintf(buf, sizeof(buf),
                      "; solve partitioned-inverse(cut=%d dense-subtrees=%d+%d/%lld entries top supernodes=%d "
-                     "columns=%d blocks=%lld row-entries=%lld segments=%d)",
+                     "columns=%d blocks=%lld row-entries=%lld segments=%d; bottom rows<=16/32/64/128 %d/%d/%d/%d "
+                     "cols<=8/16/32/64 %d/%d/%d/%d; middle rows %d/%d/%d/%d cols %d/%d/%d/%d)",
                      D.pi.cut, D.pi.nsub0, D.pi.nsub - D.pi.nsub0, (long long)D.pi.msub, D.pi.ntop, D.pi.ntc,
-                     (long long)D.pi.nblock, (long long)D.pi.nq, D.pi.nseg);
+                     (long long)D.pi.nblock, (long long)D.pi.nq, D.pi.nseg, D.pi.rhist[0][0], D.pi.rhist[0][1],
+                     D.pi.rhist[0][2], D.pi.rhist[0][3], D.pi.nhist[0][0], D.pi.nhist[0][1], D.pi.nhist[0][2],
+                     D.pi.nhist[0][3], D.pi.rhist[1][0], D.pi.rhist[1][1], D.pi.rhist[1][2], D.pi.rhist[1][3],
+                     D.pi.nhist[1][0], D.pi.nhist[1][1], D.pi.nhist[1][2], D.pi.nhist[1][3]);
         else
             snprintf(buf, sizeof(buf),
                      "; solve partitioned-inverse(cut=%d bottom=%d/%zuB%s top supernodes=%d columns=%d blocks=%lld "

@@ -189,6 +189,10 @@ int Engine::set_option(const char* key, double value) {
         if (value != 0 && value != 1 && value != 4 && value != 8 && value != 16) return ST_INVALID_ARGUMENT;
         (k == "pinv_wg_fwd" ? opt_.pinv_wg_fwd : opt_.pinv_wg_bwd) = (int)value;
         if (inst_) inst_->set_subtree_workgroups(opt_.pinv_wg_fwd, opt_.pinv_wg_bwd);
+    } else if (k == "pinv_pack") {
+        if (value != -1 && value != 0 && value != 1) return ST_INVALID_ARGUMENT;
+        opt_.pinv_pack = (int)value;
+        if (inst_) inst_->set_subtree_packing(opt_.pinv_pack);
     } else if (k == "pinv_dense") {
         opt_.pinv_dense = value != 0;
         model_dirty_ = true;  // the plan is rebuilt at the next upload
@@ -703,6 +707,7 @@ int Engine::rebuild() {
     inst_->set_dense_subtrees(opt_.pinv_dense);
     inst_->set_middle_subtrees(opt_.pinv_mid);
     inst_->set_subtree_workgroups(opt_.pinv_wg_fwd, opt_.pinv_wg_bwd);
+    inst_->set_subtree_packing(opt_.pinv_pack);
     inst_->set_meas_by_type(opt_.meas_by_type);
     inst_->set_rhs_by_location(opt_.rhs_by_location);
     inst_->set_mixed_fp64_factor(opt_.mixed_fp64_factor);
@@ -1067,6 +1072,8 @@ void Engine::take_outcome(const SolveOutcome& o, const SolveParams& p, ResultOut
     out.graph_captures = o.graph_captures;
     out.seq_fallbacks = o.seq_fallbacks;
     out.retries = o.retries;
+    out.ahead = o.ahead;
+    out.ahead_idle = o.ahead_idle;
     out.bad_data_count = 0;
     last_ = out;
 }

@@ -78,6 +78,7 @@ struct ResultOut {
     int largest_residual_idx = -1, bad_data_count = 0;
     int refactors = 0, fp64_refactors = 0, flags = 0, pivots_substituted = 0, pcg_steps = 0;
     int graph_captures = 0, seq_fallbacks = 0, retries = 0;
+    int ahead = 0, ahead_idle = 0;
 };
 
 // Engine options beyond SLE_Config (sle_ext_SetOption).
@@ -96,6 +97,8 @@ struct EngineOptions {
     bool pinv_mid = true;         // and a middle level of dense subtrees under a smaller top
     int pinv_wg_fwd = 0;          // middle subtree products: waves per subtree, forward (1: one wave
     int pinv_wg_bwd = 0;          // each; 4, 8, 16: a workgroup each; 0: by precision) and backward
+    int pinv_pack = -1;           // bottom subtree products: small subtrees two or four to a wave
+                                  // (1), one a wave (0), by precision (-1: packed for FP32 solve data)
     bool xcd_remap = true;        // XCD-aware block order of the gather kernels
     bool meas_by_type = true;     // measurement rows grouped by evaluation class on the device
     bool rhs_by_location = true;  // cached-factor iterations: RHS from per-location sums

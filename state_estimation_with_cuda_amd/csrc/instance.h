@@ -57,12 +57,14 @@ struct SolveOutcome {
     double objective = 0;
     double largest_residual = 0;  // max |r|/sigma over active rows
     int largest_residual_idx = -1;
-    double ms = 0;
+    double ms = 0;  // wall time from the solve's start to its final status
     int flags = 0;  // bit 0: dependency-driven sweeps; bit 1: FP64 factor active (mixed form fell back)
     int newton_fallbacks = 0;  // Huber-Newton gains that lost rank (IRLS used instead)
     int graph_captures = 0;    // iteration graphs captured and instantiated during the solve
     int seq_fallbacks = 0;     // replayed iterations whose status the host did not see within the spin budget
     int retries = 0;           // flat-start restarts after divergence (PRECISION only)
+    int ahead = 0;             // iterations launched before the previous iteration's status was seen
+    int ahead_idle = 0;        // of those, iterations queued behind a converged one (they change nothing)
 };
 
 // Device bytes held by an instance, and a count of its allocations and
@@ -94,6 +96,8 @@ enum Phase : int {
     PH_SUB_BWD,
     PH_MID_FWD,
     PH_MID_BWD,
+    PH_MEAS_LEAD,  // the measurement pass that opens the first iteration of a solve on the cached factor
+    PH_FINISH,     // the iteration's status (max |dx|, J) when a measurement pass closes the iteration
     PH_COUNT
 };
 
@@ -224,6 +228,9 @@ class Instance {
     // middle subtree products: waves per subtree, forward and backward (1: one
     // wave each; 4, 8 or 16: one workgroup each; 0: by precision)
     void set_subtree_workgroups(int fwd, int bwd);
+    // bottom subtree products: subtrees of at most 32 (16) rows two (four) to a
+    // wave (1), one per wave (0), or by precision (-1: packed for FP32 solve data)
+    void set_subtree_packing(int mode);
     // XCD-aware block order of the gather kernels (process-wide on this device)
     void set_xcd_remap(bool on);
     // device order of the measurement rows: grouped by evaluation class, each

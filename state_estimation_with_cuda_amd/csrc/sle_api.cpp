@@ -860,6 +860,8 @@ SLE_StatusCode ext_solve(SLE_Handle handle, const SLE_SolverConfig* config, SLE_
             result->graph_captures = r.graph_captures;
             result->seq_fallbacks = r.seq_fallbacks;
             result->retries = r.retries;
+            result->ahead = r.ahead;
+            result->ahead_idle = r.ahead_idle;
             result->gpu_ms = r.ms;
             result->host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         }

@@ -115,7 +115,7 @@ class ExtResult(C.Structure):
                 ("largest_residual", C.c_double), ("largest_residual_idx", C.c_int32), ("flags", C.c_int32),
                 ("gpu_ms", C.c_double), ("host_ms", C.c_double), ("pivots_substituted", C.c_int32),
                 ("pcg_steps", C.c_int32), ("graph_captures", C.c_int32), ("seq_fallbacks", C.c_int32),
-                ("retries", C.c_int32)]
+                ("retries", C.c_int32), ("ahead", C.c_int32), ("ahead_idle", C.c_int32)]
 
 
 # every function of the reference ABI (sle_api.h:447-1391) and the extensions
@@ -148,7 +148,7 @@ EXT_FUNCTIONS = [
     "sle_ext_NormalizedResiduals", "sle_ext_SetMeasurementActive", "sle_ext_GainRhs",
 ]
 KERNEL_CLASSES = ["power", "meas", "spgemm", "factor", "rhs", "fwd", "bwd", "update", "pcg",
-                  "sub_fwd", "pi_fwd", "pi_bwd", "sub_bwd", "mid_fwd", "mid_bwd"]
+                  "sub_fwd", "pi_fwd", "pi_bwd", "sub_bwd", "mid_fwd", "mid_bwd", "meas_lead", "finish"]
 PLAN_SIZES = ["subtrees", "subtree_entries", "subtree_update_rows", "top_supernodes", "top_columns",
               "top_block_entries", "top_row_entries", "segments", "top_path_rows", "rhs_by_location",
               "bottom_subtrees", "bottom_entries", "bottom_update_rows", "bottom_columns"]
@@ -526,7 +526,7 @@ class Engine:
                     largest_residual=r.largest_residual, largest_residual_idx=r.largest_residual_idx, flags=r.flags,
                     pivots_substituted=r.pivots_substituted, pcg_steps=r.pcg_steps,
                     graph_captures=r.graph_captures, seq_fallbacks=r.seq_fallbacks, retries=r.retries,
-                    gpu_ms=r.gpu_ms, host_ms=r.host_ms)
+                    ahead=r.ahead, ahead_idle=r.ahead_idle, gpu_ms=r.gpu_ms, host_ms=r.host_ms)
 
     def normalized_residuals(self):
         """(rn, omega) per measurement: rn = |r| / sqrt(Omega_ii), Omega = R - H G^-1 H^T."""

@@ -372,6 +372,8 @@ PLAN_OPTIONS = {
     "middle_subtrees_4_waves": {"pinv_wg_fwd": 4, "pinv_wg_bwd": 4},
     "middle_subtrees_8_waves": {"pinv_wg_fwd": 8, "pinv_wg_bwd": 8},
     "middle_subtrees_16_waves": {"pinv_wg_fwd": 16, "pinv_wg_bwd": 16},
+    "bottom_subtrees_packed": {"pinv_pack": 1},
+    "bottom_subtrees_one_a_wave": {"pinv_pack": 0},
 }
 
 
@@ -415,6 +417,30 @@ def test_solve_plans_agree(plan, fp64):
     e.close()
 
 
+@pytest.mark.parametrize("fp64", [True, False])
+def test_packed_subtree_products_are_exact(fp64):
+    """The bottom-subtree products with the small subtrees two or four to a wave
+    (option pinv_pack) add the same terms in the same order as one subtree per
+    wave: factor solves agree bit for bit, in both precisions of the solve data."""
+    g = GRIDS["geo10k"]()
+    from state_estimation_with_cuda_amd import sle
+
+    b = np.random.default_rng(9).normal(size=2 * g.n - 1)
+    out = []
+    for pack in (0, 1):
+        e = sle.Engine(double=True)
+        e.load_grid(g)
+        e.set_config(use_double_precision=int(fp64))
+        e.set_option("pinv_pack", pack)
+        e.upload()
+        e.set_telemetry_f64(g.z)
+        e.set_state(*random_state(g, seed=4))
+        assert e.factor()[0] == 0
+        out.append(e.solve_factor(b))
+        e.close()
+    assert np.array_equal(out[0], out[1])
+
+
 _O10K = {}
 
 
@@ -454,9 +480,13 @@ def test_autotune_keeps_results():
 @pytest.mark.parametrize("fp64", [True, False])
 def test_graph_replays_match_eager_iterations(fp64):
     """REALTIME iterations on the cached factor replay a captured HIP graph
-    (config use_cuda_graphs): every replay, with eager work (telemetry updates,
-    state reads, a PRECISION solve that refactors) between the replays, gives
-    bit-identical states to the same requests run eagerly."""
+    (config use_cuda_graphs), the next one queued behind the running one: every
+    replay, with eager work (telemetry updates, state reads, a PRECISION solve
+    that refactors) between the replays, gives bit-identical states, J and
+    residuals to the same requests run eagerly one iteration at a time.  A
+    request repeated without new telemetry converges at its first iteration:
+    the iteration queued behind it changes nothing (the comparison stays
+    bit-identical, and the next request still iterates)."""
     g = GRIDS["geo10k"]()
     eg = engine_for(g, double_precision=fp64, use_cuda_graphs=1)
     ee = engine_for(g, double_precision=fp64, use_cuda_graphs=0)
@@ -464,20 +494,72 @@ def test_graph_replays_match_eager_iterations(fp64):
     for e in (eg, ee):
         r = e.solve(mode=1, max_iterations=50, tolerance=1e-6, flat_start=True)
         assert r["status"] == 0
-    for step in range(6):
-        idx = np.sort(rng.choice(g.m, g.m // 10, replace=False)).astype(np.int32)
-        val = g.z[idx] * (1.0 + 1e-3 * rng.standard_normal(idx.size))
+    ahead = idle = 0
+    for step in range(8):
+        repeat = step in (2, 6)  # no new telemetry: converged at the first iteration
+        if not repeat:
+            idx = np.sort(rng.choice(g.m, g.m // 10, replace=False)).astype(np.int32)
+            val = g.z[idx] * (1.0 + 1e-3 * rng.standard_normal(idx.size))
         out = []
         for e in (eg, ee):
-            e.set_telemetry_indexed(idx, val)
-            mode = 1 if step == 3 else 0  # one refactoring solve between the replays
+            if not repeat:
+                e.set_telemetry_indexed(idx, val)
+            mode = 1 if step == 4 else 0  # one refactoring solve between the replays
             r = e.solve(mode=mode, max_iterations=10, tolerance=1e-6)
-            out.append((r["status"], r["iterations"]) + e.state())
-        (sg, ig, Vg, tg), (se, ie, Ve, te) = out
+            assert r["seq_fallbacks"] == 0, r
+            out.append((r["status"], r["iterations"], r["objective"]) + e.state() + (e.results()["r"],))
+            if e is eg:
+                ahead += r["ahead"]
+                idle += r["ahead_idle"]
+                if repeat:
+                    assert r["iterations"] == 1 and r["ahead_idle"] == 1, (step, r)
+            else:
+                assert r["ahead"] == 0, r  # eager iterations wait for each status
+        (sg, ig, jg, Vg, tg, rg), (se, ie, je, Ve, te, re_) = out
         assert (sg, ig) == (se, ie), (step, out[0][:2], out[1][:2])
+        assert jg == je, (step, jg, je)
         assert np.array_equal(Vg, Ve) and np.array_equal(tg, te), step
+        assert np.array_equal(rg, re_), step
+    assert ahead > idle > 0, (ahead, idle)
     eg.close()
     ee.close()
+
+
+@pytest.mark.parametrize("fp64", [True, False])
+def test_results_stay_those_of_the_solve(fp64):
+    """Injections, flows, h(x) and residuals are evaluated on first request after
+    a solve: they are those of the solve's estimate and telemetry even when the
+    telemetry changes before they are asked for (reference: results copied to
+    the host after every solve, sle_engine.cu:661-664), and equal the oracle's
+    evaluation at that estimate."""
+    g = GRIDS["geo2k"]()
+    e = engine_for(g, double_precision=fp64)
+    assert e.solve(mode=1, max_iterations=50, tolerance=1e-8, flat_start=True)["status"] == 0
+    rng = np.random.default_rng(5)
+    idx = np.sort(rng.choice(g.m, g.m // 10, replace=False)).astype(np.int32)
+    z1 = g.z.copy()
+    z1[idx] += rng.standard_normal(idx.size) * g.sigma[idx]
+    e.set_telemetry_indexed(idx, z1[idx])
+    r = e.solve(mode=0, max_iterations=20, tolerance=1e-8)
+    assert r["status"] == 0, r
+    V, th = e.state()
+    # new telemetry (indexed, then a full upload) before the results are read
+    e.set_telemetry_indexed(idx, g.z[idx] + 7.0 * g.sigma[idx])
+    e.set_telemetry_f64(g.z * 1.01)
+    res = e.results()
+    g1 = gg.Grid(**{**g.__dict__})
+    g1.z = z1
+    ev = Oracle(g1).eval(V, th)
+    assert (g.status == 1).all()  # every row active: r = z - h(x) throughout
+    assert rel_err(res["h"], ev["h"]) < 1e-12
+    assert np.abs(res["r"] - (z1 - ev["h"])).max() < 1e-10
+    for k in ("P", "Q", "Pf", "Qf", "Pt", "Qt"):
+        assert rel_err(res[k], ev[k]) < 1e-12, k
+    # J reported by the solve is the one of that estimate and telemetry
+    wb = 1.0 / g.sigma**2
+    J = float(np.sum(wb * (z1 - ev["h"]) ** 2))
+    assert abs(r["objective"] - J) / J < 1e-12, (r["objective"], J)
+    e.close()
 
 
 @pytest.mark.parametrize("fp64", [True, False])
