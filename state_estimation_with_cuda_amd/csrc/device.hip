@@ -1887,13 +1887,27 @@ __global__ __launch_bounds__(kBlock) void k_sub_m(SubArgs B, SolveArgs<T> A, con
     }
 }
 
-// The same M_t with one wave per subtree and all columns at once: lane j
-// carries column j through the row-oriented substitution
+// The same M_t with all columns at once: lane j carries column j through the
+// row-oriented substitution
 //   X[i][j] = (delta_ij - sum_{k<i} L[i][k] X[k][j]) / L[i][i],   i < n,
 //   M[n + r][j] = -sum_{k<n} L[n + r][k] X[k][j],
 // whose inner sums are independent LDS loads (broadcast L, conflict-free X)
-// rather than a chain of cross-lane steps.  LDS: the dense L (R x n) and X
-// (n x n), both FP64.
+// rather than a chain of cross-lane steps.  The rows of X depend on each
+// other: one wave walks them; the update rows n + r do not: once X is there,
+// the block's waves share them.  LDS: the dense L (R x n) and X (n x n), FP64.
+template <typename T>
+__device__ inline double sub_m_row(const double* Ld, const double* X, int R, int n, int i, int kend, int jc) {
+    // eight independent partial sums keep sixteen LDS loads in flight
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int k = 0;
+    for (; k + 8 <= kend; k += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc[u] -= Ld[(k + u) * R + i] * X[(k + u) * n + jc];
+    }
+    for (; k < kend; ++k) acc[0] -= Ld[k * R + i] * X[k * n + jc];
+    return ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_sub_m_rows(SubArgs B, SolveArgs<T> A, const int32_t* __restrict__ list,
                                                       T* __restrict__ Mout) {
@@ -1917,28 +1931,26 @@ __global__ __launch_bounds__(kBlock) void k_sub_m_rows(SubArgs B, SolveArgs<T> A
         }
     }
     __syncthreads();
-    if (w != 0) return;  // one wave carries all columns through the substitution
     T* Mt = Mout + B.moff[t];
     T* Mr = Mout + B.msub + B.moff[t];
     const int j = lane;
     const int jc = j < n ? j : 0;  // lanes beyond the columns read column 0 and discard it
-    for (int i = 0; i < R; ++i) {
-        const int kend = i < n ? i : n;
-        // eight independent partial sums keep sixteen LDS loads in flight
-        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        int k = 0;
-        for (; k + 8 <= kend; k += 8) {
-#pragma unroll
-            for (int u = 0; u < 8; ++u) acc[u] -= Ld[(k + u) * R + i] * X[(k + u) * n + jc];
-        }
-        for (; k < kend; ++k) acc[0] -= Ld[k * R + i] * X[k * n + jc];
-        double v = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
-        if (i == j) v += 1.0;
-        if (i < n) {
+    if (w == 0) {  // the rows of X, in order
+        for (int i = 0; i < n; ++i) {
+            double v = sub_m_row<T>(Ld, X, R, n, i, i, jc);
+            if (i == j) v += 1.0;
             v = j <= i ? v / Ld[i * R + i] : 0.0;
             if (j < n) X[i * n + j] = v;
+            if (j < n && j <= i) {
+                Mt[sub_col_off(j, R) + i - j] = (T)v;
+                Mr[sub_row_off(i, n) + j] = (T)v;
+            }
         }
-        if (j < n && j <= i) {
+    }
+    __syncthreads();
+    for (int i = n + w; i < R; i += nw) {  // the update rows, shared by the waves
+        const double v = sub_m_row<T>(Ld, X, R, n, i, n, jc);
+        if (j < n) {
             Mt[sub_col_off(j, R) + i - j] = (T)v;
             Mr[sub_row_off(i, n) + j] = (T)v;
         }
@@ -2710,19 +2722,21 @@ __global__ __launch_bounds__(kFinish) void k_maxfinish(int nparts, const double*
     }
     const int nan = st->nan, piv = st->piv, ferr = st->flow_err;
     if (stop_tol > 0.0 && !ferr && !piv && (q < stop_tol || nan)) st->stop = 1;
-    if (host_out) {
-        host_out->maxdx = q;
-        host_out->J = nj > 0 ? JJ : st->J;
-        host_out->rmax = nj > 0 ? RR : st->rmax;
-        host_out->ridx = nj > 0 ? II : st->ridx;
-        host_out->nan = nan;
-        host_out->piv = piv;
-        host_out->flow_err = ferr;
-        // the sequence number last, after the fields are visible to the host
+    if (host_out) {  // two slots, written in turn
         const uint32_t sq = st->seq + 1;
         st->seq = sq;
+        DevStatus* ho = host_out + (sq & 1);
+        ho->maxdx = q;
+        ho->J = nj > 0 ? JJ : st->J;
+        ho->rmax = nj > 0 ? RR : st->rmax;
+        ho->ridx = nj > 0 ? II : st->ridx;
+        ho->nan = nan;
+        ho->piv = piv;
+        ho->flow_err = ferr;
+        ho->stop = st->stop;
+        // the sequence number last, after the fields are visible to the host
         __threadfence_system();
-        __hip_atomic_store(&host_out->seq, sq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&ho->seq, sq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -3674,7 +3688,14 @@ struct Instance::Impl {
     DBuf<int64_t> foff, rmoff, uoff, rows_off, moff;
     DBuf<DevStatus> st;
     DevStatus* hst = nullptr;      // pinned
-    DevStatus* hst_dev = nullptr;  // its device mapping (k_maxfinish writes the iteration status there)
+    DevStatus* hst_dev = nullptr;  // its device mapping
+    // two pinned status slots that k_maxfinish writes in turn (slot seq & 1),
+    // with their device mapping: the host reads one iteration's status while
+    // the next, already queued, writes the other; published = the last
+    // sequence number the host has taken
+    DevStatus* hslot = nullptr;
+    DevStatus* hslot_dev = nullptr;
+    uint32_t published = 0;
     PrecBufs<float> f32;
     PrecBufs<double> f64;
     // per-level launch lists: [level][class] -> (offset into lists, count)
@@ -3967,6 +3988,9 @@ Instance::Instance(int device, int precision) : device_(device), precision_(prec
     HIPCHECK(hipEventCreate(&d_->ev1));
     d_->hst = static_cast<DevStatus*>(EnginePool::pinned(sizeof(DevStatus)));
     HIPCHECK(hipHostGetDevicePointer((void**)&d_->hst_dev, d_->hst, 0));
+    d_->hslot = static_cast<DevStatus*>(EnginePool::pinned(2 * sizeof(DevStatus)));
+    HIPCHECK(hipHostGetDevicePointer((void**)&d_->hslot_dev, d_->hslot, 0));
+    std::memset(d_->hslot, 0, 2 * sizeof(DevStatus));
     d_->st.alloc(1, bytes_);
     // the iteration sequence starts at zero on both sides (outside any capture)
     HIPCHECK(hipMemsetAsync(d_->st.p, 0, sizeof(DevStatus), d_->stream));
@@ -4031,6 +4055,7 @@ Instance::~Instance() {
         for (int q = 0; q < 2; ++q)
             if (d_->pev[p][q]) (void)hipEventDestroy(d_->pev[p][q]);
     if (d_->hst) EnginePool::give_back_pinned(d_->hst, sizeof(DevStatus));
+    if (d_->hslot) EnginePool::give_back_pinned(d_->hslot, 2 * sizeof(DevStatus));
     if (d_->ev0) (void)hipEventDestroy(d_->ev0);
     if (d_->ev1) (void)hipEventDestroy(d_->ev1);
     if (d_->stream) {
@@ -5285,7 +5310,7 @@ void launch_state_update(Instance::Impl& D, double vmin, double vmax, const T* d
 // see k_maxfinish.
 void launch_finish(Instance::Impl& D, bool stats, double stop_tol = -1.0) {
     const int nj = stats ? nblk((int64_t)D.n + D.nb) : 0;
-    k_maxfinish<<<1, kFinish, 0, D.stream>>>(nblk(D.n), D.mpart.p, D.st.p, D.hst_dev, nj, D.jpart.p, D.rpart.p,
+    k_maxfinish<<<1, kFinish, 0, D.stream>>>(nblk(D.n), D.mpart.p, D.st.p, D.hslot_dev, nj, D.jpart.p, D.rpart.p,
                                              D.ipart.p, stop_tol);
 }
 
@@ -5512,27 +5537,33 @@ hipGraphExec_t iteration_graph(Instance::Impl& D, const Symbolic& S, const Solve
     return exec;
 }
 
-// k_maxfinish, a graph's last kernel, writes the status into D.hst and then
-// the next sequence number: the host waits for that number instead of a
-// stream synchronization, which wakes it several microseconds later (the GPU
-// idles meanwhile unless the next iteration is already queued).  A status
-// never seen within the spin budget falls back to the synchronization; the
-// return value is the sequence number then found (want, or beyond it when an
-// iteration queued behind has published too).
-uint32_t published_seq(const Instance::Impl& D) { return *(volatile const uint32_t*)&D.hst->seq; }
-
-uint32_t wait_seq(Instance::Impl& D, uint32_t want, SolveOutcome& out) {
-    volatile uint32_t* seq = &D.hst->seq;
-    const auto t0 = std::chrono::steady_clock::now();
-    for (uint32_t spin = 0; (int32_t)(*seq - want) < 0; ++spin) {
-        if ((spin & 1023) == 1023 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(200)) {
-            HIPCHECK(hipStreamSynchronize(D.stream));
-            out.seq_fallbacks++;
-            break;
+// k_maxfinish, an iteration's last kernel, writes the status into slot
+// (seq & 1) of the pinned status block and then the sequence number: the host
+// waits for that number instead of a stream synchronization, which wakes it
+// several microseconds later (the GPU idles meanwhile unless the next
+// iteration is already queued).  A status never seen within the spin budget
+// falls back to the synchronization.  The status is copied out of its slot
+// at once; the next iteration writes the other slot.
+DevStatus take_status(Instance::Impl& D, uint32_t want, bool spin, SolveOutcome& out) {
+    DevStatus* slot = D.hslot + (want & 1);
+    volatile uint32_t* seq = &slot->seq;
+    if (spin) {
+        const auto t0 = std::chrono::steady_clock::now();
+        for (uint32_t k = 0; *seq != want; ++k) {
+            if ((k & 1023) == 1023 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(200)) {
+                HIPCHECK(hipStreamSynchronize(D.stream));
+                out.seq_fallbacks++;
+                break;
+            }
         }
     }
     std::atomic_thread_fence(std::memory_order_acquire);
-    return *seq;
+    DevStatus cur;
+    std::memcpy(&cur, slot, sizeof(DevStatus));
+    if (cur.seq != want) throw GpuError("iteration status " + std::to_string(want) + " missing (slot holds " +
+                                        std::to_string(cur.seq) + ")");
+    D.published = want;
+    return cur;
 }
 
 }  // namespace
@@ -5553,8 +5584,9 @@ SolveOutcome Instance::solve(const SolveParams& p) {
     bool retry = false;
     // the last iteration was closed by a measurement pass at the current state:
     // the next one on the cached factor can start at its right-hand side, and
-    // the status holds J and the largest residual of this state
+    // its status (last) holds J and the largest residual of this state
     bool closed = false;
+    DevStatus last{};
     do {
         if (retry) {
             k_flat<<<nblk(D.n), kBlock, 0, D.stream>>>(D.n, D.bus_type.p, D.vset.p, D.V.p, D.th.p);
@@ -5622,7 +5654,7 @@ SolveOutcome Instance::solve(const SolveParams& p) {
                         hipGraphExec_t g = iteration_graph<double>(D, S, pd, 1e-13, newton, refactor, bytes_.epoch,
                                                                    out, closing, lead);
                         running = g;
-                        want = published_seq(D) + 1;
+                        want = D.published + 1;
                         HIPCHECK(hipGraphLaunch(g, D.stream));
                     } else {
                         iteration<double>(D, S, pd, refactor, 1e-13, pf, newton, closing, lead);
@@ -5635,7 +5667,7 @@ SolveOutcome Instance::solve(const SolveParams& p) {
                     hipGraphExec_t g =
                         iteration_graph<float>(D, S, p, 1e-5f, newton, refactor, bytes_.epoch, out, closing, lead);
                     running = g;
-                    want = published_seq(D) + 1;
+                    want = D.published + 1;
                     HIPCHECK(hipGraphLaunch(g, D.stream));
                 } else {
                     iteration<float>(D, S, p, refactor, 1e-5f, pf, newton, closing, lead);
@@ -5674,26 +5706,20 @@ SolveOutcome Instance::solve(const SolveParams& p) {
                     out.ahead++;
                 }
             }
-            if (graph) {
-                const uint32_t seen = wait_seq(D, want, out);
-                if (ahead && (int32_t)(seen - ahead_want) >= 0) {
-                    // a fallback synchronization found the iteration queued behind
-                    // this one finished too: its status is the one in D.hst
-                    ahead = false;
-                    ++it;
-                }
-            } else {
-                HIPCHECK(hipStreamSynchronize(D.stream));  // status in D.hst (k_maxfinish)
-            }
+            if (!graph) HIPCHECK(hipStreamSynchronize(D.stream));
+            // this iteration's status, copied out of its slot (an iteration
+            // queued behind writes the other one)
+            const DevStatus cur = take_status(D, graph ? want : D.published + 1, graph, out);
+            last = cur;
             pf.collect(ktimes_);
-            if (D.hst->piv && newton) {
+            if (cur.piv && newton) {
                 // the gain with reduced outlier weights lost rank: IRLS weights from here on
                 newton_off = true;
                 factor_valid_ = false;
                 out.newton_fallbacks++;
                 continue;
             }
-            if (D.hst->flow_err) {
+            if (cur.flow_err) {
                 // a dependency wait timed out: redo this iteration level by level
                 ++D.plan_epoch;
                 D.use_flow = false;
@@ -5702,7 +5728,7 @@ SolveOutcome Instance::solve(const SolveParams& p) {
                 --it;
                 continue;
             }
-            if (D.hst->piv && split) {
+            if (cur.piv && split) {
                 // the FP64 factor of this gain has pivots below 1e-13 G_jj (an
                 // unobservable part): back to the state before the step, and the
                 // mixed form factors in FP32 from here, substituting such pivots
@@ -5716,12 +5742,12 @@ SolveOutcome Instance::solve(const SolveParams& p) {
                 --it;
                 continue;
             }
-            if (D.hst->piv && !fp64) {
+            if (cur.piv && !fp64) {
                 // mixed-form pivots below threshold were substituted (threshold *
                 // G_jj): the factor stays SPD; the FP64 residual / RHS of the next
                 // iterations (or CG) correct the approximate Newton step
                 out.pivots_substituted++;
-            } else if (D.hst->piv) {
+            } else if (cur.piv) {
                 factor_valid_ = false;
                 singular = true;
                 break;
@@ -5732,26 +5758,26 @@ SolveOutcome Instance::solve(const SolveParams& p) {
             }
             out.iterations = it + 1;
             const double prev = out.max_mismatch;
-            out.max_mismatch = D.hst->maxdx;
+            out.max_mismatch = cur.maxdx;
             // contraction of consecutive steps on the cached factor (the look-ahead's predictor)
-            if (!refactor && it > 0 && prev > 0.0) D.ahead_ratio = D.hst->maxdx / prev;
+            if (!refactor && it > 0 && prev > 0.0) D.ahead_ratio = cur.maxdx / prev;
             prev_step = (!refactor && it > 0) ? prev : 0.0;
-            last_step = D.hst->maxdx;
+            last_step = cur.maxdx;
             // a cached factor (REALTIME) taken at a point the state has since left
             // contracts slowly: refresh it at the current point for the next step
-            if (!refactor && it > 0 && D.hst->maxdx > kRefreshRatio * prev && D.hst->maxdx >= p.tol) stale = true;
+            if (!refactor && it > 0 && cur.maxdx > kRefreshRatio * prev && cur.maxdx >= p.tol) stale = true;
             // the FP32 factor is too coarse for this gain: a factor fresh at this
             // point (or two steps in a row) should contract far better near the
             // solution, so continue with an FP64 factor
-            const bool coarse = refactor && !split && it > 0 && D.hst->maxdx < kLinearRegime &&
-                                D.hst->maxdx > kRefreshRatio * prev && D.hst->maxdx >= p.tol;
-            if (!fp64 && it > 0 && (D.hst->nan || D.hst->maxdx > 0.9 * prev || coarse)) {
-                if (coarse || ++stalls >= 2 || D.hst->nan) {
+            const bool coarse = refactor && !split && it > 0 && cur.maxdx < kLinearRegime &&
+                                cur.maxdx > kRefreshRatio * prev && cur.maxdx >= p.tol;
+            if (!fp64 && it > 0 && (cur.nan || cur.maxdx > 0.9 * prev || coarse)) {
+                if (coarse || ++stalls >= 2 || cur.nan) {
                     fp64_factor_active_ = true;
                     factor_valid_ = false;
                     out.fp64_refactors++;
                     stalls = 0;
-                    if (D.hst->nan) {
+                    if (cur.nan) {
                         diverged = true;
                         break;
                     }
@@ -5759,11 +5785,11 @@ SolveOutcome Instance::solve(const SolveParams& p) {
             } else {
                 stalls = 0;
             }
-            if (D.hst->nan) {
+            if (cur.nan) {
                 diverged = true;
                 break;
             }
-            if (D.hst->maxdx < p.tol) {
+            if (cur.maxdx < p.tol) {
                 converged = true;
                 break;
             }
@@ -5809,11 +5835,12 @@ SolveOutcome Instance::solve(const SolveParams& p) {
         }
         HIPCHECK(hipGetLastError());
         read_status(D);
+        last = *D.hst;
     }
     D.outputs_pending = true;
-    out.objective = D.m ? D.hst->J : 0.0;
-    out.largest_residual = D.m ? D.hst->rmax : 0.0;
-    out.largest_residual_idx = (D.m && D.hst->ridx >= 0) ? D.ml[D.hst->ridx] : -1;
+    out.objective = D.m ? last.J : 0.0;
+    out.largest_residual = D.m ? last.rmax : 0.0;
+    out.largest_residual_idx = (D.m && last.ridx >= 0) ? D.ml[last.ridx] : -1;
     out.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     out.flags = (D.use_flow ? 1 : 0) | (fp64_factor_active_ ? 2 : 0) |
                 (precision_ == 0 && !fp64_factor_active_ && D.mixed64 ? 4 : 0);
@@ -6298,8 +6325,7 @@ void Instance::debug_solve(const double* b, double* dx) {
     xx.alloc(D.ns, tmp);
     bb.upload(b, D.ns, D.stream);
     for (int attempt = 0; attempt < 2; ++attempt) {
-        D.hst->flow_err = 0;
-        HIPCHECK(hipMemcpyAsync(D.st.p, D.hst, sizeof(DevStatus), hipMemcpyHostToDevice, D.stream));
+        HIPCHECK(hipMemsetAsync(&D.st.p->flow_err, 0, sizeof(int), D.stream));
         if (precision_ == 1) {
             k_permute_in<double><<<nblk(D.ns), kBlock, 0, D.stream>>>(D.ns, D.perm.p, bb.p, D.f64.x.p);
             launch_trisolve<double>(D);
