@@ -1018,7 +1018,22 @@ __device__ inline void factor_front_wave(const FactArgs<T>& A, int s, T* F) {
     const int ge_l = lane < k ? A.gdiag[first + lane] : -1;
     for (int idx = lane; idx < c * c; idx += kWave) F[idx] = T(0);
     wave_sync();
-    for (int e = g0 + lane; e < g1; e += kWave) F[A.gloc[e]] = A.Gval[e];
+    // the G entries: a batch of loads per lane before their LDS stores (one
+    // round trip per batch, not per entry)
+    constexpr int GB = 8;
+    for (int e0 = g0 + lane; e0 < g1; e0 += GB * kWave) {
+        int32_t gl[GB];
+        T gv[GB];
+#pragma unroll
+        for (int u = 0; u < GB; ++u) {
+            const int e = e0 + u * kWave;
+            gl[u] = e < g1 ? A.gloc[e] : 0;
+            gv[u] = e < g1 ? A.Gval[e] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < GB; ++u)
+            if (e0 + u * kWave < g1) F[gl[u]] = gv[u];
+    }
     int cc_l = 0, kc_l = 0;
     int64_t fo_l = 0, ro_l = 0;
     if (ch_l >= 0) {
@@ -1048,16 +1063,36 @@ __device__ inline void factor_front_wave(const FactArgs<T>& A, int s, T* F) {
         const T* Fc = A.F + fo;
         const int32_t* rm = A.rmap + ro;
         if (u > 0) {
+            // the child's update block (a >= b of u x u), a batch of GB positions
+            // per lane: every load of the batch (relative map entries, update
+            // values) issued before the LDS adds
             const int da = kWave % u, db = kWave / u;
             int b = lane / u, a = lane - b * u;
             while (b < u) {
-                if (a >= b) F[rm[b] * c + rm[a]] += Fc[(int64_t)(kc + b) * cc + kc + a];
-                a += da;
-                b += db;
-                if (a >= u) {
-                    a -= u;
-                    ++b;
+                int pa[GB], pb[GB];
+#pragma unroll
+                for (int q = 0; q < GB; ++q) {
+                    pa[q] = a;
+                    pb[q] = b;
+                    a += da;
+                    b += db;
+                    if (a >= u) {
+                        a -= u;
+                        ++b;
+                    }
                 }
+                int ra[GB], rb[GB];
+                T v[GB];
+#pragma unroll
+                for (int q = 0; q < GB; ++q) {
+                    const bool ok = pb[q] < u && pa[q] >= pb[q];
+                    ra[q] = ok ? rm[pa[q]] : 0;
+                    rb[q] = ok ? rm[pb[q]] : 0;
+                    v[q] = ok ? Fc[(int64_t)(kc + pb[q]) * cc + kc + pa[q]] : T(0);
+                }
+#pragma unroll
+                for (int q = 0; q < GB; ++q)
+                    if (pb[q] < u && pa[q] >= pb[q]) F[rb[q] * c + ra[q]] += v[q];
             }
         }
         wave_sync();
@@ -1845,49 +1880,8 @@ __device__ inline T ld_off(const T* base, uint32_t i) {
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (size_t)(i * (uint32_t)sizeof(T)));
 }
 
-// After each factorization: M_t by column-oriented forward substitution of
-// [L_BB 0; L_UB I] m = e_j, with the subtree's L gathered densely into LDS.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_sub_m(SubArgs B, SolveArgs<T> A, const int32_t* __restrict__ list,
-                                                 T* __restrict__ Mout) {
-    extern __shared__ double Ld[];  // R x n, column-major
-    const int t = list[blockIdx.x];
-    const int n = B.n[t], R = B.R[t];
-    const int nw = kBlock / kWave, w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-    for (int e = threadIdx.x; e < R * n; e += kBlock) Ld[e] = 0.0;
-    __syncthreads();
-    for (int f = B.fptr[t] + w; f < B.fptr[t + 1]; f += nw) {
-        const int s = B.fsn[f], c = A.sn_c[s], k = A.sn_k[s], dc = B.fdcol[f];
-        const T* L = A.F + A.foff[s];
-        const int32_t* dr = B.drow + B.fdoff[f];
-        for (int i = lane; i < c; i += kWave) {
-            const int r = dr[i];
-            const int jmax = i < k ? i + 1 : k;
-            for (int jj = 0; jj < jmax; ++jj) Ld[(dc + jj) * R + r] = (double)L[jj * c + i];
-        }
-    }
-    __syncthreads();
-    T* Mt = Mout + B.moff[t];
-    for (int j = w; j < n; j += nw) {
-        double m0 = lane == j ? 1.0 : 0.0, m1 = 0.0;  // rows lane and lane + 64
-        for (int k = j; k < n; ++k) {
-            const double mk = __shfl(m0, k, kWave) / Ld[k * R + k];
-            if (lane == k)
-                m0 = mk;
-            else if (lane > k && lane < R)
-                m0 -= Ld[k * R + lane] * mk;
-            if (lane + kWave < R) m1 -= Ld[k * R + lane + kWave] * mk;
-        }
-        T* col = Mt + sub_col_off(j, R);
-        if (lane >= j && lane < R) col[lane - j] = (T)m0;
-        if (lane + kWave < R) col[lane + kWave - j] = (T)m1;
-        T* rowm = Mout + B.msub + B.moff[t];  // row-major copy
-        if (lane >= j && lane < R) rowm[sub_row_off(lane, n) + j] = (T)m0;
-        if (lane + kWave < R) rowm[sub_row_off(lane + kWave, n) + j] = (T)m1;
-    }
-}
-
-// The same M_t with all columns at once: lane j carries column j through the
+// After each factorization: M_t with all columns at once, the subtree's L
+// gathered densely into LDS.  Lane j carries column j through the
 // row-oriented substitution
 //   X[i][j] = (delta_ij - sum_{k<i} L[i][k] X[k][j]) / L[i][i],   i < n,
 //   M[n + r][j] = -sum_{k<n} L[n + r][k] X[k][j],
@@ -1919,7 +1913,10 @@ __global__ __launch_bounds__(kBlock) void k_sub_m_rows(SubArgs B, SolveArgs<T> A
     double* X = sm + R * n;  // X[k * n + j]
     for (int e = threadIdx.x; e < R * n; e += kBlock) Ld[e] = 0.0;
     __syncthreads();
-    // the gather spreads the fronts over the waves (its global loads dominate)
+    // the gather spreads the fronts over the waves (its global loads dominate:
+    // each lane issues a batch of its column's loads before their LDS stores,
+    // instead of one load and store after the other)
+    constexpr int GB = 8;
     for (int f = B.fptr[t] + w; f < B.fptr[t + 1]; f += nw) {
         const int s = B.fsn[f], c = A.sn_c[s], k = A.sn_k[s], dc = B.fdcol[f];
         const T* L = A.F + A.foff[s];
@@ -1927,7 +1924,14 @@ __global__ __launch_bounds__(kBlock) void k_sub_m_rows(SubArgs B, SolveArgs<T> A
         for (int i = lane; i < c; i += kWave) {
             const int r = dr[i];
             const int jmax = i < k ? i + 1 : k;
-            for (int jj = 0; jj < jmax; ++jj) Ld[(dc + jj) * R + r] = (double)L[jj * c + i];
+            for (int j0 = 0; j0 < jmax; j0 += GB) {
+                T v[GB];
+#pragma unroll
+                for (int u = 0; u < GB; ++u) v[u] = j0 + u < jmax ? L[(j0 + u) * c + i] : T(0);
+#pragma unroll
+                for (int u = 0; u < GB; ++u)
+                    if (j0 + u < jmax) Ld[(dc + j0 + u) * R + r] = (double)v[u];
+            }
         }
     }
     __syncthreads();
@@ -3404,9 +3408,9 @@ static void make_sub_plan(const Symbolic& S, const std::vector<int8_t>& lev, PiP
     }
     }
     P.nsub = (int)P.sroot.size();
-    // launches of the subtree-matrix kernels grouped by the LDS they need:
-    // k_sub_m_rows (dense L and X) in bins 0-2, k_sub_m (dense L only) for the
-    // few subtrees whose L and X together pass 64 KiB
+    // launches of the subtree-matrix kernel k_sub_m_rows grouped by the LDS
+    // they need (dense L and X: at most 96 KiB for 128 x 64, within the 160 KiB
+    // a gfx950 workgroup may hold)
     const size_t lim[3] = {16384, 32768, 65536};
     std::vector<int32_t> bin[kSubBins];
     for (int t = 0; t < P.nsub; ++t) {
@@ -3414,7 +3418,7 @@ static void make_sub_plan(const Symbolic& S, const std::vector<int8_t>& lev, PiP
         const size_t b = (R * n + n * n) * sizeof(double);
         const int q = b <= lim[0] ? 0 : (b <= lim[1] ? 1 : (b <= lim[2] ? 2 : 3));
         bin[q].push_back(t);
-        P.bin_bytes[q] = std::max(P.bin_bytes[q], q < 3 ? b : R * n * sizeof(double));
+        P.bin_bytes[q] = std::max(P.bin_bytes[q], b);
     }
     for (int q = 0; q < kSubBins; ++q) {  // largest first within a bin
         std::stable_sort(bin[q].begin(), bin[q].end(), [&](int a, int b) {
@@ -5010,6 +5014,18 @@ SubPack sub_pack(const Instance::Impl& D) {
 
 bool sub_active(const Instance::Impl& D) { return pi_active(D) && D.pi.dense && D.pi.nsub > 0; }
 
+// A kernel's dynamic LDS above 64 KiB (gfx950: up to 160 KiB per workgroup)
+// is allowed once per process and kernel.
+void allow_big_lds(const void* kernel, size_t bytes) {
+    static std::mutex mu;
+    static std::map<const void*, size_t> done;
+    std::lock_guard<std::mutex> lk(mu);
+    size_t& have = done[kernel];
+    if (have >= bytes) return;
+    HIPCHECK(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    have = bytes;
+}
+
 // L_TT^-1 blocks of the current factor and their row-grouped copy
 template <typename T>
 void launch_pi_inverse(Instance::Impl& D) {
@@ -5019,12 +5035,9 @@ void launch_pi_inverse(Instance::Impl& D) {
         for (int q = 0; q < kSubBins; ++q) {
             const int cnt = PI.bin_off[q + 1] - PI.bin_off[q];
             if (!cnt) continue;
-            if (q < kSubBins - 1)
-                k_sub_m_rows<T><<<cnt, kBlock, PI.bin_bytes[q], D.stream>>>(sub_args(D), solve_args<T>(D),
-                                                                           PI.bins.p + PI.bin_off[q], B.msub.p);
-            else
-                k_sub_m<T><<<cnt, kBlock, PI.bin_bytes[q], D.stream>>>(sub_args(D), solve_args<T>(D),
-                                                                      PI.bins.p + PI.bin_off[q], B.msub.p);
+            if (PI.bin_bytes[q] > 65536) allow_big_lds((const void*)k_sub_m_rows<T>, PI.bin_bytes[q]);
+            k_sub_m_rows<T><<<cnt, kBlock, PI.bin_bytes[q], D.stream>>>(sub_args(D), solve_args<T>(D),
+                                                                       PI.bins.p + PI.bin_off[q], B.msub.p);
         }
     if (!pi_active(D) || PI.ntop == 0) return;
     const size_t per_wave = 3 * (size_t)PI.slice * sizeof(T);
