@@ -1664,6 +1664,12 @@ struct PiArgs {
     const int64_t* cpoff;
     const int32_t* cplen;
     const int32_t* cjj;
+    // per top supernode: its first column and its column count
+    const int32_t* ufirst;
+    const int32_t* uk;
+    // units of the backward product (supernode, first column), longest path first
+    const int32_t* bunit;
+    int nunit;
 };
 
 // One wave per top supernode s: the k_s unit right-hand sides of its columns are
@@ -1745,42 +1751,61 @@ __global__ __launch_bounds__(kBlock) void k_pi_assemble(int ntc, const int32_t* 
 // entries per segment of the forward product (a multiple of 64): 256 keeps
 // enough waves in flight (1024 measured 35% slower at 200k buses)
 constexpr int kPiSegMax = 256;
+constexpr int kPiSegWave = 4;  // forward-product segments per wave
 
-// Forward product y_T = L_TT^-1 c_T: one wave per segment of a row's entries,
-// all loads of a lane issued together.  The entries of a row are one
-// contiguous range of c_T (the top fronts are numbered in postorder), so entry
-// q multiplies c_T[segc + q]: the matrix values and c_T stream side by side,
-// with no column index per entry.  A row held by one segment (most rows) is
-// written directly; longer rows leave partial sums for k_pi_rowsum.
+// Forward product y_T = L_TT^-1 c_T, kPiSegWave segments of a row's entries per
+// wave: their headers are wave-uniform (scalar loads), then every lane issues
+// the loads of all of them before the first product, so a wave waits for
+// memory twice.  The entries of a row are one contiguous range of c_T (the top
+// fronts are numbered in postorder), so entry q multiplies c_T[segc + q]: the
+// matrix values and c_T stream side by side, with no column index per entry.
+// Out of range, a lane loads the segment's first entry and adds 0.  A row held
+// by one segment (most rows) is written directly (segdst >= 0); longer rows
+// leave partial sums for k_pi_rowsum (segdst = ~segment).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pi_fwd(int nseg, const int64_t* __restrict__ segq,
-                                                  const int64_t* __restrict__ segc, const int32_t* __restrict__ segrow,
-                                                  const int32_t* __restrict__ segptr, const T* __restrict__ val,
-                                                  const T* __restrict__ ctop, double* __restrict__ part,
-                                                  T* __restrict__ ytop) {
-    const int g = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+                                                  const int64_t* __restrict__ segc, const int32_t* __restrict__ segdst,
+                                                  const T* __restrict__ val, const T* __restrict__ ctop,
+                                                  double* __restrict__ part, T* __restrict__ ytop) {
+    const int g0 =
+        __builtin_amdgcn_readfirstlane((blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave) * kPiSegWave);
     const int lane = threadIdx.x & (kWave - 1);
-    if (g >= nseg) return;
+    if (g0 >= nseg) return;
     constexpr int kU = kPiSegMax / kWave;
-    const int64_t q0 = segq[g] + lane, q1 = segq[g + 1];
-    const T* cq = ctop + segc[g];  // c_T aligned with the entries
-    T v[kU], c[kU];
+    int64_t q0[kPiSegWave], q1[kPiSegWave];
+    const T* cq[kPiSegWave];
 #pragma unroll
-    for (int u = 0; u < kU; ++u) {
-        const int64_t q = q0 + (int64_t)u * kWave;
-        v[u] = q < q1 ? val[q] : T(0);
-        c[u] = q < q1 ? cq[q] : T(0);
+    for (int u = 0; u < kPiSegWave; ++u) {
+        const int g = min(g0 + u, nseg - 1);
+        q0[u] = segq[g];
+        q1[u] = g0 + u < nseg ? segq[g + 1] : q0[u];
+        cq[u] = ctop + segc[g];  // c_T aligned with the entries
     }
-    double acc = 0.0;
+    T v[kPiSegWave][kU], c[kPiSegWave][kU];
 #pragma unroll
-    for (int u = 0; u < kU; ++u) acc += (double)v[u] * (double)c[u];
-    acc = wave_sum(acc);
-    if (lane == 0) {
-        const int t = segrow[g];
-        if (segptr[t + 1] - segptr[t] == 1)
-            ytop[t] = (T)acc;
-        else
-            part[g] = acc;
+    for (int u = 0; u < kPiSegWave; ++u) {
+#pragma unroll
+        for (int k = 0; k < kU; ++k) {
+            const int64_t q = q0[u] + lane + (int64_t)k * kWave;
+            const int64_t qq = q < q1[u] ? q : q0[u];
+            v[u][k] = val[qq];
+            c[u][k] = cq[u][qq];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kPiSegWave; ++u) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < kU; ++k)
+            acc += q0[u] + lane + (int64_t)k * kWave < q1[u] ? (double)v[u][k] * (double)c[u][k] : 0.0;
+        acc = wave_sum(acc);
+        if (lane == 0 && g0 + u < nseg) {
+            const int d = segdst[g0 + u];
+            if (d >= 0)
+                ytop[d] = (T)acc;
+            else
+                part[~d] = acc;
+        }
     }
 }
 
@@ -1799,10 +1824,12 @@ __global__ __launch_bounds__(kBlock) void k_pi_rowsum(int nrows, const int32_t* 
     if (lane == 0) ytop[t] = (T)acc;
 }
 
-// Backward product x_T = L_TT^-T y_T: one wave per top column, a dot product
-// of its (contiguous) block column with y_T gathered along the path.
+// Backward product x_T = L_TT^-T y_T for FP64 solve data: one wave per top
+// column, a dot product of its (contiguous) block column with y_T gathered
+// along the path (measured faster than the units below for FP64 blocks: 16.7
+// against 20.1 us at 200k buses; the units hold 106 VGPRs in FP64).
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pi_bwd(PiArgs A, SolveArgs<T> SA, const T* __restrict__ B,
+__global__ __launch_bounds__(kBlock) void k_pi_bwd_col(PiArgs A, SolveArgs<T> SA, const T* __restrict__ B,
                                                   const T* __restrict__ ytop) {
     const int t = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
     const int lane = threadIdx.x & (kWave - 1);
@@ -1827,6 +1854,70 @@ __global__ __launch_bounds__(kBlock) void k_pi_bwd(PiArgs A, SolveArgs<T> SA, co
     }
     acc = wave_sum(acc);
     if (lane == 0) SA.x[col] = (T)acc;
+}
+
+// Backward product x_T = L_TT^-T y_T for FP32 solve data: one wave per unit of
+// a top supernode's columns (kPiBwdCols of them, units longest path first;
+// 14.8 -> 13.7 us against one wave per column at 200k buses).  The columns share
+// the path rows, so y_T is gathered along the path once for the unit, each
+// column with its own partial sum (column jj of the block starts at row jj;
+// rows above are zero).  Out of range, a lane loads row j0 of column j0 and
+// adds 0.
+constexpr int kPiBwdRows = 8;  // path rows of a lane in flight together
+constexpr int kPiBwdCols = 4;  // columns per unit
+constexpr int kPiUnitShift = 7;  // unit = supernode << kPiUnitShift | first column of the unit
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pi_bwd(PiArgs A, SolveArgs<T> SA, const T* __restrict__ B,
+                                                  const T* __restrict__ ytop) {
+    const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave);
+    const int lane = threadIdx.x & (kWave - 1);
+    if (w >= A.nunit) return;  // uniform per wave
+    const int unit = A.bunit[w];
+    const int u = unit >> kPiUnitShift, j0 = unit & ((1 << kPiUnitShift) - 1);
+    // per-supernode metadata, one round trip: first column, columns, path rows, block, path
+    const int col0 = A.ufirst[u], ks = A.uk[u], Ps = A.plen[u];
+    const T* Bs = B + A.boff[u];
+    const int32_t* pl = A.ploc + A.poff[u];
+    double acc[kPiBwdCols];
+#pragma unroll
+    for (int jj = 0; jj < kPiBwdCols; ++jj) acc[jj] = 0.0;
+    for (int r0 = j0 + lane; r0 < Ps; r0 += kPiBwdRows * kWave) {
+        int c[kPiBwdRows];
+        T bv[kPiBwdCols][kPiBwdRows];
+#pragma unroll
+        for (int k = 0; k < kPiBwdRows; ++k) {
+            const int r = r0 + k * kWave;
+            c[k] = pl[r < Ps ? r : j0];
+        }
+#pragma unroll
+        for (int jj = 0; jj < kPiBwdCols; ++jj) {
+            const int64_t cj = (int64_t)Ps * min(j0 + jj, ks - 1);
+#pragma unroll
+            for (int k = 0; k < kPiBwdRows; ++k) {
+                const int r = r0 + k * kWave;
+                bv[jj][k] = Bs[cj + (r < Ps ? r : j0)];
+            }
+        }
+        T y[kPiBwdRows];
+#pragma unroll
+        for (int k = 0; k < kPiBwdRows; ++k) y[k] = ytop[c[k]];
+#pragma unroll
+        for (int jj = 0; jj < kPiBwdCols; ++jj) {
+#pragma unroll
+            for (int k = 0; k < kPiBwdRows; ++k) {
+                const int r = r0 + k * kWave;
+                acc[jj] += (r < Ps && r >= j0 + jj && j0 + jj < ks) ? (double)bv[jj][k] * (double)y[k] : 0.0;
+            }
+        }
+    }
+#pragma unroll
+    for (int jj = 0; jj < kPiBwdCols; ++jj) {
+        if (j0 + jj < ks) {  // uniform per wave
+            const double v = wave_sum(acc[jj]);
+            if (lane == 0) SA.x[col0 + j0 + jj] = (T)v;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -3268,7 +3359,8 @@ struct PiPlan {
     int cut = 0;
     int ntop = 0, ntc = 0, nseg = 0, slice = 0;
     int64_t nblock = 0, nq = 0;
-    std::vector<int32_t> sn, plen, ploc, tcol, colsn, src, segptr, cptr, parent, segrow, mrows, cplen, cjj;
+    std::vector<int32_t> sn, plen, ploc, tcol, colsn, src, segptr, cptr, parent, segdst, mrows, cplen, cjj, ufirst,
+        uk, bunit;
     std::vector<int64_t> boff, poff, pos, segq, segc, cpos, cboff, cpoff;
     // dense bottom subtrees (see SubArgs)
     bool dense = false;
@@ -3529,8 +3621,12 @@ static PiPlan make_pi_plan(const Symbolic& S, int cut, bool dense, bool mid) {
     P.plen.resize(P.ntop);
     P.boff.resize(P.ntop);
     P.poff.resize(P.ntop);
+    P.ufirst.resize(P.ntop);
+    P.uk.resize(P.ntop);
     for (int u = 0; u < P.ntop; ++u) {
         const int s0 = P.sn[u];
+        P.ufirst[u] = S.sn_first[s0];
+        P.uk[u] = S.sn_k[s0];
         P.boff[u] = P.nblock;
         P.poff[u] = (int64_t)P.ploc.size();
         int len = 0;
@@ -3545,6 +3641,15 @@ static PiPlan make_pi_plan(const Symbolic& S, int cut, bool dense, bool mid) {
         }
         P.nblock += (int64_t)len * S.sn_k[s0];
     }
+    // backward-product units: kPiBwdCols columns of one supernode, longest path first
+    for (int u = 0; u < P.ntop; ++u) {
+        if (u >= (1 << (31 - kPiUnitShift)) || P.uk[u] > (1 << kPiUnitShift))
+            throw std::logic_error("partitioned inverse: top too large for the backward-product units");
+        for (int j0 = 0; j0 < P.uk[u]; j0 += kPiBwdCols) P.bunit.push_back(u << kPiUnitShift | j0);
+    }
+    std::stable_sort(P.bunit.begin(), P.bunit.end(), [&](int32_t a, int32_t b) {
+        return P.plen[a >> kPiUnitShift] > P.plen[b >> kPiUnitShift];
+    });
     // forward product grouped by row: entry (u, r, jj) lands in row ploc[r]
     std::vector<int64_t> rcnt(P.ntc + 1, 0);
     for (int u = 0; u < P.ntop; ++u) {
@@ -3578,11 +3683,13 @@ static PiPlan make_pi_plan(const Symbolic& S, int cut, bool dense, bool mid) {
     for (int t = 0; t < P.ntc; ++t) {
         for (int64_t q = rcnt[t]; q < rcnt[t + 1]; q += kPiSeg) {
             P.segq.push_back(q);
-            P.segrow.push_back(t);
             P.segc.push_back(P.src[q] - q);
         }
         P.segptr[t + 1] = (int32_t)P.segq.size();
-        if (P.segptr[t + 1] - P.segptr[t] != 1) P.mrows.push_back(t);  // several segments (or none)
+        // a row of one segment is written by it; several segments (or none) leave partial sums
+        const bool single = P.segptr[t + 1] - P.segptr[t] == 1;
+        if (!single) P.mrows.push_back(t);
+        for (int g = P.segptr[t]; g < P.segptr[t + 1]; ++g) P.segdst.push_back(single ? t : ~g);
     }
     P.nseg = (int)P.segq.size();
     P.segq.push_back(P.nq);
@@ -3735,8 +3842,8 @@ struct Instance::Impl {
         bool ok = false;
         int cut = 0, ntop = 0, ntc = 0, nseg = 0, slice = 0;
         int64_t nblock = 0, nq = 0;
-        DBuf<int32_t> sn, plen, ploc, tcol, colsn, segptr, cptr, parent, segrow, mrows, cplen, cjj;
-        int nmrows = 0;
+        DBuf<int32_t> sn, plen, ploc, tcol, colsn, segptr, cptr, parent, segdst, mrows, cplen, cjj, ufirst, uk, bunit;
+        int nmrows = 0, nunit = 0;
         DBuf<int64_t> boff, poff, pos, segq, segc, cpos, cboff, cpoff;
         int64_t usum = 0, npath = 0;  // sum of subtree root update rows; path rows of the top
         int64_t msub0 = 0, usum0 = 0, ncol0 = 0;  // of the bottom subtrees: entries, update rows, columns
@@ -3904,7 +4011,11 @@ static void build_pi(Instance::Impl& D, const Symbolic& S, Ledger& bytes, hipStr
             up(Q.tcol, PP.tcol);
             up(Q.colsn, PP.colsn);
             up(Q.segptr, PP.segptr);
-            up(Q.segrow, PP.segrow);
+            up(Q.segdst, PP.segdst);
+            up(Q.ufirst, PP.ufirst);
+            up(Q.uk, PP.uk);
+            up(Q.bunit, PP.bunit);
+            Q.nunit = (int)PP.bunit.size();
             up(Q.mrows, PP.mrows);
             Q.nmrows = (int)PP.mrows.size();
             Q.npath = (int64_t)PP.ploc.size();
@@ -4040,7 +4151,7 @@ Instance::~Instance() {
     {
         auto& P = d_->pi;
         rel(P.sn); rel(P.plen); rel(P.ploc); rel(P.tcol); rel(P.colsn); rel(P.segptr); rel(P.cptr); rel(P.parent);
-        rel(P.segrow); rel(P.mrows); rel(P.cplen); rel(P.cjj); rel(P.cboff); rel(P.cpoff);
+        rel(P.segdst); rel(P.ufirst); rel(P.uk); rel(P.bunit); rel(P.mrows); rel(P.cplen); rel(P.cjj); rel(P.cboff); rel(P.cpoff);
         rel(P.boff); rel(P.poff); rel(P.pos); rel(P.segq); rel(P.segc); rel(P.cpos); rel(P.part);
         rel(P.sroot); rel(P.sn_n); rel(P.sR); rel(P.fptr); rel(P.fsn); rel(P.fdcol); rel(P.drow); rel(P.scptr);
         rel(P.scols); rel(P.bins); rel(P.fdoff); rel(P.smoff); rel(P.suoff); rel(P.surow); rel(P.acptr); rel(P.acpos); rel(P.pkind); rel(P.psub);
@@ -4987,7 +5098,8 @@ SolveArgs<T> solve_args(Instance::Impl& D) {
 PiArgs pi_args(const Instance::Impl& D) {
     const Instance::Impl::PiDev& P = D.pi;
     return PiArgs{P.sn.p,   P.plen.p,  P.boff.p,  P.poff.p,  P.ploc.p, P.parent.p, P.tcol.p, P.colsn.p,
-                  P.ntop,   P.ntc,     P.slice,   P.cboff.p, P.cpoff.p, P.cplen.p,  P.cjj.p};
+                  P.ntop,   P.ntc,     P.slice,   P.cboff.p, P.cpoff.p, P.cplen.p,  P.cjj.p,    P.ufirst.p,
+                  P.uk.p,   P.bunit.p, P.nunit};
 }
 
 bool pi_active(const Instance::Impl& D) { return D.use_pi && D.pi.ok; }
@@ -5181,8 +5293,9 @@ void launch_trisolve(Instance::Impl& D, Prof* pf = nullptr) {
             k_pi_assemble<T><<<nblk(PI.ntc), kBlock, 0, D.stream>>>(PI.ntc, PI.tcol.p, PI.cptr.p, PI.cpos.p, A.x,
                                                                    A.ubuf, B.ctop.p);
             prof_begin(pf, PH_PI_FWD);
-            k_pi_fwd<T><<<(PI.nseg + nw - 1) / nw, kBlock, 0, D.stream>>>(
-                PI.nseg, PI.segq.p, PI.segc.p, PI.segrow.p, PI.segptr.p, B.pinv_rows.p, B.ctop.p, PI.part.p, B.ytop.p);
+            const int fw = (PI.nseg + kPiSegWave - 1) / kPiSegWave;  // waves of the forward product
+            k_pi_fwd<T><<<(fw + nw - 1) / nw, kBlock, 0, D.stream>>>(
+                PI.nseg, PI.segq.p, PI.segc.p, PI.segdst.p, B.pinv_rows.p, B.ctop.p, PI.part.p, B.ytop.p);
             prof_end(pf, PH_PI_FWD, 1);
             if (PI.nmrows)
                 k_pi_rowsum<T><<<(PI.nmrows + nw - 1) / nw, kBlock, 0, D.stream>>>(PI.nmrows, PI.mrows.p,
@@ -5192,7 +5305,10 @@ void launch_trisolve(Instance::Impl& D, Prof* pf = nullptr) {
         prof_begin(pf, PH_BWD);
         if (PI.ntc) {
             prof_begin(pf, PH_PI_BWD);
-            k_pi_bwd<T><<<(PI.ntc + nw - 1) / nw, kBlock, 0, D.stream>>>(pi_args(D), A, B.pinv.p, B.ytop.p);
+            if constexpr (sizeof(T) == 8)
+                k_pi_bwd_col<T><<<(PI.ntc + nw - 1) / nw, kBlock, 0, D.stream>>>(pi_args(D), A, B.pinv.p, B.ytop.p);
+            else
+                k_pi_bwd<T><<<(PI.nunit + nw - 1) / nw, kBlock, 0, D.stream>>>(pi_args(D), A, B.pinv.p, B.ytop.p);
             prof_end(pf, PH_PI_BWD, 1);
         }
         if (dense) {
@@ -5595,6 +5711,11 @@ SolveOutcome Instance::solve(const SolveParams& p) {
     if (p.flat_start) k_flat<<<nblk(D.n), kBlock, 0, D.stream>>>(D.n, D.bus_type.p, D.vset.p, D.V.p, D.th.p);
     int retries = 0;
     bool retry = false;
+    // contraction of the last two steps of the previous solve on this instance:
+    // steps on a cached factor contract more slowly from one iteration to the
+    // next (ratios ~0.005, 0.02, 0.04 at 200k buses), so the look-ahead takes
+    // the larger of this and the current ratio
+    const double tail_ratio = D.ahead_ratio;
     // the last iteration was closed by a measurement pass at the current state:
     // the next one on the cached factor can start at its right-hand side, and
     // its status (last) holds J and the largest residual of this state
@@ -5690,7 +5811,8 @@ SolveOutcome Instance::solve(const SolveParams& p) {
             HIPCHECK(hipGetLastError());
             // queue the next iteration behind this one when it would be another
             // closing graph iteration on the cached factor and this one is not
-            // predicted to converge (its step from the last two known steps,
+            // predicted to converge (its step from the last known step and the
+            // larger of the last contraction and the previous solve's final one,
             // kAheadMargin tol at least)
             if (graph && closing && it + 1 < p.max_iter && p.mode == 0 && !p.robust && factor_valid_ && !stale) {
                 const bool nfp64 = precision_ == 1 || fp64_factor_active_;
@@ -5698,7 +5820,7 @@ SolveOutcome Instance::solve(const SolveParams& p) {
                 const bool nclosing = D.use_loc && (nfp64 || p.pcg_steps <= 0);
                 const double ratio = prev_step > 0.0 ? last_step / prev_step : D.ahead_ratio;
                 const bool first = it == 0 || last_step >= 1e299;
-                const double predicted = first ? 1e300 : last_step * std::min(ratio, 1.0);
+                const double predicted = first ? 1e300 : last_step * std::min(std::max(ratio, tail_ratio), 1.0);
                 bool in_time = true;
                 if (p.time_limit_ms > 0)
                     in_time = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
