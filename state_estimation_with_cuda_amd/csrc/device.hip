@@ -1942,11 +1942,19 @@ struct SubArgs {
     const int32_t* root;  // root supernode
     const int32_t* n;     // columns
     const int32_t* R;     // columns + update rows
-    const int32_t* fptr;  // fronts of subtree t: fsn[fptr[t] .. fptr[t+1])
-    const int32_t* fsn;
-    const int32_t* fdcol;  // dense column of each front's first column
-    const int64_t* fdoff;  // offset of each front's dense-row list in drow
-    const int32_t* drow;   // dense row of every row of every front
+    // L of subtree t's fronts as rows of structural entries (CSR over the dense
+    // rows, columns ascending): row r holds the dense columns set in
+    // gmask[gmo[t] + r], from grs[gmo[t] + t + r].  The gather's entries e in
+    // [geptr[t], geptr[t+1]), column-major within each front (neighbouring
+    // loads touch neighbouring addresses): gsrc[e] their position from
+    // gbase[t] in the front storage, gdst[e] their position in the CSR.
+    const int64_t* geptr;
+    const int64_t* gbase;
+    const int32_t* gsrc;
+    const int32_t* gdst;
+    const int64_t* gmo;
+    const uint64_t* gmask;
+    const int32_t* grs;
     const int32_t* cptr;   // columns of subtree t: cols[cptr[t] .. cptr[t] + n[t])
     const int32_t* cols;
     const int64_t* moff;   // start of M_t
@@ -1972,25 +1980,60 @@ __device__ inline T ld_off(const T* base, uint32_t i) {
 }
 
 // After each factorization: M_t with all columns at once, the subtree's L
-// gathered densely into LDS.  Lane j carries column j through the
+// gathered into LDS.  Lane j carries column j through the
 // row-oriented substitution
 //   X[i][j] = (delta_ij - sum_{k<i} L[i][k] X[k][j]) / L[i][i],   i < n,
 //   M[n + r][j] = -sum_{k<n} L[n + r][k] X[k][j],
 // whose inner sums are independent LDS loads (broadcast L, conflict-free X)
-// rather than a chain of cross-lane steps.  The rows of X depend on each
-// other: one wave walks them; the update rows n + r do not: once X is there,
-// the block's waves share them.  LDS: the dense L (R x n) and X (n x n), FP64.
-template <typename T>
-__device__ inline double sub_m_row(const double* Ld, const double* X, int R, int n, int i, int kend, int jc) {
-    // eight independent partial sums keep sixteen LDS loads in flight
+// rather than a chain of cross-lane steps.  The sums run over the structural
+// entries of row i only: L sits in LDS as rows of its entries with a 64-bit
+// column mask per row (from the plan), X packed lower triangular, so a
+// subtree needs a fraction of the LDS of dense L and X and a CU holds several
+// (200k buses: 609 -> 294 us per refactor; most rows cost one round of LDS
+// loads instead of i / 8).  The rows of X depend on each other: one wave
+// walks them; the update rows n + r do not: once X is there, the block's
+// waves share them.
+// Row i of L times X over row i's structural entries (columns: the set bits
+// of m, wave-uniform; values: consecutive from Lr): eight at a time, eight
+// independent partial sums, so a row with at most eight entries before the
+// diagonal (most rows) waits for LDS once.  X is packed lower triangular (row
+// k holds columns 0..k); a slot past the last bit, or a lane past row k,
+// reads an entry of row 0 (computed, finite) and adds 0.
+__device__ inline int xoff(int k) { return k * (k + 1) / 2; }
+
+__device__ inline double sub_m_row(const double* Lr, const double* X, int j, uint64_t m) {
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    int k = 0;
-    for (; k + 8 <= kend; k += 8) {
+    for (int q = 0; m; q += 8) {
+        int k[8];
+        bool ok[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) acc[u] -= Ld[(k + u) * R + i] * X[(k + u) * n + jc];
+        for (int u = 0; u < 8; ++u) {
+            ok[u] = m != 0;
+            k[u] = ok[u] ? __builtin_ctzll(m) : 0;
+            m &= m - 1;
+        }
+        double lv[8], xv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const bool in = ok[u] && j <= k[u];
+            lv[u] = Lr[ok[u] ? q + u : 0];
+            xv[u] = X[in ? xoff(k[u]) + j : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc[u] -= (ok[u] && j <= k[u]) ? lv[u] * xv[u] : 0.0;
     }
-    for (; k < kend; ++k) acc[0] -= Ld[k * R + i] * X[k * n + jc];
     return ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+}
+
+__device__ inline uint64_t uniform_u64(uint64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (uint64_t)hi << 32 | lo;
+}
+
+// LDS of one subtree: its L entries and packed X (FP64), row masks, row starts
+__host__ __device__ inline size_t sub_m_lds(int64_t nnz, int n, int R) {
+    return (size_t)(nnz + (int64_t)n * (n + 1) / 2 + R) * sizeof(double) + (size_t)(R + 2) * sizeof(int32_t);
 }
 
 template <typename T>
@@ -2000,43 +2043,49 @@ __global__ __launch_bounds__(kBlock) void k_sub_m_rows(SubArgs B, SolveArgs<T> A
     const int t = list[blockIdx.x];
     const int n = B.n[t], R = B.R[t];
     const int nw = kBlock / kWave, w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-    double* Ld = sm;         // R x n, column-major
-    double* X = sm + R * n;  // X[k * n + j]
-    for (int e = threadIdx.x; e < R * n; e += kBlock) Ld[e] = 0.0;
-    __syncthreads();
-    // the gather spreads the fronts over the waves (its global loads dominate:
-    // each lane issues a batch of its column's loads before their LDS stores,
-    // instead of one load and store after the other)
+    const int64_t e0 = B.geptr[t], e1 = B.geptr[t + 1];  // e1 > e0: the diagonal at least
+    const int nnz = (int)(e1 - e0);
+    double* Lc = sm;                                               // the entries, row by row
+    double* X = Lc + nnz;                                          // packed lower triangular
+    uint64_t* mk = reinterpret_cast<uint64_t*>(X + n * (n + 1) / 2);  // per row, its columns
+    int32_t* rs = reinterpret_cast<int32_t*>(mk + R);              // per row, its first entry
+    const int64_t mo = B.gmo[t];
+    for (int r = threadIdx.x; r <= R; r += kBlock) {
+        if (r < R) mk[r] = B.gmask[mo + r];
+        rs[r] = B.grs[mo + t + r];
+    }
+    // the gather: every thread issues a batch of entry loads before their LDS stores
     constexpr int GB = 8;
-    for (int f = B.fptr[t] + w; f < B.fptr[t + 1]; f += nw) {
-        const int s = B.fsn[f], c = A.sn_c[s], k = A.sn_k[s], dc = B.fdcol[f];
-        const T* L = A.F + A.foff[s];
-        const int32_t* dr = B.drow + B.fdoff[f];
-        for (int i = lane; i < c; i += kWave) {
-            const int r = dr[i];
-            const int jmax = i < k ? i + 1 : k;
-            for (int j0 = 0; j0 < jmax; j0 += GB) {
-                T v[GB];
+    const T* Lb = A.F + B.gbase[t];
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += (int64_t)GB * kBlock) {
+        int sv[GB], dv[GB];
 #pragma unroll
-                for (int u = 0; u < GB; ++u) v[u] = j0 + u < jmax ? L[(j0 + u) * c + i] : T(0);
-#pragma unroll
-                for (int u = 0; u < GB; ++u)
-                    if (j0 + u < jmax) Ld[(dc + j0 + u) * R + r] = (double)v[u];
-            }
+        for (int u = 0; u < GB; ++u) {
+            const int64_t ee = e + (int64_t)u * kBlock;
+            const int64_t ec = ee < e1 ? ee : e0;
+            sv[u] = B.gsrc[ec];
+            dv[u] = B.gdst[ec];
         }
+        T v[GB];
+#pragma unroll
+        for (int u = 0; u < GB; ++u) v[u] = Lb[sv[u]];
+#pragma unroll
+        for (int u = 0; u < GB; ++u)
+            if (e + (int64_t)u * kBlock < e1) Lc[dv[u]] = (double)v[u];
     }
     __syncthreads();
     T* Mt = Mout + B.moff[t];
     T* Mr = Mout + B.msub + B.moff[t];
     const int j = lane;
-    const int jc = j < n ? j : 0;  // lanes beyond the columns read column 0 and discard it
     if (w == 0) {  // the rows of X, in order
         for (int i = 0; i < n; ++i) {
-            double v = sub_m_row<T>(Ld, X, R, n, i, i, jc);
+            const int r0 = __builtin_amdgcn_readfirstlane(rs[i]), r1 = __builtin_amdgcn_readfirstlane(rs[i + 1]);
+            const uint64_t m = uniform_u64(mk[i]) & ((1ull << i) - 1);  // entries before the diagonal
+            double v = sub_m_row(Lc + r0, X, j, m);
             if (i == j) v += 1.0;
-            v = j <= i ? v / Ld[i * R + i] : 0.0;
-            if (j < n) X[i * n + j] = v;
-            if (j < n && j <= i) {
+            v = v / Lc[r1 - 1];  // the diagonal: the row's last entry
+            if (j <= i) {
+                X[xoff(i) + j] = v;
                 Mt[sub_col_off(j, R) + i - j] = (T)v;
                 Mr[sub_row_off(i, n) + j] = (T)v;
             }
@@ -2044,7 +2093,8 @@ __global__ __launch_bounds__(kBlock) void k_sub_m_rows(SubArgs B, SolveArgs<T> A
     }
     __syncthreads();
     for (int i = n + w; i < R; i += nw) {  // the update rows, shared by the waves
-        const double v = sub_m_row<T>(Ld, X, R, n, i, n, jc);
+        const int r0 = __builtin_amdgcn_readfirstlane(rs[i]);
+        const double v = sub_m_row(Lc + r0, X, j, uniform_u64(mk[i]));
         if (j < n) {
             Mt[sub_col_off(j, R) + i - j] = (T)v;
             Mr[sub_row_off(i, n) + j] = (T)v;
@@ -3366,7 +3416,9 @@ struct PiPlan {
     bool dense = false;
     int nsub = 0, nsub0 = 0;  // subtrees; of them bottom ones (listed first)
     int64_t msub = 0;
-    std::vector<int32_t> sroot, sn_n, sR, fptr, fsn, fdcol, drow, scptr, scols, bins;
+    std::vector<int32_t> sroot, sn_n, sR, fptr, fsn, fdcol, drow, scptr, scols, bins, gsrc, gdst, grs;
+    std::vector<int64_t> geptr, gbase, gmo;
+    std::vector<uint64_t> gmask;
     // middle columns; and per subtree column (in scols order) the bottom roots'
     // update positions that add into it (empty for bottom subtree columns)
     std::vector<int32_t> mcol, acptr;
@@ -3485,6 +3537,44 @@ static void make_sub_plan(const Symbolic& S, const std::vector<int8_t>& lev, PiP
                 P.drow.push_back(d);
             }
         }
+        // the gather's entry list (column j of front s, rows j .. c - 1) and
+        // the rows of L as CSR: per row its column mask and first entry
+        {
+            const int64_t base = S.sn_foff[fronts[0]];  // the smallest front id: the lowest offset
+            if (P.geptr.empty()) P.geptr.push_back(0);
+            P.gbase.push_back(base);
+            P.gmo.push_back((int64_t)P.gmask.size());
+            const size_t f0 = P.fsn.size() - fronts.size();
+            std::vector<uint64_t> mask(R, 0);
+            for (int pass = 0; pass < 2; ++pass) {
+                std::vector<int32_t> rs(R + 1, 0);
+                for (int r = 0; r < R; ++r) rs[r + 1] = rs[r] + __builtin_popcountll(mask[r]);
+                for (size_t q = 0; q < fronts.size(); ++q) {
+                    const int s = fronts[q], c = S.sn_c[s], k = S.sn_k[s], dc = P.fdcol[f0 + q];
+                    const int64_t fo = S.sn_foff[s] - base;
+                    const int64_t dr = P.fdoff[f0 + q];
+                    if (fo + (int64_t)c * c > INT32_MAX) throw std::logic_error("dense subtree: fronts too far apart");
+                    for (int j = 0; j < k; ++j)
+                        for (int i = j; i < c; ++i) {
+                            const int r = P.drow[dr + i], col = dc + j;
+                            if (pass == 0) {
+                                mask[r] |= 1ull << col;
+                            } else {
+                                P.gsrc.push_back((int32_t)(fo + (int64_t)j * c + i));
+                                P.gdst.push_back(rs[r] + __builtin_popcountll(mask[r] & ((1ull << col) - 1)));
+                            }
+                        }
+                }
+                if (pass == 1) {
+                    P.gmask.insert(P.gmask.end(), mask.begin(), mask.end());
+                    P.grs.insert(P.grs.end(), rs.begin(), rs.end());
+                    for (int r = 0; r < n; ++r)  // the diagonal closes each row of L_BB
+                        if (!(mask[r] >> r & 1) || (mask[r] >> r) != 1)
+                            throw std::logic_error("dense subtree: a row of L without its diagonal last");
+                }
+            }
+            P.geptr.push_back((int64_t)P.gsrc.size());
+        }
         for (int s : fronts)
             for (int j = 0; j < S.sn_k[s]; ++j) dpos[S.sn_first[s] + j] = -1;
         for (int a = 0; a < u; ++a) dpos[S.sn_rows[S.sn_rows_off[r] + kr + a]] = -1;
@@ -3501,13 +3591,13 @@ static void make_sub_plan(const Symbolic& S, const std::vector<int8_t>& lev, PiP
     }
     P.nsub = (int)P.sroot.size();
     // launches of the subtree-matrix kernel k_sub_m_rows grouped by the LDS
-    // they need (dense L and X: at most 96 KiB for 128 x 64, within the 160 KiB
-    // a gfx950 workgroup may hold)
+    // they need (L's entries, packed X, row masks and starts: at most 82 KiB for
+    // a dense 128 x 64, within the 160 KiB a gfx950 workgroup may hold)
     const size_t lim[3] = {16384, 32768, 65536};
     std::vector<int32_t> bin[kSubBins];
     for (int t = 0; t < P.nsub; ++t) {
         const size_t n = P.sn_n[t], R = P.sR[t];
-        const size_t b = (R * n + n * n) * sizeof(double);
+        const size_t b = sub_m_lds(P.geptr[t + 1] - P.geptr[t], (int)n, (int)R);
         const int q = b <= lim[0] ? 0 : (b <= lim[1] ? 1 : (b <= lim[2] ? 2 : 3));
         bin[q].push_back(t);
         P.bin_bytes[q] = std::max(P.bin_bytes[q], b);
@@ -3855,8 +3945,10 @@ struct Instance::Impl {
         DBuf<int32_t> acptr;
         DBuf<int64_t> acpos;
         int64_t msub = 0;
-        DBuf<int32_t> sroot, sn_n, sR, fptr, fsn, fdcol, drow, scptr, scols, bins;
-        DBuf<int64_t> fdoff, smoff, suoff, surow;
+        DBuf<int32_t> sroot, sn_n, sR, scptr, scols, bins, gsrc, gdst, grs;
+        DBuf<int64_t> geptr, gbase, gmo;
+        DBuf<uint64_t> gmask;
+        DBuf<int64_t> smoff, suoff, surow;
         int bin_off[kSubBins + 1] = {0, 0, 0, 0, 0};
         size_t bin_bytes[kSubBins] = {0, 0, 0, 0};
         // subtrees of each level (bottom, middle) by rows R (<= 16, 32, 64, 128)
@@ -4053,14 +4145,20 @@ static void build_pi(Instance::Impl& D, const Symbolic& S, Ledger& bytes, hipStr
                 up(Q.sroot, PP.sroot);
                 up(Q.sn_n, PP.sn_n);
                 up(Q.sR, PP.sR);
-                up(Q.fptr, PP.fptr);
-                up(Q.fsn, PP.fsn);
-                up(Q.fdcol, PP.fdcol);
-                up(Q.drow, PP.drow);
+                up(Q.gsrc, PP.gsrc);
+                up(Q.gdst, PP.gdst);
+                up64(Q.geptr, PP.geptr);
+                up64(Q.gbase, PP.gbase);
+                up(Q.grs, PP.grs);
+                up64(Q.gmo, PP.gmo);
+                {
+                    std::vector<uint64_t> v = PP.gmask;
+                    if (v.empty()) v.push_back(0);
+                    Q.gmask.upload(v, bytes, s);
+                }
                 up(Q.scptr, PP.scptr);
                 up(Q.scols, PP.scols);
                 up(Q.bins, PP.bins);
-                up64(Q.fdoff, PP.fdoff);
                 up64(Q.smoff, PP.smoff);
                 up64(Q.suoff, PP.suoff);
                 up64(Q.surow, PP.surow);
@@ -4153,8 +4251,8 @@ Instance::~Instance() {
         rel(P.sn); rel(P.plen); rel(P.ploc); rel(P.tcol); rel(P.colsn); rel(P.segptr); rel(P.cptr); rel(P.parent);
         rel(P.segdst); rel(P.ufirst); rel(P.uk); rel(P.bunit); rel(P.mrows); rel(P.cplen); rel(P.cjj); rel(P.cboff); rel(P.cpoff);
         rel(P.boff); rel(P.poff); rel(P.pos); rel(P.segq); rel(P.segc); rel(P.cpos); rel(P.part);
-        rel(P.sroot); rel(P.sn_n); rel(P.sR); rel(P.fptr); rel(P.fsn); rel(P.fdcol); rel(P.drow); rel(P.scptr);
-        rel(P.scols); rel(P.bins); rel(P.fdoff); rel(P.smoff); rel(P.suoff); rel(P.surow); rel(P.acptr); rel(P.acpos); rel(P.pkind); rel(P.psub);
+        rel(P.sroot); rel(P.sn_n); rel(P.sR); rel(P.gsrc); rel(P.gdst); rel(P.geptr); rel(P.gbase); rel(P.grs); rel(P.gmo); rel(P.gmask); rel(P.scptr);
+        rel(P.scols); rel(P.bins); rel(P.smoff); rel(P.suoff); rel(P.surow); rel(P.acptr); rel(P.acpos); rel(P.pkind); rel(P.psub);
         for (auto& L : P.bottom) {
             rel(L.th); rel(L.meta); rel(L.lvl); rel(L.ch); rel(L.rm); rel(L.src); rel(L.mid);
         }
@@ -5106,9 +5204,9 @@ bool pi_active(const Instance::Impl& D) { return D.use_pi && D.pi.ok; }
 
 SubArgs sub_args(const Instance::Impl& D) {
     const Instance::Impl::PiDev& P = D.pi;
-    return SubArgs{P.nsub,   P.msub,    P.sroot.p, P.sn_n.p,  P.sR.p,    P.fptr.p,  P.fsn.p, P.fdcol.p,
-                   P.fdoff.p, P.drow.p, P.scptr.p, P.scols.p, P.smoff.p, P.acptr.p, P.acpos.p,
-                   P.suoff.p, P.surow.p};
+    return SubArgs{P.nsub,    P.msub,    P.sroot.p, P.sn_n.p,  P.sR.p,    P.geptr.p, P.gbase.p,
+                   P.gsrc.p,  P.gdst.p,  P.gmo.p,   P.gmask.p, P.grs.p,   P.scptr.p, P.scols.p,
+                   P.smoff.p, P.acptr.p, P.acpos.p, P.suoff.p, P.surow.p};
 }
 
 // Packed bottom-subtree products: by option, else by precision (measured at
@@ -5973,6 +6071,9 @@ SolveOutcome Instance::solve(const SolveParams& p) {
         last = *D.hst;
     }
     D.outputs_pending = true;
+    // z as this solve saw it, copied now: the copy runs while the caller turns
+    // to its next request, instead of ahead of that request's telemetry scatter
+    keep_z(D);
     out.objective = D.m ? last.J : 0.0;
     out.largest_residual = D.m ? last.rmax : 0.0;
     out.largest_residual_idx = (D.m && last.ridx >= 0) ? D.ml[last.ridx] : -1;
