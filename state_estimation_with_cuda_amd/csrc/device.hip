@@ -963,8 +963,21 @@ __global__ __launch_bounds__(kBlock) void k_spgemm(int nG, const int32_t* __rest
 // extend-add of its children's update blocks.  The first k columns are
 // factored; the trailing (c-k)^2 block is the update passed to the parent.
 // ---------------------------------------------------------------------------
+// Per-supernode and per-child-edge descriptors of the factorization: what a
+// front's wave needs before its loads, in one load each instead of a chain
+// (supernode -> its G range and child list -> each child's sizes and offsets).
+struct FrontDesc {
+    int32_t c, k, first, g0, ng, p0, nch, pad;
+};
+struct ChildDesc {  // child edge p (in child_ptr order): the child's sizes and offsets
+    int32_t cc, kc, pad0, pad1;
+    int64_t foff, rmoff;
+};
+
 template <typename T>
 struct FactArgs {
+    const FrontDesc* fdesc;
+    const ChildDesc* cdesc;
     const int32_t* sn_first;
     const int32_t* sn_k;
     const int32_t* sn_c;
@@ -1003,18 +1016,52 @@ struct Team<true> {
     static __device__ void sync() { wave_sync(); }
 };
 
-// One wave factors one front with its dependent global loads gathered into
-// few round trips: the front's and its children's metadata, the G entries and
-// the pivot thresholds are loaded lane-parallel up front, the children's
-// update matrices then one child at a time.  (The factor sweeps are bound by
-// these round trips: fronts have at most a few dozen rows.)
+// A batch of a child's update block for the extend-add: GB positions (a, b)
+// of the u x u square per lane, stepped without a division per position;
+// ra < 0 marks a position outside the lower triangle.  Every load of the batch
+// (relative map entries, update values) is issued before any use.
+constexpr int kUpdBatch = 8;
+struct UpdWalk {
+    int u, a, b, da, db, cc, kc;
+};
 template <typename T>
+__device__ inline void upd_load(UpdWalk& w, const T* Fc, const int32_t* rm, int* ra, int* rb, T* v) {
+#pragma unroll
+    for (int q = 0; q < kUpdBatch; ++q) {
+        const bool ok = w.b < w.u && w.a >= w.b;
+        ra[q] = ok ? rm[w.a] : -1;
+        rb[q] = ok ? rm[w.b] : 0;
+        v[q] = ok ? Fc[(int64_t)(w.kc + w.b) * w.cc + w.kc + w.a] : T(0);
+        w.a += w.da;
+        w.b += w.db;
+        if (w.a >= w.u) {
+            w.a -= w.u;
+            ++w.b;
+        }
+    }
+}
+
+// One wave factors one front with its dependent global loads gathered into
+// few round trips: the front's descriptor; then its children's descriptors,
+// the G entries and the pivot thresholds, lane-parallel; then the children's
+// update blocks, with PF the next child's loads issued before the current
+// child's adds (it needs the registers of a second batch: off where a
+// 1024-thread workgroup caps a wave at 128 VGPRs).  (The factor sweeps are
+// bound by these round trips: fronts have at most a few dozen rows.)
+template <typename T, bool PF = true>
 __device__ inline void factor_front_wave(const FactArgs<T>& A, int s, T* F) {
     const int lane = threadIdx.x & (kWave - 1);
-    const int c = A.sn_c[s], k = A.sn_k[s], first = A.sn_first[s];
-    const int g0 = A.g_sn_ptr[s], g1 = A.g_sn_ptr[s + 1], p0 = A.child_ptr[s], p1 = A.child_ptr[s + 1];
-    const int nch = p1 - p0;
-    int ch_l = lane < nch ? A.child[p0 + lane] : -1;
+    const FrontDesc d = A.fdesc[s];
+    const int c = d.c, k = d.k, first = d.first, g0 = d.g0, g1 = d.g0 + d.ng, p0 = d.p0, nch = d.nch;
+    int cc_l = 0, kc_l = 0;
+    int64_t fo_l = 0, ro_l = 0;
+    if (lane < nch) {
+        const ChildDesc cd = A.cdesc[p0 + lane];
+        cc_l = cd.cc;
+        kc_l = cd.kc;
+        fo_l = cd.foff;
+        ro_l = cd.rmoff;
+    }
     const int ge_l = lane < k ? A.gdiag[first + lane] : -1;
     for (int idx = lane; idx < c * c; idx += kWave) F[idx] = T(0);
     wave_sync();
@@ -1034,109 +1081,144 @@ __device__ inline void factor_front_wave(const FactArgs<T>& A, int s, T* F) {
         for (int u = 0; u < GB; ++u)
             if (e0 + u * kWave < g1) F[gl[u]] = gv[u];
     }
-    int cc_l = 0, kc_l = 0;
-    int64_t fo_l = 0, ro_l = 0;
-    if (ch_l >= 0) {
-        cc_l = A.sn_c[ch_l];
-        kc_l = A.sn_k[ch_l];
-        fo_l = A.foff[ch_l];
-        ro_l = A.rmoff[ch_l];
-    }
     const T gdl = ge_l >= 0 ? A.Gval[ge_l] : T(0);
     wave_sync();
-    for (int p = 0; p < nch; ++p) {
-        int cc, kc;
+    // child p's walk over its update block, and the block's address
+    auto child = [&](int p, UpdWalk& w, const T*& Fc, const int32_t*& rm) {
         int64_t fo, ro;
         if (p < kWave) {
-            cc = __shfl(cc_l, p, kWave);
-            kc = __shfl(kc_l, p, kWave);
+            w.cc = __shfl(cc_l, p, kWave);
+            w.kc = __shfl(kc_l, p, kWave);
             fo = __shfl(fo_l, p, kWave);
             ro = __shfl(ro_l, p, kWave);
         } else {  // more than 64 children: the rest one by one
-            const int ch = A.child[p0 + p];
-            cc = A.sn_c[ch];
-            kc = A.sn_k[ch];
-            fo = A.foff[ch];
-            ro = A.rmoff[ch];
+            const ChildDesc cd = A.cdesc[p0 + p];
+            w.cc = cd.cc;
+            w.kc = cd.kc;
+            fo = cd.foff;
+            ro = cd.rmoff;
         }
-        const int u = cc - kc;
-        const T* Fc = A.F + fo;
-        const int32_t* rm = A.rmap + ro;
-        if (u > 0) {
-            // the child's update block (a >= b of u x u), a batch of GB positions
-            // per lane: every load of the batch (relative map entries, update
-            // values) issued before the LDS adds
-            const int da = kWave % u, db = kWave / u;
-            int b = lane / u, a = lane - b * u;
-            while (b < u) {
-                int pa[GB], pb[GB];
+        w.u = w.cc - w.kc;
+        const int uu = max(w.u, 1);
+        w.da = kWave % uu;
+        w.db = kWave / uu;
+        w.b = w.u > 0 ? lane / uu : 0;
+        w.a = lane - w.b * uu;
+        Fc = A.F + fo;
+        rm = A.rmap + ro;
+    };
+    if (nch > 0) {
+        UpdWalk wc, wn;
+        const T *Fc, *Fn = nullptr;
+        const int32_t *rmc, *rmn = nullptr;
+        int ra[kUpdBatch], rb[kUpdBatch], ran[kUpdBatch], rbn[kUpdBatch];
+        T v[kUpdBatch], vn[kUpdBatch];
+        child(0, wc, Fc, rmc);
+        upd_load<T>(wc, Fc, rmc, ra, rb, v);
+        for (int p = 0; p < nch; ++p) {
+            if (PF && p + 1 < nch) {  // the next child's first batch in flight during this child's adds
+                child(p + 1, wn, Fn, rmn);
+                upd_load<T>(wn, Fn, rmn, ran, rbn, vn);
+            }
+            for (;;) {  // positions of one child are distinct: no barrier between its batches
 #pragma unroll
-                for (int q = 0; q < GB; ++q) {
-                    pa[q] = a;
-                    pb[q] = b;
-                    a += da;
-                    b += db;
-                    if (a >= u) {
-                        a -= u;
-                        ++b;
-                    }
+                for (int q = 0; q < kUpdBatch; ++q)
+                    if (ra[q] >= 0) F[rb[q] * c + ra[q]] += v[q];
+                if (wc.b >= wc.u) break;
+                upd_load<T>(wc, Fc, rmc, ra, rb, v);
+            }
+            wave_sync();  // children may add into the same positions
+            if (!PF && p + 1 < nch) {
+                child(p + 1, wc, Fc, rmc);
+                upd_load<T>(wc, Fc, rmc, ra, rb, v);
+            } else if (p + 1 < nch) {
+                wc = wn;
+                Fc = Fn;
+                rmc = rmn;
+#pragma unroll
+                for (int q = 0; q < kUpdBatch; ++q) {
+                    ra[q] = ran[q];
+                    rb[q] = rbn[q];
+                    v[q] = vn[q];
                 }
-                int ra[GB], rb[GB];
-                T v[GB];
-#pragma unroll
-                for (int q = 0; q < GB; ++q) {
-                    const bool ok = pb[q] < u && pa[q] >= pb[q];
-                    ra[q] = ok ? rm[pa[q]] : 0;
-                    rb[q] = ok ? rm[pb[q]] : 0;
-                    v[q] = ok ? Fc[(int64_t)(kc + pb[q]) * cc + kc + pa[q]] : T(0);
-                }
-#pragma unroll
-                for (int q = 0; q < GB; ++q)
-                    if (pb[q] < u && pa[q] >= pb[q]) F[rb[q] * c + ra[q]] += v[q];
             }
         }
-        wave_sync();
     }
+    // The panel (columns 0..k-1), left-looking: column j from the columns
+    // before it, one lane per row (rows j + lane, + 64, ...); the diagonal
+    // comes from lane 0 by a shuffle, so a column costs one pass and one sync.
     for (int j = 0; j < k; ++j) {
-        T dd = F[j * c + j];
         const T gd = j < kWave ? __shfl(gdl, j, kWave) : [&] {
             const int ge = A.gdiag[first + j];
             return ge >= 0 ? A.Gval[ge] : T(0);
         }();
-        wave_sync();
-        if (!(dd > A.piv_tol * gd) || !(dd > T(0))) {
-            if (lane == 0) A.st->piv = 1;
-            dd = gd > T(0) ? A.piv_tol * gd : T(1);
-        }
-        const T dj = sqrt(dd);
-        if (lane == 0) F[j * c + j] = dj;
-        for (int i = j + 1 + lane; i < c; i += kWave) F[j * c + i] = F[j * c + i] / dj;
-        wave_sync();
-        const int w = c - j - 1;
-        if (w > 0) {
-            const int da = kWave % w, db = kWave / w;
-            int b = lane / w, a = lane - b * w;
-            while (b < w) {
-                if (a >= b) {
-                    const int aa = j + 1 + a, bb = j + 1 + b;
-                    F[bb * c + aa] -= F[j * c + aa] * F[j * c + bb];
+        T dj = T(1);
+        for (int r0 = j; r0 < c; r0 += kWave) {
+            const int r = r0 + lane;
+            const int rr = r < c ? r : j;
+            T acc = F[j * c + rr];
+            int m = 0;
+            for (; m + 4 <= j; m += 4) {
+                T x[4], y[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    x[q] = F[(m + q) * c + rr];
+                    y[q] = F[(m + q) * c + j];
                 }
-                a += da;
-                b += db;
-                if (a >= w) {
-                    a -= w;
-                    ++b;
-                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc -= x[q] * y[q];
             }
+            for (; m < j; ++m) acc -= F[m * c + rr] * F[m * c + j];
+            if (r0 == j) {  // lane 0 holds the diagonal
+                T dd = __shfl(acc, 0, kWave);
+                if (!(dd > A.piv_tol * gd) || !(dd > T(0))) {
+                    if (lane == 0) A.st->piv = 1;
+                    dd = gd > T(0) ? A.piv_tol * gd : T(1);
+                }
+                dj = sqrt(dd);
+            }
+            if (r < c) F[j * c + r] = r == j ? dj : acc / dj;
         }
         wave_sync();
     }
+    // the trailing block in one pass: F_ab -= sum_{m<k} L_am L_bm, k <= b <= a
+    const int w = c - k;
+    if (w > 0) {
+        const int da = kWave % w, db = kWave / w;
+        int b = lane / w, a = lane - b * w;
+        while (b < w) {
+            if (a >= b) {
+                const int aa = k + a, bb = k + b;
+                T acc = F[bb * c + aa];
+                int m = 0;
+                for (; m + 4 <= k; m += 4) {
+                    T x[4], y[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        x[q] = F[(m + q) * c + aa];
+                        y[q] = F[(m + q) * c + bb];
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) acc -= x[q] * y[q];
+                }
+                for (; m < k; ++m) acc -= F[m * c + aa] * F[m * c + bb];
+                F[bb * c + aa] = acc;
+            }
+            a += da;
+            b += db;
+            if (a >= w) {
+                a -= w;
+                ++b;
+            }
+        }
+    }
+    wave_sync();
 }
 
-template <typename T, int NT, bool WAVE = false>
+template <typename T, int NT, bool WAVE = false, bool PF = true>
 __device__ inline void factor_front(const FactArgs<T>& A, int s, T* F) {
     if (NT == kWave) {
-        factor_front_wave<T>(A, s, F);
+        factor_front_wave<T, PF>(A, s, F);
         return;
     }
     using Tm = Team<WAVE>;
@@ -1359,8 +1441,10 @@ __global__ __launch_bounds__(kWave) void k_bwd(SolveArgs<T> A, const int32_t* __
 // below the kernel boundary.  MODE 0 factor, 1 forward solve, 2 backward solve
 // (levels walked top-down).
 // ---------------------------------------------------------------------------
-template <typename T, int MODE>
-__global__ __launch_bounds__(1024) void k_sweep_tasks(FactArgs<T> FA, SolveArgs<T> SA,
+constexpr int kBottomThreads = 256;  // workgroup of a bottom factor task
+
+template <typename T, int MODE, int MAXT = 1024>
+__global__ __launch_bounds__(MAXT) void k_sweep_tasks(FactArgs<T> FA, SolveArgs<T> SA,
                                                       const int32_t* __restrict__ fronts,
                                                       const int32_t* __restrict__ lvl_ptr,
                                                       const int32_t* __restrict__ lvl_off, int task_base,
@@ -1377,7 +1461,7 @@ __global__ __launch_bounds__(1024) void k_sweep_tasks(FactArgs<T> FA, SolveArgs<
         for (int q = a + w; q < b; q += nw) {
             const int s = fronts[q];
             if (MODE == 0) {
-                factor_front<T, kWave, true>(FA, s, my);
+                factor_front<T, kWave, true, (MAXT <= 512)>(FA, s, my);
                 const int c = FA.sn_c[s];
                 T* Fo = FA.F + FA.foff[s];
                 for (int idx = threadIdx.x & (kWave - 1); idx < c * c; idx += kWave) Fo[idx] = my[idx];
@@ -3865,6 +3949,8 @@ struct Instance::Impl {
     DBuf<int> ipart;
     // symbolic structures
     DBuf<int32_t> tptr, tpos, trow, tout, gptr, gloc, gdiag, g_sn_ptr;
+    DBuf<FrontDesc> fdesc;
+    DBuf<ChildDesc> cdesc;
     DBuf<int32_t> hpos;  // H^T position of each device H entry
     DBuf<double> hr;     // (sqrt(w) H)(sqrt(w) r) products in H^T order
     // device data layout: buses in Cuthill-McKee order from the slack, branches
@@ -3908,7 +3994,7 @@ struct Instance::Impl {
     bool tasks_ok = false;
     bool use_tasks = true;  // option: task sweeps when the plan exists
     int force_cut = 0, force_wcut = 0;  // option: factorization plan (0: the cost model's)
-    int n_bottom = 0, tail_waves = 0, bottom_waves = 4;
+    int n_bottom = 0, tail_waves = 0, bottom_waves = kBottomThreads / kWave;
     bool has_tail = false;
     DBuf<int32_t> t_fronts, t_lvl_ptr, t_lvl_off, t_mid;
     std::vector<int> mid_off, mid_cnt;
@@ -4238,7 +4324,7 @@ Instance::~Instance() {
     rel(d_->sbr);
     if (d_->pin_idx) (void)hipHostFree(d_->pin_idx);
     if (d_->pin_val) (void)hipHostFree(d_->pin_val); rel(d_->hpos); rel(d_->hr);
-    rel(d_->gptr); rel(d_->gloc); rel(d_->gdiag); rel(d_->g_sn_ptr); rel(d_->gpair); rel(d_->sn_first);
+    rel(d_->gptr); rel(d_->gloc); rel(d_->gdiag); rel(d_->g_sn_ptr); rel(d_->fdesc); rel(d_->cdesc); rel(d_->gpair); rel(d_->sn_first);
     rel(d_->sn_k); rel(d_->sn_c); rel(d_->child_ptr); rel(d_->child); rel(d_->rmap); rel(d_->rows);
     rel(d_->perm); rel(d_->pth); rel(d_->pv); rel(d_->lists); rel(d_->foff); rel(d_->rmoff); rel(d_->uoff);
     rel(d_->rows_off); rel(d_->moff); rel(d_->st); rel(d_->done); rel(d_->order_fwd); rel(d_->order_bwd); rel(d_->sn_parent);
@@ -4641,6 +4727,20 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
     D.rmoff.upload(S.sn_rmoff, bytes_, s);
     D.uoff.upload(S.sn_uoff, bytes_, s);
     D.rows_off.upload(S.sn_rows_off, bytes_, s);
+    {
+        std::vector<FrontDesc> fd(std::max<size_t>(1, S.nsn));
+        for (int q = 0; q < S.nsn; ++q)
+            fd[q] = FrontDesc{S.sn_c[q], S.sn_k[q], S.sn_first[q], S.g_sn_ptr[q], S.g_sn_ptr[q + 1] - S.g_sn_ptr[q],
+                              S.sn_child_ptr[q], S.sn_child_ptr[q + 1] - S.sn_child_ptr[q], 0};
+        std::vector<ChildDesc> cd(std::max<size_t>(1, S.sn_child.size()));
+        for (size_t p = 0; p < S.sn_child.size(); ++p) {
+            const int ch = S.sn_child[p];
+            cd[p] = ChildDesc{S.sn_c[ch], S.sn_k[ch], 0, 0, S.sn_foff[ch], S.sn_rmoff[ch]};
+        }
+        D.fdesc.upload(fd, bytes_, s);
+        D.cdesc.upload(cd, bytes_, s);
+        HIPCHECK(hipStreamSynchronize(s));
+    }
     // level lists split by front class
     std::vector<int32_t> lists;
     D.n_levels = S.n_levels;
@@ -4668,7 +4768,7 @@ void Instance::setup(const HostArrays& h, std::shared_ptr<const Symbolic> Sp) {
     D.level_cost_us = S.n_levels * (kLaunchUs + kFrontUs);
     if (S.nsn > 0 && S.max_c <= kFrontA) {
         const size_t slice_bytes = (size_t)S.max_c * S.max_c * sizeof(double);
-        D.bottom_waves = 4;
+        D.bottom_waves = kBottomThreads / kWave;
         D.tail_waves = (int)std::max<size_t>(1, std::min<size_t>(16, 65536 / std::max<size_t>(1, slice_bytes)));
         int best_cut = 0, best_w = 0;
         double best = D.level_cost_us;
@@ -5129,6 +5229,8 @@ template <typename T>
 void launch_factor_sweeps(Instance::Impl& D, T piv_tol) {
     PrecBufs<T>& B = bufs<T>(D);
     FactArgs<T> A;
+    A.fdesc = D.fdesc.p;
+    A.cdesc = D.cdesc.p;
     A.sn_first = D.sn_first.p;
     A.sn_k = D.sn_k.p;
     A.sn_c = D.sn_c.p;
@@ -5148,7 +5250,7 @@ void launch_factor_sweeps(Instance::Impl& D, T piv_tol) {
         SolveArgs<T> none{};
         const int slice = D.max_c * D.max_c;
         if (D.n_bottom)
-            k_sweep_tasks<T, 0><<<D.n_bottom, kWave * D.bottom_waves, D.bottom_waves * slice * sizeof(T), D.stream>>>(
+            k_sweep_tasks<T, 0, kBottomThreads><<<D.n_bottom, kWave * D.bottom_waves, D.bottom_waves * slice * sizeof(T), D.stream>>>(
                 A, none, D.t_fronts.p, D.t_lvl_ptr.p, D.t_lvl_off.p, 0, slice);
         for (size_t l = 0; l < D.mid_off.size(); ++l)
             k_factor_lds<T, kFrontA><<<D.mid_cnt[l], kWave, 0, D.stream>>>(A, D.t_mid.p + D.mid_off[l]);
