@@ -75,6 +75,11 @@ typedef struct SLE_ExtResult {
  *                     most 32 rows two to a wave and of at most 16 rows four
  *                     to a wave; 0: one subtree per wave; -1 (default): packed
  *                     for FP32 solve data, one per wave for FP64
+ *   "queue_plain"     REALTIME iterations on the cached factor queued one ahead
+ *                     of the running one: 2 (default) as plain kernel launches,
+ *                     the solve's first iteration replaying its captured graph;
+ *                     1: every such iteration as plain launches; 0: graph
+ *                     replays only (with use_cuda_graphs)
  *   "xcd_remap"       1 (default): the measurement, power, right-hand-side and
  *                     top-product kernels run contiguous block ranges per XCD
  *                     (process-wide on the device)

@@ -5942,19 +5942,29 @@ SolveOutcome Instance::solve(const SolveParams& p) {
         // not wait for the host between them.  If it does converge, the queued
         // iteration finds the device's stop flag and changes nothing.
         bool ahead = false;   // iteration it was launched by the previous pass
-        bool ahead_fp64 = false;
+        bool ahead_fp64 = false, ahead_graph = false;
         hipGraphExec_t ahead_exec = nullptr;
         uint32_t ahead_want = 0;
+        // closing iterations launched as plain kernels (option queue_plain)
+        // are queued and read back like graph replays: by their status's
+        // sequence number, with no stream synchronization
+        // (mode 1: every such iteration; mode 2: those queued ahead, the
+        // solve's first one replaying its graph)
+        auto plain_q = [&](bool fp64_, bool closing_, bool ahead_) {
+            return (queue_plain_ == 1 || (queue_plain_ == 2 && ahead_)) && closing_ && !profiling_ && !D.use_flow &&
+                   (fp64_ || p.pcg_steps <= 0);
+        };
         for (int it = 0; it < p.max_iter; ++it) {
             const bool split = precision_ == 0 && !fp64_factor_active_ && D.mixed64 && !D.mixed64_off;
-            bool refactor, fp64, newton, graph, closing, lead;
-            uint32_t want = 0;  // sequence number of this graph iteration's status
+            bool refactor, fp64, newton, graph, closing, lead, queued;
+            uint32_t want = 0;  // sequence number of this queued iteration's status
             hipGraphExec_t running = ahead ? ahead_exec : nullptr;
             if (ahead) {
                 refactor = false;
                 fp64 = ahead_fp64;
                 newton = false;
-                graph = closing = true;
+                graph = ahead_graph;
+                closing = queued = true;
                 lead = false;
                 want = ahead_want;
                 ahead = false;
@@ -5972,6 +5982,11 @@ SolveOutcome Instance::solve(const SolveParams& p) {
                 // location close with the measurement pass at the state they leave
                 closing = D.use_loc && !refactor && (fp64 || p.pcg_steps <= 0);
                 lead = !closed;
+                queued = graph || plain_q(fp64, closing, false);
+                if (plain_q(fp64, closing, false)) {
+                    graph = false;
+                    want = D.published + 1;
+                }
                 if (fp64) {
                     if (!D.f64.ready) D.f64.alloc(S, bytes_, D.pi.sizes());
                     SolveParams pd = p;
@@ -6014,10 +6029,12 @@ SolveOutcome Instance::solve(const SolveParams& p) {
             // predicted to converge (its step from the last known step and the
             // larger of the last contraction and the previous solve's final one,
             // kAheadMargin tol at least)
-            if (graph && closing && it + 1 < p.max_iter && p.mode == 0 && !p.robust && factor_valid_ && !stale) {
+            if (queued && closing && it + 1 < p.max_iter && p.mode == 0 && !p.robust && factor_valid_ && !stale) {
                 const bool nfp64 = precision_ == 1 || fp64_factor_active_;
-                const bool ngraph = use_graphs_ && !profiling_ && !D.use_flow && (nfp64 || p.pcg_steps <= 0);
                 const bool nclosing = D.use_loc && (nfp64 || p.pcg_steps <= 0);
+                const bool nplain = plain_q(nfp64, nclosing, true);
+                const bool ngraph =
+                    !nplain && use_graphs_ && !profiling_ && !D.use_flow && (nfp64 || p.pcg_steps <= 0);
                 const double ratio = prev_step > 0.0 ? last_step / prev_step : D.ahead_ratio;
                 const bool first = it == 0 || last_step >= 1e299;
                 const double predicted = first ? 1e300 : last_step * std::min(std::max(ratio, tail_ratio), 1.0);
@@ -6025,26 +6042,34 @@ SolveOutcome Instance::solve(const SolveParams& p) {
                 if (p.time_limit_ms > 0)
                     in_time = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
                                   .count() < p.time_limit_ms;
-                if (nfp64 == fp64 && ngraph && nclosing && in_time && predicted >= kAheadMargin * p.tol) {
+                if (nfp64 == fp64 && (ngraph || nplain) && nclosing && in_time && predicted >= kAheadMargin * p.tol) {
                     SolveParams pd = p;
                     if (fp64) pd.pcg_steps = 0;
-                    hipGraphExec_t g =
-                        fp64 ? iteration_graph<double>(D, S, pd, 1e-13, false, false, bytes_.epoch, out, true, false,
-                                                       running)
-                             : iteration_graph<float>(D, S, pd, 1e-5f, false, false, bytes_.epoch, out, true, false,
-                                                      running);
-                    HIPCHECK(hipGraphLaunch(g, D.stream));
+                    hipGraphExec_t g = nullptr;
+                    if (nplain) {
+                        if (fp64)
+                            iteration<double>(D, S, pd, false, 1e-13, pf, false, true, false);
+                        else
+                            iteration<float>(D, S, pd, false, 1e-5f, pf, false, true, false);
+                    } else {
+                        g = fp64 ? iteration_graph<double>(D, S, pd, 1e-13, false, false, bytes_.epoch, out, true,
+                                                           false, running)
+                                 : iteration_graph<float>(D, S, pd, 1e-5f, false, false, bytes_.epoch, out, true,
+                                                          false, running);
+                        HIPCHECK(hipGraphLaunch(g, D.stream));
+                    }
                     ahead = true;
                     ahead_exec = g;
+                    ahead_graph = !nplain;
                     ahead_fp64 = fp64;
                     ahead_want = want + 1;
                     out.ahead++;
                 }
             }
-            if (!graph) HIPCHECK(hipStreamSynchronize(D.stream));
+            if (!queued) HIPCHECK(hipStreamSynchronize(D.stream));
             // this iteration's status, copied out of its slot (an iteration
             // queued behind writes the other one)
-            const DevStatus cur = take_status(D, graph ? want : D.published + 1, graph, out);
+            const DevStatus cur = take_status(D, queued ? want : D.published + 1, queued, out);
             last = cur;
             pf.collect(ktimes_);
             if (cur.piv && newton) {

@@ -199,6 +199,10 @@ class Instance {
     void set_profiling(bool on) { profiling_ = on; }
     bool profiling() const { return profiling_; }
     void set_use_graphs(bool on) { use_graphs_ = on; }
+    // iterations on the cached factor that close with the measurement pass:
+    // launched as plain kernels (queued one ahead like graph replays) instead
+    // of replaying a captured graph; 0: none, 1: all, 2: those queued ahead
+    void set_queue_plain(int mode) { queue_plain_ = mode; }
     // dependency-driven sweeps (default when every front fits in LDS) or level-by-level launches
     void set_level_schedule(bool on);
     bool dataflow_sweeps() const;
@@ -275,6 +279,7 @@ class Instance {
     bool fp64_factor_active_ = false;  // mixed form fell back to FP64 after a pivot trigger
     bool profiling_ = false;
     bool use_graphs_ = false;
+    int queue_plain_ = 0;
     Ledger bytes_;
     KernelTimes ktimes_;
 };

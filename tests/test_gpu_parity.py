@@ -477,18 +477,21 @@ def test_autotune_keeps_results():
     e.close()
 
 
+@pytest.mark.parametrize("queue", [0, 1, 2])
 @pytest.mark.parametrize("fp64", [True, False])
-def test_graph_replays_match_eager_iterations(fp64):
+def test_graph_replays_match_eager_iterations(fp64, queue):
     """REALTIME iterations on the cached factor replay a captured HIP graph
-    (config use_cuda_graphs), the next one queued behind the running one: every
-    replay, with eager work (telemetry updates, state reads, a PRECISION solve
-    that refactors) between the replays, gives bit-identical states, J and
-    residuals to the same requests run eagerly one iteration at a time.  A
-    request repeated without new telemetry converges at its first iteration:
-    the iteration queued behind it changes nothing (the comparison stays
-    bit-identical, and the next request still iterates)."""
+    (config use_cuda_graphs), the next one queued behind the running one (as a
+    graph replay, or as plain launches: option queue_plain): every iteration,
+    with eager work (telemetry updates, state reads, a PRECISION solve that
+    refactors) between them, gives bit-identical states, J and residuals to
+    the same requests run eagerly one iteration at a time.  A request repeated
+    without new telemetry converges at its first iteration: the iteration
+    queued behind it changes nothing (the comparison stays bit-identical, and
+    the next request still iterates)."""
     g = GRIDS["geo10k"]()
     eg = engine_for(g, double_precision=fp64, use_cuda_graphs=1)
+    eg.set_option("queue_plain", queue)
     ee = engine_for(g, double_precision=fp64, use_cuda_graphs=0)
     rng = np.random.default_rng(21)
     for e in (eg, ee):
