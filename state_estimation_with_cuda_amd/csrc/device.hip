@@ -1754,13 +1754,21 @@ struct PiArgs {
     // units of the backward product (supernode, first column), longest path first
     const int32_t* bunit;
     int nunit;
+    // per top supernode: the fronts on its path to the root, upath[upoff[u] .. upoff[u+1])
+    const int32_t* upath;
+    const int32_t* upoff;
 };
 
 // One wave per top supernode s: the k_s unit right-hand sides of its columns are
 // carried up the path front by front.  At front p the block F (c_p x k_s) becomes
 // M_p F_top + [0; F_bot] with the front's solve matrix M_p = [L11^-1; -L21 L11^-1]:
 // its first k_p rows are the block rows of L_TT^-1 for p's columns, the rest is
-// extend-added (through rmap) into the parent's block.
+// extend-added (through rmap) into the parent's block.  The path's fronts come
+// from the plan: their sizes and offsets are loaded lane-parallel (a lane per
+// front), and the next front's M_p is loaded into registers during the
+// current front's product, so a step waits for LDS, not for memory.
+constexpr int kPiInvPre = 8;  // M_p entries per lane prefetched (fronts of c k <= 512)
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_pi_inv(PiArgs A, SolveArgs<T> SA, T* __restrict__ B) {
     extern __shared__ unsigned char smem_raw[];
@@ -1770,44 +1778,86 @@ __global__ __launch_bounds__(256) void k_pi_inv(PiArgs A, SolveArgs<T> SA, T* __
     T* F0 = reinterpret_cast<T*>(smem_raw) + (size_t)w * 3 * A.slice;
     T* F1 = F0 + A.slice;
     T* Mp = F1 + A.slice;
-    const int s0 = A.sn[u];
-    const int ks = SA.sn_k[s0];
+    const int ks = A.uk[u];
     const int Ps = A.plen[u];
     T* Bs = B + A.boff[u];
-    int p = s0;
-    int c = SA.sn_c[p];
+    const int32_t* path = A.upath + A.upoff[u];
+    const int nf = A.upoff[u + 1] - A.upoff[u];
+    // sizes and offsets of the path's fronts [chunk, chunk + 64), lane f for front chunk + f
+    int chunk = 0, c_l = 0, k_l = 0;
+    int64_t mo_l = 0, ro_l = 0;
+    auto fetch = [&](int base) {
+        chunk = base;
+        const int f = base + lane;
+        if (f < nf) {
+            const int p = path[f];
+            c_l = SA.sn_c[p];
+            k_l = SA.sn_k[p];
+            mo_l = SA.moff[p];
+            ro_l = SA.rmoff[p];
+        }
+    };
+    auto desc = [&](int f, int& c, int& k, int64_t& mo, int64_t& ro) {
+        if (f - chunk >= kWave) fetch(f);  // uniform: the next 64 fronts of a long path
+        c = __shfl(c_l, f - chunk, kWave);
+        k = __shfl(k_l, f - chunk, kWave);
+        mo = __shfl(mo_l, f - chunk, kWave);
+        ro = __shfl(ro_l, f - chunk, kWave);
+    };
+    fetch(0);
+    int c, k;
+    int64_t mo, ro;
+    desc(0, c, k, mo, ro);
+    T mv[kPiInvPre];
+    auto prefetch = [&](int64_t moff, int n) {
+#pragma unroll
+        for (int q = 0; q < kPiInvPre; ++q) {
+            const int e = lane + q * kWave;
+            mv[q] = e < n ? SA.M[moff + e] : T(0);
+        }
+    };
+    prefetch(mo, c * k);
     // F is c x ks, row-major: F[i * ks + jj]
     for (int e = lane; e < c * ks; e += kWave) {
         const int i = e / ks, jj = e - i * ks;
         F0[e] = i == jj ? T(1) : T(0);
     }
-    int ro = 0;
-    for (;;) {
-        const int k = SA.sn_k[p];
-        const int q = A.parent[p];
-        const int cq = q >= 0 ? SA.sn_c[q] : 0;
-        const T* Mg = SA.M + SA.moff[p];
-        for (int e = lane; e < c * k; e += kWave) Mp[e] = Mg[e];  // c x k, col-major
-        for (int e = lane; e < cq * ks; e += kWave) F1[e] = T(0);
+    int ro_b = 0;  // block rows written so far
+    for (int f = 0;; ++f) {
+        const bool last = f + 1 >= nf;
+        int cn = 0, kn = 0;
+        int64_t mon = 0, ron = 0;
+        if (!last) desc(f + 1, cn, kn, mon, ron);
+        // this front's M_p (c x k, column-major) into LDS: the prefetched part, then any rest
+#pragma unroll
+        for (int q = 0; q < kPiInvPre; ++q) {
+            const int e = lane + q * kWave;
+            if (e < c * k) Mp[e] = mv[q];
+        }
+        for (int e = lane + kPiInvPre * kWave; e < c * k; e += kWave) Mp[e] = SA.M[mo + e];
+        for (int e = lane; e < cn * ks; e += kWave) F1[e] = T(0);
+        if (!last) prefetch(mon, cn * kn);  // the next front's, in flight during this product
         wave_sync();
-        const int32_t* rm = SA.rmap + SA.rmoff[p];
+        const int32_t* rm = SA.rmap + ro;
         for (int e = lane; e < c * ks; e += kWave) {
             const int i = e / ks, jj = e - i * ks;
             T acc = i < k ? T(0) : F0[e];
             for (int j = 0; j < k; ++j) acc += Mp[j * c + i] * F0[j * ks + jj];
             if (i < k)
-                Bs[(ro + i) + (int64_t)Ps * jj] = acc;
+                Bs[(ro_b + i) + (int64_t)Ps * jj] = acc;
             else
                 F1[rm[i - k] * ks + jj] = acc;
         }
-        ro += k;
-        if (q < 0) break;
+        ro_b += k;
+        if (last) break;
         wave_sync();
         T* t = F0;
         F0 = F1;
         F1 = t;
-        p = q;
-        c = cq;
+        c = cn;
+        k = kn;
+        mo = mon;
+        ro = ron;
     }
 }
 
@@ -3494,7 +3544,7 @@ struct PiPlan {
     int ntop = 0, ntc = 0, nseg = 0, slice = 0;
     int64_t nblock = 0, nq = 0;
     std::vector<int32_t> sn, plen, ploc, tcol, colsn, src, segptr, cptr, parent, segdst, mrows, cplen, cjj, ufirst,
-        uk, bunit;
+        uk, bunit, upath, upoff;
     std::vector<int64_t> boff, poff, pos, segq, segc, cpos, cboff, cpoff;
     // dense bottom subtrees (see SubArgs)
     bool dense = false;
@@ -3797,10 +3847,14 @@ static PiPlan make_pi_plan(const Symbolic& S, int cut, bool dense, bool mid) {
     P.poff.resize(P.ntop);
     P.ufirst.resize(P.ntop);
     P.uk.resize(P.ntop);
+    P.upoff.assign(1, 0);
     for (int u = 0; u < P.ntop; ++u) {
         const int s0 = P.sn[u];
         P.ufirst[u] = S.sn_first[s0];
         P.uk[u] = S.sn_k[s0];
+        for (int p = s0; p >= 0; p = S.sn_parent[p]) P.upath.push_back(p);
+        if (P.upath.size() > (size_t)INT32_MAX) throw std::logic_error("partitioned inverse: top paths too long");
+        P.upoff.push_back((int32_t)P.upath.size());
         P.boff[u] = P.nblock;
         P.poff[u] = (int64_t)P.ploc.size();
         int len = 0;
@@ -4018,7 +4072,8 @@ struct Instance::Impl {
         bool ok = false;
         int cut = 0, ntop = 0, ntc = 0, nseg = 0, slice = 0;
         int64_t nblock = 0, nq = 0;
-        DBuf<int32_t> sn, plen, ploc, tcol, colsn, segptr, cptr, parent, segdst, mrows, cplen, cjj, ufirst, uk, bunit;
+        DBuf<int32_t> sn, plen, ploc, tcol, colsn, segptr, cptr, parent, segdst, mrows, cplen, cjj, ufirst, uk, bunit,
+            upath, upoff;
         int nmrows = 0, nunit = 0;
         DBuf<int64_t> boff, poff, pos, segq, segc, cpos, cboff, cpoff;
         int64_t usum = 0, npath = 0;  // sum of subtree root update rows; path rows of the top
@@ -4194,6 +4249,8 @@ static void build_pi(Instance::Impl& D, const Symbolic& S, Ledger& bytes, hipStr
             up(Q.uk, PP.uk);
             up(Q.bunit, PP.bunit);
             Q.nunit = (int)PP.bunit.size();
+            up(Q.upath, PP.upath);
+            up(Q.upoff, PP.upoff);
             up(Q.mrows, PP.mrows);
             Q.nmrows = (int)PP.mrows.size();
             Q.npath = (int64_t)PP.ploc.size();
@@ -4335,7 +4392,7 @@ Instance::~Instance() {
     {
         auto& P = d_->pi;
         rel(P.sn); rel(P.plen); rel(P.ploc); rel(P.tcol); rel(P.colsn); rel(P.segptr); rel(P.cptr); rel(P.parent);
-        rel(P.segdst); rel(P.ufirst); rel(P.uk); rel(P.bunit); rel(P.mrows); rel(P.cplen); rel(P.cjj); rel(P.cboff); rel(P.cpoff);
+        rel(P.segdst); rel(P.ufirst); rel(P.uk); rel(P.bunit); rel(P.upath); rel(P.upoff); rel(P.mrows); rel(P.cplen); rel(P.cjj); rel(P.cboff); rel(P.cpoff);
         rel(P.boff); rel(P.poff); rel(P.pos); rel(P.segq); rel(P.segc); rel(P.cpos); rel(P.part);
         rel(P.sroot); rel(P.sn_n); rel(P.sR); rel(P.gsrc); rel(P.gdst); rel(P.geptr); rel(P.gbase); rel(P.grs); rel(P.gmo); rel(P.gmask); rel(P.scptr);
         rel(P.scols); rel(P.bins); rel(P.smoff); rel(P.suoff); rel(P.surow); rel(P.acptr); rel(P.acpos); rel(P.pkind); rel(P.psub);
@@ -5299,7 +5356,7 @@ PiArgs pi_args(const Instance::Impl& D) {
     const Instance::Impl::PiDev& P = D.pi;
     return PiArgs{P.sn.p,   P.plen.p,  P.boff.p,  P.poff.p,  P.ploc.p, P.parent.p, P.tcol.p, P.colsn.p,
                   P.ntop,   P.ntc,     P.slice,   P.cboff.p, P.cpoff.p, P.cplen.p,  P.cjj.p,    P.ufirst.p,
-                  P.uk.p,   P.bunit.p, P.nunit};
+                  P.uk.p,   P.bunit.p, P.nunit,   P.upath.p, P.upoff.p};
 }
 
 bool pi_active(const Instance::Impl& D) { return D.use_pi && D.pi.ok; }
