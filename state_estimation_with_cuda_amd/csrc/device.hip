@@ -565,7 +565,7 @@ struct LocWeight {
 // of a chunk are issued together (indices clamped to the last entry), so a walk
 // waits for memory once per chunk rather than once per entry.
 constexpr int kLocChunk = 4;
-constexpr int kLocChunkFlow = 4;  // flow rows of one branch end (1, 2 and 4 measured alike)
+constexpr int kLocChunkFlow = 1;  // flow rows of one branch end, one at a time (fewer VGPRs: 120 -> 96)
 
 template <int K = kLocChunk, typename F>
 __device__ inline void loc_rows(const uint8_t* __restrict__ mcode, const double* __restrict__ zv,
