@@ -2948,6 +2948,20 @@ __global__ __launch_bounds__(kFinish) void k_maxfinish(int nparts, const double*
                                                        double stop_tol) {
     const int tid = threadIdx.x;
     const int stop = st->stop;
+    // thread 0's status fields, loaded with the partials (one round of loads)
+    int nan = 0, piv = 0, ferr = 0;
+    uint32_t seq = 0;
+    double J0 = 0.0, R0 = -1.0;
+    int I0 = -1;
+    if (tid == 0) {
+        nan = st->nan;
+        piv = st->piv;
+        ferr = st->flow_err;
+        seq = st->seq;
+        J0 = st->J;
+        R0 = st->rmax;
+        I0 = st->ridx;
+    }
     double mx = 0.0, J = 0.0, R = -1.0;
     int I = -1;
     const int np = max(nparts, nj);
@@ -2999,20 +3013,20 @@ __global__ __launch_bounds__(kFinish) void k_maxfinish(int nparts, const double*
         st->rmax = RR;
         st->ridx = II;
     }
-    const int nan = st->nan, piv = st->piv, ferr = st->flow_err;
-    if (stop_tol > 0.0 && !ferr && !piv && (q < stop_tol || nan)) st->stop = 1;
+    const int stopped = stop_tol > 0.0 && !ferr && !piv && (q < stop_tol || nan);
+    if (stopped) st->stop = 1;
     if (host_out) {  // two slots, written in turn
-        const uint32_t sq = st->seq + 1;
+        const uint32_t sq = seq + 1;
         st->seq = sq;
         DevStatus* ho = host_out + (sq & 1);
         ho->maxdx = q;
-        ho->J = nj > 0 ? JJ : st->J;
-        ho->rmax = nj > 0 ? RR : st->rmax;
-        ho->ridx = nj > 0 ? II : st->ridx;
+        ho->J = nj > 0 ? JJ : J0;
+        ho->rmax = nj > 0 ? RR : R0;
+        ho->ridx = nj > 0 ? II : I0;
         ho->nan = nan;
         ho->piv = piv;
         ho->flow_err = ferr;
-        ho->stop = st->stop;
+        ho->stop = stopped;
         // the sequence number last, after the fields are visible to the host
         __threadfence_system();
         __hip_atomic_store(&ho->seq, sq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
