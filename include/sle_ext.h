@@ -80,6 +80,11 @@ typedef struct SLE_ExtResult {
  *                     the solve's first iteration replaying its captured graph;
  *                     1: every such iteration as plain launches; 0: graph
  *                     replays only (with use_cuda_graphs)
+ *   "fork_sub_m"      1 (default): a factorization forms the solve matrices of
+ *                     the subtrees inside its bottom tasks on a second stream
+ *                     while the top of the tree factors in one workgroup; 2:
+ *                     from the end of the bottom tasks, beside the middle
+ *                     levels too; 0: in order on the instance's stream
  *   "xcd_remap"       1 (default): the measurement, power, right-hand-side and
  *                     top-product kernels run contiguous block ranges per XCD
  *                     (process-wide on the device)

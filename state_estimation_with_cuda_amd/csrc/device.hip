@@ -3976,6 +3976,12 @@ static PiPlan make_pi_plan(const Symbolic& S, int cut, bool dense, bool mid) {
 struct Instance::Impl {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // a second stream for the bottom subtrees' solve matrices, which overlap
+    // the rest of the factorization (fork after the bottom tasks, join before
+    // the top inverse's gather)
+    hipStream_t side = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    int fork_subm = 1;  // option "fork_sub_m": 0 off, 1 after the middle levels, 2 after the bottom tasks
     int n = 0, nb = 0, m = 0, ns = 0, slack = 0;
     // topology / parameters
     DBuf<int32_t> bus_type, from, to;
@@ -4106,6 +4112,11 @@ struct Instance::Impl {
         DBuf<int64_t> smoff, suoff, surow;
         int bin_off[kSubBins + 1] = {0, 0, 0, 0, 0};
         size_t bin_bytes[kSubBins] = {0, 0, 0, 0};
+        // per bin, its first bin_early subtrees lie inside the factorization's
+        // bottom tasks (split_sub_bins): their matrices can be formed while
+        // the rest of the tree factors
+        int bin_early[kSubBins] = {0, 0, 0, 0};
+        std::vector<int32_t> hbins, hroot;  // host copies: bins, and each subtree's root front
         // subtrees of each level (bottom, middle) by rows R (<= 16, 32, 64, 128)
         // and by columns n (<= 8, 16, 32, 64), for the plan description
         int rhist[2][4] = {}, nhist[2][4] = {};
@@ -4183,6 +4194,39 @@ struct Instance::Impl {
     std::vector<Staged> staged;
 };
 
+// Each bin of the subtree-matrix launches, reordered so that the subtrees
+// whose fronts all lie in the factorization's bottom tasks (a front is in them
+// when its subtree holds at most plan_cut fronts) come first, each part
+// largest first as before; bin_early[q] counts them.  Those matrices depend on
+// the bottom task sweep alone.  Rerun whenever either plan changes.
+static void split_sub_bins(Instance::Impl& D, const Symbolic& S, hipStream_t s) {
+    Instance::Impl::PiDev& PI = D.pi;
+    for (int q = 0; q < kSubBins; ++q) PI.bin_early[q] = 0;
+    if (!PI.ok || !PI.dense || PI.hbins.empty() || !D.tasks_ok || D.plan_cut <= 0) return;
+    const int nsn = S.nsn;
+    for (int32_t r : PI.hroot)
+        if (r < 0 || r >= nsn) return;  // a plan of another structure (rebuilt next)
+    std::vector<int32_t> size(nsn, 1);
+    for (int f = 0; f < nsn; ++f)
+        if (S.sn_parent[f] >= 0) size[S.sn_parent[f]] += size[f];
+    std::vector<int32_t> order(PI.hbins.size());
+    for (int q = 0; q < kSubBins; ++q) {
+        const int a = PI.bin_off[q], b = PI.bin_off[q + 1];
+        int k = a;
+        for (int pass = 0; pass < 2; ++pass)
+            for (int i = a; i < b; ++i) {
+                const int t = PI.hbins[i];
+                const bool early = size[PI.hroot[t]] <= D.plan_cut;
+                if (early == (pass == 0)) order[k++] = t;
+            }
+        for (int i = a; i < b; ++i)
+            if (size[PI.hroot[order[i]]] <= D.plan_cut) ++PI.bin_early[q];
+    }
+    PI.hbins = order;
+    PI.bins.upload(order.data(), order.size(), s);  // same size: the buffer stays
+    HIPCHECK(hipStreamSynchronize(s));
+}
+
 // Task-sweep plan of the factorization: bottom subtrees of at most `cut` fronts
 // per workgroup, top levels wider than `wcut` fronts one launch each, the rest
 // in one workgroup.
@@ -4204,6 +4248,7 @@ static void build_factor_plan(Instance::Impl& D, const Symbolic& S, int cut, int
     D.plan_wcut = wcut;
     D.plan_cost_us = P.cost_us;
     D.tasks_ok = true;
+    split_sub_bins(D, S, s);
 }
 
 // Partitioned-inverse plan for the current pi_cut / pi_waves: bottom LDS tasks
@@ -4335,6 +4380,8 @@ static void build_pi(Instance::Impl& D, const Symbolic& S, Ledger& bytes, hipStr
                 Q.npw = (int)PP.pkind.size();
                 for (int q = 0; q <= kSubBins; ++q) Q.bin_off[q] = PP.bin_off[q];
                 for (int q = 0; q < kSubBins; ++q) Q.bin_bytes[q] = PP.bin_bytes[q];
+                Q.hbins = PP.bins;
+                Q.hroot = PP.sroot;
             }
             HIPCHECK(hipStreamSynchronize(s));
             Q.cut = cut;
@@ -4347,6 +4394,7 @@ static void build_pi(Instance::Impl& D, const Symbolic& S, Ledger& bytes, hipStr
             Q.ok = true;
         }
     }
+    split_sub_bins(D, S, s);
 }
 
 Instance::Instance(int device, int precision) : device_(device), precision_(precision) {
@@ -4356,6 +4404,9 @@ Instance::Instance(int device, int precision) : device_(device), precision_(prec
     d_->stream = EnginePool::stream();
     HIPCHECK(hipEventCreate(&d_->ev0));
     HIPCHECK(hipEventCreate(&d_->ev1));
+    d_->side = EnginePool::stream();
+    HIPCHECK(hipEventCreateWithFlags(&d_->fork_ev, hipEventDisableTiming));
+    HIPCHECK(hipEventCreateWithFlags(&d_->join_ev, hipEventDisableTiming));
     d_->hst = static_cast<DevStatus*>(EnginePool::pinned(sizeof(DevStatus)));
     HIPCHECK(hipHostGetDevicePointer((void**)&d_->hst_dev, d_->hst, 0));
     d_->hslot = static_cast<DevStatus*>(EnginePool::pinned(2 * sizeof(DevStatus)));
@@ -4428,6 +4479,12 @@ Instance::~Instance() {
     if (d_->hslot) EnginePool::give_back_pinned(d_->hslot, 2 * sizeof(DevStatus));
     if (d_->ev0) (void)hipEventDestroy(d_->ev0);
     if (d_->ev1) (void)hipEventDestroy(d_->ev1);
+    if (d_->fork_ev) (void)hipEventDestroy(d_->fork_ev);
+    if (d_->join_ev) (void)hipEventDestroy(d_->join_ev);
+    if (d_->side) {
+        (void)hipStreamSynchronize(d_->side);
+        EnginePool::give_back(d_->side);
+    }
     if (d_->stream) {
         (void)hipStreamSynchronize(d_->stream);
         EnginePool::give_back(d_->stream);
@@ -5296,8 +5353,11 @@ void launch_spgemm(Instance::Impl& D, const Symbolic& S) {
                                                             bufs<T>(D).Gval.p);
 }
 
+// fork (optional): called with D.fork_ev recorded on the stream after the
+// bottom tasks (fork_at 2) or after the middle levels (fork_at 1), once the
+// launch that follows that point is queued (so that it is dispatched first).
 template <typename T>
-void launch_factor_sweeps(Instance::Impl& D, T piv_tol) {
+void launch_factor_sweeps(Instance::Impl& D, T piv_tol, void (*fork)(Instance::Impl&) = nullptr, int fork_at = 0) {
     PrecBufs<T>& B = bufs<T>(D);
     FactArgs<T> A;
     A.fdesc = D.fdesc.p;
@@ -5323,11 +5383,16 @@ void launch_factor_sweeps(Instance::Impl& D, T piv_tol) {
         if (D.n_bottom)
             k_sweep_tasks<T, 0, kBottomThreads><<<D.n_bottom, kWave * D.bottom_waves, D.bottom_waves * slice * sizeof(T), D.stream>>>(
                 A, none, D.t_fronts.p, D.t_lvl_ptr.p, D.t_lvl_off.p, 0, slice);
-        for (size_t l = 0; l < D.mid_off.size(); ++l)
+        if (fork && fork_at == 2) HIPCHECK(hipEventRecord(D.fork_ev, D.stream));
+        for (size_t l = 0; l < D.mid_off.size(); ++l) {
             k_factor_lds<T, kFrontA><<<D.mid_cnt[l], kWave, 0, D.stream>>>(A, D.t_mid.p + D.mid_off[l]);
+            if (fork && fork_at == 2 && l == 0) fork(D);
+        }
+        if (fork && fork_at == 1) HIPCHECK(hipEventRecord(D.fork_ev, D.stream));
         if (D.has_tail)
             k_sweep_tasks<T, 0><<<1, kWave * D.tail_waves, D.tail_waves * slice * sizeof(T), D.stream>>>(
                 A, none, D.t_fronts.p, D.t_lvl_ptr.p, D.t_lvl_off.p, D.n_bottom, slice);
+        if (fork && (fork_at == 1 || D.mid_off.empty())) fork(D);
         return;
     }
     if (D.use_flow) {
@@ -5410,18 +5475,44 @@ void allow_big_lds(const void* kernel, size_t bytes) {
 }
 
 // L_TT^-1 blocks of the current factor and their row-grouped copy
+// Subtree matrices of bins [first, first + count) of every bin q: part 0 the
+// early subtrees of each bin, part 1 the rest, part 2 all.
 template <typename T>
-void launch_pi_inverse(Instance::Impl& D) {
+void launch_sub_m(Instance::Impl& D, int part, hipStream_t st) {
+    Instance::Impl::PiDev& PI = D.pi;
+    for (int q = 0; q < kSubBins; ++q) {
+        const int a = PI.bin_off[q] + (part == 1 ? PI.bin_early[q] : 0);
+        const int b = part == 0 ? PI.bin_off[q] + PI.bin_early[q] : PI.bin_off[q + 1];
+        if (b <= a) continue;
+        if (PI.bin_bytes[q] > 65536) allow_big_lds((const void*)k_sub_m_rows<T>, PI.bin_bytes[q]);
+        k_sub_m_rows<T><<<b - a, kBlock, PI.bin_bytes[q], st>>>(sub_args(D), solve_args<T>(D), PI.bins.p + a,
+                                                              bufs<T>(D).msub.p);
+    }
+}
+
+// The early subtree matrices on the second stream, behind D.fork_ev (recorded
+// after the bottom task sweep or later); the factorization goes on on the first.
+template <typename T>
+void fork_sub_m(Instance::Impl& D) {
+    HIPCHECK(hipStreamWaitEvent(D.side, D.fork_ev, 0));
+    launch_sub_m<T>(D, 0, D.side);
+    HIPCHECK(hipEventRecord(D.join_ev, D.side));
+}
+
+// Whether a factorization forks the early subtree matrices (fork_sub_m).
+bool sub_m_forks(const Instance::Impl& D) {
+    if (!D.side || D.use_flow || !D.use_tasks || !D.tasks_ok || !D.n_bottom || !sub_active(D)) return false;
+    for (int q = 0; q < kSubBins; ++q)
+        if (D.pi.bin_early[q]) return true;
+    return false;
+}
+
+template <typename T>
+void launch_pi_inverse(Instance::Impl& D, bool forked = false) {
     Instance::Impl::PiDev& PI = D.pi;
     PrecBufs<T>& B = bufs<T>(D);
-    if (sub_active(D))
-        for (int q = 0; q < kSubBins; ++q) {
-            const int cnt = PI.bin_off[q + 1] - PI.bin_off[q];
-            if (!cnt) continue;
-            if (PI.bin_bytes[q] > 65536) allow_big_lds((const void*)k_sub_m_rows<T>, PI.bin_bytes[q]);
-            k_sub_m_rows<T><<<cnt, kBlock, PI.bin_bytes[q], D.stream>>>(sub_args(D), solve_args<T>(D),
-                                                                       PI.bins.p + PI.bin_off[q], B.msub.p);
-        }
+    if (sub_active(D)) launch_sub_m<T>(D, forked ? 1 : 2, D.stream);
+    if (forked) HIPCHECK(hipStreamWaitEvent(D.stream, D.join_ev, 0));  // the early ones joined
     if (!pi_active(D) || PI.ntop == 0) return;
     const size_t per_wave = 3 * (size_t)PI.slice * sizeof(T);
     const int wpb = (int)std::max<size_t>(1, std::min<size_t>(4, 65536 / per_wave));
@@ -5460,7 +5551,7 @@ void round_solve_data(Instance::Impl& D, Ledger& bytes) {
 }
 
 template <typename T>
-void launch_solve_prep(Instance::Impl& D) {
+void launch_solve_prep(Instance::Impl& D, bool forked = false) {
     // the LDS task solves (every front fits) read M of every front; with dense
     // bottom subtrees only the top inverse blocks need M (of the top fronts)
     const bool top_only = sub_active(D);
@@ -5468,13 +5559,14 @@ void launch_solve_prep(Instance::Impl& D) {
     if (nsn && D.max_c <= kFrontB)
         k_front_m<T><<<(nsn + kBlock / kWave - 1) / (kBlock / kWave), kBlock, 0, D.stream>>>(
             nsn, solve_args<T>(D), bufs<T>(D).M.p, top_only ? D.pi.sn.p : nullptr);
-    launch_pi_inverse<T>(D);
+    launch_pi_inverse<T>(D, forked);
 }
 
 template <typename T>
 void launch_factor(Instance::Impl& D, T piv_tol) {
-    launch_factor_sweeps<T>(D, piv_tol);
-    launch_solve_prep<T>(D);
+    const bool fork = D.fork_subm > 0 && sub_m_forks(D);
+    launch_factor_sweeps<T>(D, piv_tol, fork ? fork_sub_m<T> : nullptr, D.fork_subm);
+    launch_solve_prep<T>(D, fork);
 }
 
 struct Prof;
@@ -6843,6 +6935,11 @@ void Instance::set_dense_subtrees(bool on) {
     d_->pi_dense = on;
 }
 
+void Instance::set_fork_subtree_matrices(int mode) {
+    ++d_->plan_epoch;  // captured refactoring iterations hold the fork or not
+    d_->fork_subm = mode;
+}
+
 void Instance::set_middle_subtrees(bool on) {
     ++d_->plan_epoch;
     d_->pi_mid = on;
@@ -6935,16 +7032,19 @@ std::string Instance::sweep_plan() const {
     const Impl::LdsDev& L = D.lds[precision_ == 1 || fp64_factor_active_ ? 1 : 0];
     if (D.use_pi && D.pi.ok && !D.use_flow) {
         const Impl::LdsDev& PB = D.pi.bottom[precision_ == 1 || fp64_factor_active_ ? 1 : 0];
+        int forked = 0;  // subtree matrices a factorization forms on the second stream
+        if (D.fork_subm > 0 && sub_m_forks(D))
+            for (int q = 0; q < kSubBins; ++q) forked += D.pi.bin_early[q];
         if (D.pi.dense)
             snprintf(buf, sizeof(buf),
                      "; solve partitioned-inverse(cut=%d dense-subtrees=%d+%d/%lld entries top supernodes=%d "
                      "columns=%d blocks=%lld row-entries=%lld segments=%d; bottom rows<=16/32/64/128 %d/%d/%d/%d "
-                     "cols<=8/16/32/64 %d/%d/%d/%d; middle rows %d/%d/%d/%d cols %d/%d/%d/%d)",
+                     "cols<=8/16/32/64 %d/%d/%d/%d; middle rows %d/%d/%d/%d cols %d/%d/%d/%d; forked matrices=%d)",
                      D.pi.cut, D.pi.nsub0, D.pi.nsub - D.pi.nsub0, (long long)D.pi.msub, D.pi.ntop, D.pi.ntc,
                      (long long)D.pi.nblock, (long long)D.pi.nq, D.pi.nseg, D.pi.rhist[0][0], D.pi.rhist[0][1],
                      D.pi.rhist[0][2], D.pi.rhist[0][3], D.pi.nhist[0][0], D.pi.nhist[0][1], D.pi.nhist[0][2],
                      D.pi.nhist[0][3], D.pi.rhist[1][0], D.pi.rhist[1][1], D.pi.rhist[1][2], D.pi.rhist[1][3],
-                     D.pi.nhist[1][0], D.pi.nhist[1][1], D.pi.nhist[1][2], D.pi.nhist[1][3]);
+                     D.pi.nhist[1][0], D.pi.nhist[1][1], D.pi.nhist[1][2], D.pi.nhist[1][3], forked);
         else
             snprintf(buf, sizeof(buf),
                      "; solve partitioned-inverse(cut=%d bottom=%d/%zuB%s top supernodes=%d columns=%d blocks=%lld "

@@ -193,6 +193,10 @@ int Engine::set_option(const char* key, double value) {
         if (value != -1 && value != 0 && value != 1) return ST_INVALID_ARGUMENT;
         opt_.pinv_pack = (int)value;
         if (inst_) inst_->set_subtree_packing(opt_.pinv_pack);
+    } else if (k == "fork_sub_m") {
+        if (value != 0 && value != 1 && value != 2) return ST_INVALID_ARGUMENT;
+        opt_.fork_sub_m = (int)value;
+        if (inst_) inst_->set_fork_subtree_matrices(opt_.fork_sub_m);
     } else if (k == "queue_plain") {
         if (value != 0 && value != 1 && value != 2) return ST_INVALID_ARGUMENT;
         opt_.queue_plain = (int)value;
@@ -713,6 +717,7 @@ int Engine::rebuild() {
     inst_->set_subtree_workgroups(opt_.pinv_wg_fwd, opt_.pinv_wg_bwd);
     inst_->set_subtree_packing(opt_.pinv_pack);
     inst_->set_queue_plain(opt_.queue_plain);
+    inst_->set_fork_subtree_matrices(opt_.fork_sub_m);
     inst_->set_meas_by_type(opt_.meas_by_type);
     inst_->set_rhs_by_location(opt_.rhs_by_location);
     inst_->set_mixed_fp64_factor(opt_.mixed_fp64_factor);

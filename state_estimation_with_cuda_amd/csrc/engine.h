@@ -99,6 +99,7 @@ struct EngineOptions {
     int pinv_wg_bwd = 0;          // each; 4, 8, 16: a workgroup each; 0: by precision) and backward
     int pinv_pack = -1;           // bottom subtree products: small subtrees two or four to a wave
                                   // (1), one a wave (0), by precision (-1: packed for FP32 solve data)
+    int fork_sub_m = 1;           // bottom subtree matrices overlap the top (1) or the middle and top (2)
     int queue_plain = 2;          // iterations on the cached factor as plain launches, not graphs
                                   // (0: none, 1: all, 2: those queued ahead)
     bool xcd_remap = true;        // XCD-aware block order of the gather kernels

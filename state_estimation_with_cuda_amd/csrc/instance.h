@@ -226,6 +226,10 @@ class Instance {
     // partitioned-inverse solve: bottom subtrees as one dense solve matrix each
     // (true) or as LDS tasks walking their fronts (takes effect at setup)
     void set_dense_subtrees(bool on);
+    // the bottom subtrees' solve matrices formed on a second stream while the
+    // rest of the tree factors: 0 in order, 1 (default) beside the top of the
+    // tree, 2 beside the middle levels and the top
+    void set_fork_subtree_matrices(int mode);
     // with dense subtrees: a second level of them inside the forest above the
     // bottom ones, leaving a smaller top to the partitioned inverse (setup)
     void set_middle_subtrees(bool on);

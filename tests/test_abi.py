@@ -177,4 +177,6 @@ def test_options_documented_in_header_are_accepted():
         e.set_option("pinv_wg_fwd", 3)  # waves per middle subtree: 0, 1, 4, 8 or 16
     with pytest.raises(sle.SLEError):
         e.set_option("queue_plain", 3)  # 0, 1 or 2
+    with pytest.raises(sle.SLEError):
+        e.set_option("fork_sub_m", 3)  # 0, 1 or 2
     e.close()

@@ -528,6 +528,49 @@ def test_graph_replays_match_eager_iterations(fp64, queue):
     ee.close()
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("fp64", [True, False])
+@pytest.mark.parametrize("fork", [1, 2])
+def test_forked_subtree_matrices_match_in_order(fp64, fork):
+    """A factorization forms the solve matrices of the subtrees inside its
+    bottom tasks on a second stream while the middle levels and the top factor
+    (option fork_sub_m: 1 beside the top, 2 from the end of the bottom tasks),
+    joined before the top inverse's gather.  PRECISION
+    solves (a factorization in every iteration, in captured graphs) and the
+    REALTIME solves on their factor give bit-identical states, J and residuals
+    to the same requests with every matrix formed in order on one stream."""
+    import re
+
+    g = GRIDS["geo10k"]()
+    ef = engine_for(g, double_precision=fp64, use_cuda_graphs=1)
+    ef.set_option("fork_sub_m", fork)
+    eo = engine_for(g, double_precision=fp64, use_cuda_graphs=1)
+    eo.set_option("fork_sub_m", 0)
+    rng = np.random.default_rng(33)
+    for step in range(5):
+        if step:
+            idx = np.sort(rng.choice(g.m, g.m // 10, replace=False)).astype(np.int32)
+            val = g.z[idx] * (1.0 + 1e-3 * rng.standard_normal(idx.size))
+        out = []
+        for e in (ef, eo):
+            if step:
+                e.set_telemetry_indexed(idx, val)
+            mode = 1 if step in (0, 3) else 0
+            r = e.solve(mode=mode, max_iterations=50, tolerance=1e-6, flat_start=step == 0)
+            assert r["status"] == 0, (step, r)
+            out.append((r["iterations"], r["objective"]) + e.state() + (e.results()["r"],))
+        (ig, jg, Vg, tg, rg), (io, jo, Vo, to, ro) = out
+        assert (ig, jg) == (io, jo), (step, ig, io, jg, jo)
+        assert np.array_equal(Vg, Vo) and np.array_equal(tg, to), step
+        assert np.array_equal(rg, ro), step
+    nf = [int(m) for m in re.findall(r"forked matrices=(\d+)", ef.describe())]
+    no = [int(m) for m in re.findall(r"forked matrices=(\d+)", eo.describe())]
+    assert nf and nf[0] > 0, ef.describe()
+    assert no == [0], eo.describe()
+    ef.close()
+    eo.close()
+
+
 @pytest.mark.parametrize("fp64", [True, False])
 def test_results_stay_those_of_the_solve(fp64):
     """Injections, flows, h(x) and residuals are evaluated on first request after
