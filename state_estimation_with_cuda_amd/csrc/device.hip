@@ -5784,10 +5784,10 @@ void launch_trisolve(Instance::Impl& D, Prof* pf = nullptr) {
 }
 
 template <typename T>
-void launch_rhs(Instance::Impl& D) {
+void launch_rhs(Instance::Impl& D, hipStream_t st = nullptr) {
     if (D.ns)
-        k_rhs_sum<T><<<nblk((int64_t)D.ns * kRhsLanes), kBlock, 0, D.stream>>>(D.ns, D.tptr.p, D.tout.p, D.hr.p,
-                                                                               bufs<T>(D).x.p);
+        k_rhs_sum<T><<<nblk((int64_t)D.ns * kRhsLanes), kBlock, 0, st ? st : D.stream>>>(D.ns, D.tptr.p, D.tout.p,
+                                                                                         D.hr.p, bufs<T>(D).x.p);
 }
 
 // dx source: the factor-precision solve vector, or the FP64 CG solution
@@ -5933,6 +5933,15 @@ void iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p, bool 
             launch_meas(D, p.robust ? 1 : 0, p.gamma, 0, (refactor || !direct) ? 1 : 0, false, true, newton);
         pf.end(ph, 1);
     }
+    // a refactoring iteration whose factorization forks the subtree matrices:
+    // its right-hand side goes first on the second stream, beside G and the
+    // factor (no kernel of theirs touches x), and joins with the fork
+    const bool side_rhs = refactor && direct && !loc && D.fork_subm > 0 && sub_m_forks(D);
+    if (side_rhs) {
+        HIPCHECK(hipEventRecord(D.fork_ev, D.stream));
+        HIPCHECK(hipStreamWaitEvent(D.side, D.fork_ev, 0));
+        launch_rhs<T>(D, D.side);
+    }
     if (refactor) {
         pf.begin(PH_SPGEMM);
         launch_spgemm<T>(D, S);
@@ -5953,9 +5962,9 @@ void iteration(Instance::Impl& D, const Symbolic& S, const SolveParams& p, bool 
     pf.begin(PH_RHS);
     if (loc)
         launch_rhs_loc<T>(D);
-    else
+    else if (!side_rhs)
         launch_rhs<T>(D);
-    pf.end(PH_RHS, 1);
+    pf.end(PH_RHS, side_rhs ? 0 : 1);
     launch_trisolve<T>(D, &pf);
     const int skip_on_pivot = sizeof(T) == sizeof(double) ? 1 : 0;
     if (!closing) {
