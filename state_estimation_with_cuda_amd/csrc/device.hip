@@ -943,16 +943,31 @@ __global__ __launch_bounds__(kBlock) void k_jfinish(int nparts, const double* __
 // ---------------------------------------------------------------------------
 // Gain entries G_e = sum over the cached product list of (sqrt(w)H)_a (sqrt(w)H)_b
 // ---------------------------------------------------------------------------
+constexpr int kSpgemmChunk = 4;  // product pairs of a G entry loaded together
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_spgemm(int nG, const int32_t* __restrict__ gptr,
                                                    const int2* __restrict__ gpair, const double* __restrict__ hs,
                                                    T* __restrict__ Gval) {
     int e = blockIdx.x * kBlock + threadIdx.x;
     if (e >= nG) return;
+    // the pairs in chunks of kSpgemmChunk: their loads issued together
+    // (indices clamped to the entry's last pair), then summed in pair order
+    const int p0 = gptr[e], p1 = gptr[e + 1];
     double acc = 0.0;
-    for (int p = gptr[e]; p < gptr[e + 1]; ++p) {
-        int2 pr = gpair[p];
-        acc += hs[pr.x] * hs[pr.y];
+    for (int p = p0; p < p1; p += kSpgemmChunk) {
+        int2 pr[kSpgemmChunk];
+#pragma unroll
+        for (int u = 0; u < kSpgemmChunk; ++u) pr[u] = gpair[min(p + u, p1 - 1)];
+        double a[kSpgemmChunk], b[kSpgemmChunk];
+#pragma unroll
+        for (int u = 0; u < kSpgemmChunk; ++u) {
+            a[u] = hs[pr[u].x];
+            b[u] = hs[pr[u].y];
+        }
+#pragma unroll
+        for (int u = 0; u < kSpgemmChunk; ++u)
+            if (p + u < p1) acc += a[u] * b[u];
     }
     Gval[e] = (T)acc;  // FP64 products, rounded once to the factor precision
 }
