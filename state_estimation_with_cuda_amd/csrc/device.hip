@@ -294,6 +294,8 @@ struct MeasArgs {
     int* ipart;
 };
 
+constexpr int kMeasChunk = 4;  // Ybus entries of an injection row loaded together
+
 __global__ __launch_bounds__(kBlock) void k_meas(MeasArgs A, double* __restrict__ hs) {
     const int blk = blockIdx.x;
     const int i = blk * kBlock + threadIdx.x;
@@ -409,31 +411,48 @@ __global__ __launch_bounds__(kBlock) void k_meas(MeasArgs A, double* __restrict_
                 const int e0 = A.yrow[l], e1 = A.yrow[l + 1];
                 const int nth = (q1 - q0) - (e1 - e0);
                 int kth = 0;
-                const double Vi = A.V[l];
-                for (int e = e0; e < e1; ++e) {
-                    int j = A.ycol[e];
-                    double dth, dv;
-                    if (j == l) {
-                        double Gii = A.yG[e], Bii = A.yB[e];
-                        if (isP) {
-                            dth = -A.Q[l] - Vi * Vi * Bii;
-                            dv = A.P[l] / Vi + Vi * Gii;
-                        } else {
-                            dth = A.P[l] - Vi * Vi * Gii;
-                            dv = A.Q[l] / Vi - Vi * Bii;
-                        }
-                    } else {
-                        double pe = A.Pij[e], qe = A.Qij[e], Vj = A.V[j];
-                        if (isP) {
-                            dth = qe;
-                            dv = pe / Vj;
-                        } else {
-                            dth = -pe;
-                            dv = qe / Vj;
-                        }
+                const double Vi = A.V[l], Pl = A.P[l], Ql = A.Q[l];
+                // the row's entries in chunks of kMeasChunk: a chunk's loads are
+                // issued together (indices clamped to the row's last entry)
+                for (int e = e0; e < e1; e += kMeasChunk) {
+                    int jj[kMeasChunk], pj[kMeasChunk];
+                    double ga[kMeasChunk], ba[kMeasChunk], pa[kMeasChunk], qa[kMeasChunk], va[kMeasChunk];
+#pragma unroll
+                    for (int u = 0; u < kMeasChunk; ++u) {
+                        const int ee = min(e + u, e1 - 1);
+                        jj[u] = A.ycol[ee];
+                        ga[u] = A.yG[ee];
+                        ba[u] = A.yB[ee];
+                        pa[u] = A.Pij[ee];
+                        qa[u] = A.Qij[ee];
                     }
-                    if (A.pth[j] >= 0) put(q0 + kth++, dth);
-                    put(q0 + nth + (e - e0), dv);
+#pragma unroll
+                    for (int u = 0; u < kMeasChunk; ++u) {
+                        va[u] = A.V[jj[u]];
+                        pj[u] = A.pth[jj[u]];
+                    }
+#pragma unroll
+                    for (int u = 0; u < kMeasChunk; ++u) {
+                        if (e + u >= e1) break;
+                        double dth, dv;
+                        if (jj[u] == l) {
+                            if (isP) {
+                                dth = -Ql - Vi * Vi * ba[u];
+                                dv = Pl / Vi + Vi * ga[u];
+                            } else {
+                                dth = Pl - Vi * Vi * ga[u];
+                                dv = Ql / Vi - Vi * ba[u];
+                            }
+                        } else if (isP) {
+                            dth = qa[u];
+                            dv = pa[u] / va[u];
+                        } else {
+                            dth = -pa[u];
+                            dv = qa[u] / va[u];
+                        }
+                        if (pj[u] >= 0) put(q0 + kth++, dth);
+                        put(q0 + nth + (e + u - e0), dv);
+                    }
                 }
             } else {
                 // sorted columns: theta_min, theta_max, V_min, V_max (reference angles absent)
