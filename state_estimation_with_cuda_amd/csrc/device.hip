@@ -192,6 +192,8 @@ __global__ __launch_bounds__(kBlock) void k_ybus_tr(int nY, const int32_t* __res
 // ---------------------------------------------------------------------------
 // Bus injections; per Ybus entry P_ij, Q_ij terms kept for the Jacobian
 // ---------------------------------------------------------------------------
+constexpr int kPowerChunk = 4;  // Ybus entries of a row loaded together
+
 __global__ __launch_bounds__(kBlock) void k_power(int n, const int32_t* __restrict__ yrow,
                                                   const int32_t* __restrict__ ycol, const double* __restrict__ yG,
                                                   const double* __restrict__ yB, const double* __restrict__ V,
@@ -208,18 +210,44 @@ __global__ __launch_bounds__(kBlock) void k_power(int n, const int32_t* __restri
     if (i >= n) return;
     double vi = V[i], ti = th[i];
     double p = 0.0, q = 0.0;
-    for (int e = yrow[i]; e < yrow[i + 1]; ++e) {
-        int j = ycol[e];
-        double s, c;
-        sincos(ti - th[j], &s, &c);
-        double vv = vi * V[j];
-        double G = yG[e], B = yB[e];
-        double pe = vv * (G * c + B * s);
-        double qe = vv * (G * s - B * c);
-        Pij[e] = pe;
-        Qij[e] = qe;
-        p += pe;
-        q += qe;
+    // the row in chunks of kPowerChunk entries, loads issued together (indices
+    // clamped to the row's last entry); one sincos in the loop body, the chunk
+    // shifting down a register per entry
+    const int e1 = yrow[i + 1];
+    for (int e = yrow[i]; e < e1; e += kPowerChunk) {
+        int jj[kPowerChunk];
+        double G[kPowerChunk], Bq[kPowerChunk], Vq[kPowerChunk], tq[kPowerChunk];
+#pragma unroll
+        for (int u = 0; u < kPowerChunk; ++u) {
+            const int ee = min(e + u, e1 - 1);
+            jj[u] = ycol[ee];
+            G[u] = yG[ee];
+            Bq[u] = yB[ee];
+        }
+#pragma unroll
+        for (int u = 0; u < kPowerChunk; ++u) {
+            Vq[u] = V[jj[u]];
+            tq[u] = th[jj[u]];
+        }
+        const int cnt = min(kPowerChunk, e1 - e);
+        for (int u = 0; u < cnt; ++u) {
+            double s, c;
+            sincos(ti - tq[0], &s, &c);
+            const double vv = vi * Vq[0];
+            const double pe = vv * (G[0] * c + Bq[0] * s);
+            const double qe = vv * (G[0] * s - Bq[0] * c);
+            Pij[e + u] = pe;
+            Qij[e + u] = qe;
+            p += pe;
+            q += qe;
+#pragma unroll
+            for (int k = 0; k + 1 < kPowerChunk; ++k) {
+                G[k] = G[k + 1];
+                Bq[k] = Bq[k + 1];
+                Vq[k] = Vq[k + 1];
+                tq[k] = tq[k + 1];
+            }
+        }
     }
     P[i] = p;
     Q[i] = q;
